@@ -1,0 +1,74 @@
+"""ctypes binding of libcoffe_engine.so (the C-ABI of include/coffe_engine.h)."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcoffe_engine.so")
+
+_lib = None
+
+ERRORS = {
+    0: "success", 1: "invalid argument", 2: "out of memory", 3: "HIP device error",
+    4: "unsupported size", 5: "argument outside the domain", 6: "non-finite value", 7: "no HIP device",
+}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = ERRORS.get(code, f"error {code}")
+        if _lib is not None:
+            try:
+                msg = _lib.coffe_engine_strerror(code).decode()
+            except Exception:  # pragma: no cover
+                pass
+        super().__init__(f"{what}: {msg} (code {code})" if what else f"{msg} (code {code})")
+
+
+def check(code, what=""):
+    if code != 0:
+        raise EngineError(code, what)
+
+
+def load_library():
+    """Load the engine library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise EngineError(7, f"{LIB_PATH} is missing; run __graft_entry__.build() or python coffe_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, i64, d, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_size_t
+    dp = ctypes.POINTER(ctypes.c_double)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.coffe_engine_strerror.argtypes = [i]
+    L.coffe_engine_strerror.restype = ctypes.c_char_p
+    L.coffe_engine_last_error.argtypes = []
+    L.coffe_engine_create.argtypes = [i, ctypes.POINTER(vp)]
+    L.coffe_engine_destroy.argtypes = [vp]
+    L.coffe_engine_destroy.restype = None
+    L.coffe_engine_default.argtypes = []
+    L.coffe_engine_default.restype = vp
+    L.coffe_engine_synchronize.argtypes = [vp]
+    L.twofast_1bessel.argtypes = [dp, dp, sz, dp, dp, sz, d, d, d, d, d, d, ctypes.c_uint]
+    L.twofast_1bessel.restype = None
+    L.twofast_1bessel_status.argtypes = [dp, dp, sz, dp, dp, sz, d, d, d, d, d, d]
+    L.coffe_batch_create.argtypes = [vp, i, i64p, i64p, dp, dp, i, ip, ip, i, dp, i, i, ctypes.POINTER(vp)]
+    L.coffe_batch_destroy.argtypes = [vp]
+    L.coffe_batch_destroy.restype = None
+    L.coffe_batch_input_size.argtypes = [vp]
+    L.coffe_batch_input_size.restype = i64
+    L.coffe_batch_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.coffe_batch_vjp.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.coffe_batch_tables.argtypes = [vp, vp, vp, vp, vp]
+    _lib = L
+    return L
+
+
+EXPORTED = [
+    "coffe_engine_strerror", "coffe_engine_last_error", "coffe_engine_create", "coffe_engine_destroy",
+    "coffe_engine_default", "coffe_engine_synchronize", "twofast_1bessel", "twofast_1bessel_status",
+    "coffe_batch_create", "coffe_batch_destroy", "coffe_batch_input_size", "coffe_batch_forward",
+    "coffe_batch_vjp", "coffe_batch_tables",
+]

@@ -1,0 +1,85 @@
+/*
+ * Internal interface between the host C layer (engine.c, dropin.c) and the HIP
+ * kernels (fftlog_kernels.hip, quad_kernels.hip).  Plain C types only.
+ */
+#ifndef COFFE_ENGINE_INTERNAL_H
+#define COFFE_ENGINE_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CE_MAX_PASSES 16
+#define CE_MAX_TABLES 8
+#define CE_MIN_SEP_LEN 42
+
+/* FFT radix schedule for a complex transform of length M = N/2 */
+typedef struct ce_radix_plan {
+    int npass;
+    int radix[CE_MAX_PASSES];
+} ce_radix_plan;
+
+/* spectrum-independent tables for one (N, kmin, kmax) grid, device pointers */
+typedef struct ce_grid_dev {
+    ce_radix_plan rp;  /* radix schedule of the length-M complex FFT */
+    int N;             /* FFT length (even) */
+    int M;             /* N / 2 */
+    double kmin, kmax, G;
+    double k0, r0;     /* x_i = k0 p_i, r_i = r0 p_i */
+    const double *p;   /* (kmax/kmin)^(i/N), i < N  (x_i = kmin p_i, r_i = p_i) */
+    const double *tw;  /* interleaved e^{-2 pi i k/N}, k < N */
+    const double *bw[2];  /* bias (kmax/kmin)^((3-q')i/N) times the k-window, per distinct q' */
+    const double *U[CE_MAX_TABLES];   /* interleaved: kernel u_l(t_k) * W_F(k) / L, k <= M */
+    const double *os[CE_MAX_TABLES];  /* output scale: prefactor_i * r_i^(n-l), i < N */
+} ce_grid_dev;
+
+/* one spectrum of a batch (host side view, copied to device as an array) */
+typedef struct ce_spec_dev {
+    int64_t in_off;    /* offset of k / P in the concatenated input */
+    int32_t n_in;      /* input length */
+    int32_t grid;      /* index into the grid array */
+    int64_t xf_off;    /* offset (complex units) of the r2c workspace: nq * (M+1) */
+    int64_t xb_off;    /* offset (complex units) of the adjoint workspace: ntab * (M+1) */
+    int64_t ws_off;    /* offset (doubles) of the real workspace: 5 * n_in + N */
+} ce_spec_dev;
+
+/* work unit of the table kernels */
+typedef struct ce_unit_dev {
+    int32_t spec;
+    int32_t tab;       /* table slot */
+} ce_unit_dev;
+
+typedef struct ce_table_desc {
+    double n, l;       /* power nu and Bessel order of the transform */
+    int qidx;          /* which distinct q' = q + nu (0/1) */
+    int rpow;          /* multiply by r^(n-l) when n >= l (the I_l^n tables) */
+} ce_table_desc;
+
+/* ---- launchers (fftlog_kernels.hip); all return 0 or a HIP error code ---- */
+int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double k0, double r0, const double *d_p,
+                          int ntab, const ce_table_desc *tabs, const double *tab_qnu, int nq, const double *qprime,
+                          double *const *d_U, double *const *d_os, double *const *d_bw);
+
+int ce_launch_prepare(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids, int nq,
+                      const double *d_k, const double *d_pk, double *d_ws, double *d_xf, int threads, int max_M);
+
+int ce_launch_tables(void *stream, int nunit, const ce_unit_dev *d_units, const ce_spec_dev *d_spec,
+                     const ce_grid_dev *d_grids, const ce_table_desc *d_tabs, int ntab, const double *d_xf,
+                     int nsep, const int32_t *d_sidx, const double *d_sep, double *d_itab, double *d_full,
+                     int64_t full_stride, const double *d_A, const double *d_gout, int nz, int nl, double *d_xbar,
+                     double *d_wbuf, int threads, int max_M);
+
+int ce_launch_project(void *stream, int nspec, int ntab, int nz, int nl, int nsep, const double *d_A,
+                      const double *d_itab, double *d_out);
+
+int ce_launch_prepare_vjp(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
+                          const ce_table_desc *d_tabs, int ntab, int nq, const double *d_k, const double *d_pk,
+                          double *d_ws, const double *d_xbar, double *d_gpk, int threads, int max_M);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

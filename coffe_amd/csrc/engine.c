@@ -1,0 +1,632 @@
+/*
+ * Host side of the MI355X spherical-Bessel integral engine (C99).
+ *
+ * Owns the per-device state (stream, cache of spectrum-independent grid tables),
+ * validates inputs the way the reference would have exit()ed on them, builds the
+ * work lists of a batch and launches the HIP kernels through the launchers of
+ * ce_internal.h.  Reference behaviour restated here:
+ *   - q selection, twofast_selectqnu (twofast.c:240-250);
+ *   - the output grid r_i = r0 (kmax/kmin)^(i/N) (twofast.c:415-417) and the
+ *     table layout [r = 0, 42 small separations, N FFT nodes]
+ *     (integrals.c:474, 545-567), whose Akima lookups at a separation touch
+ *     nodes j-2..j+3 of the interval j containing it (common.c:704-710);
+ *   - window-limit validation (twofast.c:62, 81-88) and non-finite checks
+ *     (twofast.c:122, 210) turned into return codes.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "../../include/coffe_engine.h"
+#include "ce_internal.h"
+
+#define CE_THREADS_DEFAULT 512
+
+static __thread int ce_last_error = 0;
+
+int coffe_engine_last_error(void) { return ce_last_error; }
+
+const char *coffe_engine_strerror(int code)
+{
+    switch (code) {
+    case COFFE_OK: return "success";
+    case COFFE_EINVAL: return "invalid argument";
+    case COFFE_ENOMEM: return "out of memory";
+    case COFFE_EDEVICE: return "HIP device error";
+    case COFFE_EUNSUPPORTED: return "unsupported size (FFT length must be even, N/2 = 2^a 3^b 5^c, N <= 16384)";
+    case COFFE_EDOMAIN: return "argument outside the domain (window limits or separation outside the table)";
+    case COFFE_ENONFINITE: return "non-finite value in input or kernel";
+    case COFFE_ENODEVICE: return "no HIP device available";
+    default: return "unknown error";
+    }
+}
+
+/* ---- grids ------------------------------------------------------------------ */
+
+typedef struct ce_grid {
+    struct ce_grid *next;
+    /* key */
+    int N;
+    double kmin, kmax, k0, r0;
+    int ntab;
+    ce_table_desc tabs[CE_MAX_TABLES];
+    int nq;
+    double qprime[2];
+    /* host */
+    double *h_p;
+    /* device */
+    double *d_p, *d_tw, *d_bw[2], *d_U[CE_MAX_TABLES], *d_os[CE_MAX_TABLES];
+    ce_radix_plan rp;
+} ce_grid;
+
+struct coffe_engine {
+    int device;
+    hipStream_t stream;
+    int threads;
+    ce_grid *grids;
+    pthread_mutex_t lock;
+    /* scratch of the drop-in path */
+    void *d_scratch;
+    size_t scratch_bytes;
+};
+
+static int hip_ok(hipError_t e) { return e == hipSuccess ? COFFE_OK : COFFE_EDEVICE; }
+
+static int dev_alloc(void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    return hipMalloc(p, bytes) == hipSuccess ? COFFE_OK : COFFE_ENOMEM;
+}
+
+double ce_selectqnu(double l, double nu)
+{
+    const double n1 = 0.9, n2 = 0.9999;
+    const double qmin = fmax(n2 - 1.0 - nu, -l);
+    const double qmax = fmin(n1 + 3.0 - nu, 2.0);
+    double q = (2. + n1 + n2) / 2. - nu;
+    if (!(qmin < q && q < qmax)) q = (qmin + 2. * qmax) / 3.;
+    return q;
+}
+
+int ce_radix_schedule(int M, ce_radix_plan *rp)
+{
+    rp->npass = 0;
+    int m = M;
+    while (m % 8 == 0) { rp->radix[rp->npass++] = 8; m /= 8; }
+    while (m % 4 == 0) { rp->radix[rp->npass++] = 4; m /= 4; }
+    while (m % 2 == 0) { rp->radix[rp->npass++] = 2; m /= 2; }
+    while (m % 5 == 0) { if (rp->npass >= CE_MAX_PASSES) return COFFE_EUNSUPPORTED; rp->radix[rp->npass++] = 5; m /= 5; }
+    while (m % 3 == 0) { if (rp->npass >= CE_MAX_PASSES) return COFFE_EUNSUPPORTED; rp->radix[rp->npass++] = 3; m /= 3; }
+    return m == 1 ? COFFE_OK : COFFE_EUNSUPPORTED;
+}
+
+static void grid_free(ce_grid *g)
+{
+    if (!g) return;
+    free(g->h_p);
+    hipFree(g->d_p);
+    hipFree(g->d_tw);
+    for (int q = 0; q < 2; ++q) hipFree(g->d_bw[q]);
+    for (int t = 0; t < CE_MAX_TABLES; ++t) { hipFree(g->d_U[t]); hipFree(g->d_os[t]); }
+    free(g);
+}
+
+static int same_tabs(const ce_grid *g, int ntab, const ce_table_desc *tabs)
+{
+    if (g->ntab != ntab) return 0;
+    for (int t = 0; t < ntab; ++t)
+        if (g->tabs[t].n != tabs[t].n || g->tabs[t].l != tabs[t].l || g->tabs[t].rpow != tabs[t].rpow) return 0;
+    return 1;
+}
+
+/* assign q' = q(l, nu) + nu classes; at most two distinct values */
+static int assign_q(int ntab, ce_table_desc *tabs, double *qnu, int *nq, double *qprime)
+{
+    *nq = 0;
+    for (int t = 0; t < ntab; ++t) {
+        qnu[t] = ce_selectqnu(tabs[t].l, tabs[t].n);
+        const double qp = qnu[t] + tabs[t].n;
+        int found = -1;
+        for (int q = 0; q < *nq; ++q)
+            if (qprime[q] == qp) found = q;
+        if (found < 0) {
+            if (*nq >= 2) return COFFE_EUNSUPPORTED;
+            qprime[*nq] = qp;
+            found = (*nq)++;
+        }
+        tabs[t].qidx = found;
+    }
+    return COFFE_OK;
+}
+
+/* grid tables for (N, kmin, kmax, k0, r0, tables); cached on the engine */
+static int grid_get(coffe_engine *e, int N, double kmin, double kmax, double k0, double r0, int ntab,
+                    const ce_table_desc *tabs_in, ce_grid **out)
+{
+    if (N < 4 || (N & 1) || N > 16384) return COFFE_EUNSUPPORTED;
+    if (!(kmin > 0) || !(kmax > kmin) || !(k0 > 0) || !(r0 > 0) || !isfinite(kmax)) return COFFE_EINVAL;
+    if (ntab < 1 || ntab > CE_MAX_TABLES) return COFFE_EINVAL;
+    pthread_mutex_lock(&e->lock);
+    for (ce_grid *g = e->grids; g; g = g->next) {
+        if (g->N == N && g->kmin == kmin && g->kmax == kmax && g->k0 == k0 && g->r0 == r0 &&
+            same_tabs(g, ntab, tabs_in)) {
+            pthread_mutex_unlock(&e->lock);
+            *out = g;
+            return COFFE_OK;
+        }
+    }
+    pthread_mutex_unlock(&e->lock);
+    ce_grid *g = (ce_grid *)calloc(1, sizeof(*g));
+    if (!g) return COFFE_ENOMEM;
+    g->N = N; g->kmin = kmin; g->kmax = kmax; g->k0 = k0; g->r0 = r0; g->ntab = ntab;
+    memcpy(g->tabs, tabs_in, sizeof(ce_table_desc) * (size_t)ntab);
+    double qnu[CE_MAX_TABLES];
+    int st = assign_q(ntab, g->tabs, qnu, &g->nq, g->qprime);
+    if (st == COFFE_OK) st = ce_radix_schedule(N / 2, &g->rp);
+    /* window limits of twofast_fft_input (twofast.c:283-289) */
+    const double xmax = k0 * pow(kmax / kmin, (double)(N - 1) / N);
+    const double xl = exp(0.46) * kmin, xr = exp(-0.46) * xmax;
+    if (st == COFFE_OK && !(kmin <= xl && xl <= xr && xr <= xmax)) st = COFFE_EDOMAIN;
+    g->h_p = (double *)malloc(sizeof(double) * (size_t)N);
+    double *h_tw = (double *)malloc(sizeof(double) * 2 * (size_t)N);
+    if (st == COFFE_OK && (!g->h_p || !h_tw)) st = COFFE_ENOMEM;
+    if (st == COFFE_OK) {
+        for (int i = 0; i < N; ++i) g->h_p[i] = pow(kmax / kmin, (double)i / N);
+        for (int k = 0; k < N; ++k) {
+            const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)N;
+            h_tw[2 * k] = (double)cosl(a);
+            h_tw[2 * k + 1] = (double)sinl(a);
+        }
+    }
+    const size_t nb = sizeof(double) * (size_t)N;
+    const size_t cb = sizeof(double) * 2 * (size_t)(N / 2 + 1);
+    if (st == COFFE_OK) st = dev_alloc((void **)&g->d_p, nb);
+    if (st == COFFE_OK) st = dev_alloc((void **)&g->d_tw, 2 * nb);
+    for (int q = 0; q < g->nq && st == COFFE_OK; ++q) st = dev_alloc((void **)&g->d_bw[q], nb);
+    for (int t = 0; t < ntab && st == COFFE_OK; ++t) {
+        st = dev_alloc((void **)&g->d_U[t], cb);
+        if (st == COFFE_OK) st = dev_alloc((void **)&g->d_os[t], nb);
+    }
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(g->d_p, g->h_p, nb, hipMemcpyHostToDevice, e->stream));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(g->d_tw, h_tw, 2 * nb, hipMemcpyHostToDevice, e->stream));
+    if (st == COFFE_OK) {
+        st = ce_launch_grid_tables(e->stream, N, kmin, kmax, k0, r0, g->d_p, ntab, g->tabs, qnu, g->nq, g->qprime,
+                                   g->d_U, g->d_os, g->d_bw) ? COFFE_EDEVICE : COFFE_OK;
+    }
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(e->stream));
+    free(h_tw);
+    if (st == COFFE_OK) {
+        /* the gamma-ratio kernel must be finite (twofast.c:210-223) */
+        double *hU = (double *)malloc(cb);
+        if (!hU) st = COFFE_ENOMEM;
+        for (int t = 0; t < ntab && st == COFFE_OK; ++t) {
+            st = hip_ok(hipMemcpy(hU, g->d_U[t], cb, hipMemcpyDeviceToHost));
+            for (int k = 0; k < N + 2 && st == COFFE_OK; ++k)
+                if (!isfinite(hU[k])) st = COFFE_ENONFINITE;
+        }
+        free(hU);
+    }
+    if (st != COFFE_OK) { grid_free(g); return st; }
+    pthread_mutex_lock(&e->lock);
+    g->next = e->grids;
+    e->grids = g;
+    pthread_mutex_unlock(&e->lock);
+    *out = g;
+    return COFFE_OK;
+}
+
+static void grid_to_dev(const ce_grid *g, ce_grid_dev *d)
+{
+    memset(d, 0, sizeof(*d));
+    d->rp = g->rp;
+    d->N = g->N;
+    d->M = g->N / 2;
+    d->kmin = g->kmin;
+    d->kmax = g->kmax;
+    d->G = log(g->kmax / g->kmin);
+    d->k0 = g->k0;
+    d->r0 = g->r0;
+    d->p = g->d_p;
+    d->tw = g->d_tw;
+    for (int q = 0; q < 2; ++q) d->bw[q] = g->d_bw[q];
+    for (int t = 0; t < CE_MAX_TABLES; ++t) { d->U[t] = g->d_U[t]; d->os[t] = g->d_os[t]; }
+}
+
+/* ---- engine ----------------------------------------------------------------- */
+
+int coffe_engine_create(int device, coffe_engine **out)
+{
+    if (!out) return COFFE_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return COFFE_ENODEVICE;
+    if (device < 0 || device >= ndev) return COFFE_EINVAL;
+    coffe_engine *e = (coffe_engine *)calloc(1, sizeof(*e));
+    if (!e) return COFFE_ENOMEM;
+    e->device = device;
+    e->threads = CE_THREADS_DEFAULT;
+    const char *t = getenv("COFFE_ENGINE_THREADS");
+    if (t) {
+        int v = atoi(t);
+        if (v == 256 || v == 512 || v == 1024) e->threads = v;
+    }
+    pthread_mutex_init(&e->lock, NULL);
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess) {
+        free(e);
+        return COFFE_EDEVICE;
+    }
+    *out = e;
+    return COFFE_OK;
+}
+
+void coffe_engine_destroy(coffe_engine *e)
+{
+    if (!e) return;
+    hipStreamSynchronize(e->stream);
+    ce_grid *g = e->grids;
+    while (g) { ce_grid *n = g->next; grid_free(g); g = n; }
+    hipFree(e->d_scratch);
+    hipStreamDestroy(e->stream);
+    pthread_mutex_destroy(&e->lock);
+    free(e);
+}
+
+int coffe_engine_synchronize(coffe_engine *e)
+{
+    if (!e) return COFFE_EINVAL;
+    return hip_ok(hipStreamSynchronize(e->stream));
+}
+
+static coffe_engine *g_default = NULL;
+static pthread_mutex_t g_default_lock = PTHREAD_MUTEX_INITIALIZER;
+
+coffe_engine *coffe_engine_default(void)
+{
+    pthread_mutex_lock(&g_default_lock);
+    if (!g_default) {
+        coffe_engine *e = NULL;
+        if (coffe_engine_create(0, &e) == COFFE_OK) g_default = e;
+    }
+    pthread_mutex_unlock(&g_default_lock);
+    return g_default;
+}
+
+/* ---- batch ------------------------------------------------------------------ */
+
+struct coffe_batch {
+    coffe_engine *e;
+    int nspec, ntab, nsep, nz, nl, nq, nunit, ngrid;
+    int max_M, max_N, threads;
+    int64_t total_in;
+    ce_grid **grids;
+    ce_spec_dev *h_spec;
+    /* device */
+    ce_spec_dev *d_spec;
+    ce_grid_dev *d_grids;
+    ce_table_desc *d_tabs;
+    ce_unit_dev *d_units;
+    int32_t *d_sidx;
+    double *d_sep;
+    double *d_ws, *d_xf, *d_xb, *d_wbuf, *d_itab;
+};
+
+int64_t coffe_batch_input_size(const coffe_batch *b) { return b ? b->total_in : 0; }
+
+void coffe_batch_destroy(coffe_batch *b)
+{
+    if (!b) return;
+    free(b->grids);
+    free(b->h_spec);
+    hipFree(b->d_spec);
+    hipFree(b->d_grids);
+    hipFree(b->d_tabs);
+    hipFree(b->d_units);
+    hipFree(b->d_sidx);
+    hipFree(b->d_sep);
+    hipFree(b->d_ws);
+    hipFree(b->d_xf);
+    hipFree(b->d_xb);
+    hipFree(b->d_wbuf);
+    hipFree(b->d_itab);
+    free(b);
+}
+
+/* interval j of the FFT nodes r_i = r0 p_i containing sep, with x_j <= sep < x_{j+1} */
+static int sep_interval(const ce_grid *g, double sep, int32_t *j_out)
+{
+    const int N = g->N;
+    const double r_lo = g->r0 * g->h_p[0];
+    if (!(sep >= r_lo) || !(sep <= g->r0 * g->h_p[N - 1])) return COFFE_EDOMAIN;
+    int lo = 0, hi = N - 1;
+    while (hi > lo + 1) {
+        const int m = (lo + hi) / 2;
+        if (g->r0 * g->h_p[m] > sep) hi = m;
+        else lo = m;
+    }
+    /* the Akima stencil must stay among the FFT nodes (no small-r nodes, no
+       extrapolated end slopes) */
+    if (lo < 2 || lo + 3 > N - 1) return COFFE_EDOMAIN;
+    *j_out = lo;
+    return COFFE_OK;
+}
+
+typedef struct { int idx; int N; } unit_sort;
+static int cmp_unit(const void *a, const void *b)
+{
+    const unit_sort *x = (const unit_sort *)a, *y = (const unit_sort *)b;
+    if (x->N != y->N) return y->N - x->N;
+    return x->idx - y->idx;
+}
+
+int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const int64_t *n_fft, const double *kmin,
+                       const double *kmax, int ntab, const int *tab_n, const int *tab_l, int nsep, const double *sep,
+                       int nz, int nl, coffe_batch **out)
+{
+    if (!e || !out || nspec <= 0 || !n_in || !n_fft || !kmin || !kmax || ntab <= 0 || ntab > CE_MAX_TABLES ||
+        !tab_n || !tab_l || nsep <= 0 || !sep || nz <= 0 || nl <= 0)
+        return COFFE_EINVAL;
+    hipSetDevice(e->device);
+    coffe_batch *b = (coffe_batch *)calloc(1, sizeof(*b));
+    if (!b) return COFFE_ENOMEM;
+    b->e = e;
+    b->nspec = nspec; b->ntab = ntab; b->nsep = nsep; b->nz = nz; b->nl = nl;
+    b->threads = e->threads;
+    ce_table_desc tabs[CE_MAX_TABLES];
+    for (int t = 0; t < ntab; ++t) {
+        if (tab_n[t] < -2 || tab_n[t] > 4 || tab_l[t] < 0 || tab_l[t] > 4) { coffe_batch_destroy(b); return COFFE_EINVAL; }
+        tabs[t].n = tab_n[t];
+        tabs[t].l = tab_l[t];
+        tabs[t].rpow = 1;
+        tabs[t].qidx = 0;
+    }
+    b->grids = (ce_grid **)calloc((size_t)nspec, sizeof(ce_grid *));
+    b->h_spec = (ce_spec_dev *)calloc((size_t)nspec, sizeof(ce_spec_dev));
+    int32_t *spec_grid = (int32_t *)calloc((size_t)nspec, sizeof(int32_t));
+    if (!b->grids || !b->h_spec || !spec_grid) { free(spec_grid); coffe_batch_destroy(b); return COFFE_ENOMEM; }
+    int st = COFFE_OK;
+    int64_t in_off = 0, xf_off = 0, xb_off = 0, ws_off = 0;
+    for (int s = 0; s < nspec && st == COFFE_OK; ++s) {
+        if (n_in[s] < 5 || n_fft[s] < 4 || n_fft[s] > 16384) { st = n_in[s] < 5 ? COFFE_EINVAL : COFFE_EUNSUPPORTED; break; }
+        ce_grid *g = NULL;
+        st = grid_get(e, (int)n_fft[s], kmin[s], kmax[s], kmin[s], 1.0, ntab, tabs, &g);
+        if (st != COFFE_OK) break;
+        int gi = -1;
+        for (int q = 0; q < b->ngrid; ++q) if (b->grids[q] == g) gi = q;
+        if (gi < 0) { gi = b->ngrid; b->grids[b->ngrid++] = g; }
+        spec_grid[s] = gi;
+        const int M = g->N / 2;
+        if (M > b->max_M) b->max_M = M;
+        if (g->N > b->max_N) b->max_N = g->N;
+        b->nq = g->nq;
+        ce_spec_dev *d = &b->h_spec[s];
+        d->in_off = in_off;
+        d->n_in = (int32_t)n_in[s];
+        d->grid = gi;
+        d->xf_off = xf_off;
+        d->xb_off = xb_off;
+        d->ws_off = ws_off;
+        in_off += n_in[s];
+        xf_off += (int64_t)g->nq * (M + 1);
+        xb_off += (int64_t)ntab * (M + 1);
+        ws_off += 5 * n_in[s] + g->N;
+    }
+    b->total_in = in_off;
+    /* every FFT pass needs at least M/8 threads (at most 4 butterflies each) */
+    while (st == COFFE_OK && b->threads < b->max_M / 8) b->threads *= 2;
+    /* separation intervals per grid */
+    int32_t *sidx = NULL;
+    if (st == COFFE_OK) {
+        sidx = (int32_t *)malloc(sizeof(int32_t) * (size_t)b->ngrid * (size_t)nsep);
+        if (!sidx) st = COFFE_ENOMEM;
+        for (int gi = 0; gi < b->ngrid && st == COFFE_OK; ++gi)
+            for (int r = 0; r < nsep && st == COFFE_OK; ++r) st = sep_interval(b->grids[gi], sep[r], &sidx[gi * nsep + r]);
+    }
+    /* work units, largest FFT first */
+    ce_unit_dev *units = NULL;
+    if (st == COFFE_OK) {
+        b->nunit = nspec * ntab;
+        unit_sort *us = (unit_sort *)malloc(sizeof(unit_sort) * (size_t)b->nunit);
+        units = (ce_unit_dev *)malloc(sizeof(ce_unit_dev) * (size_t)b->nunit);
+        if (!us || !units) st = COFFE_ENOMEM;
+        for (int i = 0; i < b->nunit && st == COFFE_OK; ++i) { us[i].idx = i; us[i].N = (int)n_fft[i / ntab]; }
+        if (st == COFFE_OK) {
+            qsort(us, (size_t)b->nunit, sizeof(unit_sort), cmp_unit);
+            for (int i = 0; i < b->nunit; ++i) { units[i].spec = us[i].idx / ntab; units[i].tab = us[i].idx % ntab; }
+        }
+        free(us);
+    }
+    ce_grid_dev *hg = NULL;
+    if (st == COFFE_OK) {
+        hg = (ce_grid_dev *)calloc((size_t)b->ngrid, sizeof(ce_grid_dev));
+        if (!hg) st = COFFE_ENOMEM;
+        for (int gi = 0; gi < b->ngrid && st == COFFE_OK; ++gi) grid_to_dev(b->grids[gi], &hg[gi]);
+    }
+    if (st == COFFE_OK) {
+        /* the q classes are the same for every grid (they depend on the tables only) */
+        memcpy(tabs, b->grids[0]->tabs, sizeof(ce_table_desc) * (size_t)ntab);
+    }
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_spec, sizeof(ce_spec_dev) * (size_t)nspec);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_grids, sizeof(ce_grid_dev) * (size_t)b->ngrid);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_tabs, sizeof(ce_table_desc) * (size_t)ntab);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_units, sizeof(ce_unit_dev) * (size_t)b->nunit);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_sidx, sizeof(int32_t) * (size_t)b->ngrid * (size_t)nsep);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_sep, sizeof(double) * (size_t)nsep);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_ws, sizeof(double) * (size_t)(ws_off > 0 ? ws_off : 1));
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_xf, 2 * sizeof(double) * (size_t)xf_off);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_xb, 2 * sizeof(double) * (size_t)xb_off);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_wbuf, 6 * sizeof(double) * (size_t)b->nunit * (size_t)nsep);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_itab, sizeof(double) * (size_t)nspec * ntab * nsep);
+    hipStream_t s = e->stream;
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_spec, b->h_spec, sizeof(ce_spec_dev) * (size_t)nspec, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_grids, hg, sizeof(ce_grid_dev) * (size_t)b->ngrid, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_tabs, tabs, sizeof(ce_table_desc) * (size_t)ntab, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_units, units, sizeof(ce_unit_dev) * (size_t)b->nunit, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_sidx, sidx, sizeof(int32_t) * (size_t)b->ngrid * (size_t)nsep, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_sep, sep, sizeof(double) * (size_t)nsep, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    free(spec_grid);
+    free(sidx);
+    free(units);
+    free(hg);
+    if (st != COFFE_OK) { coffe_batch_destroy(b); return st; }
+    *out = b;
+    return COFFE_OK;
+}
+
+static hipStream_t pick_stream(coffe_batch *b, void *stream) { return stream ? (hipStream_t)stream : b->e->stream; }
+
+int coffe_batch_forward(coffe_batch *b, const double *d_k, const double *d_pk, const double *d_A, double *d_itab,
+                        double *d_out, void *stream)
+{
+    if (!b || !d_k || !d_pk || (d_out && !d_A)) return COFFE_EINVAL;
+    hipStream_t s = pick_stream(b, stream);
+    double *itab = d_itab ? d_itab : b->d_itab;
+    if (ce_launch_prepare(s, b->nspec, b->d_spec, b->d_grids, b->nq, d_k, d_pk, b->d_ws, b->d_xf, b->threads, b->max_M))
+        return COFFE_EDEVICE;
+    if (ce_launch_tables(s, b->nunit, b->d_units, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->d_xf, b->nsep,
+                         b->d_sidx, b->d_sep, itab, NULL, 0, NULL, NULL, b->nz, b->nl, NULL, b->d_wbuf, b->threads,
+                         b->max_M))
+        return COFFE_EDEVICE;
+    if (d_out && ce_launch_project(s, b->nspec, b->ntab, b->nz, b->nl, b->nsep, d_A, itab, d_out)) return COFFE_EDEVICE;
+    return COFFE_OK;
+}
+
+int coffe_batch_vjp(coffe_batch *b, const double *d_k, const double *d_pk, const double *d_A, const double *d_gout,
+                    double *d_itab, double *d_out, double *d_gpk, void *stream)
+{
+    if (!b || !d_k || !d_pk || !d_A || !d_gout || !d_gpk) return COFFE_EINVAL;
+    hipStream_t s = pick_stream(b, stream);
+    double *itab = d_itab ? d_itab : b->d_itab;
+    if (ce_launch_prepare(s, b->nspec, b->d_spec, b->d_grids, b->nq, d_k, d_pk, b->d_ws, b->d_xf, b->threads, b->max_M))
+        return COFFE_EDEVICE;
+    if (ce_launch_tables(s, b->nunit, b->d_units, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->d_xf, b->nsep,
+                         b->d_sidx, b->d_sep, itab, NULL, 0, d_A, d_gout, b->nz, b->nl, b->d_xb, b->d_wbuf,
+                         b->threads, b->max_M))
+        return COFFE_EDEVICE;
+    if (d_out && ce_launch_project(s, b->nspec, b->ntab, b->nz, b->nl, b->nsep, d_A, itab, d_out)) return COFFE_EDEVICE;
+    if (ce_launch_prepare_vjp(s, b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq, d_k, d_pk, b->d_ws,
+                              b->d_xb, d_gpk, b->threads, b->max_M))
+        return COFFE_EDEVICE;
+    return COFFE_OK;
+}
+
+int coffe_batch_tables(coffe_batch *b, const double *d_k, const double *d_pk, double *d_full, void *stream)
+{
+    if (!b || !d_k || !d_pk || !d_full) return COFFE_EINVAL;
+    hipStream_t s = pick_stream(b, stream);
+    if (ce_launch_prepare(s, b->nspec, b->d_spec, b->d_grids, b->nq, d_k, d_pk, b->d_ws, b->d_xf, b->threads, b->max_M))
+        return COFFE_EDEVICE;
+    /* the unit order is by size; the full-table rows follow the unit order, so
+       launch one unit per (spec, tab) in natural order via a temporary list */
+    ce_unit_dev *h = (ce_unit_dev *)malloc(sizeof(ce_unit_dev) * (size_t)b->nunit);
+    ce_unit_dev *d = NULL;
+    int st = h ? COFFE_OK : COFFE_ENOMEM;
+    if (st == COFFE_OK) st = dev_alloc((void **)&d, sizeof(ce_unit_dev) * (size_t)b->nunit);
+    if (st == COFFE_OK) {
+        for (int i = 0; i < b->nunit; ++i) { h[i].spec = i / b->ntab; h[i].tab = i % b->ntab; }
+        st = hip_ok(hipMemcpyAsync(d, h, sizeof(ce_unit_dev) * (size_t)b->nunit, hipMemcpyHostToDevice, s));
+    }
+    if (st == COFFE_OK &&
+        ce_launch_tables(s, b->nunit, d, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->d_xf, b->nsep, b->d_sidx,
+                         b->d_sep, NULL, d_full, b->max_N, NULL, NULL, b->nz, b->nl, NULL, b->d_wbuf, b->threads,
+                         b->max_M))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    hipFree(d);
+    free(h);
+    return st;
+}
+
+/* ---- drop-in: twofast_1bessel ------------------------------------------------ */
+
+static int scratch_reserve(coffe_engine *e, size_t bytes)
+{
+    if (e->scratch_bytes >= bytes) return COFFE_OK;
+    hipFree(e->d_scratch);
+    e->d_scratch = NULL;
+    e->scratch_bytes = 0;
+    if (hipMalloc(&e->d_scratch, bytes) != hipSuccess) return COFFE_ENOMEM;
+    e->scratch_bytes = bytes;
+    return COFFE_OK;
+}
+
+int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len, const double *input_x,
+                           const double *input_y, size_t input_len, double l, double nu, double r0, double k0,
+                           double kmin, double kmax)
+{
+    if (!output_x || !output_y || !input_x || !input_y || input_len < 5) return COFFE_EINVAL;
+    if (output_len < 4 || (output_len & 1) || output_len > 16384) return COFFE_EUNSUPPORTED;
+    for (size_t i = 0; i < input_len; ++i)
+        if (!isfinite(input_x[i]) || !isfinite(input_y[i])) return COFFE_ENONFINITE;
+    coffe_engine *e = coffe_engine_default();
+    if (!e) return COFFE_ENODEVICE;
+    hipSetDevice(e->device);
+    ce_table_desc td;
+    td.n = nu;
+    td.l = l;
+    td.rpow = 0;
+    td.qidx = 0;
+    ce_grid *g = NULL;
+    int st = grid_get(e, (int)output_len, kmin, kmax, k0, r0, 1, &td, &g);
+    if (st != COFFE_OK) return st;
+    const int N = (int)output_len, M = N / 2;
+    const size_t n = input_len;
+    /* scratch layout: k, P | ws (5n + N) | xf (M+1 cplx) | out (N) | spec, grid, tab, unit, sidx, wbuf */
+    const size_t off_k = 0, off_p = n, off_ws = 2 * n, off_xf = off_ws + 5 * n + (size_t)N;
+    const size_t off_out = off_xf + 2 * (size_t)(M + 1), off_meta = off_out + (size_t)N;
+    const size_t meta_bytes = sizeof(ce_spec_dev) + sizeof(ce_grid_dev) + sizeof(ce_table_desc) +
+                              sizeof(ce_unit_dev) + 64;
+    pthread_mutex_lock(&e->lock);
+    st = scratch_reserve(e, sizeof(double) * off_meta + meta_bytes + 64);
+    if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); return st; }
+    double *base = (double *)e->d_scratch;
+    char *meta = (char *)(base + off_meta);
+    ce_spec_dev *d_spec = (ce_spec_dev *)meta;
+    ce_grid_dev *d_grid = (ce_grid_dev *)(meta + sizeof(ce_spec_dev));
+    ce_table_desc *d_tab = (ce_table_desc *)((char *)d_grid + sizeof(ce_grid_dev));
+    ce_unit_dev *d_unit = (ce_unit_dev *)((char *)d_tab + sizeof(ce_table_desc));
+    ce_spec_dev hs;
+    memset(&hs, 0, sizeof(hs));
+    hs.in_off = 0;
+    hs.n_in = (int32_t)n;
+    hs.grid = 0;
+    hs.xf_off = 0;
+    hs.xb_off = 0;
+    hs.ws_off = 0;
+    ce_grid_dev hg;
+    grid_to_dev(g, &hg);
+    ce_unit_dev hu = {0, 0};
+    hipStream_t s = e->stream;
+    int threads = e->threads;
+    while (threads < M / 8) threads *= 2;
+    st = hip_ok(hipMemcpyAsync(base + off_k, input_x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(base + off_p, input_y, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_spec, &hs, sizeof(hs), hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_grid, &hg, sizeof(hg), hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_tab, &g->tabs[0], sizeof(ce_table_desc), hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_unit, &hu, sizeof(hu), hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK && ce_launch_prepare(s, 1, d_spec, d_grid, 1, base + off_k, base + off_p, base + off_ws,
+                                            base + off_xf, threads, M))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK && ce_launch_tables(s, 1, d_unit, d_spec, d_grid, d_tab, 1, base + off_xf, 0, NULL, NULL,
+                                           NULL, base + off_out, N, NULL, NULL, 1, 1, NULL, NULL, threads, M))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(output_y, base + off_out, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    pthread_mutex_unlock(&e->lock);
+    if (st != COFFE_OK) return st;
+    for (int i = 0; i < N; ++i) output_x[i] = r0 * g->h_p[i];
+    return COFFE_OK;
+}
+
+void twofast_1bessel(double *output_x, double *output_y, size_t output_len, double *input_x, double *input_y,
+                     size_t input_len, double l, double nu, double r0, double k0, double kmin, double kmax,
+                     unsigned flag)
+{
+    (void)flag;
+    ce_last_error = twofast_1bessel_status(output_x, output_y, output_len, input_x, input_y, input_len, l, nu, r0,
+                                           k0, kmin, kmax);
+}

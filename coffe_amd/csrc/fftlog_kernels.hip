@@ -1,0 +1,871 @@
+// HIP kernels of the FFTLog engine (gfx950, FP64).
+//
+// One workgroup owns one FFT at a time.  The length-M complex transform (M = N/2,
+// N the real FFTLog length) lives in LDS as interleaved doubles and is computed by
+// self-sorting Stockham passes: every thread first pulls the R inputs of each of its
+// butterflies into registers, the workgroup synchronises, then every thread applies
+// twiddles, the radix-R DFT and stores the R outputs.  A single LDS buffer of
+// 16*M bytes is therefore enough (M <= 8192 -> 128 KiB of the 160 KiB per CU).
+// Real transforms use the half-length complex transform plus the split step.
+//
+// Kernels (reference steps they implement):
+//   k_grid_tables  spectrum-independent tables of one (N, kmin, kmax) grid:
+//                  bias*window (twofast.c:275-290), gamma-ratio kernel u_l with the
+//                  Fourier window and 1/L folded in (twofast.c:196-224, 345-353,
+//                  426-446), output prefactor times r^(n-l) (twofast.c:415-421,
+//                  integrals.c:514-522).
+//   k_prepare      per spectrum: natural cubic spline of P(k) (twofast.c:399-401),
+//                  resample onto the log grid, bias, r2c, conj (twofast.c:252-360).
+//   k_tables       per (spectrum, table): kernel multiply, c2r (twofast.c:426-477),
+//                  then either the full table or Akima at the separations
+//                  (common.c:704-710 on the table of integrals.c:564-567); the
+//                  adjoint mode adds the Akima VJP and the transposed c2r.
+//   k_project      multipoles = sum_t A[z,l,t,r] I_t(r) (the mu-integrated weights).
+//   k_prepare_vjp  per spectrum: transposed r2c, resample and spline -> dP.
+#include <hip/hip_runtime.h>
+
+#include "ce_internal.h"
+
+namespace {
+
+struct cd {
+    double x, y;
+};
+
+__device__ __forceinline__ cd mk(double a, double b) { cd r; r.x = a; r.y = b; return r; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cmul(cd a, cd b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cd cconj(cd a) { return mk(a.x, -a.y); }
+__device__ __forceinline__ cd cscale(cd a, double s) { return mk(a.x * s, a.y * s); }
+__device__ __forceinline__ cd mul_mi(cd a) { return mk(a.y, -a.x); }   // * (-i)
+__device__ __forceinline__ cd mul_pi(cd a) { return mk(-a.y, a.x); }   // * (+i)
+
+// ---- radix-R forward DFT butterflies (sign -1) --------------------------------
+template <int R> __device__ __forceinline__ void dft(cd *v);
+
+template <> __device__ __forceinline__ void dft<2>(cd *v)
+{
+    cd a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <> __device__ __forceinline__ void dft<3>(cd *v)
+{
+    const double s3 = 0.86602540378443864676372317075294;
+    cd a = v[0], s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    v[0] = cadd(a, s);
+    cd m = mk(a.x - 0.5 * s.x, a.y - 0.5 * s.y);
+    cd rot = mul_mi(cscale(d, s3));
+    v[1] = cadd(m, rot);
+    v[2] = csub(m, rot);
+}
+
+template <> __device__ __forceinline__ void dft<4>(cd *v)
+{
+    cd s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+    cd s1 = cadd(v[1], v[3]), d1 = mul_mi(csub(v[1], v[3]));
+    v[0] = cadd(s0, s1);
+    v[2] = csub(s0, s1);
+    v[1] = cadd(d0, d1);
+    v[3] = csub(d0, d1);
+}
+
+template <> __device__ __forceinline__ void dft<5>(cd *v)
+{
+    const double c1 = 0.30901699437494742410229341718282;
+    const double c2 = -0.80901699437494742410229341718282;
+    const double s1 = 0.95105651629515357211643933337938;
+    const double s2 = 0.58778525229247312916870595463907;
+    cd a = v[0];
+    cd b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+    cd e1 = csub(v[1], v[4]), e2 = csub(v[2], v[3]);
+    v[0] = cadd(a, cadd(b1, b2));
+    cd m1 = mk(a.x + c1 * b1.x + c2 * b2.x, a.y + c1 * b1.y + c2 * b2.y);
+    cd m2 = mk(a.x + c2 * b1.x + c1 * b2.x, a.y + c2 * b1.y + c1 * b2.y);
+    cd n1 = mul_mi(mk(s1 * e1.x + s2 * e2.x, s1 * e1.y + s2 * e2.y));
+    cd n2 = mul_mi(mk(s2 * e1.x - s1 * e2.x, s2 * e1.y - s1 * e2.y));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+}
+
+template <> __device__ __forceinline__ void dft<8>(cd *v)
+{
+    const double r2 = 0.70710678118654752440084436210485;
+    cd e[4] = {v[0], v[2], v[4], v[6]};
+    cd o[4] = {v[1], v[3], v[5], v[7]};
+    dft<4>(e);
+    dft<4>(o);
+    // o[k] *= W8^k
+    cd o1 = mk(r2 * (o[1].x + o[1].y), r2 * (o[1].y - o[1].x));     // (1-i)/sqrt2
+    cd o2 = mul_mi(o[2]);                                           // -i
+    cd o3 = mk(r2 * (o[3].y - o[3].x), -r2 * (o[3].x + o[3].y));    // (-1-i)/sqrt2
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+}
+
+// One Stockham pass with radix R over s[0..M); Ns = product of earlier radices.
+// tw holds e^{-2 pi i k/N} for k < N = 2M, so W_M^k = tw[2k].
+template <int R, int MAXB>
+__device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restrict__ tw)
+{
+    const int B = M / R;
+    const int T = blockDim.x;
+    const int tid = threadIdx.x;
+    cd v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * T;
+        if (j < B) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = s[j + r * B];
+        }
+    }
+    __syncthreads();
+    const int stride2 = 2 * (M / (Ns * R));
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * T;
+        if (j < B) {
+            const int jm = j % Ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * jm * stride2]);
+            dft<R>(v[b]);
+            const int base = (j - jm) * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[base + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int R>
+__device__ __forceinline__ void fft_pass_dispatch(cd *s, int M, int Ns, const cd *__restrict__ tw)
+{
+    const int B = M / R;
+    const int per = (B + (int)blockDim.x - 1) / (int)blockDim.x;
+    // the host launches at least M/8 threads, so per <= 4 for every radix
+    if (per <= 1) fft_pass<R, 1>(s, M, Ns, tw);
+    else if (per <= 2) fft_pass<R, 2>(s, M, Ns, tw);
+    else fft_pass<R, 4>(s, M, Ns, tw);
+}
+
+// forward complex FFT of length M in LDS (natural order in and out)
+__device__ __forceinline__ void fft_lds(cd *s, int M, const ce_radix_plan &rp, const cd *__restrict__ tw)
+{
+    int Ns = 1;
+    for (int p = 0; p < rp.npass; ++p) {
+        const int R = rp.radix[p];
+        switch (R) {
+        case 8: fft_pass_dispatch<8>(s, M, Ns, tw); break;
+        case 4: fft_pass_dispatch<4>(s, M, Ns, tw); break;
+        case 2: fft_pass_dispatch<2>(s, M, Ns, tw); break;
+        case 5: fft_pass_dispatch<5>(s, M, Ns, tw); break;
+        case 3: fft_pass_dispatch<3>(s, M, Ns, tw); break;
+        default: break;
+        }
+        Ns *= R;
+    }
+}
+
+// r2c split step: from Z = FFT_M(z), z_j = x_{2j} + i x_{2j+1}, produce X_k for the
+// pair (k, M-k): X_k = (Z_k + conj Z_{M-k})/2 - i/2 W_N^k (Z_k - conj Z_{M-k}).
+__device__ __forceinline__ cd r2c_split(cd Zk, cd Zmk, cd w)
+{
+    cd b = cconj(Zmk);
+    cd e = cscale(cadd(Zk, b), 0.5);
+    cd o = mul_mi(cscale(csub(Zk, b), 0.5));
+    return cadd(e, cmul(w, o));
+}
+
+// c2r pre-step for the pair: Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N}(X_k - conj X_{M-k})
+__device__ __forceinline__ cd c2r_merge(cd Xk, cd Xmk, cd w)
+{
+    cd b = cconj(Xmk);
+    cd e = cadd(Xk, b);
+    cd o = cmul(csub(Xk, b), cconj(w));
+    return cadd(e, mul_pi(o));
+}
+
+// ---- complex helpers for the gamma-ratio kernel -------------------------------
+__device__ __forceinline__ cd c_log(cd z) { return mk(log(hypot(z.x, z.y)), atan2(z.y, z.x)); }
+__device__ __forceinline__ cd c_exp(cd z)
+{
+    double s, c;
+    sincos(z.y, &s, &c);
+    const double e = exp(z.x);
+    return mk(e * c, e * s);
+}
+__device__ __forceinline__ cd c_div(cd a, cd b)
+{
+    const double d = b.x * b.x + b.y * b.y;
+    return mk((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ cd c_sin(cd z)
+{
+    double s, c;
+    sincos(z.x, &s, &c);
+    return mk(s * cosh(z.y), c * sinh(z.y));
+}
+
+// complex log Gamma, Lanczos g = 7 with 8 coefficients (twofast.c:97-144)
+__device__ cd lgamma_lanczos7(cd z)
+{
+    const double PI = 3.14159265358979323846;
+    const double coef[8] = {676.5203681218851,    -1259.1392167224028,  771.32342877765313,
+                            -176.61502916214059,  12.507343278686905,   -0.13857109526572012,
+                            9.9843695780195716e-6, 1.5056327351493116e-7};
+    bool reflect = z.x < 0.5;
+    cd zz = reflect ? mk(1.0 - z.x, -z.y) : z;
+    cd x = mk(0.99999999999980993, 0.0);
+    zz.x -= 1.0;
+    for (int i = 0; i < 8; ++i) x = cadd(x, c_div(mk(coef[i], 0.0), mk(zz.x + (double)i + 1.0, zz.y)));
+    cd t = mk(zz.x + 7.5, zz.y);
+    cd lt = c_log(t);
+    cd r = cadd(mk(log(2 * PI) / 2.0, 0.0), cmul(mk(zz.x + 0.5, zz.y), lt));
+    r = cadd(csub(r, t), c_log(x));
+    if (reflect) {
+        cd sn = c_sin(mk(PI * z.x, PI * z.y));
+        r = csub(csub(mk(log(PI), 0.0), c_log(sn)), r);
+    }
+    return r;
+}
+
+// twofast_mql: 2^(n-1) sqrt(pi) exp(lgG((1+l+n)/2) - lgG((2+l-n)/2)) alpha^(i t - q), n = q - 1 - i t
+__device__ cd mql(double t, double q, double l, double alpha)
+{
+    const double SQRT_PI = 1.7724538509055160272981674833411451827975494561224;
+    cd n = mk(q - 1.0, -t);
+    // cpow(2, n - 1) = cexp((n - 1) clog 2)
+    const double l2 = log(2.0);
+    cd p2 = c_exp(mk((n.x - 1.0) * l2, n.y * l2));
+    cd g1 = lgamma_lanczos7(mk((1.0 + l + n.x) / 2.0, n.y / 2.0));
+    cd g2 = lgamma_lanczos7(mk((2.0 + l - n.x) / 2.0, -n.y / 2.0));
+    cd unl = cscale(cmul(p2, c_exp(csub(g1, g2))), SQRT_PI);
+    const double la = log(alpha);
+    cd pa = c_exp(mk(-q * la, t * la));
+    return cmul(pa, unl);
+}
+
+// twofast_window (twofast.c:54-90); the limits are validated on the host
+__device__ __forceinline__ double kwindow(double v, double xmin, double xmax, double xl, double xr)
+{
+    const double PI = 3.14159265358979323846;
+    if (v > xl && v < xr && v > xmin && v < xmax) return 1.0;
+    if (v <= xmin || v >= xmax) return 0.0;
+    double r = (v <= xl) ? (v - xmin) / (xl - xmin) : (xmax - v) / (xmax - xr);
+    return r - sin(2 * PI * r) / 2. / PI;
+}
+
+struct grid_tab_args {
+    int N;
+    double kmin, kmax, k0, r0;
+    const double *p;
+    int ntab, nq;
+    int qidx[CE_MAX_TABLES], rpow[CE_MAX_TABLES];
+    double n[CE_MAX_TABLES], l[CE_MAX_TABLES];
+    double qnu[CE_MAX_TABLES], qprime[2];
+    double *U[CE_MAX_TABLES], *os[CE_MAX_TABLES], *bw[2];
+};
+
+__global__ void k_grid_tables(grid_tab_args a)
+{
+    const double PI = 3.14159265358979323846;
+    const int N = a.N, M = N / 2;
+    const double ratio = a.kmax / a.kmin;
+    const double G = log(ratio);
+    const double L = 2 * PI * N / G;
+    const double k0 = a.k0;
+    const double xmax = k0 * a.p[N - 1];
+    const double xl = exp(0.46) * a.kmin, xr = exp(-0.46) * xmax;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) {
+        const double x = k0 * a.p[i];
+        const double w = kwindow(x, a.kmin, xmax, xl, xr);
+        for (int q = 0; q < a.nq; ++q) a.bw[q][i] = pow(ratio, (3. - a.qprime[q]) * i / N) * w;
+        for (int t = 0; t < a.ntab; ++t) {
+            const double qp = a.qprime[a.qidx[t]];
+            const double nu = a.n[t];
+            double v = k0 * k0 * k0 * pow(ratio, -(qp * i / N)) / PI / pow(a.r0 * k0, nu) / G;
+            if (a.rpow[t] && a.n[t] >= a.l[t]) v *= pow(a.r0 * a.p[i], a.n[t] - a.l[t]);
+            a.os[t][i] = v;
+        }
+    }
+    if (i <= M) {
+        const double wf = kwindow(k0 * a.p[M - 1 + i], a.kmin, xmax, xl, xr) / L;
+        for (int t = 0; t < a.ntab; ++t) {
+            cd u = mql(2 * PI * i / G, a.qnu[t], a.l[t], k0 * a.r0);
+            a.U[t][2 * i] = u.x * wf;
+            a.U[t][2 * i + 1] = u.y * wf;
+        }
+    }
+}
+
+// ---- natural cubic spline coefficients, chunked LDL^T with overlap -----------
+// Unknowns c_1..c_{n-2} (index s = i - 1).  Each thread owns a chunk and starts
+// its forward sweep W rows early and its backward sweep W rows late; the error of
+// the truncated start decays like prod gamma_s <= 2^-W (gamma_s < 1/2 since the
+// matrix is strictly diagonally dominant), so W = 64 reproduces the sequential
+// solve to rounding.  gam/zz are global scratch (n doubles each).
+__device__ void spline_solve(const double *__restrict__ x, const double *__restrict__ rhs_y, int n,
+                             bool rhs_is_values, const double *__restrict__ rhs_direct, double *__restrict__ c,
+                             double *__restrict__ gam, double *__restrict__ zz)
+{
+    const int sys = n - 2;
+    const int T = blockDim.x, tid = threadIdx.x;
+    const int CH = (sys + T - 1) / T;
+    const int W = 64;
+    const int a = tid * CH, b = min(a + CH, sys);
+    auto h = [&](int i) { return x[i + 1] - x[i]; };
+    auto diag = [&](int s) { return 2.0 * (h(s + 1) + h(s)); };
+    auto off = [&](int s) { return h(s + 1); };
+    auto rhs = [&](int s) {
+        if (!rhs_is_values) return rhs_direct[s];
+        const double hi = h(s), hi1 = h(s + 1);
+        const double gi = hi != 0.0 ? 1.0 / hi : 0.0, gi1 = hi1 != 0.0 ? 1.0 / hi1 : 0.0;
+        return 3.0 * ((rhs_y[s + 2] - rhs_y[s + 1]) * gi1 - (rhs_y[s + 1] - rhs_y[s]) * gi);
+    };
+    if (a < b) {
+        const int s0 = max(0, a - W);
+        double alpha = diag(s0);
+        double z = rhs(s0);
+        if (s0 >= a) {
+            gam[s0] = (s0 < sys - 1) ? off(s0) / alpha : 0.0;
+            zz[s0] = z / alpha;
+        }
+        for (int s = s0 + 1; s < b; ++s) {
+            const double gprev = off(s - 1) / alpha;
+            alpha = diag(s) - off(s - 1) * gprev;
+            z = rhs(s) - gprev * z;
+            if (s >= a) {
+                gam[s] = (s < sys - 1) ? off(s) / alpha : 0.0;
+                zz[s] = z / alpha;
+            }
+        }
+    }
+    __syncthreads();
+    if (a < b) {
+        const int e = min(sys - 1, b - 1 + W);
+        double xv = zz[e];
+        if (e < b) c[e + 1] = xv;
+        for (int s = e - 1; s >= a; --s) {
+            xv = zz[s] - gam[s] * xv;
+            if (s < b) c[s + 1] = xv;
+        }
+    }
+    if (tid == 0) {
+        c[0] = 0.0;
+        c[n - 1] = 0.0;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int first_interval(const double *__restrict__ k, int n, double x)
+{
+    int lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+        const int m = (hi + lo) >> 1;
+        if (k[m] > x) hi = m;
+        else lo = m;
+    }
+    return lo;
+}
+
+// resample S(x_i) * bw[i] into LDS (as N reals); contiguous chunk per thread
+__device__ void resample(const ce_grid_dev &g, const double *__restrict__ k, const double *__restrict__ P,
+                         const double *__restrict__ c, int n, const double *__restrict__ bw, double *__restrict__ out)
+{
+    const int N = g.N;
+    const int T = blockDim.x, tid = threadIdx.x;
+    const int CH = (N + T - 1) / T;
+    const int i0 = tid * CH, i1 = min(i0 + CH, N);
+    if (i0 >= i1) return;
+    const double k0 = g.k0;
+    const double klo = k[0], khi = k[n - 1];
+    int j = first_interval(k, n, k0 * g.p[i0]);
+    for (int i = i0; i < i1; ++i) {
+        const double xq = k0 * g.p[i];
+        while (j < n - 2 && k[j + 1] <= xq) ++j;
+        double S;
+        if (!(xq >= klo && xq <= khi)) {
+            S = __builtin_nan("");
+        } else {
+            const double dx = k[j + 1] - k[j];
+            const double dy = P[j + 1] - P[j];
+            const double delx = xq - k[j];
+            const double bb = dy / dx - dx * (c[j + 1] + 2.0 * c[j]) / 3.0;
+            const double dd = (c[j + 1] - c[j]) / (3.0 * dx);
+            S = P[j] + delx * (bb + delx * (c[j] + delx * dd));
+        }
+        out[i] = bw[i] * S;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_prepare(const ce_spec_dev *__restrict__ specs,
+                                                  const ce_grid_dev *__restrict__ grids, int nq,
+                                                  const double *__restrict__ k_all, const double *__restrict__ pk_all,
+                                                  double *__restrict__ ws_all, cd *__restrict__ xf_all)
+{
+    extern __shared__ cd smem[];
+    const ce_spec_dev sp = specs[blockIdx.x];
+    const ce_grid_dev &g = grids[sp.grid];
+    const int n = sp.n_in, M = g.M;
+    const double *k = k_all + sp.in_off;
+    const double *P = pk_all + sp.in_off;
+    double *c = ws_all + sp.ws_off;
+    double *gam = c + n, *zz = gam + n;
+    const cd *tw = reinterpret_cast<const cd *>(g.tw);
+    spline_solve(k, P, n, true, nullptr, c, gam, zz);
+    for (int q = 0; q < nq; ++q) {
+        resample(g, k, P, c, n, g.bw[q], reinterpret_cast<double *>(smem));
+        __syncthreads();
+        fft_lds(smem, M, g.rp, tw);
+        cd *xf = xf_all + sp.xf_off + (int64_t)q * (M + 1);
+        for (int kk = threadIdx.x; kk <= M / 2; kk += blockDim.x) {
+            const int k1 = kk, k2 = M - kk;
+            const cd Z1 = smem[k1];
+            const cd Z2 = smem[k2 == M ? 0 : k2];
+            xf[k1] = cconj(r2c_split(Z1, Z2, tw[k1]));
+            if (k2 != k1) xf[k2] = cconj(r2c_split(Z2, Z1, tw[k2]));
+        }
+        __syncthreads();
+    }
+}
+
+// Akima (1970, non-rounded ends) value on interval [X2, X3] of the 6-point stencil
+// X[0..5], Y[0..5] (GSL akima: slopes m, weights alpha).  With dY non-null the
+// derivative of the value with respect to Y[0..5] is returned too.
+__device__ __forceinline__ double akima_stencil(const double *X, const double *Y, double xq, double *dY)
+{
+    double m[5], inv[5];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+        inv[a] = 1.0 / (X[a + 1] - X[a]);
+        m[a] = (Y[a + 1] - Y[a]) * inv[a];
+    }
+    const double h = X[3] - X[2];
+    const double delx = xq - X[2];
+    const double A = fabs(m[1] - m[0]), B = fabs(m[3] - m[2]);
+    const double ne = B + A;
+    double val;
+    double dm[5] = {0, 0, 0, 0, 0};
+    double dY2 = 1.0;
+    if (ne == 0.0) {
+        val = Y[2] + delx * m[2];
+        dm[2] = delx;
+    } else {
+        const double C = fabs(m[2] - m[1]), D = fabs(m[4] - m[3]);
+        const double nen = D + C;
+        const double alpha = A / ne;
+        double tn, beta = 0.0;
+        if (nen == 0.0) {
+            tn = m[2];
+        } else {
+            beta = C / nen;
+            tn = (1.0 - beta) * m[2] + beta * m[3];
+        }
+        const double bb = (1.0 - alpha) * m[1] + alpha * m[2];
+        const double cc = (3.0 * m[2] - 2.0 * bb - tn) / h;
+        const double dd = (bb + tn - 2.0 * m[2]) / (h * h);
+        val = Y[2] + delx * (bb + delx * (cc + dd * delx));
+        if (dY) {
+            const double u = delx / h;
+            const double dv_db = h * (u - 2 * u * u + u * u * u);
+            const double dv_dt = h * (u * u * u - u * u);
+            const double dv_dm2 = h * (3 * u * u - 2 * u * u * u);
+            dm[2] += dv_dm2;
+            // b = m1 + alpha (m2 - m1)
+            dm[1] += dv_db * (1.0 - alpha);
+            dm[2] += dv_db * alpha;
+            const double dv_dalpha = dv_db * (m[2] - m[1]);
+            const double dalpha_dA = B / (ne * ne), dalpha_dB = -A / (ne * ne);
+            const double sA = (m[1] > m[0]) - (m[1] < m[0]);
+            const double sB = (m[3] > m[2]) - (m[3] < m[2]);
+            dm[1] += dv_dalpha * dalpha_dA * sA;
+            dm[0] -= dv_dalpha * dalpha_dA * sA;
+            dm[3] += dv_dalpha * dalpha_dB * sB;
+            dm[2] -= dv_dalpha * dalpha_dB * sB;
+            if (nen == 0.0) {
+                dm[2] += dv_dt;
+            } else {
+                dm[2] += dv_dt * (1.0 - beta);
+                dm[3] += dv_dt * beta;
+                const double dv_dbeta = dv_dt * (m[3] - m[2]);
+                const double dbeta_dC = D / (nen * nen), dbeta_dD = -C / (nen * nen);
+                const double sC = (m[2] > m[1]) - (m[2] < m[1]);
+                const double sD = (m[4] > m[3]) - (m[4] < m[3]);
+                dm[2] += dv_dbeta * dbeta_dC * sC;
+                dm[1] -= dv_dbeta * dbeta_dC * sC;
+                dm[4] += dv_dbeta * dbeta_dD * sD;
+                dm[3] -= dv_dbeta * dbeta_dD * sD;
+            }
+        }
+    }
+    if (dY) {
+#pragma unroll
+        for (int a = 0; a < 6; ++a) dY[a] = 0.0;
+        dY[2] = dY2;
+#pragma unroll
+        for (int a = 0; a < 5; ++a) {
+            dY[a + 1] += dm[a] * inv[a];
+            dY[a] -= dm[a] * inv[a];
+        }
+    }
+    return val;
+}
+
+// table value i of the c2r output held (as conj of the forward transform) in LDS
+__device__ __forceinline__ double c2r_value(const cd *s, int i)
+{
+    const cd z = s[i >> 1];
+    return (i & 1) ? -z.y : z.x;
+}
+
+struct table_args {
+    const ce_unit_dev *units;
+    const ce_spec_dev *specs;
+    const ce_grid_dev *grids;
+    const ce_table_desc *tabs;
+    int ntab;
+    const cd *xf;
+    int nsep;
+    const int32_t *sidx;   // [grid][nsep] FFT interval index
+    const double *sep;
+    double *itab;          // [spec][tab][nsep]
+    double *full;          // [unit][full_stride] or null
+    int64_t full_stride;
+    const double *A;       // [nz][nl][ntab][nsep] (adjoint only)
+    const double *gout;    // [spec][nz][nl][nsep] (adjoint only)
+    int nz, nl;
+    cd *xbar;              // adjoint output, per spec ntab*(M+1) at specs[].xb_off
+    double *wbuf;          // [unit][nsep][6] scratch
+};
+
+__global__ void __launch_bounds__(1024) k_tables(table_args a)
+{
+    extern __shared__ cd smem[];
+    const ce_unit_dev u = a.units[blockIdx.x];
+    const ce_spec_dev sp = a.specs[u.spec];
+    const ce_grid_dev &g = a.grids[sp.grid];
+    const ce_table_desc td = a.tabs[u.tab];
+    const int M = g.M, N = g.N;
+    const int T = blockDim.x, tid = threadIdx.x;
+    const cd *tw = reinterpret_cast<const cd *>(g.tw);
+    const cd *U = reinterpret_cast<const cd *>(g.U[u.tab]);
+    const double *os = g.os[u.tab];
+    const cd *xf = a.xf + sp.xf_off + (int64_t)td.qidx * (M + 1);
+    // kernel multiply + c2r merge (stored conjugated: the forward FFT then gives conj(z))
+    for (int kk = tid; kk <= M / 2; kk += T) {
+        const int k1 = kk, k2 = M - kk;
+        cd X1 = cmul(xf[k1], U[k1]);
+        cd X2 = cmul(xf[k2], U[k2]);
+        if (k1 == 0) { X1.y = 0.0; X2.y = 0.0; }
+        smem[k1] = cconj(c2r_merge(X1, X2, tw[k1]));
+        if (k2 != k1 && k2 < M) smem[k2] = cconj(c2r_merge(X2, X1, tw[k2]));
+    }
+    __syncthreads();
+    fft_lds(smem, M, g.rp, tw);
+    if (a.full) {
+        double *out = a.full + (int64_t)blockIdx.x * a.full_stride;
+        for (int i = tid; i < N; i += T) out[i] = os[i] * c2r_value(smem, i);
+    }
+    const int32_t *sidx = a.sidx + (int64_t)sp.grid * a.nsep;
+    const bool adj = a.gout != nullptr;
+    double *wb = a.wbuf + (int64_t)blockIdx.x * a.nsep * 6;
+    for (int r = tid; r < a.nsep; r += T) {
+        const int j = sidx[r];
+        double X[6], Y[6], dY[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            X[m] = g.r0 * g.p[j - 2 + m];
+            Y[m] = os[j - 2 + m] * c2r_value(smem, j - 2 + m);
+        }
+        const double v = akima_stencil(X, Y, a.sep[r], adj ? dY : nullptr);
+        if (a.itab) a.itab[((int64_t)u.spec * a.ntab + u.tab) * a.nsep + r] = v;
+        if (adj) {
+            double ib = 0.0;
+            for (int z = 0; z < a.nz; ++z)
+                for (int l = 0; l < a.nl; ++l)
+                    ib += a.A[(((int64_t)z * a.nl + l) * a.ntab + u.tab) * a.nsep + r] *
+                          a.gout[(((int64_t)u.spec * a.nz + z) * a.nl + l) * a.nsep + r];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) wb[r * 6 + m] = ib * dY[m] * os[j - 2 + m];
+        }
+    }
+    if (!adj) return;
+    __syncthreads();
+    double *sr = reinterpret_cast<double *>(smem);
+    for (int i = tid; i < N; i += T) sr[i] = 0.0;
+    __syncthreads();
+    for (int r = tid; r < a.nsep; r += T) {
+        const int j = sidx[r];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) atomicAdd(&sr[j - 2 + m], wb[r * 6 + m]);
+    }
+    __syncthreads();
+    // transposed c2r: Xbar_k = r2c(ybar)_k, doubled for 0 < k < M, real at k = 0, M
+    fft_lds(smem, M, g.rp, tw);
+    cd *xb = a.xbar + sp.xb_off + (int64_t)u.tab * (M + 1);
+    for (int kk = tid; kk <= M / 2; kk += T) {
+        const int k1 = kk, k2 = M - kk;
+        const cd Z1 = smem[k1];
+        const cd Z2 = smem[k2 == M ? 0 : k2];
+        cd R1 = r2c_split(Z1, Z2, tw[k1]);
+        cd R2 = r2c_split(Z2, Z1, tw[k2]);
+        R1 = (k1 == 0) ? mk(R1.x, 0.0) : cscale(R1, 2.0);
+        R2 = (k2 == M) ? mk(R2.x, 0.0) : cscale(R2, 2.0);
+        // X = xf * U  ->  xf_bar = conj(U) * X_bar
+        xb[k1] = cmul(cconj(U[k1]), R1);
+        if (k2 != k1) xb[k2] = cmul(cconj(U[k2]), R2);
+    }
+}
+
+__global__ void k_project(int nspec, int ntab, int nz, int nl, int nsep, const double *__restrict__ A,
+                          const double *__restrict__ itab, double *__restrict__ out)
+{
+    const int64_t total = (int64_t)nspec * nz * nl * nsep;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(idx % nsep);
+        const int64_t rest = idx / nsep;
+        const int l = (int)(rest % nl);
+        const int z = (int)((rest / nl) % nz);
+        const int64_t s = rest / ((int64_t)nl * nz);
+        double acc = 0.0;
+        for (int t = 0; t < ntab; ++t)
+            acc += A[(((int64_t)z * nl + l) * ntab + t) * nsep + r] * itab[((int64_t)s * ntab + t) * nsep + r];
+        out[idx] = acc;
+    }
+}
+
+// first resample index i with kmin p_i >= v (p increasing)
+__device__ __forceinline__ int lower_index(const double *p, int N, double kmin, double v)
+{
+    int lo = 0, hi = N;
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (kmin * p[m] < v) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(1024) k_prepare_vjp(const ce_spec_dev *__restrict__ specs,
+                                                      const ce_grid_dev *__restrict__ grids,
+                                                      const ce_table_desc *__restrict__ tabs, int ntab, int nq,
+                                                      const double *__restrict__ k_all,
+                                                      const double *__restrict__ pk_all, double *__restrict__ ws_all,
+                                                      const cd *__restrict__ xbar_all, double *__restrict__ gpk_all)
+{
+    extern __shared__ cd smem[];
+    const ce_spec_dev sp = specs[blockIdx.x];
+    const ce_grid_dev &g = grids[sp.grid];
+    const int n = sp.n_in, M = g.M, N = g.N;
+    const int T = blockDim.x, tid = threadIdx.x;
+    const double *k = k_all + sp.in_off;
+    double *c = ws_all + sp.ws_off;          // spline coefficients from the forward pass
+    double *gam = c + n, *zz = gam + n;
+    double *lam = zz + n;                    // n: cbar (rhs of the transposed solve)
+    double *sbar = lam + n;                  // N: cotangent of the resampled spline values
+    double *lsol = sbar + N;                 // n: lambda
+    const cd *tw = reinterpret_cast<const cd *>(g.tw);
+    const cd *xbar = xbar_all + sp.xb_off;
+    double *gpk = gpk_all + sp.in_off;
+    for (int q = 0; q < nq; ++q) {
+        // Ybar = conj(sum_t xbar_t);  ybar = c2r(Z), Z_0 = Ybar_0, Z_M = Ybar_M, Z_k = Ybar_k / 2
+        for (int kk = tid; kk <= M / 2; kk += T) {
+            const int k1 = kk, k2 = M - kk;
+            cd S1 = mk(0, 0), S2 = mk(0, 0);
+            for (int t = 0; t < ntab; ++t) {
+                if (tabs[t].qidx != q) continue;
+                S1 = cadd(S1, xbar[(int64_t)t * (M + 1) + k1]);
+                S2 = cadd(S2, xbar[(int64_t)t * (M + 1) + k2]);
+            }
+            cd Y1 = cconj(S1), Y2 = cconj(S2);
+            Y1 = (k1 == 0) ? mk(Y1.x, 0.0) : cscale(Y1, 0.5);
+            Y2 = (k2 == M) ? mk(Y2.x, 0.0) : cscale(Y2, 0.5);
+            smem[k1] = cconj(c2r_merge(Y1, Y2, tw[k1]));
+            if (k2 != k1 && k2 < M) smem[k2] = cconj(c2r_merge(Y2, Y1, tw[k2]));
+        }
+        __syncthreads();
+        fft_lds(smem, M, g.rp, tw);
+        const double *bw = g.bw[q];
+        for (int i = tid; i < N; i += T) {
+            const double v = bw[i] * c2r_value(smem, i);
+            sbar[i] = (q == 0) ? v : sbar[i] + v;
+        }
+        __syncthreads();
+    }
+    // spline transpose: gather the resample contributions per input node j
+    const double kmin = g.k0;
+    for (int j = tid; j < n; j += T) {
+        double pb = 0.0, cb = 0.0;
+        // interval j (node j is the lower end)
+        if (j <= n - 2) {
+            const int i0 = lower_index(g.p, N, kmin, k[j]);
+            const double dx = k[j + 1] - k[j];
+            for (int i = i0; i < N; ++i) {
+                const double xq = kmin * g.p[i];  // kmin holds k0 here
+                if (!(xq < k[j + 1] || (j == n - 2 && xq <= k[j + 1]))) break;
+                const double d = xq - k[j];
+                const double s = sbar[i];
+                pb += s * (1.0 - d / dx);
+                cb += s * (-2.0 * d * dx / 3.0 + d * d - d * d * d / (3.0 * dx));
+            }
+        }
+        // interval j-1 (node j is the upper end)
+        if (j >= 1) {
+            const int i0 = lower_index(g.p, N, kmin, k[j - 1]);
+            const double dx = k[j] - k[j - 1];
+            for (int i = i0; i < N; ++i) {
+                const double xq = kmin * g.p[i];  // kmin holds k0 here
+                if (!(xq < k[j] || (j - 1 == n - 2 && xq <= k[j]))) break;
+                const double d = xq - k[j - 1];
+                const double s = sbar[i];
+                pb += s * d / dx;
+                cb += s * (-d * dx / 3.0 + d * d * d / (3.0 * dx));
+            }
+        }
+        gpk[j] = pb;
+        if (j >= 1 && j <= n - 2) lam[j - 1] = cb;   // rhs of the transposed solve
+    }
+    __syncthreads();
+    // lambda = T^{-1} cbar (T symmetric); gam/zz are reused as scratch
+    spline_solve(k, nullptr, n, false, lam, lsol, gam, zz);
+    // lsol holds lambda at positions 1..n-2 (0 at the ends)
+    for (int j = tid; j < n; j += T) {
+        auto hh = [&](int i) { return k[i + 1] - k[i]; };
+        double acc = gpk[j];
+        // g_s = 3((y_{s+2}-y_{s+1})/h_{s+1} - (y_{s+1}-y_s)/h_s), lambda_s at sbar[s+1]
+        const int sys = n - 2;
+        // s = j - 2: dg/dy_{s+2} = 3/h_{s+1}
+        if (j - 2 >= 0 && j - 2 < sys) acc += lsol[j - 1] * 3.0 / hh(j - 1);
+        // s = j - 1: dg/dy_{s+1} = -3/h_{s+1} - 3/h_s
+        if (j - 1 >= 0 && j - 1 < sys) acc += lsol[j] * (-3.0 / hh(j) - 3.0 / hh(j - 1));
+        // s = j: dg/dy_s = 3/h_s
+        if (j < sys) acc += lsol[j + 1] * 3.0 / hh(j);
+        gpk[j] = acc;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+extern "C" {
+
+static int set_lds(const void *fn, size_t bytes)
+{
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return e == hipSuccess ? 0 : (int)e;
+}
+
+int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double k0, double r0, const double *d_p,
+                          int ntab, const ce_table_desc *tabs, const double *tab_qnu, int nq, const double *qprime,
+                          double *const *d_U, double *const *d_os, double *const *d_bw)
+{
+    grid_tab_args a;
+    a.N = N;
+    a.kmin = kmin;
+    a.kmax = kmax;
+    a.k0 = k0;
+    a.r0 = r0;
+    a.p = d_p;
+    a.ntab = ntab;
+    a.nq = nq;
+    for (int t = 0; t < ntab; ++t) {
+        a.n[t] = tabs[t].n;
+        a.l[t] = tabs[t].l;
+        a.qidx[t] = tabs[t].qidx;
+        a.rpow[t] = tabs[t].rpow;
+        a.qnu[t] = tab_qnu[t];
+        a.U[t] = d_U[t];
+        a.os[t] = d_os[t];
+    }
+    for (int q = 0; q < nq; ++q) {
+        a.qprime[q] = qprime[q];
+        a.bw[q] = d_bw[q];
+    }
+    const int threads = 256;
+    const int blocks = (N + threads - 1) / threads;
+    hipLaunchKernelGGL(k_grid_tables, dim3(blocks), dim3(threads), 0, (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+int ce_launch_prepare(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids, int nq,
+                      const double *d_k, const double *d_pk, double *d_ws, double *d_xf, int threads, int max_M)
+{
+    const size_t lds = (size_t)max_M * sizeof(cd);
+    int st = set_lds((const void *)k_prepare, lds);
+    if (st) return st;
+    hipLaunchKernelGGL(k_prepare, dim3(nspec), dim3(threads), lds, (hipStream_t)stream, d_spec, d_grids, nq, d_k,
+                       d_pk, d_ws, reinterpret_cast<cd *>(d_xf));
+    return (int)hipGetLastError();
+}
+
+int ce_launch_tables(void *stream, int nunit, const ce_unit_dev *d_units, const ce_spec_dev *d_spec,
+                     const ce_grid_dev *d_grids, const ce_table_desc *d_tabs, int ntab, const double *d_xf,
+                     int nsep, const int32_t *d_sidx, const double *d_sep, double *d_itab, double *d_full,
+                     int64_t full_stride, const double *d_A, const double *d_gout, int nz, int nl, double *d_xbar,
+                     double *d_wbuf, int threads, int max_M)
+{
+    table_args a;
+    a.units = d_units;
+    a.specs = d_spec;
+    a.grids = d_grids;
+    a.tabs = d_tabs;
+    a.ntab = ntab;
+    a.xf = reinterpret_cast<const cd *>(d_xf);
+    a.nsep = nsep;
+    a.sidx = d_sidx;
+    a.sep = d_sep;
+    a.itab = d_itab;
+    a.full = d_full;
+    a.full_stride = full_stride;
+    a.A = d_A;
+    a.gout = d_gout;
+    a.nz = nz;
+    a.nl = nl;
+    a.xbar = reinterpret_cast<cd *>(d_xbar);
+    a.wbuf = d_wbuf;
+    const size_t lds = (size_t)max_M * sizeof(cd);
+    int st = set_lds((const void *)k_tables, lds);
+    if (st) return st;
+    hipLaunchKernelGGL(k_tables, dim3(nunit), dim3(threads), lds, (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+int ce_launch_project(void *stream, int nspec, int ntab, int nz, int nl, int nsep, const double *d_A,
+                      const double *d_itab, double *d_out)
+{
+    const int64_t total = (int64_t)nspec * nz * nl * nsep;
+    const int threads = 256;
+    int64_t blocks = (total + threads - 1) / threads;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_project, dim3((unsigned)blocks), dim3(threads), 0, (hipStream_t)stream, nspec, ntab, nz,
+                       nl, nsep, d_A, d_itab, d_out);
+    return (int)hipGetLastError();
+}
+
+int ce_launch_prepare_vjp(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
+                          const ce_table_desc *d_tabs, int ntab, int nq, const double *d_k, const double *d_pk,
+                          double *d_ws, const double *d_xbar, double *d_gpk, int threads, int max_M)
+{
+    const size_t lds = (size_t)max_M * sizeof(cd);
+    int st = set_lds((const void *)k_prepare_vjp, lds);
+    if (st) return st;
+    hipLaunchKernelGGL(k_prepare_vjp, dim3(nspec), dim3(threads), lds, (hipStream_t)stream, d_spec, d_grids, d_tabs,
+                       ntab, nq, d_k, d_pk, d_ws, reinterpret_cast<const cd *>(d_xbar), d_gpk);
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+"""Python handle on the batched engine (device memory through torch tensors)."""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, load_library
+
+TABLE_ORDER = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]  # (n, l), integrals.c:599-608
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Engine:
+    """One engine per device (stream + cache of spectrum-independent grid tables)."""
+
+    def __init__(self, device=0):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        check(self.L.coffe_engine_create(device, ctypes.byref(h)), "coffe_engine_create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.L.coffe_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(self.L.coffe_engine_synchronize(self.h), "synchronize")
+
+
+class Batch:
+    """A fixed batch layout: ragged spectra, tables, separations, (z, l) rows."""
+
+    def __init__(self, engine, n_in, n_fft, kmin, kmax, tables, sep, nz, nl):
+        self.engine = engine
+        L = engine.L
+        self.nspec = len(n_in)
+        self.n_in = np.asarray(n_in, dtype=np.int64)
+        self.n_fft = np.asarray(n_fft, dtype=np.int64)
+        self.kmin = np.ascontiguousarray(kmin, dtype=np.float64)
+        self.kmax = np.ascontiguousarray(kmax, dtype=np.float64)
+        self.tables = list(tables)
+        self.sep = np.ascontiguousarray(sep, dtype=np.float64)
+        self.nz, self.nl = nz, nl
+        tn = (ctypes.c_int * len(self.tables))(*[t[0] for t in self.tables])
+        tl = (ctypes.c_int * len(self.tables))(*[t[1] for t in self.tables])
+        h = ctypes.c_void_p()
+        dp = ctypes.POINTER(ctypes.c_double)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        check(L.coffe_batch_create(engine.h, self.nspec, self.n_in.ctypes.data_as(i64p),
+                                   self.n_fft.ctypes.data_as(i64p), self.kmin.ctypes.data_as(dp),
+                                   self.kmax.ctypes.data_as(dp), len(self.tables), tn, tl, len(self.sep),
+                                   self.sep.ctypes.data_as(dp), nz, nl, ctypes.byref(h)), "coffe_batch_create")
+        self.h = h
+        self.L = L
+        self.offsets = np.concatenate([[0], np.cumsum(self.n_in)])
+
+    def close(self):
+        if self.h:
+            self.L.coffe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream():
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def forward(self, k, pk, A=None, itab=None, out=None, stream=None):
+        """k, pk: float64 cuda tensors (concatenated); A: (nz, nl, ntab, nsep)."""
+        import torch
+        ntab, nsep = len(self.tables), len(self.sep)
+        if itab is None:
+            itab = torch.empty((self.nspec, ntab, nsep), dtype=torch.float64, device=k.device)
+        if A is not None and out is None:
+            out = torch.empty((self.nspec, self.nz, self.nl, nsep), dtype=torch.float64, device=k.device)
+        check(self.L.coffe_batch_forward(self.h, _ptr(k), _ptr(pk), _ptr(A), _ptr(itab), _ptr(out),
+                                         stream if stream is not None else self._stream()), "coffe_batch_forward")
+        return itab, out
+
+    def vjp(self, k, pk, A, gout, itab=None, out=None, gpk=None, stream=None):
+        import torch
+        ntab, nsep = len(self.tables), len(self.sep)
+        if itab is None:
+            itab = torch.empty((self.nspec, ntab, nsep), dtype=torch.float64, device=k.device)
+        if out is None:
+            out = torch.empty((self.nspec, self.nz, self.nl, nsep), dtype=torch.float64, device=k.device)
+        if gpk is None:
+            gpk = torch.empty_like(pk)
+        check(self.L.coffe_batch_vjp(self.h, _ptr(k), _ptr(pk), _ptr(A), _ptr(gout), _ptr(itab), _ptr(out),
+                                     _ptr(gpk), stream if stream is not None else self._stream()), "coffe_batch_vjp")
+        return itab, out, gpk
+
+    def tables_full(self, k, pk, stream=None):
+        import torch
+        ntab = len(self.tables)
+        full = torch.zeros((self.nspec, ntab, int(self.n_fft.max())), dtype=torch.float64, device=k.device)
+        check(self.L.coffe_batch_tables(self.h, _ptr(k), _ptr(pk), _ptr(full),
+                                        stream if stream is not None else self._stream()), "coffe_batch_tables")
+        return full
+
+
+def twofast_1bessel(k, pk, N, l, nu, r0=1.0, k0=None, kmin=None, kmax=None):
+    """Drop-in twofast_1bessel (twofast.c:362-483) through the C-ABI; host arrays."""
+    L = load_library()
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    pk = np.ascontiguousarray(pk, dtype=np.float64)
+    kmin = float(k[0]) if kmin is None else kmin
+    kmax = float(k[-1]) if kmax is None else kmax
+    k0 = kmin if k0 is None else k0
+    ox, oy = np.empty(N), np.empty(N)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(L.twofast_1bessel_status(ox.ctypes.data_as(dp), oy.ctypes.data_as(dp), N, k.ctypes.data_as(dp),
+                                   pk.ctypes.data_as(dp), len(k), float(l), float(nu), r0, k0, kmin, kmax),
+          "twofast_1bessel")
+    return ox, oy

@@ -1,0 +1,113 @@
+/*
+ * coffe_engine.h -- C-ABI of the MI355X spherical-Bessel integral engine.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in replacements with the reference's own signatures, so that COFFE's
+ *     callers link unchanged (see INTEGRATION.md for the binding stubs):
+ *       twofast_1bessel        replaces twofast.c:362-483 (decl. twofast.h:44-58)
+ *       two_sph_bessel         replaces twobessel.c:16-113 (decl. twobessel.h:7)
+ *     Both are void in the reference and exit() on bad input; here they record an
+ *     error code readable with coffe_engine_last_error() and never exit.  The
+ *     *_status variants return the code directly.
+ *
+ *  2. The batched engine: many spectra with ragged k-grids and FFT lengths, any set
+ *     of I_l^n tables, evaluated at a common set of separations and projected onto
+ *     multipoles with spectrum-independent weights A[z][l][t][r] (the mu-integrals
+ *     of functions_nonintegrated, functions.c:89-1484, see coffe_mu_weights_*),
+ *     plus the vector-Jacobian product with respect to every input P(k) sample.
+ *     All array arguments of the batch calls are device pointers; the stream is a
+ *     hipStream_t (NULL = the engine's own stream).
+ *
+ * All functions return 0 on success or one of the COFFE_E* codes.  Handles are
+ * re-entrant per handle; concurrent calls on one handle must be serialised by the
+ * caller.
+ */
+#ifndef COFFE_ENGINE_H
+#define COFFE_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    COFFE_OK = 0,
+    COFFE_EINVAL = 1,        /* bad argument (sizes, ordering, NULL) */
+    COFFE_ENOMEM = 2,        /* host or device allocation failed */
+    COFFE_EDEVICE = 3,       /* HIP runtime / kernel launch error */
+    COFFE_EUNSUPPORTED = 4,  /* e.g. FFT length with a prime factor > 5 */
+    COFFE_EDOMAIN = 5,       /* window limits, separation outside the table */
+    COFFE_ENONFINITE = 6,    /* NaN/inf in an input or in the gamma-ratio kernel */
+    COFFE_ENODEVICE = 7      /* no HIP device / library built without one */
+};
+
+typedef struct coffe_engine coffe_engine;
+typedef struct coffe_batch coffe_batch;
+
+const char *coffe_engine_strerror(int code);
+/* last error recorded by a void drop-in entry point on the calling thread */
+int coffe_engine_last_error(void);
+
+int coffe_engine_create(int device, coffe_engine **out);
+void coffe_engine_destroy(coffe_engine *e);
+/* the engine used by the drop-in entry points (created on first use, device 0) */
+coffe_engine *coffe_engine_default(void);
+
+/* ---- reference drop-ins -------------------------------------------------- */
+
+/* twofast.c:362-483: 1D FFTLog of an input P(k) table (natural cubic spline
+   resampled onto output_len log points); host buffers, flag ignored (FFTW effort). */
+void twofast_1bessel(double *output_x, double *output_y, size_t output_len, double *input_x, double *input_y,
+                     size_t input_len, double l, double nu, double r0, double k0, double kmin, double kmax,
+                     unsigned flag);
+int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len, const double *input_x,
+                           const double *input_y, size_t input_len, double l, double nu, double r0, double k0,
+                           double kmin, double kmax);
+
+/* ---- batched engine ------------------------------------------------------- */
+
+/*
+ * nspec spectra; spectrum s has n_in[s] input samples (k, P) stored contiguously
+ * (spectrum after spectrum) and FFT length n_fft[s] (even, N/2 = 2^a 3^b 5^c,
+ * N <= 16384).  kmin[s] / kmax[s] must equal its first / last k (as
+ * set_power_spectrum_linear does, coffe.pyx:2961-2964).  Tables t = 0..ntab-1 are
+ * I_{tab_l[t]}^{tab_n[t]} (stored as r^(n-l) I when n >= l, integrals.c:514-522).
+ * The nsep separations (host array, Mpc) must lie strictly inside the FFT part of
+ * every table's r-grid (r >= (kmax/kmin)^(2/N)); nz * nl is the number of
+ * (redshift, multipole) rows of the weights A.
+ */
+int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const int64_t *n_fft, const double *kmin,
+                       const double *kmax, int ntab, const int *tab_n, const int *tab_l, int nsep, const double *sep,
+                       int nz, int nl, coffe_batch **out);
+void coffe_batch_destroy(coffe_batch *b);
+/* total number of input samples (sum n_in) and of output values per call */
+int64_t coffe_batch_input_size(const coffe_batch *b);
+
+/*
+ * Forward: itab[s][t][r] = I_t(sep_r) for spectrum s (may be NULL) and
+ * out[s][z][l][r] = sum_t A[z][l][t][r] itab[s][t][r] (may be NULL).
+ */
+int coffe_batch_forward(coffe_batch *b, const double *d_k, const double *d_pk, const double *d_A, double *d_itab,
+                        double *d_out, void *stream);
+
+/*
+ * Forward plus vector-Jacobian product: given the cotangent gout[s][z][l][r] of
+ * out, writes gpk (same layout as pk) = d<gout, out>/dP, and optionally itab/out.
+ */
+int coffe_batch_vjp(coffe_batch *b, const double *d_k, const double *d_pk, const double *d_A, const double *d_gout,
+                    double *d_itab, double *d_out, double *d_gpk, void *stream);
+
+/* full tables: full[s][t][i] = r^(n-l) I_t(r_i), r_i = (kmax/kmin)^(i/N), i < n_fft[s];
+   row stride = max n_fft */
+int coffe_batch_tables(coffe_batch *b, const double *d_k, const double *d_pk, double *d_full, void *stream);
+
+/* synchronise the engine's stream */
+int coffe_engine_synchronize(coffe_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

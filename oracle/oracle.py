@@ -1,0 +1,285 @@
+"""ctypes bindings for oracle/liboracle.so (test infrastructure only).
+
+Each wrapper names the reference function it restates; see the C sources for
+file:line citations.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_size = ctypes.c_size_t
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = ctypes.CDLL(_LIB)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    d, p, s, i = ctypes.c_double, c_double_p, c_size, ctypes.c_int
+    L.orc_fft.argtypes = [s, p, p, i]
+    L.orc_rfft_r2c.argtypes = [s, p, p]
+    L.orc_rfft_c2r.argtypes = [s, p, p]
+    L.orc_rfft2_r2c.argtypes = [s, s, p, p]
+    L.orc_rfft2_c2r.argtypes = [s, s, p, p]
+    L.orc_cspline_interp.argtypes = [s, p, p, s, p, p]
+    L.orc_akima_interp.argtypes = [s, p, p, s, p, p]
+    L.orc_gauss_legendre.argtypes = [s, p, p]
+    L.orc_legendre_pl.argtypes = [i, d]
+    L.orc_legendre_pl.restype = d
+    L.orc_sph_jl.argtypes = [i, d]
+    L.orc_sph_jl.restype = d
+    L.orc_twofast_window.argtypes = [d, d, d, d, d, p]
+    L.orc_twofast_lgamma.argtypes = [d, d, p]
+    L.orc_twofast_lgamma.restype = None
+    L.orc_twofast_mql.argtypes = [d, d, d, d, p]
+    L.orc_twofast_selectqnu.argtypes = [d, d]
+    L.orc_twofast_selectqnu.restype = d
+    L.orc_twofast_1bessel.argtypes = [p, p, s, p, p, s, d, d, d, d, d, d]
+    L.orc_integral_table.argtypes = [i, i, s, s, p, p, d, d, i, p, p]
+    L.orc_multipoles.argtypes = [ctypes.POINTER(c_double_p), ctypes.POINTER(c_double_p), ctypes.POINTER(s), s,
+                                 p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
+                                 ctypes.POINTER(ctypes.c_int), s, p, i, d, p]
+    L.orc_g_l.argtypes = [d, d, s, p, p]
+    L.orc_g_l.restype = None
+    L.orc_two_sph_bessel.argtypes = [p, p, p, ctypes.c_long, ctypes.c_long, d, d, d, d, d, p, p, p]
+    L.orc_covariance_fftlog_integral.argtypes = [s, p, p, d, i, i, s, s, p, d, d, p]
+    L.orc_threej000.argtypes = [i, i, i]
+    L.orc_threej000.restype = d
+    L.orc_legendre_integral.argtypes = [i, i, i, i]
+    L.orc_legendre_integral.restype = d
+    L.orc_covariance_assemble.argtypes = [s, s, ctypes.POINTER(ctypes.c_int), s, p, p, p, p, p, p, p, i, i, i,
+                                          p, p, p]
+    L.orc_qag_test.argtypes = [i, d, d, d, ctypes.POINTER(ctypes.c_int)]
+    L.orc_qag_test.restype = d
+
+
+def _p(a):
+    return a.ctypes.data_as(c_double_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(st, what):
+    if st != 0:
+        raise OracleError(f"{what} failed with oracle status {st}")
+
+
+def fft(x, sign=-1):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.empty_like(x)
+    _check(lib().orc_fft(len(x), x.ctypes.data_as(c_double_p), out.ctypes.data_as(c_double_p), sign), "fft")
+    return out
+
+
+def rfft(x):
+    """FFTW r2c (unnormalised)."""
+    x = _f64(x)
+    out = np.empty(len(x) // 2 + 1, dtype=np.complex128)
+    _check(lib().orc_rfft_r2c(len(x), _p(x), out.ctypes.data_as(c_double_p)), "rfft")
+    return out
+
+
+def irfft(X, n):
+    """FFTW c2r (unnormalised; Im of DC and Nyquist ignored)."""
+    X = np.ascontiguousarray(X, dtype=np.complex128)
+    out = np.empty(n)
+    _check(lib().orc_rfft_c2r(n, X.ctypes.data_as(c_double_p), _p(out)), "irfft")
+    return out
+
+
+def rfft2(x):
+    x = _f64(x)
+    n1, n2 = x.shape
+    out = np.empty((n1, n2 // 2 + 1), dtype=np.complex128)
+    _check(lib().orc_rfft2_r2c(n1, n2, _p(x), out.ctypes.data_as(c_double_p)), "rfft2")
+    return out
+
+
+def irfft2(X, n2):
+    X = np.ascontiguousarray(X, dtype=np.complex128)
+    n1 = X.shape[0]
+    out = np.empty((n1, n2))
+    _check(lib().orc_rfft2_c2r(n1, n2, X.ctypes.data_as(c_double_p), _p(out)), "irfft2")
+    return out
+
+
+def cspline(x, y, xq):
+    x, y, xq = _f64(x), _f64(y), _f64(np.atleast_1d(xq))
+    out = np.empty(len(xq))
+    _check(lib().orc_cspline_interp(len(x), _p(x), _p(y), len(xq), _p(xq), _p(out)), "cspline")
+    return out
+
+
+def akima(x, y, xq):
+    x, y, xq = _f64(x), _f64(y), _f64(np.atleast_1d(xq))
+    out = np.empty(len(xq))
+    _check(lib().orc_akima_interp(len(x), _p(x), _p(y), len(xq), _p(xq), _p(out)), "akima")
+    return out
+
+
+def gauss_legendre(n):
+    x, w = np.empty(n), np.empty(n)
+    lib().orc_gauss_legendre(n, _p(x), _p(w))
+    return x, w
+
+
+def legendre_pl(l, x):
+    return lib().orc_legendre_pl(l, x)
+
+
+def sph_jl(l, x):
+    return lib().orc_sph_jl(l, x)
+
+
+def qag_test(kind, a, b, epsrel):
+    st = ctypes.c_int(0)
+    r = lib().orc_qag_test(kind, a, b, epsrel, ctypes.byref(st))
+    return r, st.value
+
+
+def twofast_window(value, xmin, xmax, xleft, xright):
+    out = ctypes.c_double()
+    _check(lib().orc_twofast_window(value, xmin, xmax, xleft, xright, ctypes.byref(out)), "twofast_window")
+    return out.value
+
+
+def twofast_lgamma(z):
+    out = (ctypes.c_double * 2)()
+    lib().orc_twofast_lgamma(z.real, z.imag, out)
+    return complex(out[0], out[1])
+
+
+def twofast_mql(t, q, l, alpha):
+    out = (ctypes.c_double * 2)()
+    _check(lib().orc_twofast_mql(t, q, l, alpha, out), "twofast_mql")
+    return complex(out[0], out[1])
+
+
+def twofast_selectqnu(l, nu):
+    return lib().orc_twofast_selectqnu(l, nu)
+
+
+def twofast_1bessel(k, pk, N, l, nu, r0=1.0, k0=None, kmin=None, kmax=None):
+    """twofast.c:362-483; returns (r, y)."""
+    k, pk = _f64(k), _f64(pk)
+    kmin = k[0] if kmin is None else kmin
+    kmax = k[-1] if kmax is None else kmax
+    k0 = kmin if k0 is None else k0
+    ox, oy = np.empty(N), np.empty(N)
+    _check(lib().orc_twofast_1bessel(_p(ox), _p(oy), N, _p(k), _p(pk), len(k), float(l), float(nu), r0, k0, kmin,
+                                     kmax), "twofast_1bessel")
+    return ox, oy
+
+
+def integral_table(n, l, k, pk, N, small_r=True):
+    """integrals.c:459-573; returns the (N + 43)-point table (x, y)."""
+    k, pk = _f64(k), _f64(pk)
+    ox, oy = np.empty(N + 43), np.empty(N + 43)
+    _check(lib().orc_integral_table(n, l, N, len(k), _p(k), _p(pk), k[0], k[-1], int(small_r), _p(ox), _p(oy)),
+           "integral_table")
+    return ox, oy
+
+
+TABLE_ORDER = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]  # (n, l), integrals.c:599-608
+
+
+def multipoles(tables, bg, bias_z, bias1, bias2, z_mean, ls, sep, den=True, rsd=True, d1=False, flatsky=False,
+               quad=0, epsrel=1e-5):
+    """functions.c:89-1484 + signal.c:83-111,521-542 + multipoles.c:48-136.
+
+    tables: dict (n, l) -> (x, y); bg: dict of arrays on bg['z'] (chi, D1, f, cH, G1, G2).
+    Returns array (len(z_mean), len(ls), len(sep)).
+    """
+    T = len(TABLE_ORDER)
+    keep = []
+    xs = (c_double_p * T)()
+    ys = (c_double_p * T)()
+    ns = (c_size * T)()
+    for t, key in enumerate(TABLE_ORDER):
+        if key in tables:
+            x, y = _f64(tables[key][0]), _f64(tables[key][1])
+            keep += [x, y]
+            xs[t], ys[t], ns[t] = _p(x), _p(y), len(x)
+        else:
+            xs[t], ys[t], ns[t] = None, None, 0
+    arrs = {kk: _f64(bg[kk]) for kk in ("z", "chi", "D1", "f", "cH", "G1", "G2")}
+    bz, b1, b2 = _f64(bias_z), _f64(bias1), _f64(bias2)
+    zm, sp = _f64(np.atleast_1d(z_mean)), _f64(np.atleast_1d(sep))
+    lsa = (ctypes.c_int * len(ls))(*ls)
+    out = np.empty(len(zm) * len(ls) * len(sp))
+    _check(lib().orc_multipoles(xs, ys, ns, len(arrs["z"]), _p(arrs["z"]), _p(arrs["chi"]), _p(arrs["D1"]),
+                                _p(arrs["f"]), _p(arrs["cH"]), _p(arrs["G1"]), _p(arrs["G2"]), len(bz), _p(bz),
+                                _p(b1), _p(b2), int(den), int(rsd), int(d1), int(flatsky), len(zm), _p(zm), len(ls),
+                                lsa, len(sp), _p(sp), int(quad), float(epsrel), _p(out)), "multipoles")
+    return out.reshape(len(zm), len(ls), len(sp))
+
+
+def g_l(l, nu, eta):
+    eta = _f64(eta)
+    out = np.empty(len(eta), dtype=np.complex128)
+    lib().orc_g_l(l, nu, len(eta), _p(eta), out.ctypes.data_as(c_double_p))
+    return out
+
+
+def two_sph_bessel(k1, k2, fk, l1, l2, nu1=1.01, nu2=1.01, width=0.25):
+    k1, k2, fk = _f64(k1), _f64(k2), _f64(fk)
+    N1, N2 = len(k1), len(k2)
+    r1, r2, res = np.empty(N1), np.empty(N2), np.empty((N1, N2))
+    _check(lib().orc_two_sph_bessel(_p(k1), _p(k2), _p(fk), N1, N2, l1, l2, nu1, nu2, width, _p(r1), _p(r2),
+                                    _p(res)), "two_sph_bessel")
+    return r1, r2, res
+
+
+def covariance_fftlog_integral(k, pk, power, l1, l2, N, sep, kmin=None, kmax=None):
+    """covariance.c:357-470; returns R x R array indexed [n, m] (value at sep_m, sep_n)."""
+    k, pk, sep = _f64(k), _f64(pk), _f64(sep)
+    kmin = k[0] if kmin is None else kmin
+    kmax = k[-1] if kmax is None else kmax
+    R = len(sep)
+    out = np.empty(R * R)
+    _check(lib().orc_covariance_fftlog_integral(len(k), _p(k), _p(pk), power, l1, l2, N, R, _p(sep), kmin, kmax,
+                                                _p(out)), "covariance_fftlog_integral")
+    return out.reshape(R, R)
+
+
+def threej000(a, b, c):
+    return lib().orc_threej000(a, b, c)
+
+
+def legendre_integral(n, m, a, b):
+    return lib().orc_legendre_integral(n, m, a, b)
+
+
+def covariance_assemble(ls, sep, bias, f, D1, nbar, vol, pix, pk_int, pk2_int, poisson=True, mixed=True,
+                        cosmic=True):
+    nz, nl, R = len(bias), len(ls), len(sep)
+    lsa = (ctypes.c_int * nl)(*ls)
+    arrs = [_f64(np.atleast_1d(a)) for a in (sep, bias, f, D1, nbar, vol, pix)]
+    pk_int, pk2_int = _f64(pk_int), _f64(pk2_int)
+    out = np.empty(nz * nl * R * nl * R)
+    _check(lib().orc_covariance_assemble(nz, nl, lsa, R, *[_p(a) for a in arrs], int(poisson), int(mixed),
+                                         int(cosmic), _p(pk_int), _p(pk2_int), _p(out)), "covariance_assemble")
+    return out.reshape(nz, nl, R, nl, R)

@@ -1,0 +1,220 @@
+/*
+ * Oracle (test infrastructure only): multipoles of the local (non-integrated)
+ * density, RSD and Doppler (d1) terms, restated from
+ *   functions_nonintegrated   functions.c:89-1484 (den-den 129-229, rsd-rsd 230-362,
+ *                             d1-d1 363-573, den-rsd 670-771, den-d1 772-855,
+ *                             rsd-d1 904-1143; growth factors 1429-1469),
+ *   multipoles_nonintegrated_integrand   signal.c:83-111 (mu = 2x - 1, x P_l(mu)),
+ *   coffe_integrate NONINTEGRATED/MULTIPOLES   signal.c:521-542 ((2l+1) * QAG),
+ *   coffe_multipoles_init     multipoles.c:48-136 (z-major, then l, then r).
+ * Background quantities and the I_l^n tables are inputs (Akima splines, as the
+ * reference builds them with the default `interp_method`).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+typedef struct {
+    size_t n;
+    const double *x, *y;
+    double *b, *c, *d;
+} akima_t;
+
+static int ak_init(akima_t *a, size_t n, const double *x, const double *y)
+{
+    a->n = n; a->x = x; a->y = y;
+    a->b = (double *)malloc(sizeof(double) * 3 * n);
+    if (!a->b) return ORC_ENOMEM;
+    a->c = a->b + n; a->d = a->c + n;
+    return orc_akima_init(n, x, y, a->b, a->c, a->d);
+}
+
+static double ak(const akima_t *a, double x)
+{
+    return orc_akima_eval(a->n, a->x, a->y, a->b, a->c, a->d, x);
+}
+
+/* table order: I00, I02, I04, I11, I13, I20, I22, I31 (integrals.c:599-608) */
+enum { T00 = 0, T02, T04, T11, T13, T20, T22, T31, NTAB };
+
+typedef struct {
+    akima_t tab[NTAB];
+    int have[NTAB];
+    akima_t chi, z_of_chi, D1, f, cH, G1, G2, b1, b2;
+    int den, rsd, d1, flatsky;
+} mp_ctx;
+
+typedef struct {
+    const mp_ctx *ctx;
+    double z_mean, sep;
+    int l;
+} mp_point;
+
+static double I(const mp_ctx *c, int t, double r)
+{
+    return ak(&c->tab[t], r);
+}
+
+double orc_functions_nonintegrated(const void *vctx, double z_mean, double mu, double sep)
+{
+    const mp_ctx *c = (const mp_ctx *)vctx;
+    const double chi_mean = ak(&c->chi, z_mean);
+    const double chi1 = chi_mean - sep * mu / 2.;
+    const double chi2 = chi_mean + sep * mu / 2.;
+    const double costheta = (2. * chi_mean * chi_mean - sep * sep + mu * mu * sep * sep / 2.) /
+                            (2. * chi_mean * chi_mean - mu * mu * sep * sep / 2.);
+    const double z1 = ak(&c->z_of_chi, chi1);
+    const double z2 = ak(&c->z_of_chi, chi2);
+    const double f1 = ak(&c->f, z1), f2 = ak(&c->f, z2), fmean = ak(&c->f, z_mean);
+    const double H1 = ak(&c->cH, z1), H2 = ak(&c->cH, z2), Hmean = ak(&c->cH, z_mean);
+    const double b1 = ak(&c->b1, z1), b2 = ak(&c->b2, z2);
+    const double bm1 = ak(&c->b1, z_mean), bm2 = ak(&c->b2, z_mean);
+    const double G1 = ak(&c->G1, z1), G2 = ak(&c->G2, z2);
+    const double Gm1 = ak(&c->G1, z_mean), Gm2 = ak(&c->G2, z_mean);
+    const double P2 = orc_legendre_pl(2, mu), P4 = orc_legendre_pl(4, mu);
+    const double ct2 = costheta * costheta;
+    double result = 0;
+    if (c->den) {
+        if (c->flatsky) result += bm1 * bm2 * I(c, T00, sep);
+        else result += b1 * b2 * I(c, T00, sep);
+    }
+    if (c->rsd) {
+        if (c->flatsky) {
+            result += fmean * fmean * I(c, T00, sep) / 5. - 4 * fmean * fmean * I(c, T02, sep) / 7. * P2 +
+                      8 * fmean * fmean * I(c, T04, sep) / 35. * P4;
+        } else {
+            result += f1 * f2 * (1 + 2 * ct2) / 15 * I(c, T00, sep) -
+                      f1 * f2 / 21. * ((1 + 11. * ct2) + 18 * costheta * (ct2 - 1) * chi1 * chi2 / sep / sep) *
+                          I(c, T02, sep) +
+                      f1 * f2 *
+                          (4 * (3 * ct2 - 1) * (pow(chi1, 4) + pow(chi2, 4)) / 35. / pow(sep, 4) +
+                           chi1 * chi2 * (3 + ct2) *
+                               (3 * (3 + ct2) * chi1 * chi2 - 8 * (chi1 * chi1 + chi2 * chi2) * costheta) / 35. /
+                               pow(sep, 4)) *
+                          I(c, T04, sep);
+        }
+    }
+    if (c->d1) {
+        if (c->flatsky) {
+            result += (Hmean * Hmean * fmean * fmean * Gm1 * Gm2 * I(c, T20, sep) / 3. +
+                       2 * Hmean * Hmean * fmean * fmean * Gm1 * Gm2 * pow(sep, 2) * I(c, T22, sep) * P2 / 3.);
+        } else {
+            result += (H1 * H2 * f1 * f2 * G1 * G2 * costheta * I(c, T20, sep) / 3. -
+                       H1 * H2 * f1 * f2 * G1 * G2 *
+                           ((chi2 - chi1 * costheta) * (chi1 - chi2 * costheta) + pow(sep, 2) * costheta / 3.) *
+                           I(c, T22, sep));
+        }
+    }
+    if (c->den && c->rsd) {
+        if (c->flatsky) {
+            result += (bm1 * fmean / 3. + bm2 * fmean / 3.) * I(c, T00, sep) -
+                      (2 * bm1 * fmean / 3. + 2 * bm2 * fmean / 3.) * I(c, T02, sep) * P2;
+        } else {
+            result += (b1 * f2 / 3. + b2 * f1 / 3.) * I(c, T00, sep) -
+                      (b1 * f2 * (2. / 3. - (1. - ct2) * pow(chi1 / sep, 2)) +
+                       b2 * f1 * (2. / 3. - (1. - ct2) * pow(chi2 / sep, 2))) *
+                          I(c, T02, sep);
+        }
+    }
+    if (c->den && c->d1) {
+        if (c->flatsky) {
+            result += (bm1 * fmean * Hmean * Gm2 * mu + bm2 * fmean * Hmean * Gm1 * (-mu)) * sep * I(c, T11, sep);
+        } else {
+            result += -(b1 * f2 * H2 * G2 * (chi1 * costheta - chi2) + b2 * f1 * H1 * G1 * (chi2 * costheta - chi1)) *
+                      I(c, T11, sep);
+        }
+    }
+    if (c->rsd && c->d1) {
+        if (c->flatsky) {
+            const double P3p = orc_legendre_pl(3, mu), P3m = orc_legendre_pl(3, -mu);
+            result += (-3. / 5. * (fmean * fmean * Hmean * Gm2 * (-mu) + fmean * fmean * Hmean * Gm1 * mu) *
+                           I(c, T11, sep) * sep +
+                       (fmean * fmean * Hmean * Gm2 * 2. / 5. * P3m + fmean * fmean * Hmean * Gm1 * 2. / 5. * P3p) *
+                           I(c, T13, sep) * sep);
+        } else {
+            result += ((f1 * f2 * H2 * G2 * ((1. + 2 * ct2) * chi2 - 3 * chi1 * costheta) / 5. +
+                        f2 * f1 * H1 * G1 * ((1. + 2 * ct2) * chi1 - 3 * chi2 * costheta) / 5.) *
+                           I(c, T11, sep) +
+                       (f1 * f2 * H2 * G2 *
+                            ((1. - 3 * ct2) * pow(chi2, 3) + costheta * (5. + ct2) * pow(chi2, 2) * chi1 -
+                             2 * (2. + ct2) * chi2 * pow(chi1, 2) + 2 * pow(chi1, 3) * costheta) /
+                            5 +
+                        f2 * f1 * H1 * G1 *
+                            ((1. - 3 * ct2) * pow(chi1, 3) + costheta * (5. + ct2) * pow(chi1, 2) * chi2 -
+                             2 * (2. + ct2) * chi1 * pow(chi2, 2) + 2 * pow(chi2, 3) * costheta) /
+                            5) *
+                           I(c, T13, sep) / pow(sep, 2));
+        }
+    }
+    if (!isfinite(result)) return NAN;
+    if (c->flatsky) return result * pow(ak(&c->D1, z_mean), 2);
+    return result * ak(&c->D1, z1) * ak(&c->D1, z2);
+}
+
+static double mp_integrand(double x, void *vp)
+{
+    const mp_point *p = (const mp_point *)vp;
+    const double mu = 2 * x - 1;
+    const double v = orc_functions_nonintegrated(p->ctx, p->z_mean, mu, p->sep);
+    return p->l == 0 ? v : v * orc_legendre_pl(p->l, mu);
+}
+
+/*
+ * tables: NTAB entries (x, y, n); an entry with n == 0 is absent.
+ * bg: z grid (nbg) with chi, D1, f, conformal H, G1, G2; bias tables on zb (nb).
+ * quad: 0 = QAG-GK61 with epsrel, M > 0 = M-point Gauss-Legendre in x.
+ */
+int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
+                   const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                   const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                   const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                   const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int quad,
+                   double epsrel, double *out)
+{
+    mp_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.den = den; c.rsd = rsd; c.d1 = d1; c.flatsky = flatsky;
+    int st = ORC_OK;
+    for (int t = 0; t < NTAB && st == ORC_OK; ++t) {
+        if (tab_n[t]) { c.have[t] = 1; st = ak_init(&c.tab[t], tab_n[t], tab_x[t], tab_y[t]); }
+    }
+    if (st == ORC_OK) st = ak_init(&c.chi, nbg, bg_z, bg_chi);
+    if (st == ORC_OK) st = ak_init(&c.z_of_chi, nbg, bg_chi, bg_z);
+    if (st == ORC_OK) st = ak_init(&c.D1, nbg, bg_z, bg_D1);
+    if (st == ORC_OK) st = ak_init(&c.f, nbg, bg_z, bg_f);
+    if (st == ORC_OK) st = ak_init(&c.cH, nbg, bg_z, bg_cH);
+    if (st == ORC_OK) st = ak_init(&c.G1, nbg, bg_z, bg_G1);
+    if (st == ORC_OK) st = ak_init(&c.G2, nbg, bg_z, bg_G2);
+    if (st == ORC_OK) st = ak_init(&c.b1, nb, zb, bias1);
+    if (st == ORC_OK) st = ak_init(&c.b2, nb, zb, bias2);
+    double *glx = NULL, *glw = NULL;
+    if (st == ORC_OK && quad > 0) {
+        glx = (double *)malloc(sizeof(double) * (size_t)quad);
+        glw = (double *)malloc(sizeof(double) * (size_t)quad);
+        orc_gauss_legendre((size_t)quad, glx, glw);
+    }
+    const size_t total = nz * nl * nsep;
+    if (st == ORC_OK) {
+        #pragma omp parallel for schedule(dynamic)
+        for (size_t idx = 0; idx < total; ++idx) {
+            const size_t iz = idx / (nl * nsep), il = (idx / nsep) % nl, ir = idx % nsep;
+            mp_point p = {&c, z_mean[iz], sep[ir], ls[il]};
+            double v;
+            if (quad > 0) {
+                double s = 0.0;
+                for (int q = 0; q < quad; ++q) s += glw[q] * mp_integrand(0.5 * (glx[q] + 1.0), &p);
+                v = 0.5 * s;
+            } else {
+                v = orc_qag(mp_integrand, &p, 0, 1, 0, epsrel, 50000, NULL, NULL);
+            }
+            out[idx] = (2 * ls[il] + 1) * v;
+        }
+    }
+    free(glx); free(glw);
+    for (int t = 0; t < NTAB; ++t) if (c.have[t]) free(c.tab[t].b);
+    free(c.chi.b); free(c.z_of_chi.b); free(c.D1.b); free(c.f.b); free(c.cH.b); free(c.G1.b); free(c.G2.b);
+    free(c.b1.b); free(c.b2.b);
+    return st;
+}
