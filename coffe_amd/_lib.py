@@ -62,6 +62,11 @@ def load_library():
     L.coffe_batch_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.coffe_batch_vjp.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.coffe_batch_tables.argtypes = [vp, vp, vp, vp, vp]
+    L.coffe_akima_eval.argtypes = [sz, dp, dp, sz, dp, dp]
+    L.coffe_batch_prepare.argtypes = [vp, vp, vp, vp]
+    L.coffe_batch_tables_stage.argtypes = [vp, vp, vp, vp, vp]
+    L.coffe_batch_project.argtypes = [vp, vp, vp, vp, vp]
+    L.coffe_batch_prepare_adj.argtypes = [vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -70,5 +75,13 @@ EXPORTED = [
     "coffe_engine_strerror", "coffe_engine_last_error", "coffe_engine_create", "coffe_engine_destroy",
     "coffe_engine_default", "coffe_engine_synchronize", "twofast_1bessel", "twofast_1bessel_status",
     "coffe_batch_create", "coffe_batch_destroy", "coffe_batch_input_size", "coffe_batch_forward",
-    "coffe_batch_vjp", "coffe_batch_tables",
+    "coffe_batch_vjp", "coffe_batch_tables", "coffe_batch_prepare", "coffe_batch_tables_stage",
+    "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute",
 ]
+
+
+def declare_background(L):
+    dp = ctypes.POINTER(ctypes.c_double)
+    d, i = ctypes.c_double, ctypes.c_int
+    L.coffe_background_compute.argtypes = [d, d, d, d, d, d, i, i, dp, dp, dp, dp, dp] + [dp] * 10
+    return L

@@ -12,7 +12,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("COFFE_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["fftlog_kernels.hip"]
-C_SOURCES = ["engine.c"]
+C_SOURCES = ["engine.c", "background.c", "interp.c"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",

@@ -478,6 +478,44 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
 
 static hipStream_t pick_stream(coffe_batch *b, void *stream) { return stream ? (hipStream_t)stream : b->e->stream; }
 
+int coffe_batch_prepare(coffe_batch *b, const double *d_k, const double *d_pk, void *stream)
+{
+    if (!b || !d_k || !d_pk) return COFFE_EINVAL;
+    return ce_launch_prepare(pick_stream(b, stream), b->nspec, b->d_spec, b->d_grids, b->nq, d_k, d_pk, b->d_ws,
+                             b->d_xf, b->threads, b->max_M)
+               ? COFFE_EDEVICE
+               : COFFE_OK;
+}
+
+int coffe_batch_tables_stage(coffe_batch *b, const double *d_A, const double *d_gout, double *d_itab, void *stream)
+{
+    if (!b || (d_gout && !d_A)) return COFFE_EINVAL;
+    double *itab = d_itab ? d_itab : b->d_itab;
+    return ce_launch_tables(pick_stream(b, stream), b->nunit, b->d_units, b->d_spec, b->d_grids, b->d_tabs, b->ntab,
+                            b->d_xf, b->nsep, b->d_sidx, b->d_sep, itab, NULL, 0, d_gout ? d_A : NULL, d_gout,
+                            b->nz, b->nl, b->d_xb, b->d_wbuf, b->threads, b->max_M)
+               ? COFFE_EDEVICE
+               : COFFE_OK;
+}
+
+int coffe_batch_project(coffe_batch *b, const double *d_A, const double *d_itab, double *d_out, void *stream)
+{
+    if (!b || !d_A || !d_out) return COFFE_EINVAL;
+    return ce_launch_project(pick_stream(b, stream), b->nspec, b->ntab, b->nz, b->nl, b->nsep, d_A,
+                             d_itab ? d_itab : b->d_itab, d_out)
+               ? COFFE_EDEVICE
+               : COFFE_OK;
+}
+
+int coffe_batch_prepare_adj(coffe_batch *b, const double *d_k, const double *d_pk, double *d_gpk, void *stream)
+{
+    if (!b || !d_k || !d_pk || !d_gpk) return COFFE_EINVAL;
+    return ce_launch_prepare_vjp(pick_stream(b, stream), b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq,
+                                 d_k, d_pk, b->d_ws, b->d_xb, d_gpk, b->threads, b->max_M)
+               ? COFFE_EDEVICE
+               : COFFE_OK;
+}
+
 int coffe_batch_forward(coffe_batch *b, const double *d_k, const double *d_pk, const double *d_A, double *d_itab,
                         double *d_out, void *stream)
 {

@@ -105,6 +105,21 @@ class Batch:
                                      _ptr(gpk), stream if stream is not None else self._stream()), "coffe_batch_vjp")
         return itab, out, gpk
 
+    # stage-by-stage launches (same kernels as forward / vjp), for pipelining and timing
+    def stage_prepare(self, k, pk, stream):
+        check(self.L.coffe_batch_prepare(self.h, _ptr(k), _ptr(pk), stream), "coffe_batch_prepare")
+
+    def stage_tables(self, A, gout, itab, stream):
+        check(self.L.coffe_batch_tables_stage(self.h, _ptr(A), _ptr(gout), _ptr(itab), stream),
+              "coffe_batch_tables_stage")
+
+    def stage_project(self, A, itab, out, stream):
+        check(self.L.coffe_batch_project(self.h, _ptr(A), _ptr(itab), _ptr(out), stream), "coffe_batch_project")
+
+    def stage_prepare_adj(self, k, pk, gpk, stream):
+        check(self.L.coffe_batch_prepare_adj(self.h, _ptr(k), _ptr(pk), _ptr(gpk), stream),
+              "coffe_batch_prepare_adj")
+
     def tables_full(self, k, pk, stream=None):
         import torch
         ntab = len(self.tables)
@@ -128,3 +143,16 @@ def twofast_1bessel(k, pk, N, l, nu, r0=1.0, k0=None, kmin=None, kmax=None):
                                    pk.ctypes.data_as(dp), len(k), float(l), float(nu), r0, k0, kmin, kmax),
           "twofast_1bessel")
     return ox, oy
+
+
+def akima(x, y, xq):
+    """Host Akima interpolation through the engine library (coffe_akima_eval)."""
+    L = load_library()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    xq = np.ascontiguousarray(np.atleast_1d(xq), dtype=np.float64)
+    out = np.empty(len(xq))
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(L.coffe_akima_eval(len(x), x.ctypes.data_as(dp), y.ctypes.data_as(dp), len(xq), xq.ctypes.data_as(dp),
+                             out.ctypes.data_as(dp)), "coffe_akima_eval")
+    return out

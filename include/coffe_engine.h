@@ -67,6 +67,10 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
                            const double *input_y, size_t input_len, double l, double nu, double r0, double k0,
                            double kmin, double kmax);
 
+/* host Akima interpolation (common.c:524-576, 704-710 with the default Akima type):
+   out[q] = spline(x, y)(xq[q]), NaN outside [x_0, x_{n-1}] */
+int coffe_akima_eval(size_t n, const double *x, const double *y, size_t nq, const double *xq, double *out);
+
 /* ---- batched engine ------------------------------------------------------- */
 
 /*
@@ -103,6 +107,21 @@ int coffe_batch_vjp(coffe_batch *b, const double *d_k, const double *d_pk, const
 /* full tables: full[s][t][i] = r^(n-l) I_t(r_i), r_i = (kmax/kmin)^(i/N), i < n_fft[s];
    row stride = max n_fft */
 int coffe_batch_tables(coffe_batch *b, const double *d_k, const double *d_pk, double *d_full, void *stream);
+
+/*
+ * The stages that coffe_batch_forward / coffe_batch_vjp launch, exposed so that a
+ * caller can interleave its own work or time each kernel on its stream:
+ *   prepare      spline + resample + r2c of every spectrum (needs d_k, d_pk)
+ *   tables       kernel multiply + c2r + Akima at the separations; with d_gout
+ *                also the Akima VJP and the transposed c2r (adjoint mode)
+ *   project      out = sum_t A itab
+ *   prepare_adj  transposed r2c, resample and spline -> gpk
+ * Stages must run in this order on one stream; the batch keeps the workspace.
+ */
+int coffe_batch_prepare(coffe_batch *b, const double *d_k, const double *d_pk, void *stream);
+int coffe_batch_tables_stage(coffe_batch *b, const double *d_A, const double *d_gout, double *d_itab, void *stream);
+int coffe_batch_project(coffe_batch *b, const double *d_A, const double *d_itab, double *d_out, void *stream);
+int coffe_batch_prepare_adj(coffe_batch *b, const double *d_k, const double *d_pk, double *d_gpk, void *stream);
 
 /* synchronise the engine's stream */
 int coffe_engine_synchronize(coffe_engine *e);

@@ -77,6 +77,9 @@ int orc_twofast_1bessel(double *output_x, double *output_y, size_t N, const doub
 int orc_integral_table(int n, int l, size_t N, size_t nin, const double *kin, const double *pin, double kmin,
                        double kmax, int with_small_r, double *out_x, double *out_y);
 
+int orc_tables_at_sep_batch(int nspec, const long long *off, const double *k, const double *p, const long long *nfft,
+                            int ntab, const int *tab_n, const int *tab_l, size_t nsep, const double *sep, double *out);
+
 /* multipoles (orc_multipoles.c) */
 double orc_functions_nonintegrated(const void *ctx, double z_mean, double mu, double sep);
 int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
@@ -85,6 +88,13 @@ int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const
                    const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
                    const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int quad,
                    double epsrel, double *out);
+
+int orc_reference_batch(int nspec, const long long *off, const double *k, const double *p, const long long *nfft,
+                        size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                        const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                        const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                        const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int small_r,
+                        double epsrel, double *out);
 
 /* 2D FFTLog and covariance (orc_covariance.c) */
 void orc_g_l(double l, double nu, size_t n, const double *eta, double *gl);

@@ -66,6 +66,11 @@ def _declare(L):
     L.orc_legendre_integral.restype = d
     L.orc_covariance_assemble.argtypes = [s, s, ctypes.POINTER(ctypes.c_int), s, p, p, p, p, p, p, p, i, i, i,
                                           p, p, p]
+    ll = ctypes.POINTER(ctypes.c_longlong)
+    L.orc_reference_batch.argtypes = [i, ll, p, p, ll, s, p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
+                                      ctypes.POINTER(ctypes.c_int), s, p, i, d, p]
+    L.orc_tables_at_sep_batch.argtypes = [i, ll, p, p, ll, i, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), s, p, p]
     L.orc_qag_test.argtypes = [i, d, d, d, ctypes.POINTER(ctypes.c_int)]
     L.orc_qag_test.restype = d
 
@@ -283,3 +288,41 @@ def covariance_assemble(ls, sep, bias, f, D1, nbar, vol, pix, pk_int, pk2_int, p
     _check(lib().orc_covariance_assemble(nz, nl, lsa, R, *[_p(a) for a in arrs], int(poisson), int(mixed),
                                          int(cosmic), _p(pk_int), _p(pk2_int), _p(out)), "covariance_assemble")
     return out.reshape(nz, nl, R, nl, R)
+
+
+def reference_batch(specs, nfft, bg, bias_z, bias1, bias2, z_mean, ls, sep, den=True, rsd=True, d1=False,
+                    flatsky=True, small_r=True, epsrel=1e-5):
+    """CPU baseline: reference sequence per spectrum (8 tables + QAG multipoles), OpenMP over spectra.
+    specs: list of (k, P); returns (nspec, nz, nl, nsep)."""
+    off = np.concatenate([[0], np.cumsum([len(s[0]) for s in specs])]).astype(np.longlong)
+    k = _f64(np.concatenate([s[0] for s in specs]))
+    pk = _f64(np.concatenate([s[1] for s in specs]))
+    nf = np.ascontiguousarray(nfft, dtype=np.longlong)
+    arrs = {kk: _f64(bg[kk]) for kk in ("z", "chi", "D1", "f", "cH", "G1", "G2")}
+    bz, b1, b2 = _f64(bias_z), _f64(bias1), _f64(bias2)
+    zm, sp = _f64(np.atleast_1d(z_mean)), _f64(np.atleast_1d(sep))
+    lsa = (ctypes.c_int * len(ls))(*ls)
+    out = np.empty(len(specs) * len(zm) * len(ls) * len(sp))
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    _check(lib().orc_reference_batch(len(specs), off.ctypes.data_as(llp), _p(k), _p(pk), nf.ctypes.data_as(llp),
+                                     len(arrs["z"]), _p(arrs["z"]), _p(arrs["chi"]), _p(arrs["D1"]), _p(arrs["f"]),
+                                     _p(arrs["cH"]), _p(arrs["G1"]), _p(arrs["G2"]), len(bz), _p(bz), _p(b1), _p(b2),
+                                     int(den), int(rsd), int(d1), int(flatsky), len(zm), _p(zm), len(ls), lsa,
+                                     len(sp), _p(sp), int(small_r), float(epsrel), _p(out)), "reference_batch")
+    return out.reshape(len(specs), len(zm), len(ls), len(sp))
+
+
+def tables_at_sep_batch(specs, nfft, tables, sep):
+    """Akima of each oracle table at sep, for a list of (k, P); returns (nspec, ntab, nsep)."""
+    off = np.concatenate([[0], np.cumsum([len(s[0]) for s in specs])]).astype(np.longlong)
+    k = _f64(np.concatenate([s[0] for s in specs]))
+    pk = _f64(np.concatenate([s[1] for s in specs]))
+    nf = np.ascontiguousarray(nfft, dtype=np.longlong)
+    tn = (ctypes.c_int * len(tables))(*[t[0] for t in tables])
+    tl = (ctypes.c_int * len(tables))(*[t[1] for t in tables])
+    sp = _f64(sep)
+    out = np.empty(len(specs) * len(tables) * len(sp))
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    _check(lib().orc_tables_at_sep_batch(len(specs), off.ctypes.data_as(llp), _p(k), _p(pk), nf.ctypes.data_as(llp),
+                                         len(tables), tn, tl, len(sp), _p(sp), _p(out)), "tables_at_sep_batch")
+    return out.reshape(len(specs), len(tables), len(sp))

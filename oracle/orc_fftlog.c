@@ -252,3 +252,32 @@ done:
     free(fx); free(fy); free(akb);
     return st;
 }
+
+/* tables at separations for a batch: out[s][t][r] = Akima(table_t of spectrum s)(sep_r),
+   tables without the small-r nodes (they do not enter Akima stencils at sep >= r_2);
+   OpenMP over (spectrum, table) */
+int orc_tables_at_sep_batch(int nspec, const long long *off, const double *k, const double *p, const long long *nfft,
+                            int ntab, const int *tab_n, const int *tab_l, size_t nsep, const double *sep, double *out)
+{
+    int status = ORC_OK;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int u = 0; u < nspec * ntab; ++u) {
+        const int s = u / ntab, t = u % ntab;
+        const size_t nin = (size_t)(off[s + 1] - off[s]);
+        const size_t N = (size_t)nfft[s];
+        double *tx = (double *)malloc(sizeof(double) * (N + 43));
+        double *ty = (double *)malloc(sizeof(double) * (N + 43));
+        int st = (tx && ty) ? ORC_OK : ORC_ENOMEM;
+        if (st == ORC_OK)
+            st = orc_integral_table(tab_n[t], tab_l[t], N, nin, k + off[s], p + off[s], k[off[s]], k[off[s + 1] - 1],
+                                    0, tx, ty);
+        if (st == ORC_OK) st = orc_akima_interp(N + 43, tx, ty, nsep, sep, out + (size_t)u * nsep);
+        free(tx);
+        free(ty);
+        if (st != ORC_OK) {
+            #pragma omp atomic write
+            status = st;
+        }
+    }
+    return status;
+}

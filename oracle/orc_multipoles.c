@@ -218,3 +218,65 @@ int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const
     free(c.b1.b); free(c.b2.b);
     return st;
 }
+
+/*
+ * CPU baseline: the reference's own sequence for each spectrum -- the 8 default
+ * tables of coffe_integrals_init (integrals.c:599-660, with or without the 43
+ * small-r QAG nodes) followed by the QAG mu-integral of every (z, l, r) point
+ * (multipoles.c:88-99, signal.c:521-542) -- with OpenMP over spectra.
+ * out: [spec][z][l][r].
+ */
+int orc_reference_batch(int nspec, const long long *off, const double *k, const double *p, const long long *nfft,
+                        size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                        const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                        const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                        const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int small_r,
+                        double epsrel, double *out)
+{
+    int status = ORC_OK;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 0; s < nspec; ++s) {
+        const size_t nin = (size_t)(off[s + 1] - off[s]);
+        const size_t N = (size_t)nfft[s];
+        const double *ks = k + off[s], *ps = p + off[s];
+        double *tx[NTAB], *ty[NTAB];
+        size_t tn[NTAB];
+        int st = ORC_OK;
+        static const int tabs[NTAB][2] = {{0, 0}, {0, 2}, {0, 4}, {1, 1}, {1, 3}, {2, 0}, {2, 2}, {3, 1}};
+        for (int t = 0; t < NTAB; ++t) {
+            tx[t] = (double *)malloc(sizeof(double) * (N + 43));
+            ty[t] = (double *)malloc(sizeof(double) * (N + 43));
+            tn[t] = N + 43;
+            if (st == ORC_OK)
+                st = orc_integral_table(tabs[t][0], tabs[t][1], N, nin, ks, ps, ks[0], ks[nin - 1], small_r, tx[t],
+                                        ty[t]);
+        }
+        mp_ctx c;
+        memset(&c, 0, sizeof(c));
+        c.den = den; c.rsd = rsd; c.d1 = d1; c.flatsky = flatsky;
+        for (int t = 0; t < NTAB && st == ORC_OK; ++t) { c.have[t] = 1; st = ak_init(&c.tab[t], tn[t], tx[t], ty[t]); }
+        if (st == ORC_OK) st = ak_init(&c.chi, nbg, bg_z, bg_chi);
+        if (st == ORC_OK) st = ak_init(&c.z_of_chi, nbg, bg_chi, bg_z);
+        if (st == ORC_OK) st = ak_init(&c.D1, nbg, bg_z, bg_D1);
+        if (st == ORC_OK) st = ak_init(&c.f, nbg, bg_z, bg_f);
+        if (st == ORC_OK) st = ak_init(&c.cH, nbg, bg_z, bg_cH);
+        if (st == ORC_OK) st = ak_init(&c.G1, nbg, bg_z, bg_G1);
+        if (st == ORC_OK) st = ak_init(&c.G2, nbg, bg_z, bg_G2);
+        if (st == ORC_OK) st = ak_init(&c.b1, nb, zb, bias1);
+        if (st == ORC_OK) st = ak_init(&c.b2, nb, zb, bias2);
+        for (size_t idx = 0; idx < nz * nl * nsep && st == ORC_OK; ++idx) {
+            const size_t iz = idx / (nl * nsep), il = (idx / nsep) % nl, ir = idx % nsep;
+            mp_point pt = {&c, z_mean[iz], sep[ir], ls[il]};
+            const double v = orc_qag(mp_integrand, &pt, 0, 1, 0, epsrel, 50000, NULL, NULL);
+            out[(size_t)s * nz * nl * nsep + idx] = (2 * ls[il] + 1) * v;
+        }
+        for (int t = 0; t < NTAB; ++t) { if (c.have[t]) free(c.tab[t].b); free(tx[t]); free(ty[t]); }
+        free(c.chi.b); free(c.z_of_chi.b); free(c.D1.b); free(c.f.b); free(c.cH.b); free(c.G1.b); free(c.G2.b);
+        free(c.b1.b); free(c.b2.b);
+        if (st != ORC_OK) {
+            #pragma omp atomic write
+            status = st;
+        }
+    }
+    return status;
+}

@@ -1,0 +1,67 @@
+"""GPU adjoint (VJP) of the batched multipoles against central finite differences of
+the oracle: <gout, dM/dP v> from the engine's gpk must equal
+<gout, (M(P + eps v) - M(P - eps v)) / (2 eps)> computed on the CPU."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.synth import batch_spectra, flat_weights
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_M(specs, tables, sep, A):
+    tab = O.tables_at_sep_batch(specs, [len(s[0]) for s in specs], tables, sep)
+    return np.einsum("zltr,str->szlr", A, tab)
+
+
+@pytest.mark.parametrize("tables", [[(0, 0), (0, 2), (0, 4)], [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0),
+                                                               (2, 2), (3, 1)]])
+def test_vjp_matches_oracle_fd(engine, tables):
+    import torch
+    from coffe_amd.engine import Batch
+    specs = [(s[0], s[1]) for s in batch_spectra(4, seed=7, lengths=(1024, 2048, 4096))]
+    sep = np.linspace(5, 300, 48) / 0.67
+    ntab = len(tables)
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((2, 3, ntab, len(sep)))
+    b = Batch(engine, [len(s[0]) for s in specs], [len(s[0]) for s in specs], [s[0][0] for s in specs],
+              [s[0][-1] for s in specs], tables, sep, 2, 3)
+    k = torch.tensor(np.concatenate([s[0] for s in specs]), device="cuda")
+    p = torch.tensor(np.concatenate([s[1] for s in specs]), device="cuda")
+    gout_np = rng.standard_normal((len(specs), 2, 3, len(sep)))
+    print("vjp ntab", ntab, flush=True)
+    itab, out, gpk = b.vjp(k, p, torch.tensor(A, device="cuda"), torch.tensor(gout_np, device="cuda"))
+    gpk = gpk.cpu().numpy()
+    out = out.cpu().numpy()
+    M0 = _oracle_M(specs, tables, sep, A)
+    assert np.max(np.abs(out - M0)) <= 1e-10 * np.max(np.abs(M0))
+    for trial in range(3):
+        xi = [rng.standard_normal(len(s[1])) for s in specs]
+        v = np.concatenate([s[1] * x for s, x in zip(specs, xi)])
+        eps = 1e-6
+        plus = [(s[0], s[1] * (1 + eps * x)) for s, x in zip(specs, xi)]
+        minus = [(s[0], s[1] * (1 - eps * x)) for s, x in zip(specs, xi)]
+        fd = np.sum(gout_np * (_oracle_M(plus, tables, sep, A) - _oracle_M(minus, tables, sep, A))) / (2 * eps)
+        an = np.dot(gpk, v)
+        assert abs(an - fd) <= 1e-6 * abs(fd), (trial, an, fd)
+
+
+def test_vjp_homogeneity(engine):
+    """The map P -> M is positively homogeneous of degree 1 (spline, FFTs and Akima
+    all are), so <gpk, P> = <gout, M> exactly up to rounding."""
+    import torch
+    from coffe_amd.engine import Batch
+    specs = [(s[0], s[1]) for s in batch_spectra(3, seed=9)]
+    sep = np.linspace(10, 200, 64) / 0.67
+    tables = [(0, 0), (0, 2), (0, 4)]
+    A = flat_weights([0, 2, 4], 1.5, 0.87, 0.6, len(sep))
+    b = Batch(engine, [len(s[0]) for s in specs], [len(s[0]) for s in specs], [s[0][0] for s in specs],
+              [s[0][-1] for s in specs], tables, sep, 1, 3)
+    k = torch.tensor(np.concatenate([s[0] for s in specs]), device="cuda")
+    p = torch.tensor(np.concatenate([s[1] for s in specs]), device="cuda")
+    gout = torch.tensor(np.random.default_rng(2).standard_normal((3, 1, 3, len(sep))), device="cuda")
+    _, out, gpk = b.vjp(k, p, torch.tensor(A, device="cuda"), gout)
+    lhs = float(torch.dot(gpk, p))
+    rhs = float(torch.sum(gout * out))
+    assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
