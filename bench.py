@@ -230,7 +230,7 @@ def main():
         res["vs_plain"] = value / pp["value"]
     if not args.no_cpu:
         cores = os.cpu_count() or 1
-        sample = args.cpu_sample or max(4, min(nspec, 2 * min(cores, 16)))
+        sample = args.cpu_sample or nspec
         res["cpu_baseline"] = cpu_baseline(specs, sep, bg, phys, sample)
     print(json.dumps(res))
 
