@@ -15,6 +15,10 @@ extern "C" {
 #define CE_MAX_PASSES 16
 #define CE_MAX_TABLES 8
 #define CE_MIN_SEP_LEN 42
+/* input intervals per spline chunk and LDS nodes staged per chunk (chunk + 2 halos
+   of 64 + end nodes) */
+#define CE_CHUNK 1024
+#define CE_CHUNK_NODES 1160
 
 /* FFT radix schedule for a complex transform of length M = N/2 */
 typedef struct ce_radix_plan {
@@ -43,8 +47,14 @@ typedef struct ce_spec_dev {
     int32_t grid;      /* index into the grid array */
     int64_t xf_off;    /* offset (complex units) of the r2c workspace: nq * (M+1) */
     int64_t xb_off;    /* offset (complex units) of the adjoint workspace: ntab * (M+1) */
-    int64_t ws_off;    /* offset (doubles) of the real workspace: 5 * n_in + N */
+    int64_t ws_off;    /* offset (doubles) of the real workspace: 2 N (S(x_i), then its cotangent) */
 } ce_spec_dev;
+
+/* one chunk of input intervals [a, b) of a spectrum (b = n_in - 1 on the last) */
+typedef struct ce_chunk_dev {
+    int32_t spec;
+    int32_t a, b;
+} ce_chunk_dev;
 
 /* work unit of the table kernels */
 typedef struct ce_unit_dev {
@@ -63,8 +73,11 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
                           int ntab, const ce_table_desc *tabs, const double *tab_qnu, int nq, const double *qprime,
                           double *const *d_U, double *const *d_os, double *const *d_bw);
 
+int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
+                       const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws);
+
 int ce_launch_prepare(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids, int nq,
-                      const double *d_k, const double *d_pk, double *d_ws, double *d_xf, int threads, int max_M);
+                      const double *d_ws, double *d_xf, int threads, int max_M);
 
 int ce_launch_tables(void *stream, int nunit, const ce_unit_dev *d_units, const ce_spec_dev *d_spec,
                      const ce_grid_dev *d_grids, const ce_table_desc *d_tabs, int ntab, const double *d_xf,
@@ -75,9 +88,12 @@ int ce_launch_tables(void *stream, int nunit, const ce_unit_dev *d_units, const 
 int ce_launch_project(void *stream, int nspec, int ntab, int nz, int nl, int nsep, const double *d_A,
                       const double *d_itab, double *d_out);
 
-int ce_launch_prepare_vjp(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
-                          const ce_table_desc *d_tabs, int ntab, int nq, const double *d_k, const double *d_pk,
-                          double *d_ws, const double *d_xbar, double *d_gpk, int threads, int max_M);
+int ce_launch_adj_c2r(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
+                      const ce_table_desc *d_tabs, int ntab, int nq, double *d_ws, const double *d_xbar, int threads,
+                      int max_M);
+
+int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
+                         const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk);
 
 #ifdef __cplusplus
 }

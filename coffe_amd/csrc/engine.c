@@ -298,7 +298,7 @@ coffe_engine *coffe_engine_default(void)
 
 struct coffe_batch {
     coffe_engine *e;
-    int nspec, ntab, nsep, nz, nl, nq, nunit, ngrid;
+    int nspec, ntab, nsep, nz, nl, nq, nunit, ngrid, nchunk;
     int max_M, max_N, threads;
     int64_t total_in;
     ce_grid **grids;
@@ -308,6 +308,7 @@ struct coffe_batch {
     ce_grid_dev *d_grids;
     ce_table_desc *d_tabs;
     ce_unit_dev *d_units;
+    ce_chunk_dev *d_chunks;
     int32_t *d_sidx;
     double *d_sep;
     double *d_ws, *d_xf, *d_xb, *d_wbuf, *d_itab;
@@ -324,6 +325,7 @@ void coffe_batch_destroy(coffe_batch *b)
     hipFree(b->d_grids);
     hipFree(b->d_tabs);
     hipFree(b->d_units);
+    hipFree(b->d_chunks);
     hipFree(b->d_sidx);
     hipFree(b->d_sep);
     hipFree(b->d_ws);
@@ -411,7 +413,8 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
         in_off += n_in[s];
         xf_off += (int64_t)g->nq * (M + 1);
         xb_off += (int64_t)ntab * (M + 1);
-        ws_off += 5 * n_in[s] + g->N;
+        ws_off += 2 * (int64_t)g->N;
+        b->nchunk += (int)((n_in[s] - 1 + CE_CHUNK - 1) / CE_CHUNK);
     }
     b->total_in = in_off;
     /* every FFT pass needs at least M/8 threads (at most 4 butterflies each) */
@@ -438,6 +441,21 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
         }
         free(us);
     }
+    ce_chunk_dev *chunks = NULL;
+    if (st == COFFE_OK) {
+        chunks = (ce_chunk_dev *)malloc(sizeof(ce_chunk_dev) * (size_t)b->nchunk);
+        if (!chunks) st = COFFE_ENOMEM;
+        int c = 0;
+        for (int s = 0; s < nspec && st == COFFE_OK; ++s) {
+            const int nint = (int)n_in[s] - 1;
+            for (int a = 0; a < nint; a += CE_CHUNK) {
+                chunks[c].spec = s;
+                chunks[c].a = a;
+                chunks[c].b = a + CE_CHUNK < nint ? a + CE_CHUNK : nint;
+                ++c;
+            }
+        }
+    }
     ce_grid_dev *hg = NULL;
     if (st == COFFE_OK) {
         hg = (ce_grid_dev *)calloc((size_t)b->ngrid, sizeof(ce_grid_dev));
@@ -452,6 +470,7 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_grids, sizeof(ce_grid_dev) * (size_t)b->ngrid);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_tabs, sizeof(ce_table_desc) * (size_t)ntab);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_units, sizeof(ce_unit_dev) * (size_t)b->nunit);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_chunks, sizeof(ce_chunk_dev) * (size_t)b->nchunk);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_sidx, sizeof(int32_t) * (size_t)b->ngrid * (size_t)nsep);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_sep, sizeof(double) * (size_t)nsep);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_ws, sizeof(double) * (size_t)(ws_off > 0 ? ws_off : 1));
@@ -464,12 +483,14 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_grids, hg, sizeof(ce_grid_dev) * (size_t)b->ngrid, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_tabs, tabs, sizeof(ce_table_desc) * (size_t)ntab, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_units, units, sizeof(ce_unit_dev) * (size_t)b->nunit, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_chunks, chunks, sizeof(ce_chunk_dev) * (size_t)b->nchunk, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_sidx, sidx, sizeof(int32_t) * (size_t)b->ngrid * (size_t)nsep, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_sep, sep, sizeof(double) * (size_t)nsep, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
     free(spec_grid);
     free(sidx);
     free(units);
+    free(chunks);
     free(hg);
     if (st != COFFE_OK) { coffe_batch_destroy(b); return st; }
     *out = b;
@@ -481,8 +502,9 @@ static hipStream_t pick_stream(coffe_batch *b, void *stream) { return stream ? (
 int coffe_batch_prepare(coffe_batch *b, const double *d_k, const double *d_pk, void *stream)
 {
     if (!b || !d_k || !d_pk) return COFFE_EINVAL;
-    return ce_launch_prepare(pick_stream(b, stream), b->nspec, b->d_spec, b->d_grids, b->nq, d_k, d_pk, b->d_ws,
-                             b->d_xf, b->threads, b->max_M)
+    hipStream_t s = pick_stream(b, stream);
+    if (ce_launch_resample(s, b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, d_pk, b->d_ws)) return COFFE_EDEVICE;
+    return ce_launch_prepare(s, b->nspec, b->d_spec, b->d_grids, b->nq, b->d_ws, b->d_xf, b->threads, b->max_M)
                ? COFFE_EDEVICE
                : COFFE_OK;
 }
@@ -510,8 +532,11 @@ int coffe_batch_project(coffe_batch *b, const double *d_A, const double *d_itab,
 int coffe_batch_prepare_adj(coffe_batch *b, const double *d_k, const double *d_pk, double *d_gpk, void *stream)
 {
     if (!b || !d_k || !d_pk || !d_gpk) return COFFE_EINVAL;
-    return ce_launch_prepare_vjp(pick_stream(b, stream), b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq,
-                                 d_k, d_pk, b->d_ws, b->d_xb, d_gpk, b->threads, b->max_M)
+    hipStream_t s = pick_stream(b, stream);
+    if (ce_launch_adj_c2r(s, b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq, b->d_ws, b->d_xb,
+                          b->threads, b->max_M))
+        return COFFE_EDEVICE;
+    return ce_launch_spline_adj(s, b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, b->d_ws, d_gpk)
                ? COFFE_EDEVICE
                : COFFE_OK;
 }
@@ -522,8 +547,8 @@ int coffe_batch_forward(coffe_batch *b, const double *d_k, const double *d_pk, c
     if (!b || !d_k || !d_pk || (d_out && !d_A)) return COFFE_EINVAL;
     hipStream_t s = pick_stream(b, stream);
     double *itab = d_itab ? d_itab : b->d_itab;
-    if (ce_launch_prepare(s, b->nspec, b->d_spec, b->d_grids, b->nq, d_k, d_pk, b->d_ws, b->d_xf, b->threads, b->max_M))
-        return COFFE_EDEVICE;
+    int st = coffe_batch_prepare(b, d_k, d_pk, s);
+    if (st) return st;
     if (ce_launch_tables(s, b->nunit, b->d_units, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->d_xf, b->nsep,
                          b->d_sidx, b->d_sep, itab, NULL, 0, NULL, NULL, b->nz, b->nl, NULL, b->d_wbuf, b->threads,
                          b->max_M))
@@ -538,25 +563,22 @@ int coffe_batch_vjp(coffe_batch *b, const double *d_k, const double *d_pk, const
     if (!b || !d_k || !d_pk || !d_A || !d_gout || !d_gpk) return COFFE_EINVAL;
     hipStream_t s = pick_stream(b, stream);
     double *itab = d_itab ? d_itab : b->d_itab;
-    if (ce_launch_prepare(s, b->nspec, b->d_spec, b->d_grids, b->nq, d_k, d_pk, b->d_ws, b->d_xf, b->threads, b->max_M))
-        return COFFE_EDEVICE;
+    int st = coffe_batch_prepare(b, d_k, d_pk, s);
+    if (st) return st;
     if (ce_launch_tables(s, b->nunit, b->d_units, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->d_xf, b->nsep,
                          b->d_sidx, b->d_sep, itab, NULL, 0, d_A, d_gout, b->nz, b->nl, b->d_xb, b->d_wbuf,
                          b->threads, b->max_M))
         return COFFE_EDEVICE;
     if (d_out && ce_launch_project(s, b->nspec, b->ntab, b->nz, b->nl, b->nsep, d_A, itab, d_out)) return COFFE_EDEVICE;
-    if (ce_launch_prepare_vjp(s, b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq, d_k, d_pk, b->d_ws,
-                              b->d_xb, d_gpk, b->threads, b->max_M))
-        return COFFE_EDEVICE;
-    return COFFE_OK;
+    return coffe_batch_prepare_adj(b, d_k, d_pk, d_gpk, s);
 }
 
 int coffe_batch_tables(coffe_batch *b, const double *d_k, const double *d_pk, double *d_full, void *stream)
 {
     if (!b || !d_k || !d_pk || !d_full) return COFFE_EINVAL;
     hipStream_t s = pick_stream(b, stream);
-    if (ce_launch_prepare(s, b->nspec, b->d_spec, b->d_grids, b->nq, d_k, d_pk, b->d_ws, b->d_xf, b->threads, b->max_M))
-        return COFFE_EDEVICE;
+    int st0 = coffe_batch_prepare(b, d_k, d_pk, s);
+    if (st0) return st0;
     /* the unit order is by size; the full-table rows follow the unit order, so
        launch one unit per (spec, tab) in natural order via a temporary list */
     ce_unit_dev *h = (ce_unit_dev *)malloc(sizeof(ce_unit_dev) * (size_t)b->nunit);
@@ -612,11 +634,12 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     if (st != COFFE_OK) return st;
     const int N = (int)output_len, M = N / 2;
     const size_t n = input_len;
-    /* scratch layout: k, P | ws (5n + N) | xf (M+1 cplx) | out (N) | spec, grid, tab, unit, sidx, wbuf */
-    const size_t off_k = 0, off_p = n, off_ws = 2 * n, off_xf = off_ws + 5 * n + (size_t)N;
+    /* scratch layout: k, P | ws (2N) | xf (M+1 cplx) | out (N) | spec, grid, tab, unit, chunks */
+    const int nchunk = (int)((n - 1 + CE_CHUNK - 1) / CE_CHUNK);
+    const size_t off_k = 0, off_p = n, off_ws = 2 * n, off_xf = off_ws + 2 * (size_t)N;
     const size_t off_out = off_xf + 2 * (size_t)(M + 1), off_meta = off_out + (size_t)N;
     const size_t meta_bytes = sizeof(ce_spec_dev) + sizeof(ce_grid_dev) + sizeof(ce_table_desc) +
-                              sizeof(ce_unit_dev) + 64;
+                              sizeof(ce_unit_dev) + sizeof(ce_chunk_dev) * (size_t)nchunk + 64;
     pthread_mutex_lock(&e->lock);
     st = scratch_reserve(e, sizeof(double) * off_meta + meta_bytes + 64);
     if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); return st; }
@@ -626,6 +649,14 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     ce_grid_dev *d_grid = (ce_grid_dev *)(meta + sizeof(ce_spec_dev));
     ce_table_desc *d_tab = (ce_table_desc *)((char *)d_grid + sizeof(ce_grid_dev));
     ce_unit_dev *d_unit = (ce_unit_dev *)((char *)d_tab + sizeof(ce_table_desc));
+    ce_chunk_dev *d_chunks = (ce_chunk_dev *)((char *)d_unit + sizeof(ce_unit_dev));
+    ce_chunk_dev *h_chunks = (ce_chunk_dev *)malloc(sizeof(ce_chunk_dev) * (size_t)nchunk);
+    if (!h_chunks) { pthread_mutex_unlock(&e->lock); return COFFE_ENOMEM; }
+    for (int c = 0; c < nchunk; ++c) {
+        h_chunks[c].spec = 0;
+        h_chunks[c].a = c * CE_CHUNK;
+        h_chunks[c].b = (c + 1) * CE_CHUNK < (int)n - 1 ? (c + 1) * CE_CHUNK : (int)n - 1;
+    }
     ce_spec_dev hs;
     memset(&hs, 0, sizeof(hs));
     hs.in_off = 0;
@@ -646,8 +677,12 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_grid, &hg, sizeof(hg), hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_tab, &g->tabs[0], sizeof(ce_table_desc), hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_unit, &hu, sizeof(hu), hipMemcpyHostToDevice, s));
-    if (st == COFFE_OK && ce_launch_prepare(s, 1, d_spec, d_grid, 1, base + off_k, base + off_p, base + off_ws,
-                                            base + off_xf, threads, M))
+    if (st == COFFE_OK)
+        st = hip_ok(hipMemcpyAsync(d_chunks, h_chunks, sizeof(ce_chunk_dev) * (size_t)nchunk, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK && ce_launch_resample(s, nchunk, d_chunks, d_spec, d_grid, base + off_k, base + off_p,
+                                             base + off_ws))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK && ce_launch_prepare(s, 1, d_spec, d_grid, 1, base + off_ws, base + off_xf, threads, M))
         st = COFFE_EDEVICE;
     if (st == COFFE_OK && ce_launch_tables(s, 1, d_unit, d_spec, d_grid, d_tab, 1, base + off_xf, 0, NULL, NULL,
                                            NULL, base + off_out, N, NULL, NULL, 1, 1, NULL, NULL, threads, M))
@@ -655,6 +690,7 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(output_y, base + off_out, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, s));
     if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
     pthread_mutex_unlock(&e->lock);
+    free(h_chunks);
     if (st != COFFE_OK) return st;
     for (int i = 0; i < N; ++i) output_x[i] = r0 * g->h_p[i];
     return COFFE_OK;

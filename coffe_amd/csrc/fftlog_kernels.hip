@@ -14,14 +14,16 @@
 //                  Fourier window and 1/L folded in (twofast.c:196-224, 345-353,
 //                  426-446), output prefactor times r^(n-l) (twofast.c:415-421,
 //                  integrals.c:514-522).
-//   k_prepare      per spectrum: natural cubic spline of P(k) (twofast.c:399-401),
-//                  resample onto the log grid, bias, r2c, conj (twofast.c:252-360).
+//   k_resample     per chunk of input intervals: natural cubic spline of P(k)
+//                  (twofast.c:399-401) and its values on the log grid (275-290).
+//   k_prepare      per spectrum: bias * window * S, r2c, conj (twofast.c:252-360).
 //   k_tables       per (spectrum, table): kernel multiply, c2r (twofast.c:426-477),
 //                  then either the full table or Akima at the separations
 //                  (common.c:704-710 on the table of integrals.c:564-567); the
 //                  adjoint mode adds the Akima VJP and the transposed c2r.
 //   k_project      multipoles = sum_t A[z,l,t,r] I_t(r) (the mu-integrated weights).
-//   k_prepare_vjp  per spectrum: transposed r2c, resample and spline -> dP.
+//   k_adj_c2r      per spectrum: transposed r2c and bias -> cotangent of S(x_i).
+//   k_spline_adj   per chunk: transposed resample and spline -> dP.
 #include <hip/hip_runtime.h>
 
 #include "ce_internal.h"
@@ -310,123 +312,203 @@ __global__ void k_grid_tables(grid_tab_args a)
     }
 }
 
-// ---- natural cubic spline coefficients, chunked LDL^T with overlap -----------
-// Unknowns c_1..c_{n-2} (index s = i - 1).  Each thread owns a chunk and starts
-// its forward sweep W rows early and its backward sweep W rows late; the error of
-// the truncated start decays like prod gamma_s <= 2^-W (gamma_s < 1/2 since the
-// matrix is strictly diagonally dominant), so W = 64 reproduces the sequential
-// solve to rounding.  gam/zz are global scratch (n doubles each).
-__device__ void spline_solve(const double *__restrict__ x, const double *__restrict__ rhs_y, int n,
-                             bool rhs_is_values, const double *__restrict__ rhs_direct, double *__restrict__ c,
-                             double *__restrict__ gam, double *__restrict__ zz)
+// ---- natural cubic spline and resampling, chunked over the input nodes ---------
+// A chunk owns the input intervals [a, b) of one spectrum.  It stages the nodes
+// [a - W - 1, b + W + 1] in LDS, solves the natural-spline system (unknowns
+// c_i = y''_i / 2, i = 1..n-2, twofast.c:399-401) restricted to the nodes
+// [max(1, a - W), min(n - 2, b + W)] by parallel cyclic reduction, and keeps the
+// coefficients of [a, b].  Cutting the system W rows away perturbs the kept rows
+// by at most prod gamma <= 2^-W relative (the matrix is strictly diagonally
+// dominant, every LDL^T multiplier gamma is below 1/2), 5e-20 for W = 64; the
+// global ends carry the exact natural boundary c_0 = c_{n-1} = 0.
+
+#define CE_SPLINE_HALO 64
+
+// parallel cyclic reduction of a_i x_{i-1} + b_i x_i + c_i x_{i+1} = d_i,
+// i < ns, in place in LDS; on exit d_i holds x_i.  All threads must call it.
+__device__ void pcr_solve(double *__restrict__ A, double *__restrict__ B, double *__restrict__ C,
+                          double *__restrict__ D, int ns)
 {
-    const int sys = n - 2;
     const int T = blockDim.x, tid = threadIdx.x;
-    const int CH = (sys + T - 1) / T;
-    const int W = 64;
-    const int a = tid * CH, b = min(a + CH, sys);
-    auto h = [&](int i) { return x[i + 1] - x[i]; };
-    auto diag = [&](int s) { return 2.0 * (h(s + 1) + h(s)); };
-    auto off = [&](int s) { return h(s + 1); };
-    auto rhs = [&](int s) {
-        if (!rhs_is_values) return rhs_direct[s];
-        const double hi = h(s), hi1 = h(s + 1);
-        const double gi = hi != 0.0 ? 1.0 / hi : 0.0, gi1 = hi1 != 0.0 ? 1.0 / hi1 : 0.0;
-        return 3.0 * ((rhs_y[s + 2] - rhs_y[s + 1]) * gi1 - (rhs_y[s + 1] - rhs_y[s]) * gi);
-    };
-    if (a < b) {
-        const int s0 = max(0, a - W);
-        double alpha = diag(s0);
-        double z = rhs(s0);
-        if (s0 >= a) {
-            gam[s0] = (s0 < sys - 1) ? off(s0) / alpha : 0.0;
-            zz[s0] = z / alpha;
-        }
-        for (int s = s0 + 1; s < b; ++s) {
-            const double gprev = off(s - 1) / alpha;
-            alpha = diag(s) - off(s - 1) * gprev;
-            z = rhs(s) - gprev * z;
-            if (s >= a) {
-                gam[s] = (s < sys - 1) ? off(s) / alpha : 0.0;
-                zz[s] = z / alpha;
+    const int per = (ns + T - 1) / T;   // <= CE_PCR_PER
+    for (int stride = 1; stride < ns; stride <<= 1) {
+        double na[8], nb[8], nc[8], nd[8];
+        for (int q = 0; q < per; ++q) {
+            const int i = tid + q * T;
+            if (i >= ns) break;
+            const int il = i - stride, ir = i + stride;
+            double a = A[i], b = B[i], c = C[i], d = D[i];
+            double ra = 0.0, rc = 0.0;
+            if (il >= 0) {
+                const double al = -a / B[il];
+                ra = al * A[il];
+                b += al * C[il];
+                d += al * D[il];
             }
+            if (ir < ns) {
+                const double ga = -c / B[ir];
+                rc = ga * C[ir];
+                b += ga * A[ir];
+                d += ga * D[ir];
+            }
+            na[q] = ra; nb[q] = b; nc[q] = rc; nd[q] = d;
         }
-    }
-    __syncthreads();
-    if (a < b) {
-        const int e = min(sys - 1, b - 1 + W);
-        double xv = zz[e];
-        if (e < b) c[e + 1] = xv;
-        for (int s = e - 1; s >= a; --s) {
-            xv = zz[s] - gam[s] * xv;
-            if (s < b) c[s + 1] = xv;
+        __syncthreads();
+        for (int q = 0; q < per; ++q) {
+            const int i = tid + q * T;
+            if (i >= ns) break;
+            A[i] = na[q]; B[i] = nb[q]; C[i] = nc[q]; D[i] = nd[q];
         }
+        __syncthreads();
     }
-    if (tid == 0) {
-        c[0] = 0.0;
-        c[n - 1] = 0.0;
-    }
+    for (int i = tid; i < ns; i += T) D[i] = D[i] / B[i];
     __syncthreads();
 }
 
-__device__ __forceinline__ int first_interval(const double *__restrict__ k, int n, double x)
+// first resample index i with k0 p_i >= v (p increasing); N if none
+__device__ __forceinline__ int lower_index(const double *p, int N, double k0, double v)
 {
-    int lo = 0, hi = n - 1;
+    int lo = 0, hi = N;
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (k0 * p[m] < v) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+// interval j of the LDS-staged nodes xs[0..cnt) (global index = j + base) with
+// xs[j] <= v < xs[j+1]; bsearch semantics over the staged range
+__device__ __forceinline__ int lds_interval(const double *xs, int cnt, double v)
+{
+    int lo = 0, hi = cnt - 1;
     while (hi > lo + 1) {
-        const int m = (hi + lo) >> 1;
-        if (k[m] > x) hi = m;
+        const int m = (lo + hi) >> 1;
+        if (xs[m] > v) hi = m;
         else lo = m;
     }
     return lo;
 }
 
-// resample S(x_i) * bw[i] into LDS (as N reals); contiguous chunk per thread
-__device__ void resample(const ce_grid_dev &g, const double *__restrict__ k, const double *__restrict__ P,
-                         const double *__restrict__ c, int n, const double *__restrict__ bw, double *__restrict__ out)
+struct chunk_geom {
+    int a, b;        // owned intervals [a, b) (b = n - 1 on the last chunk)
+    int lo, hi;      // staged nodes [lo, hi]
+    int u0, u1;      // unknown nodes of the truncated system
+};
+
+__device__ __forceinline__ chunk_geom make_geom(const ce_chunk_dev &ch, int n)
 {
-    const int N = g.N;
+    chunk_geom g;
+    g.a = ch.a;
+    g.b = ch.b;
+    g.u0 = max(1, g.a - CE_SPLINE_HALO);
+    g.u1 = min(n - 2, g.b + CE_SPLINE_HALO);
+    g.lo = g.u0 - 1;
+    g.hi = g.u1 + 1;
+    return g;
+}
+
+// stage x (and y) of nodes [lo, hi] and fill the truncated spline system; the LDS
+// layout is xs | ys | A | B | C | D, each CE_CHUNK_NODES long
+__device__ void stage_and_build(const chunk_geom &G, const double *__restrict__ k, const double *__restrict__ P,
+                                double *xs, double *ys, double *A, double *B, double *C, double *D, bool values)
+{
     const int T = blockDim.x, tid = threadIdx.x;
-    const int CH = (N + T - 1) / T;
-    const int i0 = tid * CH, i1 = min(i0 + CH, N);
-    if (i0 >= i1) return;
-    const double k0 = g.k0;
+    const int cnt = G.hi - G.lo + 1;
+    for (int i = tid; i < cnt; i += T) {
+        xs[i] = k[G.lo + i];
+        if (values) ys[i] = P[G.lo + i];
+    }
+    __syncthreads();
+    const int ns = G.u1 - G.u0 + 1;
+    for (int r = tid; r < ns; r += T) {
+        const int li = r + 1;   // local index of node u0 + r
+        const double hm = xs[li] - xs[li - 1], hp = xs[li + 1] - xs[li];
+        A[r] = (r > 0) ? hm : 0.0;
+        C[r] = (r < ns - 1) ? hp : 0.0;
+        B[r] = 2.0 * (hm + hp);
+        if (values) {
+            const double gm = hm != 0.0 ? 1.0 / hm : 0.0, gp = hp != 0.0 ? 1.0 / hp : 0.0;
+            D[r] = 3.0 * ((ys[li + 1] - ys[li]) * gp - (ys[li] - ys[li - 1]) * gm);
+        }
+    }
+    __syncthreads();
+}
+
+// k_resample: S(x_i) for the resample points whose interval lies in [a, b)
+__global__ void __launch_bounds__(256) k_resample(const ce_chunk_dev *__restrict__ chunks,
+                                                  const ce_spec_dev *__restrict__ specs,
+                                                  const ce_grid_dev *__restrict__ grids,
+                                                  const double *__restrict__ k_all,
+                                                  const double *__restrict__ pk_all, double *__restrict__ ws_all)
+{
+    __shared__ double lds[6 * CE_CHUNK_NODES];
+    __shared__ int range[2];
+    const ce_chunk_dev ch = chunks[blockIdx.x];
+    const ce_spec_dev sp = specs[ch.spec];
+    const ce_grid_dev &g = grids[sp.grid];
+    const int n = sp.n_in, N = g.N;
+    const double *k = k_all + sp.in_off;
+    const double *P = pk_all + sp.in_off;
+    double *yres = ws_all + sp.ws_off;
+    const chunk_geom G = make_geom(ch, n);
+    double *xs = lds, *ys = xs + CE_CHUNK_NODES, *A = ys + CE_CHUNK_NODES, *B = A + CE_CHUNK_NODES;
+    double *C = B + CE_CHUNK_NODES, *D = C + CE_CHUNK_NODES;
+    if (threadIdx.x == 0) {
+        range[0] = (G.a == 0) ? 0 : lower_index(g.p, N, g.k0, k[G.a]);
+        range[1] = (G.b == n - 1) ? N : lower_index(g.p, N, g.k0, k[G.b]);
+    }
+    stage_and_build(G, k, P, xs, ys, A, B, C, D, true);
+    const int ns = G.u1 - G.u0 + 1;
+    pcr_solve(A, B, C, D, ns);
+    // c at staged node li: D[li - 1] for u0 <= node <= u1, 0 at the global ends
+    auto cval = [&](int li) {
+        const int node = G.lo + li;
+        return (node < 1 || node > n - 2) ? 0.0 : D[node - G.u0];
+    };
+    const int i0 = range[0], i1 = range[1];
+    const int T = blockDim.x, tid = threadIdx.x;
+    const int CH = (i1 - i0 + T - 1) / T;
+    const int s0 = i0 + tid * CH, s1 = min(s0 + CH, i1);
+    if (s0 >= s1) return;
     const double klo = k[0], khi = k[n - 1];
-    int j = first_interval(k, n, k0 * g.p[i0]);
-    for (int i = i0; i < i1; ++i) {
-        const double xq = k0 * g.p[i];
-        while (j < n - 2 && k[j + 1] <= xq) ++j;
+    const int cnt = G.hi - G.lo + 1;
+    int j = -1;
+    for (int i = s0; i < s1; ++i) {
+        const double xq = g.k0 * g.p[i];
         double S;
         if (!(xq >= klo && xq <= khi)) {
             S = __builtin_nan("");
         } else {
-            const double dx = k[j + 1] - k[j];
-            const double dy = P[j + 1] - P[j];
-            const double delx = xq - k[j];
-            const double bb = dy / dx - dx * (c[j + 1] + 2.0 * c[j]) / 3.0;
-            const double dd = (c[j + 1] - c[j]) / (3.0 * dx);
-            S = P[j] + delx * (bb + delx * (c[j] + delx * dd));
+            if (j < 0) j = lds_interval(xs, cnt, xq);
+            while (j < cnt - 2 && xs[j + 1] <= xq) ++j;
+            const double dx = xs[j + 1] - xs[j];
+            const double dy = ys[j + 1] - ys[j];
+            const double delx = xq - xs[j];
+            const double cj = cval(j), cj1 = cval(j + 1);
+            const double bb = dy / dx - dx * (cj1 + 2.0 * cj) / 3.0;
+            const double dd = (cj1 - cj) / (3.0 * dx);
+            S = ys[j] + delx * (bb + delx * (cj + delx * dd));
         }
-        out[i] = bw[i] * S;
+        yres[i] = S;
     }
 }
 
+// k_prepare: y_i = bias_q(i) window(x_i) S(x_i) -> r2c -> conj, per spectrum and q'
 __global__ void __launch_bounds__(1024) k_prepare(const ce_spec_dev *__restrict__ specs,
                                                   const ce_grid_dev *__restrict__ grids, int nq,
-                                                  const double *__restrict__ k_all, const double *__restrict__ pk_all,
-                                                  double *__restrict__ ws_all, cd *__restrict__ xf_all)
+                                                  const double *__restrict__ ws_all, cd *__restrict__ xf_all)
 {
     extern __shared__ cd smem[];
     const ce_spec_dev sp = specs[blockIdx.x];
     const ce_grid_dev &g = grids[sp.grid];
-    const int n = sp.n_in, M = g.M;
-    const double *k = k_all + sp.in_off;
-    const double *P = pk_all + sp.in_off;
-    double *c = ws_all + sp.ws_off;
-    double *gam = c + n, *zz = gam + n;
+    const int M = g.M, N = g.N;
+    const double *yres = ws_all + sp.ws_off;
     const cd *tw = reinterpret_cast<const cd *>(g.tw);
-    spline_solve(k, P, n, true, nullptr, c, gam, zz);
+    double *sr = reinterpret_cast<double *>(smem);
     for (int q = 0; q < nq; ++q) {
-        resample(g, k, P, c, n, g.bw[q], reinterpret_cast<double *>(smem));
+        const double *bw = g.bw[q];
+        for (int i = threadIdx.x; i < N; i += blockDim.x) sr[i] = bw[i] * yres[i];
         __syncthreads();
         fft_lds(smem, M, g.rp, tw);
         cd *xf = xf_all + sp.xf_off + (int64_t)q * (M + 1);
@@ -647,41 +729,22 @@ __global__ void k_project(int nspec, int ntab, int nz, int nl, int nsep, const d
     }
 }
 
-// first resample index i with kmin p_i >= v (p increasing)
-__device__ __forceinline__ int lower_index(const double *p, int N, double kmin, double v)
-{
-    int lo = 0, hi = N;
-    while (lo < hi) {
-        const int m = (lo + hi) >> 1;
-        if (kmin * p[m] < v) lo = m + 1;
-        else hi = m;
-    }
-    return lo;
-}
-
-__global__ void __launch_bounds__(1024) k_prepare_vjp(const ce_spec_dev *__restrict__ specs,
-                                                      const ce_grid_dev *__restrict__ grids,
-                                                      const ce_table_desc *__restrict__ tabs, int ntab, int nq,
-                                                      const double *__restrict__ k_all,
-                                                      const double *__restrict__ pk_all, double *__restrict__ ws_all,
-                                                      const cd *__restrict__ xbar_all, double *__restrict__ gpk_all)
+// k_adj_c2r: per spectrum, Ybar = conj(sum_t xbar_t) -> transposed r2c (a c2r with
+// the interior halved) -> times bias*window -> sbar (cotangent of S(x_i))
+__global__ void __launch_bounds__(1024) k_adj_c2r(const ce_spec_dev *__restrict__ specs,
+                                                  const ce_grid_dev *__restrict__ grids,
+                                                  const ce_table_desc *__restrict__ tabs, int ntab, int nq,
+                                                  double *__restrict__ ws_all, const cd *__restrict__ xbar_all)
 {
     extern __shared__ cd smem[];
     const ce_spec_dev sp = specs[blockIdx.x];
     const ce_grid_dev &g = grids[sp.grid];
-    const int n = sp.n_in, M = g.M, N = g.N;
+    const int M = g.M, N = g.N;
     const int T = blockDim.x, tid = threadIdx.x;
-    const double *k = k_all + sp.in_off;
-    double *c = ws_all + sp.ws_off;          // spline coefficients from the forward pass
-    double *gam = c + n, *zz = gam + n;
-    double *lam = zz + n;                    // n: cbar (rhs of the transposed solve)
-    double *sbar = lam + n;                  // N: cotangent of the resampled spline values
-    double *lsol = sbar + N;                 // n: lambda
+    double *sbar = ws_all + sp.ws_off + N;
     const cd *tw = reinterpret_cast<const cd *>(g.tw);
     const cd *xbar = xbar_all + sp.xb_off;
-    double *gpk = gpk_all + sp.in_off;
     for (int q = 0; q < nq; ++q) {
-        // Ybar = conj(sum_t xbar_t);  ybar = c2r(Z), Z_0 = Ybar_0, Z_M = Ybar_M, Z_k = Ybar_k / 2
         for (int kk = tid; kk <= M / 2; kk += T) {
             const int k1 = kk, k2 = M - kk;
             cd S1 = mk(0, 0), S2 = mk(0, 0);
@@ -705,55 +768,74 @@ __global__ void __launch_bounds__(1024) k_prepare_vjp(const ce_spec_dev *__restr
         }
         __syncthreads();
     }
-    // spline transpose: gather the resample contributions per input node j
-    const double kmin = g.k0;
-    for (int j = tid; j < n; j += T) {
-        double pb = 0.0, cb = 0.0;
-        // interval j (node j is the lower end)
-        if (j <= n - 2) {
-            const int i0 = lower_index(g.p, N, kmin, k[j]);
-            const double dx = k[j + 1] - k[j];
-            for (int i = i0; i < N; ++i) {
-                const double xq = kmin * g.p[i];  // kmin holds k0 here
-                if (!(xq < k[j + 1] || (j == n - 2 && xq <= k[j + 1]))) break;
-                const double d = xq - k[j];
-                const double s = sbar[i];
-                pb += s * (1.0 - d / dx);
-                cb += s * (-2.0 * d * dx / 3.0 + d * d - d * d * d / (3.0 * dx));
-            }
-        }
-        // interval j-1 (node j is the upper end)
-        if (j >= 1) {
-            const int i0 = lower_index(g.p, N, kmin, k[j - 1]);
-            const double dx = k[j] - k[j - 1];
-            for (int i = i0; i < N; ++i) {
-                const double xq = kmin * g.p[i];  // kmin holds k0 here
-                if (!(xq < k[j] || (j - 1 == n - 2 && xq <= k[j]))) break;
-                const double d = xq - k[j - 1];
-                const double s = sbar[i];
-                pb += s * d / dx;
-                cb += s * (-d * dx / 3.0 + d * d * d / (3.0 * dx));
-            }
-        }
-        gpk[j] = pb;
-        if (j >= 1 && j <= n - 2) lam[j - 1] = cb;   // rhs of the transposed solve
+}
+
+// k_spline_adj: transposed resample and spline on the chunk's nodes -> gpk[a..b)
+// (the last chunk also owns node n - 1).  The direct and c-coefficient cotangents
+// of every resample point are scattered with LDS atomics, lambda = T^{-1} cbar is
+// solved on the truncated system, and dP_j collects the y-dependence of the
+// right-hand side g (twofast.c:399-401 / the natural-spline equations).
+__global__ void __launch_bounds__(256) k_spline_adj(const ce_chunk_dev *__restrict__ chunks,
+                                                    const ce_spec_dev *__restrict__ specs,
+                                                    const ce_grid_dev *__restrict__ grids,
+                                                    const double *__restrict__ k_all,
+                                                    const double *__restrict__ ws_all, double *__restrict__ gpk_all)
+{
+    __shared__ double lds[7 * CE_CHUNK_NODES];
+    __shared__ int range[2];
+    const ce_chunk_dev ch = chunks[blockIdx.x];
+    const ce_spec_dev sp = specs[ch.spec];
+    const ce_grid_dev &g = grids[sp.grid];
+    const int n = sp.n_in, N = g.N;
+    const double *k = k_all + sp.in_off;
+    const double *sbar = ws_all + sp.ws_off + N;
+    double *gpk = gpk_all + sp.in_off;
+    const chunk_geom G = make_geom(ch, n);
+    const int cnt = G.hi - G.lo + 1;
+    const int ns = G.u1 - G.u0 + 1;
+    double *xs = lds, *pb = xs + CE_CHUNK_NODES, *cb = pb + CE_CHUNK_NODES, *A = cb + CE_CHUNK_NODES;
+    double *B = A + CE_CHUNK_NODES, *C = B + CE_CHUNK_NODES, *D = C + CE_CHUNK_NODES;
+    const int T = blockDim.x, tid = threadIdx.x;
+    if (tid == 0) {
+        range[0] = (G.lo == 0) ? 0 : lower_index(g.p, N, g.k0, k[G.lo]);
+        range[1] = (G.hi == n - 1) ? N : lower_index(g.p, N, g.k0, k[G.hi]);
+    }
+    for (int i = tid; i < cnt; i += T) { pb[i] = 0.0; cb[i] = 0.0; }
+    stage_and_build(G, k, nullptr, xs, nullptr, A, B, C, D, false);
+    // scatter the resample cotangents of the points with intervals in [lo, hi)
+    const int i0 = range[0], i1 = range[1];
+    const int CH = (i1 - i0 + T - 1) / T;
+    const int s0 = i0 + tid * CH, s1 = min(s0 + CH, i1);
+    const double klo = k[0], khi = k[n - 1];
+    int j = -1;
+    for (int i = s0; i < s1; ++i) {
+        const double xq = g.k0 * g.p[i];
+        if (!(xq >= klo && xq <= khi)) continue;
+        if (j < 0) j = lds_interval(xs, cnt, xq);
+        while (j < cnt - 2 && xs[j + 1] <= xq) ++j;
+        const double dx = xs[j + 1] - xs[j];
+        const double d = xq - xs[j];
+        const double sv = sbar[i];
+        atomicAdd(&pb[j], sv * (1.0 - d / dx));
+        atomicAdd(&pb[j + 1], sv * d / dx);
+        atomicAdd(&cb[j], sv * (-2.0 * d * dx / 3.0 + d * d - d * d * d / (3.0 * dx)));
+        atomicAdd(&cb[j + 1], sv * (-d * dx / 3.0 + d * d * d / (3.0 * dx)));
     }
     __syncthreads();
-    // lambda = T^{-1} cbar (T symmetric); gam/zz are reused as scratch
-    spline_solve(k, nullptr, n, false, lam, lsol, gam, zz);
-    // lsol holds lambda at positions 1..n-2 (0 at the ends)
-    for (int j = tid; j < n; j += T) {
-        auto hh = [&](int i) { return k[i + 1] - k[i]; };
-        double acc = gpk[j];
-        // g_s = 3((y_{s+2}-y_{s+1})/h_{s+1} - (y_{s+1}-y_s)/h_s), lambda_s at sbar[s+1]
+    for (int r = tid; r < ns; r += T) D[r] = cb[r + 1];   // node u0 + r sits at staged index r + 1
+    __syncthreads();
+    pcr_solve(A, B, C, D, ns);
+    auto lam = [&](int node) { return (node < G.u0 || node > G.u1) ? 0.0 : D[node - G.u0]; };
+    const int jend = (G.b == n - 1) ? n : G.b;
+    for (int jj = G.a + tid; jj < jend; jj += T) {
+        const int li = jj - G.lo;
+        double acc = pb[li];
         const int sys = n - 2;
-        // s = j - 2: dg/dy_{s+2} = 3/h_{s+1}
-        if (j - 2 >= 0 && j - 2 < sys) acc += lsol[j - 1] * 3.0 / hh(j - 1);
-        // s = j - 1: dg/dy_{s+1} = -3/h_{s+1} - 3/h_s
-        if (j - 1 >= 0 && j - 1 < sys) acc += lsol[j] * (-3.0 / hh(j) - 3.0 / hh(j - 1));
-        // s = j: dg/dy_s = 3/h_s
-        if (j < sys) acc += lsol[j + 1] * 3.0 / hh(j);
-        gpk[j] = acc;
+        // g_s (node s + 1) = 3((y_{s+2} - y_{s+1})/h_{s+1} - (y_{s+1} - y_s)/h_s)
+        if (jj - 2 >= 0 && jj - 2 < sys) acc += lam(jj - 1) * 3.0 / (xs[li] - xs[li - 1]);
+        if (jj - 1 >= 0 && jj - 1 < sys) acc += lam(jj) * (-3.0 / (xs[li + 1] - xs[li]) - 3.0 / (xs[li] - xs[li - 1]));
+        if (jj < sys) acc += lam(jj + 1) * 3.0 / (xs[li + 1] - xs[li]);
+        gpk[jj] = acc;
     }
 }
 
@@ -800,14 +882,22 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
     return (int)hipGetLastError();
 }
 
+int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
+                       const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws)
+{
+    hipLaunchKernelGGL(k_resample, dim3(nchunk), dim3(256), 0, (hipStream_t)stream, d_chunks, d_spec, d_grids, d_k,
+                       d_pk, d_ws);
+    return (int)hipGetLastError();
+}
+
 int ce_launch_prepare(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids, int nq,
-                      const double *d_k, const double *d_pk, double *d_ws, double *d_xf, int threads, int max_M)
+                      const double *d_ws, double *d_xf, int threads, int max_M)
 {
     const size_t lds = (size_t)max_M * sizeof(cd);
     int st = set_lds((const void *)k_prepare, lds);
     if (st) return st;
-    hipLaunchKernelGGL(k_prepare, dim3(nspec), dim3(threads), lds, (hipStream_t)stream, d_spec, d_grids, nq, d_k,
-                       d_pk, d_ws, reinterpret_cast<cd *>(d_xf));
+    hipLaunchKernelGGL(k_prepare, dim3(nspec), dim3(threads), lds, (hipStream_t)stream, d_spec, d_grids, nq, d_ws,
+                       reinterpret_cast<cd *>(d_xf));
     return (int)hipGetLastError();
 }
 
@@ -856,15 +946,23 @@ int ce_launch_project(void *stream, int nspec, int ntab, int nz, int nl, int nse
     return (int)hipGetLastError();
 }
 
-int ce_launch_prepare_vjp(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
-                          const ce_table_desc *d_tabs, int ntab, int nq, const double *d_k, const double *d_pk,
-                          double *d_ws, const double *d_xbar, double *d_gpk, int threads, int max_M)
+int ce_launch_adj_c2r(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
+                      const ce_table_desc *d_tabs, int ntab, int nq, double *d_ws, const double *d_xbar, int threads,
+                      int max_M)
 {
     const size_t lds = (size_t)max_M * sizeof(cd);
-    int st = set_lds((const void *)k_prepare_vjp, lds);
+    int st = set_lds((const void *)k_adj_c2r, lds);
     if (st) return st;
-    hipLaunchKernelGGL(k_prepare_vjp, dim3(nspec), dim3(threads), lds, (hipStream_t)stream, d_spec, d_grids, d_tabs,
-                       ntab, nq, d_k, d_pk, d_ws, reinterpret_cast<const cd *>(d_xbar), d_gpk);
+    hipLaunchKernelGGL(k_adj_c2r, dim3(nspec), dim3(threads), lds, (hipStream_t)stream, d_spec, d_grids, d_tabs, ntab,
+                       nq, d_ws, reinterpret_cast<const cd *>(d_xbar));
+    return (int)hipGetLastError();
+}
+
+int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
+                         const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk)
+{
+    hipLaunchKernelGGL(k_spline_adj, dim3(nchunk), dim3(256), 0, (hipStream_t)stream, d_chunks, d_spec, d_grids, d_k,
+                       d_ws, d_gpk);
     return (int)hipGetLastError();
 }
 
