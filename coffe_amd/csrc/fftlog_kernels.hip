@@ -150,14 +150,16 @@ __device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restr
     __syncthreads();
 }
 
+// Butterflies per thread: the host launches at least M/8 threads, so a radix-R pass
+// has at most ceil(8/R) butterflies per thread; only those register shapes are
+// instantiated (at most 12 complex values live per thread).
 template <int R>
 __device__ __forceinline__ void fft_pass_dispatch(cd *s, int M, int Ns, const cd *__restrict__ tw)
 {
     const int B = M / R;
     const int per = (B + (int)blockDim.x - 1) / (int)blockDim.x;
-    // the host launches at least M/8 threads, so per <= 4 for every radix
-    if (per <= 1) fft_pass<R, 1>(s, M, Ns, tw);
-    else if (per <= 2) fft_pass<R, 2>(s, M, Ns, tw);
+    if (R == 8 || per <= 1) fft_pass<R, 1>(s, M, Ns, tw);
+    else if (R == 4 || R == 5 || per <= 2) fft_pass<R, 2>(s, M, Ns, tw);
     else fft_pass<R, 4>(s, M, Ns, tw);
 }
 
@@ -377,6 +379,28 @@ __device__ __forceinline__ int lower_index(const double *p, int N, double k0, do
     return lo;
 }
 
+// Same search done by one wavefront: 64 probes per round narrow [lo, hi) 64-fold,
+// so N = 16384 takes three dependent loads instead of fourteen.  Every lane of the
+// calling wavefront must call it; all lanes return the index.
+__device__ __forceinline__ int wave_lower_index(const double *p, int N, double k0, double v)
+{
+    const int lane = threadIdx.x & 63;
+    int lo = 0, hi = N;   // answer in [lo, hi]
+    while (hi - lo > 64) {
+        const int step = (hi - lo + 63) / 64;
+        const int idx = lo + lane * step;
+        const bool below = idx < hi && k0 * p[idx] < v;
+        const int c = __popcll(__ballot(below));
+        const int nlo = (c > 0) ? lo + (c - 1) * step + 1 : lo;
+        const int nhi = min(hi, lo + c * step);
+        lo = nlo;
+        hi = nhi;
+    }
+    const int idx = lo + lane;
+    const bool below = idx < hi && k0 * p[idx] < v;
+    return lo + __popcll(__ballot(below));
+}
+
 // interval j of the LDS-staged nodes xs[0..cnt) (global index = j + base) with
 // xs[j] <= v < xs[j+1]; bsearch semantics over the staged range
 __device__ __forceinline__ int lds_interval(const double *xs, int cnt, double v)
@@ -454,9 +478,15 @@ __global__ void __launch_bounds__(256) k_resample(const ce_chunk_dev *__restrict
     const chunk_geom G = make_geom(ch, n);
     double *xs = lds, *ys = xs + CE_CHUNK_NODES, *A = ys + CE_CHUNK_NODES, *B = A + CE_CHUNK_NODES;
     double *C = B + CE_CHUNK_NODES, *D = C + CE_CHUNK_NODES;
-    if (threadIdx.x == 0) {
-        range[0] = (G.a == 0) ? 0 : lower_index(g.p, N, g.k0, k[G.a]);
-        range[1] = (G.b == n - 1) ? N : lower_index(g.p, N, g.k0, k[G.b]);
+    {
+        const int wave = threadIdx.x >> 6;
+        if (wave == 0) {
+            const int r0 = (G.a == 0) ? 0 : wave_lower_index(g.p, N, g.k0, k[G.a]);
+            if ((threadIdx.x & 63) == 0) range[0] = r0;
+        } else if (wave == 1) {
+            const int r1 = (G.b == n - 1) ? N : wave_lower_index(g.p, N, g.k0, k[G.b]);
+            if ((threadIdx.x & 63) == 0) range[1] = r1;
+        }
     }
     stage_and_build(G, k, P, xs, ys, A, B, C, D, true);
     const int ns = G.u1 - G.u0 + 1;
@@ -468,20 +498,15 @@ __global__ void __launch_bounds__(256) k_resample(const ce_chunk_dev *__restrict
     };
     const int i0 = range[0], i1 = range[1];
     const int T = blockDim.x, tid = threadIdx.x;
-    const int CH = (i1 - i0 + T - 1) / T;
-    const int s0 = i0 + tid * CH, s1 = min(s0 + CH, i1);
-    if (s0 >= s1) return;
     const double klo = k[0], khi = k[n - 1];
     const int cnt = G.hi - G.lo + 1;
-    int j = -1;
-    for (int i = s0; i < s1; ++i) {
+    for (int i = i0 + tid; i < i1; i += T) {
         const double xq = g.k0 * g.p[i];
         double S;
         if (!(xq >= klo && xq <= khi)) {
             S = __builtin_nan("");
         } else {
-            if (j < 0) j = lds_interval(xs, cnt, xq);
-            while (j < cnt - 2 && xs[j + 1] <= xq) ++j;
+            const int j = lds_interval(xs, cnt, xq);
             const double dx = xs[j + 1] - xs[j];
             const double dy = ys[j + 1] - ys[j];
             const double delx = xq - xs[j];
@@ -526,6 +551,7 @@ __global__ void __launch_bounds__(1024) k_prepare(const ce_spec_dev *__restrict_
 // Akima (1970, non-rounded ends) value on interval [X2, X3] of the 6-point stencil
 // X[0..5], Y[0..5] (GSL akima: slopes m, weights alpha).  With dY non-null the
 // derivative of the value with respect to Y[0..5] is returned too.
+template <bool WANT_DY>
 __device__ __forceinline__ double akima_stencil(const double *X, const double *Y, double xq, double *dY)
 {
     double m[5], inv[5];
@@ -559,7 +585,7 @@ __device__ __forceinline__ double akima_stencil(const double *X, const double *Y
         const double cc = (3.0 * m[2] - 2.0 * bb - tn) / h;
         const double dd = (bb + tn - 2.0 * m[2]) / (h * h);
         val = Y[2] + delx * (bb + delx * (cc + dd * delx));
-        if (dY) {
+        if (WANT_DY) {
             const double u = delx / h;
             const double dv_db = h * (u - 2 * u * u + u * u * u);
             const double dv_dt = h * (u * u * u - u * u);
@@ -592,7 +618,7 @@ __device__ __forceinline__ double akima_stencil(const double *X, const double *Y
             }
         }
     }
-    if (dY) {
+    if (WANT_DY) {
 #pragma unroll
         for (int a = 0; a < 6; ++a) dY[a] = 0.0;
         dY[2] = dY2;
@@ -671,7 +697,7 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
             X[m] = g.r0 * g.p[j - 2 + m];
             Y[m] = os[j - 2 + m] * c2r_value(smem, j - 2 + m);
         }
-        const double v = akima_stencil(X, Y, a.sep[r], adj ? dY : nullptr);
+        const double v = adj ? akima_stencil<true>(X, Y, a.sep[r], dY) : akima_stencil<false>(X, Y, a.sep[r], dY);
         if (a.itab) a.itab[((int64_t)u.spec * a.ntab + u.tab) * a.nsep + r] = v;
         if (adj) {
             double ib = 0.0;
@@ -796,23 +822,25 @@ __global__ void __launch_bounds__(256) k_spline_adj(const ce_chunk_dev *__restri
     double *xs = lds, *pb = xs + CE_CHUNK_NODES, *cb = pb + CE_CHUNK_NODES, *A = cb + CE_CHUNK_NODES;
     double *B = A + CE_CHUNK_NODES, *C = B + CE_CHUNK_NODES, *D = C + CE_CHUNK_NODES;
     const int T = blockDim.x, tid = threadIdx.x;
-    if (tid == 0) {
-        range[0] = (G.lo == 0) ? 0 : lower_index(g.p, N, g.k0, k[G.lo]);
-        range[1] = (G.hi == n - 1) ? N : lower_index(g.p, N, g.k0, k[G.hi]);
+    {
+        const int wave = tid >> 6;
+        if (wave == 0) {
+            const int r0 = (G.lo == 0) ? 0 : wave_lower_index(g.p, N, g.k0, k[G.lo]);
+            if ((tid & 63) == 0) range[0] = r0;
+        } else if (wave == 1) {
+            const int r1 = (G.hi == n - 1) ? N : wave_lower_index(g.p, N, g.k0, k[G.hi]);
+            if ((tid & 63) == 0) range[1] = r1;
+        }
     }
     for (int i = tid; i < cnt; i += T) { pb[i] = 0.0; cb[i] = 0.0; }
     stage_and_build(G, k, nullptr, xs, nullptr, A, B, C, D, false);
     // scatter the resample cotangents of the points with intervals in [lo, hi)
     const int i0 = range[0], i1 = range[1];
-    const int CH = (i1 - i0 + T - 1) / T;
-    const int s0 = i0 + tid * CH, s1 = min(s0 + CH, i1);
     const double klo = k[0], khi = k[n - 1];
-    int j = -1;
-    for (int i = s0; i < s1; ++i) {
+    for (int i = i0 + tid; i < i1; i += T) {
         const double xq = g.k0 * g.p[i];
         if (!(xq >= klo && xq <= khi)) continue;
-        if (j < 0) j = lds_interval(xs, cnt, xq);
-        while (j < cnt - 2 && xs[j + 1] <= xq) ++j;
+        const int j = lds_interval(xs, cnt, xq);
         const double dx = xs[j + 1] - xs[j];
         const double d = xq - xs[j];
         const double sv = sbar[i];
