@@ -3,7 +3,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcoffe_engine.so")
+LIB_PATH = os.environ.get("COFFE_ENGINE_LIBRARY") or os.path.join(_HERE, "libcoffe_engine.so")
 
 _lib = None
 

@@ -20,8 +20,9 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
 ]
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
-            f"-I{ROCM}/include", "-fno-fast-math"]
+            f"-I{ROCM}/include", f"-I{INCLUDE}", "-fno-fast-math"]
 
 
 def _run(cmd):

@@ -21,7 +21,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include "../../include/coffe_background.h"
+#include "coffe_background.h"
 
 #define BG_H0 (1. / 2997.92458) /* h / Mpc */
 

@@ -21,7 +21,7 @@
 
 #include <hip/hip_runtime_api.h>
 
-#include "../../include/coffe_engine.h"
+#include "coffe_engine.h"
 #include "ce_internal.h"
 
 #define CE_THREADS_DEFAULT 512

@@ -7,7 +7,7 @@
 #include <math.h>
 #include <stdlib.h>
 
-#include "../../include/coffe_engine.h"
+#include "coffe_engine.h"
 
 static size_t bsearch_lo(const double *x, double v, size_t lo, size_t hi)
 {
