@@ -43,6 +43,18 @@ __device__ __forceinline__ cd cscale(cd a, double s) { return mk(a.x * s, a.y * 
 __device__ __forceinline__ cd mul_mi(cd a) { return mk(a.y, -a.x); }   // * (-i)
 __device__ __forceinline__ cd mul_pi(cd a) { return mk(-a.y, a.x); }   // * (+i)
 
+// ---- LDS layout ----------------------------------------------------------------
+// Complex element i of a transform lives at LDS slot pidx(i) = i + i/16: one pad slot
+// per 16 elements keeps the strided stores of the early Stockham passes (stride R
+// elements of 16 bytes) on distinct banks.
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
+__host__ __device__ __forceinline__ int lds_data_len(int M) { return M + (M >> 4) + 1; }
+// real element i of a transform stored as complex pairs (x_{2j}, x_{2j+1})
+__device__ __forceinline__ double &lds_real(cd *s, int i)
+{
+    return reinterpret_cast<double *>(s)[2 * pidx(i >> 1) + (i & 1)];
+}
+
 // ---- radix-R forward DFT butterflies (sign -1) --------------------------------
 template <int R> __device__ __forceinline__ void dft(cd *v);
 
@@ -145,7 +157,7 @@ __device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restr
         const int j = tid + b * T;
         if (j < B) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = s[j + r * B];
+            for (int r = 0; r < R; ++r) v[b][r] = s[pidx(j + r * B)];
         }
     }
     __syncthreads();
@@ -162,7 +174,7 @@ __device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restr
             dft<R>(v[b]);
             const int base = (j - jm) * R + jm;
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[base + r * Ns] = v[b][r];
+            for (int r = 0; r < R; ++r) s[pidx(base + r * Ns)] = v[b][r];
         }
     }
     __syncthreads();
@@ -398,19 +410,8 @@ __device__ void pcr_solve(double *__restrict__ A, double *__restrict__ R, double
     __syncthreads();
 }
 
-// first resample index i with k0 p_i >= v (p increasing); N if none
-__device__ __forceinline__ int lower_index(const double *p, int N, double k0, double v)
-{
-    int lo = 0, hi = N;
-    while (lo < hi) {
-        const int m = (lo + hi) >> 1;
-        if (k0 * p[m] < v) lo = m + 1;
-        else hi = m;
-    }
-    return lo;
-}
-
-// Same search done by one wavefront: 64 probes per round narrow [lo, hi) 64-fold,
+// First resample index i with k0 p_i >= v (p increasing; N if none), found by one
+// wavefront: 64 probes per round narrow [lo, hi) 64-fold,
 // so N = 16384 takes three dependent loads instead of fourteen.  Every lane of the
 // calling wavefront must call it; all lanes return the index.
 __device__ __forceinline__ int wave_lower_index(const double *p, int N, double k0, double v)
@@ -562,19 +563,18 @@ __global__ void __launch_bounds__(1024) k_prepare(const ce_spec_dev *__restrict_
     const int M = g.M, N = g.N;
     const double *yres = ws_all + sp.ws_off;
     const cd *tw = reinterpret_cast<const cd *>(g.tw);
-    cd *lt = smem + max_M;
+    cd *lt = smem + lds_data_len(max_M);
     load_twiddles(lt, M, tw);
-    double *sr = reinterpret_cast<double *>(smem);
     for (int q = 0; q < nq; ++q) {
         const double *bw = g.bw[q];
-        for (int i = threadIdx.x; i < N; i += blockDim.x) sr[i] = bw[i] * yres[i];
+        for (int i = threadIdx.x; i < N; i += blockDim.x) lds_real(smem, i) = bw[i] * yres[i];
         __syncthreads();
         fft_lds(smem, M, g.rp, lt);
         cd *xf = xf_all + sp.xf_off + (int64_t)q * (M + 1);
         for (int kk = threadIdx.x; kk <= M / 2; kk += blockDim.x) {
             const int k1 = kk, k2 = M - kk;
-            const cd Z1 = smem[k1];
-            const cd Z2 = smem[k2 == M ? 0 : k2];
+            const cd Z1 = smem[pidx(k1)];
+            const cd Z2 = smem[pidx(k2 == M ? 0 : k2)];
             xf[k1] = cconj(r2c_split(Z1, Z2, tw[k1]));
             if (k2 != k1) xf[k2] = cconj(r2c_split(Z2, Z1, tw[k2]));
         }
@@ -668,7 +668,7 @@ __device__ __forceinline__ double akima_stencil(const double *X, const double *Y
 // table value i of the c2r output held (as conj of the forward transform) in LDS
 __device__ __forceinline__ double c2r_value(const cd *s, int i)
 {
-    const cd z = s[i >> 1];
+    const cd z = s[pidx(i >> 1)];
     return (i & 1) ? -z.y : z.x;
 }
 
@@ -703,7 +703,7 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     const int M = g.M, N = g.N;
     const int T = blockDim.x, tid = threadIdx.x;
     const cd *tw = reinterpret_cast<const cd *>(g.tw);
-    cd *lt = smem + a.max_M;
+    cd *lt = smem + lds_data_len(a.max_M);
     load_twiddles(lt, M, tw);
     const cd *U = reinterpret_cast<const cd *>(g.U[u.tab]);
     const double *os = g.os[u.tab];
@@ -714,8 +714,8 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
         cd X1 = cmul(xf[k1], U[k1]);
         cd X2 = cmul(xf[k2], U[k2]);
         if (k1 == 0) { X1.y = 0.0; X2.y = 0.0; }
-        smem[k1] = cconj(c2r_merge(X1, X2, tw[k1]));
-        if (k2 != k1 && k2 < M) smem[k2] = cconj(c2r_merge(X2, X1, tw[k2]));
+        smem[pidx(k1)] = cconj(c2r_merge(X1, X2, tw[k1]));
+        if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(X2, X1, tw[k2]));
     }
     __syncthreads();
     fft_lds(smem, M, g.rp, lt);
@@ -748,13 +748,12 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     }
     if (!adj) return;
     __syncthreads();
-    double *sr = reinterpret_cast<double *>(smem);
-    for (int i = tid; i < N; i += T) sr[i] = 0.0;
+    for (int i = tid; i < N; i += T) lds_real(smem, i) = 0.0;
     __syncthreads();
     for (int r = tid; r < a.nsep; r += T) {
         const int j = sidx[r];
 #pragma unroll
-        for (int m = 0; m < 6; ++m) atomicAdd(&sr[j - 2 + m], wb[r * 6 + m]);
+        for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(smem, j - 2 + m), wb[r * 6 + m]);
     }
     __syncthreads();
     // transposed c2r: Xbar_k = r2c(ybar)_k, doubled for 0 < k < M, real at k = 0, M
@@ -762,8 +761,8 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     cd *xb = a.xbar + sp.xb_off + (int64_t)u.tab * (M + 1);
     for (int kk = tid; kk <= M / 2; kk += T) {
         const int k1 = kk, k2 = M - kk;
-        const cd Z1 = smem[k1];
-        const cd Z2 = smem[k2 == M ? 0 : k2];
+        const cd Z1 = smem[pidx(k1)];
+        const cd Z2 = smem[pidx(k2 == M ? 0 : k2)];
         cd R1 = r2c_split(Z1, Z2, tw[k1]);
         cd R2 = r2c_split(Z2, Z1, tw[k2]);
         R1 = (k1 == 0) ? mk(R1.x, 0.0) : cscale(R1, 2.0);
@@ -807,7 +806,7 @@ __global__ void __launch_bounds__(1024) k_adj_c2r(const ce_spec_dev *__restrict_
     const int T = blockDim.x, tid = threadIdx.x;
     double *sbar = ws_all + sp.ws_off + N;
     const cd *tw = reinterpret_cast<const cd *>(g.tw);
-    cd *lt = smem + max_M;
+    cd *lt = smem + lds_data_len(max_M);
     load_twiddles(lt, M, tw);
     const cd *xbar = xbar_all + sp.xb_off;
     for (int q = 0; q < nq; ++q) {
@@ -822,8 +821,8 @@ __global__ void __launch_bounds__(1024) k_adj_c2r(const ce_spec_dev *__restrict_
             cd Y1 = cconj(S1), Y2 = cconj(S2);
             Y1 = (k1 == 0) ? mk(Y1.x, 0.0) : cscale(Y1, 0.5);
             Y2 = (k2 == M) ? mk(Y2.x, 0.0) : cscale(Y2, 0.5);
-            smem[k1] = cconj(c2r_merge(Y1, Y2, tw[k1]));
-            if (k2 != k1 && k2 < M) smem[k2] = cconj(c2r_merge(Y2, Y1, tw[k2]));
+            smem[pidx(k1)] = cconj(c2r_merge(Y1, Y2, tw[k1]));
+            if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(Y2, Y1, tw[k2]));
         }
         __syncthreads();
         fft_lds(smem, M, g.rp, lt);
@@ -958,7 +957,10 @@ int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, c
     return (int)hipGetLastError();
 }
 
-static size_t fft_lds_bytes(int max_M) { return ((size_t)max_M + 64 + (size_t)(max_M + 63) / 64) * sizeof(cd); }
+static size_t fft_lds_bytes(int max_M)
+{
+    return ((size_t)lds_data_len(max_M) + 64 + (size_t)(max_M + 63) / 64) * sizeof(cd);
+}
 
 int ce_launch_prepare(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids, int nq,
                       const double *d_ws, double *d_xf, int threads, int max_M)
