@@ -64,6 +64,8 @@ def _declare(L):
     L.orc_threej000.restype = d
     L.orc_legendre_integral.argtypes = [i, i, i, i]
     L.orc_legendre_integral.restype = d
+    L.orc_covariance_coefficient.argtypes = [d, d, d, i]
+    L.orc_covariance_coefficient.restype = d
     L.orc_covariance_assemble.argtypes = [s, s, ctypes.POINTER(ctypes.c_int), s, p, p, p, p, p, p, p, i, i, i,
                                           p, p, p]
     ll = ctypes.POINTER(ctypes.c_longlong)
