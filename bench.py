@@ -47,32 +47,38 @@ def build_inputs(nspec=256, nsep=512, seed=0):
 
 
 def algorithmic_flops(n_fft, n_in, nsep, ntab, nq=1):
-    """flop counts per stage (real FFT of length N = 2.5 N log2 N; complex multiply 6;
-    natural-spline solve 10 per node; resample 12 per point; Akima 40 per value and
-    80 more for its VJP)."""
+    """algorithmic flops per kernel launch (real FFT of length N = 2.5 N log2 N; complex
+    multiply 6; natural-spline system build + solve 12 per node; spline evaluation 12 per
+    resample point; Akima 40 per value and 80 more for its VJP)."""
     n_fft = np.asarray(n_fft, dtype=np.float64)
     n_in = np.asarray(n_in, dtype=np.float64)
     fft = 2.5 * n_fft * np.log2(n_fft)
-    kmul = 6.0 * (n_fft / 2 + 1)
-    prep = np.sum(10 * n_in + nq * (13 * n_fft + fft + 2 * (n_fft / 2 + 1)))
-    tables_fwd = ntab * np.sum(kmul + fft + 40.0 * nsep)
-    tables_adj = ntab * np.sum(2 * kmul + 2 * fft + 120.0 * nsep)
-    prep_adj = np.sum(nq * (fft + 6 * (n_fft / 2 + 1)) + 14 * n_fft + 20 * n_in)
-    return dict(prepare=float(prep), tables_fwd=float(tables_fwd), tables_adj=float(tables_adj),
-                prepare_adj=float(prep_adj))
+    M1 = n_fft / 2 + 1
+    return dict(
+        k_resample=float(np.sum(12 * n_in + 12 * n_fft)),
+        k_prepare=float(nq * np.sum(n_fft + fft + 8 * M1)),
+        k_tables=float(ntab * np.sum(2 * 6 * M1 + 2 * fft + 2 * 8 * M1 + 120.0 * nsep)),
+        k_project=float(2.0 * ntab * 3 * nsep * len(n_fft)),
+        k_adj_c2r=float(nq * np.sum(fft + 8 * M1 + 2 * ntab * M1 + n_fft)),
+        k_spline_adj=float(np.sum(16 * n_fft + 12 * n_in + 10 * n_in)),
+    )
 
 
 def hbm_bytes(n_fft, n_in, nsep, ntab, nspec, nz, nl, nq=1):
-    """compulsory HBM bytes per stage: inputs, grid tables read once per unit,
-    workspaces written/read once, outputs."""
+    """compulsory HBM bytes per kernel launch: inputs read once, grid tables read once
+    per work unit, workspaces written once and read once, outputs written once."""
     n_fft = np.asarray(n_fft, dtype=np.float64)
     n_in = np.asarray(n_in, dtype=np.float64)
     M1 = n_fft / 2 + 1
-    prep = np.sum(16 * n_in + 3 * 8 * n_in + nq * (8 * n_fft + 8 * n_fft + 16 * M1))
-    tab = ntab * np.sum(16 * M1 + 16 * M1 + 16 * M1 + 6 * 16 * nsep) + 8 * nspec * ntab * nsep
-    tab_adj = tab + ntab * np.sum(16 * M1) + 8 * nspec * nz * nl * nsep
-    prep_adj = np.sum(ntab * 16 * M1 + 8 * n_fft * 3 + 8 * n_in * 6)
-    return dict(prepare=float(prep), tables_adj=float(tab_adj), prepare_adj=float(prep_adj))
+    return dict(
+        k_resample=float(np.sum(16 * n_in + 8 * n_fft + 8 * n_fft)),
+        k_prepare=float(nq * np.sum(8 * n_fft + 8 * n_fft + 16 * M1)),
+        k_tables=float(ntab * np.sum(16 * M1 * 3 + 8 * 6 * 2 * nsep + 16 * M1) + 8 * nspec * ntab * nsep +
+                       8 * nspec * nz * nl * nsep),
+        k_project=float(8 * nspec * ntab * nsep + 8 * nz * nl * ntab * nsep + 8 * nspec * nz * nl * nsep),
+        k_adj_c2r=float(np.sum(ntab * 16 * M1 + 8 * n_fft * 2)),
+        k_spline_adj=float(np.sum(8 * n_in + 8 * n_fft * 2 + 8 * n_in)),
+    )
 
 
 def cpu_baseline(specs, sep, bg, phys, sample):
@@ -119,7 +125,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (5 times), to scale HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration_mb:
+        import torch
+        n = args.copy_calibration_mb * 1024 * 1024 // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: the engine is single-device (spectra shard across GPUs with no exchange; "
               "see DESIGN.md). Use --gpus 1.", file=sys.stderr)
@@ -145,25 +163,22 @@ def main():
     gpk = torch.empty_like(p)
     stream = torch.cuda.current_stream()
     sp = __import__("ctypes").c_void_p(stream.cuda_stream)
-    stages = ["prepare", "tables_adj", "project", "prepare_adj"]
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
-    stage_ms = {s: 0.0 for s in stages}
+    kernels = ["k_resample", "k_prepare", "k_tables", "k_project", "k_adj_c2r", "k_spline_adj"]
+    launches = [lambda: batch.kernel_resample(k, p, sp), lambda: batch.kernel_fft_prepare(sp),
+                lambda: batch.stage_tables(A, gout, itab, sp), lambda: batch.stage_project(A, itab, out, sp),
+                lambda: batch.kernel_adj_c2r(sp), lambda: batch.kernel_spline_adj(k, gpk, sp)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(kernels) + 1)]
+    kernel_ms = {kk: 0.0 for kk in kernels}
 
     def step(timed):
+        # one forward + adjoint pass over the batch: the six kernels in stream order,
+        # with events between them on the launch stream during timed steps
+        for i, launch in enumerate(launches):
+            if timed:
+                ev[i].record(stream)
+            launch()
         if timed:
-            ev[0].record(stream)
-        batch.stage_prepare(k, p, sp)
-        if timed:
-            ev[1].record(stream)
-        batch.stage_tables(A, gout, itab, sp)
-        if timed:
-            ev[2].record(stream)
-        batch.stage_project(A, itab, out, sp)
-        if timed:
-            ev[3].record(stream)
-        batch.stage_prepare_adj(k, p, gpk, sp)
-        if timed:
-            ev[4].record(stream)
+            ev[-1].record(stream)
 
     for _ in range(args.warmup):
         step(False)
@@ -171,15 +186,14 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(True)
-        # events of this step are read after the next synchronisation point
         torch.cuda.synchronize()
-        for i, s in enumerate(stages):
-            stage_ms[s] += ev[i].elapsed_time(ev[i + 1])
+        for i, kk in enumerate(kernels):
+            kernel_ms[kk] += ev[i].elapsed_time(ev[i + 1])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    for s in stages:
-        stage_ms[s] /= args.steps
+    for kk in kernels:
+        kernel_ms[kk] /= args.steps
 
     # parity: every point against the oracle (tables at the separations, flat-sky weights)
     ref_tab = O.tables_at_sep_batch([(s[0], s[1]) for s in specs], n_fft, TABLES, sep)
@@ -193,9 +207,12 @@ def main():
 
     flops = algorithmic_flops(n_fft, n_in, nsep, len(TABLES))
     bytes_ = hbm_bytes(n_fft, n_in, nsep, len(TABLES), nspec, 1, len(LS))
-    dom = max(["prepare", "tables_adj", "prepare_adj"], key=lambda s: stage_ms[s])
-    fl = flops[dom]
-    achieved = fl / (stage_ms[dom] / 1e3)
+    dom = max(kernels, key=lambda kk: kernel_ms[kk])
+    achieved = flops[dom] / (kernel_ms[dom] / 1e3)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_c3.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get(dom)
     res = {
         "metric": "(l,r) multipole points/sec, batched fwd+adjoint vs plain FFTLog, 1 MI355X @1e-8 rel",
         "value": value,
@@ -214,13 +231,15 @@ def main():
                    "spectra": nspec, "separations": nsep, "multipoles": LS, "tables": TABLES,
                    "points_per_step": points, "fft_lengths": sorted(set(int(n) for n in n_fft)),
                    "threads_per_wg": int(os.environ.get("COFFE_ENGINE_THREADS", 512))},
-        "roofline": {"bound": "mfma", "kernel": {"prepare": "k_prepare", "tables_adj": "k_tables",
-                                                  "prepare_adj": "k_prepare_vjp"}[dom],
+        "roofline": {"bound": "mfma", "kernel": dom,
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None,
-                     "note": "FP64 VALU (no MFMA on this path); peak = MI355X FP64 78.6 TFLOP/s",
-                     "stage_ms": stage_ms, "stage_flops": flops,
-                     "stage_hbm_GBps": {s: bytes_[s] / (stage_ms[s] / 1e3) / 1e9 for s in bytes_}},
+                     "frac": achieved / PEAK_FP64, "traffic": traffic,
+                     "note": "FP64 VALU bound (no MFMA on this path); peak = MI355X FP64 78.6 TFLOP/s; "
+                             "traffic = HBM bytes/launch from rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a copy kernel",
+                     "kernel_us": {kk: kernel_ms[kk] * 1e3 for kk in kernels},
+                     "kernel_flops": flops, "kernel_algorithmic_bytes": bytes_,
+                     "kernel_fp64_frac": {kk: flops[kk] / (kernel_ms[kk] / 1e3) / PEAK_FP64 for kk in kernels},
+                     "kernel_hbm_frac": {kk: bytes_[kk] / (kernel_ms[kk] / 1e3) / PEAK_HBM for kk in kernels}},
         "parity": {"points": points, "within_1e-8": good_points, "max_rel": float(rel.max()),
                    "median_rel": float(np.median(rel))},
     }

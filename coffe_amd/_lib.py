@@ -67,6 +67,10 @@ def load_library():
     L.coffe_batch_tables_stage.argtypes = [vp, vp, vp, vp, vp]
     L.coffe_batch_project.argtypes = [vp, vp, vp, vp, vp]
     L.coffe_batch_prepare_adj.argtypes = [vp, vp, vp, vp, vp]
+    L.coffe_batch_resample.argtypes = [vp, vp, vp, vp]
+    L.coffe_batch_fft_prepare.argtypes = [vp, vp]
+    L.coffe_batch_adj_c2r.argtypes = [vp, vp]
+    L.coffe_batch_spline_adj.argtypes = [vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -77,6 +81,7 @@ EXPORTED = [
     "coffe_batch_create", "coffe_batch_destroy", "coffe_batch_input_size", "coffe_batch_forward",
     "coffe_batch_vjp", "coffe_batch_tables", "coffe_batch_prepare", "coffe_batch_tables_stage",
     "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute",
+    "coffe_batch_resample", "coffe_batch_fft_prepare", "coffe_batch_adj_c2r", "coffe_batch_spline_adj",
 ]
 
 

@@ -499,6 +499,41 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
 
 static hipStream_t pick_stream(coffe_batch *b, void *stream) { return stream ? (hipStream_t)stream : b->e->stream; }
 
+int coffe_batch_resample(coffe_batch *b, const double *d_k, const double *d_pk, void *stream)
+{
+    if (!b || !d_k || !d_pk) return COFFE_EINVAL;
+    return ce_launch_resample(pick_stream(b, stream), b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, d_pk, b->d_ws)
+               ? COFFE_EDEVICE
+               : COFFE_OK;
+}
+
+int coffe_batch_fft_prepare(coffe_batch *b, void *stream)
+{
+    if (!b) return COFFE_EINVAL;
+    return ce_launch_prepare(pick_stream(b, stream), b->nspec, b->d_spec, b->d_grids, b->nq, b->d_ws, b->d_xf,
+                             b->threads, b->max_M)
+               ? COFFE_EDEVICE
+               : COFFE_OK;
+}
+
+int coffe_batch_adj_c2r(coffe_batch *b, void *stream)
+{
+    if (!b) return COFFE_EINVAL;
+    return ce_launch_adj_c2r(pick_stream(b, stream), b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq,
+                             b->d_ws, b->d_xb, b->threads, b->max_M)
+               ? COFFE_EDEVICE
+               : COFFE_OK;
+}
+
+int coffe_batch_spline_adj(coffe_batch *b, const double *d_k, double *d_gpk, void *stream)
+{
+    if (!b || !d_k || !d_gpk) return COFFE_EINVAL;
+    return ce_launch_spline_adj(pick_stream(b, stream), b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, b->d_ws,
+                                d_gpk)
+               ? COFFE_EDEVICE
+               : COFFE_OK;
+}
+
 int coffe_batch_prepare(coffe_batch *b, const double *d_k, const double *d_pk, void *stream)
 {
     if (!b || !d_k || !d_pk) return COFFE_EINVAL;

@@ -120,6 +120,19 @@ class Batch:
         check(self.L.coffe_batch_prepare_adj(self.h, _ptr(k), _ptr(pk), _ptr(gpk), stream),
               "coffe_batch_prepare_adj")
 
+    # the single kernels behind the prepare stages
+    def kernel_resample(self, k, pk, stream):
+        check(self.L.coffe_batch_resample(self.h, _ptr(k), _ptr(pk), stream), "coffe_batch_resample")
+
+    def kernel_fft_prepare(self, stream):
+        check(self.L.coffe_batch_fft_prepare(self.h, stream), "coffe_batch_fft_prepare")
+
+    def kernel_adj_c2r(self, stream):
+        check(self.L.coffe_batch_adj_c2r(self.h, stream), "coffe_batch_adj_c2r")
+
+    def kernel_spline_adj(self, k, gpk, stream):
+        check(self.L.coffe_batch_spline_adj(self.h, _ptr(k), _ptr(gpk), stream), "coffe_batch_spline_adj")
+
     def tables_full(self, k, pk, stream=None):
         import torch
         ntab = len(self.tables)

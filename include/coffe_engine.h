@@ -122,6 +122,12 @@ int coffe_batch_prepare(coffe_batch *b, const double *d_k, const double *d_pk, v
 int coffe_batch_tables_stage(coffe_batch *b, const double *d_A, const double *d_gout, double *d_itab, void *stream);
 int coffe_batch_project(coffe_batch *b, const double *d_A, const double *d_itab, double *d_out, void *stream);
 int coffe_batch_prepare_adj(coffe_batch *b, const double *d_k, const double *d_pk, double *d_gpk, void *stream);
+/* the single kernels behind prepare (resample, then fft_prepare) and prepare_adj
+   (adj_c2r, then spline_adj), same ordering rules */
+int coffe_batch_resample(coffe_batch *b, const double *d_k, const double *d_pk, void *stream);
+int coffe_batch_fft_prepare(coffe_batch *b, void *stream);
+int coffe_batch_adj_c2r(coffe_batch *b, void *stream);
+int coffe_batch_spline_adj(coffe_batch *b, const double *d_k, double *d_gpk, void *stream);
 
 /* synchronise the engine's stream */
 int coffe_engine_synchronize(coffe_engine *e);
