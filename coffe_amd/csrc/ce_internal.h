@@ -17,8 +17,8 @@ extern "C" {
 #define CE_MIN_SEP_LEN 42
 /* input intervals per spline chunk and LDS nodes staged per chunk (chunk + 2 halos
    of 64 + end nodes) */
-#define CE_CHUNK 1024
-#define CE_CHUNK_NODES 1160
+#define CE_CHUNK 512
+#define CE_CHUNK_NODES 648
 
 /* FFT radix schedule for a complex transform of length M = N/2 */
 typedef struct ce_radix_plan {

@@ -26,6 +26,8 @@
 //   k_spline_adj   per chunk: transposed resample and spline -> dP.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "ce_internal.h"
 
 namespace {
@@ -168,8 +170,14 @@ __device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restr
         if (j < B) {
             const int jm = j % Ns;
             if (jm != 0) {
+                // w_r = W^(r jm stride): one table lookup, the rest by binary splitting
+                // w_r = w_{r/2} w_{r - r/2} (at most three roundings deep for R <= 8)
+                cd w[R];
+                w[1] = twiddle(lt, jm * stride);
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], twiddle(lt, r * jm * stride));
+                for (int r = 2; r < R; ++r) w[r] = cmul(w[r >> 1], w[r - (r >> 1)]);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r]);
             }
             dft<R>(v[b]);
             const int base = (j - jm) * R + jm;
@@ -725,8 +733,9 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     }
     const int32_t *sidx = a.sidx + (int64_t)sp.grid * a.nsep;
     const bool adj = a.gout != nullptr;
-    double *wb = a.wbuf + (int64_t)blockIdx.x * a.nsep * 6;
-    for (int r = tid; r < a.nsep; r += T) {
+    // Akima at separation r; in adjoint mode also the cotangents of the six stencil
+    // values of the c2r output (value derivative * cotangent of I_t(r) * output scale)
+    auto akima_at = [&](int r, double *w) -> int {
         const int j = sidx[r];
         double X[6], Y[6], dY[6];
 #pragma unroll
@@ -743,17 +752,48 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
                     ib += a.A[(((int64_t)z * a.nl + l) * a.ntab + u.tab) * a.nsep + r] *
                           a.gout[(((int64_t)u.spec * a.nz + z) * a.nl + l) * a.nsep + r];
 #pragma unroll
-            for (int m = 0; m < 6; ++m) wb[r * 6 + m] = ib * dY[m] * os[j - 2 + m];
+            for (int m = 0; m < 6; ++m) w[m] = ib * dY[m] * os[j - 2 + m];
+        }
+        return j;
+    };
+    // up to two separations per thread keep their cotangents in registers across the
+    // LDS clear; more separations go through the global scratch wbuf
+    const bool inreg = a.nsep <= 2 * T;
+    double w0[6], w1[6];
+    int j0 = -1, j1 = -1;
+    double *wb = a.wbuf + (int64_t)blockIdx.x * a.nsep * 6;
+    if (inreg) {
+        if (tid < a.nsep) j0 = akima_at(tid, w0);
+        if (tid + T < a.nsep) j1 = akima_at(tid + T, w1);
+    } else {
+        for (int r = tid; r < a.nsep; r += T) {
+            double w[6];
+            akima_at(r, w);
+            if (adj) {
+#pragma unroll
+                for (int m = 0; m < 6; ++m) wb[r * 6 + m] = w[m];
+            }
         }
     }
     if (!adj) return;
     __syncthreads();
     for (int i = tid; i < N; i += T) lds_real(smem, i) = 0.0;
     __syncthreads();
-    for (int r = tid; r < a.nsep; r += T) {
-        const int j = sidx[r];
+    if (inreg) {
+        if (j0 >= 0) {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(smem, j - 2 + m), wb[r * 6 + m]);
+            for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(smem, j0 - 2 + m), w0[m]);
+        }
+        if (j1 >= 0) {
+#pragma unroll
+            for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(smem, j1 - 2 + m), w1[m]);
+        }
+    } else {
+        for (int r = tid; r < a.nsep; r += T) {
+            const int j = sidx[r];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(smem, j - 2 + m), wb[r * 6 + m]);
+        }
     }
     __syncthreads();
     // transposed c2r: Xbar_k = r2c(ybar)_k, doubled for 0 < k < M, real at k = 0, M
@@ -911,9 +951,22 @@ __global__ void __launch_bounds__(256) k_spline_adj(const ce_chunk_dev *__restri
 // ---------------------------------------------------------------------------------
 extern "C" {
 
+// raise a kernel's dynamic-LDS limit once per (kernel, size): the attribute call is a
+// host round trip, kept out of the per-step launch path
 static int set_lds(const void *fn, size_t bytes)
 {
+    static std::mutex mu;
+    static const void *fns[8] = {nullptr};
+    static size_t done[8] = {0};
+    std::lock_guard<std::mutex> g(mu);
+    int slot = -1;
+    for (int i = 0; i < 8; ++i) {
+        if (fns[i] == fn) { slot = i; break; }
+        if (!fns[i]) { fns[i] = fn; slot = i; break; }
+    }
+    if (slot >= 0 && done[slot] >= bytes) return 0;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess && slot >= 0) done[slot] = bytes;
     return e == hipSuccess ? 0 : (int)e;
 }
 
