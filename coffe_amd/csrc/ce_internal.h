@@ -95,6 +95,13 @@ int ce_launch_adj_c2r(void *stream, int nspec, const ce_spec_dev *d_spec, const 
 int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
                          const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk);
 
+/* ---- mu-quadrature weights (mu_kernels.hip) ---- */
+int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                         const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
+                         const double *bias_z, const double *bias1, const double *bias2, int den, int rsd, int d1,
+                         int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *slot,
+                         int nsep, const double *sep, int nmu, const double *mu, const double *wmu, double *A);
+
 #ifdef __cplusplus
 }
 #endif

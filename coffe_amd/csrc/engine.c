@@ -739,3 +739,117 @@ void twofast_1bessel(double *output_x, double *output_y, size_t output_len, doub
     ce_last_error = twofast_1bessel_status(output_x, output_y, output_len, input_x, input_y, input_len, l, nu, r0,
                                            k0, kmin, kmax);
 }
+
+/* ---- mu-quadrature weights ---------------------------------------------------- */
+
+/* Gauss-Legendre nodes/weights on [-1, 1] by Newton iteration on the Legendre
+   recurrence in long double */
+static void gauss_legendre(int n, double *x, double *w)
+{
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        long double z = cosl(3.14159265358979323846264338327950288L * ((long double)i + 0.75L) / ((long double)n + 0.5L));
+        long double pp = 1.0L;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1.0L, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / (long double)k;
+                p0 = p1;
+                p1 = p2;
+            }
+            pp = (long double)n * (z * p1 - p0) / (z * z - 1.0L);
+            const long double dz = p1 / pp;
+            z -= dz;
+            if (fabsl(dz) < 1e-19L) break;
+        }
+        long double p0 = 1.0L, p1 = z;
+        for (int k = 2; k <= n; ++k) {
+            const long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / (long double)k;
+            p0 = p1;
+            p1 = p2;
+        }
+        pp = (long double)n * (z * p1 - p0) / (z * z - 1.0L);
+        const long double ww = 2.0L / ((1.0L - z * z) * pp * pp);
+        x[i] = -(double)z;
+        x[n - 1 - i] = (double)z;
+        w[i] = (double)ww;
+        w[n - 1 - i] = (double)ww;
+    }
+    if (n & 1) x[n / 2] = 0.0;
+}
+
+static int std_slot(int n, int l)
+{
+    static const int tabs[8][2] = {{0, 0}, {0, 2}, {0, 4}, {1, 1}, {1, 3}, {2, 0}, {2, 2}, {3, 1}};
+    for (int s = 0; s < 8; ++s)
+        if (tabs[s][0] == n && tabs[s][1] == l) return s;
+    return -1;
+}
+
+int coffe_mu_weights(coffe_engine *e, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                     const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
+                     const double *bias_z, const double *bias1, const double *bias2, int den, int rsd, int d1,
+                     int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
+                     const int *tab_l, int nsep, const double *sep, int nmu, double *d_A, void *stream)
+{
+    if (!e || nbg < 5 || nb < 5 || nz <= 0 || nl <= 0 || nl > 4 || ntab <= 0 || ntab > CE_MAX_TABLES || nsep <= 0 ||
+        nmu < 2 || !d_A || !bg_z || !bg_chi || !bg_D1 || !bg_f || !bg_cH || !bg_G1 || !bg_G2 || !bias_z || !bias1 ||
+        !bias2 || !zmean || !ls || !tab_n || !tab_l || !sep)
+        return COFFE_EINVAL;
+    int slot[CE_MAX_TABLES], have[8] = {0};
+    for (int t = 0; t < ntab; ++t) {
+        slot[t] = std_slot(tab_n[t], tab_l[t]);
+        if (slot[t] < 0) return COFFE_EINVAL;
+        have[slot[t]] = 1;
+    }
+    /* the tables each contribution needs (functions.c:129-1143) */
+    if (den && !have[0]) return COFFE_EINVAL;
+    if (rsd && !(have[0] && have[1] && have[2])) return COFFE_EINVAL;
+    if (d1 && !(have[5] && have[6])) return COFFE_EINVAL;
+    if (den && d1 && !have[3]) return COFFE_EINVAL;
+    if (rsd && d1 && !(have[3] && have[4])) return COFFE_EINVAL;
+    for (int i = 0; i < nl; ++i)
+        if (ls[i] < 0) return COFFE_EINVAL;
+    /* chi(z_mean) -/+ sep/2 must stay inside the chi table at every |mu| <= 1 and
+       z_mean inside the bias table (the reference stops on a GSL domain error) */
+    for (int iz = 0; iz < nz; ++iz) {
+        double chim;
+        if (zmean[iz] < bg_z[0] || zmean[iz] > bg_z[nbg - 1] || zmean[iz] < bias_z[0] || zmean[iz] > bias_z[nb - 1])
+            return COFFE_EDOMAIN;
+        if (coffe_akima_eval((size_t)nbg, bg_z, bg_chi, 1, &zmean[iz], &chim) != COFFE_OK) return COFFE_EINVAL;
+        for (int r = 0; r < nsep; ++r)
+            if (!(sep[r] > 0.0) || chim - sep[r] / 2 < bg_chi[0] || chim + sep[r] / 2 > bg_chi[nbg - 1])
+                return COFFE_EDOMAIN;
+    }
+    hipSetDevice(e->device);
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    const size_t nbytes_bg = sizeof(double) * (size_t)nbg, nbytes_b = sizeof(double) * (size_t)nb;
+    const size_t total = 7 * nbytes_bg + 3 * nbytes_b + sizeof(double) * ((size_t)nz + (size_t)nsep + 2 * (size_t)nmu);
+    double *d = NULL;
+    if (dev_alloc((void **)&d, total) != COFFE_OK) return COFFE_ENOMEM;
+    double *h_mu = (double *)malloc(sizeof(double) * 2 * (size_t)nmu);
+    if (!h_mu) { hipFree(d); return COFFE_ENOMEM; }
+    gauss_legendre(nmu, h_mu, h_mu + nmu);
+    double *p = d;
+    const double *src[10] = {bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, bias_z, bias1, bias2};
+    double *dst[10];
+    int st = COFFE_OK;
+    for (int i = 0; i < 10 && st == COFFE_OK; ++i) {
+        const size_t nbyt = i < 7 ? nbytes_bg : nbytes_b;
+        dst[i] = p;
+        st = hip_ok(hipMemcpyAsync(p, src[i], nbyt, hipMemcpyHostToDevice, s));
+        p += nbyt / sizeof(double);
+    }
+    double *d_z = p, *d_sep = p + nz, *d_mu = d_sep + nsep, *d_w = d_mu + nmu;
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_z, zmean, sizeof(double) * (size_t)nz, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_sep, sep, sizeof(double) * (size_t)nsep, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_mu, h_mu, sizeof(double) * 2 * (size_t)nmu, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK &&
+        ce_launch_mu_weights(s, nbg, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], nb, dst[7], dst[8], dst[9],
+                             den, rsd, d1, flatsky, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu, d_mu, d_w, d_A))
+        st = COFFE_EDEVICE;
+    /* the staging buffer is released after the kernel has read it */
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    free(h_mu);
+    hipFree(d);
+    return st;
+}

@@ -169,3 +169,26 @@ def akima(x, y, xq):
     check(L.coffe_akima_eval(len(x), x.ctypes.data_as(dp), y.ctypes.data_as(dp), len(xq), xq.ctypes.data_as(dp),
                              out.ctypes.data_as(dp)), "coffe_akima_eval")
     return out
+
+
+def mu_weights(engine, bg, bias_z, bias1, bias2, zmean, ls, tables, sep, den=True, rsd=True, d1=False,
+               flatsky=False, nmu=256, stream=None):
+    """Spectrum-independent mu-quadrature weights A[z][l][t][r] (coffe_mu_weights), so that
+    multipoles = sum_t A I_t(sep).  bg: dict of host arrays z, chi, D1, f, cH (conformal Hubble rate), G1, G2."""
+    import torch
+    L = engine.L
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    arrs = [f64(bg[k]) for k in ("z", "chi", "D1", "f", "cH", "G1", "G2")]
+    bz, b1, b2 = f64(bias_z), f64(bias1), f64(bias2)
+    zm, sp = f64(np.atleast_1d(zmean)), f64(np.atleast_1d(sep))
+    dp = ctypes.POINTER(ctypes.c_double)
+    ls_c = (ctypes.c_int * len(ls))(*ls)
+    tn = (ctypes.c_int * len(tables))(*[t[0] for t in tables])
+    tl = (ctypes.c_int * len(tables))(*[t[1] for t in tables])
+    A = torch.empty((len(zm), len(ls), len(tables), len(sp)), dtype=torch.float64, device="cuda")
+    st = stream if stream is not None else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(L.coffe_mu_weights(engine.h, len(arrs[0]), *[a.ctypes.data_as(dp) for a in arrs], len(bz),
+                             bz.ctypes.data_as(dp), b1.ctypes.data_as(dp), b2.ctypes.data_as(dp), int(den), int(rsd),
+                             int(d1), int(flatsky), len(zm), zm.ctypes.data_as(dp), len(ls), ls_c, len(tables), tn, tl,
+                             len(sp), sp.ctypes.data_as(dp), int(nmu), _ptr(A), st), "coffe_mu_weights")
+    return A

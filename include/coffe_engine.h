@@ -129,6 +129,24 @@ int coffe_batch_fft_prepare(coffe_batch *b, void *stream);
 int coffe_batch_adj_c2r(coffe_batch *b, void *stream);
 int coffe_batch_spline_adj(coffe_batch *b, const double *d_k, double *d_gpk, void *stream);
 
+/*
+ * mu-quadrature weights of the local multipoles (functions_nonintegrated,
+ * functions.c:89-1484, integrated as in signal.c:83-111, 521-542):
+ *   A[z][l][t][r] = (2l+1)/2 sum_q w_q P_l(mu_q) G c_t(z_mean[z], mu_q, sep[r])
+ * with an nmu-node Gauss-Legendre rule, so that multipoles = sum_t A I_t(sep) (the
+ * project stage).  Background tables (length nbg, on the z grid bg_z; chi is
+ * also used inverted as z(chi)) and the galaxy-bias tables (nb samples on bias_z)
+ * are host arrays, interpolated with Akima as the reference does.  Contributions:
+ * den, rsd, d1 (0/1), flatsky selects the flat-sky forms.  Tables t are (tab_n,
+ * tab_l) among the 8 defaults and must include every table the contributions use.
+ * d_A is device memory [nz][nl][ntab][nsep]; nl <= 4.
+ */
+int coffe_mu_weights(coffe_engine *e, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                     const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
+                     const double *bias_z, const double *bias1, const double *bias2, int den, int rsd, int d1,
+                     int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
+                     const int *tab_l, int nsep, const double *sep, int nmu, double *d_A, void *stream);
+
 /* synchronise the engine's stream */
 int coffe_engine_synchronize(coffe_engine *e);
 
