@@ -1,0 +1,135 @@
+"""Differentiable multipoles: torch.autograd wrappers over the batched engine.
+
+The multipole of spectrum s at (z, l, r) is linear in the tables and in the weights,
+    M[s, z, l, r] = sum_t A[z, l, t, r] I_t[s, r](P_s),
+and the tables are a (nonlinear, through Akima) function of the P(k) samples.  The
+gradient with respect to P is the engine's adjoint (coffe_batch_vjp); the gradient
+with respect to the weights is sum_s gM[s] I[s].
+
+Derivative parameters (the reference has no such API; its galaxy bias and growth
+rate enter the local terms of functions.c:129-1143 as):
+- b: a constant galaxy bias of both tracers.  den-den ~ b^2, den-rsd and den-d1 ~ b.
+- f: a constant growth rate overriding f(z) in every term.  rsd-rsd, d1-d1 and
+  rsd-d1 ~ f^2; den-rsd and den-d1 ~ f.
+- sigma8: the input spectrum is rescaled to P sigma8^2 / sigma8(P)^2, with
+  sigma8(P)^2 = 1/(2 pi^2) int dk k^2 P(k) W(k R)^2, W(x) = 3 (sin x - x cos x) / x^3,
+  R = 8 Mpc/h (trapezoid in ln k over the input samples).  The reference applies no
+  sigma8 to a user-supplied P(k) (coffe.pyx:2929-2996).
+So the weights are the polynomial A = b^2 A_bb + b f A_bf + f^2 A_ff, whose three
+groups the engine computes once (coffe_mu_weights_terms); autograd differentiates
+the polynomial.
+"""
+import numpy as np
+import torch
+
+from .engine import (TERM_D1_D1, TERM_DEN_D1, TERM_DEN_DEN, TERM_DEN_RSD, TERM_RSD_D1, TERM_RSD_RSD, mu_weights)
+
+GROUP_BB = TERM_DEN_DEN
+GROUP_BF = TERM_DEN_RSD | TERM_DEN_D1
+GROUP_FF = TERM_RSD_RSD | TERM_D1_D1 | TERM_RSD_D1
+
+
+class MultipolesFunction(torch.autograd.Function):
+    """M = batch(P, A) with gradients for P (adjoint kernels) and A (sum over spectra)."""
+
+    @staticmethod
+    def forward(ctx, pk, A, batch, k):
+        itab, out = batch.forward(k, pk, A)
+        ctx.save_for_backward(pk, A, itab)
+        ctx.batch, ctx.k = batch, k
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        pk, A, itab = ctx.saved_tensors
+        gout = gout.contiguous()
+        gpk = gA = None
+        if ctx.needs_input_grad[0]:
+            _, _, gpk = ctx.batch.vjp(ctx.k, pk, A, gout)
+        if ctx.needs_input_grad[1]:
+            gA = torch.einsum("szlr,str->zltr", gout, itab)
+        return gpk, gA, None, None
+
+
+def multipoles(batch, k, pk, A):
+    """Differentiable batched multipoles (nspec, nz, nl, nsep) of the concatenated spectra."""
+    return MultipolesFunction.apply(pk, A, batch, k)
+
+
+def sigma8_of(k, pk, offsets, h):
+    """sigma8 of each concatenated spectrum (k in 1/Mpc, P in Mpc^3), as a torch tensor."""
+    R = 8.0 / h
+    out = []
+    for s in range(len(offsets) - 1):
+        ks, ps = k[offsets[s]:offsets[s + 1]], pk[offsets[s]:offsets[s + 1]]
+        x = ks * R
+        w = 3 * (torch.sin(x) - x * torch.cos(x)) / x ** 3
+        y = ks ** 3 * ps * w * w / (2 * np.pi ** 2)
+        out.append(torch.sqrt(torch.trapezoid(y, torch.log(ks))))
+    return torch.stack(out)
+
+
+def rescale_sigma8(k, pk, offsets, h, sigma8):
+    """P sigma8^2 / sigma8(P)^2 per spectrum; sigma8 is a scalar or a (nspec,) tensor."""
+    s8 = sigma8_of(k, pk, offsets, h)
+    scale = (torch.as_tensor(sigma8, dtype=pk.dtype, device=pk.device) / s8) ** 2
+    counts = torch.as_tensor(np.diff(offsets), device=pk.device)
+    return pk * torch.repeat_interleave(scale.expand(len(counts)), counts)
+
+
+class LocalWeights:
+    """The three weight groups (b^2, b f, f^2) of the local terms for one geometry.
+
+    bg: background tables (z, chi, D1, f, cH, G1, G2).  bias="constant" sets both
+    galaxy biases to 1 in the groups and scales them by b in weights(); bias=(bias_z,
+    bias1, bias2) keeps those tables and weights() takes no b.  With f_override the
+    growth rate is set to 1 in every term and scaled by f in weights().
+    """
+
+    def __init__(self, engine, bg, zmean, ls, tables, sep, den=True, rsd=True, d1=False, flatsky=False, nmu=256,
+                 bias="constant", f_override=False):
+        self.tables = list(tables)
+        self.constant_bias = bias is not None
+        self.f_override = bool(f_override)
+        bgw = dict(bg)
+        if f_override:
+            bgw["f"] = np.ones_like(np.asarray(bg["f"]))
+        zb = np.linspace(float(np.min(bg["z"])), float(np.max(bg["z"])), 11)
+        if isinstance(bias, str):
+            if bias != "constant":
+                raise ValueError(f"unknown bias mode {bias!r}")
+            bz, b1, b2 = zb, np.ones_like(zb), np.ones_like(zb)
+        else:
+            bz, b1, b2 = bias
+            self.constant_bias = False
+        mask = 0
+        mask |= TERM_DEN_DEN if den else 0
+        mask |= TERM_RSD_RSD if rsd else 0
+        mask |= TERM_D1_D1 if d1 else 0
+        mask |= TERM_DEN_RSD if den and rsd else 0
+        mask |= TERM_DEN_D1 if den and d1 else 0
+        mask |= TERM_RSD_D1 if rsd and d1 else 0
+        self.groups = {}
+        for name, g in (("bb", GROUP_BB), ("bf", GROUP_BF), ("ff", GROUP_FF)):
+            if mask & g:
+                self.groups[name] = mu_weights(engine, bgw, bz, b1, b2, zmean, ls, self.tables, sep,
+                                               flatsky=flatsky, nmu=nmu, terms=mask & g)
+
+    def weights(self, b=None, f=None):
+        """A(b, f) = b^2 A_bb + b f A_bf + f^2 A_ff as a differentiable torch expression."""
+        if self.constant_bias and b is None:
+            raise ValueError("constant-bias weights need b")
+        if not self.constant_bias and b is not None:
+            raise ValueError("b applies to constant-bias weights only")
+        if self.f_override and f is None:
+            raise ValueError("f-override weights need f")
+        if not self.f_override and f is not None:
+            raise ValueError("f applies to f-override weights only")
+        b = 1.0 if b is None else b
+        f = 1.0 if f is None else f
+        A = None
+        for name, coef in (("bb", lambda: b * b), ("bf", lambda: b * f), ("ff", lambda: f * f)):
+            if name in self.groups:
+                term = coef() * self.groups[name]
+                A = term if A is None else A + term
+        return A

@@ -14,13 +14,13 @@ ARCH = os.environ.get("COFFE_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["fftlog_kernels.hip", "mu_kernels.hip"]
 C_SOURCES = ["engine.c", "background.c", "interp.c"]
 
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", f"-I{INCLUDE}",
     # keep packed-FP32 instructions out of every code object
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
 ]
-INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 CC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
             f"-I{ROCM}/include", f"-I{INCLUDE}", "-fno-fast-math"]
 

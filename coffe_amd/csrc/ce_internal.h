@@ -98,7 +98,7 @@ int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks,
 /* ---- mu-quadrature weights (mu_kernels.hip) ---- */
 int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
                          const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
-                         const double *bias_z, const double *bias1, const double *bias2, int den, int rsd, int d1,
+                         const double *bias_z, const double *bias1, const double *bias2, unsigned terms,
                          int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *slot,
                          int nsep, const double *sep, int nmu, const double *mu, const double *wmu, double *A);
 

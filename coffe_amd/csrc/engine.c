@@ -785,14 +785,38 @@ static int std_slot(int n, int l)
     return -1;
 }
 
+unsigned coffe_terms_of(int den, int rsd, int d1)
+{
+    unsigned t = 0;
+    if (den) t |= COFFE_TERM_DEN_DEN;
+    if (rsd) t |= COFFE_TERM_RSD_RSD;
+    if (d1) t |= COFFE_TERM_D1_D1;
+    if (den && rsd) t |= COFFE_TERM_DEN_RSD;
+    if (den && d1) t |= COFFE_TERM_DEN_D1;
+    if (rsd && d1) t |= COFFE_TERM_RSD_D1;
+    return t;
+}
+
 int coffe_mu_weights(coffe_engine *e, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
                      const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
                      const double *bias_z, const double *bias1, const double *bias2, int den, int rsd, int d1,
                      int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
                      const int *tab_l, int nsep, const double *sep, int nmu, double *d_A, void *stream)
 {
+    return coffe_mu_weights_terms(e, nbg, bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, nb, bias_z, bias1, bias2,
+                                  coffe_terms_of(den, rsd, d1), flatsky, nz, zmean, nl, ls, ntab, tab_n, tab_l, nsep,
+                                  sep, nmu, d_A, stream);
+}
+
+int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                           const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
+                           const double *bias_z, const double *bias1, const double *bias2, unsigned terms,
+                           int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab,
+                           const int *tab_n, const int *tab_l, int nsep, const double *sep, int nmu, double *d_A,
+                           void *stream)
+{
     if (!e || nbg < 5 || nb < 5 || nz <= 0 || nl <= 0 || nl > 4 || ntab <= 0 || ntab > CE_MAX_TABLES || nsep <= 0 ||
-        nmu < 2 || !d_A || !bg_z || !bg_chi || !bg_D1 || !bg_f || !bg_cH || !bg_G1 || !bg_G2 || !bias_z || !bias1 ||
+        nmu < 2 || terms == 0 || terms > 63 || !d_A || !bg_z || !bg_chi || !bg_D1 || !bg_f || !bg_cH || !bg_G1 || !bg_G2 || !bias_z || !bias1 ||
         !bias2 || !zmean || !ls || !tab_n || !tab_l || !sep)
         return COFFE_EINVAL;
     int slot[CE_MAX_TABLES], have[8] = {0};
@@ -802,11 +826,12 @@ int coffe_mu_weights(coffe_engine *e, int nbg, const double *bg_z, const double 
         have[slot[t]] = 1;
     }
     /* the tables each contribution needs (functions.c:129-1143) */
-    if (den && !have[0]) return COFFE_EINVAL;
-    if (rsd && !(have[0] && have[1] && have[2])) return COFFE_EINVAL;
-    if (d1 && !(have[5] && have[6])) return COFFE_EINVAL;
-    if (den && d1 && !have[3]) return COFFE_EINVAL;
-    if (rsd && d1 && !(have[3] && have[4])) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_DEN_DEN) && !have[0]) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_RSD_RSD) && !(have[0] && have[1] && have[2])) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_D1_D1) && !(have[5] && have[6])) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_DEN_RSD) && !(have[0] && have[1])) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_DEN_D1) && !have[3]) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_RSD_D1) && !(have[3] && have[4])) return COFFE_EINVAL;
     for (int i = 0; i < nl; ++i)
         if (ls[i] < 0) return COFFE_EINVAL;
     /* chi(z_mean) -/+ sep/2 must stay inside the chi table at every |mu| <= 1 and
@@ -845,7 +870,7 @@ int coffe_mu_weights(coffe_engine *e, int nbg, const double *bg_z, const double 
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_mu, h_mu, sizeof(double) * 2 * (size_t)nmu, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK &&
         ce_launch_mu_weights(s, nbg, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], nb, dst[7], dst[8], dst[9],
-                             den, rsd, d1, flatsky, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu, d_mu, d_w, d_A))
+                             terms, flatsky, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu, d_mu, d_w, d_A))
         st = COFFE_EDEVICE;
     /* the staging buffer is released after the kernel has read it */
     if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ce_internal.h"
+#include "coffe_engine.h"
 
 namespace {
 
@@ -84,7 +85,8 @@ struct mu_args {
     const double *bg_z, *bg_chi, *bg_D1, *bg_f, *bg_cH, *bg_G1, *bg_G2;
     int nb;
     const double *bias_z, *bias1, *bias2;
-    int den, rsd, d1, flatsky;
+    unsigned terms;            // COFFE_TERM_* mask
+    int flatsky;
     int nz;
     const double *zmean;
     int nl;
@@ -116,11 +118,11 @@ __device__ void local_coefficients(const mu_args &a, double zm, double mu, doubl
     const double b1 = akima_general(a.bias_z, a.bias1, a.nb, z1), b2 = akima_general(a.bias_z, a.bias2, a.nb, z2);
     const double G1 = akima_general(a.bg_z, a.bg_G1, a.nbg, z1), G2 = akima_general(a.bg_z, a.bg_G2, a.nbg, z2);
     const double P2 = legendre_p(2, mu), P4 = legendre_p(4, mu);
-    if (a.den) {   // functions.c:129-229
+    if (a.terms & COFFE_TERM_DEN_DEN) {   // functions.c:129-229
         if (a.flatsky) c[S00] += bm1 * bm2;
         else c[S00] += b1 * b2;
     }
-    if (a.rsd) {   // functions.c:230-362
+    if (a.terms & COFFE_TERM_RSD_RSD) {   // functions.c:230-362
         if (a.flatsky) {
             c[S00] += fmean * fmean / 5.;
             c[S02] += -4 * fmean * fmean / 7. * P2;
@@ -134,7 +136,7 @@ __device__ void local_coefficients(const mu_args &a, double zm, double mu, doubl
                            35. / pow(sep, 4));
         }
     }
-    if (a.d1) {   // functions.c:363-573
+    if (a.terms & COFFE_TERM_D1_D1) {   // functions.c:363-573
         if (a.flatsky) {
             c[S20] += Hmean * Hmean * fmean * fmean * Gm1 * Gm2 / 3.;
             c[S22] += 2 * Hmean * Hmean * fmean * fmean * Gm1 * Gm2 * pow(sep, 2) * P2 / 3.;
@@ -144,7 +146,7 @@ __device__ void local_coefficients(const mu_args &a, double zm, double mu, doubl
                       ((chi2 - chi1 * costheta) * (chi1 - chi2 * costheta) + pow(sep, 2) * costheta / 3.);
         }
     }
-    if (a.den && a.rsd) {   // functions.c:670-771
+    if (a.terms & COFFE_TERM_DEN_RSD) {   // functions.c:670-771
         if (a.flatsky) {
             c[S00] += bm1 * fmean / 3. + bm2 * fmean / 3.;
             c[S02] += -(2 * bm1 * fmean / 3. + 2 * bm2 * fmean / 3.) * P2;
@@ -154,14 +156,14 @@ __device__ void local_coefficients(const mu_args &a, double zm, double mu, doubl
                         b2 * f1 * (2. / 3. - (1. - ct2) * pow(chi2 / sep, 2)));
         }
     }
-    if (a.den && a.d1) {   // functions.c:772-855
+    if (a.terms & COFFE_TERM_DEN_D1) {   // functions.c:772-855
         if (a.flatsky) {
             c[S11] += (bm1 * fmean * Hmean * Gm2 * mu + bm2 * fmean * Hmean * Gm1 * (-mu)) * sep;
         } else {
             c[S11] += -(b1 * f2 * H2 * G2 * (chi1 * costheta - chi2) + b2 * f1 * H1 * G1 * (chi2 * costheta - chi1));
         }
     }
-    if (a.rsd && a.d1) {   // functions.c:904-1143
+    if (a.terms & COFFE_TERM_RSD_D1) {   // functions.c:904-1143
         if (a.flatsky) {
             const double P3p = legendre_p(3, mu), P3m = legendre_p(3, -mu);
             c[S11] += -3. / 5. * (fmean * fmean * Hmean * Gm2 * (-mu) + fmean * fmean * Hmean * Gm1 * mu) * sep;
@@ -240,7 +242,7 @@ __global__ void __launch_bounds__(256) k_mu_weights(mu_args a)
 extern "C" int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, const double *bg_chi,
                                     const double *bg_D1, const double *bg_f, const double *bg_cH, const double *bg_G1,
                                     const double *bg_G2, int nb, const double *bias_z, const double *bias1,
-                                    const double *bias2, int den, int rsd, int d1, int flatsky, int nz,
+                                    const double *bias2, unsigned terms, int flatsky, int nz,
                                     const double *zmean, int nl, const int *ls, int ntab, const int *slot, int nsep,
                                     const double *sep, int nmu, const double *mu, const double *wmu, double *A)
 {
@@ -249,7 +251,7 @@ extern "C" int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, c
     a.nbg = nbg; a.bg_z = bg_z; a.bg_chi = bg_chi; a.bg_D1 = bg_D1; a.bg_f = bg_f; a.bg_cH = bg_cH;
     a.bg_G1 = bg_G1; a.bg_G2 = bg_G2;
     a.nb = nb; a.bias_z = bias_z; a.bias1 = bias1; a.bias2 = bias2;
-    a.den = den; a.rsd = rsd; a.d1 = d1; a.flatsky = flatsky;
+    a.terms = terms; a.flatsky = flatsky;
     a.nz = nz; a.zmean = zmean; a.nl = nl;
     for (int i = 0; i < nl; ++i) a.ls[i] = ls[i];
     a.ntab = ntab;

@@ -12,6 +12,19 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def _need(t, shape, name):
+    """A tensor handed to a kernel must be contiguous float64 on a GPU with exactly this shape."""
+    import torch
+    if t is None:
+        return
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor")
+    if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name}: needs a contiguous float64 CUDA tensor (got {t.dtype}, {t.device})")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)} != {tuple(shape)}")
+
+
 class Engine:
     """One engine per device (stream + cache of spectrum-independent grid tables)."""
 
@@ -63,6 +76,7 @@ class Batch:
         self.h = h
         self.L = L
         self.offsets = np.concatenate([[0], np.cumsum(self.n_in)])
+        self.n_total = int(self.offsets[-1])
 
     def close(self):
         if self.h:
@@ -74,6 +88,14 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def _check_io(self, k, pk, A, itab, out):
+        ntab, nsep = len(self.tables), len(self.sep)
+        _need(k, (self.n_total,), "k")
+        _need(pk, (self.n_total,), "pk")
+        _need(A, (self.nz, self.nl, ntab, nsep), "A")
+        _need(itab, (self.nspec, ntab, nsep), "itab")
+        _need(out, (self.nspec, self.nz, self.nl, nsep), "out")
 
     @staticmethod
     def _stream():
@@ -88,6 +110,7 @@ class Batch:
             itab = torch.empty((self.nspec, ntab, nsep), dtype=torch.float64, device=k.device)
         if A is not None and out is None:
             out = torch.empty((self.nspec, self.nz, self.nl, nsep), dtype=torch.float64, device=k.device)
+        self._check_io(k, pk, A, itab, out)
         check(self.L.coffe_batch_forward(self.h, _ptr(k), _ptr(pk), _ptr(A), _ptr(itab), _ptr(out),
                                          stream if stream is not None else self._stream()), "coffe_batch_forward")
         return itab, out
@@ -101,27 +124,38 @@ class Batch:
             out = torch.empty((self.nspec, self.nz, self.nl, nsep), dtype=torch.float64, device=k.device)
         if gpk is None:
             gpk = torch.empty_like(pk)
+        self._check_io(k, pk, A, itab, out)
+        _need(gout, (self.nspec, self.nz, self.nl, nsep), "gout")
+        _need(gpk, (self.n_total,), "gpk")
         check(self.L.coffe_batch_vjp(self.h, _ptr(k), _ptr(pk), _ptr(A), _ptr(gout), _ptr(itab), _ptr(out),
                                      _ptr(gpk), stream if stream is not None else self._stream()), "coffe_batch_vjp")
         return itab, out, gpk
 
     # stage-by-stage launches (same kernels as forward / vjp), for pipelining and timing
     def stage_prepare(self, k, pk, stream):
+        self._check_io(k, pk, None, None, None)
         check(self.L.coffe_batch_prepare(self.h, _ptr(k), _ptr(pk), stream), "coffe_batch_prepare")
 
     def stage_tables(self, A, gout, itab, stream):
+        ntab, nsep = len(self.tables), len(self.sep)
+        self._check_io(None, None, A, itab, None)
+        _need(gout, (self.nspec, self.nz, self.nl, nsep), "gout")
         check(self.L.coffe_batch_tables_stage(self.h, _ptr(A), _ptr(gout), _ptr(itab), stream),
               "coffe_batch_tables_stage")
 
     def stage_project(self, A, itab, out, stream):
+        self._check_io(None, None, A, itab, out)
         check(self.L.coffe_batch_project(self.h, _ptr(A), _ptr(itab), _ptr(out), stream), "coffe_batch_project")
 
     def stage_prepare_adj(self, k, pk, gpk, stream):
+        self._check_io(k, pk, None, None, None)
+        _need(gpk, (self.n_total,), "gpk")
         check(self.L.coffe_batch_prepare_adj(self.h, _ptr(k), _ptr(pk), _ptr(gpk), stream),
               "coffe_batch_prepare_adj")
 
     # the single kernels behind the prepare stages
     def kernel_resample(self, k, pk, stream):
+        self._check_io(k, pk, None, None, None)
         check(self.L.coffe_batch_resample(self.h, _ptr(k), _ptr(pk), stream), "coffe_batch_resample")
 
     def kernel_fft_prepare(self, stream):
@@ -131,12 +165,15 @@ class Batch:
         check(self.L.coffe_batch_adj_c2r(self.h, stream), "coffe_batch_adj_c2r")
 
     def kernel_spline_adj(self, k, gpk, stream):
+        self._check_io(k, None, None, None, None)
+        _need(gpk, (self.n_total,), "gpk")
         check(self.L.coffe_batch_spline_adj(self.h, _ptr(k), _ptr(gpk), stream), "coffe_batch_spline_adj")
 
     def tables_full(self, k, pk, stream=None):
         import torch
         ntab = len(self.tables)
         full = torch.zeros((self.nspec, ntab, int(self.n_fft.max())), dtype=torch.float64, device=k.device)
+        self._check_io(k, pk, None, None, None)
         check(self.L.coffe_batch_tables(self.h, _ptr(k), _ptr(pk), _ptr(full),
                                         stream if stream is not None else self._stream()), "coffe_batch_tables")
         return full
@@ -171,10 +208,26 @@ def akima(x, y, xq):
     return out
 
 
+TERM_DEN_DEN, TERM_RSD_RSD, TERM_D1_D1, TERM_DEN_RSD, TERM_DEN_D1, TERM_RSD_D1 = 1, 2, 4, 8, 16, 32
+
+
+def terms_of(den, rsd, d1):
+    """COFFE_TERM_* mask of the contribution flags (auto terms and cross terms)."""
+    t = 0
+    t |= TERM_DEN_DEN if den else 0
+    t |= TERM_RSD_RSD if rsd else 0
+    t |= TERM_D1_D1 if d1 else 0
+    t |= TERM_DEN_RSD if den and rsd else 0
+    t |= TERM_DEN_D1 if den and d1 else 0
+    t |= TERM_RSD_D1 if rsd and d1 else 0
+    return t
+
+
 def mu_weights(engine, bg, bias_z, bias1, bias2, zmean, ls, tables, sep, den=True, rsd=True, d1=False,
-               flatsky=False, nmu=256, stream=None):
-    """Spectrum-independent mu-quadrature weights A[z][l][t][r] (coffe_mu_weights), so that
-    multipoles = sum_t A I_t(sep).  bg: dict of host arrays z, chi, D1, f, cH (conformal Hubble rate), G1, G2."""
+               flatsky=False, nmu=256, terms=None, out=None, stream=None):
+    """Spectrum-independent mu-quadrature weights A[z][l][t][r] (coffe_mu_weights_terms), so that
+    multipoles = sum_t A I_t(sep).  bg: dict of host arrays z, chi, D1, f, cH (conformal Hubble rate),
+    G1, G2.  terms: COFFE_TERM_* mask, default from den/rsd/d1."""
     import torch
     L = engine.L
     f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
@@ -185,10 +238,14 @@ def mu_weights(engine, bg, bias_z, bias1, bias2, zmean, ls, tables, sep, den=Tru
     ls_c = (ctypes.c_int * len(ls))(*ls)
     tn = (ctypes.c_int * len(tables))(*[t[0] for t in tables])
     tl = (ctypes.c_int * len(tables))(*[t[1] for t in tables])
-    A = torch.empty((len(zm), len(ls), len(tables), len(sp)), dtype=torch.float64, device="cuda")
+    if terms is None:
+        terms = terms_of(den, rsd, d1)
+    A = out if out is not None else torch.empty((len(zm), len(ls), len(tables), len(sp)), dtype=torch.float64,
+                                                device="cuda")
+    _need(A, (len(zm), len(ls), len(tables), len(sp)), "A")
     st = stream if stream is not None else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    check(L.coffe_mu_weights(engine.h, len(arrs[0]), *[a.ctypes.data_as(dp) for a in arrs], len(bz),
-                             bz.ctypes.data_as(dp), b1.ctypes.data_as(dp), b2.ctypes.data_as(dp), int(den), int(rsd),
-                             int(d1), int(flatsky), len(zm), zm.ctypes.data_as(dp), len(ls), ls_c, len(tables), tn, tl,
-                             len(sp), sp.ctypes.data_as(dp), int(nmu), _ptr(A), st), "coffe_mu_weights")
+    check(L.coffe_mu_weights_terms(engine.h, len(arrs[0]), *[a.ctypes.data_as(dp) for a in arrs], len(bz),
+                                   bz.ctypes.data_as(dp), b1.ctypes.data_as(dp), b2.ctypes.data_as(dp), int(terms),
+                                   int(flatsky), len(zm), zm.ctypes.data_as(dp), len(ls), ls_c, len(tables), tn, tl,
+                                   len(sp), sp.ctypes.data_as(dp), int(nmu), _ptr(A), st), "coffe_mu_weights")
     return A

@@ -129,6 +129,21 @@ int coffe_batch_fft_prepare(coffe_batch *b, void *stream);
 int coffe_batch_adj_c2r(coffe_batch *b, void *stream);
 int coffe_batch_spline_adj(coffe_batch *b, const double *d_k, double *d_gpk, void *stream);
 
+/* terms of the local correlation function (functions.c:129-1143): auto terms and
+   the cross terms of density, RSD and Doppler-1 */
+enum {
+    COFFE_TERM_DEN_DEN = 1,
+    COFFE_TERM_RSD_RSD = 2,
+    COFFE_TERM_D1_D1 = 4,
+    COFFE_TERM_DEN_RSD = 8,
+    COFFE_TERM_DEN_D1 = 16,
+    COFFE_TERM_RSD_D1 = 32
+};
+
+/* the term mask of the reference's contribution flags: auto terms of each flag set
+   and cross terms of each pair set (functions.c:129-1143) */
+unsigned coffe_terms_of(int den, int rsd, int d1);
+
 /*
  * mu-quadrature weights of the local multipoles (functions_nonintegrated,
  * functions.c:89-1484, integrated as in signal.c:83-111, 521-542):
@@ -146,6 +161,16 @@ int coffe_mu_weights(coffe_engine *e, int nbg, const double *bg_z, const double 
                      const double *bias_z, const double *bias1, const double *bias2, int den, int rsd, int d1,
                      int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
                      const int *tab_l, int nsep, const double *sep, int nmu, double *d_A, void *stream);
+
+/* the same for an explicit COFFE_TERM_* mask: splitting the terms gives the weights'
+   polynomial dependence on a constant bias (den-den ~ b^2, den-rsd and den-d1 ~ b)
+   and on a constant growth rate (rsd-rsd, d1-d1, rsd-d1 ~ f^2; den-rsd, den-d1 ~ f) */
+int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                           const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
+                           const double *bias_z, const double *bias1, const double *bias2, unsigned terms,
+                           int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab,
+                           const int *tab_n, const int *tab_l, int nsep, const double *sep, int nmu, double *d_A,
+                           void *stream);
 
 /* synchronise the engine's stream */
 int coffe_engine_synchronize(coffe_engine *e);
