@@ -67,6 +67,8 @@ def load_library():
                                    ip, i, dp, i, vp, vp]
     L.coffe_mu_weights_terms.argtypes = [vp, i, dp, dp, dp, dp, dp, dp, dp, i, dp, dp, dp, ctypes.c_uint, i, i, dp, i,
                                          ip, i, ip, ip, i, dp, i, vp, vp]
+    L.coffe_integral_table.argtypes = [i, i, sz, dp, dp, sz, d, d, dp, dp]
+    L.coffe_batch_small_r.argtypes = [vp, vp, vp, vp, vp]
     L.coffe_terms_of.restype = ctypes.c_uint
     L.coffe_terms_of.argtypes = [i, i, i]
     L.coffe_batch_prepare.argtypes = [vp, vp, vp, vp]
@@ -88,7 +90,7 @@ EXPORTED = [
     "coffe_batch_vjp", "coffe_batch_tables", "coffe_batch_prepare", "coffe_batch_tables_stage",
     "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute",
     "coffe_batch_resample", "coffe_batch_fft_prepare", "coffe_batch_adj_c2r", "coffe_batch_spline_adj",
-    "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_terms_of",
+    "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_terms_of", "coffe_integral_table", "coffe_batch_small_r",
 ]
 
 

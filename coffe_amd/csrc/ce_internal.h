@@ -1,6 +1,6 @@
 /*
  * Internal interface between the host C layer (engine.c, dropin.c) and the HIP
- * kernels (fftlog_kernels.hip, quad_kernels.hip).  Plain C types only.
+ * kernels (fftlog_kernels.hip, mu_kernels.hip, quad_kernels.hip).  Plain C types only.
  */
 #ifndef COFFE_ENGINE_INTERNAL_H
 #define COFFE_ENGINE_INTERNAL_H
@@ -15,6 +15,9 @@ extern "C" {
 #define CE_MAX_PASSES 16
 #define CE_MAX_TABLES 8
 #define CE_MIN_SEP_LEN 42
+/* intervals an adaptive QAG keeps in LDS (the reference allows 50000; the small-r
+   integrands converge within a few hundred) */
+#define CE_QAG_CAP 1024
 /* input intervals per spline chunk and LDS nodes staged per chunk (chunk + 2 halos
    of 64 + end nodes) */
 #define CE_CHUNK 512
@@ -94,6 +97,11 @@ int ce_launch_adj_c2r(void *stream, int nspec, const ce_spec_dev *d_spec, const 
 
 int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
                          const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk);
+
+/* ---- small-separation table nodes (quad_kernels.hip): out/status [nspec][ntab][43] ---- */
+int ce_launch_small_r(void *stream, int nspec, int ntab, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
+                      const ce_table_desc *d_tabs, const double *d_k, const double *d_pk, double epsrel,
+                      double *d_out, int *d_status);
 
 /* ---- mu-quadrature weights (mu_kernels.hip) ---- */
 int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,

@@ -41,6 +41,7 @@ const char *coffe_engine_strerror(int code)
     case COFFE_EDOMAIN: return "argument outside the domain (window limits or separation outside the table)";
     case COFFE_ENONFINITE: return "non-finite value in input or kernel";
     case COFFE_ENODEVICE: return "no HIP device available";
+    case COFFE_ENOCONV: return "adaptive quadrature did not converge";
     default: return "unknown error";
     }
 }
@@ -312,6 +313,7 @@ struct coffe_batch {
     int32_t *d_sidx;
     double *d_sep;
     double *d_ws, *d_xf, *d_xb, *d_wbuf, *d_itab;
+    int *d_srstat;     /* status of the small-r integrals, allocated on first use */
 };
 
 int64_t coffe_batch_input_size(const coffe_batch *b) { return b ? b->total_in : 0; }
@@ -333,6 +335,7 @@ void coffe_batch_destroy(coffe_batch *b)
     hipFree(b->d_xb);
     hipFree(b->d_wbuf);
     hipFree(b->d_itab);
+    hipFree(b->d_srstat);
     free(b);
 }
 
@@ -877,4 +880,99 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
     free(h_mu);
     hipFree(d);
     return st;
+}
+
+/* ---- small-separation nodes --------------------------------------------------- */
+
+static int check_small_r_status(const int *h_stat, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        if (h_stat[i] == 3) return COFFE_ENONFINITE;
+    for (size_t i = 0; i < n; ++i)
+        if (h_stat[i] != 0) return COFFE_ENOCONV;
+    return COFFE_OK;
+}
+
+int coffe_batch_small_r(coffe_batch *b, const double *d_k, const double *d_pk, double *d_out, void *stream)
+{
+    if (!b || !d_k || !d_pk || !d_out) return COFFE_EINVAL;
+    hipSetDevice(b->e->device);
+    const size_t n = (size_t)b->nspec * (size_t)b->ntab * (1 + CE_MIN_SEP_LEN);
+    if (!b->d_srstat && dev_alloc((void **)&b->d_srstat, sizeof(int) * n) != COFFE_OK) return COFFE_ENOMEM;
+    hipStream_t s = stream ? (hipStream_t)stream : b->e->stream;
+    if (ce_launch_small_r(s, b->nspec, b->ntab, b->d_spec, b->d_grids, b->d_tabs, d_k, d_pk, 1e-5, d_out,
+                          b->d_srstat))
+        return COFFE_EDEVICE;
+    int *h = (int *)malloc(sizeof(int) * n);
+    if (!h) return COFFE_ENOMEM;
+    int st = hip_ok(hipMemcpyAsync(h, b->d_srstat, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    if (st == COFFE_OK) st = check_small_r_status(h, n);
+    free(h);
+    return st;
+}
+
+int coffe_integral_table(int n, int l, size_t N, const double *k, const double *pk, size_t nin, double kmin,
+                         double kmax, double *out_x, double *out_y)
+{
+    const size_t nodes = 1 + CE_MIN_SEP_LEN;
+    if (!k || !pk || !out_x || !out_y || nin < 5 || n < -2 || n > 4 || l < 0 || l > 4) return COFFE_EINVAL;
+    if (!(kmin >= k[0]) || !(kmax <= k[nin - 1]) || !(kmin < kmax)) return COFFE_EDOMAIN;
+    /* FFT nodes (integrals.c:496-522) */
+    int st = twofast_1bessel_status(out_x + nodes, out_y + nodes, N, k, pk, nin, (double)l, (double)n, 1.0, kmin,
+                                    kmin, kmax);
+    if (st != COFFE_OK) return st;
+    if (n >= l)
+        for (size_t i = 0; i < N; ++i) out_y[nodes + i] *= pow(out_x[nodes + i], (double)(n - l));
+    /* small-r nodes (integrals.c:524-557) */
+    coffe_engine *e = coffe_engine_default();
+    if (!e) return COFFE_ENODEVICE;
+    hipSetDevice(e->device);
+    ce_spec_dev hs;
+    memset(&hs, 0, sizeof(hs));
+    hs.n_in = (int32_t)nin;
+    ce_grid_dev hg;
+    memset(&hg, 0, sizeof(hg));
+    hg.kmin = kmin;
+    hg.kmax = kmax;
+    ce_table_desc td;
+    memset(&td, 0, sizeof(td));
+    td.n = n;
+    td.l = l;
+    const size_t off_out = 2 * nin, off_meta = off_out + nodes;
+    const size_t meta = sizeof(hs) + sizeof(hg) + sizeof(td) + sizeof(int) * nodes + 64;
+    pthread_mutex_lock(&e->lock);
+    st = scratch_reserve(e, sizeof(double) * off_meta + meta);
+    if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); return st; }
+    double *base = (double *)e->d_scratch;
+    char *m = (char *)(base + off_meta);
+    ce_spec_dev *d_spec = (ce_spec_dev *)m;
+    ce_grid_dev *d_grid = (ce_grid_dev *)(m + sizeof(hs));
+    ce_table_desc *d_tab = (ce_table_desc *)((char *)d_grid + sizeof(hg));
+    int *d_stat = (int *)((char *)d_tab + sizeof(td));
+    int h_stat[1 + CE_MIN_SEP_LEN];
+    hipStream_t s = e->stream;
+    st = hip_ok(hipMemcpyAsync(base, k, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(base + nin, pk, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_spec, &hs, sizeof(hs), hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_grid, &hg, sizeof(hg), hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_tab, &td, sizeof(td), hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK &&
+        ce_launch_small_r(s, 1, 1, d_spec, d_grid, d_tab, base, base + nin, 1e-5, base + off_out, d_stat))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(out_y, base + off_out, sizeof(double) * nodes, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(h_stat, d_stat, sizeof(int) * nodes, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    pthread_mutex_unlock(&e->lock);
+    if (st != COFFE_OK) return st;
+    st = check_small_r_status(h_stat, nodes);
+    if (st != COFFE_OK) return st;
+    static const double min_sep[CE_MIN_SEP_LEN] = {
+        1E-6, 1E-5, 1E-4, 1E-3, 2E-3, 5E-3, 7E-3, 8E-3, 9E-3, 1E-2, 1.1E-2, 1.2E-2, 1.25E-2, 1.3E-2,
+        1.35E-2, 1.5E-2, 2E-2, 2.5E-2, 3E-2, 3.5E-2, 5E-2, 7E-2, 8E-2, 9E-2, 1E-1, 1.1E-1, 1.2E-1,
+        1.25E-1, 1.3E-1, 1.35E-1, 1.5E-1, 1.75E-1, 2E-1, 2.5E-1, 3E-1, 3.5E-1, 4E-1, 5E-1, 6E-1,
+        7E-1, 8E-1, 9E-1};
+    out_x[0] = 0.0;
+    for (size_t i = 0; i < CE_MIN_SEP_LEN; ++i) out_x[1 + i] = min_sep[i];
+    return COFFE_OK;
 }

@@ -18,50 +18,9 @@
 
 #include "ce_internal.h"
 #include "coffe_engine.h"
+#include "ce_device.h"
 
 namespace {
-
-// Akima value at xq on the table (x, y) of length n (n >= 5), GSL non-rounded ends
-__device__ double akima_general(const double *__restrict__ x, const double *__restrict__ y, int n, double xq)
-{
-    if (!(xq >= x[0] && xq <= x[n - 1])) return __builtin_nan("");
-    int lo = 0, hi = n - 1;
-    while (hi > lo + 1) {
-        const int m = (lo + hi) >> 1;
-        if (x[m] > xq) hi = m;
-        else lo = m;
-    }
-    const int j = lo;
-    auto slope = [&](int a) -> double {
-        if (a >= 0 && a <= n - 2) return (y[a + 1] - y[a]) / (x[a + 1] - x[a]);
-        const double m0 = (y[1] - y[0]) / (x[1] - x[0]);
-        const double m1 = (y[2] - y[1]) / (x[2] - x[1]);
-        const double mn2 = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
-        const double mn3 = (y[n - 2] - y[n - 3]) / (x[n - 2] - x[n - 3]);
-        if (a == -1) return 2.0 * m0 - m1;
-        if (a == -2) return 3.0 * m0 - 2.0 * m1;
-        if (a == n - 1) return 2.0 * mn2 - mn3;
-        return 3.0 * mn2 - 2.0 * mn3;   // a == n
-    };
-    const double mm2 = slope(j - 2), mm1 = slope(j - 1), m0 = slope(j), mp1 = slope(j + 1), mp2 = slope(j + 2);
-    const double ne = fabs(mp1 - m0) + fabs(mm1 - mm2);
-    const double delx = xq - x[j];
-    if (ne == 0.0) return y[j] + delx * m0;
-    const double h = x[j + 1] - x[j];
-    const double nen = fabs(mp2 - mp1) + fabs(m0 - mm1);
-    const double alpha = fabs(mm1 - mm2) / ne;
-    double tn;
-    if (nen == 0.0) {
-        tn = m0;
-    } else {
-        const double an = fabs(m0 - mm1) / nen;
-        tn = (1.0 - an) * m0 + an * mp1;
-    }
-    const double b = (1.0 - alpha) * mm1 + alpha * m0;
-    const double c = (3.0 * m0 - 2.0 * b - tn) / h;
-    const double d = (b + tn - 2.0 * m0) / (h * h);
-    return y[j] + delx * (b + delx * (c + d * delx));
-}
 
 __device__ __forceinline__ double legendre_p(int l, double x)
 {
@@ -111,12 +70,12 @@ __device__ void local_coefficients(const mu_args &a, double zm, double mu, doubl
     const double costheta = (2. * chi_mean * chi_mean - sep * sep + mu * mu * sep * sep / 2.) /
                             (2. * chi_mean * chi_mean - mu * mu * sep * sep / 2.);
     const double ct2 = costheta * costheta;
-    const double z1 = akima_general(a.bg_chi, a.bg_z, a.nbg, chi1);
-    const double z2 = akima_general(a.bg_chi, a.bg_z, a.nbg, chi2);
-    const double f1 = akima_general(a.bg_z, a.bg_f, a.nbg, z1), f2 = akima_general(a.bg_z, a.bg_f, a.nbg, z2);
-    const double H1 = akima_general(a.bg_z, a.bg_cH, a.nbg, z1), H2 = akima_general(a.bg_z, a.bg_cH, a.nbg, z2);
-    const double b1 = akima_general(a.bias_z, a.bias1, a.nb, z1), b2 = akima_general(a.bias_z, a.bias2, a.nb, z2);
-    const double G1 = akima_general(a.bg_z, a.bg_G1, a.nbg, z1), G2 = akima_general(a.bg_z, a.bg_G2, a.nbg, z2);
+    const double z1 = ce::akima_general(a.bg_chi, a.bg_z, a.nbg, chi1);
+    const double z2 = ce::akima_general(a.bg_chi, a.bg_z, a.nbg, chi2);
+    const double f1 = ce::akima_general(a.bg_z, a.bg_f, a.nbg, z1), f2 = ce::akima_general(a.bg_z, a.bg_f, a.nbg, z2);
+    const double H1 = ce::akima_general(a.bg_z, a.bg_cH, a.nbg, z1), H2 = ce::akima_general(a.bg_z, a.bg_cH, a.nbg, z2);
+    const double b1 = ce::akima_general(a.bias_z, a.bias1, a.nb, z1), b2 = ce::akima_general(a.bias_z, a.bias2, a.nb, z2);
+    const double G1 = ce::akima_general(a.bg_z, a.bg_G1, a.nbg, z1), G2 = ce::akima_general(a.bg_z, a.bg_G2, a.nbg, z2);
     const double P2 = legendre_p(2, mu), P4 = legendre_p(4, mu);
     if (a.terms & COFFE_TERM_DEN_DEN) {   // functions.c:129-229
         if (a.flatsky) c[S00] += bm1 * bm2;
@@ -183,7 +142,7 @@ __device__ void local_coefficients(const mu_args &a, double zm, double mu, doubl
         }
     }
     const double G = a.flatsky ? Dmean * Dmean
-                               : akima_general(a.bg_z, a.bg_D1, a.nbg, z1) * akima_general(a.bg_z, a.bg_D1, a.nbg, z2);
+                               : ce::akima_general(a.bg_z, a.bg_D1, a.nbg, z1) * ce::akima_general(a.bg_z, a.bg_D1, a.nbg, z2);
     for (int s = 0; s < NSTD; ++s) c[s] *= G;
 }
 
@@ -196,12 +155,12 @@ __global__ void __launch_bounds__(256) k_mu_weights(mu_args a)
     if (point >= npoint) return;   // whole wavefronts exit together
     const int iz = point / a.nsep, ir = point % a.nsep;
     const double zm = a.zmean[iz], sep = a.sep[ir];
-    const double chi_mean = akima_general(a.bg_z, a.bg_chi, a.nbg, zm);
-    const double fmean = akima_general(a.bg_z, a.bg_f, a.nbg, zm);
-    const double Hmean = akima_general(a.bg_z, a.bg_cH, a.nbg, zm);
-    const double bm1 = akima_general(a.bias_z, a.bias1, a.nb, zm), bm2 = akima_general(a.bias_z, a.bias2, a.nb, zm);
-    const double Gm1 = akima_general(a.bg_z, a.bg_G1, a.nbg, zm), Gm2 = akima_general(a.bg_z, a.bg_G2, a.nbg, zm);
-    const double Dmean = akima_general(a.bg_z, a.bg_D1, a.nbg, zm);
+    const double chi_mean = ce::akima_general(a.bg_z, a.bg_chi, a.nbg, zm);
+    const double fmean = ce::akima_general(a.bg_z, a.bg_f, a.nbg, zm);
+    const double Hmean = ce::akima_general(a.bg_z, a.bg_cH, a.nbg, zm);
+    const double bm1 = ce::akima_general(a.bias_z, a.bias1, a.nb, zm), bm2 = ce::akima_general(a.bias_z, a.bias2, a.nb, zm);
+    const double Gm1 = ce::akima_general(a.bg_z, a.bg_G1, a.nbg, zm), Gm2 = ce::akima_general(a.bg_z, a.bg_G2, a.nbg, zm);
+    const double Dmean = ce::akima_general(a.bg_z, a.bg_D1, a.nbg, zm);
     // partial sums per (multipole, standard table slot); fixed bounds keep them in registers
     double acc[CE_MU_MAXL][NSTD];
 #pragma unroll

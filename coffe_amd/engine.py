@@ -169,6 +169,18 @@ class Batch:
         _need(gpk, (self.n_total,), "gpk")
         check(self.L.coffe_batch_spline_adj(self.h, _ptr(k), _ptr(gpk), stream), "coffe_batch_spline_adj")
 
+    def small_r(self, k, pk, out=None, stream=None):
+        """The 43 small-separation nodes [nspec][ntab][43]: r = 0 and the 42 min_sep values
+        (integrals.c:520-557).  Synchronises the stream."""
+        import torch
+        if out is None:
+            out = torch.empty((self.nspec, len(self.tables), 43), dtype=torch.float64, device=k.device)
+        self._check_io(k, pk, None, None, None)
+        _need(out, (self.nspec, len(self.tables), 43), "out")
+        check(self.L.coffe_batch_small_r(self.h, _ptr(k), _ptr(pk), _ptr(out),
+                                         stream if stream is not None else self._stream()), "coffe_batch_small_r")
+        return out
+
     def tables_full(self, k, pk, stream=None):
         import torch
         ntab = len(self.tables)
@@ -192,6 +204,21 @@ def twofast_1bessel(k, pk, N, l, nu, r0=1.0, k0=None, kmin=None, kmax=None):
     check(L.twofast_1bessel_status(ox.ctypes.data_as(dp), oy.ctypes.data_as(dp), N, k.ctypes.data_as(dp),
                                    pk.ctypes.data_as(dp), len(k), float(l), float(nu), r0, k0, kmin, kmax),
           "twofast_1bessel")
+    return ox, oy
+
+
+def integral_table(n, l, k, pk, N, kmin=None, kmax=None):
+    """The reference's full (N + 43)-point table (coffe_integrals_renormalizable,
+    integrals.c:459-573) through coffe_integral_table; host arrays."""
+    L = load_library()
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    pk = np.ascontiguousarray(pk, dtype=np.float64)
+    kmin = float(k[0]) if kmin is None else kmin
+    kmax = float(k[-1]) if kmax is None else kmax
+    ox, oy = np.empty(N + 43), np.empty(N + 43)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(L.coffe_integral_table(int(n), int(l), int(N), k.ctypes.data_as(dp), pk.ctypes.data_as(dp), len(k), kmin,
+                                 kmax, ox.ctypes.data_as(dp), oy.ctypes.data_as(dp)), "coffe_integral_table")
     return ox, oy
 
 

@@ -41,7 +41,8 @@ enum {
     COFFE_EUNSUPPORTED = 4,  /* e.g. FFT length with a prime factor > 5 */
     COFFE_EDOMAIN = 5,       /* window limits, separation outside the table */
     COFFE_ENONFINITE = 6,    /* NaN/inf in an input or in the gamma-ratio kernel */
-    COFFE_ENODEVICE = 7      /* no HIP device / library built without one */
+    COFFE_ENODEVICE = 7,     /* no HIP device / library built without one */
+    COFFE_ENOCONV = 8        /* adaptive quadrature did not converge (interval limit, roundoff) */
 };
 
 typedef struct coffe_engine coffe_engine;
@@ -171,6 +172,24 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
                            int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab,
                            const int *tab_n, const int *tab_l, int nsep, const double *sep, int nmu, double *d_A,
                            void *stream);
+
+/*
+ * The reference's full table (coffe_integrals_renormalizable, integrals.c:459-573):
+ * out_x/out_y (caller-allocated, N + 43 entries) = [r = 0, the 42 min_sep values
+ * (integrals.c:111-126), the N FFTLog nodes], y = I_l^n(r), times r^(n-l) when n >= l.
+ * The small-r nodes are QAG-GK61 integrals (epsrel 1e-5) over [kmin, kmax] of the
+ * Akima spline of (k, pk); the FFT nodes are twofast_1bessel(l, nu = n, r0 = 1,
+ * k0 = kmin).  Integer n in [-2, 4], l in [0, 4].  Synchronous, default engine.
+ */
+int coffe_integral_table(int n, int l, size_t N, const double *k, const double *pk, size_t nin, double kmin,
+                         double kmax, double *out_x, double *out_y);
+
+/*
+ * The 43 small-separation nodes (r = 0 and the 42 min_sep values) of every table of
+ * a batch, for its spectra: d_out[nspec][ntab][43] (device).  Synchronises the
+ * stream and returns COFFE_ENOCONV when an integral did not converge.
+ */
+int coffe_batch_small_r(coffe_batch *b, const double *d_k, const double *d_pk, double *d_out, void *stream);
 
 /* synchronise the engine's stream */
 int coffe_engine_synchronize(coffe_engine *e);

@@ -127,3 +127,27 @@ def test_integral_tables_vs_reference_fixtures(index, nl):
     # away from the zero crossings the agreement is much tighter than the test's 1e-4
     big = np.abs(y) > 1e-3 * np.max(np.abs(y))
     assert np.median(np.abs(got[big] - y[big]) / np.abs(y[big])) < 1e-5
+
+
+def test_small_r_nodes_gaussian_closed_form():
+    """The 43 small-r nodes (QAG-GK61, epsrel 1e-5) of P = exp(-k^2 R^2):
+    I_0^0(r) = exp(-r^2/4R^2) / (8 pi^(3/2) R^3) at every node; the r = 0 limits
+    (1/2pi^2) int k^(2+l-n) P / (2l+1)!! of (1,1), (2,2), (2,0), (3,1) in closed form."""
+    R = 2.0
+    k = np.logspace(-5, 1.5, 4096)
+    pk = np.exp(-(k * R) ** 2)
+    tx, ty = O.integral_table(0, 0, k, pk, 4096, small_r=True)
+    exact = np.exp(-tx[:43] ** 2 / (4 * R * R)) / (8 * np.pi ** 1.5 * R ** 3)
+    assert np.max(np.abs(ty[:43] / exact - 1)) < 1e-7
+    m2 = 1 / (8 * np.pi ** 1.5 * R ** 3)          # (1/2pi^2) int k^2 P
+    from scipy.special import erf
+    # (1/2pi^2) int_kmin^kmax P: the lower limit k = 1e-5 cuts 2e-5 of it
+    m0 = np.sqrt(np.pi) / (2 * R) * (erf(k[-1] * R) - erf(k[0] * R)) / (2 * np.pi ** 2)
+    for (n, l), want in {(1, 1): m2 / 3, (2, 2): m2 / 15, (2, 0): m0, (3, 1): m0 / 3}.items():
+        _, y = O.integral_table(n, l, k, pk, 1024, small_r=True)
+        assert abs(y[0] / want - 1) < 1e-7, (n, l)
+    # n < l: I_l^n(0) = 0; at small r, j_2(kr) ~ (kr)^2/15 gives I_2^0(r) ~ r^2 (1/2pi^2) int k^4 P / 15
+    _, y = O.integral_table(0, 2, k, pk, 1024, small_r=True)
+    assert y[0] == 0.0
+    m4 = 3 / (16 * np.pi ** 1.5 * R ** 5)          # (1/2pi^2) int k^4 P
+    assert abs(y[1] / (1e-6 ** 2 * m4 / 15) - 1) < 1e-6
