@@ -69,6 +69,15 @@ def load_library():
                                          ip, i, ip, ip, i, dp, i, vp, vp]
     L.coffe_integral_table.argtypes = [i, i, sz, dp, dp, sz, d, d, dp, dp]
     L.coffe_batch_small_r.argtypes = [vp, vp, vp, vp, vp]
+    L.coffe_cov_plan_create.argtypes = [vp, i, d, d, i, ip, i, dp, ctypes.POINTER(vp)]
+    L.coffe_cov_plan_destroy.argtypes = [vp]
+    L.coffe_cov_plan_destroy.restype = None
+    L.coffe_cov_plan_rows.argtypes = [vp]
+    L.coffe_cov_plan_pairs.argtypes = [vp]
+    L.coffe_cov_integrals.argtypes = [vp, i, i64p, vp, vp, vp, vp]
+    L.coffe_cov_assemble.argtypes = [vp, i, i, dp, dp, dp, dp, dp, dp, i, i, i, vp, vp, vp]
+    L.two_sph_bessel_status.argtypes = [dp, dp, ctypes.POINTER(dp), ctypes.c_long, ctypes.c_long, vp, dp, dp,
+                                        ctypes.POINTER(dp)]
     L.coffe_terms_of.restype = ctypes.c_uint
     L.coffe_terms_of.argtypes = [i, i, i]
     L.coffe_batch_prepare.argtypes = [vp, vp, vp, vp]
@@ -91,6 +100,8 @@ EXPORTED = [
     "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute",
     "coffe_batch_resample", "coffe_batch_fft_prepare", "coffe_batch_adj_c2r", "coffe_batch_spline_adj",
     "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_terms_of", "coffe_integral_table", "coffe_batch_small_r",
+    "coffe_cov_plan_create", "coffe_cov_plan_destroy", "coffe_cov_plan_rows", "coffe_cov_plan_pairs",
+    "coffe_cov_integrals", "coffe_cov_assemble", "two_sph_bessel", "two_sph_bessel_status",
 ]
 
 

@@ -1,0 +1,231 @@
+/*
+ * LDS-resident complex FFT building blocks shared by the HIP sources (gfx950, FP64).
+ *
+ * One workgroup owns one transform.  The length-M complex sequence lives in LDS as
+ * interleaved doubles and is transformed by self-sorting Stockham passes: every
+ * thread first pulls the R inputs of each of its butterflies into registers, the
+ * workgroup synchronises, then every thread applies twiddles, the radix-R DFT and
+ * stores the R outputs.  A single LDS buffer of 16*M bytes is therefore enough
+ * (M <= 8192 -> 128 KiB of the 160 KiB per CU).  Callers launch at least M/8 threads.
+ */
+#ifndef COFFE_ENGINE_FFT_H
+#define COFFE_ENGINE_FFT_H
+
+#include <hip/hip_runtime.h>
+
+#include "ce_internal.h"
+
+namespace ce {
+
+struct cd {
+    double x, y;
+};
+
+__device__ __forceinline__ cd mk(double a, double b) { cd r; r.x = a; r.y = b; return r; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cmul(cd a, cd b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cd cconj(cd a) { return mk(a.x, -a.y); }
+__device__ __forceinline__ cd cscale(cd a, double s) { return mk(a.x * s, a.y * s); }
+__device__ __forceinline__ cd mul_mi(cd a) { return mk(a.y, -a.x); }   // * (-i)
+__device__ __forceinline__ cd mul_pi(cd a) { return mk(-a.y, a.x); }   // * (+i)
+
+// ---- LDS layout ----------------------------------------------------------------
+// Complex element i of a transform lives at LDS slot pidx(i) = i + i/16: one pad slot
+// per 16 elements keeps the strided stores of the early Stockham passes (stride R
+// elements of 16 bytes) on distinct banks.
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
+__host__ __device__ __forceinline__ int lds_data_len(int M) { return M + (M >> 4) + 1; }
+// real element i of a transform stored as complex pairs (x_{2j}, x_{2j+1})
+__device__ __forceinline__ double &lds_real(cd *s, int i)
+{
+    return reinterpret_cast<double *>(s)[2 * pidx(i >> 1) + (i & 1)];
+}
+
+// ---- radix-R forward DFT butterflies (sign -1) --------------------------------
+template <int R> __device__ __forceinline__ void dft(cd *v);
+
+template <> __device__ __forceinline__ void dft<2>(cd *v)
+{
+    cd a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <> __device__ __forceinline__ void dft<3>(cd *v)
+{
+    const double s3 = 0.86602540378443864676372317075294;
+    cd a = v[0], s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    v[0] = cadd(a, s);
+    cd m = mk(a.x - 0.5 * s.x, a.y - 0.5 * s.y);
+    cd rot = mul_mi(cscale(d, s3));
+    v[1] = cadd(m, rot);
+    v[2] = csub(m, rot);
+}
+
+template <> __device__ __forceinline__ void dft<4>(cd *v)
+{
+    cd s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+    cd s1 = cadd(v[1], v[3]), d1 = mul_mi(csub(v[1], v[3]));
+    v[0] = cadd(s0, s1);
+    v[2] = csub(s0, s1);
+    v[1] = cadd(d0, d1);
+    v[3] = csub(d0, d1);
+}
+
+template <> __device__ __forceinline__ void dft<5>(cd *v)
+{
+    const double c1 = 0.30901699437494742410229341718282;
+    const double c2 = -0.80901699437494742410229341718282;
+    const double s1 = 0.95105651629515357211643933337938;
+    const double s2 = 0.58778525229247312916870595463907;
+    cd a = v[0];
+    cd b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+    cd e1 = csub(v[1], v[4]), e2 = csub(v[2], v[3]);
+    v[0] = cadd(a, cadd(b1, b2));
+    cd m1 = mk(a.x + c1 * b1.x + c2 * b2.x, a.y + c1 * b1.y + c2 * b2.y);
+    cd m2 = mk(a.x + c2 * b1.x + c1 * b2.x, a.y + c2 * b1.y + c1 * b2.y);
+    cd n1 = mul_mi(mk(s1 * e1.x + s2 * e2.x, s1 * e1.y + s2 * e2.y));
+    cd n2 = mul_mi(mk(s2 * e1.x - s1 * e2.x, s2 * e1.y - s1 * e2.y));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+}
+
+template <> __device__ __forceinline__ void dft<8>(cd *v)
+{
+    const double r2 = 0.70710678118654752440084436210485;
+    cd e[4] = {v[0], v[2], v[4], v[6]};
+    cd o[4] = {v[1], v[3], v[5], v[7]};
+    dft<4>(e);
+    dft<4>(o);
+    // o[k] *= W8^k
+    cd o1 = mk(r2 * (o[1].x + o[1].y), r2 * (o[1].y - o[1].x));     // (1-i)/sqrt2
+    cd o2 = mul_mi(o[2]);                                           // -i
+    cd o3 = mk(r2 * (o[3].y - o[3].x), -r2 * (o[3].x + o[3].y));    // (-1-i)/sqrt2
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+}
+
+// Twiddles W_M^k = e^{-2 pi i k / M} come from two small LDS tables,
+// W_M^k = T1[k >> 6] * T0[k & 63] (T0: 64 entries, T1: M/64 entries), both filled
+// from the correctly rounded e^{-2 pi i k/N} table of the grid; the product adds at
+// most one rounding.
+__device__ __forceinline__ cd twiddle(const cd *lt, int k) { return cmul(lt[64 + (k >> 6)], lt[k & 63]); }
+
+// fill the LDS twiddle tables of an M-point transform from tw (e^{-2 pi i k/(stride M)}):
+// stride 2 for the half-length transform of an N = 2M grid table, 1 for a table of
+// the transform's own length
+__device__ __forceinline__ void load_twiddles(cd *lt, int M, const cd *__restrict__ tw, int stride = 2)
+{
+    const int nhi = (M + 63) >> 6;
+    for (int i = threadIdx.x; i < 64 + nhi; i += blockDim.x) {
+        const int k = (i < 64) ? i : 64 * (i - 64);
+        lt[i] = (k < M) ? tw[stride * k] : mk(1.0, 0.0);
+    }
+}
+
+// One Stockham pass with radix R over s[0..M); Ns = product of earlier radices;
+// lt holds the LDS twiddle tables.
+template <int R, int MAXB>
+__device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restrict__ lt)
+{
+    const int B = M / R;
+    const int T = blockDim.x;
+    const int tid = threadIdx.x;
+    cd v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * T;
+        if (j < B) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = s[pidx(j + r * B)];
+        }
+    }
+    __syncthreads();
+    const int stride = M / (Ns * R);
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tid + b * T;
+        if (j < B) {
+            const int jm = j % Ns;
+            if (jm != 0) {
+                // w_r = W^(r jm stride): one table lookup, the rest by binary splitting
+                // w_r = w_{r/2} w_{r - r/2} (at most three roundings deep for R <= 8)
+                cd w[R];
+                w[1] = twiddle(lt, jm * stride);
+#pragma unroll
+                for (int r = 2; r < R; ++r) w[r] = cmul(w[r >> 1], w[r - (r >> 1)]);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r]);
+            }
+            dft<R>(v[b]);
+            const int base = (j - jm) * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[pidx(base + r * Ns)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Butterflies per thread: the host launches at least M/8 threads, so a radix-R pass
+// has at most ceil(8/R) butterflies per thread; only those register shapes are
+// instantiated (at most 12 complex values live per thread).
+template <int R>
+__device__ __forceinline__ void fft_pass_dispatch(cd *s, int M, int Ns, const cd *__restrict__ lt)
+{
+    const int B = M / R;
+    const int per = (B + (int)blockDim.x - 1) / (int)blockDim.x;
+    if (R == 8 || per <= 1) fft_pass<R, 1>(s, M, Ns, lt);
+    else if (R == 4 || R == 5 || per <= 2) fft_pass<R, 2>(s, M, Ns, lt);
+    else fft_pass<R, 4>(s, M, Ns, lt);
+}
+
+// forward complex FFT of length M in LDS (natural order in and out); lt = LDS
+// twiddle tables (load_twiddles), valid before the call
+__device__ __forceinline__ void fft_lds(cd *s, int M, const ce_radix_plan &rp, const cd *__restrict__ lt)
+{
+    int Ns = 1;
+    for (int p = 0; p < rp.npass; ++p) {
+        const int R = rp.radix[p];
+        switch (R) {
+        case 8: fft_pass_dispatch<8>(s, M, Ns, lt); break;
+        case 4: fft_pass_dispatch<4>(s, M, Ns, lt); break;
+        case 2: fft_pass_dispatch<2>(s, M, Ns, lt); break;
+        case 5: fft_pass_dispatch<5>(s, M, Ns, lt); break;
+        case 3: fft_pass_dispatch<3>(s, M, Ns, lt); break;
+        default: break;
+        }
+        Ns *= R;
+    }
+}
+
+// r2c split step: from Z = FFT_M(z), z_j = x_{2j} + i x_{2j+1}, produce X_k for the
+// pair (k, M-k): X_k = (Z_k + conj Z_{M-k})/2 - i/2 W_N^k (Z_k - conj Z_{M-k}).
+__device__ __forceinline__ cd r2c_split(cd Zk, cd Zmk, cd w)
+{
+    cd b = cconj(Zmk);
+    cd e = cscale(cadd(Zk, b), 0.5);
+    cd o = mul_mi(cscale(csub(Zk, b), 0.5));
+    return cadd(e, cmul(w, o));
+}
+
+// c2r pre-step for the pair: Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/N}(X_k - conj X_{M-k})
+__device__ __forceinline__ cd c2r_merge(cd Xk, cd Xmk, cd w)
+{
+    cd b = cconj(Xmk);
+    cd e = cadd(Xk, b);
+    cd o = cmul(csub(Xk, b), cconj(w));
+    return cadd(e, mul_pi(o));
+}
+
+}  // namespace ce
+
+#endif

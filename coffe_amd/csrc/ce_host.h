@@ -1,0 +1,40 @@
+/*
+ * Host-side internals shared by the C sources of the engine (engine.c,
+ * covariance.c): the per-device engine state and small HIP helpers.
+ */
+#ifndef COFFE_ENGINE_HOST_H
+#define COFFE_ENGINE_HOST_H
+
+#include <pthread.h>
+#include <stddef.h>
+
+#include <hip/hip_runtime_api.h>
+
+#include "coffe_engine.h"
+#include "ce_internal.h"
+
+struct ce_grid;
+
+struct coffe_engine {
+    int device;
+    hipStream_t stream;
+    int threads;
+    struct ce_grid *grids;
+    pthread_mutex_t lock;
+    /* scratch of the drop-in path */
+    void *d_scratch;
+    size_t scratch_bytes;
+};
+
+/* COFFE_OK or COFFE_EDEVICE */
+int ce_hip_ok(hipError_t e);
+/* hipMalloc of at least 8 bytes: COFFE_OK or COFFE_ENOMEM */
+int ce_dev_alloc(void **p, size_t bytes);
+/* interleaved e^{-2 pi i k/N}, k < N, from long-double arguments */
+void ce_twiddles(int N, double *out);
+/* the thread's last error of the void drop-in entry points */
+void ce_set_last_error(int code);
+/* radix schedule of a length-M complex FFT (radices 8, 4, 2, 5, 3) */
+int ce_radix_schedule(int M, ce_radix_plan *rp);
+
+#endif

@@ -103,6 +103,25 @@ int ce_launch_small_r(void *stream, int nspec, int ntab, const ce_spec_dev *d_sp
                       const ce_table_desc *d_tabs, const double *d_k, const double *d_pk, double epsrel,
                       double *d_out, int *d_status);
 
+/* ---- 2D FFTLog covariance (cov_kernels.hip) ---- */
+size_t ce_cov_lds_bytes(int N);
+int ce_launch_cov_ahat(void *stream, int nl, int N, const ce_radix_plan *rp, const double *d_tw, const double *d_A,
+                       double *d_ahat);
+int ce_launch_cov_diag(void *stream, int nspec, int N, const int64_t *d_in_off, const int32_t *d_n_in,
+                       const double *d_k, const double *d_pk, const double *d_kg, const double *d_k3,
+                       const double *d_kn1, const double *d_kn2, double step, double *d_d, int *d_status);
+int ce_launch_cov_rows(void *stream, int nblk, int N, const ce_radix_plan *rpN, const ce_radix_plan *rpM,
+                       const double *d_tw, const double *d_ahat, const double *d_B, const double *d_d, int npair,
+                       const int *d_pl1, const int *d_pl2, int nrows, const int *d_rows, const double *d_prn1,
+                       const double *d_prn2, int R, const int *d_flo, const int *d_fcnt, const int *d_foff,
+                       const double *d_fw, double *d_W);
+int ce_launch_cov_cols(void *stream, int nspec, int npair, int nrows, int R, const int *d_upos, const int *d_ucnt,
+                       const int *d_uoff, const double *d_uw, const double *d_fac, const double *d_W, double *d_out);
+int ce_launch_cov_assemble(void *stream, int nspec, int nz, int nl, int R, int npair, const int *d_ls,
+                           const int *d_pair_of, const double *d_sep, const double *d_cc, const double *d_cm,
+                           const double *d_D1, const double *d_nbar, const double *d_vol, const double *d_pix,
+                           int poisson, int mixed, int cosmic, const double *d_integ, double *d_out);
+
 /* ---- mu-quadrature weights (mu_kernels.hip) ---- */
 int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
                          const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,

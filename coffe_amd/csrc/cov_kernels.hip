@@ -1,0 +1,326 @@
+// 2D FFTLog covariance integrals (gfx950, FP64).
+//
+// The reference computes D_{l1 l2}(r1, r2) = int dk k^2 P^p j_l1(k r1) j_l2(k r2) with a
+// 2D FFTLog of the DIAGONAL matrix diag(k^3 P^p / dlnk) (covariance.c:357-470,
+// twobessel.c:16-113): 2D r2c, separable window and gamma-ratio kernel, conj, 2D c2r
+// (FFTW semantics: complex inverse along the first axis, then c2r along the second
+// with the imaginary parts of its DC and Nyquist columns dropped), then a bicubic
+// spline evaluated at sep x sep.
+//
+// For a diagonal input d_t the 2D r2c is D[(m+n) mod N] with D the 1D DFT of d, and
+// the whole transform collapses, row by row, to two 1D FFTs:
+//     Y_i(n) = B[n] sum_t d_t Ahat((i+t) mod N) e^{+2 pi i n t/N},  n <= N/2,
+//     row_i  = c2r(Y_i),
+// with A[m] = W_row(m) conj(H1[m]), B[n] = W_col(n) conj(H2[n]) (window times the
+// conjugated kernel) and Ahat(q) = sum_m A[m] e^{+2 pi i m q/N}, both spectrum-
+// independent.  Only the rows the bicubic needs are computed, and the bicubic
+// itself is applied as two sparse linear functionals: the Hermite weights of the
+// query and the spline-derivative weights of the nodes (which depend only on the
+// r grid) give value(m, n) = sum_i U_m(i) sum_j V_n(j) row_i(j).
+//
+// Kernels:
+//   k_cov_ahat   Ahat_l for each multipole (one workgroup each)
+//   k_cov_diag   d_t = k^3 P(k)^p / dlnk / k^nu1 / k^nu2 at the log grid (Akima P)
+//   k_cov_rows   per (spectrum, power, pair, row): the two FFTs, the output scale
+//                pi / (16 N^2 r_j^nu2 r_i^nu1) and the column functionals V_n
+//   k_cov_cols   the row functionals U_m and the (2l1+1)(2l2+1) {2,1}/pi^2 scale
+//   k_cov_assemble  Gaussian covariance of the multipoles (covariance.c:796-1060)
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "ce_device.h"
+#include "ce_fft.h"
+#include "ce_internal.h"
+
+namespace {
+
+using namespace ce;
+
+__global__ void __launch_bounds__(1024) k_cov_ahat(int N, ce_radix_plan rp, const cd *__restrict__ tw,
+                                                   const cd *__restrict__ A, cd *__restrict__ ahat)
+{
+    extern __shared__ cd smem[];
+    const int T = blockDim.x, tid = threadIdx.x;
+    cd *lt = smem + lds_data_len(N);
+    load_twiddles(lt, N, tw, 1);
+    const cd *a = A + (int64_t)blockIdx.x * N;
+    for (int i = tid; i < N; i += T) smem[pidx(i)] = cconj(a[i]);
+    __syncthreads();
+    fft_lds(smem, N, rp, lt);
+    cd *o = ahat + (int64_t)blockIdx.x * N;
+    for (int i = tid; i < N; i += T) o[i] = cconj(smem[pidx(i)]);
+}
+
+struct diag_args {
+    int N;
+    const int64_t *in_off;   // [nspec]
+    const int32_t *n_in;     // [nspec]
+    const double *k, *p;     // concatenated spectra
+    const double *kg;        // log grid k_t
+    const double *k3;        // k_t^3
+    const double *kn1, *kn2; // k_t^nu1, k_t^nu2
+    double step;
+    double *d;               // [nspec][2][N]
+    int *status;
+};
+
+__global__ void __launch_bounds__(256) k_cov_diag(diag_args a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (t >= a.N) return;
+    const int64_t off = a.in_off[s];
+    const double P = ce::akima_general(a.k + off, a.p + off, a.n_in[s], a.kg[t]);
+    // fk = k^3 P^power / dlnk, then / k^nu1 / k^nu2 (covariance.c:391-403, twobessel.c:49-55)
+    const double d1 = a.k3[t] * P / a.step / a.kn1[t] / a.kn2[t];
+    const double d2 = a.k3[t] * (P * P) / a.step / a.kn1[t] / a.kn2[t];
+    double *o = a.d + (int64_t)s * 2 * a.N;
+    o[t] = d1;
+    o[a.N + t] = d2;
+    if (!isfinite(d1) || !isfinite(d2)) a.status[0] = 1;
+}
+
+struct rows_args {
+    int N, M;
+    ce_radix_plan rpN, rpM;
+    const cd *tw;            // e^{-2 pi i k/N}, k < N
+    const cd *ahat;          // [nl][N]
+    const cd *B;             // [nl][M+1]
+    const double *d;         // [nspec][2][N]
+    const int *pl1, *pl2;    // [npair] multipole slots
+    int npair, nrows;
+    const int *rows;         // [nrows] row indices
+    const double *prn1, *prn2;  // r_j^nu1, r_j^nu2
+    int R;
+    const int *flo, *fcnt, *foff;   // functional n: first node, node count, weight offset
+    const double *fw;
+    double *W;               // [nblk][npair][nrows][R]
+};
+
+// one workgroup per (row, pair, block); a block is one diagonal d (spectrum * 2 + power)
+__global__ void __launch_bounds__(1024) k_cov_rows(rows_args a)
+{
+    extern __shared__ cd smem[];
+    const int T = blockDim.x, tid = threadIdx.x;
+    const int ir = blockIdx.x, pr = blockIdx.y, sp2 = blockIdx.z;
+    const int N = a.N, M = a.M;
+    const int i = a.rows[ir];
+    cd *ltN = smem + lds_data_len(N);
+    cd *ltM = ltN + 64 + ((N + 63) >> 6);
+    load_twiddles(ltN, N, a.tw, 1);
+    load_twiddles(ltM, M, a.tw, 2);
+    const double *dv = a.d + (int64_t)sp2 * N;
+    const cd *ah = a.ahat + (int64_t)a.pl1[pr] * N;
+    const cd *B = a.B + (int64_t)a.pl2[pr] * (M + 1);
+    // conj(u_t), u_t = d_t Ahat((i + t) mod N): the forward FFT then gives conj(V)
+    for (int t = tid; t < N; t += T) {
+        int q = i + t;
+        if (q >= N) q -= N;
+        smem[pidx(t)] = cscale(cconj(ah[q]), dv[t]);
+    }
+    __syncthreads();
+    fft_lds(smem, N, a.rpN, ltN);
+    // Y = B V on n <= M with Im Y_0 = Im Y_M = 0 (dropped by the c2r), merged into the
+    // half-length sequence (stored conjugated for the inverse transform)
+    for (int kk = tid; kk <= M / 2; kk += T) {
+        const int k1 = kk, k2 = M - kk;
+        cd Y1 = cmul(B[k1], cconj(smem[pidx(k1)]));
+        cd Y2 = cmul(B[k2], cconj(smem[pidx(k2)]));
+        if (k1 == 0) { Y1.y = 0.0; Y2.y = 0.0; }
+        smem[pidx(k1)] = cconj(c2r_merge(Y1, Y2, a.tw[k1]));
+        if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(Y2, Y1, a.tw[k2]));
+    }
+    __syncthreads();
+    fft_lds(smem, M, a.rpM, ltM);
+    // column functionals of this row: W[n] = sum_j V_n(j) row(j)
+    const double den_i = a.prn1[i];
+    const double fN = (double)N;
+    double *Wo = a.W + (((int64_t)sp2 * a.npair + pr) * a.nrows + ir) * a.R;
+    for (int n = tid; n < a.R; n += T) {
+        const int j0 = a.flo[n], cnt = a.fcnt[n];
+        const double *w = a.fw + a.foff[n];
+        double acc = 0.0;
+        for (int q = 0; q < cnt; ++q) {
+            const int j = j0 + q;
+            const cd z = smem[pidx(j >> 1)];
+            const double x = (j & 1) ? -z.y : z.x;
+            acc += w[q] * (x * 3.14159265358979323846 / (16. * fN * fN * a.prn2[j] * den_i));
+        }
+        Wo[n] = acc;
+    }
+}
+
+struct cols_args {
+    int npair, nrows, R;
+    const int *upos, *ucnt, *uoff;   // row functional m: first position in the row list, count, weight offset
+    const double *uw;
+    const double *fac;               // [2][npair]
+    const double *W;                 // [nspec][2][npair][nrows][R]
+    double *out;                     // [nspec][2][npair][R][R], value (m, n) at R n + m
+};
+
+__global__ void __launch_bounds__(256) k_cov_cols(cols_args a)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int RR = a.R * a.R;
+    if (e >= RR) return;
+    const int m = e / a.R, n = e % a.R;
+    const int pr = blockIdx.y, sp2 = blockIdx.z;
+    const int64_t blk = (int64_t)sp2 * a.npair + pr;
+    const double *W = a.W + blk * a.nrows * a.R;
+    const int p0 = a.upos[m], cnt = a.ucnt[m];
+    const double *w = a.uw + a.uoff[m];
+    double acc = 0.0;
+    for (int q = 0; q < cnt; ++q) acc += w[q] * W[(int64_t)(p0 + q) * a.R + n];
+    a.out[blk * RR + (int64_t)a.R * n + m] = acc * a.fac[(sp2 & 1) * a.npair + pr];
+}
+
+__global__ void k_fill_int(int *p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+struct asm_args {
+    int nz, nl, R, npair;
+    const int *ls;             // [nl]
+    const int *pair_of;        // [nl][nl] -> pair index of (min, max)
+    const double *sep;         // [R]
+    const double *cc, *cm;     // [nz][nl][nl]
+    const double *D1, *nbar, *vol, *pix;   // [nz]
+    int poisson, mixed, cosmic;
+    const double *integ;       // [nspec][2][npair][R][R]
+    double *out;               // [nspec][nz][nl][R][nl][R]
+};
+
+// one thread per output element (z, l1, r1, l2, r2) of one spectrum (blockIdx.y)
+__global__ void __launch_bounds__(256) k_cov_assemble(asm_args a)
+{
+    const int64_t per = (int64_t)a.nz * a.nl * a.R * a.nl * a.R;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per) return;
+    const int s = blockIdx.y;
+    const int n = (int)(e % a.R);
+    const int j = (int)((e / a.R) % a.nl);
+    const int m = (int)((e / ((int64_t)a.R * a.nl)) % a.R);
+    const int i = (int)((e / ((int64_t)a.R * a.nl * a.R)) % a.nl);
+    const int z = (int)(e / ((int64_t)a.R * a.nl * a.R * a.nl));
+    const int l1 = a.ls[i], l2 = a.ls[j];
+    const int RR = a.R * a.R;
+    const int pr = a.pair_of[i * a.nl + j];
+    // blocks l1 > l2 are the transposes of the stored (l2, l1) blocks (covariance.c:779-788)
+    const int64_t idx = (i <= j) ? (int64_t)a.R * n + m : (int64_t)a.R * m + n;
+    const double *base = a.integ + (int64_t)s * 2 * a.npair * RR;
+    const double dpk = base[(int64_t)pr * RR + idx];
+    const double dpk2 = base[((int64_t)a.npair + pr) * RR + idx];
+    const int zl = (z * a.nl + i) * a.nl + j;
+    const double D = a.D1[z], nb = a.nbar[z], V = a.vol[z];
+    const int sgn_l2 = (l2 % 2 == 0) ? 1 : -1;
+    const int sgn_d = (((l2 - l1) / 2) % 2 == 0) ? 1 : -1;
+    const int sgn_s = ((l1 + l2) % 2 == 0) ? 1 : -1;
+    double cov_p = 0.0, cov_m = 0.0, cov_c = 0.0;
+    if (a.poisson && l1 == l2 && m == n)
+        cov_p = (2 * l1 + 1) / 4. / 3.14159265358979323846 * (1 + sgn_l2) *
+                (1. / a.sep[n] / a.sep[m] / a.pix[z]) / nb / nb / V;
+    if (a.mixed) cov_m = sgn_d * sgn_s * D * D * dpk * a.cm[zl] / V;
+    if (a.cosmic) cov_c = sgn_d * D * D * D * D * dpk2 * a.cc[zl] / V;
+    a.out[(int64_t)s * per + e] = cov_p + cov_m + cov_c;
+}
+
+int set_lds(const void *fn, size_t bytes)
+{
+    static std::mutex mu;
+    static const void *fns[4] = {nullptr};
+    static size_t done[4] = {0};
+    std::lock_guard<std::mutex> g(mu);
+    int slot = -1;
+    for (int i = 0; i < 4; ++i) {
+        if (fns[i] == fn) { slot = i; break; }
+        if (!fns[i]) { fns[i] = fn; slot = i; break; }
+    }
+    if (slot >= 0 && done[slot] >= bytes) return 0;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess && slot >= 0) done[slot] = bytes;
+    return e == hipSuccess ? 0 : (int)e;
+}
+
+}  // namespace
+
+extern "C" size_t ce_cov_lds_bytes(int N)
+{
+    return sizeof(cd) * ((size_t)lds_data_len(N) + 64 + (size_t)((N + 63) >> 6) + 64 + (size_t)((N / 2 + 63) >> 6));
+}
+
+extern "C" int ce_launch_cov_ahat(void *stream, int nl, int N, const ce_radix_plan *rp, const double *d_tw,
+                                  const double *d_A, double *d_ahat)
+{
+    const size_t lds = sizeof(cd) * ((size_t)lds_data_len(N) + 64 + (size_t)((N + 63) >> 6));
+    int st = set_lds((const void *)k_cov_ahat, lds);
+    if (st) return st;
+    hipLaunchKernelGGL(k_cov_ahat, dim3(nl), dim3(1024), lds, (hipStream_t)stream, N, *rp,
+                       reinterpret_cast<const cd *>(d_tw), reinterpret_cast<const cd *>(d_A),
+                       reinterpret_cast<cd *>(d_ahat));
+    return (int)hipGetLastError();
+}
+
+extern "C" int ce_launch_cov_diag(void *stream, int nspec, int N, const int64_t *d_in_off, const int32_t *d_n_in,
+                                  const double *d_k, const double *d_pk, const double *d_kg, const double *d_k3,
+                                  const double *d_kn1, const double *d_kn2, double step, double *d_d, int *d_status)
+{
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, (hipStream_t)stream, d_status, 1, 0);
+    diag_args a;
+    a.N = N; a.in_off = d_in_off; a.n_in = d_n_in; a.k = d_k; a.p = d_pk; a.kg = d_kg; a.k3 = d_k3;
+    a.kn1 = d_kn1; a.kn2 = d_kn2; a.step = step; a.d = d_d; a.status = d_status;
+    hipLaunchKernelGGL(k_cov_diag, dim3((N + 255) / 256, nspec), dim3(256), 0, (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ce_launch_cov_rows(void *stream, int nblk, int N, const ce_radix_plan *rpN, const ce_radix_plan *rpM,
+                                  const double *d_tw, const double *d_ahat, const double *d_B, const double *d_d,
+                                  int npair, const int *d_pl1, const int *d_pl2, int nrows, const int *d_rows,
+                                  const double *d_prn1, const double *d_prn2, int R, const int *d_flo,
+                                  const int *d_fcnt, const int *d_foff, const double *d_fw, double *d_W)
+{
+    const size_t lds = ce_cov_lds_bytes(N);
+    int st = set_lds((const void *)k_cov_rows, lds);
+    if (st) return st;
+    rows_args a;
+    a.N = N; a.M = N / 2; a.rpN = *rpN; a.rpM = *rpM;
+    a.tw = reinterpret_cast<const cd *>(d_tw);
+    a.ahat = reinterpret_cast<const cd *>(d_ahat);
+    a.B = reinterpret_cast<const cd *>(d_B);
+    a.d = d_d; a.pl1 = d_pl1; a.pl2 = d_pl2; a.npair = npair; a.nrows = nrows; a.rows = d_rows;
+    a.prn1 = d_prn1; a.prn2 = d_prn2; a.R = R; a.flo = d_flo; a.fcnt = d_fcnt; a.foff = d_foff; a.fw = d_fw;
+    a.W = d_W;
+    hipLaunchKernelGGL(k_cov_rows, dim3(nrows, npair, nblk), dim3(1024), lds, (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ce_launch_cov_cols(void *stream, int nspec, int npair, int nrows, int R, const int *d_upos,
+                                  const int *d_ucnt, const int *d_uoff, const double *d_uw, const double *d_fac,
+                                  const double *d_W, double *d_out)
+{
+    cols_args a;
+    a.npair = npair; a.nrows = nrows; a.R = R; a.upos = d_upos; a.ucnt = d_ucnt; a.uoff = d_uoff; a.uw = d_uw;
+    a.fac = d_fac; a.W = d_W; a.out = d_out;
+    const int RR = R * R;
+    hipLaunchKernelGGL(k_cov_cols, dim3((RR + 255) / 256, npair, 2 * nspec), dim3(256), 0, (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ce_launch_cov_assemble(void *stream, int nspec, int nz, int nl, int R, int npair, const int *d_ls,
+                                      const int *d_pair_of, const double *d_sep, const double *d_cc,
+                                      const double *d_cm, const double *d_D1, const double *d_nbar,
+                                      const double *d_vol, const double *d_pix, int poisson, int mixed, int cosmic,
+                                      const double *d_integ, double *d_out)
+{
+    asm_args a;
+    a.nz = nz; a.nl = nl; a.R = R; a.npair = npair; a.ls = d_ls; a.pair_of = d_pair_of; a.sep = d_sep;
+    a.cc = d_cc; a.cm = d_cm; a.D1 = d_D1; a.nbar = d_nbar; a.vol = d_vol; a.pix = d_pix;
+    a.poisson = poisson; a.mixed = mixed; a.cosmic = cosmic; a.integ = d_integ; a.out = d_out;
+    const int64_t per = (int64_t)nz * nl * R * nl * R;
+    hipLaunchKernelGGL(k_cov_assemble, dim3((unsigned)((per + 255) / 256), nspec), dim3(256), 0,
+                       (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}

@@ -23,12 +23,15 @@
 
 #include "coffe_engine.h"
 #include "ce_internal.h"
+#include "ce_host.h"
 
 #define CE_THREADS_DEFAULT 512
 
 static __thread int ce_last_error = 0;
 
 int coffe_engine_last_error(void) { return ce_last_error; }
+
+void ce_set_last_error(int code) { ce_last_error = code; }
 
 const char *coffe_engine_strerror(int code)
 {
@@ -64,23 +67,25 @@ typedef struct ce_grid {
     ce_radix_plan rp;
 } ce_grid;
 
-struct coffe_engine {
-    int device;
-    hipStream_t stream;
-    int threads;
-    ce_grid *grids;
-    pthread_mutex_t lock;
-    /* scratch of the drop-in path */
-    void *d_scratch;
-    size_t scratch_bytes;
-};
-
 static int hip_ok(hipError_t e) { return e == hipSuccess ? COFFE_OK : COFFE_EDEVICE; }
+
+int ce_hip_ok(hipError_t e) { return hip_ok(e); }
 
 static int dev_alloc(void **p, size_t bytes)
 {
     if (bytes == 0) bytes = 8;
     return hipMalloc(p, bytes) == hipSuccess ? COFFE_OK : COFFE_ENOMEM;
+}
+
+int ce_dev_alloc(void **p, size_t bytes) { return dev_alloc(p, bytes); }
+
+void ce_twiddles(int N, double *out)
+{
+    for (int k = 0; k < N; ++k) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)N;
+        out[2 * k] = (double)cosl(a);
+        out[2 * k + 1] = (double)sinl(a);
+    }
 }
 
 double ce_selectqnu(double l, double nu)
@@ -177,11 +182,7 @@ static int grid_get(coffe_engine *e, int N, double kmin, double kmax, double k0,
     if (st == COFFE_OK && (!g->h_p || !h_tw)) st = COFFE_ENOMEM;
     if (st == COFFE_OK) {
         for (int i = 0; i < N; ++i) g->h_p[i] = pow(kmax / kmin, (double)i / N);
-        for (int k = 0; k < N; ++k) {
-            const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)N;
-            h_tw[2 * k] = (double)cosl(a);
-            h_tw[2 * k + 1] = (double)sinl(a);
-        }
+        ce_twiddles(N, h_tw);
     }
     const size_t nb = sizeof(double) * (size_t)N;
     const size_t cb = sizeof(double) * 2 * (size_t)(N / 2 + 1);

@@ -1,0 +1,76 @@
+/*
+ * 2D FFTLog (twobessel.h) and the Gaussian covariance of multipoles on the
+ * MI355X engine.  C-ABI: plain pointers and sizes; device pointers where named d_*.
+ */
+#ifndef COFFE_TWOBESSEL_H
+#define COFFE_TWOBESSEL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "coffe_engine.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* same layout as the reference's config (twobessel.h:1-6) */
+typedef struct config {
+    double l1, l2;
+    double nu1, nu2;
+    double c_window_width;
+    long Nk_sample;
+} config;
+
+/*
+ * Drop-in for two_sph_bessel (twobessel.h:7, twobessel.c:16-113): r1[i] = 1/k1[N-1-i],
+ * r2 likewise, result[i][j] at (r1[i], r2[j]).  The engine computes the transform of
+ * a DIAGONAL fk1k2 on a square grid (k1 == k2), the form covariance_integrate_fftlog
+ * calls (covariance.c:391-420); other inputs record COFFE_EUNSUPPORTED.  Odd lengths
+ * (where the reference calls exit) record COFFE_EINVAL.  N <= 8192, N and N/2 with
+ * prime factors 2, 3, 5 only.  The void form records the status for
+ * coffe_engine_last_error(); the _status form returns it.
+ */
+void two_sph_bessel(double *k1, double *k2, double **fk1k2, long N1, long N2, config *config, double *r1, double *r2,
+                    double **result);
+int two_sph_bessel_status(const double *k1, const double *k2, double **fk1k2, long N1, long N2, const config *cfg,
+                          double *r1, double *r2, double **result);
+
+/*
+ * Covariance plan: N sampling points (covariance_integration_bins), k_i = kmin
+ * (kmax/kmin)^(i/N) with kmin/kmax the spectrum's k_min_norm/k_max_norm, multipoles
+ * ls[nl], separations sep[R] (the bicubic queries).  Spectrum-independent tables are
+ * built once (covariance.c:357-470 with nu1 = nu2 = 1.01, window width 0.25).
+ */
+typedef struct coffe_cov_plan coffe_cov_plan;
+int coffe_cov_plan_create(coffe_engine *e, int N, double kmin, double kmax, int nl, const int *ls, int R,
+                          const double *sep, coffe_cov_plan **out);
+void coffe_cov_plan_destroy(coffe_cov_plan *p);
+/* transform rows one call computes per (spectrum, power, pair); pairs l1 <= l2 */
+int coffe_cov_plan_rows(const coffe_cov_plan *p);
+int coffe_cov_plan_pairs(const coffe_cov_plan *p);
+
+/*
+ * The scaled FFTLog integrals of P and P^2 for nspec spectra (concatenated device
+ * d_k/d_pk, host lengths n_in): d_out[s][power - 1][pair][R n + m] at (sep_m, sep_n),
+ * pairs (i <= j) in row-major order over ls, times (2 l1 + 1)(2 l2 + 1) {2, 1} / pi^2
+ * (covariance.c:728-767).  P is the Akima spline of each spectrum; [kmin, kmax] must
+ * lie inside every spectrum's k range (else COFFE_EDOMAIN).  Synchronises the stream.
+ */
+int coffe_cov_integrals(coffe_cov_plan *p, int nspec, const int64_t *n_in, const double *d_k, const double *d_pk,
+                        double *d_out, void *stream);
+
+/*
+ * Gaussian covariance of the multipoles (covariance.c:796-1060), per z bin k:
+ * galaxy biases bias[4 k .. 4 k + 3], growth rate f[k], D1[k], number density nbar[k],
+ * volume vol[k], pixel size pix[k] (host arrays); poisson/mixed/cosmic select the
+ * terms.  d_integ from coffe_cov_integrals; d_out[s][z][l1][r1][l2][r2] (device).
+ */
+int coffe_cov_assemble(coffe_cov_plan *p, int nspec, int nz, const double *bias, const double *f, const double *D1,
+                       const double *nbar, const double *vol, const double *pix, int poisson, int mixed, int cosmic,
+                       const double *d_integ, double *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

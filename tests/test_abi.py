@@ -32,6 +32,6 @@ def test_library_exports_all_declared_symbols():
 def test_error_strings_host_only():
     from coffe_amd._lib import load_library
     L = load_library()
-    for code in range(8):
+    for code in range(9):
         assert L.coffe_engine_strerror(code)
     assert L.coffe_engine_last_error() == 0
