@@ -1,0 +1,455 @@
+"""Drop-in subset of COFFE's Python API (python/coffe/coffe.pyx) for the spherical-
+Bessel path: the Coffe object with its parameters, set_power_spectrum_linear,
+galaxy biases, background lookups, integral(r, n, l), compute_multipole,
+compute_multipoles_bulk and compute_covariance_bulk, computed by the MI355X engine.
+
+Units follow the reference: k in 1/Mpc, P in Mpc^3, separations in Mpc.  Parameters
+are validated as the reference validates them (TypeError / ValueError), lazily
+computed pieces are dropped when a parameter they depend on changes
+(coffe.pyx:948-993), and results come back as Multipoles / Covariance records in the
+reference's order (multipoles.c:76-86: z, l, r; covariance.c:1046-1060: z, l1, r1,
+l2, r2).  Contributions beyond density, RSD and Doppler-1 (lensing, d2, g1-g5) and
+the CLASS / corrfunc / redshift-averaged paths are not part of this engine and raise
+NotImplementedError.
+"""
+import numpy as np
+
+from . import background as _background
+from .representation import Covariance, Multipoles
+
+TABLES = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]   # integrals.c:599-608
+_UNSUPPORTED = ("has_lensing", "has_d2", "has_g1", "has_g2", "has_g3", "has_g4", "has_g5")
+
+
+def _check_parameter(name, value, kind, xmin=None, xmax=None):
+    """Type and range check with the reference's exceptions (coffe.pyx:86-110)."""
+    if not isinstance(value, kind) or isinstance(value, bool):
+        raise TypeError(f"Expected {kind} for {name}, got {type(value)}")
+    if xmin is not None and value < xmin:
+        raise ValueError(f"Value {value} of {name} smaller than xmin ({xmin})")
+    if xmax is not None and value > xmax:
+        raise ValueError(f"Value {value} of {name} larger than xmax ({xmax})")
+
+
+def _float_list(name, value, ok):
+    try:
+        iter(value)
+    except TypeError as err:
+        raise TypeError(f"The value {value} of {name} is not iterable") from err
+    try:
+        out = [float(v) for v in value]
+    except (TypeError, ValueError) as err:
+        raise TypeError(f"Cannot convert all values of {name} to floats") from err
+    if not all(ok(v) for v in out):
+        raise ValueError(f"{name}: values out of range")
+    return np.array(out)
+
+
+class _Spline:
+    """A sampled function evaluated by Akima (the default interp_method, parser.c:807)."""
+
+    def __init__(self, x, y):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+
+    def __call__(self, v):
+        from .engine import akima
+        return akima(self.x, self.y, v)
+
+
+class Coffe:
+    """Engine-backed COFFE object; the keyword arguments go to set_parameters."""
+
+    _defaults = dict(
+        omega_cdm=0.25, omega_baryon=0.05, omega_gamma=9e-5, h=0.67, w0=-1.0, wa=0.0,
+        sep=5.0 + 3.0 * np.arange(100), l=[0, 2, 4], z_mean=[1.0], deltaz=[0.2], fsky=[1.0], pixelsize=[10.0],
+        number_density1=[1e-3], number_density2=[1e-3],
+        has_density=True, has_rsd=False, has_d1=False, has_flatsky_local=False,
+        integrals_fftlog_sampling=10000, covariance_integration_sampling=8000,
+        covariance_cosmic=True, covariance_mixed=True, covariance_poisson=True,
+        multipole_quadrature_nodes=256,
+    )
+
+    def __init__(self, **kwargs):
+        self._p = {k: (np.array(v) if isinstance(v, (list, np.ndarray)) else v) for k, v in self._defaults.items()}
+        zb = np.linspace(0.0, 15.0, 16)
+        self._bias = {i: _Spline(zb, np.ones_like(zb)) for i in (1, 2, 3, 4)}
+        self._pk = None
+        self._engine = None
+        self._free_all()
+        if kwargs:
+            self.set_parameters(**kwargs)
+
+    # ---- lazily computed state -------------------------------------------------
+    def _free_all(self):
+        self._bg = None
+        self._tables = None
+        self._multipoles = None
+        self._covariance = None
+
+    def _free_results(self):
+        self._multipoles = None
+        self._covariance = None
+
+    def _engine_handle(self):
+        if self._engine is None:
+            from .engine import Engine
+            self._engine = Engine(0)
+        return self._engine
+
+    # ---- parameters --------------------------------------------------------------
+    @property
+    def parameters(self):
+        return {k: (list(v) if isinstance(v, np.ndarray) else v) for k, v in self._p.items()}
+
+    def set_parameters(self, **kwargs):
+        for k, v in kwargs.items():
+            if k in _UNSUPPORTED:
+                setattr(self, k, v)
+                continue
+            if not hasattr(type(self), k) or not isinstance(getattr(type(self), k), property):
+                raise AttributeError(f"unknown parameter {k}")
+            setattr(self, k, v)
+
+    def _set_omega(self, name, value):
+        _check_parameter(name, value, (int, float), -100, 100)
+        trial = dict(self._p)
+        trial[name] = float(value)
+        if trial["omega_cdm"] + trial["omega_baryon"] + trial["omega_gamma"] > 1:
+            raise ValueError("Cannot satisfy the condition Omega_de >= 0")
+        if self._p[name] != float(value):
+            self._p[name] = float(value)
+            self._free_all()
+
+    omega_cdm = property(lambda s: s._p["omega_cdm"], lambda s, v: s._set_omega("omega_cdm", v))
+    omega_baryon = property(lambda s: s._p["omega_baryon"], lambda s, v: s._set_omega("omega_baryon", v))
+    omega_gamma = property(lambda s: s._p["omega_gamma"], lambda s, v: s._set_omega("omega_gamma", v))
+
+    @property
+    def omega_m(self):
+        return self._p["omega_cdm"] + self._p["omega_baryon"]
+
+    @omega_m.setter
+    def omega_m(self, value):
+        _check_parameter("omega_m", value, (int, float), -100, 100)
+        self._set_omega("omega_cdm", float(value) - self._p["omega_baryon"])
+
+    @property
+    def omega_de(self):
+        return 1 - self.omega_m - self._p["omega_gamma"]
+
+    def _set_scalar(self, name, value, kind, lo, hi, frees):
+        _check_parameter(name, value, kind, lo, hi)
+        if self._p[name] != value:
+            self._p[name] = value
+            frees()
+
+    h = property(lambda s: s._p["h"], lambda s, v: s._set_scalar("h", v, (int, float), 1e-10, 100, s._free_all))
+    w0 = property(lambda s: s._p["w0"], lambda s, v: s._set_scalar("w0", v, (int, float), -100, 100, s._free_all))
+    wa = property(lambda s: s._p["wa"], lambda s, v: s._set_scalar("wa", v, (int, float), -100, 100, s._free_all))
+    integrals_fftlog_sampling = property(
+        lambda s: s._p["integrals_fftlog_sampling"],
+        lambda s, v: s._set_scalar("integrals_fftlog_sampling", v, int, 4, 16384, s._free_all))
+    covariance_integration_sampling = property(
+        lambda s: s._p["covariance_integration_sampling"],
+        lambda s, v: s._set_scalar("covariance_integration_sampling", v, int, 8, 8192, s._free_results))
+    multipole_quadrature_nodes = property(
+        lambda s: s._p["multipole_quadrature_nodes"],
+        lambda s, v: s._set_scalar("multipole_quadrature_nodes", v, int, 2, 65536, s._free_results))
+
+    def _set_list(self, name, value, ok, frees, as_int=False):
+        if as_int:
+            try:
+                iter(value)
+            except TypeError as err:
+                raise TypeError(f"The value {value} of {name} is not iterable") from err
+            try:
+                arr = np.array([v.__index__() for v in value], dtype=int)
+            except AttributeError as err:
+                raise TypeError(f"Cannot convert all values of {name} to int") from err
+            if not all(ok(v) for v in arr):
+                raise ValueError(f"{name}: values out of range")
+        else:
+            arr = _float_list(name, value, ok)
+        self._p[name] = arr
+        frees()
+
+    sep = property(lambda s: s._p["sep"].copy(),
+                   lambda s, v: s._set_list("sep", v, lambda x: 0 < x < 10000, s._free_results))
+    z_mean = property(lambda s: s._p["z_mean"].copy(),
+                      lambda s, v: s._set_list("z_mean", v, lambda x: 0 <= x < 10, s._free_results))
+    l = property(lambda s: s._p["l"].copy(),  # noqa: E741
+                 lambda s, v: s._set_list("l", v, lambda x: 0 <= x < 10, s._free_results, as_int=True))
+    deltaz = property(lambda s: s._p["deltaz"].copy(),
+                      lambda s, v: s._set_list("deltaz", v, lambda x: 0 < x < 10, s._free_results))
+    fsky = property(lambda s: s._p["fsky"].copy(),
+                    lambda s, v: s._set_list("fsky", v, lambda x: 0 < x <= 1, s._free_results))
+    pixelsize = property(lambda s: s._p["pixelsize"].copy(),
+                         lambda s, v: s._set_list("pixelsize", v, lambda x: 0 < x < 10000, s._free_results))
+    number_density1 = property(lambda s: s._p["number_density1"].copy(),
+                               lambda s, v: s._set_list("number_density1", v, lambda x: x > 0, s._free_results))
+    number_density2 = property(lambda s: s._p["number_density2"].copy(),
+                               lambda s, v: s._set_list("number_density2", v, lambda x: x > 0, s._free_results))
+
+    def _set_flag(self, name, value):
+        value = bool(value)
+        if self._p[name] != value:
+            self._p[name] = value
+            self._free_results()
+
+    has_density = property(lambda s: s._p["has_density"], lambda s, v: s._set_flag("has_density", v))
+    has_rsd = property(lambda s: s._p["has_rsd"], lambda s, v: s._set_flag("has_rsd", v))
+    has_d1 = property(lambda s: s._p["has_d1"], lambda s, v: s._set_flag("has_d1", v))
+    has_flatsky_local = property(lambda s: s._p["has_flatsky_local"],
+                                 lambda s, v: s._set_flag("has_flatsky_local", v))
+    covariance_cosmic = property(lambda s: s._p["covariance_cosmic"], lambda s, v: s._set_flag("covariance_cosmic", v))
+    covariance_mixed = property(lambda s: s._p["covariance_mixed"], lambda s, v: s._set_flag("covariance_mixed", v))
+    covariance_poisson = property(lambda s: s._p["covariance_poisson"],
+                                  lambda s, v: s._set_flag("covariance_poisson", v))
+
+    def __getattr__(self, name):
+        if name in _UNSUPPORTED:
+            return False
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        if name in _UNSUPPORTED:
+            if value:
+                raise NotImplementedError(f"{name}: only density, RSD and Doppler-1 are computed by this engine")
+            return
+        object.__setattr__(self, name, value)
+
+    def reset_contributions(self):
+        """Sets every has_* contribution to False."""
+        for name in ("has_density", "has_rsd", "has_d1"):
+            setattr(self, name, False)
+
+    # ---- inputs ----------------------------------------------------------------------
+    def set_power_spectrum_linear(self, k, pk, z=0):
+        """The linear P(k) at z = 0 (k in 1/Mpc, P in Mpc^3); sets k_min/k_max to its ends
+        (coffe.pyx:2929-2996)."""
+        k = np.asarray(k, dtype=np.float64)
+        pk = np.asarray(pk, dtype=np.float64)
+        if not np.allclose(k, np.sort(k)):
+            raise ValueError("The input k array must be sorted")
+        if len(k) != len(pk):
+            raise ValueError("The input arrays have mismatching lengths")
+        if len(k) < 5:
+            raise ValueError("The power spectrum needs at least 5 samples")
+        self._pk = (np.ascontiguousarray(k), np.ascontiguousarray(pk))
+        self._tables = None
+        self._free_results()
+
+    @property
+    def k_min(self):
+        return None if self._pk is None else float(self._pk[0][0])
+
+    @property
+    def k_max(self):
+        return None if self._pk is None else float(self._pk[0][-1])
+
+    def power_spectrum(self, k, z):
+        """P(k, z) = D1(z)^2 P(k) (Akima of the input samples)."""
+        self._need_pk()
+        _check_parameter("k", k, (int, float), self.k_min, self.k_max)
+        return float(_Spline(*self._pk)(float(k))[0] * self.growth_factor(z) ** 2)
+
+    def _set_bias(self, i, x, y):
+        x = np.asarray(x, dtype=np.float64)
+        y = np.asarray(y, dtype=np.float64)
+        if not np.allclose(x, np.sort(x)):
+            raise ValueError("The input array must be sorted")
+        if len(x) != len(y):
+            raise ValueError(f"Mismatching lengths for x ({len(x)}) and y ({len(y)})")
+        if len(x) < 5:
+            raise ValueError("At least 5 samples are needed")
+        _check_parameter("xmin", float(x[0]), (int, float), xmin=0)
+        _check_parameter("xmax", float(x[-1]), (int, float), xmax=15)
+        self._bias[i] = _Spline(x, y)
+        self._free_results()
+
+    def _eval_bias(self, i, z):
+        s = self._bias[i]
+        _check_parameter("x", z, (int, float), float(s.x[0]), float(s.x[-1]))
+        return float(s(float(z))[0])
+
+    def set_galaxy_bias1(self, x_sampling, y_sampling):
+        self._set_bias(1, x_sampling, y_sampling)
+
+    def set_galaxy_bias2(self, x_sampling, y_sampling):
+        self._set_bias(2, x_sampling, y_sampling)
+
+    def set_galaxy_bias3(self, x_sampling, y_sampling):
+        self._set_bias(3, x_sampling, y_sampling)
+
+    def set_galaxy_bias4(self, x_sampling, y_sampling):
+        self._set_bias(4, x_sampling, y_sampling)
+
+    def galaxy_bias1(self, z):
+        return self._eval_bias(1, z)
+
+    def galaxy_bias2(self, z):
+        return self._eval_bias(2, z)
+
+    def galaxy_bias3(self, z):
+        return self._eval_bias(3, z)
+
+    def galaxy_bias4(self, z):
+        return self._eval_bias(4, z)
+
+    # ---- background -------------------------------------------------------------
+    def _background(self):
+        if self._bg is None:
+            self._bg = _background.compute(omega_cdm=self._p["omega_cdm"], omega_baryon=self._p["omega_baryon"],
+                                           omega_gamma=self._p["omega_gamma"], w0=self._p["w0"], wa=self._p["wa"],
+                                           h=self._p["h"])
+        return self._bg
+
+    def _bg_eval(self, key, z):
+        _check_parameter("z", z, (int, float), 0, 15)
+        bg = self._background()
+        return float(_Spline(bg["z"], bg[key])(float(z))[0])
+
+    def comoving_distance(self, z):
+        return self._bg_eval("chi", z)
+
+    def growth_factor(self, z):
+        return self._bg_eval("D1", z)
+
+    def growth_rate(self, z):
+        return self._bg_eval("f", z)
+
+    def hubble_rate(self, z):
+        return self._bg_eval("Hz", z)
+
+    def hubble_rate_conformal(self, z):
+        return self._bg_eval("conformal_Hz", z)
+
+    def hubble_rate_conformal_prime(self, z):
+        return self._bg_eval("conformal_Hz_prime", z)
+
+    def scale_factor(self, z):
+        return self._bg_eval("a", z)
+
+    def _bg_tables(self):
+        bg = self._background()
+        return dict(z=bg["z"], chi=bg["chi"], D1=bg["D1"], f=bg["f"], cH=bg["conformal_Hz"], G1=bg["G1"],
+                    G2=bg["G2"])
+
+    # ---- integrals ----------------------------------------------------------------
+    def _need_pk(self):
+        if self._pk is None:
+            raise ValueError("No power spectrum set: call set_power_spectrum_linear first")
+
+    def _integral_tables(self):
+        """The 8 default tables (integrals.c:580-660), each the reference's N + 43 nodes."""
+        self._need_pk()
+        if self._tables is None:
+            from .engine import integral_table
+            k, pk = self._pk
+            N = self._p["integrals_fftlog_sampling"]
+            self._tables = {t: integral_table(t[0], t[1], k, pk, N) for t in TABLES}
+        return self._tables
+
+    def integral(self, *, r, n, l):  # noqa: E741
+        """I_l^n(r) (times r^(n - l) when n > l), Akima of the table (coffe.pyx:1657-1713)."""
+        _check_parameter("r", r, (int, float), 0, 25000)
+        _check_parameter("n", n, int, -2, 4)
+        _check_parameter("l", l, int, 0, 4)
+        if (n, l) not in TABLES:
+            raise ValueError(f"Cannot find integral with n = {n}, l = {l}")
+        x, y = self._integral_tables()[(n, l)]
+        return float(_Spline(x, y)(float(r))[0])
+
+    # ---- multipoles ---------------------------------------------------------------
+    def _check_coords_multipoles(self):
+        if not (len(self._p["l"]) and len(self._p["sep"]) and len(self._p["z_mean"])):
+            raise ValueError("At least one of the following parameters is empty: l, sep, z_mean")
+
+    def _check_contributions(self):
+        if not (self._p["has_density"] or self._p["has_rsd"] or self._p["has_d1"]):
+            raise ValueError("No contributions specified, you need to specify at least one of: "
+                             "'has_density', 'has_rsd', 'has_d1'")
+
+    def _multipole_values(self, zs, seps, ls):
+        """[z][l][r] multipoles: weights on the GPU (mu quadrature), tables at the separations."""
+        from .engine import mu_weights
+        from .engine import akima as _ak
+        tabs = self._integral_tables()
+        bz = self._bias[1].x
+        if not np.array_equal(bz, self._bias[2].x):
+            raise ValueError("galaxy_bias1 and galaxy_bias2 must share their redshift sampling")
+        A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, zs, list(ls),
+                       TABLES, seps, den=self._p["has_density"], rsd=self._p["has_rsd"], d1=self._p["has_d1"],
+                       flatsky=self._p["has_flatsky_local"], nmu=self._p["multipole_quadrature_nodes"]).cpu().numpy()
+        I = np.array([_ak(tabs[t][0], tabs[t][1], seps) for t in TABLES])
+        return np.einsum("zltr,tr->zlr", A, I)
+
+    def compute_multipole(self, *, z, r, l, recompute=False):  # noqa: E741
+        """The multipole at (z_mean, r, l) (coffe.pyx:3232-3300)."""
+        self._check_coords_multipoles()
+        self._check_contributions()
+        _check_parameter("z", z, (int, float), 0, 15)
+        _check_parameter("r", r, (int, float), 0, 1500)
+        _check_parameter("l", l, int, 0, 10)
+        if recompute:
+            self._free_all()
+        return float(self._multipole_values([float(z)], [float(r)], [int(l)])[0, 0, 0])
+
+    def compute_multipoles_bulk(self, recompute=False):
+        """Multipoles at every (z_mean, l, sep), z-major then l then r (multipoles.c:76-86)."""
+        self._check_coords_multipoles()
+        self._check_contributions()
+        if recompute:
+            self._free_all()
+        if self._multipoles is None:
+            zs, ls, seps = self._p["z_mean"], self._p["l"], self._p["sep"]
+            vals = self._multipole_values(zs, seps, ls)
+            self._multipoles = np.array([Multipoles(z=float(z), r=float(r), l=int(li), value=float(vals[iz, il, ir]))
+                                         for iz, z in enumerate(zs) for il, li in enumerate(ls)
+                                         for ir, r in enumerate(seps)])
+        return self._multipoles
+
+    # ---- covariance ---------------------------------------------------------------
+    def compute_covariance_bulk(self, recompute=False):
+        """Gaussian flat-sky covariance of the multipoles with the 2D FFTLog
+        (covariance.c:477-1103); records ordered z, l1, r1, l2, r2."""
+        import torch
+        from .covariance import CovPlan
+        self._check_coords_multipoles()
+        self._check_contributions()
+        self._need_pk()
+        zs = self._p["z_mean"]
+        per_bin = [self._p[k] for k in ("deltaz", "fsky", "pixelsize", "number_density1")]
+        if any(len(v) != len(zs) for v in per_bin):
+            raise ValueError("z_mean, deltaz, fsky, pixelsize and number_density1 must have the same length")
+        if recompute:
+            self._free_all()
+        if self._covariance is None:
+            k, pk = self._pk
+            ls, seps = [int(v) for v in self._p["l"]], self._p["sep"]
+            plan = CovPlan(self._engine_handle(), self._p["covariance_integration_sampling"], k[0], k[-1], ls, seps)
+            kt = torch.tensor(k, device="cuda")
+            pt = torch.tensor(pk, device="cuda")
+            integ = plan.integrals(kt, pt, [len(k)])
+            bias, f, D1, vol = [], [], [], []
+            for iz, z in enumerate(zs):
+                dz, fsky = self._p["deltaz"][iz], self._p["fsky"][iz]
+                # biases only with a density contribution, f only with RSD (covariance.c:840-860)
+                b = [self._eval_bias(i, float(z)) if self._p["has_density"] else 0.0 for i in (1, 2, 3, 4)]
+                bias.append(b)
+                f.append(self.growth_rate(float(z)) if self._p["has_rsd"] else 0.0)
+                D1.append(self.growth_factor(float(z)))
+                chi_lo = self.comoving_distance(float(z - dz))
+                chi_hi = self.comoving_distance(float(z + dz))
+                vol.append(4 * np.pi * fsky * (chi_hi ** 3 - chi_lo ** 3) / 3.)
+            cov = plan.assemble(integ, np.array(bias), f, D1, self._p["number_density1"], vol, self._p["pixelsize"],
+                                poisson=self._p["covariance_poisson"], mixed=self._p["covariance_mixed"],
+                                cosmic=self._p["covariance_cosmic"]).cpu().numpy()[0]
+            plan.close()
+            self._covariance = np.array([
+                Covariance(r1=float(seps[m]), r2=float(seps[n]), l1=ls[i], l2=ls[j], z=float(z),
+                           value=float(cov[iz, i, m, j, n]))
+                for iz, z in enumerate(zs) for i in range(len(ls)) for m in range(len(seps))
+                for j in range(len(ls)) for n in range(len(seps))])
+        return self._covariance

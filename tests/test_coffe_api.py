@@ -1,0 +1,99 @@
+"""The drop-in Coffe object (coffe_amd.coffe) on the CPU: parameter validation and
+error behaviour as the reference's own tests exercise them (test_coffe.py:47-120),
+defaults (parser.c:733-1033) and the result records."""
+import numpy as np
+import pytest
+
+from coffe_amd.coffe import Coffe
+from coffe_amd.representation import Covariance, Multipoles
+
+
+def test_defaults():
+    c = Coffe()
+    assert c.omega_m == pytest.approx(0.3) and c.h == 0.67 and c.omega_baryon == 0.05
+    assert list(c.l) == [0, 2, 4] and list(c.z_mean) == [1.0]
+    assert len(c.sep) == 100 and c.sep[0] == 5.0
+    assert c.has_density and not c.has_rsd and not c.has_d1 and not c.has_lensing
+    assert c.integrals_fftlog_sampling == 10000 and c.covariance_integration_sampling == 8000
+
+
+def test_bias():
+    c = Coffe()
+    with pytest.raises(ValueError):
+        x = np.linspace(-1, 10, 100)
+        c.set_galaxy_bias1(x, np.zeros(len(x)))
+    with pytest.raises(ValueError):
+        x = np.linspace(0, 100, 100)
+        c.set_galaxy_bias1(x, np.zeros(len(x)))
+    with pytest.raises(ValueError):
+        x = np.linspace(0, 1, 100)
+        c.set_galaxy_bias1(x, np.zeros(len(x)))
+        c.galaxy_bias1(2)
+    with pytest.raises(ValueError):
+        c.set_galaxy_bias2([0, 2, 1, 3, 4], np.ones(5))
+    with pytest.raises(ValueError):
+        c.set_galaxy_bias2(np.linspace(0, 5, 6), np.ones(5))
+
+
+def test_parameters():
+    c = Coffe()
+    with pytest.raises(ValueError):
+        c.omega_cdm = 1.1
+    with pytest.raises(ValueError):
+        c.h = -1
+    with pytest.raises(ValueError):   # omega_baryon = 0.05 by default
+        c.omega_cdm = 0.99
+    with pytest.raises(ValueError):
+        c.omega_m = 0.99999999
+    with pytest.raises(ValueError):
+        c.z_mean = (-1,)
+    with pytest.raises(TypeError):
+        c.z_mean = 0.1
+    with pytest.raises(TypeError):
+        c.l = [0.5]
+    with pytest.raises(ValueError):
+        c.sep = [0.0, 10.0]
+    with pytest.raises(ValueError):   # only the default tables exist (coffe_find_integral)
+        c.integral(r=100, n=4, l=4)
+    with pytest.raises(TypeError):
+        c.integral(r=100, n=0.5, l=0)
+    with pytest.raises(AttributeError):
+        c.set_parameters(no_such_parameter=1)
+    with pytest.raises(NotImplementedError):
+        c.has_lensing = True
+    c.has_lensing = False
+
+
+def test_coordinates_and_contributions_checked_first():
+    c = Coffe()
+    c.z_mean = []
+    with pytest.raises(ValueError):
+        c.compute_multipoles_bulk()
+    with pytest.raises(ValueError):
+        c.compute_covariance_bulk()
+    c = Coffe()
+    c.reset_contributions()
+    with pytest.raises(ValueError):
+        c.compute_multipole(z=0.1, r=100, l=2)
+    c = Coffe()
+    with pytest.raises(ValueError):   # no P(k) yet
+        c.integral(r=100, n=0, l=0)
+
+
+def test_power_spectrum_input_checks():
+    c = Coffe()
+    with pytest.raises(ValueError):
+        c.set_power_spectrum_linear([3, 2, 1, 4, 5], [1, 1, 1, 1, 1])
+    with pytest.raises(ValueError):
+        c.set_power_spectrum_linear([1, 2, 3, 4, 5], [1, 1, 1, 1])
+    c.set_power_spectrum_linear(np.logspace(-4, 1, 10), np.ones(10))
+    assert c.k_min == 1e-4 and c.k_max == 10.0
+
+
+def test_records():
+    m = Multipoles(z=1.0, r=10.0, l=2, value=0.5)
+    assert m.to_dict() == {"z": 1.0, "r": 10.0, "l": 2, "value": 0.5}
+    c = Covariance(r1=1.0, r2=2.0, l1=0, l2=2, z=1.0, value=3.0)
+    assert c.l2 == 2 and c.to_dict()["value"] == 3.0
+    with pytest.raises(TypeError):
+        Multipoles(z=1.0, r=1.0, l=0)
