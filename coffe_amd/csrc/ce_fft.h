@@ -17,7 +17,9 @@
 
 namespace ce {
 
-struct cd {
+// 16-byte aligned so that LDS and global accesses of a complex value are single
+// 128-bit operations (ds_read_b128 / global_load_dwordx4)
+struct __attribute__((aligned(16))) cd {
     double x, y;
 };
 

@@ -162,55 +162,52 @@ __global__ void k_grid_tables(grid_tab_args a)
 #define CE_SPLINE_HALO 64
 
 // Cyclic reduction of a_i x_{i-1} + b_i x_i + c_i x_{i+1} = d_i, i < ns, in place in
-// LDS (A, C, D; R holds 1/b).  After s steps row i couples only to rows i +- 2^s with
-// weights that shrink doubly exponentially: the natural-spline matrix has
-// (|a| + |c|)/|b| = 1/2, and the ratio goes 1/2 -> 0.14 -> 0.01 -> 5e-5 -> 1e-9 ->
-// 1e-18 on a uniform grid.  CE_PCR_STEPS = 7 reductions therefore leave couplings
-// far below double rounding for any grid, and x_i = d_i / b_i.  On exit D holds x.
-// All threads must call it; ns <= CE_PCR_PER * blockDim.x.
+// LDS (A, B, C, D).  Step s eliminates x_{i -+ s} from row i with the rows i -+ s:
+//     row_i <- bL bR row_i - a bR row_L - c bL row_R,
+// which needs no division; the new row is then scaled by a power of two (exponent of
+// its diagonal), so magnitudes stay near 1 and the scaling is exact.  This is the
+// usual normalised reduction multiplied through by bL bR.  After s steps row i couples
+// only to rows i +- 2^s with weights that shrink doubly exponentially: the natural-
+// spline matrix has (|a| + |c|)/|b| = 1/2, and the ratio goes 1/2 -> 0.14 -> 0.01 ->
+// 5e-5 -> 1e-9 -> 1e-18 on a uniform grid.  CE_PCR_STEPS = 7 reductions therefore
+// leave couplings far below double rounding for any grid, and x_i = d_i / b_i.  On
+// exit D holds x.  All threads must call it; ns <= CE_PCR_PER * blockDim.x.
 #define CE_PCR_STEPS 7
 #define CE_PCR_PER 5
-__device__ void pcr_solve(double *__restrict__ A, double *__restrict__ R, double *__restrict__ C,
+__device__ void pcr_solve(double *__restrict__ A, double *__restrict__ B, double *__restrict__ C,
                           double *__restrict__ D, int ns)
 {
     const int T = blockDim.x, tid = threadIdx.x;
-    for (int i = tid; i < ns; i += T) R[i] = 1.0 / R[i];   // R enters holding b
-    __syncthreads();
     int stride = 1;
     for (int step = 0; step < CE_PCR_STEPS && stride < ns; ++step, stride <<= 1) {
-        double na[CE_PCR_PER], nc[CE_PCR_PER], nd[CE_PCR_PER], nr[CE_PCR_PER];
+        double na[CE_PCR_PER], nb[CE_PCR_PER], nc[CE_PCR_PER], nd[CE_PCR_PER];
 #pragma unroll
         for (int q = 0; q < CE_PCR_PER; ++q) {
             const int i = tid + q * T;
             if (i < ns) {
                 const int il = i - stride, ir = i + stride;
-                const double a = A[i], c = C[i];
-                double b = 1.0 / R[i], d = D[i];
-                double ra = 0.0, rc = 0.0;
-                if (il >= 0) {
-                    const double al = -a * R[il];
-                    ra = al * A[il];
-                    b += al * C[il];
-                    d += al * D[il];
-                }
-                if (ir < ns) {
-                    const double ga = -c * R[ir];
-                    rc = ga * C[ir];
-                    b += ga * A[ir];
-                    d += ga * D[ir];
-                }
-                na[q] = ra; nc[q] = rc; nd[q] = d; nr[q] = 1.0 / b;
+                const double a = A[i], b = B[i], c = C[i], d = D[i];
+                double aL = 0.0, bL = 1.0, cL = 0.0, dL = 0.0, aR = 0.0, bR = 1.0, cR = 0.0, dR = 0.0;
+                if (il >= 0) { aL = A[il]; bL = B[il]; cL = C[il]; dL = D[il]; }
+                if (ir < ns) { aR = A[ir]; bR = B[ir]; cR = C[ir]; dR = D[ir]; }
+                const double u = a * bR, v = c * bL;
+                double ra = -u * aL;
+                double rb = b * bL * bR - u * cL - v * aR;
+                double rc = -v * cR;
+                double rd = d * bL * bR - u * dL - v * dR;
+                const int e = -__builtin_amdgcn_frexp_exp(rb);
+                na[q] = ldexp(ra, e); nb[q] = ldexp(rb, e); nc[q] = ldexp(rc, e); nd[q] = ldexp(rd, e);
             }
         }
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < CE_PCR_PER; ++q) {
             const int i = tid + q * T;
-            if (i < ns) { A[i] = na[q]; C[i] = nc[q]; D[i] = nd[q]; R[i] = nr[q]; }
+            if (i < ns) { A[i] = na[q]; B[i] = nb[q]; C[i] = nc[q]; D[i] = nd[q]; }
         }
         __syncthreads();
     }
-    for (int i = tid; i < ns; i += T) D[i] = D[i] * R[i];
+    for (int i = tid; i < ns; i += T) D[i] = D[i] / B[i];
     __syncthreads();
 }
 
