@@ -116,6 +116,123 @@ def plain_path(specs, sep, A, sample):
     return dict(value=points / dt, unit="points/s", sample=f"{len(sub)} spectra, forward only, {dt:.3f} s wall")
 
 
+def run_fullsky(args):
+    """Full-sky density + RSD + Doppler-1 multipoles of one P(k) (N = 16384 input and
+    FFT samples, the 8 default tables) at 8 mean redshifts x l = 0, 2, 4 x 512
+    separations, mu integral by 256-node Gauss-Legendre on the GPU (k_mu_weights)."""
+    import torch
+    import oracle as O
+    from coffe_amd import background as B
+    from coffe_amd.engine import Batch, Engine, mu_weights
+    from tests.synth import spectrum
+    tabs = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]
+    k, pk = spectrum(16384)
+    sep = np.linspace(5.0, 300.0, 512) / H
+    zs = np.linspace(0.5, 2.0, 8)
+    bgd = B.compute(omega_cdm=0.26, omega_baryon=0.05, h=H)
+    bg = dict(z=bgd["z"], chi=bgd["chi"], D1=bgd["D1"], f=bgd["f"], cH=bgd["conformal_Hz"], G1=bgd["G1"],
+              G2=bgd["G2"])
+    zb, b = np.linspace(0, 10, 11), np.full(11, 1.5)
+    eng = Engine(0)
+    batch = Batch(eng, [len(k)], [len(k)], [k[0]], [k[-1]], tabs, sep, len(zs), len(LS))
+    kt, pt = torch.tensor(k, device="cuda"), torch.tensor(pk, device="cuda")
+    A = torch.empty((len(zs), len(LS), len(tabs), len(sep)), dtype=torch.float64, device="cuda")
+
+    def step():
+        mu_weights(eng, bg, zb, b, b, zs, LS, tabs, sep, den=1, rsd=1, d1=1, flatsky=False, nmu=256, out=A)
+        return batch.forward(kt, pt, A)[1]
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    got = out.cpu().numpy()[0]
+    otabs = {t: O.integral_table(t[0], t[1], k, pk, len(k), small_r=False) for t in tabs}
+    ref = O.multipoles(otabs, bg, zb, b, b, zs, LS, sep, den=1, rsd=1, d1=1, flatsky=False, quad=256)
+    rel = np.abs(got - ref) / np.abs(ref)
+    points = got.size
+    good = int((rel <= 1e-8).sum())
+    t1 = time.perf_counter()
+    sub = sep[::32]
+    O.multipoles(otabs, bg, zb, b, b, zs[:2], LS, sub, den=1, rsd=1, d1=1, flatsky=False, quad=0, epsrel=1e-5)
+    cpu_pts = 2 * len(LS) * len(sub) / (time.perf_counter() - t1)
+    print(json.dumps({
+        "metric": "full-sky multipole points/sec (8 z x l=0,2,4 x 512 r, 8 tables, GL-256 mu), 1 MI355X",
+        "value": points / (ms / 1e3), "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "full-sky den+RSD+d1, one P(k) with N=16384, 8 tables, z=linspace(0.5,2,8), "
+                               "512 seps 5-300 Mpc/h, GL-256; step = mu weights + tables + projection"},
+        "parity": {"points": points, "within_1e-8": good, "max_rel": float(rel.max()),
+                   "note": "GL-256 oracle; FP64 cancellation noise ~eps (chi/r)^4 at the smallest separations"},
+        "cpu_baseline": {"value": cpu_pts, "unit": "points/s", "cores": 1, "kind": "port",
+                         "sample": f"2 z x 3 l x {len(sub)} seps, QAG-GK61 mu integral per point on the oracle tables "
+                                   "(tables not counted)"}}))
+
+
+def run_covariance(args):
+    """2D FFTLog Gaussian covariance: 64 P(k), N = 8000, l = 0, 2, 4, 256 separations
+    10-200 Mpc/h, 4 redshift bins (integrals of P and P^2 for the 6 pairs l1 <= l2, then
+    the assembly)."""
+    import torch
+    import oracle as O
+    from coffe_amd.covariance import CovPlan
+    from coffe_amd.engine import Engine
+    from tests.synth import batch_spectra
+    specs = batch_spectra(64, seed=5, lengths=(4096,))
+    sep = np.linspace(10.0, 200.0, 256) / H
+    kmin = max(s[0][0] for s in specs)
+    kmax = min(s[0][-1] for s in specs)
+    eng = Engine(0)
+    plan = CovPlan(eng, 8000, kmin, kmax, LS, sep)
+    k = torch.tensor(np.concatenate([s[0] for s in specs]), device="cuda")
+    p = torch.tensor(np.concatenate([s[1] for s in specs]), device="cuda")
+    n_in = [len(s[0]) for s in specs]
+    nz = 4
+    zargs = dict(f=[0.7, 0.8, 0.85, 0.9], D1=[0.8, 0.7, 0.6, 0.5], nbar=[1e-3 * H ** 3] * nz,
+                 vol=[1e9 / H ** 3] * nz, pix=[5.0 / H] * nz)
+
+    def step():
+        integ = plan.integrals(k, p, n_in)
+        return plan.assemble(integ, [1.5] * nz, **zargs), integ
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cov, integ = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    entries = cov.numel()
+    res = {
+        "metric": "Gaussian covariance entries/sec (64 P(k), 2D FFTLog N=8000, 4 z x (3 l x 256 r)^2), 1 MI355X",
+        "value": entries / (ms / 1e3), "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "64 seeded P(k) (4096 samples), 2D FFTLog N=8000, l=0,2,4, 256 seps 10-200 Mpc/h, "
+                               "4 z bins; step = P and P^2 integrals of 6 pairs + assembly",
+                   "transform_rows_per_pair": plan.nrows, "transforms_per_step": 64 * 2 * plan.npair},
+    }
+    if args.cpu_sample:
+        ks, ps = specs[0][0], specs[0][1]
+        t1 = time.perf_counter()
+        ref = O.covariance_fftlog_integral(ks, ps, 1.0, 0, 2, 8000, sep, kmin, kmax)
+        dt = time.perf_counter() - t1
+        g = integ.cpu().numpy()[0, 0, 1] / (5 * 2.0 / np.pi ** 2)
+        res["parity"] = {"checked": "spectrum 0, P, (l1, l2) = (0, 2), 256 x 256 against the oracle",
+                         "max_abs_over_max": float(np.max(np.abs(g - ref)) / np.max(np.abs(ref)))}
+        res["cpu_baseline"] = {"value": entries / (64 * 2 * plan.npair) / dt, "unit": "entries/s", "cores": 1,
+                               "kind": "port", "sample": f"one N=8000 2D FFTLog + bicubic of the oracle, {dt:.1f} s, "
+                                                         "scaled to the entries of one transform"}
+    print(json.dumps(res))
+    plan.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +242,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--workload", choices=["batched", "fullsky", "covariance"], default="batched",
+                    help="batched: the headline forward+adjoint batch; fullsky / covariance: the other "
+                         "configurations, reported on their own lines")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (5 times), to scale HBM counters")
     args = ap.parse_args()
@@ -142,6 +262,10 @@ def main():
         print("bench.py runs on one GPU: the engine is single-device (spectra shard across GPUs with no exchange; "
               "see DESIGN.md). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "fullsky":
+        return run_fullsky(args)
+    if args.workload == "covariance":
+        return run_covariance(args)
     import torch
     from coffe_amd.engine import Batch, Engine
     import oracle as O
@@ -235,7 +359,8 @@ def main():
                                "l=0,2,4 flat-sky den+RSD z=1, forward+adjoint",
                    "spectra": nspec, "separations": nsep, "multipoles": LS, "tables": TABLES,
                    "points_per_step": points, "fft_lengths": sorted(set(int(n) for n in n_fft)),
-                   "threads_per_wg": int(os.environ.get("COFFE_ENGINE_THREADS", 512))},
+                   "threads_per_wg": int(os.environ.get("COFFE_ENGINE_THREADS", 512)),
+                   "other_configs": "bench.py --workload fullsky | covariance"},
         "roofline": {"bound": "mfma", "kernel": dom,
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
