@@ -8,17 +8,55 @@
 
 namespace ce {
 
-// Akima value at xq on the table (x, y) of length n (n >= 5), GSL non-rounded ends
-__device__ inline double akima_general(const double *__restrict__ x, const double *__restrict__ y, int n, double xq)
+// interval j of the sorted table x[0..n) with x[j] <= v < x[j+1] (j = n - 2 at
+// v = x[n-1]), the result of a bisection over [0, n-1]; v must lie in [x[0], x[n-1]]
+__device__ inline int interval_bisect(const double *__restrict__ x, int lo, int hi, double v)
 {
-    if (!(xq >= x[0] && xq <= x[n - 1])) return __builtin_nan("");
-    int lo = 0, hi = n - 1;
     while (hi > lo + 1) {
         const int m = (lo + hi) >> 1;
-        if (x[m] > xq) hi = m;
+        if (x[m] > v) hi = m;
         else lo = m;
     }
-    const int j = lo;
+    return lo;
+}
+
+// the same interval, starting from a guess g: a few steps of walking settle a good
+// guess (exact for a uniform grid and a linear guess); a poor one falls back to
+// bisection on the bracket the walk has established
+__device__ inline int interval_from(const double *__restrict__ x, int n, double v, int g)
+{
+    g = g < 0 ? 0 : (g > n - 2 ? n - 2 : g);
+#pragma unroll 1
+    for (int it = 0; it < 6; ++it) {
+        if (x[g] > v) {
+            if (g == 0) return 0;
+            --g;
+        } else if (g < n - 2 && x[g + 1] <= v) {
+            ++g;
+        } else {
+            return g;
+        }
+    }
+    return (x[g] > v) ? interval_bisect(x, 0, g, v) : interval_bisect(x, g, n - 1, v);
+}
+
+// a real-valued index estimate as an int in [0, n - 1] (NaN and infinities included)
+__device__ inline int clamp_index(double t, int n)
+{
+    t = fmin(fmax(t, 0.0), (double)(n - 1));
+    return (t == t) ? (int)t : 0;
+}
+
+// linear guess of the interval of v on x[0..n) (exact for a uniform grid)
+__device__ inline int interval_guess(const double *__restrict__ x, int n, double v)
+{
+    return clamp_index((v - x[0]) / (x[n - 1] - x[0]) * (double)(n - 1), n);
+}
+
+// Akima value at xq in interval j of the table (x, y) of length n (n >= 5), GSL
+// non-rounded ends (slopes beyond the ends extended linearly)
+__device__ inline double akima_at(const double *__restrict__ x, const double *__restrict__ y, int n, int j, double xq)
+{
     auto slope = [&](int a) -> double {
         if (a >= 0 && a <= n - 2) return (y[a + 1] - y[a]) / (x[a + 1] - x[a]);
         const double m0 = (y[1] - y[0]) / (x[1] - x[0]);
@@ -48,6 +86,13 @@ __device__ inline double akima_general(const double *__restrict__ x, const doubl
     const double c = (3.0 * m0 - 2.0 * b - tn) / h;
     const double d = (b + tn - 2.0 * m0) / (h * h);
     return y[j] + delx * (b + delx * (c + d * delx));
+}
+
+// Akima value at xq on the table (x, y) of length n (n >= 5); NaN outside [x[0], x[n-1]]
+__device__ inline double akima_general(const double *__restrict__ x, const double *__restrict__ y, int n, double xq)
+{
+    if (!(xq >= x[0] && xq <= x[n - 1])) return __builtin_nan("");
+    return akima_at(x, y, n, interval_bisect(x, 0, n - 1, xq), xq);
 }
 
 }  // namespace ce

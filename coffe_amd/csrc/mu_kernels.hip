@@ -60,9 +60,42 @@ struct mu_args {
 };
 
 // coefficients c_s (standard slots) of the local terms at one (z, mu, r), times G
-__device__ void local_coefficients(const mu_args &a, double zm, double mu, double sep, double chi_mean,
+// the background at the end point chi of a pair: z(chi) and f, conformal H, G, D1 at
+// that z (one interval search on the chi table, started from a linear extrapolation of
+// the mean's interval, and one on the z grid shared by the four lookups), galaxy bias
+struct endpoint {
+    double z, f, H, G1, G2, D1, b1, b2;
+};
+
+__device__ __forceinline__ endpoint background_at(const mu_args &a, double chi, int jbar, double chib, double dchi)
+{
+    endpoint e;
+    const int n = a.nbg;
+    if (!(chi >= a.bg_chi[0] && chi <= a.bg_chi[n - 1])) {
+        e.z = e.f = e.H = e.G1 = e.G2 = e.D1 = e.b1 = e.b2 = __builtin_nan("");
+        return e;
+    }
+    const int jc = ce::interval_from(a.bg_chi, n, chi, ce::clamp_index(jbar + floor((chi - chib) / dchi), n));
+    e.z = ce::akima_at(a.bg_chi, a.bg_z, n, jc, chi);
+    const int k = ce::interval_from(a.bg_z, n, e.z, ce::interval_guess(a.bg_z, n, e.z));
+    e.f = ce::akima_at(a.bg_z, a.bg_f, n, k, e.z);
+    e.H = ce::akima_at(a.bg_z, a.bg_cH, n, k, e.z);
+    e.G1 = ce::akima_at(a.bg_z, a.bg_G1, n, k, e.z);
+    e.G2 = ce::akima_at(a.bg_z, a.bg_G2, n, k, e.z);
+    e.D1 = ce::akima_at(a.bg_z, a.bg_D1, n, k, e.z);
+    if (!(e.z >= a.bias_z[0] && e.z <= a.bias_z[a.nb - 1])) {
+        e.b1 = e.b2 = __builtin_nan("");
+    } else {
+        const int kb = ce::interval_from(a.bias_z, a.nb, e.z, ce::interval_guess(a.bias_z, a.nb, e.z));
+        e.b1 = ce::akima_at(a.bias_z, a.bias1, a.nb, kb, e.z);
+        e.b2 = ce::akima_at(a.bias_z, a.bias2, a.nb, kb, e.z);
+    }
+    return e;
+}
+
+__device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, double mu, double sep, double chi_mean,
                                    double fmean, double Hmean, double bm1, double bm2, double Gm1, double Gm2,
-                                   double Dmean, double *c)
+                                   double Dmean, int jbar, double chib, double dchi, double *c)
 {
     for (int s = 0; s < NSTD; ++s) c[s] = 0.0;
     const double chi1 = chi_mean - sep * mu / 2.;
@@ -70,12 +103,17 @@ __device__ void local_coefficients(const mu_args &a, double zm, double mu, doubl
     const double costheta = (2. * chi_mean * chi_mean - sep * sep + mu * mu * sep * sep / 2.) /
                             (2. * chi_mean * chi_mean - mu * mu * sep * sep / 2.);
     const double ct2 = costheta * costheta;
-    const double z1 = ce::akima_general(a.bg_chi, a.bg_z, a.nbg, chi1);
-    const double z2 = ce::akima_general(a.bg_chi, a.bg_z, a.nbg, chi2);
-    const double f1 = ce::akima_general(a.bg_z, a.bg_f, a.nbg, z1), f2 = ce::akima_general(a.bg_z, a.bg_f, a.nbg, z2);
-    const double H1 = ce::akima_general(a.bg_z, a.bg_cH, a.nbg, z1), H2 = ce::akima_general(a.bg_z, a.bg_cH, a.nbg, z2);
-    const double b1 = ce::akima_general(a.bias_z, a.bias1, a.nb, z1), b2 = ce::akima_general(a.bias_z, a.bias2, a.nb, z2);
-    const double G1 = ce::akima_general(a.bg_z, a.bg_G1, a.nbg, z1), G2 = ce::akima_general(a.bg_z, a.bg_G2, a.nbg, z2);
+    // the flat-sky forms use the mean's background only
+    endpoint e1, e2;
+    if (!a.flatsky) {
+        e1 = background_at(a, chi1, jbar, chib, dchi);
+        e2 = background_at(a, chi2, jbar, chib, dchi);
+    } else {
+        e1.z = e1.f = e1.H = e1.G1 = e1.G2 = e1.D1 = e1.b1 = e1.b2 = 0.0;
+        e2 = e1;
+    }
+    const double f1 = e1.f, f2 = e2.f, H1 = e1.H, H2 = e2.H, b1 = e1.b1, b2 = e2.b2;
+    const double G1 = e1.G1, G2 = e2.G2;
     const double P2 = legendre_p(2, mu), P4 = legendre_p(4, mu);
     if (a.terms & COFFE_TERM_DEN_DEN) {   // functions.c:129-229
         if (a.flatsky) c[S00] += bm1 * bm2;
@@ -141,8 +179,7 @@ __device__ void local_coefficients(const mu_args &a, double zm, double mu, doubl
                       pow(sep, 2);
         }
     }
-    const double G = a.flatsky ? Dmean * Dmean
-                               : ce::akima_general(a.bg_z, a.bg_D1, a.nbg, z1) * ce::akima_general(a.bg_z, a.bg_D1, a.nbg, z2);
+    const double G = a.flatsky ? Dmean * Dmean : e1.D1 * e2.D1;
     for (int s = 0; s < NSTD; ++s) c[s] *= G;
 }
 
@@ -161,6 +198,10 @@ __global__ void __launch_bounds__(256) k_mu_weights(mu_args a)
     const double bm1 = ce::akima_general(a.bias_z, a.bias1, a.nb, zm), bm2 = ce::akima_general(a.bias_z, a.bias2, a.nb, zm);
     const double Gm1 = ce::akima_general(a.bg_z, a.bg_G1, a.nbg, zm), Gm2 = ce::akima_general(a.bg_z, a.bg_G2, a.nbg, zm);
     const double Dmean = ce::akima_general(a.bg_z, a.bg_D1, a.nbg, zm);
+    // the mean's interval on the z grid (the chi table shares its indices) and the local
+    // slope of chi, from which the end points' chi intervals are extrapolated
+    const int jbar = ce::interval_bisect(a.bg_z, 0, a.nbg - 1, zm);
+    const double chib = a.bg_chi[jbar], dchi = a.bg_chi[jbar + 1] - a.bg_chi[jbar];
     // partial sums per (multipole, standard table slot); fixed bounds keep them in registers
     double acc[CE_MU_MAXL][NSTD];
 #pragma unroll
@@ -170,7 +211,7 @@ __global__ void __launch_bounds__(256) k_mu_weights(mu_args a)
     for (int q = lane; q < a.nmu; q += 64) {
         const double mu = a.mu[q], w = a.wmu[q];
         double c[NSTD];
-        local_coefficients(a, zm, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, c);
+        local_coefficients(a, zm, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, jbar, chib, dchi, c);
 #pragma unroll
         for (int il = 0; il < CE_MU_MAXL; ++il) {
             if (il < a.nl) {
