@@ -123,11 +123,13 @@ int ce_launch_cov_assemble(void *stream, int nspec, int nz, int nl, int R, int n
                            int poisson, int mixed, int cosmic, const double *d_integ, double *d_out);
 
 /* ---- mu-quadrature weights (mu_kernels.hip) ---- */
+/* coef: workspace of 3 (6 nbg + 2 nb) doubles for the end-point Akima coefficients */
 int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
                          const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
                          const double *bias_z, const double *bias1, const double *bias2, unsigned terms,
                          int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *slot,
-                         int nsep, const double *sep, int nmu, const double *mu, const double *wmu, double *A);
+                         int nsep, const double *sep, int nmu, const double *mu, const double *wmu, double *A,
+                         double *coef);
 
 #ifdef __cplusplus
 }

@@ -781,6 +781,32 @@ static void gauss_legendre(int n, double *x, double *w)
     if (n & 1) x[n / 2] = 0.0;
 }
 
+/* Gauss-Legendre rules, nodes then weights (2 n doubles), computed once per order for
+   the first 64 orders a process asks for (kept until exit); later orders are computed
+   per call into a buffer the caller frees (*owned = 1) */
+static const double *gl_nodes(int n, int *owned)
+{
+    static pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+    static struct { int n; double *xw; } cache[64];
+    static int ncache = 0;
+    const double *out = NULL;
+    *owned = 0;
+    pthread_mutex_lock(&mu);
+    for (int i = 0; i < ncache && !out; ++i)
+        if (cache[i].n == n) out = cache[i].xw;
+    if (!out) {
+        double *xw = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+        if (xw) {
+            gauss_legendre(n, xw, xw + n);
+            if (ncache < 64) { cache[ncache].n = n; cache[ncache].xw = xw; ++ncache; }
+            else *owned = 1;
+            out = xw;
+        }
+    }
+    pthread_mutex_unlock(&mu);
+    return out;
+}
+
 static int std_slot(int n, int l)
 {
     static const int tabs[8][2] = {{0, 0}, {0, 2}, {0, 4}, {1, 1}, {1, 3}, {2, 0}, {2, 2}, {3, 1}};
@@ -852,16 +878,20 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
     hipSetDevice(e->device);
     hipStream_t s = stream ? (hipStream_t)stream : e->stream;
     const size_t nbytes_bg = sizeof(double) * (size_t)nbg, nbytes_b = sizeof(double) * (size_t)nb;
-    const size_t total = 7 * nbytes_bg + 3 * nbytes_b + sizeof(double) * ((size_t)nz + (size_t)nsep + 2 * (size_t)nmu);
-    double *d = NULL;
-    if (dev_alloc((void **)&d, total) != COFFE_OK) return COFFE_ENOMEM;
-    double *h_mu = (double *)malloc(sizeof(double) * 2 * (size_t)nmu);
-    if (!h_mu) { hipFree(d); return COFFE_ENOMEM; }
-    gauss_legendre(nmu, h_mu, h_mu + nmu);
-    double *p = d;
+    const size_t ncoef = 3 * (6 * (size_t)nbg + 2 * (size_t)nb);
+    const size_t total = 7 * nbytes_bg + 3 * nbytes_b + sizeof(double) * ((size_t)nz + (size_t)nsep + 2 * (size_t)nmu) +
+                         sizeof(double) * ncoef;
+    int owned = 0;
+    const double *h_mu = gl_nodes(nmu, &owned);
+    if (!h_mu) return COFFE_ENOMEM;
+    /* the inputs are staged in the engine's scratch (the drop-in calls share it under the
+       lock); the call returns after the kernel has read them */
+    pthread_mutex_lock(&e->lock);
+    int st = scratch_reserve(e, total);
+    if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); return st; }
+    double *p = (double *)e->d_scratch;
     const double *src[10] = {bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, bias_z, bias1, bias2};
     double *dst[10];
-    int st = COFFE_OK;
     for (int i = 0; i < 10 && st == COFFE_OK; ++i) {
         const size_t nbyt = i < 7 ? nbytes_bg : nbytes_b;
         dst[i] = p;
@@ -874,12 +904,12 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_mu, h_mu, sizeof(double) * 2 * (size_t)nmu, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK &&
         ce_launch_mu_weights(s, nbg, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], nb, dst[7], dst[8], dst[9],
-                             terms, flatsky, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu, d_mu, d_w, d_A))
+                             terms, flatsky, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu, d_mu, d_w, d_A,
+                             d_w + nmu))
         st = COFFE_EDEVICE;
-    /* the staging buffer is released after the kernel has read it */
     if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
-    free(h_mu);
-    hipFree(d);
+    pthread_mutex_unlock(&e->lock);
+    if (owned) free((void *)h_mu);
     return st;
 }
 

@@ -34,7 +34,15 @@ __device__ __forceinline__ double legendre_p(int l, double x)
     return p1;
 }
 
+// integer powers of the local terms (functions.c writes them as pow(x, n)): x^2 is the
+// correctly rounded square either way; x^3 and x^4 by products differ from a library pow
+// in the last bits, far below the FP64 cancellation noise of the full-sky terms
+__device__ __forceinline__ double sq(double x) { return x * x; }
+__device__ __forceinline__ double cube(double x) { return x * x * x; }
+__device__ __forceinline__ double p4(double x) { return sq(sq(x)); }
+
 #define CE_MU_MAXL 4
+#define CE_MU_NCOEF 8
 
 // standard table slots (integrals.c:599-608)
 enum { S00 = 0, S02, S04, S11, S13, S20, S22, S31, NSTD };
@@ -57,7 +65,70 @@ struct mu_args {
     int nmu;
     const double *mu, *wmu;
     double *A;                 // [nz][nl][ntab][nsep]
+    // Akima coefficients (b, c, d per interval) of the end-point tables: z(chi), then
+    // f, conformal H, G1, G2, D1 on the z grid, then bias1, bias2 on bias_z
+    const double *cb[CE_MU_NCOEF], *cc[CE_MU_NCOEF], *cd_[CE_MU_NCOEF];
 };
+
+// y(xq) in interval j from precomputed Akima coefficients: the value akima_at returns
+__device__ __forceinline__ double akima_coef(const double *__restrict__ x, const double *__restrict__ y,
+                                             const double *__restrict__ b, const double *__restrict__ c,
+                                             const double *__restrict__ d, int j, double xq)
+{
+    const double delx = xq - x[j];
+    return y[j] + delx * (b[j] + delx * (c[j] + d[j] * delx));
+}
+
+struct coef_args {
+    const double *x[CE_MU_NCOEF], *y[CE_MU_NCOEF];
+    int n[CE_MU_NCOEF];
+    double *b[CE_MU_NCOEF], *c[CE_MU_NCOEF], *d[CE_MU_NCOEF];
+};
+
+// Akima (GSL, non-rounded ends) coefficients of every interval of the tables: one thread
+// per (interval, table); the same slopes and formulas as akima_at
+__global__ void __launch_bounds__(256) k_akima_coef(coef_args a)
+{
+    const int t = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.n[t];
+    if (j >= n - 1) return;
+    const double *x = a.x[t], *y = a.y[t];
+    auto slope = [&](int i) -> double {
+        if (i >= 0 && i <= n - 2) return (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+        const double m0 = (y[1] - y[0]) / (x[1] - x[0]);
+        const double m1 = (y[2] - y[1]) / (x[2] - x[1]);
+        const double mn2 = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+        const double mn3 = (y[n - 2] - y[n - 3]) / (x[n - 2] - x[n - 3]);
+        if (i == -1) return 2.0 * m0 - m1;
+        if (i == -2) return 3.0 * m0 - 2.0 * m1;
+        if (i == n - 1) return 2.0 * mn2 - mn3;
+        return 3.0 * mn2 - 2.0 * mn3;
+    };
+    const double mm2 = slope(j - 2), mm1 = slope(j - 1), m0 = slope(j), mp1 = slope(j + 1), mp2 = slope(j + 2);
+    const double ne = fabs(mp1 - m0) + fabs(mm1 - mm2);
+    double b, c, d;
+    if (ne == 0.0) {
+        b = m0; c = 0.0; d = 0.0;
+    } else {
+        const double h = x[j + 1] - x[j];
+        const double nen = fabs(mp2 - mp1) + fabs(m0 - mm1);
+        const double alpha = fabs(mm1 - mm2) / ne;
+        double tn;
+        if (nen == 0.0) {
+            tn = m0;
+        } else {
+            const double an = fabs(m0 - mm1) / nen;
+            tn = (1.0 - an) * m0 + an * mp1;
+        }
+        b = (1.0 - alpha) * mm1 + alpha * m0;
+        c = (3.0 * m0 - 2.0 * b - tn) / h;
+        d = (b + tn - 2.0 * m0) / (h * h);
+    }
+    a.b[t][j] = b;
+    a.c[t][j] = c;
+    a.d[t][j] = d;
+}
 
 // coefficients c_s (standard slots) of the local terms at one (z, mu, r), times G
 // the background at the end point chi of a pair: z(chi) and f, conformal H, G, D1 at
@@ -76,19 +147,19 @@ __device__ __forceinline__ endpoint background_at(const mu_args &a, double chi, 
         return e;
     }
     const int jc = ce::interval_from(a.bg_chi, n, chi, ce::clamp_index(jbar + floor((chi - chib) / dchi), n));
-    e.z = ce::akima_at(a.bg_chi, a.bg_z, n, jc, chi);
+    e.z = akima_coef(a.bg_chi, a.bg_z, a.cb[0], a.cc[0], a.cd_[0], jc, chi);
     const int k = ce::interval_from(a.bg_z, n, e.z, ce::interval_guess(a.bg_z, n, e.z));
-    e.f = ce::akima_at(a.bg_z, a.bg_f, n, k, e.z);
-    e.H = ce::akima_at(a.bg_z, a.bg_cH, n, k, e.z);
-    e.G1 = ce::akima_at(a.bg_z, a.bg_G1, n, k, e.z);
-    e.G2 = ce::akima_at(a.bg_z, a.bg_G2, n, k, e.z);
-    e.D1 = ce::akima_at(a.bg_z, a.bg_D1, n, k, e.z);
+    e.f = akima_coef(a.bg_z, a.bg_f, a.cb[1], a.cc[1], a.cd_[1], k, e.z);
+    e.H = akima_coef(a.bg_z, a.bg_cH, a.cb[2], a.cc[2], a.cd_[2], k, e.z);
+    e.G1 = akima_coef(a.bg_z, a.bg_G1, a.cb[3], a.cc[3], a.cd_[3], k, e.z);
+    e.G2 = akima_coef(a.bg_z, a.bg_G2, a.cb[4], a.cc[4], a.cd_[4], k, e.z);
+    e.D1 = akima_coef(a.bg_z, a.bg_D1, a.cb[5], a.cc[5], a.cd_[5], k, e.z);
     if (!(e.z >= a.bias_z[0] && e.z <= a.bias_z[a.nb - 1])) {
         e.b1 = e.b2 = __builtin_nan("");
     } else {
         const int kb = ce::interval_from(a.bias_z, a.nb, e.z, ce::interval_guess(a.bias_z, a.nb, e.z));
-        e.b1 = ce::akima_at(a.bias_z, a.bias1, a.nb, kb, e.z);
-        e.b2 = ce::akima_at(a.bias_z, a.bias2, a.nb, kb, e.z);
+        e.b1 = akima_coef(a.bias_z, a.bias1, a.cb[6], a.cc[6], a.cd_[6], kb, e.z);
+        e.b2 = akima_coef(a.bias_z, a.bias2, a.cb[7], a.cc[7], a.cd_[7], kb, e.z);
     }
     return e;
 }
@@ -128,19 +199,19 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
             c[S00] += f1 * f2 * (1 + 2 * ct2) / 15;
             c[S02] += -f1 * f2 / 21. * ((1 + 11. * ct2) + 18 * costheta * (ct2 - 1) * chi1 * chi2 / sep / sep);
             c[S04] += f1 * f2 *
-                      (4 * (3 * ct2 - 1) * (pow(chi1, 4) + pow(chi2, 4)) / 35. / pow(sep, 4) +
+                      (4 * (3 * ct2 - 1) * (p4(chi1) + p4(chi2)) / 35. / p4(sep) +
                        chi1 * chi2 * (3 + ct2) * (3 * (3 + ct2) * chi1 * chi2 - 8 * (chi1 * chi1 + chi2 * chi2) * costheta) /
-                           35. / pow(sep, 4));
+                           35. / p4(sep));
         }
     }
     if (a.terms & COFFE_TERM_D1_D1) {   // functions.c:363-573
         if (a.flatsky) {
             c[S20] += Hmean * Hmean * fmean * fmean * Gm1 * Gm2 / 3.;
-            c[S22] += 2 * Hmean * Hmean * fmean * fmean * Gm1 * Gm2 * pow(sep, 2) * P2 / 3.;
+            c[S22] += 2 * Hmean * Hmean * fmean * fmean * Gm1 * Gm2 * sq(sep) * P2 / 3.;
         } else {
             c[S20] += H1 * H2 * f1 * f2 * G1 * G2 * costheta / 3.;
             c[S22] += -H1 * H2 * f1 * f2 * G1 * G2 *
-                      ((chi2 - chi1 * costheta) * (chi1 - chi2 * costheta) + pow(sep, 2) * costheta / 3.);
+                      ((chi2 - chi1 * costheta) * (chi1 - chi2 * costheta) + sq(sep) * costheta / 3.);
         }
     }
     if (a.terms & COFFE_TERM_DEN_RSD) {   // functions.c:670-771
@@ -149,8 +220,8 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
             c[S02] += -(2 * bm1 * fmean / 3. + 2 * bm2 * fmean / 3.) * P2;
         } else {
             c[S00] += b1 * f2 / 3. + b2 * f1 / 3.;
-            c[S02] += -(b1 * f2 * (2. / 3. - (1. - ct2) * pow(chi1 / sep, 2)) +
-                        b2 * f1 * (2. / 3. - (1. - ct2) * pow(chi2 / sep, 2)));
+            c[S02] += -(b1 * f2 * (2. / 3. - (1. - ct2) * sq(chi1 / sep)) +
+                        b2 * f1 * (2. / 3. - (1. - ct2) * sq(chi2 / sep)));
         }
     }
     if (a.terms & COFFE_TERM_DEN_D1) {   // functions.c:772-855
@@ -169,14 +240,14 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
             c[S11] += f1 * f2 * H2 * G2 * ((1. + 2 * ct2) * chi2 - 3 * chi1 * costheta) / 5. +
                       f2 * f1 * H1 * G1 * ((1. + 2 * ct2) * chi1 - 3 * chi2 * costheta) / 5.;
             c[S13] += (f1 * f2 * H2 * G2 *
-                           ((1. - 3 * ct2) * pow(chi2, 3) + costheta * (5. + ct2) * pow(chi2, 2) * chi1 -
-                            2 * (2. + ct2) * chi2 * pow(chi1, 2) + 2 * pow(chi1, 3) * costheta) /
+                           ((1. - 3 * ct2) * cube(chi2) + costheta * (5. + ct2) * sq(chi2) * chi1 -
+                            2 * (2. + ct2) * chi2 * sq(chi1) + 2 * cube(chi1) * costheta) /
                            5 +
                        f2 * f1 * H1 * G1 *
-                           ((1. - 3 * ct2) * pow(chi1, 3) + costheta * (5. + ct2) * pow(chi1, 2) * chi2 -
-                            2 * (2. + ct2) * chi1 * pow(chi2, 2) + 2 * pow(chi2, 3) * costheta) /
+                           ((1. - 3 * ct2) * cube(chi1) + costheta * (5. + ct2) * sq(chi1) * chi2 -
+                            2 * (2. + ct2) * chi1 * sq(chi2) + 2 * cube(chi2) * costheta) /
                            5) /
-                      pow(sep, 2);
+                      sq(sep);
         }
     }
     const double G = a.flatsky ? Dmean * Dmean : e1.D1 * e2.D1;
@@ -244,10 +315,27 @@ extern "C" int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, c
                                     const double *bg_G2, int nb, const double *bias_z, const double *bias1,
                                     const double *bias2, unsigned terms, int flatsky, int nz,
                                     const double *zmean, int nl, const int *ls, int ntab, const int *slot, int nsep,
-                                    const double *sep, int nmu, const double *mu, const double *wmu, double *A)
+                                    const double *sep, int nmu, const double *mu, const double *wmu, double *A,
+                                    double *coef)
 {
     if (nl > CE_MU_MAXL || ntab > CE_MAX_TABLES) return (int)hipErrorInvalidValue;
     mu_args a;
+    coef_args ca;
+    {
+        const double *xs[CE_MU_NCOEF] = {bg_chi, bg_z, bg_z, bg_z, bg_z, bg_z, bias_z, bias_z};
+        const double *ys[CE_MU_NCOEF] = {bg_z, bg_f, bg_cH, bg_G1, bg_G2, bg_D1, bias1, bias2};
+        double *p = coef;
+        int nmax = 0;
+        for (int t = 0; t < CE_MU_NCOEF; ++t) {
+            const int n = t < 6 ? nbg : nb;
+            ca.x[t] = xs[t]; ca.y[t] = ys[t]; ca.n[t] = n;
+            ca.b[t] = p; ca.c[t] = p + n; ca.d[t] = p + 2 * n;
+            a.cb[t] = ca.b[t]; a.cc[t] = ca.c[t]; a.cd_[t] = ca.d[t];
+            p += 3 * n;
+            if (n > nmax) nmax = n;
+        }
+        hipLaunchKernelGGL(k_akima_coef, dim3((nmax + 255) / 256, CE_MU_NCOEF), dim3(256), 0, (hipStream_t)stream, ca);
+    }
     a.nbg = nbg; a.bg_z = bg_z; a.bg_chi = bg_chi; a.bg_D1 = bg_D1; a.bg_f = bg_f; a.bg_cH = bg_cH;
     a.bg_G1 = bg_G1; a.bg_G2 = bg_G2;
     a.nb = nb; a.bias_z = bias_z; a.bias1 = bias1; a.bias2 = bias2;
