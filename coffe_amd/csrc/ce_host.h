@@ -24,6 +24,9 @@ struct coffe_engine {
     /* scratch of the drop-in path */
     void *d_scratch;
     size_t scratch_bytes;
+    /* pinned host staging for uploads of many small host arrays in one copy */
+    void *h_pinned;
+    size_t pinned_bytes;
 };
 
 /* COFFE_OK or COFFE_EDEVICE */

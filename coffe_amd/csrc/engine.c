@@ -271,6 +271,7 @@ void coffe_engine_destroy(coffe_engine *e)
     ce_grid *g = e->grids;
     while (g) { ce_grid *n = g->next; grid_free(g); g = n; }
     hipFree(e->d_scratch);
+    hipHostFree(e->h_pinned);
     hipStreamDestroy(e->stream);
     pthread_mutex_destroy(&e->lock);
     free(e);
@@ -866,15 +867,20 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
         if (ls[i] < 0) return COFFE_EINVAL;
     /* chi(z_mean) -/+ sep/2 must stay inside the chi table at every |mu| <= 1 and
        z_mean inside the bias table (the reference stops on a GSL domain error) */
-    for (int iz = 0; iz < nz; ++iz) {
-        double chim;
+    for (int iz = 0; iz < nz; ++iz)
         if (zmean[iz] < bg_z[0] || zmean[iz] > bg_z[nbg - 1] || zmean[iz] < bias_z[0] || zmean[iz] > bias_z[nb - 1])
             return COFFE_EDOMAIN;
-        if (coffe_akima_eval((size_t)nbg, bg_z, bg_chi, 1, &zmean[iz], &chim) != COFFE_OK) return COFFE_EINVAL;
-        for (int r = 0; r < nsep; ++r)
-            if (!(sep[r] > 0.0) || chim - sep[r] / 2 < bg_chi[0] || chim + sep[r] / 2 > bg_chi[nbg - 1])
-                return COFFE_EDOMAIN;
+    double *chim = (double *)malloc(sizeof(double) * (size_t)nz);
+    if (!chim) return COFFE_ENOMEM;
+    if (coffe_akima_eval((size_t)nbg, bg_z, bg_chi, (size_t)nz, zmean, chim) != COFFE_OK) { free(chim); return COFFE_EINVAL; }
+    double smax = 0.0;
+    for (int r = 0; r < nsep; ++r) {
+        if (!(sep[r] > 0.0)) { free(chim); return COFFE_EDOMAIN; }
+        if (sep[r] > smax) smax = sep[r];
     }
+    for (int iz = 0; iz < nz; ++iz)
+        if (!(chim[iz] - smax / 2 >= bg_chi[0] && chim[iz] + smax / 2 <= bg_chi[nbg - 1])) { free(chim); return COFFE_EDOMAIN; }
+    free(chim);
     hipSetDevice(e->device);
     hipStream_t s = stream ? (hipStream_t)stream : e->stream;
     const size_t nbytes_bg = sizeof(double) * (size_t)nbg, nbytes_b = sizeof(double) * (size_t)nb;
@@ -889,19 +895,35 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
     pthread_mutex_lock(&e->lock);
     int st = scratch_reserve(e, total);
     if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); return st; }
+    /* every host input goes through one pinned staging buffer and one copy */
+    const size_t nstage = total / sizeof(double) - ncoef;   /* everything but the coefficient workspace */
+    if (st == COFFE_OK && e->pinned_bytes < sizeof(double) * nstage) {
+        hipHostFree(e->h_pinned);
+        e->h_pinned = NULL;
+        e->pinned_bytes = 0;
+        if (hipHostMalloc(&e->h_pinned, sizeof(double) * nstage, hipHostMallocDefault) == hipSuccess)
+            e->pinned_bytes = sizeof(double) * nstage;
+        else
+            st = COFFE_ENOMEM;
+    }
     double *p = (double *)e->d_scratch;
     const double *src[10] = {bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, bias_z, bias1, bias2};
     double *dst[10];
+    double *hp = (double *)e->h_pinned;
     for (int i = 0; i < 10 && st == COFFE_OK; ++i) {
-        const size_t nbyt = i < 7 ? nbytes_bg : nbytes_b;
+        const size_t nv = (i < 7 ? nbytes_bg : nbytes_b) / sizeof(double);
         dst[i] = p;
-        st = hip_ok(hipMemcpyAsync(p, src[i], nbyt, hipMemcpyHostToDevice, s));
-        p += nbyt / sizeof(double);
+        memcpy(hp, src[i], sizeof(double) * nv);
+        p += nv;
+        hp += nv;
     }
     double *d_z = p, *d_sep = p + nz, *d_mu = d_sep + nsep, *d_w = d_mu + nmu;
-    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_z, zmean, sizeof(double) * (size_t)nz, hipMemcpyHostToDevice, s));
-    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_sep, sep, sizeof(double) * (size_t)nsep, hipMemcpyHostToDevice, s));
-    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d_mu, h_mu, sizeof(double) * 2 * (size_t)nmu, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) {
+        memcpy(hp, zmean, sizeof(double) * (size_t)nz);
+        memcpy(hp + nz, sep, sizeof(double) * (size_t)nsep);
+        memcpy(hp + nz + nsep, h_mu, sizeof(double) * 2 * (size_t)nmu);
+        st = hip_ok(hipMemcpyAsync(e->d_scratch, e->h_pinned, sizeof(double) * nstage, hipMemcpyHostToDevice, s));
+    }
     if (st == COFFE_OK &&
         ce_launch_mu_weights(s, nbg, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], nb, dst[7], dst[8], dst[9],
                              terms, flatsky, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu, d_mu, d_w, d_A,
