@@ -32,6 +32,15 @@ PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
 
 
+def traffic_of(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of `workload` (None when absent)"""
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(tfile):
+        return None
+    return json.load(open(tfile)).get(kernel)
+
+
 def build_inputs(nspec=256, nsep=512, seed=0):
     from tests.synth import batch_spectra, flat_weights
     from coffe_amd import background as B
@@ -150,6 +159,18 @@ def run_fullsky(args):
         out = step()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the mu-weight launch timed on its stream: 10 calls back to back between two HIP
+    # events (each call = one upload of the inputs, k_akima_coef, k_mu_weights)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    reps = 10
+    ev[0].record(stream)
+    for _ in range(reps):
+        mu_weights(eng, bg, zb, b, b, zs, LS, tabs, sep, den=1, rsd=1, d1=1, flatsky=False, nmu=256, out=A)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    mu_us = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+    mu_flops = 600.0 * len(zs) * len(LS) * len(sep) * 256
     got = out.cpu().numpy()[0]
     otabs = {t: O.integral_table(t[0], t[1], k, pk, len(k), small_r=False) for t in tabs}
     ref = O.multipoles(otabs, bg, zb, b, b, zs, LS, sep, den=1, rsd=1, d1=1, flatsky=False, quad=256)
@@ -167,6 +188,12 @@ def run_fullsky(args):
         "data": "synthetic",
         "config": {"workload": "full-sky den+RSD+d1, one P(k) with N=16384, 8 tables, z=linspace(0.5,2,8), "
                                "512 seps 5-300 Mpc/h, GL-256; step = mu weights + tables + projection"},
+        "roofline": {"bound": "mfma", "kernel": "k_mu_weights", "achieved": mu_flops / (mu_us * 1e-6) / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": mu_flops / (mu_us * 1e-6) / PEAK_FP64,
+                     "traffic": traffic_of("fullsky_8z_512sep", "k_mu_weights"), "launch_us": mu_us,
+                     "note": "600 flop per (z, l, r, mu node) x 8 z x 3 l x 512 r x 256 nodes per launch; FP64 VALU "
+                             "bound (no matrix work), peak = MI355X FP64 78.6 TFLOP/s; launch time from HIP events "
+                             "over 10 back-to-back calls, each including the input upload and k_akima_coef"},
         "parity": {"points": points, "within_1e-8": good, "max_rel": float(rel.max()),
                    "note": "GL-256 oracle; FP64 cancellation noise ~eps (chi/r)^4 at the smallest separations"},
         "cpu_baseline": {"value": cpu_pts, "unit": "points/s", "cores": 1, "kind": "port",
@@ -209,6 +236,21 @@ def run_covariance(args):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     entries = cov.numel()
+    # the integrals launch (k_cov_diag, k_cov_rows, k_cov_cols) timed on its stream with
+    # HIP events over 3 calls back to back; k_cov_rows is the dominant kernel
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    reps = 3
+    ev[0].record(stream)
+    for _ in range(reps):
+        plan.integrals(k, p, n_in, out=integ)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    rows_us = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+    N = 8000
+    row_flops = 7.5 * N * np.log2(N) + 8.0 * N + 2.0 * 72 * len(sep)
+    nrows_launch = 64 * 2 * plan.npair * plan.nrows
+    cov_flops = row_flops * nrows_launch
     res = {
         "metric": "Gaussian covariance entries/sec (64 P(k), 2D FFTLog N=8000, 4 z x (3 l x 256 r)^2), 1 MI355X",
         "value": entries / (ms / 1e3), "unit": "entries/s", "n_gpus": 1, "steps": args.steps,
@@ -217,6 +259,13 @@ def run_covariance(args):
         "config": {"workload": "64 seeded P(k) (4096 samples), 2D FFTLog N=8000, l=0,2,4, 256 seps 10-200 Mpc/h, "
                                "4 z bins; step = P and P^2 integrals of 6 pairs + assembly",
                    "transform_rows_per_pair": plan.nrows, "transforms_per_step": 64 * 2 * plan.npair},
+        "roofline": {"bound": "mfma", "kernel": "k_cov_rows", "achieved": cov_flops / (rows_us * 1e-6) / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": cov_flops / (rows_us * 1e-6) / PEAK_FP64,
+                     "traffic": traffic_of("covariance_64spec_n8000", "k_cov_rows"), "launch_us": rows_us,
+                     "note": "per transform row 7.5 N log2 N (complex FFT_N + c2r) + 8 N (kernel products) + "
+                             "2 x 72 x R (column functionals), N = 8000, R = 256, x 64 spectra x 2 powers x 6 pairs x "
+                             "rows per pair; time = HIP events over 3 integrals calls (k_cov_diag + k_cov_rows + "
+                             "k_cov_cols, rows > 95 % in the kernel trace); FP64 peak 78.6 TFLOP/s"},
     }
     if args.cpu_sample:
         ks, ps = specs[0][0], specs[0][1]
@@ -338,10 +387,7 @@ def main():
     bytes_ = hbm_bytes(n_fft, n_in, nsep, len(TABLES), nspec, 1, len(LS))
     dom = max(kernels, key=lambda kk: kernel_ms[kk])
     achieved = flops[dom] / (kernel_ms[dom] / 1e3)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_c3.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(dom)
+    traffic = traffic_of("batched_256spec", dom)
     res = {
         "metric": "(l,r) multipole points/sec, batched fwd+adjoint vs plain FFTLog, 1 MI355X @1e-8 rel",
         "value": value,

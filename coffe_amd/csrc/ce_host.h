@@ -27,6 +27,10 @@ struct coffe_engine {
     /* pinned host staging for uploads of many small host arrays in one copy */
     void *h_pinned;
     size_t pinned_bytes;
+    /* asynchronous users of the scratch: ev_scratch follows the last kernel that reads
+       it, ev_stage the last copy out of the pinned staging (valid when *_pending) */
+    hipEvent_t ev_scratch, ev_stage;
+    int scratch_pending, stage_pending;
 };
 
 /* COFFE_OK or COFFE_EDEVICE */

@@ -260,6 +260,12 @@ int coffe_engine_create(int device, coffe_engine **out)
         free(e);
         return COFFE_EDEVICE;
     }
+    if (hipEventCreateWithFlags(&e->ev_scratch, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_stage, hipEventDisableTiming) != hipSuccess) {
+        hipStreamDestroy(e->stream);
+        free(e);
+        return COFFE_EDEVICE;
+    }
     *out = e;
     return COFFE_OK;
 }
@@ -268,10 +274,14 @@ void coffe_engine_destroy(coffe_engine *e)
 {
     if (!e) return;
     hipStreamSynchronize(e->stream);
+    if (e->scratch_pending) hipEventSynchronize(e->ev_scratch);
+    if (e->stage_pending) hipEventSynchronize(e->ev_stage);
     ce_grid *g = e->grids;
     while (g) { ce_grid *n = g->next; grid_free(g); g = n; }
     hipFree(e->d_scratch);
     hipHostFree(e->h_pinned);
+    hipEventDestroy(e->ev_scratch);
+    hipEventDestroy(e->ev_stage);
     hipStreamDestroy(e->stream);
     pthread_mutex_destroy(&e->lock);
     free(e);
@@ -642,9 +652,17 @@ int coffe_batch_tables(coffe_batch *b, const double *d_k, const double *d_pk, do
 
 /* ---- drop-in: twofast_1bessel ------------------------------------------------ */
 
-static int scratch_reserve(coffe_engine *e, size_t bytes)
+/* the scratch of at least `bytes`, for work on stream s (under e->lock): work on s
+   starts after the last asynchronous kernel that read the scratch, and a reallocation
+   waits for that kernel */
+static int scratch_reserve(coffe_engine *e, size_t bytes, hipStream_t s)
 {
+    if (e->scratch_pending && hip_ok(hipStreamWaitEvent(s, e->ev_scratch, 0)) != COFFE_OK) return COFFE_EDEVICE;
     if (e->scratch_bytes >= bytes) return COFFE_OK;
+    if (e->scratch_pending) {
+        if (hip_ok(hipEventSynchronize(e->ev_scratch)) != COFFE_OK) return COFFE_EDEVICE;
+        e->scratch_pending = 0;
+    }
     hipFree(e->d_scratch);
     e->d_scratch = NULL;
     e->scratch_bytes = 0;
@@ -681,7 +699,7 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     const size_t meta_bytes = sizeof(ce_spec_dev) + sizeof(ce_grid_dev) + sizeof(ce_table_desc) +
                               sizeof(ce_unit_dev) + sizeof(ce_chunk_dev) * (size_t)nchunk + 64;
     pthread_mutex_lock(&e->lock);
-    st = scratch_reserve(e, sizeof(double) * off_meta + meta_bytes + 64);
+    st = scratch_reserve(e, sizeof(double) * off_meta + meta_bytes + 64, e->stream);
     if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); return st; }
     double *base = (double *)e->d_scratch;
     char *meta = (char *)(base + off_meta);
@@ -891,10 +909,16 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
     const double *h_mu = gl_nodes(nmu, &owned);
     if (!h_mu) return COFFE_ENOMEM;
     /* the inputs are staged in the engine's scratch (the drop-in calls share it under the
-       lock); the call returns after the kernel has read them */
+       lock).  The call is asynchronous on the stream: the next user of the scratch waits
+       for this kernel (ev_scratch) and the next writer of the pinned staging for this
+       copy (ev_stage) */
     pthread_mutex_lock(&e->lock);
-    int st = scratch_reserve(e, total);
-    if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); return st; }
+    int st = scratch_reserve(e, total, s);
+    if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); if (owned) free((void *)h_mu); return st; }
+    if (e->stage_pending) {
+        st = hip_ok(hipEventSynchronize(e->ev_stage));
+        e->stage_pending = 0;
+    }
     /* every host input goes through one pinned staging buffer and one copy */
     const size_t nstage = total / sizeof(double) - ncoef;   /* everything but the coefficient workspace */
     if (st == COFFE_OK && e->pinned_bytes < sizeof(double) * nstage) {
@@ -923,13 +947,18 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
         memcpy(hp + nz, sep, sizeof(double) * (size_t)nsep);
         memcpy(hp + nz + nsep, h_mu, sizeof(double) * 2 * (size_t)nmu);
         st = hip_ok(hipMemcpyAsync(e->d_scratch, e->h_pinned, sizeof(double) * nstage, hipMemcpyHostToDevice, s));
+        if (st == COFFE_OK && hip_ok(hipEventRecord(e->ev_stage, s)) == COFFE_OK) e->stage_pending = 1;
     }
     if (st == COFFE_OK &&
         ce_launch_mu_weights(s, nbg, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], nb, dst[7], dst[8], dst[9],
                              terms, flatsky, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu, d_mu, d_w, d_A,
                              d_w + nmu))
         st = COFFE_EDEVICE;
-    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    if (st == COFFE_OK) {
+        st = hip_ok(hipEventRecord(e->ev_scratch, s));
+        e->scratch_pending = (st == COFFE_OK);
+    }
+    if (st != COFFE_OK) hipStreamSynchronize(s);   /* nothing of a failed call stays in flight */
     pthread_mutex_unlock(&e->lock);
     if (owned) free((void *)h_mu);
     return st;
@@ -995,7 +1024,7 @@ int coffe_integral_table(int n, int l, size_t N, const double *k, const double *
     const size_t off_out = 2 * nin, off_meta = off_out + nodes;
     const size_t meta = sizeof(hs) + sizeof(hg) + sizeof(td) + sizeof(int) * nodes + 64;
     pthread_mutex_lock(&e->lock);
-    st = scratch_reserve(e, sizeof(double) * off_meta + meta);
+    st = scratch_reserve(e, sizeof(double) * off_meta + meta, e->stream);
     if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); return st; }
     double *base = (double *)e->d_scratch;
     char *m = (char *)(base + off_meta);

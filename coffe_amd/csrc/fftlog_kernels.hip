@@ -150,101 +150,147 @@ __global__ void k_grid_tables(grid_tab_args a)
 }
 
 // ---- natural cubic spline and resampling, chunked over the input nodes ---------
-// A chunk owns the input intervals [a, b) of one spectrum.  It stages the nodes
-// [a - W - 1, b + W + 1] in LDS, solves the natural-spline system (unknowns
-// c_i = y''_i / 2, i = 1..n-2, twofast.c:399-401) restricted to the nodes
-// [max(1, a - W), min(n - 2, b + W)] by parallel cyclic reduction, and keeps the
-// coefficients of [a, b].  Cutting the system W rows away perturbs the kept rows
-// by at most prod gamma <= 2^-W relative (the matrix is strictly diagonally
-// dominant, every LDL^T multiplier gamma is below 1/2), 5e-20 for W = 64; the
-// global ends carry the exact natural boundary c_0 = c_{n-1} = 0.
+// A chunk owns the input intervals [a, b) of one spectrum and is handled by one
+// wavefront.  It stages the nodes [a - W - 1, b + W + 1] in LDS, solves the natural-
+// spline system (unknowns c_i = y''_i / 2, i = 1..n-2, twofast.c:399-401) restricted
+// to the nodes [max(1, a - W), min(n - 2, b + W)], and keeps the coefficients of
+// [a, b].  Cutting the system W rows away perturbs the kept rows by at most
+// prod gamma <= 2^-W relative (the matrix is strictly diagonally dominant, every
+// LDL^T multiplier gamma is below 1/2), 5e-20 for W = 64; the global ends carry the
+// exact natural boundary c_0 = c_{n-1} = 0.
 
 #define CE_SPLINE_HALO 64
+// rows per lane of the one-wavefront solve: 64 lanes hold the largest truncated
+// system, CE_CHUNK + 2 CE_SPLINE_HALO + 1 rows
+#define CE_SOLVE_M 11
+static_assert(64 * CE_SOLVE_M >= CE_CHUNK + 2 * CE_SPLINE_HALO + 1, "spline solve capacity");
+static_assert(CE_CHUNK_NODES >= CE_CHUNK + 2 * CE_SPLINE_HALO + 3, "spline staging capacity");
 
-// Cyclic reduction of a_i x_{i-1} + b_i x_i + c_i x_{i+1} = d_i, i < ns, in place in
-// LDS (A, B, C, D).  Step s eliminates x_{i -+ s} from row i with the rows i -+ s:
-//     row_i <- bL bR row_i - a bR row_L - c bL row_R,
-// which needs no division; the new row is then scaled by a power of two (exponent of
-// its diagonal), so magnitudes stay near 1 and the scaling is exact.  This is the
-// usual normalised reduction multiplied through by bL bR.  After s steps row i couples
-// only to rows i +- 2^s with weights that shrink doubly exponentially: the natural-
-// spline matrix has (|a| + |c|)/|b| = 1/2, and the ratio goes 1/2 -> 0.14 -> 0.01 ->
-// 5e-5 -> 1e-9 -> 1e-18 on a uniform grid.  CE_PCR_STEPS = 7 reductions therefore
-// leave couplings far below double rounding for any grid, and x_i = d_i / b_i.  On
-// exit D holds x.  All threads must call it; ns <= CE_PCR_PER * blockDim.x.
-#define CE_PCR_STEPS 7
-#define CE_PCR_PER 5
-__device__ void pcr_solve(double *__restrict__ A, double *__restrict__ B, double *__restrict__ C,
-                          double *__restrict__ D, int ns)
+// The natural-spline system of the staged nodes xs (row r = node u0 + r, staged at
+// r + 1; h_r = xs[r+1] - xs[r]):
+//     h_r c_{r-1} + 2 (h_r + h_{r+1}) c_r + h_{r+1} c_{r+1} = d_r,   r < ns,
+// without the left term in row 0 and the right term in row ns - 1, solved by one
+// wavefront as a partitioned system.  Lane p owns rows [p M, p M + M); the last is
+// its separator s_p.  Its interior rows are eliminated in registers (Thomas, with
+// the two coupling columns as extra right-hand sides):
+//     x_j = y_j - v_j s_{p-1} - w_j s_p,
+// so the separator rows form a 64-row tridiagonal system, solved exactly by six
+// steps of cyclic reduction across the lanes (lane shuffles; each step forms
+// bL bR row_i - a bR row_L - c bL row_R and rescales it by a power of two, which
+// needs no division and is exact).  Rows past ns are identity rows.  rhs(r, hm, hp,
+// first) gives d_r, called in increasing r per lane (first on the lane's first
+// row).  The solution goes to X[r], r < ns; every lane of the wavefront must call.
+template <class Rhs>
+__device__ __forceinline__ void wave_spline_solve(const double *xs, int ns, Rhs rhs, double *X)
 {
-    const int T = blockDim.x, tid = threadIdx.x;
-    int stride = 1;
-    for (int step = 0; step < CE_PCR_STEPS && stride < ns; ++step, stride <<= 1) {
-        double na[CE_PCR_PER], nb[CE_PCR_PER], nc[CE_PCR_PER], nd[CE_PCR_PER];
-#pragma unroll
-        for (int q = 0; q < CE_PCR_PER; ++q) {
-            const int i = tid + q * T;
-            if (i < ns) {
-                const int il = i - stride, ir = i + stride;
-                const double a = A[i], b = B[i], c = C[i], d = D[i];
-                double aL = 0.0, bL = 1.0, cL = 0.0, dL = 0.0, aR = 0.0, bR = 1.0, cR = 0.0, dR = 0.0;
-                if (il >= 0) { aL = A[il]; bL = B[il]; cL = C[il]; dL = D[il]; }
-                if (ir < ns) { aR = A[ir]; bR = B[ir]; cR = C[ir]; dR = D[ir]; }
-                const double u = a * bR, v = c * bL;
-                double ra = -u * aL;
-                double rb = b * bL * bR - u * cL - v * aR;
-                double rc = -v * cR;
-                double rd = d * bL * bR - u * dL - v * dR;
-                const int e = -__builtin_amdgcn_frexp_exp(rb);
-                na[q] = ldexp(ra, e); nb[q] = ldexp(rb, e); nc[q] = ldexp(rc, e); nd[q] = ldexp(rd, e);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < CE_PCR_PER; ++q) {
-            const int i = tid + q * T;
-            if (i < ns) { A[i] = na[q]; B[i] = nb[q]; C[i] = nc[q]; D[i] = nd[q]; }
-        }
-        __syncthreads();
-    }
-    for (int i = tid; i < ns; i += T) D[i] = D[i] / B[i];
-    __syncthreads();
-}
-
-// First resample index i with k0 p_i >= v (p increasing; N if none), found by one
-// wavefront: 64 probes per round narrow [lo, hi) 64-fold,
-// so N = 16384 takes three dependent loads instead of fourteen.  Every lane of the
-// calling wavefront must call it; all lanes return the index.
-__device__ __forceinline__ int wave_lower_index(const double *p, int N, double k0, double v)
-{
+    constexpr int M = CE_SOLVE_M;
     const int lane = threadIdx.x & 63;
-    int lo = 0, hi = N;   // answer in [lo, hi]
-    while (hi - lo > 64) {
-        const int step = (hi - lo + 63) / 64;
-        const int idx = lo + lane * step;
-        const bool below = idx < hi && k0 * p[idx] < v;
-        const int c = __popcll(__ballot(below));
-        const int nlo = (c > 0) ? lo + (c - 1) * step + 1 : lo;
-        const int nhi = min(hi, lo + c * step);
-        lo = nlo;
-        hi = nhi;
+    const int r0 = lane * M;
+    double cp[M - 1], y[M - 1], v[M - 1], w[M - 1];
+    double cprev = 0.0, yprev = 0.0, vprev = 0.0;
+#pragma unroll
+    for (int j = 0; j < M - 1; ++j) {
+        const int r = r0 + j;
+        double a = 0.0, b = 1.0, c = 0.0, d = 0.0;
+        if (r < ns) {
+            const double hm = xs[r + 1] - xs[r], hp = xs[r + 2] - xs[r + 1];
+            a = (r > 0) ? hm : 0.0;
+            c = (r < ns - 1) ? hp : 0.0;
+            b = 2.0 * (hm + hp);
+            d = rhs(r, hm, hp, j == 0);
+        }
+        // row 0 of the lane: its left term multiplies s_{p-1} (right-hand side v)
+        const double rd = 1.0 / (b - a * cprev);
+        cp[j] = c * rd;
+        y[j] = (d - a * yprev) * rd;
+        v[j] = (j == 0 ? a : -a * vprev) * rd;
+        cprev = cp[j];
+        yprev = y[j];
+        vprev = v[j];
     }
-    const int idx = lo + lane;
-    const bool below = idx < hi && k0 * p[idx] < v;
-    return lo + __popcll(__ballot(below));
+    // the right term of the last interior row multiplies s_p (right-hand side w)
+    w[M - 2] = cp[M - 2];
+#pragma unroll
+    for (int j = M - 3; j >= 0; --j) {
+        y[j] -= cp[j] * y[j + 1];
+        v[j] -= cp[j] * v[j + 1];
+        w[j] = -cp[j] * w[j + 1];
+    }
+    // separator row: a x_{pM+M-2} + b s_p + c x_{(p+1)M} = d with both neighbours
+    // written through the separators
+    double as = 0.0, bs = 1.0, cs = 0.0, ds = 0.0;
+    {
+        const int r = r0 + M - 1;
+        if (r < ns) {
+            const double hm = xs[r + 1] - xs[r], hp = xs[r + 2] - xs[r + 1];
+            as = hm;
+            cs = (r < ns - 1) ? hp : 0.0;
+            bs = 2.0 * (hm + hp);
+            ds = rhs(r, hm, hp, false);
+        }
+    }
+    double yn = __shfl_down(y[0], 1), vn = __shfl_down(v[0], 1), wn = __shfl_down(w[0], 1);
+    if (lane == 63) { yn = 0.0; vn = 0.0; wn = 0.0; }
+    double al = -as * v[M - 2];
+    double be = bs - as * w[M - 2] - cs * vn;
+    double ga = -cs * wn;
+    double de = ds - as * y[M - 2] - cs * yn;
+#pragma unroll
+    for (int st = 1; st < 64; st <<= 1) {
+        double aL = __shfl_up(al, st), bL = __shfl_up(be, st), cL = __shfl_up(ga, st), dL = __shfl_up(de, st);
+        double aR = __shfl_down(al, st), bR = __shfl_down(be, st), cR = __shfl_down(ga, st), dR = __shfl_down(de, st);
+        if (lane < st) { aL = 0.0; bL = 1.0; cL = 0.0; dL = 0.0; }
+        if (lane + st >= 64) { aR = 0.0; bR = 1.0; cR = 0.0; dR = 0.0; }
+        const double u = al * bR, q = ga * bL;
+        const double ra = -u * aL;
+        const double rb = be * bL * bR - u * cL - q * aR;
+        const double rc = -q * cR;
+        const double rdd = de * bL * bR - u * dL - q * dR;
+        const int e = -__builtin_amdgcn_frexp_exp(rb);
+        al = ldexp(ra, e);
+        be = ldexp(rb, e);
+        ga = ldexp(rc, e);
+        de = ldexp(rdd, e);
+    }
+    const double sp = de / be;
+    double sm = __shfl_up(sp, 1);
+    if (lane == 0) sm = 0.0;
+#pragma unroll
+    for (int j = 0; j < M - 1; ++j)
+        if (r0 + j < ns) X[r0 + j] = y[j] - v[j] * sm - w[j] * sp;
+    if (r0 + M - 1 < ns) X[r0 + M - 1] = sp;
 }
 
-// interval j of the LDS-staged nodes xs[0..cnt) (global index = j + base) with
-// xs[j] <= v < xs[j+1]; bsearch semantics over the staged range
-__device__ __forceinline__ int lds_interval(const double *xs, int cnt, double v)
+// first grid index i with k0 p_i >= v (N if none): the closed form
+// i = N log(v / k0) / log(kmax / kmin), settled against the table
+__device__ __forceinline__ int grid_lower_index(const ce_grid_dev &g, double v)
 {
-    int lo = 0, hi = cnt - 1;
-    while (hi > lo + 1) {
-        const int m = (lo + hi) >> 1;
-        if (xs[m] > v) hi = m;
-        else lo = m;
+    const double t = (double)g.N * log(v / g.k0) / log(g.kmax / g.kmin);
+    int i = (t > 0.0) ? (t < (double)g.N ? (int)ceil(t) : g.N) : 0;
+    while (i > 0 && g.k0 * g.p[i - 1] >= v) --i;
+    while (i < g.N && g.k0 * g.p[i] < v) ++i;
+    return i;
+}
+
+// interval j of the LDS-staged nodes xs[0..cnt) with xs[j] <= v < xs[j+1] (the
+// largest j <= cnt - 2 with xs[j] <= v; 0 below xs[0]), for CE_PTS_PER values at
+// once: binary lifting with a fixed step sequence, so the loads of all values
+// go out together at every step (cnt <= CE_CHUNK_NODES <= 1025)
+#define CE_PTS_PER 8
+static_assert(CE_CHUNK_NODES <= 1025, "interval search steps");
+__device__ __forceinline__ void lds_intervals(const double *xs, int cnt, const double (&v)[CE_PTS_PER],
+                                              int (&j)[CE_PTS_PER])
+{
+#pragma unroll
+    for (int t = 0; t < CE_PTS_PER; ++t) j[t] = 0;
+#pragma unroll
+    for (int step = 512; step > 0; step >>= 1) {
+#pragma unroll
+        for (int t = 0; t < CE_PTS_PER; ++t) {
+            const int c = min(j[t] + step, cnt - 2);
+            if (xs[c] <= v[t]) j[t] = c;
+        }
     }
-    return lo;
 }
 
 struct chunk_geom {
@@ -265,90 +311,90 @@ __device__ __forceinline__ chunk_geom make_geom(const ce_chunk_dev &ch, int n)
     return g;
 }
 
-// stage x (and y) of nodes [lo, hi] and fill the truncated spline system; the LDS
-// layout is xs | ys | A | B | C | D, each CE_CHUNK_NODES long
-__device__ void stage_and_build(const chunk_geom &G, const double *__restrict__ k, const double *__restrict__ P,
-                                double *xs, double *ys, double *A, double *B, double *C, double *D, bool values)
-{
-    const int T = blockDim.x, tid = threadIdx.x;
-    const int cnt = G.hi - G.lo + 1;
-    for (int i = tid; i < cnt; i += T) {
-        xs[i] = k[G.lo + i];
-        if (values) ys[i] = P[G.lo + i];
+// right-hand side of the interpolating spline, 3 (slope_{r+1} - slope_r), the slope
+// of the row's left interval carried over from the previous row of the lane
+struct spline_rhs {
+    const double *ys;
+    double sprev;
+    __device__ __forceinline__ double operator()(int r, double hm, double hp, bool first)
+    {
+        const double sm = first ? (hm != 0.0 ? (ys[r + 1] - ys[r]) / hm : 0.0) : sprev;
+        const double sp = hp != 0.0 ? (ys[r + 2] - ys[r + 1]) / hp : 0.0;
+        sprev = sp;
+        return 3.0 * (sp - sm);
     }
-    __syncthreads();
-    const int ns = G.u1 - G.u0 + 1;
-    for (int r = tid; r < ns; r += T) {
-        const int li = r + 1;   // local index of node u0 + r
-        const double hm = xs[li] - xs[li - 1], hp = xs[li + 1] - xs[li];
-        A[r] = (r > 0) ? hm : 0.0;
-        C[r] = (r < ns - 1) ? hp : 0.0;
-        B[r] = 2.0 * (hm + hp);
-        if (values) {
-            const double gm = hm != 0.0 ? 1.0 / hm : 0.0, gp = hp != 0.0 ? 1.0 / hp : 0.0;
-            D[r] = 3.0 * ((ys[li + 1] - ys[li]) * gp - (ys[li] - ys[li - 1]) * gm);
-        }
-    }
-    __syncthreads();
-}
+};
 
-// k_resample: S(x_i) for the resample points whose interval lies in [a, b)
-__global__ void __launch_bounds__(256) k_resample(const ce_chunk_dev *__restrict__ chunks,
-                                                  const ce_spec_dev *__restrict__ specs,
-                                                  const ce_grid_dev *__restrict__ grids,
-                                                  const double *__restrict__ k_all,
-                                                  const double *__restrict__ pk_all, double *__restrict__ ws_all)
+// k_resample: S(x_i) for the resample points whose interval lies in [a, b); one
+// wavefront per chunk
+__global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict__ chunks,
+                                                 const ce_spec_dev *__restrict__ specs,
+                                                 const ce_grid_dev *__restrict__ grids,
+                                                 const double *__restrict__ k_all,
+                                                 const double *__restrict__ pk_all, double *__restrict__ ws_all)
 {
-    __shared__ double lds[6 * CE_CHUNK_NODES];
-    __shared__ int range[2];
+    __shared__ double lds[3 * CE_CHUNK_NODES];
     const ce_chunk_dev ch = chunks[blockIdx.x];
     const ce_spec_dev sp = specs[ch.spec];
     const ce_grid_dev &g = grids[sp.grid];
-    const int n = sp.n_in, N = g.N;
+    const int n = sp.n_in;
     const double *k = k_all + sp.in_off;
     const double *P = pk_all + sp.in_off;
     double *yres = ws_all + sp.ws_off;
     const chunk_geom G = make_geom(ch, n);
-    double *xs = lds, *ys = xs + CE_CHUNK_NODES, *A = ys + CE_CHUNK_NODES, *B = A + CE_CHUNK_NODES;
-    double *C = B + CE_CHUNK_NODES, *D = C + CE_CHUNK_NODES;
+    double *xs = lds, *ys = xs + CE_CHUNK_NODES, *D = ys + CE_CHUNK_NODES;
+    const int lane = threadIdx.x;
+    // resample points [i0, i1): lane 0 finds the first, the other lanes the end
+    int ib;
     {
-        const int wave = threadIdx.x >> 6;
-        if (wave == 0) {
-            const int r0 = (G.a == 0) ? 0 : wave_lower_index(g.p, N, g.k0, k[G.a]);
-            if ((threadIdx.x & 63) == 0) range[0] = r0;
-        } else if (wave == 1) {
-            const int r1 = (G.b == n - 1) ? N : wave_lower_index(g.p, N, g.k0, k[G.b]);
-            if ((threadIdx.x & 63) == 0) range[1] = r1;
-        }
+        const bool first = lane == 0;
+        const bool edge = first ? (G.a == 0) : (G.b == n - 1);
+        ib = edge ? (first ? 0 : g.N) : grid_lower_index(g, k[first ? G.a : G.b]);
     }
-    stage_and_build(G, k, P, xs, ys, A, B, C, D, true);
+    const int i0 = __shfl(ib, 0), i1 = __shfl(ib, 1);
+    const int cnt = G.hi - G.lo + 1;
+    for (int i = lane; i < cnt; i += 64) {
+        xs[i] = k[G.lo + i];
+        ys[i] = P[G.lo + i];
+    }
+    __syncthreads();
     const int ns = G.u1 - G.u0 + 1;
-    pcr_solve(A, B, C, D, ns);
+    wave_spline_solve(xs, ns, spline_rhs{ys, 0.0}, D);
+    __syncthreads();
     // c at staged node li: D[li - 1] for u0 <= node <= u1, 0 at the global ends
     auto cval = [&](int li) {
         const int node = G.lo + li;
         return (node < 1 || node > n - 2) ? 0.0 : D[node - G.u0];
     };
-    const int i0 = range[0], i1 = range[1];
-    const int T = blockDim.x, tid = threadIdx.x;
     const double klo = k[0], khi = k[n - 1];
-    const int cnt = G.hi - G.lo + 1;
-    for (int i = i0 + tid; i < i1; i += T) {
-        const double xq = g.k0 * g.p[i];
-        double S;
-        if (!(xq >= klo && xq <= khi)) {
-            S = __builtin_nan("");
-        } else {
-            const int j = lds_interval(xs, cnt, xq);
-            const double dx = xs[j + 1] - xs[j];
-            const double dy = ys[j + 1] - ys[j];
-            const double delx = xq - xs[j];
-            const double cj = cval(j), cj1 = cval(j + 1);
-            const double bb = dy / dx - dx * (cj1 + 2.0 * cj) / 3.0;
-            const double dd = (cj1 - cj) / (3.0 * dx);
-            S = ys[j] + delx * (bb + delx * (cj + delx * dd));
+    for (int base = i0 + lane; base < i1; base += 64 * CE_PTS_PER) {
+        double xq[CE_PTS_PER];
+        int jv[CE_PTS_PER];
+#pragma unroll
+        for (int t = 0; t < CE_PTS_PER; ++t) {
+            const int i = base + 64 * t;
+            xq[t] = (i < i1) ? g.k0 * g.p[i] : klo;
         }
-        yres[i] = S;
+        lds_intervals(xs, cnt, xq, jv);
+#pragma unroll
+        for (int t = 0; t < CE_PTS_PER; ++t) {
+            const int i = base + 64 * t;
+            if (i >= i1) break;
+            const int j = jv[t];
+            double S;
+            if (!(xq[t] >= klo && xq[t] <= khi)) {
+                S = __builtin_nan("");
+            } else {
+                const double dx = xs[j + 1] - xs[j], rdx = 1.0 / dx;
+                const double dy = ys[j + 1] - ys[j];
+                const double delx = xq[t] - xs[j];
+                const double cj = cval(j), cj1 = cval(j + 1);
+                const double bb = dy * rdx - dx * (cj1 + 2.0 * cj) / 3.0;
+                const double dd = (cj1 - cj) * rdx / 3.0;
+                S = ys[j] + delx * (bb + delx * (cj + delx * dd));
+            }
+            yres[i] = S;
+        }
     }
 }
 
@@ -673,14 +719,13 @@ __global__ void __launch_bounds__(1024) k_adj_c2r(const ce_spec_dev *__restrict_
 // of every resample point are scattered with LDS atomics, lambda = T^{-1} cbar is
 // solved on the truncated system, and dP_j collects the y-dependence of the
 // right-hand side g (twofast.c:399-401 / the natural-spline equations).
-__global__ void __launch_bounds__(256) k_spline_adj(const ce_chunk_dev *__restrict__ chunks,
-                                                    const ce_spec_dev *__restrict__ specs,
-                                                    const ce_grid_dev *__restrict__ grids,
-                                                    const double *__restrict__ k_all,
-                                                    const double *__restrict__ ws_all, double *__restrict__ gpk_all)
+__global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restrict__ chunks,
+                                                   const ce_spec_dev *__restrict__ specs,
+                                                   const ce_grid_dev *__restrict__ grids,
+                                                   const double *__restrict__ k_all,
+                                                   const double *__restrict__ ws_all, double *__restrict__ gpk_all)
 {
-    __shared__ double lds[7 * CE_CHUNK_NODES];
-    __shared__ int range[2];
+    __shared__ double lds[3 * CE_CHUNK_NODES];
     const ce_chunk_dev ch = chunks[blockIdx.x];
     const ce_spec_dev sp = specs[ch.spec];
     const ce_grid_dev &g = grids[sp.grid];
@@ -691,43 +736,58 @@ __global__ void __launch_bounds__(256) k_spline_adj(const ce_chunk_dev *__restri
     const chunk_geom G = make_geom(ch, n);
     const int cnt = G.hi - G.lo + 1;
     const int ns = G.u1 - G.u0 + 1;
-    double *xs = lds, *pb = xs + CE_CHUNK_NODES, *cb = pb + CE_CHUNK_NODES, *A = cb + CE_CHUNK_NODES;
-    double *B = A + CE_CHUNK_NODES, *C = B + CE_CHUNK_NODES, *D = C + CE_CHUNK_NODES;
-    const int T = blockDim.x, tid = threadIdx.x;
+    double *xs = lds, *pb = xs + CE_CHUNK_NODES, *cb = pb + CE_CHUNK_NODES;
+    const int lane = threadIdx.x;
+    // resample points with intervals in the staged range [lo, hi)
+    int ib;
     {
-        const int wave = tid >> 6;
-        if (wave == 0) {
-            const int r0 = (G.lo == 0) ? 0 : wave_lower_index(g.p, N, g.k0, k[G.lo]);
-            if ((tid & 63) == 0) range[0] = r0;
-        } else if (wave == 1) {
-            const int r1 = (G.hi == n - 1) ? N : wave_lower_index(g.p, N, g.k0, k[G.hi]);
-            if ((tid & 63) == 0) range[1] = r1;
+        const bool first = lane == 0;
+        const bool edge = first ? (G.lo == 0) : (G.hi == n - 1);
+        ib = edge ? (first ? 0 : N) : grid_lower_index(g, k[first ? G.lo : G.hi]);
+    }
+    const int i0 = __shfl(ib, 0), i1 = __shfl(ib, 1);
+    for (int i = lane; i < cnt; i += 64) {
+        xs[i] = k[G.lo + i];
+        pb[i] = 0.0;
+        cb[i] = 0.0;
+    }
+    __syncthreads();
+    // scatter the resample cotangents of the points with intervals in [lo, hi)
+    const double klo = k[0], khi = k[n - 1];
+    for (int base = i0 + lane; base < i1; base += 64 * CE_PTS_PER) {
+        double xq[CE_PTS_PER], sv[CE_PTS_PER];
+        int jv[CE_PTS_PER];
+#pragma unroll
+        for (int t = 0; t < CE_PTS_PER; ++t) {
+            const int i = base + 64 * t;
+            xq[t] = (i < i1) ? g.k0 * g.p[i] : klo;
+            sv[t] = (i < i1) ? sbar[i] : 0.0;
+        }
+        lds_intervals(xs, cnt, xq, jv);
+#pragma unroll
+        for (int t = 0; t < CE_PTS_PER; ++t) {
+            if (base + 64 * t >= i1) break;
+            if (!(xq[t] >= klo && xq[t] <= khi)) continue;
+            const int j = jv[t];
+            const double dx = xs[j + 1] - xs[j], rdx = 1.0 / dx;
+            const double d = xq[t] - xs[j];
+            const double d3 = d * d * d * rdx / 3.0;
+            atomicAdd(&pb[j], sv[t] * (1.0 - d * rdx));
+            atomicAdd(&pb[j + 1], sv[t] * d * rdx);
+            atomicAdd(&cb[j], sv[t] * (-2.0 * d * dx / 3.0 + d * d - d3));
+            atomicAdd(&cb[j + 1], sv[t] * (-d * dx / 3.0 + d3));
         }
     }
-    for (int i = tid; i < cnt; i += T) { pb[i] = 0.0; cb[i] = 0.0; }
-    stage_and_build(G, k, nullptr, xs, nullptr, A, B, C, D, false);
-    // scatter the resample cotangents of the points with intervals in [lo, hi)
-    const int i0 = range[0], i1 = range[1];
-    const double klo = k[0], khi = k[n - 1];
-    for (int i = i0 + tid; i < i1; i += T) {
-        const double xq = g.k0 * g.p[i];
-        if (!(xq >= klo && xq <= khi)) continue;
-        const int j = lds_interval(xs, cnt, xq);
-        const double dx = xs[j + 1] - xs[j];
-        const double d = xq - xs[j];
-        const double sv = sbar[i];
-        atomicAdd(&pb[j], sv * (1.0 - d / dx));
-        atomicAdd(&pb[j + 1], sv * d / dx);
-        atomicAdd(&cb[j], sv * (-2.0 * d * dx / 3.0 + d * d - d * d * d / (3.0 * dx)));
-        atomicAdd(&cb[j + 1], sv * (-d * dx / 3.0 + d * d * d / (3.0 * dx)));
-    }
     __syncthreads();
-    for (int r = tid; r < ns; r += T) D[r] = cb[r + 1];   // node u0 + r sits at staged index r + 1
+    // the transposed solve: the spline matrix is symmetric; node u0 + r sits at staged
+    // index r + 1.  The solution overwrites cb in place (D[r] = cb[r]): the one
+    // wavefront has read every right-hand side before its first store
+    double *D = cb;
+    wave_spline_solve(xs, ns, [&](int r, double, double, bool) { return cb[r + 1]; }, D);
     __syncthreads();
-    pcr_solve(A, B, C, D, ns);
     auto lam = [&](int node) { return (node < G.u0 || node > G.u1) ? 0.0 : D[node - G.u0]; };
     const int jend = (G.b == n - 1) ? n : G.b;
-    for (int jj = G.a + tid; jj < jend; jj += T) {
+    for (int jj = G.a + lane; jj < jend; jj += 64) {
         const int li = jj - G.lo;
         double acc = pb[li];
         const int sys = n - 2;
@@ -798,7 +858,7 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
 int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
                        const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws)
 {
-    hipLaunchKernelGGL(k_resample, dim3(nchunk), dim3(256), 0, (hipStream_t)stream, d_chunks, d_spec, d_grids, d_k,
+    hipLaunchKernelGGL(k_resample, dim3(nchunk), dim3(64), 0, (hipStream_t)stream, d_chunks, d_spec, d_grids, d_k,
                        d_pk, d_ws);
     return (int)hipGetLastError();
 }
@@ -880,7 +940,7 @@ int ce_launch_adj_c2r(void *stream, int nspec, const ce_spec_dev *d_spec, const 
 int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
                          const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk)
 {
-    hipLaunchKernelGGL(k_spline_adj, dim3(nchunk), dim3(256), 0, (hipStream_t)stream, d_chunks, d_spec, d_grids, d_k,
+    hipLaunchKernelGGL(k_spline_adj, dim3(nchunk), dim3(64), 0, (hipStream_t)stream, d_chunks, d_spec, d_grids, d_k,
                        d_ws, d_gpk);
     return (int)hipGetLastError();
 }

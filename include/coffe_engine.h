@@ -155,7 +155,9 @@ unsigned coffe_terms_of(int den, int rsd, int d1);
  * are host arrays, interpolated with Akima as the reference does.  Contributions:
  * den, rsd, d1 (0/1), flatsky selects the flat-sky forms.  Tables t are (tab_n,
  * tab_l) among the 8 defaults and must include every table the contributions use.
- * d_A is device memory [nz][nl][ntab][nsep]; nl <= 4.
+ * d_A is device memory [nz][nl][ntab][nsep]; nl <= 4.  Asynchronous on `stream`
+ * (NULL: the engine's stream): the host arrays may be reused when the call returns,
+ * and d_A is complete when the stream reaches this point.
  */
 int coffe_mu_weights(coffe_engine *e, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
                      const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
