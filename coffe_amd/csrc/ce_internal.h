@@ -57,6 +57,9 @@ typedef struct ce_spec_dev {
 typedef struct ce_chunk_dev {
     int32_t spec;
     int32_t a, b;
+    int32_t grid;      /* copies of the spectrum's fields: one load gives everything */
+    int64_t in_off, ws_off;
+    int32_t n_in, pad;
 } ce_chunk_dev;
 
 /* work unit of the table kernels */

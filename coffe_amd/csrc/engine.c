@@ -465,6 +465,11 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
             const int nint = (int)n_in[s] - 1;
             for (int a = 0; a < nint; a += CE_CHUNK) {
                 chunks[c].spec = s;
+                chunks[c].grid = b->h_spec[s].grid;
+                chunks[c].in_off = b->h_spec[s].in_off;
+                chunks[c].ws_off = b->h_spec[s].ws_off;
+                chunks[c].n_in = b->h_spec[s].n_in;
+                chunks[c].pad = 0;
                 chunks[c].a = a;
                 chunks[c].b = a + CE_CHUNK < nint ? a + CE_CHUNK : nint;
                 ++c;
@@ -712,6 +717,11 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     if (!h_chunks) { pthread_mutex_unlock(&e->lock); return COFFE_ENOMEM; }
     for (int c = 0; c < nchunk; ++c) {
         h_chunks[c].spec = 0;
+        h_chunks[c].grid = 0;
+        h_chunks[c].in_off = 0;
+        h_chunks[c].ws_off = 0;
+        h_chunks[c].n_in = (int32_t)n;
+        h_chunks[c].pad = 0;
         h_chunks[c].a = c * CE_CHUNK;
         h_chunks[c].b = (c + 1) * CE_CHUNK < (int)n - 1 ? (c + 1) * CE_CHUNK : (int)n - 1;
     }

@@ -160,6 +160,29 @@ __global__ void k_grid_tables(grid_tab_args a)
 // exact natural boundary c_0 = c_{n-1} = 0.
 
 #define CE_SPLINE_HALO 64
+
+// 1/x within about one ulp without the division sequence: the hardware estimate
+// refined by two Newton steps (no VCC, so independent reciprocals overlap); for
+// normal x of moderate size, as the spacings and diagonals here are
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// the barrier of a one-wavefront workgroup: a wavefront's LDS operations complete
+// in issue order, so only the compiler has to keep the accesses on their side (and
+// global loads still in flight stay in flight)
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // rows per lane of the one-wavefront solve: 64 lanes hold the largest truncated
 // system, CE_CHUNK + 2 CE_SPLINE_HALO + 1 rows
 #define CE_SOLVE_M 11
@@ -200,7 +223,7 @@ __device__ __forceinline__ void wave_spline_solve(const double *xs, int ns, Rhs 
             d = rhs(r, hm, hp, j == 0);
         }
         // row 0 of the lane: its left term multiplies s_{p-1} (right-hand side v)
-        const double rd = 1.0 / (b - a * cprev);
+        const double rd = rcp_nr(b - a * cprev);
         cp[j] = c * rd;
         y[j] = (d - a * yprev) * rd;
         v[j] = (j == 0 ? a : -a * vprev) * rd;
@@ -261,14 +284,27 @@ __device__ __forceinline__ void wave_spline_solve(const double *xs, int ns, Rhs 
     if (r0 + M - 1 < ns) X[r0 + M - 1] = sp;
 }
 
+// global-memory view of a table pointer that was itself read from memory: loads
+// through it are global (not flat) loads, whose counters the compiler can wait on
+// one at a time
+typedef __attribute__((address_space(1))) const double gmem_double;
+__device__ __forceinline__ const gmem_double *as_global(const double *p) { return (const gmem_double *)p; }
+
 // first grid index i with k0 p_i >= v (N if none): the closed form
-// i = N log(v / k0) / log(kmax / kmin), settled against the table
+// i = N log(v / k0) / G, G = log(kmax / kmin), settled against the table
 __device__ __forceinline__ int grid_lower_index(const ce_grid_dev &g, double v)
 {
-    const double t = (double)g.N * log(v / g.k0) / log(g.kmax / g.kmin);
-    int i = (t > 0.0) ? (t < (double)g.N ? (int)ceil(t) : g.N) : 0;
-    while (i > 0 && g.k0 * g.p[i - 1] >= v) --i;
-    while (i < g.N && g.k0 * g.p[i] < v) ++i;
+    const gmem_double *p = as_global(g.p);
+    const double k0 = g.k0;
+    const int N = g.N;
+    const double t = (double)N * log(v / k0) / g.G;
+    int i = (t > 0.0) ? (t < (double)N ? (int)ceil(t) : N) : 0;
+    // both neighbours are loaded at once; a walk follows only for a poor guess
+    const double below = k0 * p[max(i - 1, 0)], at = k0 * p[min(i, N - 1)];
+    if ((i > 0 && below >= v) || (i < N && at < v)) {
+        while (i > 0 && k0 * p[i - 1] >= v) --i;
+        while (i < N && k0 * p[i] < v) ++i;
+    }
     return i;
 }
 
@@ -318,8 +354,8 @@ struct spline_rhs {
     double sprev;
     __device__ __forceinline__ double operator()(int r, double hm, double hp, bool first)
     {
-        const double sm = first ? (hm != 0.0 ? (ys[r + 1] - ys[r]) / hm : 0.0) : sprev;
-        const double sp = hp != 0.0 ? (ys[r + 2] - ys[r + 1]) / hp : 0.0;
+        const double sm = first ? (hm != 0.0 ? (ys[r + 1] - ys[r]) * rcp_nr(hm) : 0.0) : sprev;
+        const double sp = hp != 0.0 ? (ys[r + 2] - ys[r + 1]) * rcp_nr(hp) : 0.0;
         sprev = sp;
         return 3.0 * (sp - sm);
     }
@@ -335,15 +371,26 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
 {
     __shared__ double lds[3 * CE_CHUNK_NODES];
     const ce_chunk_dev ch = chunks[blockIdx.x];
-    const ce_spec_dev sp = specs[ch.spec];
-    const ce_grid_dev &g = grids[sp.grid];
-    const int n = sp.n_in;
-    const double *k = k_all + sp.in_off;
-    const double *P = pk_all + sp.in_off;
-    double *yres = ws_all + sp.ws_off;
+    const ce_grid_dev &g = grids[ch.grid];
+    const gmem_double *gp = as_global(g.p);
+    const double gk0 = g.k0;
+    const int n = ch.n_in;
+    const double *k = k_all + ch.in_off;
+    const double *P = pk_all + ch.in_off;
+    double *yres = ws_all + ch.ws_off;
     const chunk_geom G = make_geom(ch, n);
     double *xs = lds, *ys = xs + CE_CHUNK_NODES, *D = ys + CE_CHUNK_NODES;
     const int lane = threadIdx.x;
+    const int cnt = G.hi - G.lo + 1;
+    // the staged nodes go to registers first, so that their loads overlap the search
+    // for the resample range and the loads of the first resample points
+    constexpr int SPL = (CE_CHUNK_NODES + 63) / 64;
+    double kv[SPL], pv[SPL];
+#pragma unroll
+    for (int u = 0; u < SPL; ++u) {
+        const int i = lane + 64 * u;
+        if (i < cnt) { kv[u] = k[G.lo + i]; pv[u] = P[G.lo + i]; }
+    }
     // resample points [i0, i1): lane 0 finds the first, the other lanes the end
     int ib;
     {
@@ -352,29 +399,35 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
         ib = edge ? (first ? 0 : g.N) : grid_lower_index(g, k[first ? G.a : G.b]);
     }
     const int i0 = __shfl(ib, 0), i1 = __shfl(ib, 1);
-    const int cnt = G.hi - G.lo + 1;
-    for (int i = lane; i < cnt; i += 64) {
-        xs[i] = k[G.lo + i];
-        ys[i] = P[G.lo + i];
+    const double klo = k[0], khi = k[n - 1];
+    // loads at clamped indices, unconditionally, so that they all go out at once
+    const int ilast = max(i1 - 1, 0);
+    double pq[CE_PTS_PER];   // the raw table values: nothing waits on them before use
+#pragma unroll
+    for (int t = 0; t < CE_PTS_PER; ++t) pq[t] = gp[min(i0 + lane + 64 * t, ilast)];
+#pragma unroll
+    for (int u = 0; u < SPL; ++u) {
+        const int i = lane + 64 * u;
+        if (i < cnt) { xs[i] = kv[u]; ys[i] = pv[u]; }
     }
-    __syncthreads();
+    wave_sync();
     const int ns = G.u1 - G.u0 + 1;
     wave_spline_solve(xs, ns, spline_rhs{ys, 0.0}, D);
-    __syncthreads();
+    wave_sync();
     // c at staged node li: D[li - 1] for u0 <= node <= u1, 0 at the global ends
     auto cval = [&](int li) {
         const int node = G.lo + li;
         return (node < 1 || node > n - 2) ? 0.0 : D[node - G.u0];
     };
-    const double klo = k[0], khi = k[n - 1];
     for (int base = i0 + lane; base < i1; base += 64 * CE_PTS_PER) {
-        double xq[CE_PTS_PER];
         int jv[CE_PTS_PER];
+        if (base != i0 + lane) {
 #pragma unroll
-        for (int t = 0; t < CE_PTS_PER; ++t) {
-            const int i = base + 64 * t;
-            xq[t] = (i < i1) ? g.k0 * g.p[i] : klo;
+            for (int t = 0; t < CE_PTS_PER; ++t) pq[t] = gp[min(base + 64 * t, ilast)];
         }
+        double xq[CE_PTS_PER];
+#pragma unroll
+        for (int t = 0; t < CE_PTS_PER; ++t) xq[t] = gk0 * pq[t];
         lds_intervals(xs, cnt, xq, jv);
 #pragma unroll
         for (int t = 0; t < CE_PTS_PER; ++t) {
@@ -385,7 +438,7 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
             if (!(xq[t] >= klo && xq[t] <= khi)) {
                 S = __builtin_nan("");
             } else {
-                const double dx = xs[j + 1] - xs[j], rdx = 1.0 / dx;
+                const double dx = xs[j + 1] - xs[j], rdx = rcp_nr(dx);
                 const double dy = ys[j + 1] - ys[j];
                 const double delx = xq[t] - xs[j];
                 const double cj = cval(j), cj1 = cval(j + 1);
@@ -727,17 +780,26 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
 {
     __shared__ double lds[3 * CE_CHUNK_NODES];
     const ce_chunk_dev ch = chunks[blockIdx.x];
-    const ce_spec_dev sp = specs[ch.spec];
-    const ce_grid_dev &g = grids[sp.grid];
-    const int n = sp.n_in, N = g.N;
-    const double *k = k_all + sp.in_off;
-    const double *sbar = ws_all + sp.ws_off + N;
-    double *gpk = gpk_all + sp.in_off;
+    const ce_grid_dev &g = grids[ch.grid];
+    const gmem_double *gp = as_global(g.p);
+    const double gk0 = g.k0;
+    const int n = ch.n_in, N = g.N;
+    const double *k = k_all + ch.in_off;
+    const double *sbar = ws_all + ch.ws_off + N;
+    double *gpk = gpk_all + ch.in_off;
     const chunk_geom G = make_geom(ch, n);
     const int cnt = G.hi - G.lo + 1;
     const int ns = G.u1 - G.u0 + 1;
     double *xs = lds, *pb = xs + CE_CHUNK_NODES, *cb = pb + CE_CHUNK_NODES;
     const int lane = threadIdx.x;
+    // staged nodes to registers first (their loads overlap the range search)
+    constexpr int SPL = (CE_CHUNK_NODES + 63) / 64;
+    double kv[SPL];
+#pragma unroll
+    for (int u = 0; u < SPL; ++u) {
+        const int i = lane + 64 * u;
+        if (i < cnt) kv[u] = k[G.lo + i];
+    }
     // resample points with intervals in the staged range [lo, hi)
     int ib;
     {
@@ -746,22 +808,36 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
         ib = edge ? (first ? 0 : N) : grid_lower_index(g, k[first ? G.lo : G.hi]);
     }
     const int i0 = __shfl(ib, 0), i1 = __shfl(ib, 1);
-    for (int i = lane; i < cnt; i += 64) {
-        xs[i] = k[G.lo + i];
-        pb[i] = 0.0;
-        cb[i] = 0.0;
-    }
-    __syncthreads();
-    // scatter the resample cotangents of the points with intervals in [lo, hi)
     const double klo = k[0], khi = k[n - 1];
-    for (int base = i0 + lane; base < i1; base += 64 * CE_PTS_PER) {
-        double xq[CE_PTS_PER], sv[CE_PTS_PER];
-        int jv[CE_PTS_PER];
+    // loads at clamped indices, unconditionally, so that they all go out at once
+    const int ilast = max(i1 - 1, 0);
+    double xq[CE_PTS_PER], sv[CE_PTS_PER];
 #pragma unroll
-        for (int t = 0; t < CE_PTS_PER; ++t) {
-            const int i = base + 64 * t;
-            xq[t] = (i < i1) ? g.k0 * g.p[i] : klo;
-            sv[t] = (i < i1) ? sbar[i] : 0.0;
+    for (int t = 0; t < CE_PTS_PER; ++t) {
+        const int i = min(i0 + lane + 64 * t, ilast);
+        xq[t] = gk0 * gp[i];
+        sv[t] = sbar[i];
+    }
+#pragma unroll
+    for (int u = 0; u < SPL; ++u) {
+        const int i = lane + 64 * u;
+        if (i < cnt) {
+            xs[i] = kv[u];
+            pb[i] = 0.0;
+            cb[i] = 0.0;
+        }
+    }
+    wave_sync();
+    // scatter the resample cotangents of the points with intervals in [lo, hi)
+    for (int base = i0 + lane; base < i1; base += 64 * CE_PTS_PER) {
+        int jv[CE_PTS_PER];
+        if (base != i0 + lane) {
+#pragma unroll
+            for (int t = 0; t < CE_PTS_PER; ++t) {
+                const int i = min(base + 64 * t, ilast);
+                xq[t] = gk0 * gp[i];
+                sv[t] = sbar[i];
+            }
         }
         lds_intervals(xs, cnt, xq, jv);
 #pragma unroll
@@ -769,7 +845,7 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
             if (base + 64 * t >= i1) break;
             if (!(xq[t] >= klo && xq[t] <= khi)) continue;
             const int j = jv[t];
-            const double dx = xs[j + 1] - xs[j], rdx = 1.0 / dx;
+            const double dx = xs[j + 1] - xs[j], rdx = rcp_nr(dx);
             const double d = xq[t] - xs[j];
             const double d3 = d * d * d * rdx / 3.0;
             atomicAdd(&pb[j], sv[t] * (1.0 - d * rdx));
@@ -778,13 +854,13 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
             atomicAdd(&cb[j + 1], sv[t] * (-d * dx / 3.0 + d3));
         }
     }
-    __syncthreads();
+    wave_sync();
     // the transposed solve: the spline matrix is symmetric; node u0 + r sits at staged
     // index r + 1.  The solution overwrites cb in place (D[r] = cb[r]): the one
     // wavefront has read every right-hand side before its first store
     double *D = cb;
     wave_spline_solve(xs, ns, [&](int r, double, double, bool) { return cb[r + 1]; }, D);
-    __syncthreads();
+    wave_sync();
     auto lam = [&](int node) { return (node < G.u0 || node > G.u1) ? 0.0 : D[node - G.u0]; };
     const int jend = (G.b == n - 1) ? n : G.b;
     for (int jj = G.a + lane; jj < jend; jj += 64) {
@@ -792,9 +868,12 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
         double acc = pb[li];
         const int sys = n - 2;
         // g_s (node s + 1) = 3((y_{s+2} - y_{s+1})/h_{s+1} - (y_{s+1} - y_s)/h_s)
-        if (jj - 2 >= 0 && jj - 2 < sys) acc += lam(jj - 1) * 3.0 / (xs[li] - xs[li - 1]);
-        if (jj - 1 >= 0 && jj - 1 < sys) acc += lam(jj) * (-3.0 / (xs[li + 1] - xs[li]) - 3.0 / (xs[li] - xs[li - 1]));
-        if (jj < sys) acc += lam(jj + 1) * 3.0 / (xs[li + 1] - xs[li]);
+        const double xl = xs[max(li - 1, 0)], xc = xs[li], xr = xs[min(li + 1, cnt - 1)];
+        const double gm = (jj >= 1) ? 3.0 * rcp_nr(xc - xl) : 0.0;
+        const double gp = (jj <= n - 2) ? 3.0 * rcp_nr(xr - xc) : 0.0;
+        if (jj - 2 >= 0 && jj - 2 < sys) acc += lam(jj - 1) * gm;
+        if (jj - 1 >= 0 && jj - 1 < sys) acc += lam(jj) * (-gp - gm);
+        if (jj < sys) acc += lam(jj + 1) * gp;
         gpk[jj] = acc;
     }
 }
