@@ -79,8 +79,11 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
                           int ntab, const ce_table_desc *tabs, const double *tab_qnu, int nq, const double *qprime,
                           double *const *d_U, double *const *d_os, double *const *d_bw);
 
+/* d_ranges (4 ints per chunk, may be NULL): the resample index ranges of the chunk's
+   owned and staged intervals, written for ce_launch_spline_adj */
 int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
-                       const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws);
+                       const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws,
+                       int32_t *d_ranges);
 
 int ce_launch_prepare(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids, int nq,
                       const double *d_ws, double *d_xf, int threads, int max_M);
@@ -99,7 +102,8 @@ int ce_launch_adj_c2r(void *stream, int nspec, const ce_spec_dev *d_spec, const 
                       int max_M);
 
 int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
-                         const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk);
+                         const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk,
+                         const int32_t *d_ranges);
 
 /* ---- small-separation table nodes (quad_kernels.hip): out/status [nspec][ntab][43] ---- */
 int ce_launch_small_r(void *stream, int nspec, int ntab, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,

@@ -322,6 +322,7 @@ struct coffe_batch {
     ce_table_desc *d_tabs;
     ce_unit_dev *d_units;
     ce_chunk_dev *d_chunks;
+    int32_t *d_ranges; /* resample index ranges per chunk (4 each), from the prepare stage */
     int32_t *d_sidx;
     double *d_sep;
     double *d_ws, *d_xf, *d_xb, *d_wbuf, *d_itab;
@@ -340,6 +341,7 @@ void coffe_batch_destroy(coffe_batch *b)
     hipFree(b->d_tabs);
     hipFree(b->d_units);
     hipFree(b->d_chunks);
+    hipFree(b->d_ranges);
     hipFree(b->d_sidx);
     hipFree(b->d_sep);
     hipFree(b->d_ws);
@@ -491,6 +493,8 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_tabs, sizeof(ce_table_desc) * (size_t)ntab);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_units, sizeof(ce_unit_dev) * (size_t)b->nunit);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_chunks, sizeof(ce_chunk_dev) * (size_t)b->nchunk);
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_ranges, 4 * sizeof(int32_t) * (size_t)b->nchunk);
+    if (st == COFFE_OK) st = hip_ok(hipMemset(b->d_ranges, 0, 4 * sizeof(int32_t) * (size_t)b->nchunk));
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_sidx, sizeof(int32_t) * (size_t)b->ngrid * (size_t)nsep);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_sep, sizeof(double) * (size_t)nsep);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_ws, sizeof(double) * (size_t)(ws_off > 0 ? ws_off : 1));
@@ -522,7 +526,8 @@ static hipStream_t pick_stream(coffe_batch *b, void *stream) { return stream ? (
 int coffe_batch_resample(coffe_batch *b, const double *d_k, const double *d_pk, void *stream)
 {
     if (!b || !d_k || !d_pk) return COFFE_EINVAL;
-    return ce_launch_resample(pick_stream(b, stream), b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, d_pk, b->d_ws)
+    return ce_launch_resample(pick_stream(b, stream), b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, d_pk, b->d_ws,
+                              b->d_ranges)
                ? COFFE_EDEVICE
                : COFFE_OK;
 }
@@ -549,7 +554,7 @@ int coffe_batch_spline_adj(coffe_batch *b, const double *d_k, double *d_gpk, voi
 {
     if (!b || !d_k || !d_gpk) return COFFE_EINVAL;
     return ce_launch_spline_adj(pick_stream(b, stream), b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, b->d_ws,
-                                d_gpk)
+                                d_gpk, b->d_ranges)
                ? COFFE_EDEVICE
                : COFFE_OK;
 }
@@ -558,7 +563,8 @@ int coffe_batch_prepare(coffe_batch *b, const double *d_k, const double *d_pk, v
 {
     if (!b || !d_k || !d_pk) return COFFE_EINVAL;
     hipStream_t s = pick_stream(b, stream);
-    if (ce_launch_resample(s, b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, d_pk, b->d_ws)) return COFFE_EDEVICE;
+    if (ce_launch_resample(s, b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, d_pk, b->d_ws, b->d_ranges))
+        return COFFE_EDEVICE;
     return ce_launch_prepare(s, b->nspec, b->d_spec, b->d_grids, b->nq, b->d_ws, b->d_xf, b->threads, b->max_M)
                ? COFFE_EDEVICE
                : COFFE_OK;
@@ -591,7 +597,7 @@ int coffe_batch_prepare_adj(coffe_batch *b, const double *d_k, const double *d_p
     if (ce_launch_adj_c2r(s, b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq, b->d_ws, b->d_xb,
                           b->threads, b->max_M))
         return COFFE_EDEVICE;
-    return ce_launch_spline_adj(s, b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, b->d_ws, d_gpk)
+    return ce_launch_spline_adj(s, b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, b->d_ws, d_gpk, b->d_ranges)
                ? COFFE_EDEVICE
                : COFFE_OK;
 }
@@ -748,7 +754,7 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     if (st == COFFE_OK)
         st = hip_ok(hipMemcpyAsync(d_chunks, h_chunks, sizeof(ce_chunk_dev) * (size_t)nchunk, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK && ce_launch_resample(s, nchunk, d_chunks, d_spec, d_grid, base + off_k, base + off_p,
-                                             base + off_ws))
+                                             base + off_ws, NULL))
         st = COFFE_EDEVICE;
     if (st == COFFE_OK && ce_launch_prepare(s, 1, d_spec, d_grid, 1, base + off_ws, base + off_xf, threads, M))
         st = COFFE_EDEVICE;

@@ -367,7 +367,8 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
                                                  const ce_spec_dev *__restrict__ specs,
                                                  const ce_grid_dev *__restrict__ grids,
                                                  const double *__restrict__ k_all,
-                                                 const double *__restrict__ pk_all, double *__restrict__ ws_all)
+                                                 const double *__restrict__ pk_all, double *__restrict__ ws_all,
+                                                 int32_t *__restrict__ ranges)
 {
     __shared__ double lds[3 * CE_CHUNK_NODES];
     const ce_chunk_dev ch = chunks[blockIdx.x];
@@ -391,13 +392,16 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
         const int i = lane + 64 * u;
         if (i < cnt) { kv[u] = k[G.lo + i]; pv[u] = P[G.lo + i]; }
     }
-    // resample points [i0, i1): lane 0 finds the first, the other lanes the end
+    // resample index ranges: lanes 0, 1 bound the points of the owned intervals
+    // [a, b), lanes 2, 3 those of the staged intervals [lo, hi) (kept for the adjoint)
     int ib;
     {
-        const bool first = lane == 0;
-        const bool edge = first ? (G.a == 0) : (G.b == n - 1);
-        ib = edge ? (first ? 0 : g.N) : grid_lower_index(g, k[first ? G.a : G.b]);
+        const int q = lane & 3;
+        const int node = q == 0 ? G.a : (q == 1 ? G.b : (q == 2 ? G.lo : G.hi));
+        const bool edge = (q & 1) ? (node == n - 1) : (node == 0);
+        ib = edge ? ((q & 1) ? g.N : 0) : grid_lower_index(g, k[node]);
     }
+    if (ranges && lane < 4) ranges[4 * (int)blockIdx.x + lane] = ib;
     const int i0 = __shfl(ib, 0), i1 = __shfl(ib, 1);
     const double klo = k[0], khi = k[n - 1];
     // loads at clamped indices, unconditionally, so that they all go out at once
@@ -776,7 +780,8 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
                                                    const ce_spec_dev *__restrict__ specs,
                                                    const ce_grid_dev *__restrict__ grids,
                                                    const double *__restrict__ k_all,
-                                                   const double *__restrict__ ws_all, double *__restrict__ gpk_all)
+                                                   const double *__restrict__ ws_all, double *__restrict__ gpk_all,
+                                                   const int32_t *__restrict__ ranges)
 {
     __shared__ double lds[3 * CE_CHUNK_NODES];
     const ce_chunk_dev ch = chunks[blockIdx.x];
@@ -800,14 +805,11 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
         const int i = lane + 64 * u;
         if (i < cnt) kv[u] = k[G.lo + i];
     }
-    // resample points with intervals in the staged range [lo, hi)
-    int ib;
-    {
-        const bool first = lane == 0;
-        const bool edge = first ? (G.lo == 0) : (G.hi == n - 1);
-        ib = edge ? (first ? 0 : N) : grid_lower_index(g, k[first ? G.lo : G.hi]);
-    }
-    const int i0 = __shfl(ib, 0), i1 = __shfl(ib, 1);
+    // resample points with intervals in the staged range [lo, hi), as k_resample of
+    // the same prepare stage found them
+    // (clamped to the grid, so that a stage run out of order stays in bounds)
+    const int i1 = min(max(ranges[4 * (int)blockIdx.x + 3], 0), N);
+    const int i0 = min(max(ranges[4 * (int)blockIdx.x + 2], 0), i1);
     const double klo = k[0], khi = k[n - 1];
     // loads at clamped indices, unconditionally, so that they all go out at once
     const int ilast = max(i1 - 1, 0);
@@ -935,10 +937,11 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
 }
 
 int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
-                       const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws)
+                       const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws,
+                       int32_t *d_ranges)
 {
     hipLaunchKernelGGL(k_resample, dim3(nchunk), dim3(64), 0, (hipStream_t)stream, d_chunks, d_spec, d_grids, d_k,
-                       d_pk, d_ws);
+                       d_pk, d_ws, d_ranges);
     return (int)hipGetLastError();
 }
 
@@ -1017,10 +1020,11 @@ int ce_launch_adj_c2r(void *stream, int nspec, const ce_spec_dev *d_spec, const 
 }
 
 int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
-                         const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk)
+                         const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk,
+                         const int32_t *d_ranges)
 {
     hipLaunchKernelGGL(k_spline_adj, dim3(nchunk), dim3(64), 0, (hipStream_t)stream, d_chunks, d_spec, d_grids, d_k,
-                       d_ws, d_gpk);
+                       d_ws, d_gpk, d_ranges);
     return (int)hipGetLastError();
 }
 
