@@ -32,6 +32,36 @@ __device__ __forceinline__ cd cscale(cd a, double s) { return mk(a.x * s, a.y * 
 __device__ __forceinline__ cd mul_mi(cd a) { return mk(a.y, -a.x); }   // * (-i)
 __device__ __forceinline__ cd mul_pi(cd a) { return mk(-a.y, a.x); }   // * (+i)
 
+// ---- global-memory views ------------------------------------------------------
+// Pointers read from memory (the fields of the grid and batch structures) are
+// generic pointers, and a generic (flat) load might address LDS: the compiler keeps
+// it behind every earlier LDS store and waits for it on both counters, which
+// serialises load-compute-store loops.  Through these views the loads are global
+// loads, which it issues early and waits for one at a time.
+typedef __attribute__((address_space(1))) const double gdouble;
+typedef __attribute__((address_space(1))) const cd gcd;
+typedef __attribute__((address_space(1))) double gdouble_out;
+typedef __attribute__((address_space(1))) cd gcd_out;
+__device__ __forceinline__ gdouble *as_global(const double *p) { return (gdouble *)p; }
+__device__ __forceinline__ gcd *as_global(const cd *p) { return (gcd *)p; }
+__device__ __forceinline__ gdouble_out *as_global_out(double *p) { return (gdouble_out *)p; }
+__device__ __forceinline__ gcd_out *as_global_out(cd *p) { return (gcd_out *)p; }
+// a complex value through a global view (one 128-bit access: the type is 16-byte aligned)
+__device__ __forceinline__ cd ldg(gcd *p) { return mk(p->x, p->y); }
+__device__ __forceinline__ void stg(gcd_out *p, cd v) { p->x = v.x; p->y = v.y; }
+
+// ---- workgroup barrier for LDS traffic ---------------------------------------------
+// The transforms exchange data through LDS only.  A full __syncthreads() also waits
+// for every global load in flight; this barrier orders the LDS accesses alone, so
+// loads issued ahead of a transform (operands of the phase after it) stay in flight
+// across all its passes.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---- LDS layout ----------------------------------------------------------------
 // Complex element i of a transform lives at LDS slot pidx(i) = i + i/16: one pad slot
 // per 16 elements keeps the strided stores of the early Stockham passes (stride R
@@ -125,12 +155,12 @@ __device__ __forceinline__ cd twiddle(const cd *lt, int k) { return cmul(lt[64 +
 // fill the LDS twiddle tables of an M-point transform from tw (e^{-2 pi i k/(stride M)}):
 // stride 2 for the half-length transform of an N = 2M grid table, 1 for a table of
 // the transform's own length
-__device__ __forceinline__ void load_twiddles(cd *lt, int M, const cd *__restrict__ tw, int stride = 2)
+__device__ __forceinline__ void load_twiddles(cd *lt, int M, gcd *tw, int stride = 2)
 {
     const int nhi = (M + 63) >> 6;
     for (int i = threadIdx.x; i < 64 + nhi; i += blockDim.x) {
         const int k = (i < 64) ? i : 64 * (i - 64);
-        lt[i] = (k < M) ? tw[stride * k] : mk(1.0, 0.0);
+        lt[i] = (k < M) ? ldg(tw + stride * k) : mk(1.0, 0.0);
     }
 }
 
@@ -151,7 +181,7 @@ __device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restr
             for (int r = 0; r < R; ++r) v[b][r] = s[pidx(j + r * B)];
         }
     }
-    __syncthreads();
+    lds_barrier();
     const int stride = M / (Ns * R);
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
@@ -174,7 +204,7 @@ __device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restr
             for (int r = 0; r < R; ++r) s[pidx(base + r * Ns)] = v[b][r];
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // Butterflies per thread: the host launches at least M/8 threads, so a radix-R pass
@@ -190,22 +220,21 @@ __device__ __forceinline__ void fft_pass_dispatch(cd *s, int M, int Ns, const cd
     else fft_pass<R, 4>(s, M, Ns, lt);
 }
 
-// forward complex FFT of length M in LDS (natural order in and out); lt = LDS
-// twiddle tables (load_twiddles), valid before the call
-__device__ __forceinline__ void fft_lds(cd *s, int M, const ce_radix_plan &rp, const cd *__restrict__ lt)
+// forward complex FFT of length M in LDS (natural order in and out); code = the
+// packed radix schedule (ce_radix_plan.code, held in a register: no memory access
+// per pass); lt = LDS twiddle tables (load_twiddles), valid before the call
+__device__ __forceinline__ void fft_lds(cd *s, int M, uint32_t code, const cd *__restrict__ lt)
 {
     int Ns = 1;
-    for (int p = 0; p < rp.npass; ++p) {
-        const int R = rp.radix[p];
-        switch (R) {
-        case 8: fft_pass_dispatch<8>(s, M, Ns, lt); break;
-        case 4: fft_pass_dispatch<4>(s, M, Ns, lt); break;
-        case 2: fft_pass_dispatch<2>(s, M, Ns, lt); break;
-        case 5: fft_pass_dispatch<5>(s, M, Ns, lt); break;
-        case 3: fft_pass_dispatch<3>(s, M, Ns, lt); break;
+    for (uint32_t c = code; c != 0; c >>= 3) {
+        switch (c & 7) {
+        case 5: fft_pass_dispatch<8>(s, M, Ns, lt); Ns *= 8; break;
+        case 3: fft_pass_dispatch<4>(s, M, Ns, lt); Ns *= 4; break;
+        case 1: fft_pass_dispatch<2>(s, M, Ns, lt); Ns *= 2; break;
+        case 4: fft_pass_dispatch<5>(s, M, Ns, lt); Ns *= 5; break;
+        case 2: fft_pass_dispatch<3>(s, M, Ns, lt); Ns *= 3; break;
         default: break;
         }
-        Ns *= R;
     }
 }
 

@@ -43,11 +43,11 @@ __global__ void __launch_bounds__(1024) k_cov_ahat(int N, ce_radix_plan rp, cons
     extern __shared__ cd smem[];
     const int T = blockDim.x, tid = threadIdx.x;
     cd *lt = smem + lds_data_len(N);
-    load_twiddles(lt, N, tw, 1);
+    load_twiddles(lt, N, as_global(tw), 1);
     const cd *a = A + (int64_t)blockIdx.x * N;
     for (int i = tid; i < N; i += T) smem[pidx(i)] = cconj(a[i]);
-    __syncthreads();
-    fft_lds(smem, N, rp, lt);
+    lds_barrier();
+    fft_lds(smem, N, rp.code, lt);
     cd *o = ahat + (int64_t)blockIdx.x * N;
     for (int i = tid; i < N; i += T) o[i] = cconj(smem[pidx(i)]);
 }
@@ -108,8 +108,8 @@ __global__ void __launch_bounds__(1024) k_cov_rows(rows_args a)
     const int i = a.rows[ir];
     cd *ltN = smem + lds_data_len(N);
     cd *ltM = ltN + 64 + ((N + 63) >> 6);
-    load_twiddles(ltN, N, a.tw, 1);
-    load_twiddles(ltM, M, a.tw, 2);
+    load_twiddles(ltN, N, as_global(a.tw), 1);
+    load_twiddles(ltM, M, as_global(a.tw), 2);
     const double *dv = a.d + (int64_t)sp2 * N;
     const cd *ah = a.ahat + (int64_t)a.pl1[pr] * N;
     const cd *B = a.B + (int64_t)a.pl2[pr] * (M + 1);
@@ -119,8 +119,8 @@ __global__ void __launch_bounds__(1024) k_cov_rows(rows_args a)
         if (q >= N) q -= N;
         smem[pidx(t)] = cscale(cconj(ah[q]), dv[t]);
     }
-    __syncthreads();
-    fft_lds(smem, N, a.rpN, ltN);
+    lds_barrier();
+    fft_lds(smem, N, a.rpN.code, ltN);
     // Y = B V on n <= M with Im Y_0 = Im Y_M = 0 (dropped by the c2r), merged into the
     // half-length sequence (stored conjugated for the inverse transform)
     for (int kk = tid; kk <= M / 2; kk += T) {
@@ -131,8 +131,8 @@ __global__ void __launch_bounds__(1024) k_cov_rows(rows_args a)
         smem[pidx(k1)] = cconj(c2r_merge(Y1, Y2, a.tw[k1]));
         if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(Y2, Y1, a.tw[k2]));
     }
-    __syncthreads();
-    fft_lds(smem, M, a.rpM, ltM);
+    lds_barrier();
+    fft_lds(smem, M, a.rpM.code, ltM);
     // column functionals of this row: W[n] = sum_j V_n(j) row(j)
     const double den_i = a.prn1[i];
     const double fN = (double)N;

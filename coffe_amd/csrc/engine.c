@@ -107,7 +107,13 @@ int ce_radix_schedule(int M, ce_radix_plan *rp)
     while (m % 2 == 0) { rp->radix[rp->npass++] = 2; m /= 2; }
     while (m % 5 == 0) { if (rp->npass >= CE_MAX_PASSES) return COFFE_EUNSUPPORTED; rp->radix[rp->npass++] = 5; m /= 5; }
     while (m % 3 == 0) { if (rp->npass >= CE_MAX_PASSES) return COFFE_EUNSUPPORTED; rp->radix[rp->npass++] = 3; m /= 3; }
-    return m == 1 ? COFFE_OK : COFFE_EUNSUPPORTED;
+    if (m != 1 || rp->npass > 10) return COFFE_EUNSUPPORTED;
+    rp->code = 0;
+    for (int p = rp->npass - 1; p >= 0; --p) {
+        const uint32_t c = rp->radix[p] == 2 ? 1 : rp->radix[p] == 3 ? 2 : rp->radix[p] == 4 ? 3 : rp->radix[p] == 5 ? 4 : 5;
+        rp->code = (rp->code << 3) | c;
+    }
+    return COFFE_OK;
 }
 
 static void grid_free(ce_grid *g)

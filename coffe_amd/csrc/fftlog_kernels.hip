@@ -284,17 +284,11 @@ __device__ __forceinline__ void wave_spline_solve(const double *xs, int ns, Rhs 
     if (r0 + M - 1 < ns) X[r0 + M - 1] = sp;
 }
 
-// global-memory view of a table pointer that was itself read from memory: loads
-// through it are global (not flat) loads, whose counters the compiler can wait on
-// one at a time
-typedef __attribute__((address_space(1))) const double gmem_double;
-__device__ __forceinline__ const gmem_double *as_global(const double *p) { return (const gmem_double *)p; }
-
 // first grid index i with k0 p_i >= v (N if none): the closed form
 // i = N log(v / k0) / G, G = log(kmax / kmin), settled against the table
 __device__ __forceinline__ int grid_lower_index(const ce_grid_dev &g, double v)
 {
-    const gmem_double *p = as_global(g.p);
+    gdouble *p = as_global(g.p);
     const double k0 = g.k0;
     const int N = g.N;
     const double t = (double)N * log(v / k0) / g.G;
@@ -373,7 +367,7 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
     __shared__ double lds[3 * CE_CHUNK_NODES];
     const ce_chunk_dev ch = chunks[blockIdx.x];
     const ce_grid_dev &g = grids[ch.grid];
-    const gmem_double *gp = as_global(g.p);
+    gdouble *gp = as_global(g.p);
     const double gk0 = g.k0;
     const int n = ch.n_in;
     const double *k = k_all + ch.in_off;
@@ -464,25 +458,54 @@ __global__ void __launch_bounds__(1024) k_prepare(const ce_spec_dev *__restrict_
     extern __shared__ cd smem[];
     const ce_spec_dev sp = specs[blockIdx.x];
     const ce_grid_dev &g = grids[sp.grid];
-    const int M = g.M, N = g.N;
-    const double *yres = ws_all + sp.ws_off;
-    const cd *tw = reinterpret_cast<const cd *>(g.tw);
+    const int M = g.M;
+    const uint32_t plan = g.rp.code;
+    const int T = blockDim.x, tid = threadIdx.x;
+    // S(x_i) as pairs (S_2j, S_2j+1): complex slot j of the transform input
+    gcd *ypair = as_global(reinterpret_cast<const cd *>(ws_all + sp.ws_off));
+    gcd *tw = as_global(reinterpret_cast<const cd *>(g.tw));
     cd *lt = smem + lds_data_len(max_M);
     load_twiddles(lt, M, tw);
     for (int q = 0; q < nq; ++q) {
-        const double *bw = g.bw[q];
-        for (int i = threadIdx.x; i < N; i += blockDim.x) lds_real(smem, i) = bw[i] * yres[i];
-        __syncthreads();
-        fft_lds(smem, M, g.rp, lt);
-        cd *xf = xf_all + sp.xf_off + (int64_t)q * (M + 1);
-        for (int kk = threadIdx.x; kk <= M / 2; kk += blockDim.x) {
-            const int k1 = kk, k2 = M - kk;
-            const cd Z1 = smem[pidx(k1)];
-            const cd Z2 = smem[pidx(k2 == M ? 0 : k2)];
-            xf[k1] = cconj(r2c_split(Z1, Z2, tw[k1]));
-            if (k2 != k1) xf[k2] = cconj(r2c_split(Z2, Z1, tw[k2]));
+        gcd *bwpair = as_global(reinterpret_cast<const cd *>(g.bw[q]));
+        // four rounds of loads in flight at a time
+        for (int j0 = 0; j0 * T < M; j0 += 4) {
+            cd bv[4], yv[4];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int j = min(tid + (j0 + h) * T, M - 1);
+                bv[h] = ldg(bwpair + j);
+                yv[h] = ldg(ypair + j);
+            }
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int j = tid + (j0 + h) * T;
+                if (j < M) smem[pidx(j)] = mk(bv[h].x * yv[h].x, bv[h].y * yv[h].y);
+            }
         }
-        __syncthreads();
+        lds_barrier();
+        fft_lds(smem, M, plan, lt);
+        gcd_out *xf = as_global_out(xf_all + sp.xf_off + (int64_t)q * (M + 1));
+        for (int it0 = 0; it0 * T <= M / 2; it0 += 3) {
+            cd w1[3], w2[3];
+#pragma unroll
+            for (int h = 0; h < 3; ++h) {
+                const int kk = min(tid + (it0 + h) * T, M / 2);
+                w1[h] = ldg(tw + kk);
+                w2[h] = mk(-w1[h].x, w1[h].y);   // tw[M - kk] = -conj(tw[kk])
+            }
+#pragma unroll
+            for (int h = 0; h < 3; ++h) {
+                const int k1 = tid + (it0 + h) * T, k2 = M - k1;
+                if (k1 <= M / 2) {
+                    const cd Z1 = smem[pidx(k1)];
+                    const cd Z2 = smem[pidx(k2 == M ? 0 : k2)];
+                    stg(xf + k1, cconj(r2c_split(Z1, Z2, w1[h])));
+                    if (k2 != k1) stg(xf + k2, cconj(r2c_split(Z2, Z1, w2[h])));
+                }
+            }
+        }
+        lds_barrier();
     }
 }
 
@@ -605,61 +628,118 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     const ce_grid_dev &g = a.grids[sp.grid];
     const ce_table_desc td = a.tabs[u.tab];
     const int M = g.M, N = g.N;
+    const uint32_t plan = g.rp.code;
+    const double r0 = g.r0;
     const int T = blockDim.x, tid = threadIdx.x;
-    const cd *tw = reinterpret_cast<const cd *>(g.tw);
-    cd *lt = smem + lds_data_len(a.max_M);
-    load_twiddles(lt, M, tw);
-    const cd *U = reinterpret_cast<const cd *>(g.U[u.tab]);
-    const double *os = g.os[u.tab];
-    const cd *xf = a.xf + sp.xf_off + (int64_t)td.qidx * (M + 1);
-    // kernel multiply + c2r merge (stored conjugated: the forward FFT then gives conj(z))
-    for (int kk = tid; kk <= M / 2; kk += T) {
-        const int k1 = kk, k2 = M - kk;
-        cd X1 = cmul(xf[k1], U[k1]);
-        cd X2 = cmul(xf[k2], U[k2]);
-        if (k1 == 0) { X1.y = 0.0; X2.y = 0.0; }
-        smem[pidx(k1)] = cconj(c2r_merge(X1, X2, tw[k1]));
-        if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(X2, X1, tw[k2]));
-    }
-    __syncthreads();
-    fft_lds(smem, M, g.rp, lt);
-    if (a.full) {
-        double *out = a.full + (int64_t)blockIdx.x * a.full_stride;
-        for (int i = tid; i < N; i += T) out[i] = os[i] * c2r_value(smem, i);
-    }
+    gcd *tw = as_global(reinterpret_cast<const cd *>(g.tw));
+    gcd *U = as_global(reinterpret_cast<const cd *>(g.U[u.tab]));
+    gdouble *os = as_global(g.os[u.tab]);
+    gdouble *gp = as_global(g.p);
+    gcd *xf = as_global(a.xf + sp.xf_off + (int64_t)td.qidx * (M + 1));
+    gdouble *sep = as_global(a.sep);
     const int32_t *sidx = a.sidx + (int64_t)sp.grid * a.nsep;
     const bool adj = a.gout != nullptr;
-    // Akima at separation r; in adjoint mode also the cotangents of the six stencil
-    // values of the c2r output (value derivative * cotangent of I_t(r) * output scale)
-    auto akima_at = [&](int r, double *w) -> int {
-        const int j = sidx[r];
-        double X[6], Y[6], dY[6];
+    cd *lt = smem + lds_data_len(a.max_M);
+    load_twiddles(lt, M, tw);
+    // Akima operands of this thread's first separation, fetched before the transform
+    // (they do not depend on it): the interval first, the stencil abscissae and output
+    // scales after the kernel multiply has issued its loads, so that they arrive
+    // during the transform
+    const bool inreg = a.nsep <= 2 * T;
+    const int r_a = tid;
+    const bool has_a = inreg && r_a < a.nsep;
+    const int j_a = has_a ? sidx[r_a] : 2;
+    // kernel multiply + c2r merge (stored conjugated: the forward FFT then gives conj(z)),
+    // over the pairs (k, M - k), three rounds of loads in flight at a time; the merge
+    // twiddle of M - k is -conj of that of k (N = 2M), exactly so for the correctly
+    // rounded table
+    for (int it0 = 0; it0 * T <= M / 2; it0 += 3) {
+        cd x1[3], x2[3], u1[3], u2[3], w1[3];
+#pragma unroll
+        for (int h = 0; h < 3; ++h) {
+            const int kk = min(tid + (it0 + h) * T, M / 2);
+            x1[h] = ldg(xf + kk);
+            u1[h] = ldg(U + kk);
+            x2[h] = ldg(xf + (M - kk));
+            u2[h] = ldg(U + (M - kk));
+            w1[h] = ldg(tw + kk);
+        }
+#pragma unroll
+        for (int h = 0; h < 3; ++h) {
+            const int k1 = tid + (it0 + h) * T, k2 = M - k1;
+            if (k1 <= M / 2) {
+                cd X1 = cmul(x1[h], u1[h]);
+                cd X2 = cmul(x2[h], u2[h]);
+                if (k1 == 0) { X1.y = 0.0; X2.y = 0.0; }
+                smem[pidx(k1)] = cconj(c2r_merge(X1, X2, w1[h]));
+                if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(X2, X1, mk(-w1[h].x, w1[h].y)));
+            }
+        }
+    }
+    double X_a[6], O_a[6], sep_a = 0.0;
+    if (has_a) {
+        sep_a = sep[r_a];
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-            X[m] = g.r0 * g.p[j - 2 + m];
-            Y[m] = os[j - 2 + m] * c2r_value(smem, j - 2 + m);
+            X_a[m] = gp[j_a - 2 + m];
+            O_a[m] = os[j_a - 2 + m];
         }
-        const double v = adj ? akima_stencil<true>(X, Y, a.sep[r], dY) : akima_stencil<false>(X, Y, a.sep[r], dY);
+    }
+    lds_barrier();
+    fft_lds(smem, M, plan, lt);
+    if (a.full) {
+        gdouble_out *out = as_global_out(a.full + (int64_t)blockIdx.x * a.full_stride);
+        for (int i = tid; i < N; i += T) out[i] = os[i] * c2r_value(smem, i);
+    }
+    // Akima at separation r from gathered operands; in adjoint mode also the
+    // cotangents of the six stencil values of the c2r output (value derivative *
+    // cotangent of I_t(r) * output scale)
+    auto akima_finish = [&](int r, int j, const double *X, const double *O, double ib, double xq, double *w) {
+        double Y[6], dY[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) Y[m] = O[m] * c2r_value(smem, j - 2 + m);
+        const double v = adj ? akima_stencil<true>(X, Y, xq, dY) : akima_stencil<false>(X, Y, xq, dY);
         if (a.itab) a.itab[((int64_t)u.spec * a.ntab + u.tab) * a.nsep + r] = v;
         if (adj) {
-            double ib = 0.0;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) w[m] = ib * dY[m] * O[m];
+        }
+    };
+    auto akima_at = [&](int r, double *w) -> int {
+        const int j = sidx[r];
+        double X[6], O[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            X[m] = r0 * gp[j - 2 + m];
+            O[m] = os[j - 2 + m];
+        }
+        double ib = 0.0;
+        if (adj)
             for (int z = 0; z < a.nz; ++z)
                 for (int l = 0; l < a.nl; ++l)
                     ib += a.A[(((int64_t)z * a.nl + l) * a.ntab + u.tab) * a.nsep + r] *
                           a.gout[(((int64_t)u.spec * a.nz + z) * a.nl + l) * a.nsep + r];
-#pragma unroll
-            for (int m = 0; m < 6; ++m) w[m] = ib * dY[m] * os[j - 2 + m];
-        }
+        akima_finish(r, j, X, O, ib, sep[r], w);
         return j;
     };
     // up to two separations per thread keep their cotangents in registers across the
     // LDS clear; more separations go through the global scratch wbuf
-    const bool inreg = a.nsep <= 2 * T;
     double w0[6], w1[6];
     int j0 = -1, j1 = -1;
     double *wb = a.wbuf + (int64_t)blockIdx.x * a.nsep * 6;
     if (inreg) {
-        if (tid < a.nsep) j0 = akima_at(tid, w0);
+        if (has_a) {
+            double ib_a = 0.0;
+            if (adj)
+                for (int z = 0; z < a.nz; ++z)
+                    for (int l = 0; l < a.nl; ++l)
+                        ib_a += a.A[(((int64_t)z * a.nl + l) * a.ntab + u.tab) * a.nsep + r_a] *
+                                a.gout[(((int64_t)u.spec * a.nz + z) * a.nl + l) * a.nsep + r_a];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) X_a[m] *= r0;
+            akima_finish(r_a, j_a, X_a, O_a, ib_a, sep_a, w0);
+            j0 = j_a;
+        }
         if (tid + T < a.nsep) j1 = akima_at(tid + T, w1);
     } else {
         for (int r = tid; r < a.nsep; r += T) {
@@ -672,9 +752,9 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
         }
     }
     if (!adj) return;
-    __syncthreads();
+    lds_barrier();
     for (int i = tid; i < N; i += T) lds_real(smem, i) = 0.0;
-    __syncthreads();
+    lds_barrier();
     if (inreg) {
         if (j0 >= 0) {
 #pragma unroll
@@ -691,21 +771,35 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
             for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(smem, j - 2 + m), wb[r * 6 + m]);
         }
     }
-    __syncthreads();
+    lds_barrier();
     // transposed c2r: Xbar_k = r2c(ybar)_k, doubled for 0 < k < M, real at k = 0, M
-    fft_lds(smem, M, g.rp, lt);
-    cd *xb = a.xbar + sp.xb_off + (int64_t)u.tab * (M + 1);
-    for (int kk = tid; kk <= M / 2; kk += T) {
-        const int k1 = kk, k2 = M - kk;
-        const cd Z1 = smem[pidx(k1)];
-        const cd Z2 = smem[pidx(k2 == M ? 0 : k2)];
-        cd R1 = r2c_split(Z1, Z2, tw[k1]);
-        cd R2 = r2c_split(Z2, Z1, tw[k2]);
-        R1 = (k1 == 0) ? mk(R1.x, 0.0) : cscale(R1, 2.0);
-        R2 = (k2 == M) ? mk(R2.x, 0.0) : cscale(R2, 2.0);
-        // X = xf * U  ->  xf_bar = conj(U) * X_bar
-        xb[k1] = cmul(cconj(U[k1]), R1);
-        if (k2 != k1) xb[k2] = cmul(cconj(U[k2]), R2);
+    fft_lds(smem, M, plan, lt);
+    gcd_out *xb = as_global_out(a.xbar + sp.xb_off + (int64_t)u.tab * (M + 1));
+    for (int it0 = 0; it0 * T <= M / 2; it0 += 3) {
+        cd u1[3], u2[3], w1[3], w2[3];
+#pragma unroll
+        for (int h = 0; h < 3; ++h) {
+            const int kk = min(tid + (it0 + h) * T, M / 2);
+            u1[h] = ldg(U + kk);
+            u2[h] = ldg(U + (M - kk));
+            w1[h] = ldg(tw + kk);
+            w2[h] = mk(-w1[h].x, w1[h].y);   // tw[M - kk] = -conj(tw[kk])
+        }
+#pragma unroll
+        for (int h = 0; h < 3; ++h) {
+            const int k1 = tid + (it0 + h) * T, k2 = M - k1;
+            if (k1 <= M / 2) {
+                const cd Z1 = smem[pidx(k1)];
+                const cd Z2 = smem[pidx(k2 == M ? 0 : k2)];
+                cd R1 = r2c_split(Z1, Z2, w1[h]);
+                cd R2 = r2c_split(Z2, Z1, w2[h]);
+                R1 = (k1 == 0) ? mk(R1.x, 0.0) : cscale(R1, 2.0);
+                R2 = (k2 == M) ? mk(R2.x, 0.0) : cscale(R2, 2.0);
+                // X = xf * U  ->  xf_bar = conj(U) * X_bar
+                stg(xb + k1, cmul(cconj(u1[h]), R1));
+                if (k2 != k1) stg(xb + k2, cmul(cconj(u2[h]), R2));
+            }
+        }
     }
 }
 
@@ -739,35 +833,70 @@ __global__ void __launch_bounds__(1024) k_adj_c2r(const ce_spec_dev *__restrict_
     const ce_spec_dev sp = specs[blockIdx.x];
     const ce_grid_dev &g = grids[sp.grid];
     const int M = g.M, N = g.N;
+    const uint32_t plan = g.rp.code;
     const int T = blockDim.x, tid = threadIdx.x;
-    double *sbar = ws_all + sp.ws_off + N;
-    const cd *tw = reinterpret_cast<const cd *>(g.tw);
+    // the cotangent of S(x_i) as pairs (2j, 2j+1)
+    gcd_out *sbar = as_global_out(reinterpret_cast<cd *>(ws_all + sp.ws_off + N));
+    gcd *tw = as_global(reinterpret_cast<const cd *>(g.tw));
     cd *lt = smem + lds_data_len(max_M);
     load_twiddles(lt, M, tw);
-    const cd *xbar = xbar_all + sp.xb_off;
+    gcd *xbar = as_global(xbar_all + sp.xb_off);
     for (int q = 0; q < nq; ++q) {
-        for (int kk = tid; kk <= M / 2; kk += T) {
-            const int k1 = kk, k2 = M - kk;
-            cd S1 = mk(0, 0), S2 = mk(0, 0);
+        // the tables' cotangent spectra summed per pair (k, M - k), two rounds of loads
+        // in flight at a time
+        for (int it0 = 0; it0 * T <= M / 2; it0 += 2) {
+            cd S1[2], S2[2], w1[2], w2[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int kk = min(tid + (it0 + h) * T, M / 2);
+                w1[h] = ldg(tw + kk);
+                w2[h] = mk(-w1[h].x, w1[h].y);   // tw[M - kk] = -conj(tw[kk])
+                S1[h] = mk(0.0, 0.0);
+                S2[h] = mk(0.0, 0.0);
+            }
             for (int t = 0; t < ntab; ++t) {
                 if (tabs[t].qidx != q) continue;
-                S1 = cadd(S1, xbar[(int64_t)t * (M + 1) + k1]);
-                S2 = cadd(S2, xbar[(int64_t)t * (M + 1) + k2]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int kk = min(tid + (it0 + h) * T, M / 2);
+                    S1[h] = cadd(S1[h], ldg(xbar + ((int64_t)t * (M + 1) + kk)));
+                    S2[h] = cadd(S2[h], ldg(xbar + ((int64_t)t * (M + 1) + M - kk)));
+                }
             }
-            cd Y1 = cconj(S1), Y2 = cconj(S2);
-            Y1 = (k1 == 0) ? mk(Y1.x, 0.0) : cscale(Y1, 0.5);
-            Y2 = (k2 == M) ? mk(Y2.x, 0.0) : cscale(Y2, 0.5);
-            smem[pidx(k1)] = cconj(c2r_merge(Y1, Y2, tw[k1]));
-            if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(Y2, Y1, tw[k2]));
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k1 = tid + (it0 + h) * T, k2 = M - k1;
+                if (k1 <= M / 2) {
+                    cd Y1 = cconj(S1[h]), Y2 = cconj(S2[h]);
+                    Y1 = (k1 == 0) ? mk(Y1.x, 0.0) : cscale(Y1, 0.5);
+                    Y2 = (k2 == M) ? mk(Y2.x, 0.0) : cscale(Y2, 0.5);
+                    smem[pidx(k1)] = cconj(c2r_merge(Y1, Y2, w1[h]));
+                    if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(Y2, Y1, w2[h]));
+                }
+            }
         }
-        __syncthreads();
-        fft_lds(smem, M, g.rp, lt);
-        const double *bw = g.bw[q];
-        for (int i = tid; i < N; i += T) {
-            const double v = bw[i] * c2r_value(smem, i);
-            sbar[i] = (q == 0) ? v : sbar[i] + v;
+        lds_barrier();
+        fft_lds(smem, M, plan, lt);
+        gcd *bwpair = as_global(reinterpret_cast<const cd *>(g.bw[q]));
+        for (int j0 = 0; j0 * T < M; j0 += 4) {
+            cd bv[4], pv[4];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int j = min(tid + (j0 + h) * T, M - 1);
+                bv[h] = ldg(bwpair + j);
+                pv[h] = (q == 0) ? mk(0.0, 0.0) : mk(sbar[j].x, sbar[j].y);
+            }
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int j = tid + (j0 + h) * T;
+                if (j < M) {
+                    // c2r outputs 2j, 2j+1 of slot j (the transform holds their conjugate)
+                    const cd z = smem[pidx(j)];
+                    stg(sbar + j, mk(bv[h].x * z.x + pv[h].x, bv[h].y * (-z.y) + pv[h].y));
+                }
+            }
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -786,7 +915,7 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
     __shared__ double lds[3 * CE_CHUNK_NODES];
     const ce_chunk_dev ch = chunks[blockIdx.x];
     const ce_grid_dev &g = grids[ch.grid];
-    const gmem_double *gp = as_global(g.p);
+    gdouble *gp = as_global(g.p);
     const double gk0 = g.k0;
     const int n = ch.n_in, N = g.N;
     const double *k = k_all + ch.in_off;
