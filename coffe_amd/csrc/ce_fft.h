@@ -165,13 +165,12 @@ __device__ __forceinline__ void load_twiddles(cd *lt, int M, gcd *tw, int stride
 }
 
 // One Stockham pass with radix R over s[0..M); Ns = product of earlier radices;
-// lt holds the LDS twiddle tables.
+// lt holds the LDS twiddle tables; threads tid < T of the group own the transform
+// (every thread of the workgroup calls: the pass ends on workgroup barriers).
 template <int R, int MAXB>
-__device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restrict__ lt)
+__device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restrict__ lt, int tid, int T)
 {
     const int B = M / R;
-    const int T = blockDim.x;
-    const int tid = threadIdx.x;
     cd v[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
@@ -207,35 +206,40 @@ __device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restr
     lds_barrier();
 }
 
-// Butterflies per thread: the host launches at least M/8 threads, so a radix-R pass
+// Butterflies per thread: a transform gets at least M/8 threads, so a radix-R pass
 // has at most ceil(8/R) butterflies per thread; only those register shapes are
 // instantiated (at most 12 complex values live per thread).
 template <int R>
-__device__ __forceinline__ void fft_pass_dispatch(cd *s, int M, int Ns, const cd *__restrict__ lt)
+__device__ __forceinline__ void fft_pass_dispatch(cd *s, int M, int Ns, const cd *__restrict__ lt, int tid, int T)
 {
     const int B = M / R;
-    const int per = (B + (int)blockDim.x - 1) / (int)blockDim.x;
-    if (R == 8 || per <= 1) fft_pass<R, 1>(s, M, Ns, lt);
-    else if (R == 4 || R == 5 || per <= 2) fft_pass<R, 2>(s, M, Ns, lt);
-    else fft_pass<R, 4>(s, M, Ns, lt);
+    const int per = (B + T - 1) / T;
+    if (R == 8 || per <= 1) fft_pass<R, 1>(s, M, Ns, lt, tid, T);
+    else if (R == 4 || R == 5 || per <= 2) fft_pass<R, 2>(s, M, Ns, lt, tid, T);
+    else fft_pass<R, 4>(s, M, Ns, lt, tid, T);
 }
 
-// forward complex FFT of length M in LDS (natural order in and out); code = the
+// forward complex FFT of length M in LDS (natural order in and out) by the threads
+// tid < T of a group (T >= M/8; every thread of the workgroup calls); code = the
 // packed radix schedule (ce_radix_plan.code, held in a register: no memory access
 // per pass); lt = LDS twiddle tables (load_twiddles), valid before the call
-__device__ __forceinline__ void fft_lds(cd *s, int M, uint32_t code, const cd *__restrict__ lt)
+__device__ __forceinline__ void fft_lds(cd *s, int M, uint32_t code, const cd *__restrict__ lt, int tid, int T)
 {
     int Ns = 1;
     for (uint32_t c = code; c != 0; c >>= 3) {
         switch (c & 7) {
-        case 5: fft_pass_dispatch<8>(s, M, Ns, lt); Ns *= 8; break;
-        case 3: fft_pass_dispatch<4>(s, M, Ns, lt); Ns *= 4; break;
-        case 1: fft_pass_dispatch<2>(s, M, Ns, lt); Ns *= 2; break;
-        case 4: fft_pass_dispatch<5>(s, M, Ns, lt); Ns *= 5; break;
-        case 2: fft_pass_dispatch<3>(s, M, Ns, lt); Ns *= 3; break;
+        case 5: fft_pass_dispatch<8>(s, M, Ns, lt, tid, T); Ns *= 8; break;
+        case 3: fft_pass_dispatch<4>(s, M, Ns, lt, tid, T); Ns *= 4; break;
+        case 1: fft_pass_dispatch<2>(s, M, Ns, lt, tid, T); Ns *= 2; break;
+        case 4: fft_pass_dispatch<5>(s, M, Ns, lt, tid, T); Ns *= 5; break;
+        case 2: fft_pass_dispatch<3>(s, M, Ns, lt, tid, T); Ns *= 3; break;
         default: break;
         }
     }
+}
+__device__ __forceinline__ void fft_lds(cd *s, int M, uint32_t code, const cd *__restrict__ lt)
+{
+    fft_lds(s, M, code, lt, threadIdx.x, blockDim.x);
 }
 
 // r2c split step: from Z = FFT_M(z), z_j = x_{2j} + i x_{2j+1}, produce X_k for the

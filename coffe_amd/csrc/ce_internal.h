@@ -82,6 +82,11 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
                           int ntab, const ce_table_desc *tabs, const double *tab_qnu, int nq, const double *qprime,
                           double *const *d_U, double *const *d_os, double *const *d_bw);
 
+/* d_packs (4 ints per workgroup: first unit, unit count, groups, 0; NULL: one unit
+   per workgroup): units of one transform length share a workgroup, one group of
+   threads / groups threads and lds_data_len(M) complex of LDS each; lt_off = LDS
+   offset of the twiddle tables (complex), at least groups * lds_data_len(M) of every
+   pack (0: lds_data_len(max_M)) */
 /* d_ranges (4 ints per chunk, may be NULL): the resample index ranges of the chunk's
    owned and staged intervals, written for ce_launch_spline_adj */
 int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
@@ -95,7 +100,7 @@ int ce_launch_tables(void *stream, int nunit, const ce_unit_dev *d_units, const 
                      const ce_grid_dev *d_grids, const ce_table_desc *d_tabs, int ntab, const double *d_xf,
                      int nsep, const int32_t *d_sidx, const double *d_sep, double *d_itab, double *d_full,
                      int64_t full_stride, const double *d_A, const double *d_gout, int nz, int nl, double *d_xbar,
-                     double *d_wbuf, int threads, int max_M);
+                     double *d_wbuf, int threads, int max_M, const int32_t *d_packs, int npack, int lt_off);
 
 int ce_launch_project(void *stream, int nspec, int ntab, int nz, int nl, int nsep, const double *d_A,
                       const double *d_itab, double *d_out);

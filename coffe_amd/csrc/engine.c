@@ -329,6 +329,8 @@ struct coffe_batch {
     ce_unit_dev *d_units;
     ce_chunk_dev *d_chunks;
     int32_t *d_ranges; /* resample index ranges per chunk (4 each), from the prepare stage */
+    int32_t *d_packs;  /* table workgroups: first unit, unit count, groups, 0 */
+    int npack, lt_off;
     int32_t *d_sidx;
     double *d_sep;
     double *d_ws, *d_xf, *d_xb, *d_wbuf, *d_itab;
@@ -348,6 +350,7 @@ void coffe_batch_destroy(coffe_batch *b)
     hipFree(b->d_units);
     hipFree(b->d_chunks);
     hipFree(b->d_ranges);
+    hipFree(b->d_packs);
     hipFree(b->d_sidx);
     hipFree(b->d_sep);
     hipFree(b->d_ws);
@@ -508,7 +511,43 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_xb, 2 * sizeof(double) * (size_t)xb_off);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_wbuf, 6 * sizeof(double) * (size_t)b->nunit * (size_t)nsep);
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_itab, sizeof(double) * (size_t)nspec * ntab * nsep);
+    /* table workgroups: consecutive units (sorted by size) of one transform length
+       share a workgroup, up to the largest power of two P with threads / P >= M/8
+       (each group keeps M/8 threads or more) and threads / P >= nsep (the Akima
+       phase keeps one separation per thread) and 16 at most, so that the small transforms of a batch run many to a
+       CU instead of one (the LDS of one workgroup holds them all) */
+    int32_t *packs = NULL;
+    if (st == COFFE_OK) {
+        packs = (int32_t *)malloc(4 * sizeof(int32_t) * (size_t)b->nunit);
+        if (!packs) st = COFFE_ENOMEM;
+        const char *pe = getenv("COFFE_ENGINE_PACK");
+        const int allow = !(pe && pe[0] == '0');
+        b->npack = 0;
+        b->lt_off = (int)(b->max_M + b->max_M / 16 + 1);
+        for (int i = 0; i < b->nunit && st == COFFE_OK;) {
+            const int M = (int)(n_fft[units[i].spec] / 2);
+            const int per = (M + 7) / 8;
+            /* a power of two (the thread count is one): groups of exactly threads / P,
+               whole wavefronts (64 threads or more) */
+            int P = 1;
+            while (allow && P < 16 && 2 * P * per <= b->threads && b->threads / (2 * P) >= 64 &&
+                   2 * P * nsep <= b->threads)
+                P *= 2;
+            int c = 1;
+            while (c < P && i + c < b->nunit && n_fft[units[i + c].spec] / 2 == M) ++c;
+            int32_t *pk = packs + 4 * b->npack++;
+            pk[0] = i;
+            pk[1] = c;
+            pk[2] = P;
+            pk[3] = 0;
+            const int need = P * (M + M / 16 + 1);
+            if (need > b->lt_off) b->lt_off = need;
+            i += c;
+        }
+    }
+    if (st == COFFE_OK) st = dev_alloc((void **)&b->d_packs, 4 * sizeof(int32_t) * (size_t)(b->npack > 0 ? b->npack : 1));
     hipStream_t s = e->stream;
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_packs, packs, 4 * sizeof(int32_t) * (size_t)b->npack, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_spec, b->h_spec, sizeof(ce_spec_dev) * (size_t)nspec, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_grids, hg, sizeof(ce_grid_dev) * (size_t)b->ngrid, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_tabs, tabs, sizeof(ce_table_desc) * (size_t)ntab, hipMemcpyHostToDevice, s));
@@ -519,6 +558,7 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
     if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
     free(spec_grid);
     free(sidx);
+    free(packs);
     free(units);
     free(chunks);
     free(hg);
@@ -582,7 +622,7 @@ int coffe_batch_tables_stage(coffe_batch *b, const double *d_A, const double *d_
     double *itab = d_itab ? d_itab : b->d_itab;
     return ce_launch_tables(pick_stream(b, stream), b->nunit, b->d_units, b->d_spec, b->d_grids, b->d_tabs, b->ntab,
                             b->d_xf, b->nsep, b->d_sidx, b->d_sep, itab, NULL, 0, d_gout ? d_A : NULL, d_gout,
-                            b->nz, b->nl, b->d_xb, b->d_wbuf, b->threads, b->max_M)
+                            b->nz, b->nl, b->d_xb, b->d_wbuf, b->threads, b->max_M, b->d_packs, b->npack, b->lt_off)
                ? COFFE_EDEVICE
                : COFFE_OK;
 }
@@ -618,7 +658,7 @@ int coffe_batch_forward(coffe_batch *b, const double *d_k, const double *d_pk, c
     if (st) return st;
     if (ce_launch_tables(s, b->nunit, b->d_units, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->d_xf, b->nsep,
                          b->d_sidx, b->d_sep, itab, NULL, 0, NULL, NULL, b->nz, b->nl, NULL, b->d_wbuf, b->threads,
-                         b->max_M))
+                         b->max_M, b->d_packs, b->npack, b->lt_off))
         return COFFE_EDEVICE;
     if (d_out && ce_launch_project(s, b->nspec, b->ntab, b->nz, b->nl, b->nsep, d_A, itab, d_out)) return COFFE_EDEVICE;
     return COFFE_OK;
@@ -634,7 +674,7 @@ int coffe_batch_vjp(coffe_batch *b, const double *d_k, const double *d_pk, const
     if (st) return st;
     if (ce_launch_tables(s, b->nunit, b->d_units, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->d_xf, b->nsep,
                          b->d_sidx, b->d_sep, itab, NULL, 0, d_A, d_gout, b->nz, b->nl, b->d_xb, b->d_wbuf,
-                         b->threads, b->max_M))
+                         b->threads, b->max_M, b->d_packs, b->npack, b->lt_off))
         return COFFE_EDEVICE;
     if (d_out && ce_launch_project(s, b->nspec, b->ntab, b->nz, b->nl, b->nsep, d_A, itab, d_out)) return COFFE_EDEVICE;
     return coffe_batch_prepare_adj(b, d_k, d_pk, d_gpk, s);
@@ -659,7 +699,7 @@ int coffe_batch_tables(coffe_batch *b, const double *d_k, const double *d_pk, do
     if (st == COFFE_OK &&
         ce_launch_tables(s, b->nunit, d, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->d_xf, b->nsep, b->d_sidx,
                          b->d_sep, NULL, d_full, b->max_N, NULL, NULL, b->nz, b->nl, NULL, b->d_wbuf, b->threads,
-                         b->max_M))
+                         b->max_M, NULL, 0, 0))
         st = COFFE_EDEVICE;
     if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
     hipFree(d);
@@ -765,7 +805,8 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     if (st == COFFE_OK && ce_launch_prepare(s, 1, d_spec, d_grid, 1, base + off_ws, base + off_xf, threads, M))
         st = COFFE_EDEVICE;
     if (st == COFFE_OK && ce_launch_tables(s, 1, d_unit, d_spec, d_grid, d_tab, 1, base + off_xf, 0, NULL, NULL,
-                                           NULL, base + off_out, N, NULL, NULL, 1, 1, NULL, NULL, threads, M))
+                                           NULL, base + off_out, N, NULL, NULL, 1, 1, NULL, NULL, threads, M, NULL, 0,
+                                           0))
         st = COFFE_EDEVICE;
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(output_y, base + off_out, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, s));
     if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));

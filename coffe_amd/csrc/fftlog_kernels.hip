@@ -617,20 +617,38 @@ struct table_args {
     int nz, nl;
     cd *xbar;              // adjoint output, per spec ntab*(M+1) at specs[].xb_off
     double *wbuf;          // [unit][nsep][6] scratch
-    int max_M;             // LDS data region length (complex); twiddle tables follow it
+    int max_M;
+    const int4 *packs;     // per workgroup: first unit, unit count, groups (null: one unit each)
+    int lt_off;            // LDS offset (complex) of the twiddle tables, after the groups' data
 };
 
 __global__ void __launch_bounds__(1024) k_tables(table_args a)
 {
     extern __shared__ cd smem[];
-    const ce_unit_dev u = a.units[blockIdx.x];
+    // the workgroup's units: `count` units of one transform length, one per group of
+    // blockDim.x / P threads with its own LDS region (groups past count repeat the
+    // first unit and store nothing, so that every group passes the same barriers)
+    int first = blockIdx.x, count = 1, P = 1;
+    if (a.packs) {
+        const int4 pk = a.packs[blockIdx.x];
+        first = pk.x;
+        count = pk.y;
+        P = pk.z;
+    }
+    const int T = blockDim.x / P;
+    // groups are whole wavefronts: the group index (and all that follows from it) is
+    // uniform across the wavefront
+    const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / T);
+    const int tid = threadIdx.x - grp * T;
+    const bool live = grp < count;
+    const int ui = first + (live ? grp : 0);
+    const ce_unit_dev u = a.units[ui];
     const ce_spec_dev sp = a.specs[u.spec];
     const ce_grid_dev &g = a.grids[sp.grid];
     const ce_table_desc td = a.tabs[u.tab];
     const int M = g.M, N = g.N;
     const uint32_t plan = g.rp.code;
     const double r0 = g.r0;
-    const int T = blockDim.x, tid = threadIdx.x;
     gcd *tw = as_global(reinterpret_cast<const cd *>(g.tw));
     gcd *U = as_global(reinterpret_cast<const cd *>(g.U[u.tab]));
     gdouble *os = as_global(g.os[u.tab]);
@@ -639,12 +657,10 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     gdouble *sep = as_global(a.sep);
     const int32_t *sidx = a.sidx + (int64_t)sp.grid * a.nsep;
     const bool adj = a.gout != nullptr;
-    cd *lt = smem + lds_data_len(a.max_M);
-    load_twiddles(lt, M, tw);
-    // Akima operands of this thread's first separation, fetched before the transform
-    // (they do not depend on it): the interval first, the stencil abscissae and output
-    // scales after the kernel multiply has issued its loads, so that they arrive
-    // during the transform
+    cd *lt = smem + a.lt_off;
+    load_twiddles(lt, M, tw);   // by the whole workgroup: its groups share M
+    cd *sm = smem + grp * lds_data_len(M);
+    // the interval of this thread's first separation, fetched ahead of the transform
     const bool inreg = a.nsep <= 2 * T;
     const int r_a = tid;
     const bool has_a = inreg && r_a < a.nsep;
@@ -671,25 +687,17 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
                 cd X1 = cmul(x1[h], u1[h]);
                 cd X2 = cmul(x2[h], u2[h]);
                 if (k1 == 0) { X1.y = 0.0; X2.y = 0.0; }
-                smem[pidx(k1)] = cconj(c2r_merge(X1, X2, w1[h]));
-                if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(X2, X1, mk(-w1[h].x, w1[h].y)));
+                sm[pidx(k1)] = cconj(c2r_merge(X1, X2, w1[h]));
+                if (k2 != k1 && k2 < M) sm[pidx(k2)] = cconj(c2r_merge(X2, X1, mk(-w1[h].x, w1[h].y)));
             }
         }
     }
-    double X_a[6], O_a[6], sep_a = 0.0;
-    if (has_a) {
-        sep_a = sep[r_a];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-            X_a[m] = gp[j_a - 2 + m];
-            O_a[m] = os[j_a - 2 + m];
-        }
-    }
     lds_barrier();
-    fft_lds(smem, M, plan, lt);
+    fft_lds(sm, M, plan, lt, tid, T);
     if (a.full) {
-        gdouble_out *out = as_global_out(a.full + (int64_t)blockIdx.x * a.full_stride);
-        for (int i = tid; i < N; i += T) out[i] = os[i] * c2r_value(smem, i);
+        gdouble_out *out = as_global_out(a.full + (int64_t)ui * a.full_stride);
+        if (live)
+            for (int i = tid; i < N; i += T) out[i] = os[i] * c2r_value(sm, i);
     }
     // Akima at separation r from gathered operands; in adjoint mode also the
     // cotangents of the six stencil values of the c2r output (value derivative *
@@ -697,9 +705,9 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     auto akima_finish = [&](int r, int j, const double *X, const double *O, double ib, double xq, double *w) {
         double Y[6], dY[6];
 #pragma unroll
-        for (int m = 0; m < 6; ++m) Y[m] = O[m] * c2r_value(smem, j - 2 + m);
+        for (int m = 0; m < 6; ++m) Y[m] = O[m] * c2r_value(sm, j - 2 + m);
         const double v = adj ? akima_stencil<true>(X, Y, xq, dY) : akima_stencil<false>(X, Y, xq, dY);
-        if (a.itab) a.itab[((int64_t)u.spec * a.ntab + u.tab) * a.nsep + r] = v;
+        if (a.itab && live) a.itab[((int64_t)u.spec * a.ntab + u.tab) * a.nsep + r] = v;
         if (adj) {
 #pragma unroll
             for (int m = 0; m < 6; ++m) w[m] = ib * dY[m] * O[m];
@@ -726,26 +734,31 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     // LDS clear; more separations go through the global scratch wbuf
     double w0[6], w1[6];
     int j0 = -1, j1 = -1;
-    double *wb = a.wbuf + (int64_t)blockIdx.x * a.nsep * 6;
+    double *wb = a.wbuf + (int64_t)ui * a.nsep * 6;
     if (inreg) {
         if (has_a) {
+            double X_a[6], O_a[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                X_a[m] = r0 * gp[j_a - 2 + m];
+                O_a[m] = os[j_a - 2 + m];
+            }
             double ib_a = 0.0;
             if (adj)
                 for (int z = 0; z < a.nz; ++z)
                     for (int l = 0; l < a.nl; ++l)
                         ib_a += a.A[(((int64_t)z * a.nl + l) * a.ntab + u.tab) * a.nsep + r_a] *
                                 a.gout[(((int64_t)u.spec * a.nz + z) * a.nl + l) * a.nsep + r_a];
-#pragma unroll
-            for (int m = 0; m < 6; ++m) X_a[m] *= r0;
-            akima_finish(r_a, j_a, X_a, O_a, ib_a, sep_a, w0);
+            akima_finish(r_a, j_a, X_a, O_a, ib_a, sep[r_a], w0);
             j0 = j_a;
         }
         if (tid + T < a.nsep) j1 = akima_at(tid + T, w1);
     } else {
+#pragma unroll 2
         for (int r = tid; r < a.nsep; r += T) {
             double w[6];
             akima_at(r, w);
-            if (adj) {
+            if (adj && live) {
 #pragma unroll
                 for (int m = 0; m < 6; ++m) wb[r * 6 + m] = w[m];
             }
@@ -753,27 +766,27 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     }
     if (!adj) return;
     lds_barrier();
-    for (int i = tid; i < N; i += T) lds_real(smem, i) = 0.0;
+    for (int i = tid; i < N; i += T) lds_real(sm, i) = 0.0;
     lds_barrier();
     if (inreg) {
         if (j0 >= 0) {
 #pragma unroll
-            for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(smem, j0 - 2 + m), w0[m]);
+            for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(sm, j0 - 2 + m), w0[m]);
         }
         if (j1 >= 0) {
 #pragma unroll
-            for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(smem, j1 - 2 + m), w1[m]);
+            for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(sm, j1 - 2 + m), w1[m]);
         }
     } else {
         for (int r = tid; r < a.nsep; r += T) {
             const int j = sidx[r];
 #pragma unroll
-            for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(smem, j - 2 + m), wb[r * 6 + m]);
+            for (int m = 0; m < 6; ++m) atomicAdd(&lds_real(sm, j - 2 + m), live ? wb[r * 6 + m] : 0.0);
         }
     }
     lds_barrier();
     // transposed c2r: Xbar_k = r2c(ybar)_k, doubled for 0 < k < M, real at k = 0, M
-    fft_lds(smem, M, plan, lt);
+    fft_lds(sm, M, plan, lt, tid, T);
     gcd_out *xb = as_global_out(a.xbar + sp.xb_off + (int64_t)u.tab * (M + 1));
     for (int it0 = 0; it0 * T <= M / 2; it0 += 3) {
         cd u1[3], u2[3], w1[3], w2[3];
@@ -788,9 +801,9 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
 #pragma unroll
         for (int h = 0; h < 3; ++h) {
             const int k1 = tid + (it0 + h) * T, k2 = M - k1;
-            if (k1 <= M / 2) {
-                const cd Z1 = smem[pidx(k1)];
-                const cd Z2 = smem[pidx(k2 == M ? 0 : k2)];
+            if (k1 <= M / 2 && live) {
+                const cd Z1 = sm[pidx(k1)];
+                const cd Z2 = sm[pidx(k2 == M ? 0 : k2)];
                 cd R1 = r2c_split(Z1, Z2, w1[h]);
                 cd R2 = r2c_split(Z2, Z1, w2[h]);
                 R1 = (k1 == 0) ? mk(R1.x, 0.0) : cscale(R1, 2.0);
@@ -1094,7 +1107,7 @@ int ce_launch_tables(void *stream, int nunit, const ce_unit_dev *d_units, const 
                      const ce_grid_dev *d_grids, const ce_table_desc *d_tabs, int ntab, const double *d_xf,
                      int nsep, const int32_t *d_sidx, const double *d_sep, double *d_itab, double *d_full,
                      int64_t full_stride, const double *d_A, const double *d_gout, int nz, int nl, double *d_xbar,
-                     double *d_wbuf, int threads, int max_M)
+                     double *d_wbuf, int threads, int max_M, const int32_t *d_packs, int npack, int lt_off)
 {
     table_args a;
     a.units = d_units;
@@ -1116,10 +1129,12 @@ int ce_launch_tables(void *stream, int nunit, const ce_unit_dev *d_units, const 
     a.xbar = reinterpret_cast<cd *>(d_xbar);
     a.wbuf = d_wbuf;
     a.max_M = max_M;
-    const size_t lds = fft_lds_bytes(max_M);
+    a.packs = reinterpret_cast<const int4 *>(d_packs);
+    a.lt_off = (d_packs && lt_off > 0) ? lt_off : lds_data_len(max_M);
+    const size_t lds = ((size_t)a.lt_off + 64 + (size_t)(max_M + 63) / 64) * sizeof(cd);
     int st = set_lds((const void *)k_tables, lds);
     if (st) return st;
-    hipLaunchKernelGGL(k_tables, dim3(nunit), dim3(threads), lds, (hipStream_t)stream, a);
+    hipLaunchKernelGGL(k_tables, dim3(d_packs ? npack : nunit), dim3(threads), lds, (hipStream_t)stream, a);
     return (int)hipGetLastError();
 }
 
