@@ -94,7 +94,7 @@ struct rows_args {
     const double *prn1, *prn2;  // r_j^nu1, r_j^nu2
     int R;
     const int *flo, *fcnt, *foff;   // functional n: first node, node count, weight offset
-    const double *fw;
+    const double *fw;        // weights times pi / (16 N^2 r_j^nu2)
     double *W;               // [nblk][npair][nrows][R]
 };
 
@@ -106,48 +106,80 @@ __global__ void __launch_bounds__(1024) k_cov_rows(rows_args a)
     const int ir = blockIdx.x, pr = blockIdx.y, sp2 = blockIdx.z;
     const int N = a.N, M = a.M;
     const int i = a.rows[ir];
+    gcd *tw = as_global(a.tw);
     cd *ltN = smem + lds_data_len(N);
     cd *ltM = ltN + 64 + ((N + 63) >> 6);
-    load_twiddles(ltN, N, as_global(a.tw), 1);
-    load_twiddles(ltM, M, as_global(a.tw), 2);
-    const double *dv = a.d + (int64_t)sp2 * N;
-    const cd *ah = a.ahat + (int64_t)a.pl1[pr] * N;
-    const cd *B = a.B + (int64_t)a.pl2[pr] * (M + 1);
-    // conj(u_t), u_t = d_t Ahat((i + t) mod N): the forward FFT then gives conj(V)
-    for (int t = tid; t < N; t += T) {
-        int q = i + t;
-        if (q >= N) q -= N;
-        smem[pidx(t)] = cscale(cconj(ah[q]), dv[t]);
+    load_twiddles(ltN, N, tw, 1);
+    load_twiddles(ltM, M, tw, 2);
+    gdouble *dv = as_global(a.d + (int64_t)sp2 * N);
+    gcd *ah = as_global(a.ahat + (int64_t)a.pl1[pr] * N);
+    gcd *B = as_global(a.B + (int64_t)a.pl2[pr] * (M + 1));
+    // conj(u_t), u_t = d_t Ahat((i + t) mod N): the forward FFT then gives conj(V);
+    // four rounds of loads in flight at a time
+    for (int t0 = 0; t0 * T < N; t0 += 4) {
+        cd av[4];
+        double dd[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int t = min(tid + (t0 + h) * T, N - 1);
+            int q = i + t;
+            if (q >= N) q -= N;
+            av[h] = ldg(ah + q);
+            dd[h] = dv[t];
+        }
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int t = tid + (t0 + h) * T;
+            if (t < N) smem[pidx(t)] = cscale(cconj(av[h]), dd[h]);
+        }
     }
     lds_barrier();
     fft_lds(smem, N, a.rpN.code, ltN);
     // Y = B V on n <= M with Im Y_0 = Im Y_M = 0 (dropped by the c2r), merged into the
     // half-length sequence (stored conjugated for the inverse transform)
-    for (int kk = tid; kk <= M / 2; kk += T) {
-        const int k1 = kk, k2 = M - kk;
-        cd Y1 = cmul(B[k1], cconj(smem[pidx(k1)]));
-        cd Y2 = cmul(B[k2], cconj(smem[pidx(k2)]));
-        if (k1 == 0) { Y1.y = 0.0; Y2.y = 0.0; }
-        smem[pidx(k1)] = cconj(c2r_merge(Y1, Y2, a.tw[k1]));
-        if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(Y2, Y1, a.tw[k2]));
+    for (int it0 = 0; it0 * T <= M / 2; it0 += 2) {
+        cd b1[2], b2[2], w1[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int kk = min(tid + (it0 + h) * T, M / 2);
+            b1[h] = ldg(B + kk);
+            b2[h] = ldg(B + (M - kk));
+            w1[h] = ldg(tw + kk);
+        }
+        // each pair (k, M - k) is read and rewritten by its own thread
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k1 = tid + (it0 + h) * T, k2 = M - k1;
+            if (k1 <= M / 2) {
+                cd Y1 = cmul(b1[h], cconj(smem[pidx(k1)]));
+                cd Y2 = cmul(b2[h], cconj(smem[pidx(k2)]));
+                if (k1 == 0) { Y1.y = 0.0; Y2.y = 0.0; }
+                // e^{-2 pi i (M - k)/N} = -conj(e^{-2 pi i k/N}) (N = 2M)
+                smem[pidx(k1)] = cconj(c2r_merge(Y1, Y2, w1[h]));
+                if (k2 != k1 && k2 < M) smem[pidx(k2)] = cconj(c2r_merge(Y2, Y1, mk(-w1[h].x, w1[h].y)));
+            }
+        }
     }
     lds_barrier();
     fft_lds(smem, M, a.rpM.code, ltM);
-    // column functionals of this row: W[n] = sum_j V_n(j) row(j)
+    // column functionals of this row: W[n] = sum_j V_n(j) row(j), the weights carrying
+    // pi / (16 N^2 r_j^nu2); four threads per functional, combined by lane shuffles
     const double den_i = a.prn1[i];
-    const double fN = (double)N;
-    double *Wo = a.W + (((int64_t)sp2 * a.npair + pr) * a.nrows + ir) * a.R;
-    for (int n = tid; n < a.R; n += T) {
+    gdouble_out *Wo = as_global_out(a.W + (((int64_t)sp2 * a.npair + pr) * a.nrows + ir) * a.R);
+    gdouble *fw = as_global(a.fw);
+    for (int e = tid; e < 4 * a.R; e += T) {
+        const int n = e >> 2, sub = e & 3;
         const int j0 = a.flo[n], cnt = a.fcnt[n];
-        const double *w = a.fw + a.foff[n];
+        gdouble *w = fw + a.foff[n];
         double acc = 0.0;
-        for (int q = 0; q < cnt; ++q) {
+        for (int q = sub; q < cnt; q += 4) {
             const int j = j0 + q;
             const cd z = smem[pidx(j >> 1)];
-            const double x = (j & 1) ? -z.y : z.x;
-            acc += w[q] * (x * 3.14159265358979323846 / (16. * fN * fN * a.prn2[j] * den_i));
+            acc += w[q] * ((j & 1) ? -z.y : z.x);
         }
-        Wo[n] = acc;
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        if (sub == 0) Wo[n] = acc / den_i;
     }
 }
 

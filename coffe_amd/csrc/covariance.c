@@ -255,6 +255,7 @@ struct coffe_cov_plan {
     double *h_sep;
     /* device: spectrum-independent */
     double *d_tw, *d_A, *d_ahat, *d_B, *d_kg, *d_k3, *d_kn1, *d_kn2, *d_prn1, *d_prn2, *d_fac, *d_fw, *d_sep;
+    double *d_fws;     /* the column weights times pi / (16 N^2 r_j^nu2) (the row transform's scale) */
     int *d_rows, *d_pl1, *d_pl2, *d_flo, *d_fcnt, *d_foff, *d_upos, *d_status, *d_ls, *d_pair_of;
     /* per call, grown on demand */
     int spec_cap;
@@ -268,7 +269,7 @@ void coffe_cov_plan_destroy(coffe_cov_plan *p)
     if (!p) return;
     free(p->h_sep);
     double *dd[] = {p->d_tw, p->d_A, p->d_ahat, p->d_B, p->d_kg, p->d_k3, p->d_kn1, p->d_kn2, p->d_prn1,
-                    p->d_prn2, p->d_fac, p->d_fw, p->d_sep, p->d_d, p->d_W};
+                    p->d_prn2, p->d_fac, p->d_fw, p->d_fws, p->d_sep, p->d_d, p->d_W};
     for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); ++i) hipFree(dd[i]);
     int *di[] = {p->d_rows, p->d_pl1, p->d_pl2, p->d_flo, p->d_fcnt, p->d_foff, p->d_upos, p->d_status,
                  p->d_ls, p->d_pair_of};
@@ -376,6 +377,15 @@ int coffe_cov_plan_create(coffe_engine *e, int N, double kmin, double kmax, int 
     if (st == COFFE_OK) st = upload((void **)&p->d_prn2, prn, nb, s);
     if (st == COFFE_OK) st = upload((void **)&p->d_fac, fac, sizeof(double) * 2 * (size_t)p->npair, s);
     if (st == COFFE_OK) st = upload((void **)&p->d_fw, f.w, sizeof(double) * (size_t)p->nw, s);
+    if (st == COFFE_OK) {
+        double *fws = (double *)malloc(sizeof(double) * (size_t)p->nw);
+        if (!fws) st = COFFE_ENOMEM;
+        for (int m = 0; m < R && st == COFFE_OK; ++m)
+            for (int q = 0; q < f.cnt[m]; ++q)
+                fws[f.off[m] + q] = f.w[f.off[m] + q] * (M_PI / (16.0 * (double)N * (double)N * prn[f.lo[m] + q]));
+        if (st == COFFE_OK) st = upload((void **)&p->d_fws, fws, sizeof(double) * (size_t)p->nw, s);
+        free(fws);
+    }
     if (st == COFFE_OK) st = upload((void **)&p->d_sep, sep, sizeof(double) * (size_t)R, s);
     if (st == COFFE_OK) st = upload((void **)&p->d_rows, rows, sizeof(int) * (size_t)p->nrows, s);
     if (st == COFFE_OK) st = upload((void **)&p->d_pl1, p->pl1, sizeof(int) * (size_t)p->npair, s);
@@ -443,7 +453,7 @@ int coffe_cov_integrals(coffe_cov_plan *p, int nspec, const int64_t *n_in, const
     if (st == COFFE_OK &&
         ce_launch_cov_rows(s, 2 * nspec, p->N, &p->rpN, &p->rpM, p->d_tw, p->d_ahat, p->d_B, p->d_d, p->npair,
                            p->d_pl1, p->d_pl2, p->nrows, p->d_rows, p->d_prn1, p->d_prn2, p->R, p->d_flo, p->d_fcnt,
-                           p->d_foff, p->d_fw, p->d_W))
+                           p->d_foff, p->d_fws, p->d_W))
         st = COFFE_EDEVICE;
     if (st == COFFE_OK && ce_launch_cov_cols(s, nspec, p->npair, p->nrows, p->R, p->d_upos, p->d_fcnt, p->d_foff,
                                              p->d_fw, p->d_fac, p->d_W, d_out))
@@ -605,7 +615,7 @@ int two_sph_bessel_status(const double *k1, const double *k2, double **fk1k2, lo
             p2[t] = pow(r2[t], cfg->nu2);
             idx[t] = t;
             one[t] = 1;
-            wone[t] = 1.0;
+            wone[t] = M_PI / (16.0 * (double)N * (double)N * p2[t]);   /* the row transform's scale */
         }
     }
     double *d_tw = NULL, *d_A = NULL, *d_ahat = NULL, *d_B = NULL, *d_d = NULL, *d_p1 = NULL, *d_p2 = NULL;
