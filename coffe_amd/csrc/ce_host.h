@@ -24,13 +24,16 @@ struct coffe_engine {
     /* scratch of the drop-in path */
     void *d_scratch;
     size_t scratch_bytes;
-    /* pinned host staging for uploads of many small host arrays in one copy */
-    void *h_pinned;
-    size_t pinned_bytes;
+    /* pinned host staging for uploads of many small host arrays in one copy, two
+       buffers used in turn, so that a call can be staged while the previous one's
+       copy is still queued */
+    void *h_pinned[2];
+    size_t pinned_bytes[2];
+    int stage_next;
     /* asynchronous users of the scratch: ev_scratch follows the last kernel that reads
-       it, ev_stage the last copy out of the pinned staging (valid when *_pending) */
-    hipEvent_t ev_scratch, ev_stage;
-    int scratch_pending, stage_pending;
+       it, ev_stage[i] the last copy out of staging buffer i (valid when *_pending) */
+    hipEvent_t ev_scratch, ev_stage[2];
+    int scratch_pending, stage_pending[2];
 };
 
 /* COFFE_OK or COFFE_EDEVICE */

@@ -267,7 +267,8 @@ int coffe_engine_create(int device, coffe_engine **out)
         return COFFE_EDEVICE;
     }
     if (hipEventCreateWithFlags(&e->ev_scratch, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->ev_stage, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&e->ev_stage[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_stage[1], hipEventDisableTiming) != hipSuccess) {
         hipStreamDestroy(e->stream);
         free(e);
         return COFFE_EDEVICE;
@@ -281,13 +282,16 @@ void coffe_engine_destroy(coffe_engine *e)
     if (!e) return;
     hipStreamSynchronize(e->stream);
     if (e->scratch_pending) hipEventSynchronize(e->ev_scratch);
-    if (e->stage_pending) hipEventSynchronize(e->ev_stage);
+    for (int i = 0; i < 2; ++i)
+        if (e->stage_pending[i]) hipEventSynchronize(e->ev_stage[i]);
     ce_grid *g = e->grids;
     while (g) { ce_grid *n = g->next; grid_free(g); g = n; }
     hipFree(e->d_scratch);
-    hipHostFree(e->h_pinned);
+    for (int i = 0; i < 2; ++i) {
+        hipHostFree(e->h_pinned[i]);
+        hipEventDestroy(e->ev_stage[i]);
+    }
     hipEventDestroy(e->ev_scratch);
-    hipEventDestroy(e->ev_stage);
     hipStreamDestroy(e->stream);
     pthread_mutex_destroy(&e->lock);
     free(e);
@@ -973,30 +977,32 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
     if (!h_mu) return COFFE_ENOMEM;
     /* the inputs are staged in the engine's scratch (the drop-in calls share it under the
        lock).  The call is asynchronous on the stream: the next user of the scratch waits
-       for this kernel (ev_scratch) and the next writer of the pinned staging for this
-       copy (ev_stage) */
+       for this kernel (ev_scratch) and the next writer of this pinned staging buffer for
+       this copy (ev_stage) */
     pthread_mutex_lock(&e->lock);
     int st = scratch_reserve(e, total, s);
     if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); if (owned) free((void *)h_mu); return st; }
-    if (e->stage_pending) {
-        st = hip_ok(hipEventSynchronize(e->ev_stage));
-        e->stage_pending = 0;
+    const int sb = e->stage_next;
+    e->stage_next ^= 1;
+    if (e->stage_pending[sb]) {
+        st = hip_ok(hipEventSynchronize(e->ev_stage[sb]));
+        e->stage_pending[sb] = 0;
     }
     /* every host input goes through one pinned staging buffer and one copy */
     const size_t nstage = total / sizeof(double) - ncoef;   /* everything but the coefficient workspace */
-    if (st == COFFE_OK && e->pinned_bytes < sizeof(double) * nstage) {
-        hipHostFree(e->h_pinned);
-        e->h_pinned = NULL;
-        e->pinned_bytes = 0;
-        if (hipHostMalloc(&e->h_pinned, sizeof(double) * nstage, hipHostMallocDefault) == hipSuccess)
-            e->pinned_bytes = sizeof(double) * nstage;
+    if (st == COFFE_OK && e->pinned_bytes[sb] < sizeof(double) * nstage) {
+        hipHostFree(e->h_pinned[sb]);
+        e->h_pinned[sb] = NULL;
+        e->pinned_bytes[sb] = 0;
+        if (hipHostMalloc(&e->h_pinned[sb], sizeof(double) * nstage, hipHostMallocDefault) == hipSuccess)
+            e->pinned_bytes[sb] = sizeof(double) * nstage;
         else
             st = COFFE_ENOMEM;
     }
     double *p = (double *)e->d_scratch;
     const double *src[10] = {bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, bias_z, bias1, bias2};
     double *dst[10];
-    double *hp = (double *)e->h_pinned;
+    double *hp = (double *)e->h_pinned[sb];
     for (int i = 0; i < 10 && st == COFFE_OK; ++i) {
         const size_t nv = (i < 7 ? nbytes_bg : nbytes_b) / sizeof(double);
         dst[i] = p;
@@ -1009,8 +1015,8 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
         memcpy(hp, zmean, sizeof(double) * (size_t)nz);
         memcpy(hp + nz, sep, sizeof(double) * (size_t)nsep);
         memcpy(hp + nz + nsep, h_mu, sizeof(double) * 2 * (size_t)nmu);
-        st = hip_ok(hipMemcpyAsync(e->d_scratch, e->h_pinned, sizeof(double) * nstage, hipMemcpyHostToDevice, s));
-        if (st == COFFE_OK && hip_ok(hipEventRecord(e->ev_stage, s)) == COFFE_OK) e->stage_pending = 1;
+        st = hip_ok(hipMemcpyAsync(e->d_scratch, e->h_pinned[sb], sizeof(double) * nstage, hipMemcpyHostToDevice, s));
+        if (st == COFFE_OK && hip_ok(hipEventRecord(e->ev_stage[sb], s)) == COFFE_OK) e->stage_pending[sb] = 1;
     }
     if (st == COFFE_OK &&
         ce_launch_mu_weights(s, nbg, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], nb, dst[7], dst[8], dst[9],
