@@ -62,7 +62,8 @@ typedef struct ce_chunk_dev {
     int32_t a, b;
     int32_t grid;      /* copies of the spectrum's fields: one load gives everything */
     int64_t in_off, ws_off;
-    int32_t n_in, pad;
+    int32_t n_in;
+    int32_t id;        /* the chunk's index in the batch: its slot of the resample ranges */
 } ce_chunk_dev;
 
 /* work unit of the table kernels */
@@ -87,8 +88,8 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
    threads / groups threads and lds_data_len(M) complex of LDS each; lt_off = LDS
    offset of the twiddle tables (complex), at least groups * lds_data_len(M) of every
    pack (0: lds_data_len(max_M)) */
-/* d_ranges (4 ints per chunk, may be NULL): the resample index ranges of the chunk's
-   owned and staged intervals, written for ce_launch_spline_adj */
+/* d_ranges (4 ints per chunk id, may be NULL): the resample index ranges of the
+   chunk's owned and staged intervals, written for ce_launch_spline_adj */
 int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
                        const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws,
                        int32_t *d_ranges);

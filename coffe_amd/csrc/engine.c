@@ -484,7 +484,7 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
                 chunks[c].in_off = b->h_spec[s].in_off;
                 chunks[c].ws_off = b->h_spec[s].ws_off;
                 chunks[c].n_in = b->h_spec[s].n_in;
-                chunks[c].pad = 0;
+                chunks[c].id = c;
                 chunks[c].a = a;
                 chunks[c].b = a + CE_CHUNK < nint ? a + CE_CHUNK : nint;
                 ++c;
@@ -777,7 +777,7 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
         h_chunks[c].in_off = 0;
         h_chunks[c].ws_off = 0;
         h_chunks[c].n_in = (int32_t)n;
-        h_chunks[c].pad = 0;
+        h_chunks[c].id = c;
         h_chunks[c].a = c * CE_CHUNK;
         h_chunks[c].b = (c + 1) * CE_CHUNK < (int)n - 1 ? (c + 1) * CE_CHUNK : (int)n - 1;
     }

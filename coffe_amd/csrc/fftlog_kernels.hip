@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
         const bool edge = (q & 1) ? (node == n - 1) : (node == 0);
         ib = edge ? ((q & 1) ? g.N : 0) : grid_lower_index(g, k[node]);
     }
-    if (ranges && lane < 4) ranges[4 * (int)blockIdx.x + lane] = ib;
+    if (ranges && lane < 4) ranges[4 * ch.id + lane] = ib;
     const int i0 = __shfl(ib, 0), i1 = __shfl(ib, 1);
     const double klo = k[0], khi = k[n - 1];
     // loads at clamped indices, unconditionally, so that they all go out at once
@@ -950,8 +950,8 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
     // resample points with intervals in the staged range [lo, hi), as k_resample of
     // the same prepare stage found them
     // (clamped to the grid, so that a stage run out of order stays in bounds)
-    const int i1 = min(max(ranges[4 * (int)blockIdx.x + 3], 0), N);
-    const int i0 = min(max(ranges[4 * (int)blockIdx.x + 2], 0), i1);
+    const int i1 = min(max(ranges[4 * ch.id + 3], 0), N);
+    const int i0 = min(max(ranges[4 * ch.id + 2], 0), i1);
     const double klo = k[0], khi = k[n - 1];
     // loads at clamped indices, unconditionally, so that they all go out at once
     const int ilast = max(i1 - 1, 0);
