@@ -336,17 +336,17 @@ def main():
     gpk = torch.empty_like(p)
     stream = torch.cuda.current_stream()
     sp = __import__("ctypes").c_void_p(stream.cuda_stream)
-    kernels = ["k_resample", "k_prepare", "k_tables", "k_project", "k_adj_c2r", "k_spline_adj"]
+    kernels = ["k_resample", "k_prepare", "k_tables", "k_adj_c2r", "k_spline_adj"]
     launches = [lambda: batch.kernel_resample(k, p, sp), lambda: batch.kernel_fft_prepare(sp),
-                lambda: batch.stage_tables(A, gout, itab, sp), lambda: batch.stage_project(A, itab, out, sp),
-                lambda: batch.kernel_adj_c2r(sp), lambda: batch.kernel_spline_adj(k, gpk, sp)]
+                lambda: batch.stage_tables(A, gout, itab, sp), lambda: batch.kernel_adj_c2r(sp),
+                lambda: batch.kernel_spline_adj(k, gpk, sp)]
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     kernel_ms = {kk: 0.0 for kk in kernels}
 
     def step():
-        # one forward + adjoint pass over the batch: a single C call launches the six
-        # kernels back to back on the stream (resample, prepare, tables, project,
-        # adj_c2r, spline_adj)
+        # one forward + adjoint pass over the batch: a single C call launches the five
+        # kernels back to back on the stream (resample, prepare, tables, adj_c2r with
+        # the projection to multipoles, spline_adj)
         batch.vjp(k, p, A, gout, itab=itab, out=out, gpk=gpk, stream=sp)
 
     for _ in range(args.warmup):

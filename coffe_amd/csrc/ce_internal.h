@@ -106,9 +106,11 @@ int ce_launch_tables(void *stream, int nunit, const ce_unit_dev *d_units, const 
 int ce_launch_project(void *stream, int nspec, int ntab, int nz, int nl, int nsep, const double *d_A,
                       const double *d_itab, double *d_out);
 
+/* with d_out (and d_A, d_itab) each spectrum's workgroup also writes its multipoles
+   out[s][z][l][r] = sum_t A itab (the projection of ce_launch_project) */
 int ce_launch_adj_c2r(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
                       const ce_table_desc *d_tabs, int ntab, int nq, double *d_ws, const double *d_xbar, int threads,
-                      int max_M);
+                      int max_M, const double *d_A, const double *d_itab, double *d_out, int nz, int nl, int nsep);
 
 int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
                          const ce_grid_dev *d_grids, const double *d_k, const double *d_ws, double *d_gpk,

@@ -595,7 +595,7 @@ int coffe_batch_adj_c2r(coffe_batch *b, void *stream)
 {
     if (!b) return COFFE_EINVAL;
     return ce_launch_adj_c2r(pick_stream(b, stream), b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq,
-                             b->d_ws, b->d_xb, b->threads, b->max_M)
+                             b->d_ws, b->d_xb, b->threads, b->max_M, NULL, NULL, NULL, 0, 0, 0)
                ? COFFE_EDEVICE
                : COFFE_OK;
 }
@@ -645,7 +645,7 @@ int coffe_batch_prepare_adj(coffe_batch *b, const double *d_k, const double *d_p
     if (!b || !d_k || !d_pk || !d_gpk) return COFFE_EINVAL;
     hipStream_t s = pick_stream(b, stream);
     if (ce_launch_adj_c2r(s, b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq, b->d_ws, b->d_xb,
-                          b->threads, b->max_M))
+                          b->threads, b->max_M, NULL, NULL, NULL, 0, 0, 0))
         return COFFE_EDEVICE;
     return ce_launch_spline_adj(s, b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, b->d_ws, d_gpk, b->d_ranges)
                ? COFFE_EDEVICE
@@ -680,8 +680,13 @@ int coffe_batch_vjp(coffe_batch *b, const double *d_k, const double *d_pk, const
                          b->d_sidx, b->d_sep, itab, NULL, 0, d_A, d_gout, b->nz, b->nl, b->d_xb, b->d_wbuf,
                          b->threads, b->max_M, b->d_packs, b->npack, b->lt_off))
         return COFFE_EDEVICE;
-    if (d_out && ce_launch_project(s, b->nspec, b->ntab, b->nz, b->nl, b->nsep, d_A, itab, d_out)) return COFFE_EDEVICE;
-    return coffe_batch_prepare_adj(b, d_k, d_pk, d_gpk, s);
+    /* the transposed r2c of every spectrum, with its projection (out) in the same workgroup */
+    if (ce_launch_adj_c2r(s, b->nspec, b->d_spec, b->d_grids, b->d_tabs, b->ntab, b->nq, b->d_ws, b->d_xb,
+                          b->threads, b->max_M, d_A, itab, d_out, b->nz, b->nl, b->nsep))
+        return COFFE_EDEVICE;
+    return ce_launch_spline_adj(s, b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, b->d_ws, d_gpk, b->d_ranges)
+               ? COFFE_EDEVICE
+               : COFFE_OK;
 }
 
 int coffe_batch_tables(coffe_batch *b, const double *d_k, const double *d_pk, double *d_full, void *stream)

@@ -836,13 +836,35 @@ __global__ void k_project(int nspec, int ntab, int nz, int nl, int nsep, const d
 
 // k_adj_c2r: per spectrum, Ybar = conj(sum_t xbar_t) -> transposed r2c (a c2r with
 // the interior halved) -> times bias*window -> sbar (cotangent of S(x_i))
+// the multipoles of one spectrum, out[z][l][r] = sum_t A[z][l][t][r] itab[t][r] (the
+// projection of k_project), folded into k_adj_c2r in the composite adjoint call
+struct proj_args {
+    const double *A;       // [nz][nl][ntab][nsep]
+    const double *itab;    // [spec][ntab][nsep]
+    double *out;           // [spec][nz][nl][nsep], or null: no projection
+    int nz, nl, nsep;
+};
+
 __global__ void __launch_bounds__(1024) k_adj_c2r(const ce_spec_dev *__restrict__ specs,
                                                   const ce_grid_dev *__restrict__ grids,
                                                   const ce_table_desc *__restrict__ tabs, int ntab, int nq,
                                                   double *__restrict__ ws_all, const cd *__restrict__ xbar_all,
-                                                  int max_M)
+                                                  int max_M, proj_args pj)
 {
     extern __shared__ cd smem[];
+    if (pj.out) {
+        const int64_t s = blockIdx.x;
+        const int nout = pj.nz * pj.nl * pj.nsep;
+        gdouble *A = as_global(pj.A);
+        gdouble *it = as_global(pj.itab + s * ntab * pj.nsep);
+        gdouble_out *o = as_global_out(pj.out + s * nout);
+        for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+            const int r = e % pj.nsep, zl = e / pj.nsep;
+            double acc = 0.0;
+            for (int t = 0; t < ntab; ++t) acc += A[((int64_t)zl * ntab + t) * pj.nsep + r] * it[t * pj.nsep + r];
+            o[e] = acc;
+        }
+    }
     const ce_spec_dev sp = specs[blockIdx.x];
     const ce_grid_dev &g = grids[sp.grid];
     const int M = g.M, N = g.N;
@@ -1153,13 +1175,20 @@ int ce_launch_project(void *stream, int nspec, int ntab, int nz, int nl, int nse
 
 int ce_launch_adj_c2r(void *stream, int nspec, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
                       const ce_table_desc *d_tabs, int ntab, int nq, double *d_ws, const double *d_xbar, int threads,
-                      int max_M)
+                      int max_M, const double *d_A, const double *d_itab, double *d_out, int nz, int nl, int nsep)
 {
     const size_t lds = fft_lds_bytes(max_M);
     int st = set_lds((const void *)k_adj_c2r, lds);
     if (st) return st;
+    proj_args pj;
+    pj.A = d_A;
+    pj.itab = d_itab;
+    pj.out = (d_A && d_itab) ? d_out : nullptr;
+    pj.nz = nz;
+    pj.nl = nl;
+    pj.nsep = nsep;
     hipLaunchKernelGGL(k_adj_c2r, dim3(nspec), dim3(threads), lds, (hipStream_t)stream, d_spec, d_grids, d_tabs, ntab,
-                       nq, d_ws, reinterpret_cast<const cd *>(d_xbar), max_M);
+                       nq, d_ws, reinterpret_cast<const cd *>(d_xbar), max_M, pj);
     return (int)hipGetLastError();
 }
 
