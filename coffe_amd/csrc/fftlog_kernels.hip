@@ -200,9 +200,10 @@ static_assert(CE_CHUNK_NODES >= CE_CHUNK + 2 * CE_SPLINE_HALO + 3, "spline stagi
 // so the separator rows form a 64-row tridiagonal system, solved exactly by six
 // steps of cyclic reduction across the lanes (lane shuffles; each step forms
 // bL bR row_i - a bR row_L - c bL row_R and rescales it by a power of two, which
-// needs no division and is exact).  Rows past ns are identity rows.  rhs(r, hm, hp,
-// first) gives d_r, called in increasing r per lane (first on the lane's first
-// row).  The solution goes to X[r], r < ns; every lane of the wavefront must call.
+// needs no division and is exact).  Rows past ns are identity rows.  rhs(j, r, hm,
+// hp) gives d_r for the lane's row j (a constant after unrolling), r = lane M + j < ns,
+// called in increasing j.  The solution goes to X[r], r < ns; every lane of the
+// wavefront must call.
 template <class Rhs>
 __device__ __forceinline__ void wave_spline_solve(const double *xs, int ns, Rhs rhs, double *X)
 {
@@ -220,7 +221,7 @@ __device__ __forceinline__ void wave_spline_solve(const double *xs, int ns, Rhs 
             a = (r > 0) ? hm : 0.0;
             c = (r < ns - 1) ? hp : 0.0;
             b = 2.0 * (hm + hp);
-            d = rhs(r, hm, hp, j == 0);
+            d = rhs(j, r, hm, hp);
         }
         // row 0 of the lane: its left term multiplies s_{p-1} (right-hand side v)
         const double rd = rcp_nr(b - a * cprev);
@@ -249,7 +250,7 @@ __device__ __forceinline__ void wave_spline_solve(const double *xs, int ns, Rhs 
             as = hm;
             cs = (r < ns - 1) ? hp : 0.0;
             bs = 2.0 * (hm + hp);
-            ds = rhs(r, hm, hp, false);
+            ds = rhs(M - 1, r, hm, hp);
         }
     }
     double yn = __shfl_down(y[0], 1), vn = __shfl_down(v[0], 1), wn = __shfl_down(w[0], 1);
@@ -341,15 +342,16 @@ __device__ __forceinline__ chunk_geom make_geom(const ce_chunk_dev &ch, int n)
     return g;
 }
 
-// right-hand side of the interpolating spline, 3 (slope_{r+1} - slope_r), the slope
-// of the row's left interval carried over from the previous row of the lane
+// right-hand side of the interpolating spline, 3 (slope_{r+1} - slope_r), from the
+// lane's own values yl[q] = y at staged index lane M + q (q <= M + 1, in registers);
+// the slope of the row's left interval is carried over from the previous row
 struct spline_rhs {
-    const double *ys;
+    const double *yl;
     double sprev;
-    __device__ __forceinline__ double operator()(int r, double hm, double hp, bool first)
+    __device__ __forceinline__ double operator()(int j, int, double hm, double hp)
     {
-        const double sm = first ? (hm != 0.0 ? (ys[r + 1] - ys[r]) * rcp_nr(hm) : 0.0) : sprev;
-        const double sp = hp != 0.0 ? (ys[r + 2] - ys[r + 1]) * rcp_nr(hp) : 0.0;
+        const double sm = (j == 0) ? (hm != 0.0 ? (yl[1] - yl[0]) * rcp_nr(hm) : 0.0) : sprev;
+        const double sp = hp != 0.0 ? (yl[j + 2] - yl[j + 1]) * rcp_nr(hp) : 0.0;
         sprev = sp;
         return 3.0 * (sp - sm);
     }
@@ -357,35 +359,42 @@ struct spline_rhs {
 
 // k_resample: S(x_i) for the resample points whose interval lies in [a, b); one
 // wavefront per chunk
-__global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict__ chunks,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_resample(const ce_chunk_dev *__restrict__ chunks,
                                                  const ce_spec_dev *__restrict__ specs,
                                                  const ce_grid_dev *__restrict__ grids,
                                                  const double *__restrict__ k_all,
                                                  const double *__restrict__ pk_all, double *__restrict__ ws_all,
                                                  int32_t *__restrict__ ranges)
 {
-    __shared__ double lds[3 * CE_CHUNK_NODES];
+    // two LDS arrays (nodes and spline coefficients; the values stay in global memory
+    // and registers), 10.4 KiB: 15 chunks fit a CU
+    __shared__ double lds[2 * CE_CHUNK_NODES];
     const ce_chunk_dev ch = chunks[blockIdx.x];
     const ce_grid_dev &g = grids[ch.grid];
     gdouble *gp = as_global(g.p);
     const double gk0 = g.k0;
     const int n = ch.n_in;
     const double *k = k_all + ch.in_off;
-    const double *P = pk_all + ch.in_off;
     double *yres = ws_all + ch.ws_off;
     const chunk_geom G = make_geom(ch, n);
-    double *xs = lds, *ys = xs + CE_CHUNK_NODES, *D = ys + CE_CHUNK_NODES;
+    gdouble *ys = as_global(pk_all + ch.in_off + G.lo);   // y at staged index i
+    double *xs = lds, *D = xs + CE_CHUNK_NODES;
     const int lane = threadIdx.x;
     const int cnt = G.hi - G.lo + 1;
     // the staged nodes go to registers first, so that their loads overlap the search
-    // for the resample range and the loads of the first resample points
+    // for the resample range and the loads of the first resample points; each lane
+    // also loads the values of its own rows of the spline system
     constexpr int SPL = (CE_CHUNK_NODES + 63) / 64;
-    double kv[SPL], pv[SPL];
+    double kv[SPL];
 #pragma unroll
     for (int u = 0; u < SPL; ++u) {
         const int i = lane + 64 * u;
-        if (i < cnt) { kv[u] = k[G.lo + i]; pv[u] = P[G.lo + i]; }
+        if (i < cnt) kv[u] = k[G.lo + i];
     }
+    double yl[CE_SOLVE_M + 2];
+#pragma unroll
+    for (int q = 0; q < CE_SOLVE_M + 2; ++q) yl[q] = ys[min(lane * CE_SOLVE_M + q, cnt - 1)];
     // resample index ranges: lanes 0, 1 bound the points of the owned intervals
     // [a, b), lanes 2, 3 those of the staged intervals [lo, hi) (kept for the adjoint)
     int ib;
@@ -406,11 +415,11 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
 #pragma unroll
     for (int u = 0; u < SPL; ++u) {
         const int i = lane + 64 * u;
-        if (i < cnt) { xs[i] = kv[u]; ys[i] = pv[u]; }
+        if (i < cnt) xs[i] = kv[u];
     }
     wave_sync();
     const int ns = G.u1 - G.u0 + 1;
-    wave_spline_solve(xs, ns, spline_rhs{ys, 0.0}, D);
+    wave_spline_solve(xs, ns, spline_rhs{yl, 0.0}, D);
     wave_sync();
     // c at staged node li: D[li - 1] for u0 <= node <= u1, 0 at the global ends
     auto cval = [&](int li) {
@@ -427,6 +436,12 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
 #pragma unroll
         for (int t = 0; t < CE_PTS_PER; ++t) xq[t] = gk0 * pq[t];
         lds_intervals(xs, cnt, xq, jv);
+        double y0[CE_PTS_PER], y1[CE_PTS_PER];
+#pragma unroll
+        for (int t = 0; t < CE_PTS_PER; ++t) {
+            y0[t] = ys[jv[t]];
+            y1[t] = ys[jv[t] + 1];
+        }
 #pragma unroll
         for (int t = 0; t < CE_PTS_PER; ++t) {
             const int i = base + 64 * t;
@@ -437,12 +452,12 @@ __global__ void __launch_bounds__(64) k_resample(const ce_chunk_dev *__restrict_
                 S = __builtin_nan("");
             } else {
                 const double dx = xs[j + 1] - xs[j], rdx = rcp_nr(dx);
-                const double dy = ys[j + 1] - ys[j];
+                const double dy = y1[t] - y0[t];
                 const double delx = xq[t] - xs[j];
                 const double cj = cval(j), cj1 = cval(j + 1);
                 const double bb = dy * rdx - dx * (cj1 + 2.0 * cj) / 3.0;
                 const double dd = (cj1 - cj) * rdx / 3.0;
-                S = ys[j] + delx * (bb + delx * (cj + delx * dd));
+                S = y0[t] + delx * (bb + delx * (cj + delx * dd));
             }
             yres[i] = S;
         }
@@ -1025,7 +1040,7 @@ __global__ void __launch_bounds__(64) k_spline_adj(const ce_chunk_dev *__restric
     // index r + 1.  The solution overwrites cb in place (D[r] = cb[r]): the one
     // wavefront has read every right-hand side before its first store
     double *D = cb;
-    wave_spline_solve(xs, ns, [&](int r, double, double, bool) { return cb[r + 1]; }, D);
+    wave_spline_solve(xs, ns, [&](int, int r, double, double) { return cb[r + 1]; }, D);
     wave_sync();
     auto lam = [&](int node) { return (node < G.u0 || node > G.u1) ? 0.0 : D[node - G.u0]; };
     const int jend = (G.b == n - 1) ? n : G.b;
