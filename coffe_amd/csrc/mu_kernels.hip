@@ -255,7 +255,10 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
 }
 
 // one wavefront per (z, r); blockDim = 256 (four points)
-__global__ void __launch_bounds__(256) k_mu_weights(mu_args a)
+// held to 128 VGPRs (4 waves per SIMD, every (z, r) wavefront of the 8 x 512 grid
+// resident at once): the few values that spill cost less than the second round of
+// wavefronts that 224 registers (2 waves per SIMD) needed (measured -8 %)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_mu_weights(mu_args a)
 {
     const int lane = threadIdx.x & 63;
     const int point = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
