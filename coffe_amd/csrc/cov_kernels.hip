@@ -192,42 +192,20 @@ struct cols_args {
     double *out;                     // [nspec][2][npair][R][R], value (m, n) at R n + m
 };
 
-// one workgroup per tile of CE_COLS_TM consecutive row functionals m (and one block):
-// thread n sums, for each m of the tile, the W rows m's functional touches at column
-// n.  The functionals of consecutive m share most of their rows, so the tile reads
-// its W window once from memory (L2 serves the repeats) instead of once per m, and
-// each thread writes CE_COLS_TM consecutive outputs.
-#define CE_COLS_TM 16
 __global__ void __launch_bounds__(256) k_cov_cols(cols_args a)
 {
-    const int m0 = blockIdx.x * CE_COLS_TM;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int RR = a.R * a.R;
+    if (e >= RR) return;
+    const int m = e / a.R, n = e % a.R;
     const int pr = blockIdx.y, sp2 = blockIdx.z;
     const int64_t blk = (int64_t)sp2 * a.npair + pr;
-    const int64_t RR = (int64_t)a.R * a.R;
-    gdouble *W = as_global(a.W + blk * a.nrows * a.R);
-    gdouble *uw = as_global(a.uw);
-    const double fac = a.fac[(sp2 & 1) * a.npair + pr];
-    gdouble_out *out = as_global_out(a.out + blk * RR);
-    const int mend = min(CE_COLS_TM, a.R - m0);
-    for (int n = threadIdx.x; n < a.R; n += blockDim.x) {
-        double acc[CE_COLS_TM];
-#pragma unroll
-        for (int mm = 0; mm < CE_COLS_TM; ++mm) acc[mm] = 0.0;
-#pragma unroll
-        for (int mm = 0; mm < CE_COLS_TM; ++mm) {
-            if (mm < mend) {
-                const int m = m0 + mm;
-                const int p0 = a.upos[m], cnt = a.ucnt[m];
-                gdouble *w = uw + a.uoff[m];
-                double s0 = 0.0;
-                for (int q = 0; q < cnt; ++q) s0 += w[q] * W[(int64_t)(p0 + q) * a.R + n];
-                acc[mm] = s0;
-            }
-        }
-#pragma unroll
-        for (int mm = 0; mm < CE_COLS_TM; ++mm)
-            if (mm < mend) out[(int64_t)a.R * n + m0 + mm] = acc[mm] * fac;
-    }
+    const double *W = a.W + blk * a.nrows * a.R;
+    const int p0 = a.upos[m], cnt = a.ucnt[m];
+    const double *w = a.uw + a.uoff[m];
+    double acc = 0.0;
+    for (int q = 0; q < cnt; ++q) acc += w[q] * W[(int64_t)(p0 + q) * a.R + n];
+    a.out[blk * RR + (int64_t)a.R * n + m] = acc * a.fac[(sp2 & 1) * a.npair + pr];
 }
 
 __global__ void k_fill_int(int *p, int n, int v)
@@ -358,8 +336,8 @@ extern "C" int ce_launch_cov_cols(void *stream, int nspec, int npair, int nrows,
     cols_args a;
     a.npair = npair; a.nrows = nrows; a.R = R; a.upos = d_upos; a.ucnt = d_ucnt; a.uoff = d_uoff; a.uw = d_uw;
     a.fac = d_fac; a.W = d_W; a.out = d_out;
-    hipLaunchKernelGGL(k_cov_cols, dim3((R + CE_COLS_TM - 1) / CE_COLS_TM, npair, 2 * nspec), dim3(256), 0,
-                       (hipStream_t)stream, a);
+    const int RR = R * R;
+    hipLaunchKernelGGL(k_cov_cols, dim3((RR + 255) / 256, npair, 2 * nspec), dim3(256), 0, (hipStream_t)stream, a);
     return (int)hipGetLastError();
 }
 
