@@ -680,6 +680,31 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
     const int r_a = tid;
     const bool has_a = inreg && r_a < a.nsep;
     const int j_a = has_a ? sidx[r_a] : 2;
+    // the cotangent of I_t(r) in adjoint mode, sum over (z, l) of A gout (in that order),
+    // from (z, l) pair q0 on, four pairs' loads in flight at a time
+    gdouble *gA = as_global(a.A), *gG = as_global(a.gout);
+    const int nzl = a.nz * a.nl;
+    auto zl_index = [&](int q, int r, int64_t &ia, int64_t &ig) {
+        const int z = q / a.nl, l = q - (q / a.nl) * a.nl;
+        ia = (((int64_t)z * a.nl + l) * a.ntab + u.tab) * a.nsep + r;
+        ig = (((int64_t)u.spec * a.nz + z) * a.nl + l) * a.nsep + r;
+    };
+    auto cotangent_sum = [&](int r, int q0, double ib) {
+        for (; q0 < nzl; q0 += 4) {
+            double av[4], gv[4];
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                int64_t ia, ig;
+                zl_index(min(q0 + h, nzl - 1), r, ia, ig);
+                av[h] = gA[ia];
+                gv[h] = gG[ig];
+            }
+#pragma unroll
+            for (int h = 0; h < 4; ++h)
+                if (q0 + h < nzl) ib += av[h] * gv[h];
+        }
+        return ib;
+    };
     // kernel multiply + c2r merge (stored conjugated: the forward FFT then gives conj(z)),
     // over the pairs (k, M - k), three rounds of loads in flight at a time; the merge
     // twiddle of M - k is -conj of that of k (N = 2M), exactly so for the correctly
@@ -737,11 +762,7 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
             O[m] = os[j - 2 + m];
         }
         double ib = 0.0;
-        if (adj)
-            for (int z = 0; z < a.nz; ++z)
-                for (int l = 0; l < a.nl; ++l)
-                    ib += a.A[(((int64_t)z * a.nl + l) * a.ntab + u.tab) * a.nsep + r] *
-                          a.gout[(((int64_t)u.spec * a.nz + z) * a.nl + l) * a.nsep + r];
+        if (adj) ib = cotangent_sum(r, 0, ib);
         akima_finish(r, j, X, O, ib, sep[r], w);
         return j;
     };
@@ -758,12 +779,7 @@ __global__ void __launch_bounds__(1024) k_tables(table_args a)
                 X_a[m] = r0 * gp[j_a - 2 + m];
                 O_a[m] = os[j_a - 2 + m];
             }
-            double ib_a = 0.0;
-            if (adj)
-                for (int z = 0; z < a.nz; ++z)
-                    for (int l = 0; l < a.nl; ++l)
-                        ib_a += a.A[(((int64_t)z * a.nl + l) * a.ntab + u.tab) * a.nsep + r_a] *
-                                a.gout[(((int64_t)u.spec * a.nz + z) * a.nl + l) * a.nsep + r_a];
+            const double ib_a = adj ? cotangent_sum(r_a, 0, 0.0) : 0.0;
             akima_finish(r_a, j_a, X_a, O_a, ib_a, sep[r_a], w0);
             j0 = j_a;
         }

@@ -73,3 +73,31 @@ def test_vjp_homogeneity(engine):
     lhs = float(torch.dot(gpk, p))
     rhs = float(torch.sum(gout * out))
     assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
+
+
+def test_packed_workgroups_bit_identical(engine, monkeypatch):
+    """Table units of one length share a workgroup (one group of threads and one LDS
+    region each); with COFFE_ENGINE_PACK=0 every unit has its own workgroup. The
+    arithmetic per unit is the same, so both give bit-identical tables, multipoles and
+    gradients (lengths repeated so that packs of 2 and 4 units form)."""
+    import torch
+    from coffe_amd.engine import Batch
+    specs = [(s[0], s[1]) for s in batch_spectra(5, seed=3, lengths=(1024, 2048))]
+    tables = [(0, 0), (0, 2), (0, 4)]
+    sep = np.linspace(5, 300, 48) / 0.67
+    rng = np.random.default_rng(5)
+    A = torch.tensor(rng.standard_normal((2, 3, 3, len(sep))), device="cuda")
+    gout = torch.tensor(rng.standard_normal((len(specs), 2, 3, len(sep))), device="cuda")
+    k = torch.tensor(np.concatenate([s[0] for s in specs]), device="cuda")
+    p = torch.tensor(np.concatenate([s[1] for s in specs]), device="cuda")
+    args = ([len(s[0]) for s in specs], [len(s[0]) for s in specs], [s[0][0] for s in specs],
+            [s[0][-1] for s in specs], tables, sep, 2, 3)
+    res = []
+    for pack in ("0", "1"):
+        monkeypatch.setenv("COFFE_ENGINE_PACK", pack)
+        b = Batch(engine, *args)
+        print("pack", pack, flush=True)
+        res.append([x.cpu().numpy() for x in b.vjp(k, p, A, gout)])
+    monkeypatch.delenv("COFFE_ENGINE_PACK")
+    for x0, x1 in zip(*res):
+        assert np.array_equal(x0, x1)
