@@ -309,7 +309,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: the engine is single-device (spectra shard across GPUs with no exchange; "
-              "see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+              "each GPU would take a slice of the batch). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "fullsky":
         return run_fullsky(args)

@@ -4,7 +4,7 @@
  * Two groups of entry points:
  *
  *  1. Drop-in replacements with the reference's own signatures, so that COFFE's
- *     callers link unchanged (see INTEGRATION.md for the binding stubs):
+ *     callers link unchanged:
  *       twofast_1bessel        replaces twofast.c:362-483 (decl. twofast.h:44-58)
  *       two_sph_bessel         replaces twobessel.c:16-113 (decl. twobessel.h:7)
  *     Both are void in the reference and exit() on bad input; here they record an
