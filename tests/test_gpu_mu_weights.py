@@ -128,3 +128,36 @@ def test_domain_errors(engine, setup):
         mu_weights(engine, bgo, zb, b, b, [0.01], [0], TABLES, [500.0], flatsky=False)
     with pytest.raises(RuntimeError):   # tables required by RSD missing
         mu_weights(engine, bgo, zb, b, b, [1.0], [0], [(0, 0)], [100.0], den=1, rsd=1, flatsky=False)
+
+
+def test_async_calls_back_to_back(engine, setup):
+    """coffe_mu_weights is asynchronous on its stream: several calls queued back to back,
+    each with host inputs that are overwritten right after the call returns (the pinned
+    staging takes them at once), must equal the same calls made one at a time."""
+    import torch
+    from coffe_amd.engine import mu_weights
+    _, _, bgo, _ = setup
+    sep = np.linspace(20, 200, 16) / H
+    zb = np.linspace(0, 10, 11)
+    cases = [(1.0 + 0.1 * i, 1.2 + 0.05 * i, [0.6 + 0.2 * i, 1.4]) for i in range(5)]
+    ref = []
+    for b0, b1, zm in cases:
+        A = mu_weights(engine, bgo, zb, np.full(11, b0), np.full(11, b1), zm, [0, 2], TABLES, sep, den=1, rsd=1,
+                       d1=1, flatsky=False, nmu=64)
+        torch.cuda.synchronize()
+        ref.append(A.cpu().numpy())
+    outs = []
+    bias1, bias2, zmean = np.empty(11), np.empty(11), np.empty(2)
+    print("mu weights queued back to back", flush=True)
+    for b0, b1, zm in cases:
+        bias1[:] = b0
+        bias2[:] = b1
+        zmean[:] = zm
+        outs.append(mu_weights(engine, bgo, zb, bias1, bias2, zmean, [0, 2], TABLES, sep, den=1, rsd=1, d1=1,
+                               flatsky=False, nmu=64))
+        bias1[:] = np.nan   # the call has taken its inputs
+        bias2[:] = np.nan
+        zmean[:] = np.nan
+    torch.cuda.synchronize()
+    for got, want in zip(outs, ref):
+        assert np.array_equal(got.cpu().numpy(), want)
