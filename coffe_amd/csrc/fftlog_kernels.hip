@@ -1,12 +1,11 @@
 // HIP kernels of the FFTLog engine (gfx950, FP64).
 //
-// One workgroup owns one FFT at a time.  The length-M complex transform (M = N/2,
-// N the real FFTLog length) lives in LDS as interleaved doubles and is computed by
-// self-sorting Stockham passes: every thread first pulls the R inputs of each of its
-// butterflies into registers, the workgroup synchronises, then every thread applies
-// twiddles, the radix-R DFT and stores the R outputs.  A single LDS buffer of
-// 16*M bytes is therefore enough (M <= 8192 -> 128 KiB of the 160 KiB per CU).
-// Real transforms use the half-length complex transform plus the split step.
+// A workgroup (or, in k_tables, each of its thread groups) owns one FFT at a time.
+// The length-M complex transform (M = N/2, N the real FFTLog length) lives in LDS as
+// interleaved doubles and is computed by self-sorting Stockham passes (ce_fft.h): a
+// single LDS buffer of 16*M bytes (M <= 8192 -> 128 KiB of the 160 KiB per CU).  Real
+// transforms use the half-length complex transform plus the split step.  The spline
+// kernels run one wavefront per chunk of input intervals.
 //
 // Kernels (reference steps they implement):
 //   k_grid_tables  spectrum-independent tables of one (N, kmin, kmax) grid:
@@ -17,12 +16,14 @@
 //   k_resample     per chunk of input intervals: natural cubic spline of P(k)
 //                  (twofast.c:399-401) and its values on the log grid (275-290).
 //   k_prepare      per spectrum: bias * window * S, r2c, conj (twofast.c:252-360).
-//   k_tables       per (spectrum, table): kernel multiply, c2r (twofast.c:426-477),
+//   k_tables       per (spectrum, table) unit, units of one length two to a workgroup:
+//                  kernel multiply, c2r (twofast.c:426-477),
 //                  then either the full table or Akima at the separations
 //                  (common.c:704-710 on the table of integrals.c:564-567); the
 //                  adjoint mode adds the Akima VJP and the transposed c2r.
 //   k_project      multipoles = sum_t A[z,l,t,r] I_t(r) (the mu-integrated weights).
-//   k_adj_c2r      per spectrum: transposed r2c and bias -> cotangent of S(x_i).
+//   k_adj_c2r      per spectrum: transposed r2c and bias -> cotangent of S(x_i);
+//                  in the composite adjoint call also the projection of k_project.
 //   k_spline_adj   per chunk: transposed resample and spline -> dP.
 #include <hip/hip_runtime.h>
 
