@@ -53,3 +53,34 @@ def test_batch_tables_full(engine):
             assert _rel(full[s, t, :N], ty[43:]) < 1e-10, (s, n, l)
             ref = O.akima(tx, ty, sep)
             assert _rel(itab[s, t], ref, 0) < 1e-10, (s, n, l)
+
+
+@pytest.mark.parametrize("n_in", [5, 9, 100, 513, 514, 1100])
+def test_small_and_chunked_inputs(engine, n_in):
+    """The spline of the input runs in chunks of 512 intervals with 64-node halos, one
+    wavefront each (11 rows of the system per lane): inputs of a few nodes (most lanes
+    idle), one chunk exactly, and several chunks must all reproduce the oracle, in the
+    drop-in call and in a batch (with its adjoint's ranges)."""
+    import torch
+    from coffe_amd.engine import Batch, twofast_1bessel
+    k, p = spectrum(n_in)
+    print(f"twofast n_in={n_in}", flush=True)
+    rx, ry = twofast_1bessel(k, p, 1024, 0, 0)
+    ox, oy = O.twofast_1bessel(k, p, 1024, 0, 0)
+    assert np.array_equal(rx, ox)
+    assert _rel(ry, oy) < 1e-10
+    sep = np.linspace(5, 300, 32) / 0.67
+    b = Batch(engine, [n_in], [1024], [k[0]], [k[-1]], [(0, 0), (0, 2)], sep, 1, 2)
+    kt, pt = torch.tensor(k, device="cuda"), torch.tensor(p, device="cuda")
+    full = b.tables_full(kt, pt).cpu().numpy()
+    for t, (n, l) in enumerate([(0, 0), (0, 2)]):
+        tx, ty = O.integral_table(n, l, k, p, 1024, small_r=False)
+        assert _rel(full[0, t, :1024], ty[43:]) < 1e-10, (n, l)
+    # the adjoint: the map P -> multipoles is positively homogeneous of degree 1, so
+    # <gpk, P> = <gout, out>
+    A = torch.tensor(np.random.default_rng(4).standard_normal((1, 2, 2, len(sep))), device="cuda")
+    gout = torch.tensor(np.random.default_rng(6).standard_normal((1, 1, 2, len(sep))), device="cuda")
+    _, out, gpk = b.vjp(kt, pt, A, gout)
+    lhs = float(torch.dot(gpk, pt))
+    rhs = float(torch.sum(gout * out))
+    assert abs(lhs - rhs) <= 1e-10 * (abs(rhs) + float(torch.sum(torch.abs(gout * out))))
