@@ -258,6 +258,9 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
 // held to 128 VGPRs (4 waves per SIMD, every (z, r) wavefront of the 8 x 512 grid
 // resident at once): the few values that spill cost less than the second round of
 // wavefronts that 224 registers (2 waves per SIMD) needed (measured -8 %)
+// NL = the number of multipoles the partial sums are kept for (>= a.nl), a template
+// argument so that the sums of absent multipoles take no registers
+template <int NL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_mu_weights(mu_args a)
 {
     const int lane = threadIdx.x & 63;
@@ -277,9 +280,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const int jbar = ce::interval_bisect(a.bg_z, 0, a.nbg - 1, zm);
     const double chib = a.bg_chi[jbar], dchi = a.bg_chi[jbar + 1] - a.bg_chi[jbar];
     // partial sums per (multipole, standard table slot); fixed bounds keep them in registers
-    double acc[CE_MU_MAXL][NSTD];
+    double acc[NL][NSTD];
 #pragma unroll
-    for (int l = 0; l < CE_MU_MAXL; ++l)
+    for (int l = 0; l < NL; ++l)
 #pragma unroll
         for (int s = 0; s < NSTD; ++s) acc[l][s] = 0.0;
     for (int q = lane; q < a.nmu; q += 64) {
@@ -287,7 +290,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         double c[NSTD];
         local_coefficients(a, zm, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, jbar, chib, dchi, c);
 #pragma unroll
-        for (int il = 0; il < CE_MU_MAXL; ++il) {
+        for (int il = 0; il < NL; ++il) {
             if (il < a.nl) {
                 const double pl = w * legendre_p(a.ls[il], mu);
 #pragma unroll
@@ -296,7 +299,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         }
     }
 #pragma unroll
-    for (int il = 0; il < CE_MU_MAXL; ++il) {
+    for (int il = 0; il < NL; ++il) {
         if (il >= a.nl) break;
 #pragma unroll
         for (int s = 0; s < NSTD; ++s) {
@@ -350,6 +353,9 @@ extern "C" int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, c
     a.nsep = nsep; a.sep = sep; a.nmu = nmu; a.mu = mu; a.wmu = wmu; a.A = A;
     const int npoint = nz * nsep;
     const int blocks = (npoint + 3) / 4;
-    hipLaunchKernelGGL(k_mu_weights, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
+    if (nl <= 1) hipLaunchKernelGGL(k_mu_weights<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
+    else if (nl == 2) hipLaunchKernelGGL(k_mu_weights<2>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
+    else if (nl == 3) hipLaunchKernelGGL(k_mu_weights<3>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
+    else hipLaunchKernelGGL(k_mu_weights<CE_MU_MAXL>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
     return (int)hipGetLastError();
 }
