@@ -146,6 +146,32 @@ template <> __device__ __forceinline__ void dft<8>(cd *v)
     v[7] = csub(e[3], o3);
 }
 
+// radix 10 as 2 x 5 in place: x[5 n1 + n2] -> sums (n1 = 0 + 1) in v[0..4] and twiddled
+// differences in v[5..9], DFT5 of each half, X[2 k2] = v[k2] and X[2 k2 + 1] = v[5 + k2]
+template <> __device__ __forceinline__ void dft<10>(cd *v)
+{
+    const double c1 = 0.80901699437494742410229341718282;   // cos(pi/5)
+    const double s1 = 0.58778525229247312916870595463907;   // sin(pi/5)
+    const double c2 = 0.30901699437494742410229341718282;   // cos(2 pi/5)
+    const double s2 = 0.95105651629515357211643933337938;   // sin(2 pi/5)
+#pragma unroll
+    for (int n2 = 0; n2 < 5; ++n2) {
+        const cd a = v[n2], b = v[5 + n2];
+        v[n2] = cadd(a, b);
+        v[5 + n2] = csub(a, b);
+    }
+    // W10^m = e^{-i pi m/5}
+    v[6] = cmul(v[6], mk(c1, -s1));
+    v[7] = cmul(v[7], mk(c2, -s2));
+    v[8] = cmul(v[8], mk(-c2, -s2));
+    v[9] = cmul(v[9], mk(-c1, -s1));
+    dft<5>(v);
+    dft<5>(v + 5);
+    const cd t1 = v[1], t2 = v[2], t3 = v[3], t4 = v[4], t5 = v[5], t6 = v[6], t7 = v[7], t8 = v[8];
+    v[2] = t1; v[4] = t2; v[6] = t3; v[8] = t4;
+    v[1] = t5; v[3] = t6; v[5] = t7; v[7] = t8;
+}
+
 // Twiddles W_M^k = e^{-2 pi i k / M} come from two small LDS tables,
 // W_M^k = T1[k >> 6] * T0[k & 63] (T0: 64 entries, T1: M/64 entries), both filled
 // from the correctly rounded e^{-2 pi i k/N} table of the grid; the product adds at
@@ -214,7 +240,7 @@ __device__ __forceinline__ void fft_pass_dispatch(cd *s, int M, int Ns, const cd
 {
     const int B = M / R;
     const int per = (B + T - 1) / T;
-    if (R == 8 || per <= 1) fft_pass<R, 1>(s, M, Ns, lt, tid, T);
+    if (R >= 8 || per <= 1) fft_pass<R, 1>(s, M, Ns, lt, tid, T);
     else if (R == 4 || R == 5 || per <= 2) fft_pass<R, 2>(s, M, Ns, lt, tid, T);
     else fft_pass<R, 4>(s, M, Ns, lt, tid, T);
 }
@@ -223,11 +249,15 @@ __device__ __forceinline__ void fft_pass_dispatch(cd *s, int M, int Ns, const cd
 // tid < T of a group (T >= M/8; every thread of the workgroup calls); code = the
 // packed radix schedule (ce_radix_plan.code, held in a register: no memory access
 // per pass); lt = LDS twiddle tables (load_twiddles), valid before the call
-__device__ __forceinline__ void fft_lds(cd *s, int M, uint32_t code, const cd *__restrict__ lt, int tid, int T)
+template <bool RADIX10>
+__device__ __forceinline__ void fft_lds_t(cd *s, int M, uint32_t code, const cd *__restrict__ lt, int tid, int T)
 {
     int Ns = 1;
     for (uint32_t c = code; c != 0; c >>= 3) {
         switch (c & 7) {
+        case 7:
+            if (RADIX10) { fft_pass_dispatch<10>(s, M, Ns, lt, tid, T); Ns *= 10; }
+            break;
         case 5: fft_pass_dispatch<8>(s, M, Ns, lt, tid, T); Ns *= 8; break;
         case 3: fft_pass_dispatch<4>(s, M, Ns, lt, tid, T); Ns *= 4; break;
         case 1: fft_pass_dispatch<2>(s, M, Ns, lt, tid, T); Ns *= 2; break;
@@ -237,9 +267,20 @@ __device__ __forceinline__ void fft_lds(cd *s, int M, uint32_t code, const cd *_
         }
     }
 }
+// the plans of ce_radix_schedule (radices 8, 4, 2, 5, 3)
+__device__ __forceinline__ void fft_lds(cd *s, int M, uint32_t code, const cd *__restrict__ lt, int tid, int T)
+{
+    fft_lds_t<false>(s, M, code, lt, tid, T);
+}
 __device__ __forceinline__ void fft_lds(cd *s, int M, uint32_t code, const cd *__restrict__ lt)
 {
-    fft_lds(s, M, code, lt, threadIdx.x, blockDim.x);
+    fft_lds_t<false>(s, M, code, lt, threadIdx.x, blockDim.x);
+}
+// the plans of ce_radix_schedule_ex with radix-10 passes (the kernels that instantiate
+// the wider pass: its registers count toward every path of a kernel)
+__device__ __forceinline__ void fft_lds10(cd *s, int M, uint32_t code, const cd *__restrict__ lt)
+{
+    fft_lds_t<true>(s, M, code, lt, threadIdx.x, blockDim.x);
 }
 
 // r2c split step: from Z = FFT_M(z), z_j = x_{2j} + i x_{2j+1}, produce X_k for the

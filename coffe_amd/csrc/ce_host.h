@@ -46,5 +46,8 @@ void ce_twiddles(int N, double *out);
 void ce_set_last_error(int code);
 /* radix schedule of a length-M complex FFT (radices 8, 4, 2, 5, 3) */
 int ce_radix_schedule(int M, ce_radix_plan *rp);
+/* the same with radix-10 passes for the pairs of factors 2 and 5 (fewer passes for
+   lengths such as 8000 = 8 * 10^3); for kernels that use fft_lds10 */
+int ce_radix_schedule_ex(int M, ce_radix_plan *rp, int pair10);
 
 #endif

@@ -28,7 +28,7 @@ typedef struct ce_radix_plan {
     int npass;
     int radix[CE_MAX_PASSES];
     /* the same schedule packed 3 bits a pass, first pass lowest, 0 ends it: codes
-       1..5 = radix 2, 3, 4, 5, 8 (at most 10 passes) */
+       1..5 = radix 2, 3, 4, 5, 8 and 7 = radix 10 (at most 10 passes) */
     uint32_t code;
 } ce_radix_plan;
 

@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(1024) k_cov_ahat(int N, ce_radix_plan rp, cons
     const cd *a = A + (int64_t)blockIdx.x * N;
     for (int i = tid; i < N; i += T) smem[pidx(i)] = cconj(a[i]);
     lds_barrier();
-    fft_lds(smem, N, rp.code, lt);
+    fft_lds10(smem, N, rp.code, lt);
     cd *o = ahat + (int64_t)blockIdx.x * N;
     for (int i = tid; i < N; i += T) o[i] = cconj(smem[pidx(i)]);
 }
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(1024) k_cov_rows(rows_args a)
         }
     }
     lds_barrier();
-    fft_lds(smem, N, a.rpN.code, ltN);
+    fft_lds10(smem, N, a.rpN.code, ltN);
     // Y = B V on n <= M with Im Y_0 = Im Y_M = 0 (dropped by the c2r), merged into the
     // half-length sequence (stored conjugated for the inverse transform)
     for (int it0 = 0; it0 * T <= M / 2; it0 += 2) {
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(1024) k_cov_rows(rows_args a)
         }
     }
     lds_barrier();
-    fft_lds(smem, M, a.rpM.code, ltM);
+    fft_lds10(smem, M, a.rpM.code, ltM);
     // column functionals of this row: W[n] = sum_j V_n(j) row(j), the weights carrying
     // pi / (16 N^2 r_j^nu2); four threads per functional, combined by lane shuffles
     const double den_i = a.prn1[i];

@@ -102,7 +102,7 @@ static int cov_length_ok(int N, double width)
 {
     ce_radix_plan rp;
     if (N < 8 || N > 8192 || (N & 1)) return 0;
-    if (ce_radix_schedule(N, &rp) != COFFE_OK || ce_radix_schedule(N / 2, &rp) != COFFE_OK) return 0;
+    if (ce_radix_schedule_ex(N, &rp, 1) != COFFE_OK || ce_radix_schedule_ex(N / 2, &rp, 1) != COFFE_OK) return 0;
     const long ncut = (long)((N / 2) * width);
     return ncut >= 1 && N / 2 + 1 + ncut <= N - 1;
 }
@@ -302,8 +302,8 @@ int coffe_cov_plan_create(coffe_engine *e, int N, double kmin, double kmax, int 
     p->e = e; p->N = N; p->M = N / 2; p->nl = nl; p->R = R; p->kmin = kmin; p->kmax = kmax;
     memcpy(p->ls, ls, sizeof(int) * (size_t)nl);
     p->step = log(kmax / kmin) / N;
-    int st = ce_radix_schedule(N, &p->rpN);
-    if (st == COFFE_OK) st = ce_radix_schedule(N / 2, &p->rpM);
+    int st = ce_radix_schedule_ex(N, &p->rpN, 1);
+    if (st == COFFE_OK) st = ce_radix_schedule_ex(N / 2, &p->rpM, 1);
     int pair_of[CV_MAXL * CV_MAXL];
     double fac[2 * CV_MAXPAIR];
     p->npair = 0;
@@ -587,8 +587,8 @@ int two_sph_bessel_status(const double *k1, const double *k2, double **fk1k2, lo
     for (int i = 0; i < N; ++i) r1[i] = 1. / k1[N - 1 - i];
     for (int i = 0; i < N; ++i) r2[i] = 1. / k2[N - 1 - i];
     ce_radix_plan rpN, rpM;
-    ce_radix_schedule(N, &rpN);
-    ce_radix_schedule(M, &rpM);
+    ce_radix_schedule_ex(N, &rpN, 1);
+    ce_radix_schedule_ex(M, &rpM, 1);
     double *tw = (double *)malloc(sizeof(double) * 2 * (size_t)N);
     double *A = (double *)malloc(sizeof(double) * 2 * (size_t)N);
     double *B = (double *)malloc(sizeof(double) * 2 * (size_t)(M + 1));

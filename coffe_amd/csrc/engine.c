@@ -98,19 +98,35 @@ double ce_selectqnu(double l, double nu)
     return q;
 }
 
-int ce_radix_schedule(int M, ce_radix_plan *rp)
+int ce_radix_schedule(int M, ce_radix_plan *rp) { return ce_radix_schedule_ex(M, rp, 0); }
+
+int ce_radix_schedule_ex(int M, ce_radix_plan *rp, int pair10)
 {
     rp->npass = 0;
     int m = M;
+    int n10 = 0;
+    if (pair10 && M > 0) {
+        int a = 0, b = 0, t = M;
+        while (t % 2 == 0) { ++a; t /= 2; }
+        t = M;
+        while (t % 5 == 0) { ++b; t /= 5; }
+        n10 = a < b ? a : b;
+        for (int i = 0; i < n10; ++i) m /= 10;
+    }
     while (m % 8 == 0) { rp->radix[rp->npass++] = 8; m /= 8; }
     while (m % 4 == 0) { rp->radix[rp->npass++] = 4; m /= 4; }
     while (m % 2 == 0) { rp->radix[rp->npass++] = 2; m /= 2; }
     while (m % 5 == 0) { if (rp->npass >= CE_MAX_PASSES) return COFFE_EUNSUPPORTED; rp->radix[rp->npass++] = 5; m /= 5; }
     while (m % 3 == 0) { if (rp->npass >= CE_MAX_PASSES) return COFFE_EUNSUPPORTED; rp->radix[rp->npass++] = 3; m /= 3; }
+    for (int i = 0; i < n10; ++i) {
+        if (rp->npass >= CE_MAX_PASSES) return COFFE_EUNSUPPORTED;
+        rp->radix[rp->npass++] = 10;
+    }
     if (m != 1 || rp->npass > 10) return COFFE_EUNSUPPORTED;
     rp->code = 0;
     for (int p = rp->npass - 1; p >= 0; --p) {
-        const uint32_t c = rp->radix[p] == 2 ? 1 : rp->radix[p] == 3 ? 2 : rp->radix[p] == 4 ? 3 : rp->radix[p] == 5 ? 4 : 5;
+        const int r = rp->radix[p];
+        const uint32_t c = r == 2 ? 1 : r == 3 ? 2 : r == 4 ? 3 : r == 5 ? 4 : r == 8 ? 5 : 7;
         rp->code = (rp->code << 3) | c;
     }
     return COFFE_OK;
