@@ -190,32 +190,64 @@ __device__ __forceinline__ void load_twiddles(cd *lt, int M, gcd *tw, int stride
     }
 }
 
-// One Stockham pass with radix R over s[0..M); Ns = product of earlier radices;
-// lt holds the LDS twiddle tables; threads tid < T of the group own the transform
-// (every thread of the workgroup calls: the pass ends on workgroup barriers).
-template <int R, int MAXB>
-__device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restrict__ lt, int tid, int T)
+// One Stockham pass with radix R over s[0..M); Ns = product of earlier radices and
+// stride = M/(Ns R) (the twiddle step), both uniform over the wavefront; lt holds the
+// LDS twiddle tables; threads tid < T of the group own the transform (every thread of
+// the workgroup calls: the pass ends on workgroup barriers).
+//
+// Index arithmetic is kept off the per-butterfly path: j mod Ns is a mask when Ns is
+// a power of two and otherwise a float-reciprocal quotient with one correction each
+// way (j, Ns < 2^24, so the estimate is off by at most one); the padded LDS slots of a
+// butterfly are one computed slot plus uniform offsets whenever no pad slot falls
+// between them at a position that depends on the lane (B or Ns a multiple of 16, or
+// power-of-two R and Ns with Ns < 16, where the R outputs stay inside one 16-slot
+// block or start at its first slots).
+template <int R, int MAXB, bool TW>
+__device__ __forceinline__ void fft_pass(cd *s, int Ns, int stride, const cd *__restrict__ lt, int tid, int T)
 {
-    const int B = M / R;
+    const int B = Ns * stride;   // M / R butterflies
     cd v[MAXB][R];
+    if ((B & 15) == 0) {
+        const int Bs = B + (B >> 4);   // pidx(j + r B) = pidx(j) + r Bs
 #pragma unroll
-    for (int b = 0; b < MAXB; ++b) {
-        const int j = tid + b * T;
-        if (j < B) {
+        for (int b = 0; b < MAXB; ++b) {
+            const int j = tid + b * T;
+            if (j < B) {
+                const cd *p = s + pidx(j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = s[pidx(j + r * B)];
+                for (int r = 0; r < R; ++r) v[b][r] = p[r * Bs];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+            const int j = tid + b * T;
+            if (j < B) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b][r] = s[pidx(j + r * B)];
+            }
         }
     }
     lds_barrier();
-    const int stride = M / (Ns * R);
+    const bool pow2 = (Ns & (Ns - 1)) == 0;
+    const bool lin = (Ns & 15) == 0 || ((R & (R - 1)) == 0 && pow2 && Ns < 16);
+    const float rns = __builtin_amdgcn_rcpf((float)Ns);
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int j = tid + b * T;
         if (j < B) {
-            const int jm = j % Ns;
-            if (jm != 0) {
+            int jm;
+            if (pow2) {
+                jm = j & (Ns - 1);
+            } else {
+                jm = j - Ns * (int)((float)j * rns);
+                jm += (jm < 0) ? Ns : 0;
+                jm -= (jm >= Ns) ? Ns : 0;
+            }
+            if (TW) {
                 // w_r = W^(r jm stride): one table lookup, the rest by binary splitting
-                // w_r = w_{r/2} w_{r - r/2} (at most three roundings deep for R <= 8)
+                // w_r = w_{r/2} w_{r - r/2} (at most four roundings deep for R <= 10);
+                // jm = 0 gives w_r = 1 exactly
                 cd w[R];
                 w[1] = twiddle(lt, jm * stride);
 #pragma unroll
@@ -225,8 +257,14 @@ __device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restr
             }
             dft<R>(v[b]);
             const int base = (j - jm) * R + jm;
+            if (lin) {
+                cd *p = s + pidx(base);
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[pidx(base + r * Ns)] = v[b][r];
+                for (int r = 0; r < R; ++r) p[r * Ns + ((r * Ns) >> 4)] = v[b][r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) s[pidx(base + r * Ns)] = v[b][r];
+            }
         }
     }
     lds_barrier();
@@ -235,37 +273,48 @@ __device__ __forceinline__ void fft_pass(cd *s, int M, int Ns, const cd *__restr
 // Butterflies per thread: a transform gets at least M/8 threads, so a radix-R pass
 // has at most ceil(8/R) butterflies per thread; only those register shapes are
 // instantiated (at most 12 complex values live per thread).
-template <int R>
-__device__ __forceinline__ void fft_pass_dispatch(cd *s, int M, int Ns, const cd *__restrict__ lt, int tid, int T)
+// TW = false for the first pass (Ns = 1: every twiddle is 1), a separate instance so
+// that no register copies join the two paths.
+template <int R, bool TW>
+__device__ __forceinline__ void fft_pass_dispatch(cd *s, int Ns, int stride, const cd *__restrict__ lt, int tid, int T)
 {
-    const int B = M / R;
-    const int per = (B + T - 1) / T;
-    if (R >= 8 || per <= 1) fft_pass<R, 1>(s, M, Ns, lt, tid, T);
-    else if (R == 4 || R == 5 || per <= 2) fft_pass<R, 2>(s, M, Ns, lt, tid, T);
-    else fft_pass<R, 4>(s, M, Ns, lt, tid, T);
+    const int B = Ns * stride;
+    if (R >= 8 || B <= T) fft_pass<R, 1, TW>(s, Ns, stride, lt, tid, T);
+    else if (R == 4 || R == 5 || B <= 2 * T) fft_pass<R, 2, TW>(s, Ns, stride, lt, tid, T);
+    else fft_pass<R, 4, TW>(s, Ns, stride, lt, tid, T);
+}
+
+// one pass of radix code c (3 bits) at Ns; rem = M / Ns on entry, M / (Ns R) on exit
+template <bool RADIX10, bool TW>
+__device__ __forceinline__ void fft_pass_code(cd *s, uint32_t c, int &Ns, unsigned &rem, const cd *__restrict__ lt, int tid, int T)
+{
+    switch (c) {
+    case 7:
+        if (RADIX10) { rem /= 10; fft_pass_dispatch<10, TW>(s, Ns, (int)rem, lt, tid, T); Ns *= 10; }
+        break;
+    case 5: rem >>= 3; fft_pass_dispatch<8, TW>(s, Ns, (int)rem, lt, tid, T); Ns *= 8; break;
+    case 3: rem >>= 2; fft_pass_dispatch<4, TW>(s, Ns, (int)rem, lt, tid, T); Ns *= 4; break;
+    case 1: rem >>= 1; fft_pass_dispatch<2, TW>(s, Ns, (int)rem, lt, tid, T); Ns *= 2; break;
+    case 4: rem /= 5; fft_pass_dispatch<5, TW>(s, Ns, (int)rem, lt, tid, T); Ns *= 5; break;
+    case 2: rem /= 3; fft_pass_dispatch<3, TW>(s, Ns, (int)rem, lt, tid, T); Ns *= 3; break;
+    default: break;
+    }
 }
 
 // forward complex FFT of length M in LDS (natural order in and out) by the threads
-// tid < T of a group (T >= M/8; every thread of the workgroup calls); code = the
-// packed radix schedule (ce_radix_plan.code, held in a register: no memory access
-// per pass); lt = LDS twiddle tables (load_twiddles), valid before the call
+// tid < T of a group (T >= M/8; every thread of the workgroup calls); M and code are
+// uniform over each wavefront (a group is whole wavefronts); code = the packed radix
+// schedule (ce_radix_plan.code, held in a scalar register: no memory access per pass);
+// lt = LDS twiddle tables (load_twiddles), valid before the call
 template <bool RADIX10>
 __device__ __forceinline__ void fft_lds_t(cd *s, int M, uint32_t code, const cd *__restrict__ lt, int tid, int T)
 {
     int Ns = 1;
-    for (uint32_t c = code; c != 0; c >>= 3) {
-        switch (c & 7) {
-        case 7:
-            if (RADIX10) { fft_pass_dispatch<10>(s, M, Ns, lt, tid, T); Ns *= 10; }
-            break;
-        case 5: fft_pass_dispatch<8>(s, M, Ns, lt, tid, T); Ns *= 8; break;
-        case 3: fft_pass_dispatch<4>(s, M, Ns, lt, tid, T); Ns *= 4; break;
-        case 1: fft_pass_dispatch<2>(s, M, Ns, lt, tid, T); Ns *= 2; break;
-        case 4: fft_pass_dispatch<5>(s, M, Ns, lt, tid, T); Ns *= 5; break;
-        case 2: fft_pass_dispatch<3>(s, M, Ns, lt, tid, T); Ns *= 3; break;
-        default: break;
-        }
-    }
+    unsigned rem = (unsigned)__builtin_amdgcn_readfirstlane(M);   // M / Ns
+    uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)code);
+    if (c == 0) return;
+    fft_pass_code<RADIX10, false>(s, c & 7, Ns, rem, lt, tid, T);
+    for (c >>= 3; c != 0; c >>= 3) fft_pass_code<RADIX10, true>(s, c & 7, Ns, rem, lt, tid, T);
 }
 // the plans of ce_radix_schedule (radices 8, 4, 2, 5, 3)
 __device__ __forceinline__ void fft_lds(cd *s, int M, uint32_t code, const cd *__restrict__ lt, int tid, int T)
