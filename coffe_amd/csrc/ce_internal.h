@@ -24,6 +24,10 @@ extern "C" {
 #define CE_CHUNK_NODES 648
 
 /* FFT radix schedule for a complex transform of length M = N/2 */
+/* complex LDS slots of one transform of length M (the swizzled layout of the FFT
+   kernels permutes slots inside aligned blocks of eight) */
+#define CE_LDS_DATA_LEN(M) ((((M) + 7) / 8) * 8)
+
 typedef struct ce_radix_plan {
     int npass;
     int radix[CE_MAX_PASSES];

@@ -543,7 +543,7 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
         const char *pe = getenv("COFFE_ENGINE_PACK");
         const int allow = !(pe && pe[0] == '0');
         b->npack = 0;
-        b->lt_off = (int)(b->max_M + b->max_M / 16 + 1);
+        b->lt_off = (int)CE_LDS_DATA_LEN(b->max_M);
         for (int i = 0; i < b->nunit && st == COFFE_OK;) {
             const int M = (int)(n_fft[units[i].spec] / 2);
             const int per = (M + 7) / 8;
@@ -560,7 +560,7 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
             pk[1] = c;
             pk[2] = P;
             pk[3] = 0;
-            const int need = P * (M + M / 16 + 1);
+            const int need = P * CE_LDS_DATA_LEN(M);
             if (need > b->lt_off) b->lt_off = need;
             i += c;
         }

@@ -27,7 +27,8 @@ def _fixture_spectrum():
     return k * H, pk / H ** 3
 
 
-@pytest.mark.parametrize("N", [64, 256, 1000, 2048])
+# 1000 and 1200 = 2^4 * 3 * 5^2 take the radix-10 passes next to radix 2, 3, 4 and 5
+@pytest.mark.parametrize("N", [64, 256, 1000, 1200, 2048])
 @pytest.mark.parametrize("l12", [(0, 0), (0, 2), (2, 4), (4, 4)])
 def test_two_sph_bessel_dropin(engine, N, l12):
     from coffe_amd.covariance import two_sph_bessel
