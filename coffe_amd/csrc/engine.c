@@ -565,6 +565,32 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
             i += c;
         }
     }
+    /* the ntab tables of a spectrum all read its transformed input: runs of 8 spectra
+       in single-unit packs are reordered so that the packs of one spectrum get
+       workgroup indices equal mod 8, i.e. one XCD (workgroups go to the 8 XCDs round
+       robin), whose L2 then serves that input to all but the first table; the order
+       stays largest first at the granularity of the run */
+    const char *xe = getenv("COFFE_ENGINE_XCD");
+    if (st == COFFE_OK && ntab > 1 && !(xe && xe[0] == '0')) {
+        const int run = 8 * ntab;
+        int32_t tmp[4 * 8 * CE_MAX_TABLES];
+        for (int p = 0; p + run <= b->npack;) {
+            int ok = 1;
+            for (int q = 0; q < run && ok; ++q) {
+                const int32_t *pk = packs + 4 * (p + q);
+                const int32_t *head = packs + 4 * (p + (q / ntab) * ntab);
+                ok = pk[1] == 1 && pk[2] == 1 && units[pk[0]].tab == q % ntab &&
+                     units[pk[0]].spec == units[head[0]].spec;
+            }
+            if (!ok) { ++p; continue; }
+            for (int q = 0; q < run; ++q) {
+                const int g = q / ntab, t = q % ntab;
+                memcpy(tmp + 4 * (8 * t + g), packs + 4 * (p + q), 4 * sizeof(int32_t));
+            }
+            memcpy(packs + 4 * p, tmp, 4 * sizeof(int32_t) * (size_t)run);
+            p += run;
+        }
+    }
     if (st == COFFE_OK) st = dev_alloc((void **)&b->d_packs, 4 * sizeof(int32_t) * (size_t)(b->npack > 0 ? b->npack : 1));
     hipStream_t s = e->stream;
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(b->d_packs, packs, 4 * sizeof(int32_t) * (size_t)b->npack, hipMemcpyHostToDevice, s));
