@@ -10,7 +10,9 @@ computed pieces are dropped when a parameter they depend on changes
 reference's order (multipoles.c:76-86: z, l, r; covariance.c:1046-1060: z, l1, r1,
 l2, r2).  Contributions beyond density, RSD and Doppler-1 (lensing, d2, g1-g5) and
 the CLASS / corrfunc / redshift-averaged paths are not part of this engine and raise
-NotImplementedError.
+NotImplementedError; the divergent table those terms read, I_0^4, is available as
+integral(r=..., n=4, l=0), and its renormalisation grid through
+coffe_amd.engine.divergent_tables.
 """
 import numpy as np
 
@@ -84,6 +86,7 @@ class Coffe:
     def _free_all(self):
         self._bg = None
         self._tables = None
+        self._divergent = None
         self._multipoles = None
         self._covariance = None
 
@@ -238,6 +241,7 @@ class Coffe:
             raise ValueError("The power spectrum needs at least 5 samples")
         self._pk = (np.ascontiguousarray(k), np.ascontiguousarray(pk))
         self._tables = None
+        self._divergent = None
         self._free_results()
 
     @property
@@ -351,11 +355,29 @@ class Coffe:
             self._tables = {t: integral_table(t[0], t[1], k, pk, N) for t in TABLES}
         return self._tables
 
+    def _divergent_table(self):
+        """I_0^4(r) r^4 at the reference's separations (integrals.c:772-834), the table the
+        reference builds when d2 or g1-g5 is on; here built on first use."""
+        self._need_pk()
+        if self._divergent is None:
+            from .engine import divergent_tables
+            k, pk = self._pk
+            sep, i40, _, _, status, _ = divergent_tables(k, pk, return_status=True)
+            if np.any(status[:len(sep)] == 1):
+                raise RuntimeError("I_0^4: an integral reached the interval cap")
+            self._divergent = (sep, i40)
+        return self._divergent
+
     def integral(self, *, r, n, l):  # noqa: E741
-        """I_l^n(r) (times r^(n - l) when n > l), Akima of the table (coffe.pyx:1657-1713)."""
+        """I_l^n(r) (times r^(n - l) when n > l), Akima of the table (coffe.pyx:1657-1713).
+        (n, l) = (4, 0) is the divergent table, which the reference holds only when d2
+        or g1-g5 is on (coffe.pyx:1108-1109, integrals.c:772-834)."""
         _check_parameter("r", r, (int, float), 0, 25000)
         _check_parameter("n", n, int, -2, 4)
         _check_parameter("l", l, int, 0, 4)
+        if (n, l) == (4, 0):
+            x, y = self._divergent_table()
+            return float(_Spline(x, y)(float(r))[0])
         if (n, l) not in TABLES:
             raise ValueError(f"Cannot find integral with n = {n}, l = {l}")
         x, y = self._integral_tables()[(n, l)]

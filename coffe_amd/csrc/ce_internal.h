@@ -124,6 +124,11 @@ int ce_launch_spline_adj(void *stream, int nchunk, const ce_chunk_dev *d_chunks,
 int ce_launch_small_r(void *stream, int nspec, int ntab, const ce_spec_dev *d_spec, const ce_grid_dev *d_grids,
                       const ce_table_desc *d_tabs, const double *d_k, const double *d_pk, double epsrel,
                       double *d_out, int *d_status);
+/* the divergent I_0^4 table and its renormalisations (quad_kernels.hip): status
+   and interval counts (may be null) [2 nsep + nchi^2] in the order i40, r0, ren */
+int ce_launch_divergent(void *stream, const double *d_k, const double *d_pk, int n_in, double kmin, double kmax,
+                        const double *d_sep, int nsep, const double *d_chi, int nchi, double epsrel, double *d_i40,
+                        double *d_r0, double *d_ren, int *d_status, int *d_nint);
 
 /* ---- 2D FFTLog covariance (cov_kernels.hip) ---- */
 size_t ce_cov_lds_bytes(int N);

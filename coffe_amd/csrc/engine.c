@@ -1110,6 +1110,73 @@ int coffe_batch_small_r(coffe_batch *b, const double *d_k, const double *d_pk, d
     return st;
 }
 
+int coffe_divergent_tables(const double *k, const double *pk, size_t nin, double kmin, double kmax,
+                           const double *sep, size_t nsep, const double *chi, size_t nchi, double *out_i40,
+                           double *out_r0, double *out_ren, int *out_status, int *out_intervals)
+{
+    if (!k || !pk || nin < 5 || (nsep && (!sep || !out_i40 || !out_r0)) || (nchi && (!chi || !out_ren)))
+        return COFFE_EINVAL;
+    if (!(kmin > 0) || !(kmin >= k[0]) || !(kmax <= k[nin - 1]) || !(kmin < kmax)) return COFFE_EDOMAIN;
+    if (nsep > (1u << 20) || nchi > 4096 || nin > ((size_t)1 << 28)) return COFFE_EINVAL;
+    for (size_t i = 0; i < nsep; ++i)
+        if (!(sep[i] >= 0) || !isfinite(sep[i])) return COFFE_EDOMAIN;
+    for (size_t i = 0; i < nchi; ++i)
+        if (!(chi[i] >= 0) || !isfinite(chi[i])) return COFFE_EDOMAIN;
+    const size_t units = 2 * nsep + nchi * nchi;
+    if (units == 0) return COFFE_OK;
+    coffe_engine *e = coffe_engine_default();
+    if (!e) return COFFE_ENODEVICE;
+    hipSetDevice(e->device);
+    /* scratch: k, P | sep | chi | i40 | r0 | ren | status | interval counts */
+    const size_t o_sep = 2 * nin, o_chi = o_sep + nsep, o_i40 = o_chi + nchi, o_r0 = o_i40 + nsep,
+                 o_ren = o_r0 + nsep, o_stat = o_ren + nchi * nchi;
+    int *h_stat = (int *)malloc(sizeof(int) * units);
+    const int want_n = out_intervals != NULL;
+    if (!h_stat) return COFFE_ENOMEM;
+    pthread_mutex_lock(&e->lock);
+    int st = scratch_reserve(e, sizeof(double) * o_stat + 2 * sizeof(int) * units + 64, e->stream);
+    if (st != COFFE_OK) { pthread_mutex_unlock(&e->lock); free(h_stat); return st; }
+    double *base = (double *)e->d_scratch;
+    int *d_stat = (int *)(base + o_stat);
+    int *d_nint = d_stat + units;
+    hipStream_t s = e->stream;
+    st = hip_ok(hipMemcpyAsync(base, k, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(base + nin, pk, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK && nsep)
+        st = hip_ok(hipMemcpyAsync(base + o_sep, sep, sizeof(double) * nsep, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK && nchi)
+        st = hip_ok(hipMemcpyAsync(base + o_chi, chi, sizeof(double) * nchi, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK &&
+        ce_launch_divergent(s, base, base + nin, (int)nin, kmin, kmax, base + o_sep, (int)nsep, base + o_chi,
+                            (int)nchi, 1e-5, base + o_i40, base + o_r0, base + o_ren, d_stat,
+                            want_n ? d_nint : NULL))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK && nsep) {
+        st = hip_ok(hipMemcpyAsync(out_i40, base + o_i40, sizeof(double) * nsep, hipMemcpyDeviceToHost, s));
+        if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(out_r0, base + o_r0, sizeof(double) * nsep, hipMemcpyDeviceToHost, s));
+    }
+    if (st == COFFE_OK && nchi)
+        st = hip_ok(hipMemcpyAsync(out_ren, base + o_ren, sizeof(double) * nchi * nchi, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(h_stat, d_stat, sizeof(int) * units, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK && want_n)
+        st = hip_ok(hipMemcpyAsync(out_intervals, d_nint, sizeof(int) * units, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    if (st != COFFE_OK) hipStreamSynchronize(s);
+    pthread_mutex_unlock(&e->lock);
+    if (st == COFFE_OK) {
+        if (out_status) memcpy(out_status, h_stat, sizeof(int) * units);
+        /* the reference runs these integrals with GSL's error handler off
+           (integrals.c:593) and keeps roundoff-limited results; only a non-finite
+           value or this engine's interval cap (1024, the reference's 50000) fails */
+        for (size_t i = 0; i < units && st == COFFE_OK; ++i)
+            if (h_stat[i] == 4) st = COFFE_ENONFINITE;
+        for (size_t i = 0; i < units && st == COFFE_OK; ++i)
+            if (h_stat[i] == 1) st = COFFE_ENOCONV;
+    }
+    free(h_stat);
+    return st;
+}
+
 int coffe_integral_table(int n, int l, size_t N, const double *k, const double *pk, size_t nin, double kmin,
                          double kmax, double *out_x, double *out_y)
 {

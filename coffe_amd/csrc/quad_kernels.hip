@@ -93,20 +93,41 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// integrands over k of the Akima spline P(k) of the input samples
+enum {
+    SR_BESSEL = 0,    // k^2 P j_l(k r) / (k r)^n              (integrals.c:280-325)
+    SR_PREFACTOR = 1, // k^(2+l-n) P                            (integrals.c:238-269)
+    SR_RENORM0 = 2,   // P (1 - j_0(k r)^2) / k^2                (integrals.c:389-398)
+    SR_RENORM = 3     // P j_0(k r) j_0(k r2) / k^2              (integrals.c:405-417)
+};
+
 struct sr_unit {
     const double *k, *p;   // input samples of the spectrum
     int n_in;
     int n, l;
-    double r;              // 0 for node 0
-    int prefactor;         // node 0: k^(2+l-n) P(k)
+    double r, r2;
+    int kind;
 };
 
 __device__ __forceinline__ double integrand(const sr_unit &u, double k)
 {
     const double pk = ce::akima_general(u.k, u.p, u.n_in, k);
-    if (u.prefactor) return pow(k, (double)(2 + u.l - u.n)) * pk;
-    const double x = k * u.r;
-    return k * k * pk * sph_jl(u.l, x) / pow(x, (double)u.n);
+    switch (u.kind) {
+    case SR_PREFACTOR: return pow(k, (double)(2 + u.l - u.n)) * pk;
+    case SR_RENORM0: {
+        const double j0 = sph_jl(0, k * u.r);
+        return pk * (1. - j0 * j0) / k / k;
+    }
+    case SR_RENORM: {
+        // the product of the two Bessel factors first: symmetric in (r, r2) bit for bit
+        const double jj = sph_jl(0, k * u.r) * sph_jl(0, k * u.r2);
+        return pk * jj / k / k;
+    }
+    default: {
+        const double x = k * u.r;
+        return k * k * pk * sph_jl(u.l, x) / pow(x, (double)u.n);
+    }
+    }
 }
 
 struct qk_out {
@@ -159,6 +180,82 @@ struct sr_args {
     int *status;      // same layout: 0 converged, 1 interval limit, 2 roundoff, 3 bad integrand
 };
 
+// QUADPACK QAG (epsabs 0, the 61-point rule) of u over [lo, hi] by one wavefront, the
+// interval list in LDS (cap CE_QAG_CAP); every lane returns the same sum.  Status: 0
+// converged, 1 interval limit, 2 roundoff, 3 bad integrand behaviour.
+__device__ int qag_wave(const sr_unit &u, double lo, double hi, double epsrel, int lane, double *ia, double *ib,
+                        double *ir, double *ie, double &result, int *nint = nullptr)
+{
+    if (nint) *nint = 1;
+    qk_out q0 = qk61(u, lo, hi, lane);
+    int st = 0;
+    result = q0.result;
+    double tol = epsrel * fabs(q0.result);
+    const double round_off = 50.0 * DBL_EPSILON * q0.resabs;
+    const bool done0 = (q0.abserr <= round_off && q0.abserr > tol) ||
+                       (q0.abserr <= tol && q0.abserr != q0.resasc) || q0.abserr == 0.0;
+    if (q0.abserr <= round_off && q0.abserr > tol) st = 2;
+    if (done0) return st;
+    if (lane == 0) {
+        ia[0] = lo; ib[0] = hi; ir[0] = q0.result; ie[0] = q0.abserr;
+    }
+    __syncthreads();
+    int size = 1, imax = 0, iter = 1, rt1 = 0, rt2 = 0;
+    double area = q0.result, errsum = q0.abserr;
+    do {
+        const double a1 = ia[imax], b2 = ib[imax];
+        const double b1 = 0.5 * (a1 + b2), a2 = b1;
+        const double ri = ir[imax], ei = ie[imax];
+        const qk_out q1 = qk61(u, a1, b1, lane);
+        const qk_out q2 = qk61(u, a2, b2, lane);
+        const double area12 = q1.result + q2.result, err12 = q1.abserr + q2.abserr;
+        errsum += err12 - ei;
+        area += area12 - ri;
+        if (q1.resasc != q1.abserr && q2.resasc != q2.abserr) {
+            const double delta = ri - area12;
+            if (fabs(delta) <= 1.0e-5 * fabs(area12) && err12 >= 0.99 * ei) rt1++;
+            if (iter >= 10 && err12 > ei) rt2++;
+        }
+        tol = epsrel * fabs(area);
+        if (errsum > tol) {
+            if (rt1 >= 6 || rt2 >= 20) st = 2;
+            const double tmp = (1 + 100 * DBL_EPSILON) * (fabs(a2) + 1000 * DBL_MIN);
+            if (fabs(a1) <= tmp && fabs(b2) <= tmp) st = 3;
+        }
+        __syncthreads();   // every lane has read the interval being replaced
+        if (lane == 0) {
+            // the refined interval keeps the half with the larger error
+            const bool right = q2.abserr > q1.abserr;
+            ia[imax] = right ? a2 : a1; ib[imax] = right ? b2 : b1;
+            ir[imax] = right ? q2.result : q1.result; ie[imax] = right ? q2.abserr : q1.abserr;
+            ia[size] = right ? a1 : a2; ib[size] = right ? b1 : b2;
+            ir[size] = right ? q1.result : q2.result; ie[size] = right ? q1.abserr : q2.abserr;
+        }
+        __syncthreads();
+        size++;
+        // the interval with the largest error, first one on ties
+        double be = -1.0;
+        int bi = 0x7fffffff;
+        for (int kk = lane; kk < size; kk += 64) {
+            if (ie[kk] > be) { be = ie[kk]; bi = kk; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double oe = __shfl_xor(be, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (oe > be || (oe == be && oi < bi)) { be = oe; bi = oi; }
+        }
+        imax = bi < size ? bi : 0;
+        iter++;
+        if (iter >= CE_QAG_CAP && errsum > tol && !st) st = 1;
+    } while (iter < CE_QAG_CAP && !st && errsum > tol);
+    double part = 0.0;
+    for (int kk = lane; kk < size; kk += 64) part += ir[kk];
+    result = wave_sum(part);
+    if (nint) *nint = size;
+    return st;
+}
+
 // one wavefront (block of 64) per (spectrum, table, node)
 __global__ void __launch_bounds__(64) k_small_r(sr_args a)
 {
@@ -177,8 +274,9 @@ __global__ void __launch_bounds__(64) k_small_r(sr_args a)
     u.n_in = sp.n_in;
     u.n = (int)a.tabs[t].n;
     u.l = (int)a.tabs[t].l;
-    u.prefactor = node == 0;
+    u.kind = node == 0 ? SR_PREFACTOR : SR_BESSEL;
     u.r = node == 0 ? 0.0 : c_min_sep[node - 1];
+    u.r2 = 0.0;
     const int64_t o = (int64_t)unit;
     if (node == 0 && u.n < u.l) {   // I_l^n(0) = 0 for n < l
         if (lane == 0) {
@@ -187,73 +285,8 @@ __global__ void __launch_bounds__(64) k_small_r(sr_args a)
         }
         return;
     }
-    const double lo = g.kmin, hi = g.kmax;
-    qk_out q0 = qk61(u, lo, hi, lane);
-    int st = 0;
-    double result = q0.result;
-    double tol = a.epsrel * fabs(q0.result);
-    const double round_off = 50.0 * DBL_EPSILON * q0.resabs;
-    const bool done0 = (q0.abserr <= round_off && q0.abserr > tol) ||
-                       (q0.abserr <= tol && q0.abserr != q0.resasc) || q0.abserr == 0.0;
-    if (q0.abserr <= round_off && q0.abserr > tol) st = 2;
-    if (!done0) {
-        if (lane == 0) {
-            ia[0] = lo; ib[0] = hi; ir[0] = q0.result; ie[0] = q0.abserr;
-        }
-        __syncthreads();
-        int size = 1, imax = 0, iter = 1, rt1 = 0, rt2 = 0;
-        double area = q0.result, errsum = q0.abserr;
-        do {
-            const double a1 = ia[imax], b2 = ib[imax];
-            const double b1 = 0.5 * (a1 + b2), a2 = b1;
-            const double ri = ir[imax], ei = ie[imax];
-            const qk_out q1 = qk61(u, a1, b1, lane);
-            const qk_out q2 = qk61(u, a2, b2, lane);
-            const double area12 = q1.result + q2.result, err12 = q1.abserr + q2.abserr;
-            errsum += err12 - ei;
-            area += area12 - ri;
-            if (q1.resasc != q1.abserr && q2.resasc != q2.abserr) {
-                const double delta = ri - area12;
-                if (fabs(delta) <= 1.0e-5 * fabs(area12) && err12 >= 0.99 * ei) rt1++;
-                if (iter >= 10 && err12 > ei) rt2++;
-            }
-            tol = a.epsrel * fabs(area);
-            if (errsum > tol) {
-                if (rt1 >= 6 || rt2 >= 20) st = 2;
-                const double tmp = (1 + 100 * DBL_EPSILON) * (fabs(a2) + 1000 * DBL_MIN);
-                if (fabs(a1) <= tmp && fabs(b2) <= tmp) st = 3;
-            }
-            __syncthreads();   // every lane has read the interval being replaced
-            if (lane == 0) {
-                // the refined interval keeps the half with the larger error
-                const bool right = q2.abserr > q1.abserr;
-                ia[imax] = right ? a2 : a1; ib[imax] = right ? b2 : b1;
-                ir[imax] = right ? q2.result : q1.result; ie[imax] = right ? q2.abserr : q1.abserr;
-                ia[size] = right ? a1 : a2; ib[size] = right ? b1 : b2;
-                ir[size] = right ? q1.result : q2.result; ie[size] = right ? q1.abserr : q2.abserr;
-            }
-            __syncthreads();
-            size++;
-            // the interval with the largest error, first one on ties
-            double be = -1.0;
-            int bi = 0x7fffffff;
-            for (int kk = lane; kk < size; kk += 64) {
-                if (ie[kk] > be) { be = ie[kk]; bi = kk; }
-            }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double oe = __shfl_xor(be, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (oe > be || (oe == be && oi < bi)) { be = oe; bi = oi; }
-            }
-            imax = bi < size ? bi : 0;
-            iter++;
-            if (iter >= CE_QAG_CAP && errsum > tol && !st) st = 1;
-        } while (iter < CE_QAG_CAP && !st && errsum > tol);
-        double part = 0.0;
-        for (int kk = lane; kk < size; kk += 64) part += ir[kk];
-        result = wave_sum(part);
-    }
+    double result;
+    int st = qag_wave(u, g.kmin, g.kmax, a.epsrel, lane, ia, ib, ir, ie, result);
     if (lane == 0) {
         const double pi = 3.14159265358979323846;
         double v = result / 2. / pi / pi;
@@ -267,6 +300,62 @@ __global__ void __launch_bounds__(64) k_small_r(sr_args a)
         if (!isfinite(v) && st == 0) st = 3;
         a.out[o] = v;
         a.status[o] = st;
+    }
+}
+
+struct div_args {
+    const double *k, *p;   // input samples of the spectrum
+    int n_in;
+    double kmin, kmax, epsrel;
+    const double *sep;     // [nsep]
+    int nsep;
+    const double *chi;     // [nchi]
+    int nchi;
+    double *i40, *r0;      // [nsep] each
+    double *ren;           // [nchi][nchi]: (chi[j], chi[i]) at i nchi + j
+    int *status;           // [2 nsep + nchi^2]: the QAG status, 4 for a non-finite value
+    int *nint;             // same layout: final interval count, or null
+};
+
+// the divergent table I_0^4 and its renormalisations (integrals.c:772-909), one
+// wavefront per integral: units [0, nsep) I_0^4(sep) sep^4, [nsep, 2 nsep) the r = 0
+// renormalisation at sep (times sep^4 as well, integrals.c:351-356, n = 4, l = 0), then
+// the nchi^2 grid of int P j_0 j_0 / k^2; all divided by 2 pi^2
+__global__ void __launch_bounds__(64) k_divergent(div_args a)
+{
+    __shared__ double ia[CE_QAG_CAP], ib[CE_QAG_CAP], ir[CE_QAG_CAP], ie[CE_QAG_CAP];
+    const int unit = blockIdx.x;
+    const int lane = threadIdx.x;
+    sr_unit u;
+    u.k = a.k;
+    u.p = a.p;
+    u.n_in = a.n_in;
+    u.n = 4;
+    u.l = 0;
+    u.r2 = 0.0;
+    if (unit < 2 * a.nsep) {
+        u.kind = unit < a.nsep ? SR_BESSEL : SR_RENORM0;
+        u.r = a.sep[unit < a.nsep ? unit : unit - a.nsep];
+    } else {
+        const int q = unit - 2 * a.nsep;
+        u.kind = SR_RENORM;
+        u.r = a.chi[q % a.nchi];
+        u.r2 = a.chi[q / a.nchi];
+    }
+    double result;
+    int nint;
+    int st = qag_wave(u, a.kmin, a.kmax, a.epsrel, lane, ia, ib, ir, ie, result, &nint);
+    if (lane == 0) {
+        const double pi = 3.14159265358979323846;
+        if (a.nint) a.nint[unit] = nint;
+        double v = result;
+        if (u.kind != SR_RENORM && u.r > 0) v *= pow(u.r, 4.0);
+        v = v / 2. / pi / pi;
+        if (!isfinite(v)) st = 4;
+        if (unit < a.nsep) a.i40[unit] = v;
+        else if (unit < 2 * a.nsep) a.r0[unit - a.nsep] = v;
+        else a.ren[unit - 2 * a.nsep] = v;
+        a.status[unit] = st;
     }
 }
 
@@ -296,5 +385,27 @@ extern "C" int ce_launch_small_r(void *stream, int nspec, int ntab, const ce_spe
     a.k = d_k; a.p = d_pk; a.epsrel = epsrel; a.out = d_out; a.status = d_status;
     const int units = nspec * ntab * (1 + CE_MIN_SEP_LEN);
     hipLaunchKernelGGL(k_small_r, dim3(units), dim3(64), 0, s, a);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ce_launch_divergent(void *stream, const double *d_k, const double *d_pk, int n_in, double kmin,
+                                   double kmax, const double *d_sep, int nsep, const double *d_chi, int nchi,
+                                   double epsrel, double *d_i40, double *d_r0, double *d_ren, int *d_status,
+                                   int *d_nint)
+{
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_xgk), GK61K_X, sizeof(GK61K_X), 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_wgk), GK61K_W, sizeof(GK61K_W), 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_wg), GK61G_W, sizeof(GK61G_W), 0, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return (int)e;
+    div_args a;
+    a.k = d_k; a.p = d_pk; a.n_in = n_in; a.kmin = kmin; a.kmax = kmax; a.epsrel = epsrel;
+    a.sep = d_sep; a.nsep = nsep; a.chi = d_chi; a.nchi = nchi;
+    a.i40 = d_i40; a.r0 = d_r0; a.ren = d_ren; a.status = d_status; a.nint = d_nint;
+    const int units = 2 * nsep + nchi * nchi;
+    if (units <= 0) return 0;
+    hipLaunchKernelGGL(k_divergent, dim3(units), dim3(64), 0, s, a);
     return (int)hipGetLastError();
 }

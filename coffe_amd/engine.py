@@ -3,7 +3,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, load_library
+from ._lib import COFFE_ENOCONV, check, load_library
 
 TABLE_ORDER = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]  # (n, l), integrals.c:599-608
 
@@ -220,6 +220,49 @@ def integral_table(n, l, k, pk, N, kmin=None, kmax=None):
     check(L.coffe_integral_table(int(n), int(l), int(N), k.ctypes.data_as(dp), pk.ctypes.data_as(dp), len(k), kmin,
                                  kmax, ox.ctypes.data_as(dp), oy.ctypes.data_as(dp)), "coffe_integral_table")
     return ox, oy
+
+
+# separations of the divergent table I_0^4, in Mpc (integrals.c:76-99)
+COFFE_SEP = np.array(
+    [0.000001, 0.000005, 0.0001, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4, 0.8, 1., 1.5]
+    + list(range(2, 31)) + list(range(32, 81, 2)) + list(range(81, 96))
+    + [95.5, 96., 96.5, 97., 97.5, 98., 98.5, 99., 99.5, 100., 100.5, 101., 101.5, 102., 102.5, 103., 103.5,
+       104., 104.5, 105.] + list(range(106, 111)) + list(range(112, 121, 2)) + list(range(124, 181, 4))
+    + list(range(185, 241, 5)) + list(range(250, 301, 10)) + list(range(320, 701, 20))
+    + list(range(750, 6601, 50)) + list(range(7000, 20001, 1000)), dtype=np.float64)
+
+
+def divergent_tables(k, pk, sep=None, chi=None, kmin=None, kmax=None, return_status=False):
+    """The divergent table I_0^4 and its renormalisations (integrals.c:772-909) through
+    coffe_divergent_tables, host arrays: returns (sep, I_0^4(sep) sep^4, the r = 0
+    renormalisation at sep times sep^4, the chi x chi grid ren[a, b] = (chi_b, chi_a)),
+    all divided by 2 pi^2.  sep defaults to the reference's separation list, chi to no
+    grid.  return_status adds the per-integral QUADPACK codes and interval counts (in
+    the order sep for I_0^4, sep for the renormalisation, then the grid) and returns
+    the values also when an integral reached the interval cap."""
+    L = load_library()
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    pk = np.ascontiguousarray(pk, dtype=np.float64)
+    sep = COFFE_SEP.copy() if sep is None else np.ascontiguousarray(sep, dtype=np.float64)
+    chi = np.zeros(0) if chi is None else np.ascontiguousarray(chi, dtype=np.float64)
+    kmin = float(k[0]) if kmin is None else kmin
+    kmax = float(k[-1]) if kmax is None else kmax
+    i40, r0 = np.empty(len(sep)), np.empty(len(sep))
+    ren = np.empty((len(chi), len(chi)))
+    units = 2 * len(sep) + len(chi) ** 2
+    status = np.empty(units, dtype=np.int32)
+    nint = np.empty(units, dtype=np.int32)
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    rc = L.coffe_divergent_tables(k.ctypes.data_as(dp), pk.ctypes.data_as(dp), len(k), kmin, kmax,
+                                  sep.ctypes.data_as(dp), len(sep), chi.ctypes.data_as(dp), len(chi),
+                                  i40.ctypes.data_as(dp), r0.ctypes.data_as(dp), ren.ctypes.data_as(dp),
+                                  status.ctypes.data_as(ip), nint.ctypes.data_as(ip))
+    if return_status and rc == COFFE_ENOCONV:
+        return sep, i40, r0, ren, status, nint
+    check(rc, "coffe_divergent_tables")
+    if return_status:
+        return sep, i40, r0, ren, status, nint
+    return sep, i40, r0, ren
 
 
 def akima(x, y, xq):

@@ -183,6 +183,32 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
  * Akima spline of (k, pk); the FFT nodes are twofast_1bessel(l, nu = n, r0 = 1,
  * k0 = kmin).  Integer n in [-2, 4], l in [0, 4].  Synchronous, default engine.
  */
+/*
+ * The divergent table I_0^4 and its renormalisations (integrals.c:772-909, the
+ * reference's `divergent` branch, built when d2 or g1-g5 is requested), each a
+ * QAG-GK61 integral (epsrel 1e-5, integrals.c:331-374) over [kmin, kmax] of the Akima
+ * spline P of (k, pk), divided by 2 pi^2:
+ *   out_i40[i] = sep_i^4 int dk k^2 P j_0(k sep_i)/(k sep_i)^4   (integrals.c:797-811)
+ *   out_r0[i]  = sep_i^4 int dk P (1 - j_0(k sep_i)^2)/k^2       (integrals.c:813-825,
+ *                the same sep^(n-l) factor of integrals.c:351-356; the reference then
+ *                replaces the first node by (0, 0) for its spline, 838-839)
+ *   out_ren[a nchi + b] = int dk P j_0(k chi_b) j_0(k chi_a)/k^2 (integrals.c:883-895,
+ *                the (nbins+1)^2 grid result2d[k (nbins+1) + j])
+ * sep >= 0 (nsep), chi >= 0 (nchi <= 4096), host arrays; one wavefront per integral
+ * (intervals capped at 1024 per integral).  out_status (or NULL), 2 nsep + nchi^2
+ * entries in the order i40, r0, ren: QUADPACK's code (0 converged, 1 interval cap,
+ * 2 roundoff, 3 bad integrand behaviour) or 4 for a non-finite value; out_intervals
+ * (or NULL), same layout: the final number of intervals (GSL's workspace size; the
+ * rule was evaluated 61 (2 n - 1) times).  As the
+ * reference (its GSL error handler is off here, integrals.c:593) roundoff-limited
+ * results are kept; returns COFFE_ENONFINITE for a non-finite value and COFFE_ENOCONV
+ * when an integral reached the interval cap, the values still written.  Synchronous,
+ * default engine.
+ */
+int coffe_divergent_tables(const double *k, const double *pk, size_t nin, double kmin, double kmax,
+                           const double *sep, size_t nsep, const double *chi, size_t nchi, double *out_i40,
+                           double *out_r0, double *out_ren, int *out_status, int *out_intervals);
+
 int coffe_integral_table(int n, int l, size_t N, const double *k, const double *pk, size_t nin, double kmin,
                          double kmax, double *out_x, double *out_y);
 

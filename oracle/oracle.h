@@ -77,6 +77,9 @@ int orc_twofast_1bessel(double *output_x, double *output_y, size_t N, const doub
 int orc_integral_table(int n, int l, size_t N, size_t nin, const double *kin, const double *pin, double kmin,
                        double kmax, int with_small_r, double *out_x, double *out_y);
 
+int orc_divergent_tables(size_t nin, const double *kin, const double *pin, double kmin, double kmax, size_t nsep,
+                         const double *sep, size_t nchi, const double *chi, double *i40, double *r0, double *ren);
+
 int orc_tables_at_sep_batch(int nspec, const long long *off, const double *k, const double *p, const long long *nfft,
                             int ntab, const int *tab_n, const int *tab_l, size_t nsep, const double *sep, double *out);
 

@@ -53,6 +53,7 @@ def _declare(L):
     L.orc_twofast_selectqnu.restype = d
     L.orc_twofast_1bessel.argtypes = [p, p, s, p, p, s, d, d, d, d, d, d]
     L.orc_integral_table.argtypes = [i, i, s, s, p, p, d, d, i, p, p]
+    L.orc_divergent_tables.argtypes = [s, p, p, d, d, s, p, s, p, p, p, p]
     L.orc_multipoles.argtypes = [ctypes.POINTER(c_double_p), ctypes.POINTER(c_double_p), ctypes.POINTER(s), s,
                                  p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
                                  ctypes.POINTER(ctypes.c_int), s, p, i, d, p]
@@ -208,6 +209,19 @@ def integral_table(n, l, k, pk, N, small_r=True):
     _check(lib().orc_integral_table(n, l, N, len(k), _p(k), _p(pk), k[0], k[-1], int(small_r), _p(ox), _p(oy)),
            "integral_table")
     return ox, oy
+
+
+def divergent_tables(k, pk, sep, chi, kmin=None, kmax=None):
+    """integrals.c:772-909: (I_0^4(sep) sep^4, the r = 0 renormalisation at sep (times
+    sep^4), the chi x chi renormalisation grid [a, b] = (chi_b, chi_a)), all / 2 pi^2."""
+    k, pk, sep, chi = _f64(k), _f64(pk), _f64(sep), _f64(chi)
+    kmin = k[0] if kmin is None else kmin
+    kmax = k[-1] if kmax is None else kmax
+    i40, r0 = np.empty(len(sep)), np.empty(len(sep))
+    ren = np.empty((len(chi), len(chi)))
+    _check(lib().orc_divergent_tables(len(k), _p(k), _p(pk), kmin, kmax, len(sep), _p(sep), len(chi), _p(chi),
+                                      _p(i40), _p(r0), _p(ren)), "divergent_tables")
+    return i40, r0, ren
 
 
 TABLE_ORDER = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]  # (n, l), integrals.c:599-608

@@ -253,6 +253,68 @@ done:
     return st;
 }
 
+/* the divergent table I_0^4 and its renormalisations (integrals.c:772-909): QAG-GK61
+   at epsrel 1e-5 with a 50000-interval workspace (common.c:1004-1040) of
+     k^2 P j_0(k r)/(k r)^4                  times r^4      (integrals.c:280-325, 351-356)
+     P (1 - j_0(k r)^2)/k^2                  times r^4      (integrals.c:389-398, 351-356)
+     P j_0(k chi_1) j_0(k chi_2)/k^2                        (integrals.c:405-448)
+   each divided by 2 pi^2; ren[a nchi + b] = (chi_b, chi_a) as result2d[k (nbins+1) + j]
+   (integrals.c:883-895).  OpenMP over the integrals. */
+typedef struct {
+    bessel_params bp;
+    double r2;
+} renorm_params;
+
+static double integrand_renorm0(double k, void *vp)
+{
+    const bessel_params *p = (const bessel_params *)vp;
+    const double j0 = orc_sph_jl(0, k * p->r);
+    return orc_akima_eval(p->nin, p->k, p->p, p->b, p->c, p->d, k) * (1. - j0 * j0) / k / k;
+}
+
+static double integrand_renorm(double k, void *vp)
+{
+    const renorm_params *q = (const renorm_params *)vp;
+    const bessel_params *p = &q->bp;
+    const double jj = orc_sph_jl(0, k * p->r) * orc_sph_jl(0, k * q->r2);
+    return orc_akima_eval(p->nin, p->k, p->p, p->b, p->c, p->d, k) * jj / k / k;
+}
+
+int orc_divergent_tables(size_t nin, const double *kin, const double *pin, double kmin, double kmax, size_t nsep,
+                         const double *sep, size_t nchi, const double *chi, double *i40, double *r0, double *ren)
+{
+    double *akb = (double *)malloc(sizeof(double) * 3 * nin);
+    if (!akb) return ORC_ENOMEM;
+    int st = orc_akima_init(nin, kin, pin, akb, akb + nin, akb + 2 * nin);
+    if (st != ORC_OK) { free(akb); return st; }
+    const long long units = (long long)(2 * nsep + nchi * nchi);
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (long long u = 0; u < units; ++u) {
+        renorm_params q;
+        bessel_params bp = {nin, kin, pin, akb, akb + nin, akb + 2 * nin, 0.0, 4.0, 0.0};
+        q.bp = bp;
+        q.r2 = 0.0;
+        double v;
+        if (u < (long long)(2 * nsep)) {
+            const int direct = u < (long long)nsep;
+            const double r = sep[direct ? u : u - (long long)nsep];
+            q.bp.r = r;
+            v = orc_qag(direct ? integrand_bessel : integrand_renorm0, &q.bp, kmin, kmax, 0, 1e-5, 50000, NULL, NULL);
+            if (r > 0) v *= pow(r, 4.0);
+            v = v / 2. / ORC_PI / ORC_PI;
+            if (direct) i40[u] = v; else r0[u - (long long)nsep] = v;
+        } else {
+            const long long w = u - (long long)(2 * nsep);
+            q.bp.r = chi[w % (long long)nchi];
+            q.r2 = chi[w / (long long)nchi];
+            v = orc_qag(integrand_renorm, &q, kmin, kmax, 0, 1e-5, 50000, NULL, NULL);
+            ren[w] = v / 2. / ORC_PI / ORC_PI;
+        }
+    }
+    free(akb);
+    return ORC_OK;
+}
+
 /* tables at separations for a batch: out[s][t][r] = Akima(table_t of spectrum s)(sep_r),
    tables without the small-r nodes (they do not enter Akima stencils at sep >= r_2);
    OpenMP over (spectrum, table) */
