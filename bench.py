@@ -282,6 +282,101 @@ def run_covariance(args):
     plan.close()
 
 
+DIV_FLOP_PER_EVAL = 145.0
+
+
+def run_divergent(args):
+    """The divergent table I_0^4 of the reference's configuration (integrals.c:772-909):
+    the CLASS spectrum of the test suite, k in [1e-5, 300], the 297 separations of
+    integrals.c:76-99 (I_0^4 and the r = 0 renormalisation) and the 201 x 201
+    renormalisation grid on [0, chi_max], chi_max = 2477.9 Mpc (as in
+    benchmark_integral8_renormalization.dat): 41,005 QAG-GK61 integrals per step, one
+    wavefront each (k_divergent)."""
+    import ctypes
+    import torch
+    import oracle as O
+    from coffe_amd._lib import check, load_library
+    from coffe_amd.engine import COFFE_SEP, Engine
+    k, pk = np.loadtxt(os.path.join(ROOT, "tests", "golden", "PkL_CLASS.dat")).T
+    kmin, kmax = 1e-5, 300.0
+    chi = np.linspace(0.0, 0.8265275 * 2997.92458, 201)
+    sep = COFFE_SEP
+    units = 2 * len(sep) + len(chi) ** 2
+    eng = Engine(0)
+    L = load_library()
+    dev = dict(device="cuda", dtype=torch.float64)
+    kt, pt = torch.tensor(k, **dev), torch.tensor(pk, **dev)
+    st, ct = torch.tensor(sep, **dev), torch.tensor(chi, **dev)
+    i40, r0 = torch.empty(len(sep), **dev), torch.empty(len(sep), **dev)
+    ren = torch.empty((len(chi), len(chi)), **dev)
+    status = torch.empty(units, dtype=torch.int32, device="cuda")
+    nint = torch.empty(units, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        check(L.coffe_divergent_tables_device(eng.h, kt.data_ptr(), pt.data_ptr(), len(k), kmin, kmax,
+                                              st.data_ptr(), len(sep), ct.data_ptr(), len(chi), i40.data_ptr(),
+                                              r0.data_ptr(), ren.data_ptr(), status.data_ptr(), nint.data_ptr(),
+                                              ctypes.c_void_p(stream.cuda_stream)), "coffe_divergent_tables_device")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the kernel alone: HIP events around back-to-back launches on the launch stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    reps = 10
+    ev[0].record(stream)
+    for _ in range(reps):
+        step()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    us = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+    n = nint.cpu().numpy().astype(np.float64)
+    evals = float(np.sum(61.0 * (2.0 * n - 1.0)))
+    flops = DIV_FLOP_PER_EVAL * evals
+    stat = status.cpu().numpy()
+    x, y = np.loadtxt(os.path.join(ROOT, "tests", "golden", "benchmark_integral8.dat")).T
+    rel = np.abs(i40.cpu().numpy() * (1 / 2997.92458) ** 4 / y - 1)
+    x1, x2, v = np.loadtxt(os.path.join(ROOT, "tests", "golden", "benchmark_integral8_renormalization_every10.dat")).T
+    g = ren.cpu().numpy()[::10, ::10].reshape(-1) * (1 / 2997.92458) ** 4
+    rel2 = np.abs(g / v - 1)
+    res = {
+        "metric": "divergent-table integrals/sec (I_0^4 at 297 seps + r=0 renormalisation + 201x201 grid), 1 MI355X",
+        "value": units / (ms / 1e3), "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "reference test spectrum (PkL_CLASS.dat)",
+        "config": {"workload": "I_0^4 and its renormalisations of the reference's test configuration: k in [1e-5, 300], "
+                               "297 separations, 201 x 201 chi grid to 2477.9 Mpc; QAG-GK61 epsrel 1e-5 per integral",
+                   "integrals_per_step": units},
+        "roofline": {"bound": "mfma", "kernel": "k_divergent", "achieved": flops / (us * 1e-6) / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": flops / (us * 1e-6) / PEAK_FP64,
+                     "traffic": traffic_of("divergent_ref_config", "k_divergent"), "launch_us": us,
+                     "note": f"{DIV_FLOP_PER_EVAL:.0f} flop per integrand evaluation (Akima 50, two j_0 with sincos "
+                             "80, rule sums 15), 61 (2 n - 1) evaluations per integral of n final intervals (counted "
+                             f"on the device: {evals:.3e} per launch); one wavefront per integral, its bisections "
+                             "serial: latency-bound; FP64 peak 78.6 TFLOP/s"},
+        "parity": {"i40_vs_fixture_max_rel": float(rel.max()), "grid_vs_fixture_max_rel": float(rel2.max()),
+                   "interval_cap_hits": int(np.sum(stat == 1)),
+                   "note": "benchmark_integral8.dat (11 digits) and every 10th node of "
+                           "benchmark_integral8_renormalization.dat (7 digits); the one cap hit is the r = 0 "
+                           "renormalisation at sep = 1e-6 Mpc, which the reference discards (integrals.c:838-839)"},
+    }
+    if not args.no_cpu:
+        t1 = time.perf_counter()
+        O.divergent_tables(k, pk, sep, chi[::2], kmin=kmin, kmax=kmax)
+        dt = time.perf_counter() - t1
+        n_cpu = 2 * len(sep) + len(chi[::2]) ** 2
+        res["cpu_baseline"] = {"value": n_cpu / dt, "unit": "integrals/s", "cores": os.cpu_count(), "kind": "port",
+                               "sample": f"the oracle (C, OpenMP over the integrals) on the 297 separations and a "
+                                         f"101 x 101 grid ({n_cpu} integrals, {dt:.2f} s)"}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,7 +386,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=16)
-    ap.add_argument("--workload", choices=["batched", "fullsky", "covariance"], default="batched",
+    ap.add_argument("--workload", choices=["batched", "fullsky", "covariance", "divergent"], default="batched",
                     help="batched: the headline forward+adjoint batch; fullsky / covariance: the other "
                          "configurations, reported on their own lines")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -315,6 +410,8 @@ def main():
         return run_fullsky(args)
     if args.workload == "covariance":
         return run_covariance(args)
+    if args.workload == "divergent":
+        return run_divergent(args)
     import torch
     from coffe_amd.engine import Batch, Engine
     import oracle as O

@@ -1110,6 +1110,25 @@ int coffe_batch_small_r(coffe_batch *b, const double *d_k, const double *d_pk, d
     return st;
 }
 
+int coffe_divergent_tables_device(coffe_engine *e, const double *d_k, const double *d_pk, size_t nin, double kmin,
+                                  double kmax, const double *d_sep, size_t nsep, const double *d_chi, size_t nchi,
+                                  double *d_i40, double *d_r0, double *d_ren, int *d_status, int *d_intervals,
+                                  void *stream)
+{
+    if (!e || !d_k || !d_pk || nin < 5 || !d_status || (nsep && (!d_sep || !d_i40 || !d_r0)) ||
+        (nchi && (!d_chi || !d_ren)))
+        return COFFE_EINVAL;
+    if (!(kmin > 0) || !(kmin < kmax) || !isfinite(kmax)) return COFFE_EDOMAIN;
+    if (nsep > (1u << 20) || nchi > 4096 || nin > ((size_t)1 << 28)) return COFFE_EINVAL;
+    if (2 * nsep + nchi * nchi == 0) return COFFE_OK;
+    hipSetDevice(e->device);
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    if (ce_launch_divergent(s, d_k, d_pk, (int)nin, kmin, kmax, d_sep, (int)nsep, d_chi, (int)nchi, 1e-5, d_i40,
+                            d_r0, d_ren, d_status, d_intervals))
+        return COFFE_EDEVICE;
+    return COFFE_OK;
+}
+
 int coffe_divergent_tables(const double *k, const double *pk, size_t nin, double kmin, double kmax,
                            const double *sep, size_t nsep, const double *chi, size_t nchi, double *out_i40,
                            double *out_r0, double *out_ren, int *out_status, int *out_intervals)
@@ -1146,11 +1165,9 @@ int coffe_divergent_tables(const double *k, const double *pk, size_t nin, double
         st = hip_ok(hipMemcpyAsync(base + o_sep, sep, sizeof(double) * nsep, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK && nchi)
         st = hip_ok(hipMemcpyAsync(base + o_chi, chi, sizeof(double) * nchi, hipMemcpyHostToDevice, s));
-    if (st == COFFE_OK &&
-        ce_launch_divergent(s, base, base + nin, (int)nin, kmin, kmax, base + o_sep, (int)nsep, base + o_chi,
-                            (int)nchi, 1e-5, base + o_i40, base + o_r0, base + o_ren, d_stat,
-                            want_n ? d_nint : NULL))
-        st = COFFE_EDEVICE;
+    if (st == COFFE_OK)
+        st = coffe_divergent_tables_device(e, base, base + nin, nin, kmin, kmax, base + o_sep, nsep, base + o_chi, nchi,
+                                           base + o_i40, base + o_r0, base + o_ren, d_stat, want_n ? d_nint : NULL, s);
     if (st == COFFE_OK && nsep) {
         st = hip_ok(hipMemcpyAsync(out_i40, base + o_i40, sizeof(double) * nsep, hipMemcpyDeviceToHost, s));
         if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(out_r0, base + o_r0, sizeof(double) * nsep, hipMemcpyDeviceToHost, s));

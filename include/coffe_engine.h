@@ -208,6 +208,14 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
 int coffe_divergent_tables(const double *k, const double *pk, size_t nin, double kmin, double kmax,
                            const double *sep, size_t nsep, const double *chi, size_t nchi, double *out_i40,
                            double *out_r0, double *out_ren, int *out_status, int *out_intervals);
+/* the same on device arrays, asynchronous on stream (NULL = the engine's): d_status
+   (required) and d_intervals (or NULL) as above, the caller reads them after the
+   stream; [kmin, kmax] must lie inside [d_k[0], d_k[nin-1]] (outside, the Akima
+   spline is NaN and the integral reports status 4) */
+int coffe_divergent_tables_device(coffe_engine *e, const double *d_k, const double *d_pk, size_t nin, double kmin,
+                                  double kmax, const double *d_sep, size_t nsep, const double *d_chi, size_t nchi,
+                                  double *d_i40, double *d_r0, double *d_ren, int *d_status, int *d_intervals,
+                                  void *stream);
 
 int coffe_integral_table(int n, int l, size_t N, const double *k, const double *pk, size_t nin, double kmin,
                          double kmax, double *out_x, double *out_y);

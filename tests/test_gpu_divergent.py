@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
-def test_fixtures_full_grid():
+def test_fixtures_full_grid(engine):
     """The reference's configuration: the 297 separations and the 201 x 201 grid on
     [0, chi_max]; I_0^4 against benchmark_integral8.dat (11 digits), the grid against
     every 10th node of benchmark_integral8_renormalization.dat (7 digits); the grid is
@@ -42,7 +42,7 @@ def test_fixtures_full_grid():
     assert np.max(np.abs((r5 / s5[1:] ** 4 + np.diag(g5)[1:]) / g5[0, 0] - 1)) < 5e-5
 
 
-def test_against_oracle():
+def test_against_oracle(engine):
     """A synthetic spectrum: every separation and a 9 x 9 grid against the oracle (the
     same QAG bisection sequence)."""
     from coffe_amd.engine import divergent_tables
@@ -56,7 +56,7 @@ def test_against_oracle():
     assert np.max(np.abs(r0[1:] / orr[1:] - 1)) < 1e-8
 
 
-def test_gaussian_closed_forms():
+def test_gaussian_closed_forms(engine):
     from coffe_amd.engine import divergent_tables
     R = 8.0
     k = np.logspace(-8, np.log10(4.0), 4000)
@@ -73,7 +73,7 @@ def test_gaussian_closed_forms():
     assert np.max(np.abs(ren / wg - 1)) < 2e-5
 
 
-def test_coffe_integral_n4_l0():
+def test_coffe_integral_n4_l0(engine):
     """Coffe.integral(r, n=4, l=0): Akima of the table at the reference's separations;
     node values exactly, and between nodes the oracle's direct integral to 1e-4."""
     from coffe_amd.coffe import Coffe
