@@ -22,6 +22,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include "coffe_engine.h"
+#include "coffe_integrals.h"
 #include "ce_internal.h"
 #include "ce_host.h"
 
@@ -1108,6 +1109,49 @@ int coffe_batch_small_r(coffe_batch *b, const double *d_k, const double *d_pk, d
     if (st == COFFE_OK) st = check_small_r_status(h, n);
     free(h);
     return st;
+}
+
+/* the reference's coffe_interpolation (common.h:228-232) and the head of GSL's
+   gsl_spline it points to (interp, x, y, size: gsl_spline.h, GSL 2.7) */
+typedef struct {
+    void *interp;
+    double *x;
+    double *y;
+    size_t size;
+} ce_gsl_spline_view;
+typedef struct {
+    ce_gsl_spline_view *spline;
+    void *accel;
+} ce_interpolation_view;
+
+int coffe_integrals_renormalizable(double **output_x, double **output_y, const size_t output_len,
+                                   size_t *real_output_len, const struct coffe_interpolation *spectrum, const int n,
+                                   const int l, const int state_n, const int state_l, const double x_min,
+                                   const double x_max)
+{
+    if (!output_x || !output_y || !real_output_len || !spectrum) return COFFE_EINVAL;
+    if (state_n != 0 || state_l != 0) return COFFE_EUNSUPPORTED;
+    const ce_interpolation_view *sp = (const ce_interpolation_view *)spectrum;
+    if (!sp->spline || !sp->spline->x || !sp->spline->y) return COFFE_EINVAL;
+    const size_t len = output_len + 1 + CE_MIN_SEP_LEN;
+    double *x = (double *)malloc(sizeof(double) * len);
+    double *y = (double *)malloc(sizeof(double) * len);
+    if (!x || !y) {
+        free(x);
+        free(y);
+        return COFFE_ENOMEM;
+    }
+    const int st = coffe_integral_table(n, l, output_len, sp->spline->x, sp->spline->y, sp->spline->size, x_min,
+                                        x_max, x, y);
+    if (st != COFFE_OK) {
+        free(x);
+        free(y);
+        return st;
+    }
+    *output_x = x;
+    *output_y = y;
+    *real_output_len = len;
+    return COFFE_OK;
 }
 
 int coffe_divergent_tables_device(coffe_engine *e, const double *d_k, const double *d_pk, size_t nin, double kmin,

@@ -35,3 +35,41 @@ def test_error_strings_host_only():
     for code in range(9):
         assert L.coffe_engine_strerror(code)
     assert L.coffe_engine_last_error() == 0
+
+
+class _GslSplineView:
+    """ctypes mirrors of the reference's coffe_interpolation (common.h:228-232) and the
+    head of GSL's gsl_spline (interp, x, y, size)."""
+
+    @staticmethod
+    def make(x, y):
+        class Spline(ctypes.Structure):
+            _fields_ = [("interp", ctypes.c_void_p), ("x", ctypes.POINTER(ctypes.c_double)),
+                        ("y", ctypes.POINTER(ctypes.c_double)), ("size", ctypes.c_size_t)]
+
+        class Interp(ctypes.Structure):
+            _fields_ = [("spline", ctypes.POINTER(Spline)), ("accel", ctypes.c_void_p)]
+
+        dp = ctypes.POINTER(ctypes.c_double)
+        sp = Spline(None, x.ctypes.data_as(dp), y.ctypes.data_as(dp), len(x))
+        return Interp(ctypes.pointer(sp), None), sp
+
+
+def test_integrals_renormalizable_rejects_half_integer_states():
+    """The drop-in of integrals.c:459-573 returns COFFE_EUNSUPPORTED for half-integer
+    states (flat-sky lensing) before any device work, and allocates nothing."""
+    import numpy as np
+    from coffe_amd._lib import load_library
+    L = load_library()
+    k = np.logspace(-4, 1, 64)
+    pk = k / (1 + k ** 2)
+    interp, keep = _GslSplineView.make(k, pk)
+    ox, oy = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_double)()
+    n_out = ctypes.c_size_t(0)
+    f = L.coffe_integrals_renormalizable
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    rc = f(ctypes.byref(ox), ctypes.byref(oy), 1024, ctypes.byref(n_out), ctypes.byref(interp), 1, -1, 1, 1,
+           float(k[0]), float(k[-1]))
+    assert rc == 4
+    assert not ox and not oy and n_out.value == 0

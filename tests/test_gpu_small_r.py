@@ -54,3 +54,35 @@ def test_batch_small_r_vs_oracle(engine):
         for t, (n, l) in enumerate(TABLES):
             _, oy = O.integral_table(n, l, ks, ps, 1024, small_r=True)
             assert _close(got[s, t], oy[:43]), (s, n, l)
+
+
+def test_integrals_renormalizable_dropin(engine):
+    """The drop-in of coffe_integrals_renormalizable (integrals.c:459-573) reads the
+    spectrum from the reference's coffe_interpolation / gsl_spline layout, mallocs the
+    (N + 43)-point table and returns exactly coffe_integral_table's values."""
+    import ctypes
+    from coffe_amd._lib import load_library
+    from coffe_amd.engine import integral_table
+    from tests.test_abi import _GslSplineView
+    L = load_library()
+    k, pk = np.loadtxt(os.path.join(GOLD, "PkL_CLASS.dat")).T
+    k, pk = np.ascontiguousarray(k * H), np.ascontiguousarray(pk / H ** 3)
+    interp, keep = _GslSplineView.make(k, pk)
+    f = L.coffe_integrals_renormalizable
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    for n, l in ((0, 0), (1, 3), (2, 0)):
+        print(f"coffe_integrals_renormalizable n={n} l={l}", flush=True)
+        ox, oy = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_double)()
+        n_out = ctypes.c_size_t(0)
+        rc = f(ctypes.byref(ox), ctypes.byref(oy), 2048, ctypes.byref(n_out), ctypes.byref(interp), n, l, 0, 0,
+               float(k[0]), float(k[-1]))
+        assert rc == 0 and n_out.value == 2048 + 43
+        x = np.ctypeslib.as_array(ox, shape=(n_out.value,)).copy()
+        y = np.ctypeslib.as_array(oy, shape=(n_out.value,)).copy()
+        libc.free(ctypes.cast(ox, ctypes.c_void_p))
+        libc.free(ctypes.cast(oy, ctypes.c_void_p))
+        tx, ty = integral_table(n, l, k, pk, 2048)
+        assert np.array_equal(x, tx) and np.array_equal(y, ty)
