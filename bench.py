@@ -384,6 +384,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nspec", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batched: pick the workgroup layout and each length's radix order by timing them "
+                         "on the batch first (coffe_amd/autotune.py); default: the stored wisdom file")
+    ap.add_argument("--write-wisdom", default="", help="with --autotune: write the choice to this JSON file")
+    ap.add_argument("--no-wisdom", action="store_true", help="batched: the engine's default layout and radix order")
     ap.add_argument("--graph", action="store_true",
                     help="batched: replay the step captured in a hipGraph (measured slower than the eager "
                          "call: 0.167 against 0.163 ms)")
@@ -424,7 +429,11 @@ def main():
     n_in = [len(s[0]) for s in specs]
     n_fft = n_in
     eng = Engine(0)
-    batch = Batch(eng, n_in, n_fft, [s[0][0] for s in specs], [s[0][-1] for s in specs], TABLES, sep, 1, len(LS))
+
+    def make_batch():
+        return Batch(eng, n_in, n_fft, [s[0][0] for s in specs], [s[0][-1] for s in specs], TABLES, sep, 1, len(LS))
+
+    batch = make_batch()
     dev = "cuda"
     k = torch.tensor(np.concatenate([s[0] for s in specs]), device=dev)
     p = torch.tensor(np.concatenate([s[1] for s in specs]), device=dev)
@@ -434,6 +443,21 @@ def main():
     itab = torch.empty((nspec, len(TABLES), nsep), dtype=torch.float64, device=dev)
     out = torch.empty((nspec, 1, len(LS), nsep), dtype=torch.float64, device=dev)
     gpk = torch.empty_like(p)
+    tuned = None
+    from coffe_amd import autotune as AT
+    if args.autotune:
+        # workgroup layout and per-length radix order measured on this batch
+        batch.close()
+        batch, rep = AT.autotune(make_batch, k, p, A, gout, n_fft)
+        tuned = {"source": "measured in this run", "report": rep, "wisdom": AT.wisdom_of(rep)}
+        if args.write_wisdom:
+            with open(args.write_wisdom, "w") as f:
+                json.dump(AT.wisdom_of(rep), f, indent=1)
+    elif not args.no_wisdom and AT.load_wisdom() is not None:
+        # the stored choice of an earlier autotune run on this batch shape
+        batch.close()
+        batch = AT.apply_wisdom(make_batch, AT.load_wisdom())
+        tuned = {"source": "coffe_amd/wisdom_mi355x.json", "wisdom": AT.load_wisdom()}
     stream = torch.cuda.current_stream()
     sp = __import__("ctypes").c_void_p(stream.cuda_stream)
     kernels = ["k_resample", "k_prepare", "k_tables", "k_adj_c2r", "k_spline_adj"]
@@ -528,6 +552,7 @@ def main():
                    "points_per_step": points, "fft_lengths": sorted(set(int(n) for n in n_fft)),
                    "threads_per_wg": int(os.environ.get("COFFE_ENGINE_THREADS", 512)),
                    "launch": "hipGraph replay of the captured step" if args.graph else "eager (one C call)",
+                   "autotune": tuned,
                    "other_configs": "bench.py --workload fullsky | covariance"},
         "roofline": {"bound": "mfma", "kernel": dom,
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",

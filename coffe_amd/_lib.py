@@ -76,6 +76,7 @@ def load_library():
     L.coffe_integral_table.argtypes = [i, i, sz, dp, dp, sz, d, d, dp, dp]
     L.coffe_divergent_tables.argtypes = [dp, dp, sz, d, d, dp, sz, dp, sz, dp, dp, dp, ctypes.POINTER(i),
                                          ctypes.POINTER(i)]
+    L.coffe_batch_set_radix.argtypes = [vp, i, i, ctypes.POINTER(i)]
     L.coffe_divergent_tables_device.argtypes = [vp, vp, vp, sz, d, d, vp, sz, vp, sz, vp, vp, vp, vp, vp, vp]
     L.coffe_batch_small_r.argtypes = [vp, vp, vp, vp, vp]
     L.coffe_cov_plan_create.argtypes = [vp, i, d, d, i, ip, i, dp, ctypes.POINTER(vp)]
@@ -108,7 +109,7 @@ EXPORTED = [
     "coffe_batch_vjp", "coffe_batch_tables", "coffe_batch_prepare", "coffe_batch_tables_stage",
     "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute",
     "coffe_batch_resample", "coffe_batch_fft_prepare", "coffe_batch_adj_c2r", "coffe_batch_spline_adj",
-    "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_terms_of", "coffe_integral_table", "coffe_divergent_tables", "coffe_divergent_tables_device", "coffe_batch_small_r",
+    "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_terms_of", "coffe_integral_table", "coffe_divergent_tables", "coffe_divergent_tables_device", "coffe_batch_set_radix", "coffe_batch_small_r",
     "coffe_cov_plan_create", "coffe_cov_plan_destroy", "coffe_cov_plan_rows", "coffe_cov_plan_pairs",
     "coffe_cov_integrals", "coffe_cov_assemble", "two_sph_bessel", "two_sph_bessel_status",
 ]

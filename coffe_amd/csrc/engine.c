@@ -99,6 +99,17 @@ double ce_selectqnu(double l, double nu)
     return q;
 }
 
+/* the schedule packed 3 bits a pass, first pass lowest (ce_radix_plan.code) */
+static void radix_code(ce_radix_plan *rp)
+{
+    rp->code = 0;
+    for (int p = rp->npass - 1; p >= 0; --p) {
+        const int r = rp->radix[p];
+        const uint32_t c = r == 2 ? 1 : r == 3 ? 2 : r == 4 ? 3 : r == 5 ? 4 : r == 8 ? 5 : 7;
+        rp->code = (rp->code << 3) | c;
+    }
+}
+
 int ce_radix_schedule(int M, ce_radix_plan *rp) { return ce_radix_schedule_ex(M, rp, 0); }
 
 int ce_radix_schedule_ex(int M, ce_radix_plan *rp, int pair10)
@@ -114,9 +125,20 @@ int ce_radix_schedule_ex(int M, ce_radix_plan *rp, int pair10)
         n10 = a < b ? a : b;
         for (int i = 0; i < n10; ++i) m /= 10;
     }
-    while (m % 8 == 0) { rp->radix[rp->npass++] = 8; m /= 8; }
-    while (m % 4 == 0) { rp->radix[rp->npass++] = 4; m /= 4; }
-    while (m % 2 == 0) { rp->radix[rp->npass++] = 2; m /= 2; }
+    /* the power of two as radix-8 passes, then one radix-4 pass for a remaining 4, or
+       two radix-4 passes in place of 8 x 2 for a remaining 2 (same pass count; the
+       radix-2 pass with four butterflies per thread measured slower) */
+    int a = 0;
+    while (m % 2 == 0) { ++a; m /= 2; }
+    int n8 = a / 3;
+    const int rem = a % 3;
+    if (rem == 1 && n8 > 0) --n8;
+    for (int i = 0; i < n8; ++i) rp->radix[rp->npass++] = 8;
+    if (rem == 2) rp->radix[rp->npass++] = 4;
+    if (rem == 1) {
+        if (a > 1) { rp->radix[rp->npass++] = 4; rp->radix[rp->npass++] = 4; }
+        else rp->radix[rp->npass++] = 2;
+    }
     while (m % 5 == 0) { if (rp->npass >= CE_MAX_PASSES) return COFFE_EUNSUPPORTED; rp->radix[rp->npass++] = 5; m /= 5; }
     while (m % 3 == 0) { if (rp->npass >= CE_MAX_PASSES) return COFFE_EUNSUPPORTED; rp->radix[rp->npass++] = 3; m /= 3; }
     for (int i = 0; i < n10; ++i) {
@@ -124,12 +146,7 @@ int ce_radix_schedule_ex(int M, ce_radix_plan *rp, int pair10)
         rp->radix[rp->npass++] = 10;
     }
     if (m != 1 || rp->npass > 10) return COFFE_EUNSUPPORTED;
-    rp->code = 0;
-    for (int p = rp->npass - 1; p >= 0; --p) {
-        const int r = rp->radix[p];
-        const uint32_t c = r == 2 ? 1 : r == 3 ? 2 : r == 4 ? 3 : r == 5 ? 4 : r == 8 ? 5 : 7;
-        rp->code = (rp->code << 3) | c;
-    }
+    radix_code(rp);
     return COFFE_OK;
 }
 
@@ -730,6 +747,35 @@ int coffe_batch_vjp(coffe_batch *b, const double *d_k, const double *d_pk, const
     return ce_launch_spline_adj(s, b->nchunk, b->d_chunks, b->d_spec, b->d_grids, d_k, b->d_ws, d_gpk, b->d_ranges)
                ? COFFE_EDEVICE
                : COFFE_OK;
+}
+
+int coffe_batch_set_radix(coffe_batch *b, int M, int npass, const int *radix)
+{
+    if (!b || !radix || npass < 1 || npass > 10) return COFFE_EINVAL;
+    ce_radix_plan rp;
+    memset(&rp, 0, sizeof(rp));
+    long long prod = 1;
+    for (int p = 0; p < npass; ++p) {
+        const int r = radix[p];
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return COFFE_EINVAL;
+        prod *= r;
+        rp.radix[p] = r;
+    }
+    if (prod != M) return COFFE_EINVAL;
+    rp.npass = npass;
+    radix_code(&rp);
+    hipSetDevice(b->e->device);
+    int found = 0, st = COFFE_OK;
+    for (int gi = 0; gi < b->ngrid && st == COFFE_OK; ++gi) {
+        if (b->grids[gi]->N / 2 != M) continue;
+        ce_grid_dev hg;
+        grid_to_dev(b->grids[gi], &hg);
+        hg.rp = rp;
+        st = hip_ok(hipMemcpy(b->d_grids + gi, &hg, sizeof(hg), hipMemcpyHostToDevice));
+        found = 1;
+    }
+    if (st != COFFE_OK) return st;
+    return found ? COFFE_OK : COFFE_EDOMAIN;
 }
 
 int coffe_batch_tables(coffe_batch *b, const double *d_k, const double *d_pk, double *d_full, void *stream)

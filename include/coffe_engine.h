@@ -105,6 +105,12 @@ int coffe_batch_forward(coffe_batch *b, const double *d_k, const double *d_pk, c
 int coffe_batch_vjp(coffe_batch *b, const double *d_k, const double *d_pk, const double *d_A, const double *d_gout,
                     double *d_itab, double *d_out, double *d_gpk, void *stream);
 
+/* the FFT pass order of the batch's transforms of length M = n_fft/2: radix[0..npass)
+   from {2, 3, 4, 5, 8} with product M, first pass first (npass <= 10); the default is
+   radix 8 first, then 4, 2, 5, 3.  Synchronous (one upload); COFFE_EDOMAIN when the
+   batch has no transform of that length.  Results change at rounding level only. */
+int coffe_batch_set_radix(coffe_batch *b, int M, int npass, const int *radix);
+
 /* full tables: full[s][t][i] = r^(n-l) I_t(r_i), r_i = (kmax/kmin)^(i/N), i < n_fft[s];
    row stride = max n_fft */
 int coffe_batch_tables(coffe_batch *b, const double *d_k, const double *d_pk, double *d_full, void *stream);
