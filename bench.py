@@ -247,6 +247,17 @@ def run_covariance(args):
     ev[1].record(stream)
     torch.cuda.synchronize()
     rows_us = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+    # the adjoint of the integrals (d<G, integrals>/dP, coffe_cov_integrals_vjp) on the
+    # same batch, a random cotangent; the call synchronises its stream
+    gint = torch.randn(integ.shape, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    gpk = plan.integrals_vjp(k, p, n_in, gint)
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for _ in range(reps):
+        plan.integrals_vjp(k, p, n_in, gint, gpk=gpk)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    vjp_us = ev[0].elapsed_time(ev[1]) * 1e3 / reps
     N = 8000
     row_flops = 7.5 * N * np.log2(N) + 8.0 * N + 2.0 * 72 * len(sep)
     nrows_launch = 64 * 2 * plan.npair * plan.nrows
@@ -266,6 +277,9 @@ def run_covariance(args):
                              "2 x 72 x R (column functionals), N = 8000, R = 256, x 64 spectra x 2 powers x 6 pairs x "
                              "rows per pair; time = HIP events over 3 integrals calls (k_cov_diag + k_cov_rows + "
                              "k_cov_cols, rows > 95 % in the kernel trace); FP64 peak 78.6 TFLOP/s"},
+        "adjoint": {"integrals_us": rows_us, "integrals_vjp_us": vjp_us,
+                    "note": "coffe_cov_integrals_vjp on the same 64 spectra (d<G, integrals>/dP for every input "
+                            "sample), HIP events over 3 calls"},
     }
     if args.cpu_sample:
         ks, ps = specs[0][0], specs[0][1]

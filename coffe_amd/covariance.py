@@ -78,6 +78,26 @@ class CovPlan:
                                          _ptr(k), _ptr(pk), _ptr(out), st), "coffe_cov_integrals")
         return out
 
+    def integrals_vjp(self, k, pk, n_in, gint, gpk=None, stream=None):
+        """Vector-Jacobian product of `integrals`: the gradient of sum(gint * integrals(k, pk))
+        with respect to pk (the Akima interpolation of the samples included).  gint has the
+        shape of `integrals`' output; returns gpk, shaped like pk."""
+        import torch
+        n_in = np.ascontiguousarray(n_in, dtype=np.int64)
+        total = int(n_in.sum())
+        _need(k, (total,), "k")
+        _need(pk, (total,), "pk")
+        nspec = len(n_in)
+        _need(gint, (nspec, 2, self.npair, self.R, self.R), "gint")
+        if gpk is None:
+            gpk = torch.empty_like(pk)
+        _need(gpk, (total,), "gpk")
+        st = stream if stream is not None else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(self.L.coffe_cov_integrals_vjp(self.h, nspec, n_in.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                             _ptr(k), _ptr(pk), _ptr(gint), _ptr(gpk), st),
+              "coffe_cov_integrals_vjp")
+        return gpk
+
     def assemble(self, integ, bias, f, D1, nbar, vol, pix, poisson=True, mixed=True, cosmic=True, stream=None):
         """Gaussian covariance (nspec, nz, nl, R, nl, R); bias is (nz, 4) or (nz,) (one population)."""
         import torch

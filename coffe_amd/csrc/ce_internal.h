@@ -142,6 +142,17 @@ int ce_launch_cov_rows(void *stream, int nblk, int N, const ce_radix_plan *rpN, 
                        const int *d_pl1, const int *d_pl2, int nrows, const int *d_rows, const double *d_prn1,
                        const double *d_prn2, int R, const int *d_flo, const int *d_fcnt, const int *d_foff,
                        const double *d_fw, double *d_W);
+/* the adjoint of coffe_cov_integrals: d_gint -> d_gpk (zeroed here), workspaces
+   d_Wbar [2 nspec][npair][nrows][R] and d_dbar [nspec][2][N] */
+int ce_launch_cov_vjp(void *stream, int nspec, int N, const ce_radix_plan *rpN, const ce_radix_plan *rpM,
+                      const double *d_tw, const double *d_ahat, const double *d_B, int npair, const int *d_pl1,
+                      const int *d_pl2, int nrows, const int *d_rows, const double *d_prn1, int R, const int *d_flo,
+                      const int *d_fcnt, const int *d_foff, const double *d_fws, const int *d_upos,
+                      const int *d_uelo, const int *d_uehi,
+                      const double *d_fw, const double *d_fac, const double *d_gint, double *d_Wbar, double *d_dbar,
+                      const int64_t *d_in_off, const int32_t *d_n_in, const double *d_k, const double *d_pk,
+                      const double *d_kg, const double *d_k3, const double *d_kn1, const double *d_kn2, double step,
+                      double *d_gpk, int64_t n_total);
 int ce_launch_cov_cols(void *stream, int nspec, int npair, int nrows, int R, const int *d_upos, const int *d_ucnt,
                        const int *d_uoff, const double *d_uw, const double *d_fac, const double *d_W, double *d_out);
 int ce_launch_cov_assemble(void *stream, int nspec, int nz, int nl, int R, int npair, const int *d_ls,

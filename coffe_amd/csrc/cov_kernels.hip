@@ -277,6 +277,283 @@ int set_lds(const void *fn, size_t bytes)
     return e == hipSuccess ? 0 : (int)e;
 }
 
+// ---- adjoint: d<gint, integrals>/dP for every input sample ----------------------------
+// The integrals are linear in the diagonal d (per spectrum and power), so the adjoint
+// runs the chain backwards: the row functionals transposed (Wbar), per row the column
+// functionals transposed, the transposed c2r (an r2c, interior doubled), times conj(B),
+// the forward FFT_N and Re(conj(Ahat(i+t)) u_t) summed over rows into dbar; then dbar
+// through d = k^3 P^p / dlnk / k^nu1 / k^nu2 and the Akima spline onto the samples.
+
+// Wbar[blk][ir][n] = fac sum_m U_m(ir) gint[blk][m][n] over the row functionals whose
+// row window [upos_m, upos_m + cnt_m) holds ir; they lie in the range [mlo, mhi] that
+// binary searches over the windows' monotone envelopes give
+struct colsT_args {
+    int npair, nrows, R;
+    const int *upos, *ucnt, *uoff;
+    const int *elo, *ehi;  // min of upos over m' >= m, max of upos + ucnt over m' <= m
+    const double *uw, *fac;
+    const double *gint;    // [nspec][2][npair][R][R], (m, n) at R n + m
+    double *Wbar;          // [nspec][2][npair][nrows][R]
+};
+
+__global__ void __launch_bounds__(256) k_cov_cols_T(colsT_args a)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nrows * a.R) return;
+    const int ir = e / a.R, n = e % a.R;
+    const int pr = blockIdx.y, sp2 = blockIdx.z;
+    const int64_t blk = (int64_t)sp2 * a.npair + pr;
+    const double *g = a.gint + blk * a.R * a.R;
+    // the windows that hold row ir lie between the first m whose envelope end passes ir
+    // and the last m whose envelope start is at or before it (the envelopes are
+    // monotone in m whatever the order of the separations; membership checked below)
+    int lo = 0, hi = a.R;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a.ehi[mid] > ir) hi = mid; else lo = mid + 1;
+    }
+    const int mlo = lo;
+    lo = 0; hi = a.R;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a.elo[mid] <= ir) lo = mid + 1; else hi = mid;
+    }
+    const int mhi = lo - 1;
+    double acc = 0.0;
+    for (int m = mlo; m <= mhi; ++m) {
+        const int q = ir - a.upos[m];
+        if (q >= 0 && q < a.ucnt[m]) acc += a.uw[a.uoff[m] + q] * g[(int64_t)a.R * n + m];
+    }
+    a.Wbar[(blk * a.nrows + ir) * a.R + n] = acc * a.fac[(sp2 & 1) * a.npair + pr];
+}
+
+struct rowsT_args {
+    int N, M;
+    ce_radix_plan rpN, rpM;
+    const cd *tw, *ahat, *B;
+    const int *pl1, *pl2;
+    int npair, nrows, chunk;   // rows per workgroup
+    const int *rows;
+    const double *prn1;
+    int R;
+    const int *flo, *fcnt, *foff;
+    const double *fw;          // the scaled column weights (as the forward)
+    const double *Wbar;        // [nblk][npair][nrows][R]
+    double *dbar;              // [nblk][N], accumulated
+};
+
+#define CE_ROWS_T_ACC 8        // N <= 8 x 1024
+// one workgroup per (row chunk, pair, block): every row of the chunk goes through the
+// transposed transform in LDS, its contribution to dbar stays in registers (eight t per
+// thread) until the chunk ends
+__global__ void __launch_bounds__(1024) k_cov_rows_T(rowsT_args a)
+{
+    extern __shared__ cd smem[];
+    const int T = blockDim.x, tid = threadIdx.x;
+    const int c = blockIdx.x, pr = blockIdx.y, sp2 = blockIdx.z;
+    const int N = a.N, M = a.M;
+    gcd *tw = as_global(a.tw);
+    cd *ltN = smem + lds_data_len(N);
+    cd *ltM = ltN + 64 + ((N + 63) >> 6);
+    load_twiddles(ltN, N, tw, 1);
+    load_twiddles(ltM, M, tw, 2);
+    gcd *ah = as_global(a.ahat + (int64_t)a.pl1[pr] * N);
+    gcd *B = as_global(a.B + (int64_t)a.pl2[pr] * (M + 1));
+    gdouble *fw = as_global(a.fw);
+    double acc[CE_ROWS_T_ACC];
+#pragma unroll
+    for (int h = 0; h < CE_ROWS_T_ACC; ++h) acc[h] = 0.0;
+    const int r0 = c * a.chunk, r1 = min(r0 + a.chunk, a.nrows);
+    for (int ir = r0; ir < r1; ++ir) {
+        const int i = a.rows[ir];
+        gdouble *Wb = as_global(a.Wbar + (((int64_t)sp2 * a.npair + pr) * a.nrows + ir) * a.R);
+        const double den_i = a.prn1[i];
+        lds_barrier();   // the previous row's reads of smem are done
+        for (int q = tid; q < M + 1; q += T) smem[pidx(q)] = mk(0.0, 0.0);
+        lds_barrier();
+        // rbar(j) = sum_n Wbar_n fw_n(j) / den_i, four threads per functional
+        for (int e = tid; e < 4 * a.R; e += T) {
+            const int n = e >> 2, sub = e & 3;
+            const int j0 = a.flo[n], cnt = a.fcnt[n];
+            gdouble *w = fw + a.foff[n];
+            const double wn = Wb[n] / den_i;
+            for (int q = sub; q < cnt; q += 4) atomicAdd(&lds_real(smem, j0 + q), w[q] * wn);
+        }
+        lds_barrier();
+        fft_lds10(smem, M, a.rpM.code, ltM);
+        // Ybar = r2c(rbar) (interior doubled), Vbar = conj(B) Ybar on k <= M, 0 above
+        for (int k1 = tid; k1 <= M / 2; k1 += T) {
+            const int k2 = M - k1;
+            const cd Z1 = smem[pidx(k1)];
+            const cd Z2 = smem[pidx(k2 == M ? 0 : k2)];
+            const cd w1 = ldg(tw + k1), w2 = mk(-w1.x, w1.y);   // e^{-2 pi i (M - k)/N} = -conj(e^{-2 pi i k/N})
+            cd R1 = r2c_split(Z1, Z2, w1);
+            cd R2 = r2c_split(Z2, Z1, w2);
+            R1 = (k1 == 0) ? mk(R1.x, 0.0) : cscale(R1, 2.0);
+            R2 = (k2 == M) ? mk(R2.x, 0.0) : cscale(R2, 2.0);
+            const cd b1 = ldg(B + k1), b2 = ldg(B + k2);
+            // each pair (k, M - k) is read and rewritten by its own thread
+            smem[pidx(k1)] = cmul(cconj(b1), R1);
+            if (k2 != k1) smem[pidx(k2)] = cmul(cconj(b2), R2);
+        }
+        for (int q = M + 1 + tid; q < N; q += T) smem[pidx(q)] = mk(0.0, 0.0);
+        lds_barrier();
+        fft_lds10(smem, N, a.rpN.code, ltN);
+        // dbar_t += Re(conj(Ahat(i + t)) u_t)
+#pragma unroll
+        for (int h = 0; h < CE_ROWS_T_ACC; ++h) {
+            const int t = tid + h * T;
+            if (t < N) {
+                int q = i + t;
+                if (q >= N) q -= N;
+                const cd av = ldg(ah + q), u = smem[pidx(t)];
+                acc[h] += av.x * u.x + av.y * u.y;
+            }
+        }
+    }
+    double *db = a.dbar + (int64_t)sp2 * N;
+#pragma unroll
+    for (int h = 0; h < CE_ROWS_T_ACC; ++h) {
+        const int t = tid + h * T;
+        if (t < N && r1 > r0) atomicAdd(&db[t], acc[h]);
+    }
+}
+
+// value and derivatives with respect to six samples (forward-mode, for the Akima VJP)
+struct dual6 {
+    double v, d[6];
+};
+__device__ __forceinline__ dual6 dconst(double x)
+{
+    dual6 r;
+    r.v = x;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) r.d[q] = 0.0;
+    return r;
+}
+__device__ __forceinline__ dual6 dadd(const dual6 &a, const dual6 &b)
+{
+    dual6 r;
+    r.v = a.v + b.v;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) r.d[q] = a.d[q] + b.d[q];
+    return r;
+}
+__device__ __forceinline__ dual6 dsub(const dual6 &a, const dual6 &b)
+{
+    dual6 r;
+    r.v = a.v - b.v;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) r.d[q] = a.d[q] - b.d[q];
+    return r;
+}
+__device__ __forceinline__ dual6 dmul(const dual6 &a, const dual6 &b)
+{
+    dual6 r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) r.d[q] = a.d[q] * b.v + a.v * b.d[q];
+    return r;
+}
+__device__ __forceinline__ dual6 dscale(const dual6 &a, double s)
+{
+    dual6 r;
+    r.v = a.v * s;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) r.d[q] = a.d[q] * s;
+    return r;
+}
+__device__ __forceinline__ dual6 ddiv(const dual6 &a, const dual6 &b)
+{
+    dual6 r;
+    r.v = a.v / b.v;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) r.d[q] = (a.d[q] - r.v * b.d[q]) / b.v;
+    return r;
+}
+__device__ __forceinline__ dual6 dabs(const dual6 &a) { return a.v < 0 ? dscale(a, -1.0) : a; }
+
+// the Akima value at xq in interval j of (x, y) (ce::akima_at, GSL's non-rounded ends)
+// with its derivatives with respect to y[base .. base + 5] (every sample it reads)
+__device__ dual6 akima_dual(const double *__restrict__ x, const double *__restrict__ y, int n, int j, double xq,
+                            int base)
+{
+    auto Y = [&](int idx) {
+        dual6 r = dconst(y[idx]);
+        const int q = idx - base;
+        if (q >= 0 && q < 6) r.d[q] = 1.0;
+        return r;
+    };
+    auto sl = [&](int a0) { return dscale(dsub(Y(a0 + 1), Y(a0)), 1.0 / (x[a0 + 1] - x[a0])); };
+    auto slope = [&](int a0) -> dual6 {
+        if (a0 >= 0 && a0 <= n - 2) return sl(a0);
+        if (a0 < 0) {
+            const dual6 m0 = sl(0), m1 = sl(1);
+            return a0 == -1 ? dsub(dscale(m0, 2.0), m1) : dsub(dscale(m0, 3.0), dscale(m1, 2.0));
+        }
+        const dual6 mn2 = sl(n - 2), mn3 = sl(n - 3);
+        return a0 == n - 1 ? dsub(dscale(mn2, 2.0), mn3) : dsub(dscale(mn2, 3.0), dscale(mn3, 2.0));
+    };
+    const dual6 mm2 = slope(j - 2), mm1 = slope(j - 1), m0 = slope(j), mp1 = slope(j + 1), mp2 = slope(j + 2);
+    const dual6 ne = dadd(dabs(dsub(mp1, m0)), dabs(dsub(mm1, mm2)));
+    const double delx = xq - x[j];
+    if (ne.v == 0.0) return dadd(Y(j), dscale(m0, delx));
+    const double h = x[j + 1] - x[j];
+    const dual6 nen = dadd(dabs(dsub(mp2, mp1)), dabs(dsub(m0, mm1)));
+    const dual6 alpha = ddiv(dabs(dsub(mm1, mm2)), ne);
+    dual6 tn;
+    if (nen.v == 0.0) {
+        tn = m0;
+    } else {
+        const dual6 an = ddiv(dabs(dsub(m0, mm1)), nen);
+        tn = dadd(dmul(dsub(dconst(1.0), an), m0), dmul(an, mp1));
+    }
+    const dual6 b = dadd(dmul(dsub(dconst(1.0), alpha), mm1), dmul(alpha, m0));
+    const dual6 cc = dscale(dsub(dsub(dscale(m0, 3.0), dscale(b, 2.0)), tn), 1.0 / h);
+    const dual6 dd = dscale(dsub(dadd(b, tn), dscale(m0, 2.0)), 1.0 / (h * h));
+    // y_j + delx (b + delx (c + d delx))
+    return dadd(Y(j), dscale(dadd(b, dscale(dadd(cc, dscale(dd, delx)), delx)), delx));
+}
+
+struct diagT_args {
+    int N;
+    const int64_t *in_off;
+    const int32_t *n_in;
+    const double *k, *p, *kg, *k3, *kn1, *kn2;
+    double step;
+    const double *dbar;   // [nspec][2][N]
+    double *gpk;          // like p, accumulated
+};
+
+// one thread per grid point: Pbar = dbar_1 k^3/dlnk/k^nu1/k^nu2 + dbar_2 2P k^3/...,
+// then the Akima VJP onto the six samples of its stencil
+__global__ void __launch_bounds__(256) k_cov_diag_T(diagT_args a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (t >= a.N) return;
+    const int64_t off = a.in_off[s];
+    const int n = a.n_in[s];
+    const double *x = a.k + off, *y = a.p + off;
+    const double xq = a.kg[t];
+    if (!(xq >= x[0] && xq <= x[n - 1])) return;
+    const int j = ce::interval_bisect(x, 0, n - 1, xq);
+    const int base = max(0, min(j - 2, n - 6));
+    const dual6 P = akima_dual(x, y, n, j, xq, base);
+    const double sc = a.k3[t] / a.step / a.kn1[t] / a.kn2[t];
+    const double *db = a.dbar + (int64_t)s * 2 * a.N;
+    const double pbar = db[t] * sc + db[a.N + t] * 2.0 * P.v * sc;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        if (P.d[q] != 0.0) atomicAdd(&a.gpk[off + base + q], pbar * P.d[q]);
+}
+
+__global__ void k_fill_double(double *p, int64_t n, double v)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 }  // namespace
 
 extern "C" size_t ce_cov_lds_bytes(int N)
@@ -354,5 +631,49 @@ extern "C" int ce_launch_cov_assemble(void *stream, int nspec, int nz, int nl, i
     const int64_t per = (int64_t)nz * nl * R * nl * R;
     hipLaunchKernelGGL(k_cov_assemble, dim3((unsigned)((per + 255) / 256), nspec), dim3(256), 0,
                        (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ce_launch_cov_vjp(void *stream, int nspec, int N, const ce_radix_plan *rpN, const ce_radix_plan *rpM,
+                                 const double *d_tw, const double *d_ahat, const double *d_B, int npair,
+                                 const int *d_pl1, const int *d_pl2, int nrows, const int *d_rows,
+                                 const double *d_prn1, int R, const int *d_flo, const int *d_fcnt, const int *d_foff,
+                                 const double *d_fws, const int *d_upos, const int *d_uelo, const int *d_uehi,
+                                 const double *d_fw, const double *d_fac,
+                                 const double *d_gint, double *d_Wbar, double *d_dbar, const int64_t *d_in_off,
+                                 const int32_t *d_n_in, const double *d_k, const double *d_pk, const double *d_kg,
+                                 const double *d_k3, const double *d_kn1, const double *d_kn2, double step,
+                                 double *d_gpk, int64_t n_total)
+{
+    if (N > CE_ROWS_T_ACC * 1024) return (int)hipErrorInvalidValue;
+    hipStream_t s = (hipStream_t)stream;
+    colsT_args c;
+    c.npair = npair; c.nrows = nrows; c.R = R; c.upos = d_upos; c.ucnt = d_fcnt; c.uoff = d_foff; c.uw = d_fw;
+    c.elo = d_uelo; c.ehi = d_uehi;
+    c.fac = d_fac; c.gint = d_gint; c.Wbar = d_Wbar;
+    hipLaunchKernelGGL(k_cov_cols_T, dim3((nrows * R + 255) / 256, npair, 2 * nspec), dim3(256), 0, s, c);
+    const int64_t nd = (int64_t)2 * nspec * N;
+    hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, d_dbar, nd, 0.0);
+    hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((n_total + 255) / 256)), dim3(256), 0, s, d_gpk, n_total, 0.0);
+    const size_t lds = ce_cov_lds_bytes(N);
+    int st = set_lds((const void *)k_cov_rows_T, lds);
+    if (st) return st;
+    rowsT_args a;
+    a.N = N; a.M = N / 2; a.rpN = *rpN; a.rpM = *rpM;
+    a.tw = reinterpret_cast<const cd *>(d_tw);
+    a.ahat = reinterpret_cast<const cd *>(d_ahat);
+    a.B = reinterpret_cast<const cd *>(d_B);
+    a.pl1 = d_pl1; a.pl2 = d_pl2; a.npair = npair; a.nrows = nrows; a.rows = d_rows; a.prn1 = d_prn1;
+    a.R = R; a.flo = d_flo; a.fcnt = d_fcnt; a.foff = d_foff; a.fw = d_fws; a.Wbar = d_Wbar; a.dbar = d_dbar;
+    // about 2 x 256 workgroups in all: rows per workgroup
+    const int want = (512 + npair * 2 * nspec - 1) / (npair * 2 * nspec);
+    const int nchunk = want < 1 ? 1 : (want > nrows ? nrows : want);
+    a.chunk = (nrows + nchunk - 1) / nchunk;
+    const int grid_x = (nrows + a.chunk - 1) / a.chunk;
+    hipLaunchKernelGGL(k_cov_rows_T, dim3(grid_x, npair, 2 * nspec), dim3(1024), lds, s, a);
+    diagT_args d;
+    d.N = N; d.in_off = d_in_off; d.n_in = d_n_in; d.k = d_k; d.p = d_pk; d.kg = d_kg; d.k3 = d_k3;
+    d.kn1 = d_kn1; d.kn2 = d_kn2; d.step = step; d.dbar = d_dbar; d.gpk = d_gpk;
+    hipLaunchKernelGGL(k_cov_diag_T, dim3((N + 255) / 256, nspec), dim3(256), 0, s, d);
     return (int)hipGetLastError();
 }

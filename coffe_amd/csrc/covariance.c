@@ -257,11 +257,15 @@ struct coffe_cov_plan {
     double *d_tw, *d_A, *d_ahat, *d_B, *d_kg, *d_k3, *d_kn1, *d_kn2, *d_prn1, *d_prn2, *d_fac, *d_fw, *d_sep;
     double *d_fws;     /* the column weights times pi / (16 N^2 r_j^nu2) (the row transform's scale) */
     int *d_rows, *d_pl1, *d_pl2, *d_flo, *d_fcnt, *d_foff, *d_upos, *d_status, *d_ls, *d_pair_of;
+    int *d_uelo, *d_uehi;   /* monotone envelopes of the windows in row order (adjoint) */
     /* per call, grown on demand */
     int spec_cap;
     int64_t *d_in_off;
     int32_t *d_n_in;
     double *d_d, *d_W;
+    /* adjoint workspaces (coffe_cov_integrals_vjp), allocated on first use */
+    int adj_cap;
+    double *d_Wbar, *d_dbar;
 };
 
 void coffe_cov_plan_destroy(coffe_cov_plan *p)
@@ -271,11 +275,13 @@ void coffe_cov_plan_destroy(coffe_cov_plan *p)
     double *dd[] = {p->d_tw, p->d_A, p->d_ahat, p->d_B, p->d_kg, p->d_k3, p->d_kn1, p->d_kn2, p->d_prn1,
                     p->d_prn2, p->d_fac, p->d_fw, p->d_fws, p->d_sep, p->d_d, p->d_W};
     for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); ++i) hipFree(dd[i]);
-    int *di[] = {p->d_rows, p->d_pl1, p->d_pl2, p->d_flo, p->d_fcnt, p->d_foff, p->d_upos, p->d_status,
+    int *di[] = {p->d_rows, p->d_pl1, p->d_pl2, p->d_flo, p->d_fcnt, p->d_foff, p->d_upos, p->d_uelo, p->d_uehi, p->d_status,
                  p->d_ls, p->d_pair_of};
     for (size_t i = 0; i < sizeof(di) / sizeof(di[0]); ++i) hipFree(di[i]);
     hipFree(p->d_in_off);
     hipFree(p->d_n_in);
+    hipFree(p->d_Wbar);
+    hipFree(p->d_dbar);
     free(p);
 }
 
@@ -394,6 +400,20 @@ int coffe_cov_plan_create(coffe_engine *e, int N, double kmin, double kmax, int 
     if (st == COFFE_OK) st = upload((void **)&p->d_fcnt, f.cnt, sizeof(int) * (size_t)R, s);
     if (st == COFFE_OK) st = upload((void **)&p->d_foff, f.off, sizeof(int) * (size_t)R, s);
     if (st == COFFE_OK) st = upload((void **)&p->d_upos, upos, sizeof(int) * (size_t)R, s);
+    if (st == COFFE_OK) {
+        /* envelopes: start min over m' >= m, end max over m' <= m (both nondecreasing) */
+        int *env = (int *)malloc(sizeof(int) * 2 * (size_t)R);
+        if (!env) st = COFFE_ENOMEM;
+        for (int m = R - 1; m >= 0 && st == COFFE_OK; --m)
+            env[m] = (m == R - 1 || upos[m] < env[m + 1]) ? upos[m] : env[m + 1];
+        for (int m = 0; m < R && st == COFFE_OK; ++m) {
+            const int e = upos[m] + f.cnt[m];
+            env[R + m] = (m == 0 || e > env[R + m - 1]) ? e : env[R + m - 1];
+        }
+        if (st == COFFE_OK) st = upload((void **)&p->d_uelo, env, sizeof(int) * (size_t)R, s);
+        if (st == COFFE_OK) st = upload((void **)&p->d_uehi, env + R, sizeof(int) * (size_t)R, s);
+        free(env);
+    }
     if (st == COFFE_OK) st = upload((void **)&p->d_ls, ls, sizeof(int) * (size_t)nl, s);
     if (st == COFFE_OK) st = upload((void **)&p->d_pair_of, pair_of, sizeof(int) * (size_t)nl * (size_t)nl, s);
     if (st == COFFE_OK) st = ce_dev_alloc((void **)&p->d_status, sizeof(int));
@@ -462,6 +482,49 @@ int coffe_cov_integrals(coffe_cov_plan *p, int nspec, const int64_t *n_in, const
     if (st == COFFE_OK) st = ce_hip_ok(hipMemcpyAsync(&h_status, p->d_status, sizeof(int), hipMemcpyDeviceToHost, s));
     if (st == COFFE_OK) st = ce_hip_ok(hipStreamSynchronize(s));
     if (st == COFFE_OK && h_status) st = COFFE_EDOMAIN;   /* P^p not finite on [kmin, kmax] */
+    free(off); free(cnt);
+    return st;
+}
+
+/* ---- adjoint of the integrals ------------------------------------------------------- */
+
+int coffe_cov_integrals_vjp(coffe_cov_plan *p, int nspec, const int64_t *n_in, const double *d_k, const double *d_pk,
+                            const double *d_gint, double *d_gpk, void *stream)
+{
+    if (!p || nspec <= 0 || !n_in || !d_k || !d_pk || !d_gint || !d_gpk) return COFFE_EINVAL;
+    hipSetDevice(p->e->device);
+    hipStream_t s = stream ? (hipStream_t)stream : p->e->stream;
+    int st = COFFE_OK;
+    int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (size_t)nspec);
+    int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)nspec);
+    if (!off || !cnt) { free(off); free(cnt); return COFFE_ENOMEM; }
+    int64_t o = 0;
+    for (int i = 0; i < nspec; ++i) {
+        if (n_in[i] < 5 || n_in[i] > INT32_MAX) st = COFFE_EINVAL;
+        off[i] = o;
+        cnt[i] = (int32_t)n_in[i];
+        o += n_in[i];
+    }
+    if (st == COFFE_OK) st = grow(p, nspec);
+    if (st == COFFE_OK && nspec > p->adj_cap) {
+        hipFree(p->d_Wbar); hipFree(p->d_dbar);
+        p->d_Wbar = NULL; p->d_dbar = NULL; p->adj_cap = 0;
+        st = ce_dev_alloc((void **)&p->d_Wbar, sizeof(double) * 2 * (size_t)nspec * (size_t)p->npair *
+                                                   (size_t)p->nrows * (size_t)p->R);
+        if (st == COFFE_OK) st = ce_dev_alloc((void **)&p->d_dbar, sizeof(double) * 2 * (size_t)p->N * (size_t)nspec);
+        if (st == COFFE_OK) p->adj_cap = nspec;
+    }
+    if (st == COFFE_OK) st = ce_hip_ok(hipMemcpyAsync(p->d_in_off, off, sizeof(int64_t) * (size_t)nspec,
+                                                      hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK) st = ce_hip_ok(hipMemcpyAsync(p->d_n_in, cnt, sizeof(int32_t) * (size_t)nspec,
+                                                      hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK &&
+        ce_launch_cov_vjp(s, nspec, p->N, &p->rpN, &p->rpM, p->d_tw, p->d_ahat, p->d_B, p->npair, p->d_pl1, p->d_pl2,
+                          p->nrows, p->d_rows, p->d_prn1, p->R, p->d_flo, p->d_fcnt, p->d_foff, p->d_fws, p->d_upos,
+                          p->d_uelo, p->d_uehi, p->d_fw, p->d_fac, d_gint, p->d_Wbar, p->d_dbar, p->d_in_off, p->d_n_in, d_k, d_pk,
+                          p->d_kg, p->d_k3, p->d_kn1, p->d_kn2, p->step, d_gpk, o))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK) st = ce_hip_ok(hipStreamSynchronize(s));
     free(off); free(cnt);
     return st;
 }

@@ -61,6 +61,16 @@ int coffe_cov_integrals(coffe_cov_plan *p, int nspec, const int64_t *n_in, const
                         double *d_out, void *stream);
 
 /*
+ * Vector-Jacobian product of coffe_cov_integrals (no reference counterpart; the
+ * differentiation of the covariance with respect to P(k)): given the cotangent
+ * d_gint of d_out (same layout), d_gpk (same layout as d_pk) = d<gint, out>/dP for
+ * every input sample, through the Akima spline, the diagonal k^3 P^p / dlnk and the
+ * row transforms and functionals transposed.  Synchronises the stream.
+ */
+int coffe_cov_integrals_vjp(coffe_cov_plan *p, int nspec, const int64_t *n_in, const double *d_k, const double *d_pk,
+                            const double *d_gint, double *d_gpk, void *stream);
+
+/*
  * Gaussian covariance of the multipoles (covariance.c:796-1060), per z bin k:
  * galaxy biases bias[4 k .. 4 k + 3], growth rate f[k], D1[k], number density nbar[k],
  * volume vol[k], pixel size pix[k] (host arrays); poisson/mixed/cosmic select the
