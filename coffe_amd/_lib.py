@@ -86,6 +86,7 @@ def load_library():
     L.coffe_cov_plan_pairs.argtypes = [vp]
     L.coffe_cov_integrals.argtypes = [vp, i, i64p, vp, vp, vp, vp]
     L.coffe_cov_integrals_vjp.argtypes = [vp, i, i64p, vp, vp, vp, vp, vp]
+    L.coffe_cov_assemble_vjp.argtypes = [vp, i, i, dp, dp, dp, dp, dp, i, i, vp, vp, vp, dp, dp, vp]
     L.coffe_cov_assemble.argtypes = [vp, i, i, dp, dp, dp, dp, dp, dp, i, i, i, vp, vp, vp]
     L.two_sph_bessel_status.argtypes = [dp, dp, ctypes.POINTER(dp), ctypes.c_long, ctypes.c_long, vp, dp, dp,
                                         ctypes.POINTER(dp)]
@@ -112,7 +113,7 @@ EXPORTED = [
     "coffe_batch_resample", "coffe_batch_fft_prepare", "coffe_batch_adj_c2r", "coffe_batch_spline_adj",
     "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_terms_of", "coffe_integral_table", "coffe_divergent_tables", "coffe_divergent_tables_device", "coffe_batch_set_radix", "coffe_batch_small_r",
     "coffe_cov_plan_create", "coffe_cov_plan_destroy", "coffe_cov_plan_rows", "coffe_cov_plan_pairs",
-    "coffe_cov_integrals", "coffe_cov_integrals_vjp", "coffe_cov_assemble", "two_sph_bessel", "two_sph_bessel_status",
+    "coffe_cov_integrals", "coffe_cov_integrals_vjp", "coffe_cov_assemble", "coffe_cov_assemble_vjp", "two_sph_bessel", "two_sph_bessel_status",
 ]
 
 

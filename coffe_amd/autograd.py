@@ -1,4 +1,6 @@
-"""Differentiable multipoles: torch.autograd wrappers over the batched engine.
+"""Differentiable multipoles and covariance: torch.autograd wrappers over the batched
+engine and the 2D FFTLog covariance (CovarianceFunction: coffe_cov_assemble_vjp and
+coffe_cov_integrals_vjp, gradients for P, the biases and f of the z bins).
 
 The multipole of spectrum s at (z, l, r) is linear in the tables and in the weights,
     M[s, z, l, r] = sum_t A[z, l, t, r] I_t[s, r](P_s),
@@ -54,6 +56,52 @@ class MultipolesFunction(torch.autograd.Function):
 def multipoles(batch, k, pk, A):
     """Differentiable batched multipoles (nspec, nz, nl, nsep) of the concatenated spectra."""
     return MultipolesFunction.apply(pk, A, batch, k)
+
+
+class CovarianceFunction(torch.autograd.Function):
+    """Gaussian covariance cov = assemble(integrals(P); bias, f) with gradients for P
+    (coffe_cov_assemble_vjp, then coffe_cov_integrals_vjp), the biases and f (the
+    coefficient polynomials of the assembly, summed over the spectra)."""
+
+    @staticmethod
+    def forward(ctx, pk, bias, f, plan, k, n_in, zargs, flags):
+        bias_np = bias.detach().cpu().numpy()
+        f_np = f.detach().cpu().numpy()
+        integ = plan.integrals(k, pk, n_in)
+        cov = plan.assemble(integ, bias_np, f_np, **zargs, **flags)
+        ctx.save_for_backward(pk, integ)
+        ctx.plan, ctx.k, ctx.n_in, ctx.zargs, ctx.flags = plan, k, n_in, zargs, flags
+        ctx.bias_np, ctx.f_np, ctx.bias_shape = bias_np, f_np, bias.shape
+        return cov
+
+    @staticmethod
+    def backward(ctx, gout):
+        pk, integ = ctx.saved_tensors
+        z = ctx.zargs
+        ginteg, gbias, gf = ctx.plan.assemble_vjp(integ, gout.contiguous(), ctx.bias_np, ctx.f_np, z["D1"],
+                                                  z["nbar"], z["vol"], mixed=ctx.flags["mixed"],
+                                                  cosmic=ctx.flags["cosmic"])
+        gpk = gb = gfr = None
+        if ctx.needs_input_grad[0]:
+            gpk = ctx.plan.integrals_vjp(ctx.k, pk, ctx.n_in, ginteg)
+        if ctx.needs_input_grad[1]:
+            g = gbias.sum(axis=0)
+            if len(ctx.bias_shape) == 1:      # one population: the four biases are the one b
+                g = g.sum(axis=1)
+            gb = torch.tensor(g, dtype=pk.dtype)
+        if ctx.needs_input_grad[2]:
+            gfr = torch.tensor(gf.sum(axis=0), dtype=pk.dtype)
+        return gpk, gb, gfr, None, None, None, None, None
+
+
+def covariance(plan, k, pk, n_in, bias, f, D1, nbar, vol, pix, poisson=True, mixed=True, cosmic=True):
+    """Differentiable Gaussian covariance (nspec, nz, nl, R, nl, R) of the concatenated
+    spectra (k, pk on the GPU, n_in samples each); bias is a CPU tensor (nz, 4) or (nz,)
+    (one population), f a CPU tensor (nz,).  Gradients flow to pk (and through
+    rescale_sigma8 to sigma8), bias and f."""
+    zargs = dict(D1=D1, nbar=nbar, vol=vol, pix=pix)
+    flags = dict(poisson=poisson, mixed=mixed, cosmic=cosmic)
+    return CovarianceFunction.apply(pk, bias, f, plan, k, n_in, zargs, flags)
 
 
 def sigma8_of(k, pk, offsets, h):

@@ -120,3 +120,33 @@ class CovPlan:
                                         int(poisson), int(mixed), int(cosmic), _ptr(integ), _ptr(out), st),
               "coffe_cov_assemble")
         return out
+
+    def assemble_vjp(self, integ, gout, bias, f, D1, nbar, vol, mixed=True, cosmic=True, stream=None):
+        """Vector-Jacobian product of `assemble` for the cotangent gout (its output's shape):
+        (ginteg shaped like integ, gbias (nspec, nz, 4), gf (nspec, nz)), the gradients of
+        sum(gout * cov) per spectrum with respect to the integrals, the four biases and f of
+        every z bin (the Poisson term depends on none of them)."""
+        import torch
+        nspec = integ.shape[0]
+        _need(integ, (nspec, 2, self.npair, self.R, self.R), "integ")
+        f64 = lambda a: np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)  # noqa: E731
+        bias = np.asarray(bias, dtype=np.float64)
+        if bias.ndim == 1:
+            bias = np.repeat(bias[:, None], 4, axis=1)
+        bias = f64(bias)
+        nz = bias.shape[0]
+        arrs = [f64(a) for a in (f, D1, nbar, vol)]
+        if any(len(a) != nz for a in arrs):
+            raise ValueError("per-bin arrays must have one entry per z bin")
+        nl = len(self.ls)
+        _need(gout, (nspec, nz, nl, self.R, nl, self.R), "gout")
+        ginteg = torch.empty_like(integ)
+        gbias = np.zeros((nspec, nz, 4))
+        gf = np.zeros((nspec, nz))
+        dp = ctypes.POINTER(ctypes.c_double)
+        st = stream if stream is not None else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(self.L.coffe_cov_assemble_vjp(self.h, nspec, nz, bias.ctypes.data_as(dp),
+                                            *[a.ctypes.data_as(dp) for a in arrs], int(mixed), int(cosmic),
+                                            _ptr(integ), _ptr(gout), _ptr(ginteg), gbias.ctypes.data_as(dp),
+                                            gf.ctypes.data_as(dp), st), "coffe_cov_assemble_vjp")
+        return ginteg, gbias, gf

@@ -159,6 +159,10 @@ int ce_launch_cov_assemble(void *stream, int nspec, int nz, int nl, int R, int n
                            const int *d_pair_of, const double *d_sep, const double *d_cc, const double *d_cm,
                            const double *d_D1, const double *d_nbar, const double *d_vol, const double *d_pix,
                            int poisson, int mixed, int cosmic, const double *d_integ, double *d_out);
+int ce_launch_cov_assemble_vjp(void *stream, int nspec, int nz, int nl, int R, int npair, const int *d_ls,
+                               const int *d_pair_of, const double *d_cc, const double *d_cm, const double *d_D1,
+                               const double *d_vol, int mixed, int cosmic, const double *d_integ,
+                               const double *d_gout, double *d_ginteg, double *d_gcoef);
 
 /* ---- mu-quadrature weights (mu_kernels.hip) ---- */
 /* coef: workspace of 3 (6 nbg + 2 nb) doubles for the end-point Akima coefficients */

@@ -27,6 +27,7 @@
 //   k_cov_assemble  Gaussian covariance of the multipoles (covariance.c:796-1060)
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <mutex>
 
 #include "ce_device.h"
@@ -631,6 +632,108 @@ extern "C" int ce_launch_cov_assemble(void *stream, int nspec, int nz, int nl, i
     const int64_t per = (int64_t)nz * nl * R * nl * R;
     hipLaunchKernelGGL(k_cov_assemble, dim3((unsigned)((per + 255) / 256), nspec), dim3(256), 0,
                        (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+// ---- adjoint of the assembly ------------------------------------------------------
+// The covariance is linear in the stored integrals and in the per-(z, l1, l2)
+// coefficients cm (mixed) and cc (cosmic).  k_cov_assemble_T: one thread per stored
+// integral element (spectrum, pair (a, b), n_s, m_s) sums the cotangent over the z bins
+// and the blocks that read it, (a, b) at (m_s, n_s) and, for a < b, (b, a) at (n_s, m_s).
+__global__ void __launch_bounds__(256) k_cov_assemble_T(asm_args a, const double *__restrict__ gout,
+                                                        double *__restrict__ ginteg)
+{
+    const int RR = a.R * a.R;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)a.npair * RR) return;
+    const int s = blockIdx.y;
+    const int pr = (int)(e / RR);
+    const int idx = (int)(e % RR);
+    const int ns = idx / a.R, ms = idx % a.R;
+    int ia = 0, ib = 0;
+    for (int i = 0; i < a.nl; ++i)
+        for (int j = i; j < a.nl; ++j)
+            if (a.pair_of[i * a.nl + j] == pr) { ia = i; ib = j; }
+    const int64_t per = (int64_t)a.nz * a.nl * a.R * a.nl * a.R;
+    const double *g = gout + (int64_t)s * per;
+    double g1 = 0.0, g2 = 0.0;
+    for (int z = 0; z < a.nz; ++z) {
+        const double D = a.D1[z], V = a.vol[z];
+        for (int o = 0; o < ((ia < ib) ? 2 : 1); ++o) {
+            const int i = o ? ib : ia, j = o ? ia : ib;
+            const int m = o ? ns : ms, n = o ? ms : ns;
+            const int l1 = a.ls[i], l2 = a.ls[j];
+            const int sgn_d = (((l2 - l1) / 2) % 2 == 0) ? 1 : -1;
+            const int sgn_s = ((l1 + l2) % 2 == 0) ? 1 : -1;
+            const int zl = (z * a.nl + i) * a.nl + j;
+            const double go = g[((((int64_t)z * a.nl + i) * a.R + m) * a.nl + j) * a.R + n];
+            if (a.mixed) g1 += sgn_d * sgn_s * D * D * a.cm[zl] / V * go;
+            if (a.cosmic) g2 += sgn_d * D * D * D * D * a.cc[zl] / V * go;
+        }
+    }
+    double *gi = ginteg + (int64_t)s * 2 * a.npair * RR;
+    gi[(int64_t)pr * RR + idx] = g1;
+    gi[((int64_t)a.npair + pr) * RR + idx] = g2;
+}
+
+// k_cov_coef_T: one workgroup per (spectrum, z, l1, l2) block; gcoef[..][0] = d<gout,
+// cov>/d cm and gcoef[..][1] = d<gout, cov>/d cc of the block (tree sum in LDS)
+__global__ void __launch_bounds__(256) k_cov_coef_T(asm_args a, const double *__restrict__ gout,
+                                                    double *__restrict__ gcoef)
+{
+    __shared__ double red[2][256];
+    const int blk = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    const int j = blk % a.nl, i = (blk / a.nl) % a.nl, z = blk / (a.nl * a.nl);
+    const int l1 = a.ls[i], l2 = a.ls[j];
+    const int RR = a.R * a.R;
+    const int pr = a.pair_of[i * a.nl + j];
+    const double *base = a.integ + (int64_t)s * 2 * a.npair * RR;
+    const int64_t per = (int64_t)a.nz * a.nl * a.R * a.nl * a.R;
+    const double *g = gout + (int64_t)s * per;
+    double t1 = 0.0, t2 = 0.0;
+    for (int e = tid; e < RR; e += blockDim.x) {
+        const int m = e / a.R, n = e % a.R;
+        const int64_t idx = (i <= j) ? (int64_t)a.R * n + m : (int64_t)a.R * m + n;
+        const double go = g[((((int64_t)z * a.nl + i) * a.R + m) * a.nl + j) * a.R + n];
+        t1 += go * base[(int64_t)pr * RR + idx];
+        t2 += go * base[((int64_t)a.npair + pr) * RR + idx];
+    }
+    red[0][tid] = t1;
+    red[1][tid] = t2;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            red[0][tid] += red[0][tid + w];
+            red[1][tid] += red[1][tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double D = a.D1[z], V = a.vol[z];
+        const int sgn_d = (((l2 - l1) / 2) % 2 == 0) ? 1 : -1;
+        const int sgn_s = ((l1 + l2) % 2 == 0) ? 1 : -1;
+        double *o = gcoef + (((int64_t)s * a.nz + z) * a.nl * a.nl + i * a.nl + j) * 2;
+        o[0] = a.mixed ? sgn_d * sgn_s * D * D / V * red[0][0] : 0.0;
+        o[1] = a.cosmic ? sgn_d * D * D * D * D / V * red[1][0] : 0.0;
+    }
+}
+
+extern "C" int ce_launch_cov_assemble_vjp(void *stream, int nspec, int nz, int nl, int R, int npair, const int *d_ls,
+                                          const int *d_pair_of, const double *d_cc, const double *d_cm,
+                                          const double *d_D1, const double *d_vol, int mixed, int cosmic,
+                                          const double *d_integ, const double *d_gout, double *d_ginteg,
+                                          double *d_gcoef)
+{
+    asm_args a;
+    memset(&a, 0, sizeof(a));
+    a.nz = nz; a.nl = nl; a.R = R; a.npair = npair; a.ls = d_ls; a.pair_of = d_pair_of;
+    a.cc = d_cc; a.cm = d_cm; a.D1 = d_D1; a.vol = d_vol;
+    a.mixed = mixed; a.cosmic = cosmic; a.integ = d_integ;
+    const int64_t nel = (int64_t)npair * R * R;
+    hipLaunchKernelGGL(k_cov_assemble_T, dim3((unsigned)((nel + 255) / 256), nspec), dim3(256), 0,
+                       (hipStream_t)stream, a, d_gout, d_ginteg);
+    hipLaunchKernelGGL(k_cov_coef_T, dim3((unsigned)(nz * nl * nl), nspec), dim3(256), 0, (hipStream_t)stream, a,
+                       d_gout, d_gcoef);
     return (int)hipGetLastError();
 }
 

@@ -626,6 +626,132 @@ int coffe_cov_assemble(coffe_cov_plan *p, int nspec, int nz, const double *bias,
     return st;
 }
 
+/* ---- adjoint of the assembly ------------------------------------------------------- */
+
+/* a value and its derivatives by the four galaxy biases of a bin and f */
+typedef struct { double v, d[5]; } cv_dual;
+
+static cv_dual dl_var(double v, int slot)
+{
+    cv_dual r;
+    memset(&r, 0, sizeof(r));
+    r.v = v;
+    if (slot >= 0) r.d[slot] = 1.0;
+    return r;
+}
+
+static cv_dual dl_lin(double a, cv_dual x, double b, cv_dual y)   /* a x + b y */
+{
+    cv_dual r;
+    r.v = a * x.v + b * y.v;
+    for (int q = 0; q < 5; ++q) r.d[q] = a * x.d[q] + b * y.d[q];
+    return r;
+}
+
+static cv_dual dl_mul(cv_dual x, cv_dual y)
+{
+    cv_dual r;
+    r.v = x.v * y.v;
+    for (int q = 0; q < 5; ++q) r.d[q] = x.d[q] * y.v + x.v * y.d[q];
+    return r;
+}
+
+/* cov_coefficient with derivatives (covariance.c:40-57) */
+static cv_dual cov_coefficient_dual(cv_dual b1, cv_dual b2, cv_dual f, int ell)
+{
+    const cv_dual sum = dl_lin(1.0, b1, 1.0, b2);
+    const cv_dual ff = dl_mul(f, f);
+    switch (ell) {
+    case 0: return dl_lin(1.0, dl_lin(1.0, dl_mul(b1, b2), 1.0 / 3., dl_mul(sum, f)), 1.0 / 5., ff);
+    case 2: return dl_lin(2.0 / 3., dl_mul(sum, f), 4.0 / 7., ff);
+    case 4: return dl_lin(8.0 / 35., ff, 0.0, ff);
+    default: return dl_var(0.0, -1);
+    }
+}
+
+int coffe_cov_assemble_vjp(coffe_cov_plan *p, int nspec, int nz, const double *bias, const double *f, const double *D1,
+                           const double *nbar, const double *vol, int mixed, int cosmic, const double *d_integ,
+                           const double *d_gout, double *d_ginteg, double *out_gbias, double *out_gf, void *stream)
+{
+    if (!p || nspec <= 0 || nz <= 0 || !bias || !f || !D1 || !nbar || !vol || !d_integ || !d_gout || !d_ginteg)
+        return COFFE_EINVAL;
+    hipSetDevice(p->e->device);
+    hipStream_t s = stream ? (hipStream_t)stream : p->e->stream;
+    const int nl = p->nl;
+    const size_t nzl = (size_t)nz * (size_t)nl * (size_t)nl;
+    double *cc = (double *)malloc(sizeof(double) * nzl);
+    double *cm = (double *)malloc(sizeof(double) * nzl);
+    cv_dual *dc = (cv_dual *)malloc(sizeof(cv_dual) * 2 * nzl);
+    double *zbuf = (double *)malloc(sizeof(double) * 2 * (size_t)nz);
+    double *gcoef = (double *)malloc(sizeof(double) * 2 * nzl * (size_t)nspec);
+    if (!cc || !cm || !dc || !zbuf || !gcoef) {
+        free(cc); free(cm); free(dc); free(zbuf); free(gcoef);
+        return COFFE_ENOMEM;
+    }
+    for (int k = 0; k < nz; ++k) {
+        cv_dual gb[4];
+        for (int q = 0; q < 4; ++q) gb[q] = dl_var(bias[4 * (size_t)k + q], q);
+        const cv_dual gr = dl_var(f[k], 4);
+        for (int i = 0; i < nl; ++i)
+            for (int j = 0; j < nl; ++j) {
+                const int l1 = p->ls[i], l2 = p->ls[j];
+                cv_dual c = dl_var(0.0, -1), mm = dl_var(0.0, -1);
+                for (int a = 0; a <= 4; a += 2)
+                    for (int b = 0; b <= 4; b += 2) {
+                        const cv_dual t = dl_lin(1.0, dl_mul(cov_coefficient_dual(gb[0], gb[2], gr, a),
+                                                             cov_coefficient_dual(gb[1], gb[3], gr, b)),
+                                                 (double)sgnp(b), dl_mul(cov_coefficient_dual(gb[0], gb[3], gr, a),
+                                                                         cov_coefficient_dual(gb[1], gb[2], gr, b)));
+                        c = dl_lin(1.0, c, legendre_integral(l1, l2, a, b), t);
+                    }
+                for (int a = 0; a <= 4; a += 2) {
+                    cv_dual t = dl_lin(1.0, cov_coefficient_dual(gb[0], gb[2], gr, a), 1.0,
+                                       cov_coefficient_dual(gb[1], gb[3], gr, a));
+                    t = dl_lin(1.0, t, 1.0, cov_coefficient_dual(gb[0], gb[3], gr, a));
+                    t = dl_lin(1.0, t, 1.0, cov_coefficient_dual(gb[1], gb[2], gr, a));
+                    mm = dl_lin(1.0, mm, legendre_integral(l1, l2, a, 0) / nbar[k], t);
+                }
+                const size_t zl = ((size_t)k * nl + i) * nl + j;
+                c = dl_lin(0.25, c, 0.0, c);
+                mm = dl_lin(0.125, mm, 0.0, mm);
+                cc[zl] = c.v;
+                cm[zl] = mm.v;
+                dc[2 * zl] = mm;
+                dc[2 * zl + 1] = c;
+            }
+        zbuf[k] = D1[k];
+        zbuf[nz + k] = vol[k];
+    }
+    double *d_cc = NULL, *d_cm = NULL, *d_z = NULL, *d_gc = NULL;
+    int st = upload((void **)&d_cc, cc, sizeof(double) * nzl, s);
+    if (st == COFFE_OK) st = upload((void **)&d_cm, cm, sizeof(double) * nzl, s);
+    if (st == COFFE_OK) st = upload((void **)&d_z, zbuf, sizeof(double) * 2 * (size_t)nz, s);
+    if (st == COFFE_OK) st = ce_dev_alloc((void **)&d_gc, sizeof(double) * 2 * nzl * (size_t)nspec);
+    if (st == COFFE_OK &&
+        ce_launch_cov_assemble_vjp(s, nspec, nz, nl, p->R, p->npair, p->d_ls, p->d_pair_of, d_cc, d_cm, d_z, d_z + nz,
+                                   mixed, cosmic, d_integ, d_gout, d_ginteg, d_gc))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK)
+        st = ce_hip_ok(hipMemcpyAsync(gcoef, d_gc, sizeof(double) * 2 * nzl * (size_t)nspec, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK) st = ce_hip_ok(hipStreamSynchronize(s));
+    if (st == COFFE_OK)
+        for (int sp = 0; sp < nspec; ++sp)
+            for (int k = 0; k < nz; ++k) {
+                double g[5] = {0, 0, 0, 0, 0};
+                for (size_t e = 0; e < (size_t)nl * nl; ++e) {
+                    const size_t zl = (size_t)k * nl * nl + e;
+                    const double *gc = gcoef + 2 * ((size_t)sp * nzl + zl);
+                    for (int q = 0; q < 5; ++q) g[q] += gc[0] * dc[2 * zl].d[q] + gc[1] * dc[2 * zl + 1].d[q];
+                }
+                if (out_gbias)
+                    for (int q = 0; q < 4; ++q) out_gbias[((size_t)sp * nz + k) * 4 + q] = g[q];
+                if (out_gf) out_gf[(size_t)sp * nz + k] = g[4];
+            }
+    hipFree(d_cc); hipFree(d_cm); hipFree(d_z); hipFree(d_gc);
+    free(cc); free(cm); free(dc); free(zbuf); free(gcoef);
+    return st;
+}
+
 /* ---- drop-in two_sph_bessel ------------------------------------------------------ */
 
 int two_sph_bessel_status(const double *k1, const double *k2, double **fk1k2, long N1, long N2, const config *cfg,

@@ -80,6 +80,18 @@ int coffe_cov_assemble(coffe_cov_plan *p, int nspec, int nz, const double *bias,
                        const double *nbar, const double *vol, const double *pix, int poisson, int mixed, int cosmic,
                        const double *d_integ, double *d_out, void *stream);
 
+/*
+ * Vector-Jacobian product of coffe_cov_assemble (no reference counterpart; the
+ * differentiation of the covariance): given the cotangent d_gout of d_out, writes
+ * d_ginteg (layout of d_integ) = d<gout, cov>/d integrals, and on the host
+ * out_gbias[s][z][4] and out_gf[s][z] = d<gout_s, cov_s>/d bias and /d f of every z bin,
+ * per spectrum (either may be NULL).  The Poisson term depends on neither.  Same
+ * per-bin arguments as coffe_cov_assemble; synchronises the stream.
+ */
+int coffe_cov_assemble_vjp(coffe_cov_plan *p, int nspec, int nz, const double *bias, const double *f, const double *D1,
+                           const double *nbar, const double *vol, int mixed, int cosmic, const double *d_integ,
+                           const double *d_gout, double *d_ginteg, double *out_gbias, double *out_gf, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
