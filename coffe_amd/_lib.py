@@ -109,7 +109,7 @@ EXPORTED = [
     "coffe_engine_default", "coffe_engine_synchronize", "twofast_1bessel", "twofast_1bessel_status",
     "coffe_batch_create", "coffe_batch_destroy", "coffe_batch_input_size", "coffe_batch_forward",
     "coffe_batch_vjp", "coffe_batch_tables", "coffe_batch_prepare", "coffe_batch_tables_stage",
-    "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute",
+    "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute", "coffe_background_jvp",
     "coffe_batch_resample", "coffe_batch_fft_prepare", "coffe_batch_adj_c2r", "coffe_batch_spline_adj",
     "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_terms_of", "coffe_integral_table", "coffe_divergent_tables", "coffe_divergent_tables_device", "coffe_batch_set_radix", "coffe_batch_small_r",
     "coffe_cov_plan_create", "coffe_cov_plan_destroy", "coffe_cov_plan_rows", "coffe_cov_plan_pairs",
@@ -121,4 +121,5 @@ def declare_background(L):
     dp = ctypes.POINTER(ctypes.c_double)
     d, i = ctypes.c_double, ctypes.c_int
     L.coffe_background_compute.argtypes = [d, d, d, d, d, d, i, i, dp, dp, dp, dp, dp] + [dp] * 10
+    L.coffe_background_jvp.argtypes = [dp, dp, i, i, dp, dp, dp, dp, dp, dp, dp]
     return L
