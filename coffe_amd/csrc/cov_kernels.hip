@@ -297,35 +297,65 @@ struct colsT_args {
     double *Wbar;          // [nspec][2][npair][nrows][R]
 };
 
+#define CE_COLS_T_RT 32     // rows per workgroup
+#define CE_COLS_T_MC 16     // functionals per LDS tile
+// one workgroup per (32 rows, pair, block); thread = column n (chunks of 256): a tile of
+// G[n][m0 .. m0 + 16) is staged in LDS (rows padded to 17 doubles), the weights U_m(row)
+// are wave-uniform loads, the 32 row sums stay in registers; m ascends as in a
+// per-element loop, so the sums are the same
 __global__ void __launch_bounds__(256) k_cov_cols_T(colsT_args a)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.nrows * a.R) return;
-    const int ir = e / a.R, n = e % a.R;
+    __shared__ double gL[256 * (CE_COLS_T_MC + 1)];
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * CE_COLS_T_RT;
+    const int r1 = min(r0 + CE_COLS_T_RT, a.nrows) - 1;
     const int pr = blockIdx.y, sp2 = blockIdx.z;
     const int64_t blk = (int64_t)sp2 * a.npair + pr;
     const double *g = a.gint + blk * a.R * a.R;
-    // the windows that hold row ir lie between the first m whose envelope end passes ir
-    // and the last m whose envelope start is at or before it (the envelopes are
-    // monotone in m whatever the order of the separations; membership checked below)
+    // functionals whose envelope reaches rows r0..r1
     int lo = 0, hi = a.R;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (a.ehi[mid] > ir) hi = mid; else lo = mid + 1;
+        if (a.ehi[mid] > r0) hi = mid; else lo = mid + 1;
     }
     const int mlo = lo;
     lo = 0; hi = a.R;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (a.elo[mid] <= ir) lo = mid + 1; else hi = mid;
+        if (a.elo[mid] <= r1) lo = mid + 1; else hi = mid;
     }
     const int mhi = lo - 1;
-    double acc = 0.0;
-    for (int m = mlo; m <= mhi; ++m) {
-        const int q = ir - a.upos[m];
-        if (q >= 0 && q < a.ucnt[m]) acc += a.uw[a.uoff[m] + q] * g[(int64_t)a.R * n + m];
+    const double fac = a.fac[(sp2 & 1) * a.npair + pr];
+    for (int nc = 0; nc < a.R; nc += 256) {
+        double acc[CE_COLS_T_RT];
+#pragma unroll
+        for (int r = 0; r < CE_COLS_T_RT; ++r) acc[r] = 0.0;
+        for (int m0 = mlo; m0 <= mhi; m0 += CE_COLS_T_MC) {
+            const int mcn = min(CE_COLS_T_MC, mhi - m0 + 1);
+            __syncthreads();
+            for (int e = tid; e < 256 * CE_COLS_T_MC; e += 256) {
+                const int nn = e / CE_COLS_T_MC, mc = e % CE_COLS_T_MC;
+                const int ng = nc + nn;
+                gL[nn * (CE_COLS_T_MC + 1) + mc] = (ng < a.R && mc < mcn) ? g[(int64_t)a.R * ng + m0 + mc] : 0.0;
+            }
+            __syncthreads();
+            for (int mc = 0; mc < mcn; ++mc) {
+                const int m = m0 + mc;
+                const int up = a.upos[m], cn = a.ucnt[m], off = a.uoff[m];
+                const double gv = gL[tid * (CE_COLS_T_MC + 1) + mc];
+#pragma unroll
+                for (int r = 0; r < CE_COLS_T_RT; ++r) {
+                    const int q = r0 + r - up;
+                    if (q >= 0 && q < cn) acc[r] += a.uw[off + q] * gv;
+                }
+            }
+        }
+        const int n = nc + tid;
+        if (n < a.R)
+#pragma unroll
+            for (int r = 0; r < CE_COLS_T_RT; ++r)
+                if (r0 + r < a.nrows) a.Wbar[(blk * a.nrows + r0 + r) * a.R + n] = acc[r] * fac;
     }
-    a.Wbar[(blk * a.nrows + ir) * a.R + n] = acc * a.fac[(sp2 & 1) * a.npair + pr];
 }
 
 struct rowsT_args {
@@ -754,7 +784,8 @@ extern "C" int ce_launch_cov_vjp(void *stream, int nspec, int N, const ce_radix_
     c.npair = npair; c.nrows = nrows; c.R = R; c.upos = d_upos; c.ucnt = d_fcnt; c.uoff = d_foff; c.uw = d_fw;
     c.elo = d_uelo; c.ehi = d_uehi;
     c.fac = d_fac; c.gint = d_gint; c.Wbar = d_Wbar;
-    hipLaunchKernelGGL(k_cov_cols_T, dim3((nrows * R + 255) / 256, npair, 2 * nspec), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(k_cov_cols_T, dim3((nrows + CE_COLS_T_RT - 1) / CE_COLS_T_RT, npair, 2 * nspec), dim3(256), 0,
+                       s, c);
     const int64_t nd = (int64_t)2 * nspec * N;
     hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s, d_dbar, nd, 0.0);
     hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((n_total + 255) / 256)), dim3(256), 0, s, d_gpk, n_total, 0.0);
