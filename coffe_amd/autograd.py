@@ -125,6 +125,63 @@ def rescale_sigma8(k, pk, offsets, h, sigma8):
     return pk * torch.repeat_interleave(scale.expand(len(counts)), counts)
 
 
+class GrowthAt(torch.autograd.Function):
+    """(D1(z), f(z)) at the mean redshifts as functions of the six background parameters
+    theta = (Omega_cdm, Omega_baryon, Omega_gamma, w0, wa, h) (a CPU tensor): values from
+    coffe_background_compute, the Jacobian from coffe_background_jvp (one direction per
+    parameter); both are read off the tables by Akima at z (the tangent as the Akima of
+    the tangent table, which on the smooth 10000-bin tables differs from the derivative
+    of the interpolant at the interpolation-error level)."""
+
+    @staticmethod
+    def forward(ctx, theta, zmean, bins):
+        from . import background as B
+        from .engine import akima
+        th = theta.detach().cpu().numpy().astype(np.float64)
+        kw = dict(zip(B.PARAMS, th.tolist()), bins=bins)
+        zq = np.asarray(zmean, dtype=np.float64)
+        jac = np.zeros((len(zq), 2, len(B.PARAMS)))
+        tab = None
+        for q, name in enumerate(B.PARAMS):
+            tab, der = B.jvp({name: 1.0}, **kw)
+            jac[:, 0, q] = akima(tab["z"], der["D1"], zq)
+            jac[:, 1, q] = akima(tab["z"], der["f"], zq)
+        out = np.stack([akima(tab["z"], tab["D1"], zq), akima(tab["z"], tab["f"], zq)], axis=1)
+        ctx.jac = torch.tensor(jac, dtype=theta.dtype)
+        return torch.tensor(out, dtype=theta.dtype)
+
+    @staticmethod
+    def backward(ctx, gout):
+        return torch.einsum("zc,zcq->q", gout.to(ctx.jac.dtype).cpu(), ctx.jac), None, None
+
+
+def growth_at(theta, zmean, bins=10000):
+    """Differentiable (nz, 2) tensor [D1(z), f(z)] at the mean redshifts; theta is a CPU
+    tensor (Omega_cdm, Omega_baryon, Omega_gamma, w0, wa, h)."""
+    return GrowthAt.apply(theta, list(zmean), bins)
+
+
+def flat_weights_of_background(lw, theta, theta0, zmean, b, bins=10000):
+    """Flat-sky density + RSD weights with the growth taken from the background at theta:
+    A(theta) = (D1(z; theta) / D1(z; theta0))^2 A(b, f(z; theta)), where lw are the
+    f-override flat-sky LocalWeights built on the background at theta0 (flat sky, the
+    local terms depend on the background only through D1(z)^2 and f(z), functions.c:
+    1457-1460).  Differentiable in theta (self-consistent d/dOmega, d/dw, ...) and b."""
+    if not lw.f_override:
+        raise ValueError("the weights must be built with f_override=True")
+    g = growth_at(theta, zmean, bins)
+    g0 = growth_at(theta0.detach(), zmean, bins)
+    dev = lw.groups[next(iter(lw.groups))].device
+    scale = ((g[:, 0] / g0[:, 0]) ** 2).to(dev)
+    f = g[:, 1].to(dev)
+    A = None
+    for name, coef in (("bb", lambda: b * b * torch.ones_like(f)), ("bf", lambda: b * f), ("ff", lambda: f * f)):
+        if name in lw.groups:
+            term = (coef() * scale)[:, None, None, None] * lw.groups[name]
+            A = term if A is None else A + term
+    return A
+
+
 class LocalWeights:
     """The three weight groups (b^2, b f, f^2) of the local terms for one geometry.
 
