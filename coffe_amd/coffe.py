@@ -17,7 +17,7 @@ coffe_amd.engine.divergent_tables.
 import numpy as np
 
 from . import background as _background
-from .representation import Covariance, Multipoles
+from .representation import AverageMultipoles, Covariance, Multipoles
 
 TABLES = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]   # integrals.c:599-608
 _UNSUPPORTED = ("has_lensing", "has_d2", "has_g1", "has_g2", "has_g3", "has_g4", "has_g5")
@@ -65,6 +65,7 @@ class Coffe:
     _defaults = dict(
         omega_cdm=0.25, omega_baryon=0.05, omega_gamma=9e-5, h=0.67, w0=-1.0, wa=0.0,
         sep=5.0 + 3.0 * np.arange(100), l=[0, 2, 4], z_mean=[1.0], deltaz=[0.2], fsky=[1.0], pixelsize=[10.0],
+        z_min=[0.9], z_max=[1.1],
         number_density1=[1e-3], number_density2=[1e-3],
         has_density=True, has_rsd=False, has_d1=False, has_flatsky_local=False,
         integrals_fftlog_sampling=10000, covariance_integration_sampling=8000,
@@ -88,10 +89,12 @@ class Coffe:
         self._tables = None
         self._divergent = None
         self._multipoles = None
+        self._average_multipoles = None
         self._covariance = None
 
     def _free_results(self):
         self._multipoles = None
+        self._average_multipoles = None
         self._covariance = None
 
     def _engine_handle(self):
@@ -181,6 +184,10 @@ class Coffe:
                    lambda s, v: s._set_list("sep", v, lambda x: 0 < x < 10000, s._free_results))
     z_mean = property(lambda s: s._p["z_mean"].copy(),
                       lambda s, v: s._set_list("z_mean", v, lambda x: 0 <= x < 10, s._free_results))
+    z_min = property(lambda s: s._p["z_min"].copy(),
+                     lambda s, v: s._set_list("z_min", v, lambda x: 0 <= x < 10, s._free_results))
+    z_max = property(lambda s: s._p["z_max"].copy(),
+                     lambda s, v: s._set_list("z_max", v, lambda x: 0 <= x < 10, s._free_results))
     l = property(lambda s: s._p["l"].copy(),  # noqa: E741
                  lambda s, v: s._set_list("l", v, lambda x: 0 <= x < 10, s._free_results, as_int=True))
     deltaz = property(lambda s: s._p["deltaz"].copy(),
@@ -431,6 +438,72 @@ class Coffe:
                                          for iz, z in enumerate(zs) for il, li in enumerate(ls)
                                          for ir, r in enumerate(seps)])
         return self._multipoles
+
+    # ---- redshift-averaged multipoles ----------------------------------------------
+    def _check_coords_average_multipoles(self):
+        zmin, zmax = self._p["z_min"], self._p["z_max"]
+        if not (len(self._p["l"]) and len(self._p["sep"]) and len(zmin)):
+            raise ValueError("At least one of the following parameters is empty: l, sep, z_min")
+        if len(zmin) != len(zmax):
+            raise ValueError("z_min and z_max must have the same length")
+        if np.any(np.asarray(zmin) >= np.asarray(zmax)):
+            raise ValueError("every z_min must be below its z_max")
+
+    def _average_values(self, zmin, zmax, seps, ls, nodes=32):
+        """[l][r] multipoles averaged over the redshift bin (signal.c:249-311, 543-550): for
+        each separation r, z runs over [z1, z2] with z1 = z(chi(zmin) + r/2) and
+        z2 = z(chi(zmax) - r/2), and the average is int_0^1 dv M_l(z(v), r) h H0 / H(z(v)),
+        z(v) = z1 + (z2 - z1) v (the reference's integrand, no further normalisation); the
+        v integral is a Gauss-Legendre rule of `nodes` points, the M_l(z, r) the engine's
+        weights (GPU mu quadrature) at the (z, r) node pairs times the tables."""
+        from .engine import akima as _ak
+        from .engine import mu_weights
+        bg = self._background()
+        z_of_chi = _Spline(bg["chi"], bg["z"])
+        chi_of_z = _Spline(bg["z"], bg["chi"])
+        hz = _Spline(bg["z"], bg["Hz"])
+        x, w = np.polynomial.legendre.leggauss(nodes)
+        v, w = 0.5 * (x + 1), 0.5 * w
+        seps = np.asarray(seps, dtype=np.float64)
+        z1 = z_of_chi(chi_of_z(float(zmin))[0] + seps / 2)
+        z2 = z_of_chi(chi_of_z(float(zmax))[0] - seps / 2)
+        if np.any(z2 <= z1):
+            raise ValueError("a separation exceeds the comoving depth of the redshift bin")
+        zn = z1[:, None] + (z2 - z1)[:, None] * v[None, :]                  # [r][v]
+        hH0 = self._p["h"] / 2997.92458
+        wz = w[None, :] * hH0 / hz(zn.ravel()).reshape(zn.shape)           # [r][v]
+        tabs = self._integral_tables()
+        bz = self._bias[1].x
+        if not np.array_equal(bz, self._bias[2].x):
+            raise ValueError("galaxy_bias1 and galaxy_bias2 must share their redshift sampling")
+        A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, zn.ravel(),
+                       list(ls), TABLES, seps, den=self._p["has_density"], rsd=self._p["has_rsd"],
+                       d1=self._p["has_d1"], flatsky=self._p["has_flatsky_local"],
+                       nmu=self._p["multipole_quadrature_nodes"]).cpu().numpy()   # [r*v][l][t][r']
+        nr = len(seps)
+        A = A.reshape(nr, nodes, len(ls), len(TABLES), nr)
+        Ad = A[np.arange(nr), :, :, :, np.arange(nr)]                          # [r][v][l][t]
+        Aavg = np.einsum("rv,rvlt->ltr", wz, Ad)
+        I = np.array([_ak(tabs[t][0], tabs[t][1], seps) for t in TABLES])
+        return np.einsum("ltr,tr->lr", Aavg, I)
+
+    def compute_average_multipoles_bulk(self, recompute=False):
+        """Redshift-averaged multipoles at every (z_min/z_max bin, l, sep), bin-major then
+        l then r (average_multipoles.c:90-146)."""
+        self._check_coords_average_multipoles()
+        self._check_contributions()
+        if recompute:
+            self._free_all()
+        if self._average_multipoles is None:
+            ls, seps = self._p["l"], self._p["sep"]
+            out = []
+            for zmin, zmax in zip(self._p["z_min"], self._p["z_max"]):
+                vals = self._average_values(zmin, zmax, seps, ls)
+                out += [AverageMultipoles(z_min=float(zmin), z_max=float(zmax), r=float(r), l=int(li),
+                                          value=float(vals[il, ir]))
+                        for il, li in enumerate(ls) for ir, r in enumerate(seps)]
+            self._average_multipoles = np.array(out)
+        return self._average_multipoles
 
     # ---- covariance ---------------------------------------------------------------
     def compute_covariance_bulk(self, recompute=False):

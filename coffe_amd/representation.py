@@ -36,6 +36,12 @@ class Multipoles(Representation):
     z, r, l, value = _ro("z"), _ro("r"), _ro("l"), _ro("value")
 
 
+class AverageMultipoles(Representation):
+    """One redshift-averaged multipole point: bin [z_min, z_max], separation r (Mpc), l."""
+    _fields = ("z_min", "z_max", "r", "l", "value")
+    z_min, z_max, r, l, value = _ro("z_min"), _ro("z_max"), _ro("r"), _ro("l"), _ro("value")
+
+
 class Covariance(Representation):
     """One covariance entry between (l1, r1) and (l2, r2) at mean redshift z."""
     _fields = ("r1", "r2", "l1", "l2", "z", "value")

@@ -97,3 +97,17 @@ def test_records():
     assert c.l2 == 2 and c.to_dict()["value"] == 3.0
     with pytest.raises(TypeError):
         Multipoles(z=1.0, r=1.0, l=0)
+
+
+def test_average_multipoles_coordinates_checked():
+    from coffe_amd.coffe import Coffe
+    c = Coffe()
+    c.z_min = [0.9, 1.2]
+    c.z_max = [1.1]
+    with pytest.raises(ValueError):
+        c.compute_average_multipoles_bulk()
+    c.z_max = [1.1, 1.1]
+    with pytest.raises(ValueError):
+        c.compute_average_multipoles_bulk()
+    with pytest.raises(ValueError):
+        c.z_min = [12.0]

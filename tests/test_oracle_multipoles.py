@@ -77,3 +77,31 @@ def test_gauss_legendre_256_matches_qag_fullsky(setup):
         assert np.max(rel[..., sep * H >= 60]) < 1e-8
         assert np.max(rel) < 5e-5
         assert np.max(np.abs(q - ref5) / np.abs(q)) < 5e-4
+
+
+def test_average_multipoles_vs_reference_fixture(setup):
+    """Redshift-averaged density + RSD multipoles, bin [0.9, 1.1] (test_coffe.py:305-326):
+    the oracle's full-sky multipoles at mean redshifts z(v) = z1 + (z2 - z1) v,
+    z1 = z(chi(0.9) + r/2), z2 = z(chi(1.1) - r/2), weighted by h H0 / H(z) and summed
+    with a 32-point Gauss-Legendre rule in v (signal.c:249-311, 543-550) against
+    benchmark_avg_multipoles{0,2,4}.dat (rtol 5e-4 as the reference's test; 5e-5 here)."""
+    k, pk, bg, bgo, tabs = setup
+    sep = np.array([10, 20, 40, 100, 150]) / H
+    zb, b = np.arange(11.0), np.ones(11)
+    x, w = np.polynomial.legendre.leggauss(32)
+    v, w = 0.5 * (x + 1), 0.5 * w
+    chi_at = lambda z: O.akima(bg["z"], bg["chi"], np.atleast_1d(z))   # noqa: E731
+    z_at = lambda c: O.akima(bg["chi"], bg["z"], np.atleast_1d(c))     # noqa: E731
+    got = np.zeros((3, len(sep)))
+    for i, r in enumerate(sep):
+        z1 = z_at(chi_at(0.9)[0] + r / 2)[0]
+        z2 = z_at(chi_at(1.1)[0] - r / 2)[0]
+        zs = z1 + (z2 - z1) * v
+        mp = O.multipoles(tabs, bgo, zb, b, b, zs, [0, 2, 4], [r], flatsky=False, den=1, rsd=1)[:, :, 0]
+        wz = w * (H / 2997.92458) / O.akima(bg["z"], bg["Hz"], zs)
+        got[:, i] = wz @ mp
+    for il, l in enumerate([0, 2, 4]):
+        x_, y = np.loadtxt(os.path.join(GOLD, f"benchmark_avg_multipoles{l}.dat")).T
+        assert np.allclose(sep * H, x_)
+        assert np.allclose(got[il], y, rtol=5e-4)
+        assert np.max(np.abs(got[il] / y - 1)) < 5e-5, (l, got[il] / y - 1)
