@@ -449,13 +449,15 @@ class Coffe:
         if np.any(np.asarray(zmin) >= np.asarray(zmax)):
             raise ValueError("every z_min must be below its z_max")
 
-    def _average_values(self, zmin, zmax, seps, ls, nodes=32):
+    def _average_values(self, zmin, zmax, seps, ls, nodes=32, chunk=8):
         """[l][r] multipoles averaged over the redshift bin (signal.c:249-311, 543-550): for
         each separation r, z runs over [z1, z2] with z1 = z(chi(zmin) + r/2) and
         z2 = z(chi(zmax) - r/2), and the average is int_0^1 dv M_l(z(v), r) h H0 / H(z(v)),
         z(v) = z1 + (z2 - z1) v (the reference's integrand, no further normalisation); the
         v integral is a Gauss-Legendre rule of `nodes` points, the M_l(z, r) the engine's
-        weights (GPU mu quadrature) at the (z, r) node pairs times the tables."""
+        weights (GPU mu quadrature) at the (z, r) node pairs times the tables.  The weights
+        are computed for `chunk` separations at a time (all node redshifts of the chunk
+        times its separations, of which the matching pairs are kept)."""
         from .engine import akima as _ak
         from .engine import mu_weights
         bg = self._background()
@@ -476,14 +478,17 @@ class Coffe:
         bz = self._bias[1].x
         if not np.array_equal(bz, self._bias[2].x):
             raise ValueError("galaxy_bias1 and galaxy_bias2 must share their redshift sampling")
-        A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, zn.ravel(),
-                       list(ls), TABLES, seps, den=self._p["has_density"], rsd=self._p["has_rsd"],
-                       d1=self._p["has_d1"], flatsky=self._p["has_flatsky_local"],
-                       nmu=self._p["multipole_quadrature_nodes"]).cpu().numpy()   # [r*v][l][t][r']
-        nr = len(seps)
-        A = A.reshape(nr, nodes, len(ls), len(TABLES), nr)
-        Ad = A[np.arange(nr), :, :, :, np.arange(nr)]                          # [r][v][l][t]
-        Aavg = np.einsum("rv,rvlt->ltr", wz, Ad)
+        Aavg = np.empty((len(ls), len(TABLES), len(seps)))
+        for c0 in range(0, len(seps), chunk):
+            sl = slice(c0, min(c0 + chunk, len(seps)))
+            nr = sl.stop - sl.start
+            A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y,
+                           zn[sl].ravel(), list(ls), TABLES, seps[sl], den=self._p["has_density"],
+                           rsd=self._p["has_rsd"], d1=self._p["has_d1"], flatsky=self._p["has_flatsky_local"],
+                           nmu=self._p["multipole_quadrature_nodes"]).cpu().numpy()   # [r*v][l][t][r']
+            A = A.reshape(nr, nodes, len(ls), len(TABLES), nr)
+            Ad = A[np.arange(nr), :, :, :, np.arange(nr)]                      # [r][v][l][t]
+            Aavg[:, :, sl] = np.einsum("rv,rvlt->ltr", wz[sl], Ad)
         I = np.array([_ak(tabs[t][0], tabs[t][1], seps) for t in TABLES])
         return np.einsum("ltr,tr->lr", Aavg, I)
 

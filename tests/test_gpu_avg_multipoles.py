@@ -31,3 +31,18 @@ def test_average_multipoles_fixture(engine):
         got = np.array([p.value for p in pts])
         assert np.allclose(got, y, rtol=5e-4)
         assert np.max(np.abs(got / y - 1)) < 5e-5, (l, got / y - 1)
+
+
+def test_average_multipoles_chunking(engine):
+    """The weights computed two separations at a time equal the single-chunk ones
+    (the matching (node, r) pairs are the same wavefronts' work)."""
+    from coffe_amd.coffe import Coffe
+    sep = np.array([10, 20, 40, 100, 150]) / H
+    c = Coffe(sep=sep, l=[0, 2, 4])
+    k, pk = np.loadtxt(os.path.join(GOLD, "PkL_CLASS.dat")).T
+    c.set_power_spectrum_linear(k * H, pk / H ** 3)
+    c.has_rsd = True
+    print("average multipoles: chunks of 2 vs 8", flush=True)
+    a = c._average_values(0.9, 1.1, sep, [0, 2, 4], chunk=8)
+    b = c._average_values(0.9, 1.1, sep, [0, 2, 4], chunk=2)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a))
