@@ -247,7 +247,7 @@ double orc_sph_jl(int l, double x)
 }
 
 /* test hook: QAG on closed-form integrands (kind 0: x^9 + cos(x); 1: sin(50 x) e^{-x};
-   2: 1/sqrt(x)) so the adaptive driver can be pinned against exact values */
+   2: 1/sqrt(x)) so the adaptive bisection can be pinned against exact values */
 static double qt0(double x, void *p) { (void)p; return pow(x, 9) + cos(x); }
 static double qt1(double x, void *p) { (void)p; return sin(50 * x) * exp(-x); }
 static double qt2(double x, void *p) { (void)p; return 1.0 / sqrt(x); }
