@@ -86,8 +86,8 @@ def load_library():
     L.coffe_cov_plan_pairs.argtypes = [vp]
     L.coffe_cov_integrals.argtypes = [vp, i, i64p, vp, vp, vp, vp]
     L.coffe_cov_integrals_vjp.argtypes = [vp, i, i64p, vp, vp, vp, vp, vp]
-    L.coffe_cov_assemble_vjp.argtypes = [vp, i, i, dp, dp, dp, dp, dp, i, i, vp, vp, vp, dp, dp, vp]
-    L.coffe_cov_assemble.argtypes = [vp, i, i, dp, dp, dp, dp, dp, dp, i, i, i, vp, vp, vp]
+    L.coffe_cov_assemble_vjp.argtypes = [vp, i, i, ip, dp, dp, dp, dp, dp, dp, i, i, vp, vp, vp, dp, dp, vp]
+    L.coffe_cov_assemble.argtypes = [vp, i, i, ip, dp, dp, dp, dp, dp, dp, dp, i, i, i, i, vp, vp, vp]
     L.two_sph_bessel_status.argtypes = [dp, dp, ctypes.POINTER(dp), ctypes.c_long, ctypes.c_long, vp, dp, dp,
                                         ctypes.POINTER(dp)]
     L.coffe_terms_of.restype = ctypes.c_uint

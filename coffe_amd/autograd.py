@@ -80,7 +80,8 @@ class CovarianceFunction(torch.autograd.Function):
         z = ctx.zargs
         ginteg, gbias, gf = ctx.plan.assemble_vjp(integ, gout.contiguous(), ctx.bias_np, ctx.f_np, z["D1"],
                                                   z["nbar"], z["vol"], mixed=ctx.flags["mixed"],
-                                                  cosmic=ctx.flags["cosmic"])
+                                                  cosmic=ctx.flags["cosmic"], pops=ctx.flags["pops"],
+                                                  nbar2=z["nbar2"])
         gpk = gb = gfr = None
         if ctx.needs_input_grad[0]:
             gpk = ctx.plan.integrals_vjp(ctx.k, pk, ctx.n_in, ginteg)
@@ -94,13 +95,15 @@ class CovarianceFunction(torch.autograd.Function):
         return gpk, gb, gfr, None, None, None, None, None
 
 
-def covariance(plan, k, pk, n_in, bias, f, D1, nbar, vol, pix, poisson=True, mixed=True, cosmic=True):
+def covariance(plan, k, pk, n_in, bias, f, D1, nbar, vol, pix, poisson=True, mixed=True, cosmic=True, pops=None,
+               nbar2=None, window=False):
     """Differentiable Gaussian covariance (nspec, nz, nl, R, nl, R) of the concatenated
-    spectra (k, pk on the GPU, n_in samples each); bias is a CPU tensor (nz, 4) or (nz,)
-    (one population), f a CPU tensor (nz,).  Gradients flow to pk (and through
-    rescale_sigma8 to sigma8), bias and f."""
-    zargs = dict(D1=D1, nbar=nbar, vol=vol, pix=pix)
-    flags = dict(poisson=poisson, mixed=mixed, cosmic=cosmic)
+    spectra (k, pk on the GPU, n_in samples each); bias is a CPU tensor (nz, 4) (galaxy
+    biases 1..4, read through the populations pops, default 1, 1, 1, 1) or (nz,) (one value
+    in all four), f a CPU tensor (nz,); nbar / nbar2 the densities of the first and second
+    tracer.  Gradients flow to pk (and through rescale_sigma8 to sigma8), bias and f."""
+    zargs = dict(D1=D1, nbar=nbar, nbar2=nbar if nbar2 is None else nbar2, vol=vol, pix=pix)
+    flags = dict(poisson=poisson, mixed=mixed, cosmic=cosmic, pops=pops, window=window)
     return CovarianceFunction.apply(pk, bias, f, plan, k, n_in, zargs, flags)
 
 
@@ -169,6 +172,15 @@ def flat_weights_of_background(lw, theta, theta0, zmean, b, bins=10000):
     1457-1460).  Differentiable in theta (self-consistent d/dOmega, d/dw, ...) and b."""
     if not lw.f_override:
         raise ValueError("the weights must be built with f_override=True")
+    # only flat-sky density + RSD weights with a constant bias depend on the background
+    # through D1(z)^2 and f(z) alone; Doppler terms read H, G1 and chi, full-sky terms
+    # chi and D1 at the two end points, and bias tables would be multiplied in again
+    if not lw.flatsky:
+        raise ValueError("flat_weights_of_background needs flat-sky weights (flatsky=True)")
+    if lw.d1:
+        raise ValueError("flat_weights_of_background does not cover the Doppler terms (d1=False)")
+    if not lw.constant_bias:
+        raise ValueError("flat_weights_of_background needs constant-bias weights (bias='constant')")
     g = growth_at(theta, zmean, bins)
     g0 = growth_at(theta0.detach(), zmean, bins)
     dev = lw.groups[next(iter(lw.groups))].device
@@ -196,6 +208,8 @@ class LocalWeights:
         self.tables = list(tables)
         self.constant_bias = bias is not None
         self.f_override = bool(f_override)
+        self.flatsky = bool(flatsky)
+        self.d1 = bool(d1)
         bgw = dict(bg)
         if f_override:
             bgw["f"] = np.ones_like(np.asarray(bg["f"]))

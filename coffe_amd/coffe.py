@@ -70,6 +70,7 @@ class Coffe:
         has_density=True, has_rsd=False, has_d1=False, has_flatsky_local=False,
         integrals_fftlog_sampling=10000, covariance_integration_sampling=8000,
         covariance_cosmic=True, covariance_mixed=True, covariance_poisson=True,
+        has_binned_covariance=False, covariance_populations=[1, 1, 1, 1],
         multipole_quadrature_nodes=256,
     )
 
@@ -196,10 +197,33 @@ class Coffe:
                     lambda s, v: s._set_list("fsky", v, lambda x: 0 < x <= 1, s._free_results))
     pixelsize = property(lambda s: s._p["pixelsize"].copy(),
                          lambda s, v: s._set_list("pixelsize", v, lambda x: 0 < x < 10000, s._free_results))
+    # coffe.pyx:2097-2157: values in [0, 10); a zero density fails at compute time here
     number_density1 = property(lambda s: s._p["number_density1"].copy(),
-                               lambda s, v: s._set_list("number_density1", v, lambda x: x > 0, s._free_results))
+                               lambda s, v: s._set_list("number_density1", v, lambda x: 0 <= x < 10, s._free_results))
     number_density2 = property(lambda s: s._p["number_density2"].copy(),
-                               lambda s, v: s._set_list("number_density2", v, lambda x: x > 0, s._free_results))
+                               lambda s, v: s._set_list("number_density2", v, lambda x: 0 <= x < 10, s._free_results))
+
+    @property
+    def covariance_populations(self):
+        """The populations of the four fields of the covariance (coffe.pyx:2160-2192)."""
+        return self._p["covariance_populations"].copy()
+
+    @covariance_populations.setter
+    def covariance_populations(self, value):
+        try:
+            iter(value)
+        except TypeError as err:
+            raise TypeError(f"The value {value} is not iterable") from err
+        try:
+            temp = [int(v) for v in value]
+        except (TypeError, ValueError) as err:
+            raise TypeError("Cannot convert all values to integers") from err
+        if len(temp) != 4:
+            raise ValueError("The array for setting `covariance_populations` must have length 4")
+        if any(v not in (1, 2, 3, 4) for v in value):
+            raise ValueError("The values for `covariance_populations` must be one of: [1, 2, 3, 4]")
+        self._p["covariance_populations"] = np.array(temp)
+        self._free_results()
 
     def _set_flag(self, name, value):
         value = bool(value)
@@ -216,6 +240,9 @@ class Coffe:
     covariance_mixed = property(lambda s: s._p["covariance_mixed"], lambda s, v: s._set_flag("covariance_mixed", v))
     covariance_poisson = property(lambda s: s._p["covariance_poisson"],
                                   lambda s, v: s._set_flag("covariance_poisson", v))
+    # the bin-averaged Poisson term (covariance_window, coffe.pyx:2525-2545)
+    has_binned_covariance = property(lambda s: s._p["has_binned_covariance"],
+                                     lambda s, v: s._set_flag("has_binned_covariance", v))
 
     def __getattr__(self, name):
         if name in _UNSUPPORTED:
@@ -520,14 +547,19 @@ class Coffe:
         self._check_contributions()
         self._need_pk()
         zs = self._p["z_mean"]
-        per_bin = [self._p[k] for k in ("deltaz", "fsky", "pixelsize", "number_density1")]
+        per_bin = [self._p[k] for k in ("deltaz", "number_density1", "number_density2", "pixelsize", "fsky")]
         if any(len(v) != len(zs) for v in per_bin):
-            raise ValueError("z_mean, deltaz, fsky, pixelsize and number_density1 must have the same length")
+            raise ValueError("Mismatching lengths for covariance parameters (`z_mean`, `deltaz`, `number_density1`, "
+                             "`number_density2`, `pixelsize`, `fsky`")
         if recompute:
             self._free_all()
         if self._covariance is None:
             k, pk = self._pk
             ls, seps = [int(v) for v in self._p["l"]], self._p["sep"]
+            n1 = np.asarray(self._p["number_density1"], dtype=np.float64)
+            n2 = np.asarray(self._p["number_density2"], dtype=np.float64)
+            if np.any(n1 <= 0) or np.any(n2 <= 0):
+                raise ValueError("number_density1 and number_density2 must be positive for the covariance")
             plan = CovPlan(self._engine_handle(), self._p["covariance_integration_sampling"], k[0], k[-1], ls, seps)
             kt = torch.tensor(k, device="cuda")
             pt = torch.tensor(pk, device="cuda")
@@ -535,7 +567,8 @@ class Coffe:
             bias, f, D1, vol = [], [], [], []
             for iz, z in enumerate(zs):
                 dz, fsky = self._p["deltaz"][iz], self._p["fsky"][iz]
-                # biases only with a density contribution, f only with RSD (covariance.c:840-860)
+                # galaxy biases 1..4 only with a density contribution, f only with RSD
+                # (covariance.c:842-860); the populations pick which of them each field reads
                 b = [self._eval_bias(i, float(z)) if self._p["has_density"] else 0.0 for i in (1, 2, 3, 4)]
                 bias.append(b)
                 f.append(self.growth_rate(float(z)) if self._p["has_rsd"] else 0.0)
@@ -543,9 +576,10 @@ class Coffe:
                 chi_lo = self.comoving_distance(float(z - dz))
                 chi_hi = self.comoving_distance(float(z + dz))
                 vol.append(4 * np.pi * fsky * (chi_hi ** 3 - chi_lo ** 3) / 3.)
-            cov = plan.assemble(integ, np.array(bias), f, D1, self._p["number_density1"], vol, self._p["pixelsize"],
+            cov = plan.assemble(integ, np.array(bias), f, D1, n1, vol, self._p["pixelsize"],
                                 poisson=self._p["covariance_poisson"], mixed=self._p["covariance_mixed"],
-                                cosmic=self._p["covariance_cosmic"]).cpu().numpy()[0]
+                                cosmic=self._p["covariance_cosmic"], pops=self._p["covariance_populations"],
+                                nbar2=n2, window=self._p["has_binned_covariance"]).cpu().numpy()[0]
             plan.close()
             self._covariance = np.array([
                 Covariance(r1=float(seps[m]), r2=float(seps[n]), l1=ls[i], l2=ls[j], z=float(z),

@@ -98,46 +98,56 @@ class CovPlan:
               "coffe_cov_integrals_vjp")
         return gpk
 
-    def assemble(self, integ, bias, f, D1, nbar, vol, pix, poisson=True, mixed=True, cosmic=True, stream=None):
-        """Gaussian covariance (nspec, nz, nl, R, nl, R); bias is (nz, 4) or (nz,) (one population)."""
-        import torch
-        nspec = integ.shape[0]
-        _need(integ, (nspec, 2, self.npair, self.R, self.R), "integ")
+    @staticmethod
+    def _pops(pops):
+        p = (1, 1, 1, 1) if pops is None else tuple(int(v) for v in pops)
+        if len(p) != 4:
+            raise ValueError("pops must have length 4")
+        return (ctypes.c_int * 4)(*p)
+
+    def _bin_args(self, bias, per_bin):
         f64 = lambda a: np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)  # noqa: E731
         bias = np.asarray(bias, dtype=np.float64)
         if bias.ndim == 1:
             bias = np.repeat(bias[:, None], 4, axis=1)
+        if bias.ndim != 2 or bias.shape[1] != 4:
+            raise ValueError("bias must be (nz,) or (nz, 4)")
         bias = f64(bias)
         nz = bias.shape[0]
-        arrs = [f64(a) for a in (f, D1, nbar, vol, pix)]
+        arrs = [f64(a) for a in per_bin]
         if any(len(a) != nz for a in arrs):
             raise ValueError("per-bin arrays must have one entry per z bin")
+        return bias, nz, arrs
+
+    def assemble(self, integ, bias, f, D1, nbar, vol, pix, poisson=True, mixed=True, cosmic=True, pops=None,
+                 nbar2=None, window=False, stream=None):
+        """Gaussian covariance (nspec, nz, nl, R, nl, R) (covariance.c:790-1060).  bias is (nz, 4),
+        the galaxy biases 1..4 per bin, or (nz,) for one value in all four; pops are the
+        populations covariance_pop1..4 (default 1, 1, 1, 1); nbar / nbar2 the densities of the
+        first and second tracer (nbar2 defaults to nbar); window the bin-averaged Poisson term."""
+        import torch
+        nspec = integ.shape[0]
+        _need(integ, (nspec, 2, self.npair, self.R, self.R), "integ")
+        bias, nz, arrs = self._bin_args(bias, (f, D1, nbar, nbar if nbar2 is None else nbar2, vol, pix))
         nl = len(self.ls)
         out = torch.empty((nspec, nz, nl, self.R, nl, self.R), dtype=torch.float64, device=integ.device)
         dp = ctypes.POINTER(ctypes.c_double)
         st = stream if stream is not None else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        check(self.L.coffe_cov_assemble(self.h, nspec, nz, bias.ctypes.data_as(dp), *[a.ctypes.data_as(dp) for a in arrs],
-                                        int(poisson), int(mixed), int(cosmic), _ptr(integ), _ptr(out), st),
-              "coffe_cov_assemble")
+        check(self.L.coffe_cov_assemble(self.h, nspec, nz, self._pops(pops), bias.ctypes.data_as(dp),
+                                        *[a.ctypes.data_as(dp) for a in arrs], int(poisson), int(mixed), int(cosmic),
+                                        int(window), _ptr(integ), _ptr(out), st), "coffe_cov_assemble")
         return out
 
-    def assemble_vjp(self, integ, gout, bias, f, D1, nbar, vol, mixed=True, cosmic=True, stream=None):
+    def assemble_vjp(self, integ, gout, bias, f, D1, nbar, vol, mixed=True, cosmic=True, pops=None, nbar2=None,
+                     stream=None):
         """Vector-Jacobian product of `assemble` for the cotangent gout (its output's shape):
         (ginteg shaped like integ, gbias (nspec, nz, 4), gf (nspec, nz)), the gradients of
-        sum(gout * cov) per spectrum with respect to the integrals, the four biases and f of
-        every z bin (the Poisson term depends on none of them)."""
+        sum(gout * cov) per spectrum with respect to the integrals, the galaxy biases 1..4 and f
+        of every z bin (the Poisson term depends on none of them)."""
         import torch
         nspec = integ.shape[0]
         _need(integ, (nspec, 2, self.npair, self.R, self.R), "integ")
-        f64 = lambda a: np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)  # noqa: E731
-        bias = np.asarray(bias, dtype=np.float64)
-        if bias.ndim == 1:
-            bias = np.repeat(bias[:, None], 4, axis=1)
-        bias = f64(bias)
-        nz = bias.shape[0]
-        arrs = [f64(a) for a in (f, D1, nbar, vol)]
-        if any(len(a) != nz for a in arrs):
-            raise ValueError("per-bin arrays must have one entry per z bin")
+        bias, nz, arrs = self._bin_args(bias, (f, D1, nbar, nbar if nbar2 is None else nbar2, vol))
         nl = len(self.ls)
         _need(gout, (nspec, nz, nl, self.R, nl, self.R), "gout")
         ginteg = torch.empty_like(integ)
@@ -145,7 +155,7 @@ class CovPlan:
         gf = np.zeros((nspec, nz))
         dp = ctypes.POINTER(ctypes.c_double)
         st = stream if stream is not None else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        check(self.L.coffe_cov_assemble_vjp(self.h, nspec, nz, bias.ctypes.data_as(dp),
+        check(self.L.coffe_cov_assemble_vjp(self.h, nspec, nz, self._pops(pops), bias.ctypes.data_as(dp),
                                             *[a.ctypes.data_as(dp) for a in arrs], int(mixed), int(cosmic),
                                             _ptr(integ), _ptr(gout), _ptr(ginteg), gbias.ctypes.data_as(dp),
                                             gf.ctypes.data_as(dp), st), "coffe_cov_assemble_vjp")

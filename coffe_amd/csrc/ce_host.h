@@ -13,6 +13,10 @@
 #include "coffe_engine.h"
 #include "ce_internal.h"
 
+#ifdef __cplusplus
+extern "C" {
+#endif
+
 struct ce_grid;
 
 struct coffe_engine {
@@ -49,5 +53,10 @@ int ce_radix_schedule(int M, ce_radix_plan *rp);
 /* the same with radix-10 passes for the pairs of factors 2 and 5 (fewer passes for
    lengths such as 8000 = 8 * 10^3); for kernels that use fft_lds10 */
 int ce_radix_schedule_ex(int M, ce_radix_plan *rp, int pair10);
+/* q of twofast_selectqnu (twofast.c:240-250) */
+double ce_selectqnu(double l, double nu);
 
+#ifdef __cplusplus
+}
+#endif
 #endif

@@ -157,8 +157,9 @@ int ce_launch_cov_cols(void *stream, int nspec, int npair, int nrows, int R, con
                        const int *d_uoff, const double *d_uw, const double *d_fac, const double *d_W, double *d_out);
 int ce_launch_cov_assemble(void *stream, int nspec, int nz, int nl, int R, int npair, const int *d_ls,
                            const int *d_pair_of, const double *d_sep, const double *d_cc, const double *d_cm,
-                           const double *d_D1, const double *d_nbar, const double *d_vol, const double *d_pix,
-                           int poisson, int mixed, int cosmic, const double *d_integ, double *d_out);
+                           const double *d_D1, const double *d_pscale, const double *d_vol, const double *d_pix,
+                           int pd_same, int pd_swap, int window, int mixed, int cosmic, const double *d_integ,
+                           double *d_out);
 int ce_launch_cov_assemble_vjp(void *stream, int nspec, int nz, int nl, int R, int npair, const int *d_ls,
                                const int *d_pair_of, const double *d_cc, const double *d_cm, const double *d_D1,
                                const double *d_vol, int mixed, int cosmic, const double *d_integ,

@@ -221,44 +221,84 @@ struct asm_args {
     const int *pair_of;        // [nl][nl] -> pair index of (min, max)
     const double *sep;         // [R]
     const double *cc, *cm;     // [nz][nl][nl]
-    const double *D1, *nbar, *vol, *pix;   // [nz]
-    int poisson, mixed, cosmic;
+    const double *D1, *pscale, *vol, *pix;   // [nz]; pscale = 1 / (density1 density2 volume)
+    int pd_same, pd_swap;      // Poisson deltas d(p1,p3) d(p2,p4) and d(p1,p4) d(p2,p3); both 0: no Poisson term
+    int window, mixed, cosmic;
     const double *integ;       // [nspec][2][npair][R][R]
     double *out;               // [nspec][nz][nl][R][nl][R]
 };
 
-// one thread per output element (z, l1, r1, l2, r2) of one spectrum (blockIdx.y)
+#define CE_ASM_T 32            // tile edge of k_cov_assemble
+
+// k_cov_assemble: one workgroup (32 x 8 threads) per 32 x 32 tile (m, n) of one block
+// (l1 = ls[i], l2 = ls[j]) of one spectrum.  The tile's integrals of P and P^2 are
+// read once, coalesced along whichever index is contiguous in the stored block (m for
+// i <= j, n for the transposed blocks i > j, covariance.c:779-788), through LDS, and
+// the tile of every z bin is written with the threads along n (the output's fastest
+// index).  Cosmic, mixed and Poisson terms as covariance.c:866-1031.
 __global__ void __launch_bounds__(256) k_cov_assemble(asm_args a)
 {
-    const int64_t per = (int64_t)a.nz * a.nl * a.R * a.nl * a.R;
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= per) return;
-    const int s = blockIdx.y;
-    const int n = (int)(e % a.R);
-    const int j = (int)((e / a.R) % a.nl);
-    const int m = (int)((e / ((int64_t)a.R * a.nl)) % a.R);
-    const int i = (int)((e / ((int64_t)a.R * a.nl * a.R)) % a.nl);
-    const int z = (int)(e / ((int64_t)a.R * a.nl * a.R * a.nl));
-    const int l1 = a.ls[i], l2 = a.ls[j];
-    const int RR = a.R * a.R;
+    __shared__ double t1[CE_ASM_T][CE_ASM_T + 1], t2[CE_ASM_T][CE_ASM_T + 1];   // [m][n]
+    const int ntile = (a.R + CE_ASM_T - 1) / CE_ASM_T;
+    const int tm = blockIdx.x / ntile, tn = blockIdx.x % ntile;
+    const int blk = blockIdx.y, s = blockIdx.z;
+    const int i = blk / a.nl, j = blk % a.nl;
+    const int m0 = tm * CE_ASM_T, n0 = tn * CE_ASM_T;
+    const int tx = threadIdx.x % CE_ASM_T, ty = threadIdx.x / CE_ASM_T;
+    const int64_t RR = (int64_t)a.R * a.R;
     const int pr = a.pair_of[i * a.nl + j];
-    // blocks l1 > l2 are the transposes of the stored (l2, l1) blocks (covariance.c:779-788)
-    const int64_t idx = (i <= j) ? (int64_t)a.R * n + m : (int64_t)a.R * m + n;
-    const double *base = a.integ + (int64_t)s * 2 * a.npair * RR;
-    const double dpk = base[(int64_t)pr * RR + idx];
-    const double dpk2 = base[((int64_t)a.npair + pr) * RR + idx];
-    const int zl = (z * a.nl + i) * a.nl + j;
-    const double D = a.D1[z], nb = a.nbar[z], V = a.vol[z];
+    const double *b1 = a.integ + ((int64_t)s * 2 * a.npair + pr) * RR;
+    const double *b2 = a.integ + ((int64_t)s * 2 * a.npair + a.npair + pr) * RR;
+    for (int r = ty; r < CE_ASM_T; r += 8) {
+        if (i <= j) {   // stored at R n + m: threads along m
+            const int m = m0 + tx, n = n0 + r;
+            if (m < a.R && n < a.R) {
+                t1[tx][r] = b1[(int64_t)a.R * n + m];
+                t2[tx][r] = b2[(int64_t)a.R * n + m];
+            }
+        } else {        // the transpose of block (j, i), stored at R m + n: threads along n
+            const int m = m0 + r, n = n0 + tx;
+            if (m < a.R && n < a.R) {
+                t1[r][tx] = b1[(int64_t)a.R * m + n];
+                t2[r][tx] = b2[(int64_t)a.R * m + n];
+            }
+        }
+    }
+    __syncthreads();
+    const int l1 = a.ls[i], l2 = a.ls[j];
     const int sgn_l2 = (l2 % 2 == 0) ? 1 : -1;
     const int sgn_d = (((l2 - l1) / 2) % 2 == 0) ? 1 : -1;
     const int sgn_s = ((l1 + l2) % 2 == 0) ? 1 : -1;
-    double cov_p = 0.0, cov_m = 0.0, cov_c = 0.0;
-    if (a.poisson && l1 == l2 && m == n)
-        cov_p = (2 * l1 + 1) / 4. / 3.14159265358979323846 * (1 + sgn_l2) *
-                (1. / a.sep[n] / a.sep[m] / a.pix[z]) / nb / nb / V;
-    if (a.mixed) cov_m = sgn_d * sgn_s * D * D * dpk * a.cm[zl] / V;
-    if (a.cosmic) cov_c = sgn_d * D * D * D * D * dpk2 * a.cc[zl] / V;
-    a.out[(int64_t)s * per + e] = cov_p + cov_m + cov_c;
+    const int n = n0 + tx;
+    if (n >= a.R) return;
+    const int64_t per = (int64_t)a.nz * a.nl * a.R * a.nl * a.R;
+    double *out = a.out + (int64_t)s * per;
+    const double pdelta = (double)(a.pd_same + sgn_l2 * a.pd_swap);
+    for (int r = ty; r < CE_ASM_T; r += 8) {
+        const int m = m0 + r;
+        if (m >= a.R) break;
+        const double dpk = t1[r][tx], dpk2 = t2[r][tx];
+        const bool diag = l1 == l2 && m == n && pdelta != 0.0;   // the Poisson term's support
+        for (int z = 0; z < a.nz; ++z) {
+            const int zl = (z * a.nl + i) * a.nl + j;
+            const double D = a.D1[z], V = a.vol[z];
+            double cov = 0.0;
+            if (diag) {
+                const double sm = a.sep[m];
+                double g;
+                if (a.window) {
+                    const double lo = sm - a.pix[z] / 2., hi = sm + a.pix[z] / 2.;
+                    g = 3.0 / (hi * hi * hi - lo * lo * lo);
+                } else {
+                    g = 1. / sm / sm / a.pix[z];
+                }
+                cov += (2 * l1 + 1) / 4. / 3.14159265358979323846 * pdelta * g * a.pscale[z];
+            }
+            if (a.mixed) cov += sgn_d * sgn_s * D * D * dpk * a.cm[zl] / V;
+            if (a.cosmic) cov += sgn_d * D * D * D * D * dpk2 * a.cc[zl] / V;
+            out[((((int64_t)z * a.nl + i) * a.R + m) * a.nl + j) * a.R + n] = cov;
+        }
+    }
 }
 
 int set_lds(const void *fn, size_t bytes)
@@ -651,17 +691,19 @@ extern "C" int ce_launch_cov_cols(void *stream, int nspec, int npair, int nrows,
 
 extern "C" int ce_launch_cov_assemble(void *stream, int nspec, int nz, int nl, int R, int npair, const int *d_ls,
                                       const int *d_pair_of, const double *d_sep, const double *d_cc,
-                                      const double *d_cm, const double *d_D1, const double *d_nbar,
-                                      const double *d_vol, const double *d_pix, int poisson, int mixed, int cosmic,
-                                      const double *d_integ, double *d_out)
+                                      const double *d_cm, const double *d_D1, const double *d_pscale,
+                                      const double *d_vol, const double *d_pix, int pd_same, int pd_swap, int window,
+                                      int mixed, int cosmic, const double *d_integ, double *d_out)
 {
     asm_args a;
+    memset(&a, 0, sizeof(a));
     a.nz = nz; a.nl = nl; a.R = R; a.npair = npair; a.ls = d_ls; a.pair_of = d_pair_of; a.sep = d_sep;
-    a.cc = d_cc; a.cm = d_cm; a.D1 = d_D1; a.nbar = d_nbar; a.vol = d_vol; a.pix = d_pix;
-    a.poisson = poisson; a.mixed = mixed; a.cosmic = cosmic; a.integ = d_integ; a.out = d_out;
-    const int64_t per = (int64_t)nz * nl * R * nl * R;
-    hipLaunchKernelGGL(k_cov_assemble, dim3((unsigned)((per + 255) / 256), nspec), dim3(256), 0,
-                       (hipStream_t)stream, a);
+    a.cc = d_cc; a.cm = d_cm; a.D1 = d_D1; a.pscale = d_pscale; a.vol = d_vol; a.pix = d_pix;
+    a.pd_same = pd_same; a.pd_swap = pd_swap; a.window = window;
+    a.mixed = mixed; a.cosmic = cosmic; a.integ = d_integ; a.out = d_out;
+    const int ntile = (R + CE_ASM_T - 1) / CE_ASM_T;
+    hipLaunchKernelGGL(k_cov_assemble, dim3((unsigned)(ntile * ntile), (unsigned)(nl * nl), (unsigned)nspec),
+                       dim3(256), 0, (hipStream_t)stream, a);
     return (int)hipGetLastError();
 }
 

@@ -557,76 +557,10 @@ static double legendre_integral(int n, int m, int a, int b)
     return 2 * r;
 }
 
-/* covariance.c:40-57 */
-static double cov_coefficient(double b1, double b2, double f, int ell)
-{
-    switch (ell) {
-    case 0: return b1 * b2 + (b1 + b2) * f / 3 + pow(f, 2) / 5.;
-    case 2: return 2 * (b1 + b2) * f / 3. + 4 * pow(f, 2) / 7.;
-    case 4: return 8 * pow(f, 2) / 35.;
-    default: return 0;
-    }
-}
 
 static int sgnp(int m) { return (abs(m) % 2 == 0) ? 1 : -1; }
 
-int coffe_cov_assemble(coffe_cov_plan *p, int nspec, int nz, const double *bias, const double *f, const double *D1,
-                       const double *nbar, const double *vol, const double *pix, int poisson, int mixed, int cosmic,
-                       const double *d_integ, double *d_out, void *stream)
-{
-    if (!p || nspec <= 0 || nz <= 0 || !bias || !f || !D1 || !nbar || !vol || !pix || !d_integ || !d_out)
-        return COFFE_EINVAL;
-    hipSetDevice(p->e->device);
-    hipStream_t s = stream ? (hipStream_t)stream : p->e->stream;
-    const int nl = p->nl;
-    const size_t nzl = (size_t)nz * (size_t)nl * (size_t)nl;
-    double *cc = (double *)malloc(sizeof(double) * nzl);
-    double *cm = (double *)malloc(sizeof(double) * nzl);
-    double *zbuf = (double *)malloc(sizeof(double) * 4 * (size_t)nz);
-    if (!cc || !cm || !zbuf) { free(cc); free(cm); free(zbuf); return COFFE_ENOMEM; }
-    for (int k = 0; k < nz; ++k) {
-        const double *gb = bias + 4 * (size_t)k;   /* galaxy biases 1..4 of the bin */
-        const double gr = f[k];
-        for (int i = 0; i < nl; ++i)
-            for (int j = 0; j < nl; ++j) {
-                const int l1 = p->ls[i], l2 = p->ls[j];
-                double c = 0;
-                for (int a = 0; a <= 4; a += 2)
-                    for (int b = 0; b <= 4; b += 2)
-                        c += (cov_coefficient(gb[0], gb[2], gr, a) * cov_coefficient(gb[1], gb[3], gr, b) +
-                              sgnp(b) * cov_coefficient(gb[0], gb[3], gr, a) * cov_coefficient(gb[1], gb[2], gr, b)) *
-                             legendre_integral(l1, l2, a, b);
-                c /= 4.;
-                double mm = 0;
-                for (int a = 0; a <= 4; a += 2)
-                    mm += (cov_coefficient(gb[0], gb[2], gr, a) / nbar[k] + cov_coefficient(gb[1], gb[3], gr, a) / nbar[k] +
-                           cov_coefficient(gb[0], gb[3], gr, a) / nbar[k] + cov_coefficient(gb[1], gb[2], gr, a) / nbar[k]) *
-                          legendre_integral(l1, l2, a, 0);
-                mm /= 8.;
-                cc[((size_t)k * nl + i) * nl + j] = c;
-                cm[((size_t)k * nl + i) * nl + j] = mm;
-            }
-        zbuf[k] = D1[k];
-        zbuf[nz + k] = nbar[k];
-        zbuf[2 * nz + k] = vol[k];
-        zbuf[3 * nz + k] = pix[k];
-    }
-    double *d_cc = NULL, *d_cm = NULL, *d_z = NULL;
-    int st = upload((void **)&d_cc, cc, sizeof(double) * nzl, s);
-    if (st == COFFE_OK) st = upload((void **)&d_cm, cm, sizeof(double) * nzl, s);
-    if (st == COFFE_OK) st = upload((void **)&d_z, zbuf, sizeof(double) * 4 * (size_t)nz, s);
-    if (st == COFFE_OK &&
-        ce_launch_cov_assemble(s, nspec, nz, nl, p->R, p->npair, p->d_ls, p->d_pair_of, p->d_sep, d_cc, d_cm, d_z,
-                               d_z + nz, d_z + 2 * nz, d_z + 3 * nz, poisson, mixed, cosmic, d_integ, d_out))
-        st = COFFE_EDEVICE;
-    /* the coefficient buffers are released after the kernel has read them */
-    if (st == COFFE_OK) st = ce_hip_ok(hipStreamSynchronize(s));
-    hipFree(d_cc); hipFree(d_cm); hipFree(d_z);
-    free(cc); free(cm); free(zbuf);
-    return st;
-}
-
-/* ---- adjoint of the assembly ------------------------------------------------------- */
+static int kdelta(int i, int j) { return i == j; }
 
 /* a value and its derivatives by the four galaxy biases of a bin and f */
 typedef struct { double v, d[5]; } cv_dual;
@@ -656,9 +590,11 @@ static cv_dual dl_mul(cv_dual x, cv_dual y)
     return r;
 }
 
-/* cov_coefficient with derivatives (covariance.c:40-57) */
-static cv_dual cov_coefficient_dual(cv_dual b1, cv_dual b2, cv_dual f, int ell)
+/* the coefficient of populations (pa, pb), covariance_coefficient_pair
+   (covariance.c:40-74: biases bias[p - 1]), with derivatives */
+static cv_dual cov_pair_dual(const cv_dual *gb, int pa, int pb, cv_dual f, int ell)
 {
+    const cv_dual b1 = gb[pa - 1], b2 = gb[pb - 1];
     const cv_dual sum = dl_lin(1.0, b1, 1.0, b2);
     const cv_dual ff = dl_mul(f, f);
     switch (ell) {
@@ -669,12 +605,109 @@ static cv_dual cov_coefficient_dual(cv_dual b1, cv_dual b2, cv_dual f, int ell)
     }
 }
 
-int coffe_cov_assemble_vjp(coffe_cov_plan *p, int nspec, int nz, const double *bias, const double *f, const double *D1,
-                           const double *nbar, const double *vol, int mixed, int cosmic, const double *d_integ,
-                           const double *d_gout, double *d_ginteg, double *out_gbias, double *out_gf, void *stream)
+/* populations pops[0..3] (NULL: 1, 1, 1, 1) into p[4]; 0 when one is outside 1..4 */
+static int read_pops(const int *pops, int *p)
 {
-    if (!p || nspec <= 0 || nz <= 0 || !bias || !f || !D1 || !nbar || !vol || !d_integ || !d_gout || !d_ginteg)
+    for (int q = 0; q < 4; ++q) {
+        p[q] = pops ? pops[q] : 1;
+        if (p[q] < 1 || p[q] > 4) return 0;
+    }
+    return 1;
+}
+
+/* the per-(z, l1, l2) coefficients of covariance.c:866-937 for populations pp, biases
+   gb[0..3] and f of one bin and the densities n1 (density1) and n2 (density2):
+   *cc of the cosmic term, *cm of the mixed term */
+static void cov_coefficients(const int *pp, const cv_dual *gb, cv_dual f, double n1, double n2, int l1, int l2,
+                             cv_dual *cc, cv_dual *cm)
+{
+    const int p1 = pp[0], p2 = pp[1], p3 = pp[2], p4 = pp[3];
+    cv_dual c = dl_var(0.0, -1), mm = dl_var(0.0, -1);
+    for (int a = 0; a <= 4; a += 2)
+        for (int b = 0; b <= 4; b += 2) {
+            const cv_dual t = dl_lin(1.0, dl_mul(cov_pair_dual(gb, p1, p3, f, a), cov_pair_dual(gb, p2, p4, f, b)),
+                                     (double)sgnp(b),
+                                     dl_mul(cov_pair_dual(gb, p1, p4, f, a), cov_pair_dual(gb, p2, p3, f, b)));
+            c = dl_lin(1.0, c, legendre_integral(l1, l2, a, b), t);
+        }
+    for (int a = 0; a <= 4; a += 2) {
+        cv_dual t = dl_lin(kdelta(p2, p4) / n2, cov_pair_dual(gb, p1, p3, f, a), kdelta(p1, p3) / n1,
+                           cov_pair_dual(gb, p2, p4, f, a));
+        t = dl_lin(1.0, t, kdelta(p2, p3) / n2, cov_pair_dual(gb, p1, p4, f, a));
+        t = dl_lin(1.0, t, kdelta(p1, p4) / n1, cov_pair_dual(gb, p2, p3, f, a));
+        mm = dl_lin(1.0, mm, legendre_integral(l1, l2, a, 0), t);
+    }
+    *cc = dl_lin(0.25, c, 0.0, c);
+    *cm = dl_lin(0.125, mm, 0.0, mm);
+}
+
+int coffe_cov_assemble(coffe_cov_plan *p, int nspec, int nz, const int *pops, const double *bias, const double *f,
+                       const double *D1, const double *nbar1, const double *nbar2, const double *vol,
+                       const double *pix, int poisson, int mixed, int cosmic, int window, const double *d_integ,
+                       double *d_out, void *stream)
+{
+    if (!p || nspec <= 0 || nz <= 0 || !bias || !f || !D1 || !nbar1 || !nbar2 || !vol || !pix || !d_integ || !d_out)
         return COFFE_EINVAL;
+    int pp[4];
+    if (!read_pops(pops, pp)) return COFFE_EINVAL;
+    for (int k = 0; k < nz; ++k)
+        if (!(nbar1[k] > 0) || !(nbar2[k] > 0) || !(vol[k] > 0) || !(pix[k] > 0)) return COFFE_EDOMAIN;
+    hipSetDevice(p->e->device);
+    hipStream_t s = stream ? (hipStream_t)stream : p->e->stream;
+    const int nl = p->nl;
+    const size_t nzl = (size_t)nz * (size_t)nl * (size_t)nl;
+    double *cc = (double *)malloc(sizeof(double) * nzl);
+    double *cm = (double *)malloc(sizeof(double) * nzl);
+    double *zbuf = (double *)malloc(sizeof(double) * 4 * (size_t)nz);
+    if (!cc || !cm || !zbuf) { free(cc); free(cm); free(zbuf); return COFFE_ENOMEM; }
+    for (int k = 0; k < nz; ++k) {
+        cv_dual gb[4];
+        for (int q = 0; q < 4; ++q) gb[q] = dl_var(bias[4 * (size_t)k + q], -1);   /* galaxy biases 1..4 */
+        for (int i = 0; i < nl; ++i)
+            for (int j = 0; j < nl; ++j) {
+                cv_dual c, mm;
+                cov_coefficients(pp, gb, dl_var(f[k], -1), nbar1[k], nbar2[k], p->ls[i], p->ls[j], &c, &mm);
+                cc[((size_t)k * nl + i) * nl + j] = c.v;
+                cm[((size_t)k * nl + i) * nl + j] = mm.v;
+            }
+        zbuf[k] = D1[k];
+        zbuf[nz + k] = 1.0 / (nbar1[k] * nbar2[k] * vol[k]);   /* the Poisson term's scale */
+        zbuf[2 * nz + k] = vol[k];
+        zbuf[3 * nz + k] = pix[k];
+    }
+    /* the Poisson term's population deltas (covariance.c:945-967) */
+    const int pd_same = kdelta(pp[0], pp[2]) * kdelta(pp[1], pp[3]);
+    const int pd_swap = kdelta(pp[0], pp[3]) * kdelta(pp[1], pp[2]);
+    double *d_cc = NULL, *d_cm = NULL, *d_z = NULL;
+    int st = upload((void **)&d_cc, cc, sizeof(double) * nzl, s);
+    if (st == COFFE_OK) st = upload((void **)&d_cm, cm, sizeof(double) * nzl, s);
+    if (st == COFFE_OK) st = upload((void **)&d_z, zbuf, sizeof(double) * 4 * (size_t)nz, s);
+    if (st == COFFE_OK &&
+        ce_launch_cov_assemble(s, nspec, nz, nl, p->R, p->npair, p->d_ls, p->d_pair_of, p->d_sep, d_cc, d_cm, d_z,
+                               d_z + nz, d_z + 2 * nz, d_z + 3 * nz, poisson ? pd_same : 0, poisson ? pd_swap : 0,
+                               window, mixed, cosmic, d_integ, d_out))
+        st = COFFE_EDEVICE;
+    /* the coefficient buffers are released after the kernel has read them */
+    if (st == COFFE_OK) st = ce_hip_ok(hipStreamSynchronize(s));
+    hipFree(d_cc); hipFree(d_cm); hipFree(d_z);
+    free(cc); free(cm); free(zbuf);
+    return st;
+}
+
+/* ---- adjoint of the assembly ------------------------------------------------------- */
+
+int coffe_cov_assemble_vjp(coffe_cov_plan *p, int nspec, int nz, const int *pops, const double *bias,
+                           const double *f, const double *D1, const double *nbar1, const double *nbar2,
+                           const double *vol, int mixed, int cosmic, const double *d_integ, const double *d_gout,
+                           double *d_ginteg, double *out_gbias, double *out_gf, void *stream)
+{
+    if (!p || nspec <= 0 || nz <= 0 || !bias || !f || !D1 || !nbar1 || !nbar2 || !vol || !d_integ || !d_gout ||
+        !d_ginteg)
+        return COFFE_EINVAL;
+    int pp[4];
+    if (!read_pops(pops, pp)) return COFFE_EINVAL;
+    for (int k = 0; k < nz; ++k)
+        if (!(nbar1[k] > 0) || !(nbar2[k] > 0) || !(vol[k] > 0)) return COFFE_EDOMAIN;
     hipSetDevice(p->e->device);
     hipStream_t s = stream ? (hipStream_t)stream : p->e->stream;
     const int nl = p->nl;
@@ -694,26 +727,9 @@ int coffe_cov_assemble_vjp(coffe_cov_plan *p, int nspec, int nz, const double *b
         const cv_dual gr = dl_var(f[k], 4);
         for (int i = 0; i < nl; ++i)
             for (int j = 0; j < nl; ++j) {
-                const int l1 = p->ls[i], l2 = p->ls[j];
-                cv_dual c = dl_var(0.0, -1), mm = dl_var(0.0, -1);
-                for (int a = 0; a <= 4; a += 2)
-                    for (int b = 0; b <= 4; b += 2) {
-                        const cv_dual t = dl_lin(1.0, dl_mul(cov_coefficient_dual(gb[0], gb[2], gr, a),
-                                                             cov_coefficient_dual(gb[1], gb[3], gr, b)),
-                                                 (double)sgnp(b), dl_mul(cov_coefficient_dual(gb[0], gb[3], gr, a),
-                                                                         cov_coefficient_dual(gb[1], gb[2], gr, b)));
-                        c = dl_lin(1.0, c, legendre_integral(l1, l2, a, b), t);
-                    }
-                for (int a = 0; a <= 4; a += 2) {
-                    cv_dual t = dl_lin(1.0, cov_coefficient_dual(gb[0], gb[2], gr, a), 1.0,
-                                       cov_coefficient_dual(gb[1], gb[3], gr, a));
-                    t = dl_lin(1.0, t, 1.0, cov_coefficient_dual(gb[0], gb[3], gr, a));
-                    t = dl_lin(1.0, t, 1.0, cov_coefficient_dual(gb[1], gb[2], gr, a));
-                    mm = dl_lin(1.0, mm, legendre_integral(l1, l2, a, 0) / nbar[k], t);
-                }
+                cv_dual c, mm;
+                cov_coefficients(pp, gb, gr, nbar1[k], nbar2[k], p->ls[i], p->ls[j], &c, &mm);
                 const size_t zl = ((size_t)k * nl + i) * nl + j;
-                c = dl_lin(0.25, c, 0.0, c);
-                mm = dl_lin(0.125, mm, 0.0, mm);
                 cc[zl] = c.v;
                 cm[zl] = mm.v;
                 dc[2 * zl] = mm;

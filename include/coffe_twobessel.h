@@ -71,26 +71,36 @@ int coffe_cov_integrals_vjp(coffe_cov_plan *p, int nspec, const int64_t *n_in, c
                             const double *d_gint, double *d_gpk, void *stream);
 
 /*
- * Gaussian covariance of the multipoles (covariance.c:796-1060), per z bin k:
- * galaxy biases bias[4 k .. 4 k + 3], growth rate f[k], D1[k], number density nbar[k],
- * volume vol[k], pixel size pix[k] (host arrays); poisson/mixed/cosmic select the
- * terms.  d_integ from coffe_cov_integrals; d_out[s][z][l1][r1][l2][r2] (device).
+ * Gaussian covariance of the multipoles (covariance.c:790-1060, the FFTLog branch), per
+ * z bin k: the galaxy biases 1..4 at bias[4 k .. 4 k + 3], growth rate f[k], D1[k], the
+ * number densities of the first and second tracer nbar1[k] and nbar2[k] (density1 /
+ * density2), volume vol[k], pixel size pix[k] (host arrays).  pops[0..3] are the
+ * populations of the four fields (covariance_pop1..4, each 1..4; NULL means 1, 1, 1, 1,
+ * the reference's default, parser.c:945-948): a coefficient of populations (a, b) reads
+ * biases bias[a - 1] and bias[b - 1] (covariance.c:60-74) and the mixed and Poisson terms
+ * carry the reference's Kronecker deltas of the populations with density1 / density2
+ * (covariance.c:889-967).  poisson/mixed/cosmic select the terms; window selects the
+ * bin-averaged Poisson term (covariance_window, covariance.c:931-944).  d_integ from
+ * coffe_cov_integrals; d_out[s][z][l1][r1][l2][r2] (device).  COFFE_EINVAL for a
+ * population outside 1..4, COFFE_EDOMAIN for a non-positive density, volume or pixel.
  */
-int coffe_cov_assemble(coffe_cov_plan *p, int nspec, int nz, const double *bias, const double *f, const double *D1,
-                       const double *nbar, const double *vol, const double *pix, int poisson, int mixed, int cosmic,
-                       const double *d_integ, double *d_out, void *stream);
+int coffe_cov_assemble(coffe_cov_plan *p, int nspec, int nz, const int *pops, const double *bias, const double *f,
+                       const double *D1, const double *nbar1, const double *nbar2, const double *vol,
+                       const double *pix, int poisson, int mixed, int cosmic, int window, const double *d_integ,
+                       double *d_out, void *stream);
 
 /*
  * Vector-Jacobian product of coffe_cov_assemble (no reference counterpart; the
  * differentiation of the covariance): given the cotangent d_gout of d_out, writes
  * d_ginteg (layout of d_integ) = d<gout, cov>/d integrals, and on the host
- * out_gbias[s][z][4] and out_gf[s][z] = d<gout_s, cov_s>/d bias and /d f of every z bin,
- * per spectrum (either may be NULL).  The Poisson term depends on neither.  Same
- * per-bin arguments as coffe_cov_assemble; synchronises the stream.
+ * out_gbias[s][z][4] and out_gf[s][z] = d<gout_s, cov_s>/d galaxy bias 1..4 and /d f of
+ * every z bin, per spectrum (either may be NULL).  The Poisson term depends on neither.
+ * Same populations and per-bin arguments as coffe_cov_assemble; synchronises the stream.
  */
-int coffe_cov_assemble_vjp(coffe_cov_plan *p, int nspec, int nz, const double *bias, const double *f, const double *D1,
-                           const double *nbar, const double *vol, int mixed, int cosmic, const double *d_integ,
-                           const double *d_gout, double *d_ginteg, double *out_gbias, double *out_gf, void *stream);
+int coffe_cov_assemble_vjp(coffe_cov_plan *p, int nspec, int nz, const int *pops, const double *bias,
+                           const double *f, const double *D1, const double *nbar1, const double *nbar2,
+                           const double *vol, int mixed, int cosmic, const double *d_integ, const double *d_gout,
+                           double *d_ginteg, double *out_gbias, double *out_gf, void *stream);
 
 #ifdef __cplusplus
 }

@@ -108,10 +108,10 @@ int orc_covariance_fftlog_integral(size_t nin, const double *kin, const double *
 double orc_threej000(int a, int b, int c);
 double orc_legendre_integral(int n, int m, int a, int b);
 double orc_covariance_coefficient(double b1, double b2, double f, int ell);
-int orc_covariance_assemble(size_t nz, size_t nl, const int *ls, size_t R, const double *sep, const double *bias,
-                            const double *f, const double *D1, const double *nbar, const double *vol,
-                            const double *pix, int poisson, int mixed, int cosmic, const double *pk_int,
-                            const double *pk2_int, double *out);
+int orc_covariance_assemble(size_t nz, size_t nl, const int *ls, size_t R, const double *sep, const int *pops,
+                            const double *bias, const double *f, const double *D1, const double *nbar1,
+                            const double *nbar2, const double *vol, const double *pix, int poisson, int mixed,
+                            int cosmic, int window, const double *pk_int, const double *pk2_int, double *out);
 
 #ifdef __cplusplus
 }

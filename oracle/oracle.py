@@ -67,8 +67,8 @@ def _declare(L):
     L.orc_legendre_integral.restype = d
     L.orc_covariance_coefficient.argtypes = [d, d, d, i]
     L.orc_covariance_coefficient.restype = d
-    L.orc_covariance_assemble.argtypes = [s, s, ctypes.POINTER(ctypes.c_int), s, p, p, p, p, p, p, p, i, i, i,
-                                          p, p, p]
+    L.orc_covariance_assemble.argtypes = [s, s, ctypes.POINTER(ctypes.c_int), s, p, ctypes.POINTER(ctypes.c_int),
+                                          p, p, p, p, p, p, p, i, i, i, i, p, p, p]
     ll = ctypes.POINTER(ctypes.c_longlong)
     L.orc_reference_batch.argtypes = [i, ll, p, p, ll, s, p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
                                       ctypes.POINTER(ctypes.c_int), s, p, i, d, p]
@@ -295,14 +295,24 @@ def legendre_integral(n, m, a, b):
 
 
 def covariance_assemble(ls, sep, bias, f, D1, nbar, vol, pix, pk_int, pk2_int, poisson=True, mixed=True,
-                        cosmic=True):
+                        cosmic=True, pops=None, nbar2=None, window=False):
+    """covariance.c:790-1060.  bias: (nz,) (one value for all four biases) or (nz, 4)
+    (galaxy biases 1..4 per bin); pops: covariance_pop1..4 (default 1, 1, 1, 1);
+    nbar / nbar2: density1 / density2 per bin (nbar2 defaults to nbar)."""
+    bias = np.asarray(bias, dtype=np.float64)
+    if bias.ndim == 1:
+        bias = np.repeat(bias[:, None], 4, axis=1)
     nz, nl, R = len(bias), len(ls), len(sep)
     lsa = (ctypes.c_int * nl)(*ls)
-    arrs = [_f64(np.atleast_1d(a)) for a in (sep, bias, f, D1, nbar, vol, pix)]
+    pa = (ctypes.c_int * 4)(*(pops if pops is not None else (1, 1, 1, 1)))
+    nbar2 = nbar if nbar2 is None else nbar2
+    arrs = [_f64(np.atleast_1d(a)) for a in (sep, bias, f, D1, nbar, nbar2, vol, pix)]
     pk_int, pk2_int = _f64(pk_int), _f64(pk2_int)
     out = np.empty(nz * nl * R * nl * R)
-    _check(lib().orc_covariance_assemble(nz, nl, lsa, R, *[_p(a) for a in arrs], int(poisson), int(mixed),
-                                         int(cosmic), _p(pk_int), _p(pk2_int), _p(out)), "covariance_assemble")
+    p = [_p(a) for a in arrs]
+    _check(lib().orc_covariance_assemble(nz, nl, lsa, R, p[0], pa, *p[1:], int(poisson), int(mixed),
+                                         int(cosmic), int(window), _p(pk_int), _p(pk2_int), _p(out)),
+           "covariance_assemble")
     return out.reshape(nz, nl, R, nl, R)
 
 

@@ -194,24 +194,41 @@ double orc_covariance_coefficient(double b1, double b2, double f, int ell)
 
 static int sgn(int m) { return (abs(m) % 2 == 0) ? 1 : -1; }
 
-/*
- * Gaussian flat-sky covariance of multipoles, FFTLog method, one population,
- * linear P(k) (redshift-independent D/G integrals scaled by D1^2 and D1^4).
- * Per redshift bin iz: bias b[iz], growth f[iz], D1[iz], density nbar[iz],
- * volume vol[iz], pixel size pix[iz].  Output ordered (z, l1, r1, l2, r2).
- * pk_int / pk2_int: L*L blocks of R*R integrals (already scaled), only the
- * l1 <= l2 blocks are read; the others are filled by transposition as the
- * reference does.
- */
-int orc_covariance_assemble(size_t nz, size_t nl, const int *ls, size_t R, const double *sep, const double *bias,
-                            const double *f, const double *D1, const double *nbar, const double *vol,
-                            const double *pix, int poisson, int mixed, int cosmic, const double *pk_int,
-                            const double *pk2_int, double *out)
+static int kdelta(int i, int j) { return i == j ? 1 : 0; }   /* common.c:1401-1407 */
+
+/* covariance_coefficient_pair (covariance.c:60-74): the coefficient of populations
+   (p1, p2), biases bias[p - 1] */
+static double coefficient_pair(int p1, int p2, const double *bias, double f, int ell)
 {
+    return orc_covariance_coefficient(bias[p1 - 1], bias[p2 - 1], f, ell);
+}
+
+/*
+ * Gaussian flat-sky covariance of multipoles, FFTLog method, linear P(k)
+ * (redshift-independent integrals scaled by D1^2 and D1^4), covariance.c:790-1060.
+ * Populations pops[0..3] (covariance_pop1..4, each 1..4; NULL = 1, 1, 1, 1); per
+ * redshift bin k the galaxy biases 1..4 at bias[4 k .. 4 k + 3], growth f[k], D1[k],
+ * the densities of the first and second tracer nbar1[k], nbar2[k] (density1/2), the
+ * volume vol[k] and the pixel size pix[k].  window selects the bin-averaged Poisson
+ * term (covariance_window, covariance.c:931-944).  Output ordered (z, l1, r1, l2, r2).
+ * pk_int / pk2_int: L*L blocks of R*R integrals (already scaled), only the l1 <= l2
+ * blocks are read; the others are filled by transposition as the reference does
+ * (covariance.c:779-788).
+ */
+int orc_covariance_assemble(size_t nz, size_t nl, const int *ls, size_t R, const double *sep, const int *pops,
+                            const double *bias, const double *f, const double *D1, const double *nbar1,
+                            const double *nbar2, const double *vol, const double *pix, int poisson, int mixed,
+                            int cosmic, int window, const double *pk_int, const double *pk2_int, double *out)
+{
+    static const int one[4] = {1, 1, 1, 1};
+    const int *pp = pops ? pops : one;
+    for (int q = 0; q < 4; ++q)
+        if (pp[q] < 1 || pp[q] > 4) return ORC_EINVAL;
+    const int p1 = pp[0], p2 = pp[1], p3 = pp[2], p4 = pp[3];
     size_t idx = 0;
     const size_t RR = R * R;
     for (size_t k = 0; k < nz; ++k) {
-        const double gb[4] = {bias[k], bias[k], bias[k], bias[k]};
+        const double *gb = bias + 4 * k;
         const double gr = f[k];
         const double D = D1[k];
         for (size_t i = 0; i < nl; ++i)
@@ -222,23 +239,20 @@ int orc_covariance_assemble(size_t nz, size_t nl, const int *ls, size_t R, const
                         double cc = 0;
                         for (int a = 0; a <= 4; a += 2)
                             for (int b = 0; b <= 4; b += 2)
-                                cc += (orc_covariance_coefficient(gb[0], gb[2], gr, a) *
-                                           orc_covariance_coefficient(gb[1], gb[3], gr, b) +
-                                       sgn(b) * orc_covariance_coefficient(gb[0], gb[3], gr, a) *
-                                           orc_covariance_coefficient(gb[1], gb[2], gr, b)) *
+                                cc += (coefficient_pair(p1, p3, gb, gr, a) * coefficient_pair(p2, p4, gb, gr, b) +
+                                       sgn(b) * coefficient_pair(p1, p4, gb, gr, a) *
+                                           coefficient_pair(p2, p3, gb, gr, b)) *
                                       orc_legendre_integral(l1, l2, a, b);
                         cc /= 4.;
                         double cm = 0;
                         for (int a = 0; a <= 4; a += 2)
-                            cm += (orc_covariance_coefficient(gb[0], gb[2], gr, a) / nbar[k] +
-                                   orc_covariance_coefficient(gb[1], gb[3], gr, a) / nbar[k] +
-                                   orc_covariance_coefficient(gb[0], gb[3], gr, a) / nbar[k] +
-                                   orc_covariance_coefficient(gb[1], gb[2], gr, a) / nbar[k]) *
+                            cm += (coefficient_pair(p1, p3, gb, gr, a) * kdelta(p2, p4) / nbar2[k] +
+                                   coefficient_pair(p2, p4, gb, gr, a) * kdelta(p1, p3) / nbar1[k] +
+                                   coefficient_pair(p1, p4, gb, gr, a) * kdelta(p2, p3) / nbar2[k] +
+                                   coefficient_pair(p2, p3, gb, gr, a) * kdelta(p1, p4) / nbar1[k]) *
                                   orc_legendre_integral(l1, l2, a, 0);
                         cm /= 8.;
                         const int dll = l1 == l2, dmn = m == n;
-                        /* blocks: (i, j) with i <= j stored at [i*nl + j][R*n + m];
-                           (i > j) is the transpose of block (j, i) */
                         double dpk, dpk2;
                         if (i <= j) {
                             dpk = pk_int[(i * nl + j) * RR + R * n + m];
@@ -247,9 +261,18 @@ int orc_covariance_assemble(size_t nz, size_t nl, const int *ls, size_t R, const
                             dpk = pk_int[(j * nl + i) * RR + R * m + n];
                             dpk2 = pk2_int[(j * nl + i) * RR + R * m + n];
                         }
-                        const double cov_p = poisson ? (2 * l1 + 1) / 4. / ORC_PI * dll * dmn * (1 + sgn(l2)) *
-                                                           (1. / sep[n] / sep[m] / pix[k]) / nbar[k] / nbar[k] / vol[k]
-                                                     : 0;
+                        double geom;
+                        if (window) {
+                            const double lo = sep[m] - pix[k] / 2., hi = sep[m] + pix[k] / 2.;
+                            geom = 1. / ((pow(hi, 3) - pow(lo, 3)) / 3.);
+                        } else {
+                            geom = 1. / sep[n] / sep[m] / pix[k];
+                        }
+                        const double cov_p =
+                            poisson ? (2 * l1 + 1) / 4. / ORC_PI * dll * dmn *
+                                          (kdelta(p1, p3) * kdelta(p2, p4) + sgn(l2) * kdelta(p1, p4) * kdelta(p2, p3)) *
+                                          geom / nbar1[k] / nbar2[k] / vol[k]
+                                    : 0;
                         const double cov_m = mixed ? sgn((l2 - l1) / 2) * sgn(l1 + l2) * D * D * dpk * cm / vol[k] : 0;
                         const double cov_c = cosmic ? sgn((l2 - l1) / 2) * D * D * D * D * dpk2 * cc / vol[k] : 0;
                         out[idx++] = cov_p + cov_m + cov_c;

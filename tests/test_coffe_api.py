@@ -111,3 +111,56 @@ def test_average_multipoles_coordinates_checked():
         c.compute_average_multipoles_bulk()
     with pytest.raises(ValueError):
         c.z_min = [12.0]
+
+
+def test_covariance_populations_and_densities():
+    """covariance_populations (coffe.pyx:2160-2192: default 1, 1, 1, 1, length 4, values
+    in 1..4), number_density1/2 (coffe.pyx:2097-2157: values in [0, 10)) and
+    has_binned_covariance (coffe.pyx:2525-2545)."""
+    c = Coffe()
+    assert list(c.covariance_populations) == [1, 1, 1, 1]
+    c.covariance_populations = [1, 2, 1, 2]
+    assert list(c.covariance_populations) == [1, 2, 1, 2]
+    with pytest.raises(ValueError):
+        c.covariance_populations = [1, 2, 1]
+    with pytest.raises(ValueError):
+        c.covariance_populations = [1, 2, 5, 1]
+    with pytest.raises(TypeError):
+        c.covariance_populations = 3
+    with pytest.raises(TypeError):
+        c.covariance_populations = [1, None, 1, 1]
+    assert list(c.covariance_populations) == [1, 2, 1, 2]
+    c.number_density2 = [2e-4]
+    assert list(c.number_density2) == [2e-4]
+    with pytest.raises(ValueError):
+        c.number_density2 = [11.0]
+    with pytest.raises(ValueError):
+        c.number_density1 = [-1e-3]
+    with pytest.raises(TypeError):
+        c.number_density1 = 1e-3
+    assert c.has_binned_covariance is False
+    c.has_binned_covariance = 1
+    assert c.has_binned_covariance is True
+
+
+def test_covariance_lengths_checked():
+    """Mismatching per-bin covariance lists raise ValueError before any computation
+    (coffe.pyx:3448-3452)."""
+    c = Coffe(z_mean=[1.0, 1.5], deltaz=[0.1, 0.1], fsky=[0.2, 0.2], pixelsize=[10.0, 10.0],
+              number_density1=[1e-3, 1e-3], number_density2=[1e-3])
+    c.set_power_spectrum_linear(np.logspace(-4, 1, 64), np.ones(64))
+    with pytest.raises(ValueError):
+        c.compute_covariance_bulk()
+
+
+def test_flat_weights_of_background_rejects_unsupported_weights():
+    """Only flat-sky density + RSD weights with a constant bias depend on the background
+    through D1^2 and f alone; other LocalWeights are refused (no silent wrong gradient)."""
+    import torch
+    from coffe_amd.autograd import LocalWeights, flat_weights_of_background
+    theta = torch.zeros(6, dtype=torch.float64)
+    for flatsky, d1, const in [(False, False, True), (True, True, True), (True, False, False)]:
+        lw = LocalWeights.__new__(LocalWeights)
+        lw.f_override, lw.flatsky, lw.d1, lw.constant_bias, lw.groups = True, flatsky, d1, const, {}
+        with pytest.raises(ValueError):
+            flat_weights_of_background(lw, theta, theta, [1.0], 1.5)

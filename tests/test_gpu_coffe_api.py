@@ -7,6 +7,8 @@ import os
 import numpy as np
 import pytest
 
+import oracle as O
+
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -81,3 +83,45 @@ def test_covariance_multipoles(engine):
     M = np.array([r.value for r in result]).reshape(3 * R, 3 * R)
     assert np.allclose(M, M.T, rtol=1e-6, atol=1e-6 * np.max(np.abs(M)))
     assert np.all(np.linalg.eigvalsh(0.5 * (M + M.T)) > 0)
+
+
+def test_covariance_populations_api(engine):
+    """The Coffe object's multi-tracer covariance: with the default populations setting
+    galaxy_bias2 changes nothing (the reference reads galaxy_bias1 in every slot,
+    covariance.c:60-74, parser.c:945-948); populations (1, 2, 1, 2) with number_density2
+    read bias 2 and density2 and equal the oracle's assembly of the same integrals."""
+    import torch
+    from coffe_amd.coffe import Coffe
+    from coffe_amd.covariance import CovPlan
+    zb = np.linspace(0, 3, 16)
+    common = dict(has_density=True, has_rsd=True, number_density1=[1e-3 * H ** 3],
+                  number_density2=[3e-4 * H ** 3], z_mean=[1.0], deltaz=[0.1], fsky=[0.2], pixelsize=[50 / H],
+                  sep=np.arange(50, 350, 50) / H, l=[0, 2, 4], covariance_integration_sampling=1024)
+    base = Coffe(**common)
+    base.set_power_spectrum_linear(*_pk())
+    base.set_galaxy_bias1(zb, 1.4 * np.ones_like(zb))
+    v0 = np.array([r.value for r in base.compute_covariance_bulk()])
+    base.set_galaxy_bias2(zb, 2.3 * np.ones_like(zb))
+    v1 = np.array([r.value for r in base.compute_covariance_bulk()])
+    assert np.array_equal(v0, v1)
+    print("Coffe covariance populations (1, 2, 1, 2)", flush=True)
+    base.covariance_populations = [1, 2, 1, 2]
+    v2 = np.array([r.value for r in base.compute_covariance_bulk()])
+    # the same integrals through the oracle's assembly
+    k, pk = _pk()
+    sep = base.sep
+    plan = CovPlan(base._engine_handle(), 1024, k[0], k[-1], [0, 2, 4], sep)
+    integ = plan.integrals(torch.tensor(k, device="cuda"), torch.tensor(pk, device="cuda"), [len(k)]).cpu().numpy()[0]
+    plan.close()
+    pk_int = np.zeros((9, 36))
+    pk2_int = np.zeros((9, 36))
+    for q, (i, j) in enumerate([(i, j) for i in range(3) for j in range(i, 3)]):
+        pk_int[i * 3 + j] = integ[0, q].reshape(-1)
+        pk2_int[i * 3 + j] = integ[1, q].reshape(-1)
+    z, dz, fsky = 1.0, 0.1, 0.2
+    vol = 4 * np.pi * fsky * (base.comoving_distance(z + dz) ** 3 - base.comoving_distance(z - dz) ** 3) / 3
+    ref = O.covariance_assemble([0, 2, 4], sep, np.array([[1.4, 2.3, 1.0, 1.0]]), [base.growth_rate(z)],
+                                [base.growth_factor(z)], [1e-3 * H ** 3], [vol], [50 / H], pk_int, pk2_int,
+                                pops=(1, 2, 1, 2), nbar2=[3e-4 * H ** 3]).reshape(-1)
+    assert np.max(np.abs(v2 - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert np.max(np.abs(v2 - v1)) > 1e-3 * np.max(np.abs(v1))

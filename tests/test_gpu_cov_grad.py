@@ -40,15 +40,23 @@ def _dot(a, b):
     return float((a * b).sum().item())
 
 
-def test_assemble_vjp(engine):
+@pytest.mark.parametrize("pops", [None, (1, 2, 1, 2), (2, 1, 1, 2), (1, 3, 4, 2)])
+def test_assemble_vjp(engine, pops):
+    """Default populations (only galaxy bias 1 enters: the others' gradients are zero) and
+    multi-tracer populations with density2 != density1."""
     import torch
     plan, k, p, n_in, z, bias, f, dev = _setup(engine)
-    print(f"cov assemble vjp: rows={plan.nrows}", flush=True)
+    print(f"cov assemble vjp: rows={plan.nrows} pops={pops}", flush=True)
+    z = dict(z, pops=pops, nbar2=[2.5e-3 * H ** 3, 7e-4 * H ** 3])
     integ = plan.integrals(k, p, n_in)
     cov = plan.assemble(integ, bias, f, **z)
     G = torch.tensor(np.random.default_rng(3).standard_normal(tuple(cov.shape)), **dev)
-    zv = dict(D1=z["D1"], nbar=z["nbar"], vol=z["vol"])
+    zv = dict(D1=z["D1"], nbar=z["nbar"], vol=z["vol"], pops=pops, nbar2=z["nbar2"])
     ginteg, gbias, gf = plan.assemble_vjp(integ, G, bias, f, **zv)
+    used = {0} if pops is None else {q - 1 for q in pops}
+    for q in range(4):
+        if q not in used:
+            assert np.all(gbias[:, :, q] == 0.0), (q, gbias[:, :, q])
     # linearity in the integrals (no Poisson term)
     X = torch.tensor(np.random.default_rng(4).standard_normal(tuple(integ.shape)), **dev)
     for Y in (X, integ):
