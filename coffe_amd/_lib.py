@@ -60,6 +60,7 @@ def load_library():
     L.twofast_1bessel.argtypes = [dp, dp, sz, dp, dp, sz, d, d, d, d, d, d, ctypes.c_uint]
     L.twofast_1bessel.restype = None
     L.twofast_1bessel_status.argtypes = [dp, dp, sz, dp, dp, sz, d, d, d, d, d, d]
+    L.twofast_1bessel_noalloc.argtypes = [vp, vp, sz, vp, vp, sz, d, d, d, d, d, d, vp]
     L.coffe_batch_create.argtypes = [vp, i, i64p, i64p, dp, dp, i, ip, ip, i, dp, i, i, ctypes.POINTER(vp)]
     L.coffe_batch_destroy.argtypes = [vp]
     L.coffe_batch_destroy.restype = None
@@ -107,6 +108,7 @@ def load_library():
 EXPORTED = [
     "coffe_engine_strerror", "coffe_engine_last_error", "coffe_engine_create", "coffe_engine_destroy",
     "coffe_engine_default", "coffe_engine_synchronize", "twofast_1bessel", "twofast_1bessel_status",
+    "twofast_1bessel_noalloc",
     "coffe_batch_create", "coffe_batch_destroy", "coffe_batch_input_size", "coffe_batch_forward",
     "coffe_batch_vjp", "coffe_batch_tables", "coffe_batch_prepare", "coffe_batch_tables_stage",
     "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute", "coffe_background_jvp",

@@ -68,6 +68,26 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
                            const double *input_y, size_t input_len, double l, double nu, double r0, double k0,
                            double kmin, double kmax);
 
+/*
+ * twofast_1bessel_status on device arrays, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  d_input_x/y
+ * (input_len samples) and d_output_x/y (output_len values, output_x = r0 (kmax /
+ * kmin)^(i/N)) are device arrays on the current device; all workspace, including the
+ * grid tables (recomputed every call, nothing is cached), lives in one fixed device
+ * array of the library (about 1.2 MB per GPU, kept until the process ends).  Limits:
+ * output_len <= 16384 (even, N/2 = 2^a 3^b 5^c), input_len <= 65536
+ * (COFFE_EUNSUPPORTED beyond).  The call is synchronous on `stream` (NULL = the null
+ * stream); calls in one process are serialised by a lock, and every call on a GPU
+ * reuses that GPU's array, so calls that overlap on one GPU from several processes
+ * race.  Returns COFFE_ENONFINITE for a non-finite input sample or gamma-ratio kernel
+ * (checked before the transform runs; the outputs are then not written), and the
+ * validation codes of twofast_1bessel_status otherwise.
+ */
+int twofast_1bessel_noalloc(double *d_output_x, double *d_output_y, size_t output_len, const double *d_input_x,
+                            const double *d_input_y, size_t input_len, double l, double nu, double r0, double k0,
+                            double kmin, double kmax, void *stream);
+
 /* host Akima interpolation (common.c:524-576, 704-710 with the default Akima type):
    out[q] = spline(x, y)(xq[q]), NaN outside [x_0, x_{n-1}] */
 int coffe_akima_eval(size_t n, const double *x, const double *y, size_t nq, const double *xq, double *out);
