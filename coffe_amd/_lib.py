@@ -54,6 +54,8 @@ def load_library():
     L.coffe_engine_create.argtypes = [i, ctypes.POINTER(vp)]
     L.coffe_engine_destroy.argtypes = [vp]
     L.coffe_engine_destroy.restype = None
+    L.coffe_engine_inject_alloc_failure.argtypes = [i]
+    L.coffe_engine_inject_alloc_failure.restype = None
     L.coffe_engine_default.argtypes = []
     L.coffe_engine_default.restype = vp
     L.coffe_engine_synchronize.argtypes = [vp]
@@ -108,7 +110,7 @@ def load_library():
 EXPORTED = [
     "coffe_engine_strerror", "coffe_engine_last_error", "coffe_engine_create", "coffe_engine_destroy",
     "coffe_engine_default", "coffe_engine_synchronize", "twofast_1bessel", "twofast_1bessel_status",
-    "twofast_1bessel_noalloc",
+    "twofast_1bessel_noalloc", "coffe_engine_inject_alloc_failure",
     "coffe_batch_create", "coffe_batch_destroy", "coffe_batch_input_size", "coffe_batch_forward",
     "coffe_batch_vjp", "coffe_batch_tables", "coffe_batch_prepare", "coffe_batch_tables_stage",
     "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute", "coffe_background_jvp",

@@ -72,9 +72,26 @@ static int hip_ok(hipError_t e) { return e == hipSuccess ? COFFE_OK : COFFE_EDEV
 
 int ce_hip_ok(hipError_t e) { return hip_ok(e); }
 
+/* allocation-failure injection (coffe_engine_inject_alloc_failure): the countdown of
+   device allocations until the one that fails, 0 = off */
+static int ce_fail_countdown = 0;
+
+void coffe_engine_inject_alloc_failure(int n) { __atomic_store_n(&ce_fail_countdown, n > 0 ? n : 0, __ATOMIC_SEQ_CST); }
+
+static int injected_failure(void)
+{
+    int c = __atomic_load_n(&ce_fail_countdown, __ATOMIC_SEQ_CST);
+    while (c > 0) {
+        if (__atomic_compare_exchange_n(&ce_fail_countdown, &c, c - 1, 0, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST))
+            return c == 1;
+    }
+    return 0;
+}
+
 static int dev_alloc(void **p, size_t bytes)
 {
     if (bytes == 0) bytes = 8;
+    if (injected_failure()) { *p = NULL; return COFFE_ENOMEM; }
     return hipMalloc(p, bytes) == hipSuccess ? COFFE_OK : COFFE_ENOMEM;
 }
 
@@ -821,7 +838,7 @@ static int scratch_reserve(coffe_engine *e, size_t bytes, hipStream_t s)
     hipFree(e->d_scratch);
     e->d_scratch = NULL;
     e->scratch_bytes = 0;
-    if (hipMalloc(&e->d_scratch, bytes) != hipSuccess) return COFFE_ENOMEM;
+    if (injected_failure() || hipMalloc(&e->d_scratch, bytes) != hipSuccess) { e->d_scratch = NULL; return COFFE_ENOMEM; }
     e->scratch_bytes = bytes;
     return COFFE_OK;
 }

@@ -53,6 +53,10 @@ const char *coffe_engine_strerror(int code);
 int coffe_engine_last_error(void);
 
 int coffe_engine_create(int device, coffe_engine **out);
+/* testing hook (no reference counterpart): the n-th device allocation of the library
+   from now on fails with COFFE_ENOMEM (n >= 1; 0 switches the injection off), so that
+   tests can drive the clean-up paths of every allocating call */
+void coffe_engine_inject_alloc_failure(int n);
 void coffe_engine_destroy(coffe_engine *e);
 /* the engine used by the drop-in entry points (created on first use, device 0) */
 coffe_engine *coffe_engine_default(void);

@@ -80,6 +80,11 @@ int orc_integral_table(int n, int l, size_t N, size_t nin, const double *kin, co
 int orc_divergent_tables(size_t nin, const double *kin, const double *pin, double kmin, double kmax, size_t nsep,
                          const double *sep, size_t nchi, const double *chi, double *i40, double *r0, double *ren);
 
+int orc_tables_at_sep_jvp(int nspec, const long long *off, const double *k, const double *p, const double *v,
+                          const long long *nfft, int ntab, const int *tab_n, const int *tab_l, size_t nsep,
+                          const double *sep, double *out, double *dout);
+int orc_akima_jvp(size_t n, const double *x, const double *y, const double *dy, size_t nq, const double *xq,
+                  double *out, double *dout);
 int orc_tables_at_sep_batch(int nspec, const long long *off, const double *k, const double *p, const long long *nfft,
                             int ntab, const int *tab_n, const int *tab_l, size_t nsep, const double *sep, double *out);
 
@@ -101,6 +106,7 @@ int orc_reference_batch(int nspec, const long long *off, const double *k, const 
 
 /* 2D FFTLog and covariance (orc_covariance.c) */
 void orc_g_l(double l, double nu, size_t n, const double *eta, double *gl);
+void orc_window_2d(double *out, double width, long halfN1, long halfN2);
 int orc_two_sph_bessel(const double *k1, const double *k2, const double *fk, long N1, long N2, double l1, double l2,
                        double nu1, double nu2, double width, double *r1, double *r2, double *result);
 int orc_covariance_fftlog_integral(size_t nin, const double *kin, const double *pin, double power, int l1, int l2,

@@ -59,6 +59,11 @@ def _declare(L):
                                  ctypes.POINTER(ctypes.c_int), s, p, i, d, p]
     L.orc_g_l.argtypes = [d, d, s, p, p]
     L.orc_g_l.restype = None
+    L.orc_window_2d.argtypes = [p, d, ctypes.c_long, ctypes.c_long]
+    L.orc_window_2d.restype = None
+    L.orc_cspline_init.argtypes = [s, p, p, p]
+    L.orc_cspline_deriv.argtypes = [s, p, p, p, d]
+    L.orc_cspline_deriv.restype = d
     L.orc_two_sph_bessel.argtypes = [p, p, p, ctypes.c_long, ctypes.c_long, d, d, d, d, d, p, p, p]
     L.orc_covariance_fftlog_integral.argtypes = [s, p, p, d, i, i, s, s, p, d, d, p]
     L.orc_threej000.argtypes = [i, i, i]
@@ -74,6 +79,9 @@ def _declare(L):
                                       ctypes.POINTER(ctypes.c_int), s, p, i, d, p]
     L.orc_tables_at_sep_batch.argtypes = [i, ll, p, p, ll, i, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), s, p, p]
+    L.orc_tables_at_sep_jvp.argtypes = [i, ll, p, p, p, ll, i, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int), s, p, p, p]
+    L.orc_akima_jvp.argtypes = [s, p, p, p, s, p, p, p]
     L.orc_qag_test.argtypes = [i, d, d, d, ctypes.POINTER(ctypes.c_int)]
     L.orc_qag_test.restype = d
 
@@ -352,3 +360,32 @@ def tables_at_sep_batch(specs, nfft, tables, sep):
     _check(lib().orc_tables_at_sep_batch(len(specs), off.ctypes.data_as(llp), _p(k), _p(pk), nf.ctypes.data_as(llp),
                                          len(tables), tn, tl, len(sp), _p(sp), _p(out)), "tables_at_sep_batch")
     return out.reshape(len(specs), len(tables), len(sp))
+
+
+def akima_jvp(x, y, dy, xq):
+    """Akima interpolant at xq and its derivative along dy of the node values."""
+    x, y, dy, xq = _f64(x), _f64(y), _f64(dy), _f64(np.atleast_1d(xq))
+    out, dout = np.empty(len(xq)), np.empty(len(xq))
+    _check(lib().orc_akima_jvp(len(x), _p(x), _p(y), _p(dy), len(xq), _p(xq), _p(out), _p(dout)), "akima_jvp")
+    return out, dout
+
+
+def tables_at_sep_jvp(specs, nfft, tables, sep, dirs):
+    """orc_tables_at_sep_batch and its forward-mode derivative along dirs (one array per
+    spectrum, like its P); returns (values, tangents), each (nspec, ntab, nsep)."""
+    off = np.concatenate([[0], np.cumsum([len(s[0]) for s in specs])]).astype(np.longlong)
+    k = _f64(np.concatenate([s[0] for s in specs]))
+    pk = _f64(np.concatenate([s[1] for s in specs]))
+    v = _f64(np.concatenate(dirs))
+    nf = np.ascontiguousarray(nfft, dtype=np.longlong)
+    tn = (ctypes.c_int * len(tables))(*[t[0] for t in tables])
+    tl = (ctypes.c_int * len(tables))(*[t[1] for t in tables])
+    sp = _f64(sep)
+    out = np.empty(len(specs) * len(tables) * len(sp))
+    dout = np.empty_like(out)
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    _check(lib().orc_tables_at_sep_jvp(len(specs), off.ctypes.data_as(llp), _p(k), _p(pk), _p(v),
+                                       nf.ctypes.data_as(llp), len(tables), tn, tl, len(sp), _p(sp), _p(out),
+                                       _p(dout)), "tables_at_sep_jvp")
+    shape = (len(specs), len(tables), len(sp))
+    return out.reshape(shape), dout.reshape(shape)

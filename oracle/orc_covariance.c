@@ -64,6 +64,12 @@ static void c_window_2d(cplx *out, double width, long halfN1, long halfN2)
     }
 }
 
+/* c_window_2d on an N1 x (halfN2 + 1) interleaved complex array (the tests' handle) */
+void orc_window_2d(double *out, double width, long halfN1, long halfN2)
+{
+    c_window_2d((cplx *)out, width, halfN1, halfN2);
+}
+
 /* fk: N1 x N2 row-major; result: N1 x N2 row-major */
 int orc_two_sph_bessel(const double *k1, const double *k2, const double *fk, long N1, long N2, double l1, double l2,
                        double nu1, double nu2, double width, double *r1, double *r2, double *result)
@@ -87,11 +93,13 @@ int orc_two_sph_bessel(const double *k1, const double *k2, const double *fk, lon
     for (long i = 0; i < N1; i++) r1[i] = 1. / k1[N1 - 1 - i];
     for (long i = 0; i < N2; i++) r2[i] = 1. / k2[N2 - 1 - i];
     const double r10 = r1[0], r20 = r2[0];
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < N1; i++)
         for (long j = 0; j < N2; j++) Pb[i * N2 + j] = fk[i * N2 + j] / pow(k1[i], nu1) / pow(k2[j], nu2);
     st = orc_rfft2_r2c((size_t)N1, (size_t)N2, Pb, (double *)out);
     if (st != ORC_OK) goto done;
     c_window_2d(out, width, h1, h2);
+#pragma omp parallel for schedule(static)
     for (long i = 0; i <= h1; i++) {
         for (long j = 0; j <= h2; j++) {
             const long ij = i * (h2 + 1) + j;
@@ -101,6 +109,7 @@ int orc_two_sph_bessel(const double *k1, const double *k2, const double *fk, lon
             out[ij] = conj(out[ij]);
         }
     }
+#pragma omp parallel for schedule(static)
     for (long i = h1 + 1; i < N1; i++) {
         for (long j = 0; j <= h2; j++) {
             const long ij = i * (h2 + 1) + j;
@@ -113,6 +122,7 @@ int orc_two_sph_bessel(const double *k1, const double *k2, const double *fk, lon
     }
     st = orc_rfft2_c2r((size_t)N1, (size_t)N2, (const double *)out, Pb);
     if (st != ORC_OK) goto done;
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < N1; i++)
         for (long j = 0; j < N2; j++)
             result[i * N2 + j] = Pb[i * N2 + j] * ORC_PI / (16. * N1 * N2 * pow(r2[j], nu2) * pow(r1[i], nu1));
@@ -145,8 +155,9 @@ int orc_covariance_fftlog_integral(size_t nin, const double *kin, const double *
     }
     st = orc_two_sph_bessel(k, k, fk, (long)N, (long)N, l1, l2, 1.01, 1.01, 0.25, r1, r2, res);
     if (st != ORC_OK) goto done;
-    for (size_t m = 0; m < N; ++m)
-        for (size_t n = 0; n < N; ++n) res2[n * N + m] = res[m * N + n];
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < (long)N; ++m)
+        for (size_t n = 0; n < N; ++n) res2[n * N + (size_t)m] = res[(size_t)m * N + n];
     bc = orc_bicubic_new(N, N, r1, r2, res2);
     if (!bc) { st = ORC_ENOMEM; goto done; }
     for (size_t m = 0; m < R; ++m)

@@ -322,43 +322,73 @@ int orc_rfft_c2r(size_t n, const double *X, double *x)
     return ORC_OK;
 }
 
-/* 2D row-major n1 x n2 real -> n1 x (n2/2+1) complex, FFTW r2c_2d semantics */
+/* 2D row-major n1 x n2 real -> n1 x (n2/2+1) complex, FFTW r2c_2d semantics.  Rows,
+   then columns, each transform independent: OpenMP threads take rows and columns with
+   plans and buffers of their own (the arithmetic per transform is the serial one) */
 int orc_rfft2_r2c(size_t n1, size_t n2, const double *x, double *X)
 {
     const size_t h2 = n2 / 2 + 1;
-    orc_rfft *r = orc_rfft_new(n2);
-    orc_fft_plan *c = plan_new(n1);
-    cplx *col = (cplx *)malloc(sizeof(cplx) * n1);
-    cplx *w = (cplx *)malloc(sizeof(cplx) * ((c && c->bluestein) ? c->m : n1));
-    if (!r || !c || !col || !w) { orc_rfft_free(r); plan_free(c); free(col); free(w); return ORC_ENOMEM; }
-    for (size_t i = 0; i < n1; ++i) orc_rfft_forward(r, x + i * n2, X + 2 * i * h2);
-    for (size_t j = 0; j < h2; ++j) {
-        for (size_t i = 0; i < n1; ++i) col[i] = X[2 * (i * h2 + j)] + I * X[2 * (i * h2 + j) + 1];
-        execute(c, col, w);
-        for (size_t i = 0; i < n1; ++i) { X[2 * (i * h2 + j)] = creal(col[i]); X[2 * (i * h2 + j) + 1] = cimag(col[i]); }
+    int fail = 0;
+#pragma omp parallel
+    {
+        orc_rfft *r = orc_rfft_new(n2);
+        orc_fft_plan *c = plan_new(n1);
+        cplx *col = (cplx *)malloc(sizeof(cplx) * n1);
+        cplx *w = (cplx *)malloc(sizeof(cplx) * ((c && c->bluestein) ? c->m : n1));
+        const int ok = r && c && col && w;
+        if (!ok) {
+#pragma omp atomic write
+            fail = 1;
+        }
+#pragma omp for schedule(static)
+        for (long i = 0; i < (long)n1; ++i)
+            if (ok) orc_rfft_forward(r, x + (size_t)i * n2, X + 2 * (size_t)i * h2);
+#pragma omp for schedule(static)
+        for (long jj = 0; jj < (long)h2; ++jj) {
+            if (!ok) continue;
+            const size_t j = (size_t)jj;
+            for (size_t i = 0; i < n1; ++i) col[i] = X[2 * (i * h2 + j)] + I * X[2 * (i * h2 + j) + 1];
+            execute(c, col, w);
+            for (size_t i = 0; i < n1; ++i) { X[2 * (i * h2 + j)] = creal(col[i]); X[2 * (i * h2 + j) + 1] = cimag(col[i]); }
+        }
+        orc_rfft_free(r); plan_free(c); free(col); free(w);
     }
-    orc_rfft_free(r); plan_free(c); free(col); free(w);
-    return ORC_OK;
+    return fail ? ORC_ENOMEM : ORC_OK;
 }
 
 /* inverse of the above (unnormalised, FFTW c2r_2d semantics: the last dimension is
    Hermitian-completed, imaginary parts of its DC and Nyquist columns are dropped
-   after the column transforms) */
+   after the column transforms); threads as above */
 int orc_rfft2_c2r(size_t n1, size_t n2, const double *X, double *x)
 {
     const size_t h2 = n2 / 2 + 1;
-    orc_rfft *r = orc_rfft_new(n2);
-    orc_fft_plan *c = plan_new(n1);
-    cplx *col = (cplx *)malloc(sizeof(cplx) * n1);
     double *tmp = (double *)malloc(sizeof(double) * 2 * n1 * h2);
-    cplx *w = (cplx *)malloc(sizeof(cplx) * ((c && c->bluestein) ? c->m : n1));
-    if (!r || !c || !col || !tmp || !w) { orc_rfft_free(r); plan_free(c); free(col); free(tmp); free(w); return ORC_ENOMEM; }
-    for (size_t j = 0; j < h2; ++j) {
-        for (size_t i = 0; i < n1; ++i) col[i] = X[2 * (i * h2 + j)] - I * X[2 * (i * h2 + j) + 1];
-        execute(c, col, w);
-        for (size_t i = 0; i < n1; ++i) { tmp[2 * (i * h2 + j)] = creal(col[i]); tmp[2 * (i * h2 + j) + 1] = -cimag(col[i]); }
+    if (!tmp) return ORC_ENOMEM;
+    int fail = 0;
+#pragma omp parallel
+    {
+        orc_rfft *r = orc_rfft_new(n2);
+        orc_fft_plan *c = plan_new(n1);
+        cplx *col = (cplx *)malloc(sizeof(cplx) * n1);
+        cplx *w = (cplx *)malloc(sizeof(cplx) * ((c && c->bluestein) ? c->m : n1));
+        const int ok = r && c && col && w;
+        if (!ok) {
+#pragma omp atomic write
+            fail = 1;
+        }
+#pragma omp for schedule(static)
+        for (long jj = 0; jj < (long)h2; ++jj) {
+            if (!ok) continue;
+            const size_t j = (size_t)jj;
+            for (size_t i = 0; i < n1; ++i) col[i] = X[2 * (i * h2 + j)] - I * X[2 * (i * h2 + j) + 1];
+            execute(c, col, w);
+            for (size_t i = 0; i < n1; ++i) { tmp[2 * (i * h2 + j)] = creal(col[i]); tmp[2 * (i * h2 + j) + 1] = -cimag(col[i]); }
+        }
+#pragma omp for schedule(static)
+        for (long i = 0; i < (long)n1; ++i)
+            if (ok) orc_rfft_backward(r, tmp + 2 * (size_t)i * h2, x + (size_t)i * n2);
+        orc_rfft_free(r); plan_free(c); free(col); free(w);
     }
-    for (size_t i = 0; i < n1; ++i) orc_rfft_backward(r, tmp + 2 * i * h2, x + i * n2);
-    orc_rfft_free(r); plan_free(c); free(col); free(tmp); free(w);
-    return ORC_OK;
+    free(tmp);
+    return fail ? ORC_ENOMEM : ORC_OK;
 }

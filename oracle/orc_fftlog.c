@@ -343,3 +343,40 @@ int orc_tables_at_sep_batch(int nspec, const long long *off, const double *k, co
     }
     return status;
 }
+
+/*
+ * Forward-mode derivative of orc_tables_at_sep_batch along a direction v of the input
+ * samples (same layout as p): the tables (FFTLog of the spline of the samples,
+ * twofast.c:362-483) are linear in the samples, so their tangent is the table of v;
+ * the Akima lookups at the separations (common.c:704-710) are differentiated by
+ * orc_akima_jvp.  out as orc_tables_at_sep_batch, dout the tangent, both
+ * [nspec][ntab][nsep].
+ */
+int orc_tables_at_sep_jvp(int nspec, const long long *off, const double *k, const double *p, const double *v,
+                          const long long *nfft, int ntab, const int *tab_n, const int *tab_l, size_t nsep,
+                          const double *sep, double *out, double *dout)
+{
+    int status = ORC_OK;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int u = 0; u < nspec * ntab; ++u) {
+        const int s = u / ntab, t = u % ntab;
+        const size_t nin = (size_t)(off[s + 1] - off[s]);
+        const size_t N = (size_t)nfft[s];
+        double *tx = (double *)malloc(sizeof(double) * (N + 43));
+        double *ty = (double *)malloc(sizeof(double) * (N + 43));
+        double *vx = (double *)malloc(sizeof(double) * (N + 43));
+        double *vy = (double *)malloc(sizeof(double) * (N + 43));
+        int st = (tx && ty && vx && vy) ? ORC_OK : ORC_ENOMEM;
+        const double kmin = k[off[s]], kmax = k[off[s + 1] - 1];
+        if (st == ORC_OK) st = orc_integral_table(tab_n[t], tab_l[t], N, nin, k + off[s], p + off[s], kmin, kmax, 0, tx, ty);
+        if (st == ORC_OK) st = orc_integral_table(tab_n[t], tab_l[t], N, nin, k + off[s], v + off[s], kmin, kmax, 0, vx, vy);
+        if (st == ORC_OK)
+            st = orc_akima_jvp(N + 43, tx, ty, vy, nsep, sep, out + (size_t)u * nsep, dout + (size_t)u * nsep);
+        free(tx); free(ty); free(vx); free(vy);
+        if (st != ORC_OK) {
+            #pragma omp atomic write
+            status = st;
+        }
+    }
+    return status;
+}

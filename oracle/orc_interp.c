@@ -155,6 +155,79 @@ double orc_akima_eval(size_t n, const double *x, const double *y, const double *
     return y[i] + delx * (b[i] + delx * (c[i] + d[i] * delx));
 }
 
+/* ---- forward-mode derivative of the Akima interpolant in its node values ------ */
+typedef struct { double v, d; } orc_dual;
+
+static orc_dual dd_sub(orc_dual a, orc_dual b) { orc_dual r = {a.v - b.v, a.d - b.d}; return r; }
+static orc_dual dd_add(orc_dual a, orc_dual b) { orc_dual r = {a.v + b.v, a.d + b.d}; return r; }
+static orc_dual dd_scale(double s, orc_dual a) { orc_dual r = {s * a.v, s * a.d}; return r; }
+static orc_dual dd_mul(orc_dual a, orc_dual b) { orc_dual r = {a.v * b.v, a.d * b.v + a.v * b.d}; return r; }
+static orc_dual dd_div(orc_dual a, orc_dual b)
+{
+    orc_dual r = {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)};
+    return r;
+}
+static orc_dual dd_abs(orc_dual a)
+{
+    orc_dual r = {fabs(a.v), a.v > 0 ? a.d : (a.v < 0 ? -a.d : 0.0)};
+    return r;
+}
+
+/*
+ * The Akima interpolant of orc_akima_init / orc_akima_eval at xq[q] (out) and its
+ * directional derivative with respect to the node values y along dy (dout): every
+ * step of the slope formula carried as a (value, derivative) pair, |u| with
+ * derivative sign(u) du (the interpolant is differentiable wherever no weight
+ * |m_{i+1} - m_i| vanishes).
+ */
+int orc_akima_jvp(size_t n, const double *x, const double *y, const double *dy, size_t nq, const double *xq,
+                  double *out, double *dout)
+{
+    if (n < 5) return ORC_EINVAL;
+    orc_dual *mm = (orc_dual *)malloc(sizeof(orc_dual) * (n + 4));
+    if (!mm) return ORC_ENOMEM;
+    orc_dual *m = mm + 2;
+    for (size_t i = 0; i + 1 < n; ++i) {
+        const double h = x[i + 1] - x[i];
+        m[i].v = (y[i + 1] - y[i]) / h;
+        m[i].d = (dy[i + 1] - dy[i]) / h;
+    }
+    m[-2] = dd_sub(dd_scale(3.0, m[0]), dd_scale(2.0, m[1]));
+    m[-1] = dd_sub(dd_scale(2.0, m[0]), m[1]);
+    m[n - 1] = dd_sub(dd_scale(2.0, m[n - 2]), m[n - 3]);
+    m[n] = dd_sub(dd_scale(3.0, m[n - 2]), dd_scale(2.0, m[n - 3]));
+    for (size_t q = 0; q < nq; ++q) {
+        if (!(xq[q] >= x[0] && xq[q] <= x[n - 1])) { out[q] = NAN; dout[q] = NAN; continue; }
+        const long i = (long)orc_bsearch(x, xq[q], 0, n - 1);
+        const double h = x[i + 1] - x[i];
+        const orc_dual ne = dd_add(dd_abs(dd_sub(m[i + 1], m[i])), dd_abs(dd_sub(m[i - 1], m[i - 2])));
+        orc_dual b, c, d;
+        if (ne.v == 0.0) {
+            b = m[i];
+            c.v = c.d = d.v = d.d = 0.0;
+        } else {
+            const orc_dual ne_next = dd_add(dd_abs(dd_sub(m[i + 2], m[i + 1])), dd_abs(dd_sub(m[i], m[i - 1])));
+            const orc_dual alpha = dd_div(dd_abs(dd_sub(m[i - 1], m[i - 2])), ne);
+            const orc_dual one = {1.0, 0.0};
+            orc_dual t_next;
+            if (ne_next.v == 0.0) {
+                t_next = m[i];
+            } else {
+                const orc_dual an = dd_div(dd_abs(dd_sub(m[i], m[i - 1])), ne_next);
+                t_next = dd_add(dd_mul(dd_sub(one, an), m[i]), dd_mul(an, m[i + 1]));
+            }
+            b = dd_add(dd_mul(dd_sub(one, alpha), m[i - 1]), dd_mul(alpha, m[i]));
+            c = dd_scale(1.0 / h, dd_sub(dd_sub(dd_scale(3.0, m[i]), dd_scale(2.0, b)), t_next));
+            d = dd_scale(1.0 / (h * h), dd_sub(dd_add(b, t_next), dd_scale(2.0, m[i])));
+        }
+        const double delx = xq[q] - x[i];
+        out[q] = y[i] + delx * (b.v + delx * (c.v + d.v * delx));
+        dout[q] = dy[i] + delx * (b.d + delx * (c.d + d.d * delx));
+    }
+    free(mm);
+    return ORC_OK;
+}
+
 /* convenience: build + evaluate at many points */
 int orc_akima_interp(size_t n, const double *x, const double *y, size_t nq, const double *xq, double *out)
 {
@@ -203,14 +276,31 @@ orc_bicubic *orc_bicubic_new(size_t nx, size_t ny, const double *x, const double
     s->zx = (double *)malloc(sizeof(double) * nx * ny);
     s->zy = (double *)malloc(sizeof(double) * nx * ny);
     s->zxy = (double *)malloc(sizeof(double) * nx * ny);
-    size_t mx = nx > ny ? nx : ny;
-    double *yb = (double *)malloc(sizeof(double) * mx);
-    double *cb = (double *)malloc(sizeof(double) * mx);
-    if (!s->zx || !s->zy || !s->zxy || !yb || !cb) { free(yb); free(cb); orc_bicubic_free(s); return NULL; }
-    for (size_t j = 0; j < ny; ++j) spline_derivs(nx, x, z + j * nx, 1, s->zx + j * nx, 1, yb, cb);
-    for (size_t i = 0; i < nx; ++i) spline_derivs(ny, y, z + i, nx, s->zy + i, nx, yb, cb);
-    for (size_t i = 0; i < nx; ++i) spline_derivs(ny, y, s->zx + i, nx, s->zxy + i, nx, yb, cb);
-    free(yb); free(cb);
+    if (!s->zx || !s->zy || !s->zxy) { orc_bicubic_free(s); return NULL; }
+    const size_t mx = nx > ny ? nx : ny;
+    int fail = 0;
+    /* every row / column spline is independent: OpenMP threads with buffers of their own */
+#pragma omp parallel
+    {
+        double *yb = (double *)malloc(sizeof(double) * mx);
+        double *cb = (double *)malloc(sizeof(double) * mx);
+        const int ok = yb && cb;
+        if (!ok) {
+#pragma omp atomic write
+            fail = 1;
+        }
+#pragma omp for schedule(static)
+        for (long j = 0; j < (long)ny; ++j)
+            if (ok) spline_derivs(nx, x, z + (size_t)j * nx, 1, s->zx + (size_t)j * nx, 1, yb, cb);
+#pragma omp for schedule(static)
+        for (long i = 0; i < (long)nx; ++i)
+            if (ok) spline_derivs(ny, y, z + i, nx, s->zy + i, nx, yb, cb);
+#pragma omp for schedule(static)
+        for (long i = 0; i < (long)nx; ++i)
+            if (ok) spline_derivs(ny, y, s->zx + i, nx, s->zxy + i, nx, yb, cb);
+        free(yb); free(cb);
+    }
+    if (fail) { orc_bicubic_free(s); return NULL; }
     return s;
 }
 

@@ -158,3 +158,61 @@ def test_cov_vjp_dense_shuffled_separations(engine):
     assert abs(lhs - rhs) <= 1e-12 * ref, (lhs, rhs)
     ps.close()
     pu.close()
+
+
+def _euler_and_affine(plan, k_np, p_np, seed, h=1e-3, affine_tol=1e-9):
+    """Per spectrum: the homogeneity identity sum gpk P = <G_1, I_1> + 2 <G_2, I_2> (1e-12)
+    and central differences along two affine directions a + b k (exact up to rounding)."""
+    import torch
+    dev = dict(device="cuda", dtype=torch.float64)
+    k = torch.tensor(k_np, **dev)
+    p = torch.tensor(p_np, **dev)
+    n_in = [len(k_np)]
+    G = torch.tensor(np.random.default_rng(seed).standard_normal((1, 2, plan.npair, plan.R, plan.R)), **dev)
+    I = plan.integrals(k, p, n_in)
+    g = plan.integrals_vjp(k, p, n_in, G).cpu().numpy()
+    assert np.all(np.isfinite(g))
+    lhs = float(g @ p_np)
+    rhs = (G[0, 0] * I[0, 0]).sum().item() + 2 * (G[0, 1] * I[0, 1]).sum().item()
+    ref = (torch.abs(G[0, 0] * I[0, 0]).sum() + 2 * torch.abs(G[0, 1] * I[0, 1]).sum()).item()
+    print(f"  homogeneity: {lhs:.15e} {rhs:.15e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * ref, (lhs, rhs)
+    rng = np.random.default_rng(seed + 1)
+    for trial in range(2):
+        a, b = rng.standard_normal(2)
+        v = np.max(p_np) * 0.05 * (a + b * k_np / k_np[-1])
+        ip = plan.integrals(k, torch.tensor(p_np + h * v, **dev), n_in)
+        im = plan.integrals(k, torch.tensor(p_np - h * v, **dev), n_in)
+        fd = ((G * ip).sum() - (G * im).sum()).item() / (2 * h)
+        an = float(g @ v)
+        scale = float(np.sum(np.abs(g * v)))
+        print(f"  affine {trial}: adjoint {an:.15e} fd {fd:.15e} scale {scale:.3e}", flush=True)
+        assert abs(an - fd) <= affine_tol * scale, (trial, an, fd, scale)
+
+
+def test_cov_vjp_n8000(engine):
+    """The adjoint at the covariance's default length N = 8000 (radix-10 transforms, about
+    1200 rows), l = 0, 2, 4, 16 separations 10-200 Mpc/h, the fixture spectrum."""
+    import os
+    from coffe_amd.covariance import CovPlan
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    k, pk = np.loadtxt(os.path.join(gold, "PkL_CLASS.dat")).T
+    k, pk = k * H, pk / H ** 3
+    sep = np.linspace(10.0, 200.0, 16) / H
+    plan = CovPlan(engine, 8000, k[0], k[-1], [0, 2, 4], sep)
+    print(f"cov vjp: N=8000 rows={plan.nrows}", flush=True)
+    _euler_and_affine(plan, k, pk, 21)
+    plan.close()
+
+
+def test_cov_vjp_more_than_256_separations(engine):
+    """300 separations (the transposed column functionals run over them in chunks of 256,
+    so the second chunk is exercised), N = 512, l = 0, 2."""
+    from coffe_amd.covariance import CovPlan
+    from tests.synth import spectrum
+    k_np, p_np = spectrum(2048)
+    sep = np.linspace(10.0, 250.0, 300) / H
+    plan = CovPlan(engine, 512, k_np[0], k_np[-1], [0, 2], sep)
+    print(f"cov vjp: 300 seps, N=512 rows={plan.nrows}", flush=True)
+    _euler_and_affine(plan, k_np, p_np, 31)
+    plan.close()

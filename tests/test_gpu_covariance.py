@@ -198,3 +198,28 @@ def test_cov_vs_reference_fixtures(engine):
             worst = max(worst, float(np.max(rel)))
             print(f"  l1={l1} l2={l2} max rel (of block max) {np.max(rel):.2e}", flush=True)
     assert worst < 2e-3
+
+
+def test_cov_integrals_n8000_vs_oracle(engine):
+    """The covariance's default transform length (covariance_integration_bins = 8000, the
+    bench's configuration) against the oracle's generic N x N transform and bicubic:
+    the fixture spectrum, 16 separations 10-200 Mpc/h, pairs (0, 2) and (4, 4), P and P^2,
+    1e-10 of each block's largest |value|."""
+    import torch
+    from coffe_amd.covariance import CovPlan
+    k, pk = _fixture_spectrum()
+    sep = np.linspace(10.0, 200.0, 16) / H
+    plan = CovPlan(engine, 8000, k[0], k[-1], LS, sep)
+    print(f"cov integrals N=8000 rows={plan.nrows}", flush=True)
+    got = plan.integrals(torch.tensor(k, device="cuda"), torch.tensor(pk, device="cuda"), [len(k)]).cpu().numpy()
+    plan.close()
+    for q, (i, j) in enumerate([(i, j) for i in range(3) for j in range(i, 3)]):
+        if (LS[i], LS[j]) not in ((0, 2), (4, 4)):
+            continue
+        for power in (1, 2):
+            print(f"  oracle N=8000 l=({LS[i]},{LS[j]}) power {power}", flush=True)
+            ref = _scaled(O.covariance_fftlog_integral(k, pk, float(power), LS[i], LS[j], 8000, sep), LS[i], LS[j],
+                          power)
+            err = np.max(np.abs(got[0, power - 1, q] - ref)) / np.max(np.abs(ref))
+            print(f"    max err (of block max) {err:.2e}", flush=True)
+            assert err <= 1e-10, (LS[i], LS[j], power, err)

@@ -20,7 +20,7 @@ def _rel(a, b, floor_frac=1e-6):
 
 # 6144 = 2^11 * 3 and 10000 = 2^4 * 5^4 take the radix-2, -3 and -5 passes of the transforms
 @pytest.mark.parametrize("N", [1024, 4096, 6144, 10000, 16384])
-@pytest.mark.parametrize("nl", [(0, 0), (0, 2), (2, 0), (3, 1), (0.5, -0.5)])
+@pytest.mark.parametrize("nl", [(0, 0), (0, 2), (0, 4), (2, 0), (3, 1), (0.5, -0.5)])
 def test_twofast_dropin(engine, N, nl):
     from coffe_amd.engine import twofast_1bessel
     nu, l = nl

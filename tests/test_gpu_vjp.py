@@ -101,3 +101,39 @@ def test_packed_workgroups_bit_identical(engine, monkeypatch):
     monkeypatch.delenv("COFFE_ENGINE_PACK")
     for x0, x1 in zip(*res):
         assert np.array_equal(x0, x1)
+
+
+@pytest.mark.parametrize("tables", [[(0, 0), (0, 2), (0, 4)], [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0),
+                                                               (2, 2), (3, 1)]])
+def test_vjp_exact_dot_product_white_noise(engine, tables):
+    """The adjoint along arbitrary (white-noise) directions v of the samples, with no finite
+    step: <gpk, v> must equal <gout, J v>, J v from the oracle's forward-mode derivative
+    (the tables of v, then the dual-number Akima lookups), to 1e-9 of the sum of the two
+    products' absolute terms (white noise puts intermediate FFT values far above the
+    outputs, so rounding alone reaches ~1e-10 of it)."""
+    import torch
+    from coffe_amd.engine import Batch
+    specs = [(s[0], s[1]) for s in batch_spectra(4, seed=7, lengths=(1024, 2048, 4096))]
+    sep = np.linspace(5, 300, 48) / 0.67
+    ntab = len(tables)
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((2, 3, ntab, len(sep)))
+    b = Batch(engine, [len(s[0]) for s in specs], [len(s[0]) for s in specs], [s[0][0] for s in specs],
+              [s[0][-1] for s in specs], tables, sep, 2, 3)
+    k = torch.tensor(np.concatenate([s[0] for s in specs]), device="cuda")
+    p = torch.tensor(np.concatenate([s[1] for s in specs]), device="cuda")
+    gout = rng.standard_normal((len(specs), 2, 3, len(sep)))
+    print(f"vjp dot product ntab={ntab}", flush=True)
+    _, _, gpk = b.vjp(k, p, torch.tensor(A, device="cuda"), torch.tensor(gout, device="cuda"))
+    gpk = gpk.cpu().numpy()
+    for trial in range(3):
+        dirs = [s[1] * rng.standard_normal(len(s[1])) for s in specs]
+        _, tang = O.tables_at_sep_jvp(specs, [len(s[0]) for s in specs], tables, sep, dirs)
+        Jv = np.einsum("zltr,str->szlr", A, tang)
+        lhs = float(np.sum(gout * Jv))
+        rhs = float(gpk @ np.concatenate(dirs))
+        scale = float(np.sum(np.abs(gout * Jv))) + float(np.sum(np.abs(gpk * np.concatenate(dirs))))
+        print(f"  trial {trial}: <gout, Jv> {lhs:.15e} <gpk, v> {rhs:.15e} scale {scale:.3e} "
+              f"rel {abs(lhs - rhs) / scale:.2e}", flush=True)
+        assert abs(lhs - rhs) <= 1e-9 * scale, (trial, lhs, rhs)
+    b.close()

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from tests.synth import batch_spectra
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 H = 0.67
@@ -151,3 +152,43 @@ def test_small_r_nodes_gaussian_closed_form():
     assert y[0] == 0.0
     m4 = 3 / (16 * np.pi ** 1.5 * R ** 5)          # (1/2pi^2) int k^4 P
     assert abs(y[1] / (1e-6 ** 2 * m4 / 15) - 1) < 1e-6
+
+
+def test_akima_jvp_against_differences_and_homogeneity():
+    """orc_akima_jvp: the derivative of the Akima interpolant in its node values, against
+    central differences on random node values (no weight |m_{i+1} - m_i| near zero, so the
+    map is smooth there) and, Akima being homogeneous of degree 1, dout = out along dy = y."""
+    rng = np.random.default_rng(41)
+    x = np.sort(rng.uniform(0, 10, 60))
+    y = rng.standard_normal(60)
+    xq = rng.uniform(x[0], x[-1], 200)
+    out, dout = O.akima_jvp(x, y, y, xq)
+    assert np.allclose(out, O.akima(x, y, xq), rtol=0, atol=1e-14)
+    assert np.allclose(dout, out, rtol=1e-13, atol=1e-14)
+    for trial in range(3):
+        dy = rng.standard_normal(60)
+        _, an = O.akima_jvp(x, y, dy, xq)
+        h = 1e-6
+        fd = (O.akima(x, y + h * dy, xq) - O.akima(x, y - h * dy, xq)) / (2 * h)
+        assert np.max(np.abs(an - fd)) <= 1e-7 * np.max(np.abs(fd)), trial
+
+
+def test_tables_jvp_linear_tables_and_differences():
+    """orc_tables_at_sep_jvp: along v = P the tangent equals the values (the tables are
+    linear in P and Akima homogeneous of degree 1); along a smooth direction it matches
+    central differences of orc_tables_at_sep_batch."""
+    specs = [(k, p) for k, p, _ in batch_spectra(2, seed=2, lengths=(1024, 2048))]
+    tables = [(0, 0), (0, 2), (1, 1), (2, 2)]
+    sep = np.linspace(8.0, 400.0, 30)
+    nfft = [len(s[0]) for s in specs]
+    vals, tang = O.tables_at_sep_jvp(specs, nfft, tables, sep, [s[1] for s in specs])
+    assert np.allclose(tang, vals, rtol=1e-12, atol=1e-14 * np.max(np.abs(vals)))
+    dirs = [s[1] * np.exp(-0.5 * ((np.log(s[0]) + 3.0) / 1.5) ** 2) for s in specs]
+    _, tang = O.tables_at_sep_jvp(specs, nfft, tables, sep, dirs)
+    h = 1e-6
+    plus = O.tables_at_sep_batch([(s[0], s[1] + h * d) for s, d in zip(specs, dirs)], nfft, tables, sep)
+    minus = O.tables_at_sep_batch([(s[0], s[1] - h * d) for s, d in zip(specs, dirs)], nfft, tables, sep)
+    fd = (plus - minus) / (2 * h)
+    for s in range(2):
+        for t in range(len(tables)):
+            assert np.max(np.abs(tang[s, t] - fd[s, t])) <= 1e-6 * np.max(np.abs(fd[s, t])), (s, t)

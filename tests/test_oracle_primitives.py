@@ -150,3 +150,89 @@ def test_bicubic_reproduces_bilinear_and_separable_cubic():
         x, y = RNG.uniform(xs[0], xs[-1]), RNG.uniform(ys[0], ys[-1])
         assert abs(L.orc_bicubic_eval(h, x, y) - (1 + 2 * x - y + 0.5 * x * y)) <= 1e-12
     L.orc_bicubic_free(h)
+
+
+def _bicubic(xs, ys, z):
+    import ctypes
+    L = O.lib()
+    L.orc_bicubic_new.restype = ctypes.c_void_p
+    L.orc_bicubic_new.argtypes = [ctypes.c_size_t, ctypes.c_size_t, O.c_double_p, O.c_double_p, O.c_double_p]
+    L.orc_bicubic_eval.restype = ctypes.c_double
+    L.orc_bicubic_eval.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+    L.orc_bicubic_free.argtypes = [ctypes.c_void_p]
+    zz = np.ascontiguousarray(z)
+    h = L.orc_bicubic_new(len(xs), len(ys), xs.ctypes.data_as(O.c_double_p), ys.ctypes.data_as(O.c_double_p),
+                          zz.ctypes.data_as(O.c_double_p))
+    return L, h, zz
+
+
+def test_bicubic_separable_equals_product_of_natural_splines():
+    """GSL's bicubic (common.c:579-613, 712-719) is a Hermite patch whose node derivatives
+    are natural-cubic-spline derivatives along x, along y and (cross) along both.  For
+    separable data g(x) h(y) those are S_g'(x_i) h_j, g_i S_h'(y_j) and S_g'(x_i) S_h'(y_j),
+    and a cubic Hermite interpolant with spline node slopes is the spline itself, so the
+    bicubic equals S_g(x) S_h(y) with S the natural cubic splines (scipy CubicSpline,
+    bc_type='natural') -- end conditions, slope formula and cell lookup included."""
+    from scipy.interpolate import CubicSpline
+    rng = np.random.default_rng(77)
+    xs = np.sort(rng.uniform(0.1, 3.0, 23))
+    ys = np.sort(rng.uniform(-1.0, 2.0, 17))
+    g = np.sin(2.3 * xs) + 0.3 * xs ** 2
+    h = np.exp(-0.7 * ys) * np.cos(1.9 * ys)
+    z = h[:, None] * g[None, :]          # z[j, i] = g(x_i) h(y_j)
+    L, hnd, zz = _bicubic(xs, ys, z)
+    Sg = CubicSpline(xs, g, bc_type="natural")
+    Sh = CubicSpline(ys, h, bc_type="natural")
+    pts = [(rng.uniform(xs[0], xs[-1]), rng.uniform(ys[0], ys[-1])) for _ in range(300)]
+    pts += [(xs[0], ys[0]), (xs[-1], ys[-1]), (xs[3], ys[5]), (0.5 * (xs[0] + xs[1]), ys[-1])]
+    for x, y in pts:
+        ref = float(Sg(x) * Sh(y))
+        assert abs(L.orc_bicubic_eval(hnd, x, y) - ref) <= 1e-12 * (1 + abs(ref)), (x, y)
+    L.orc_bicubic_free(hnd)
+
+
+def test_cspline_derivative_matches_scipy():
+    """orc_cspline_deriv (the natural spline's first derivative, the node slopes of the
+    bicubic) against scipy's CubicSpline(bc_type='natural').derivative(), at the nodes
+    and between them."""
+    from scipy.interpolate import CubicSpline
+    L = O.lib()
+    rng = np.random.default_rng(78)
+    x = np.sort(rng.uniform(0, 5, 40))
+    y = np.cos(1.7 * x) * np.exp(0.2 * x)
+    c = np.empty(len(x))
+    assert L.orc_cspline_init(len(x), x.ctypes.data_as(O.c_double_p), y.ctypes.data_as(O.c_double_p),
+                              c.ctypes.data_as(O.c_double_p)) == 0
+    dS = CubicSpline(x, y, bc_type="natural").derivative()
+    for xq in list(x) + list(rng.uniform(x[0], x[-1], 200)):
+        got = L.orc_cspline_deriv(len(x), x.ctypes.data_as(O.c_double_p), y.ctypes.data_as(O.c_double_p),
+                                  c.ctypes.data_as(O.c_double_p), float(xq))
+        assert abs(got - float(dS(xq))) <= 1e-12 * (1 + abs(float(dS(xq)))), xq
+
+
+def test_window_2d_closed_form():
+    """c_window_2d (utils.c:224-258) on an array of ones equals, entry by entry,
+    w_row(i) w_col(j) with w(t) = t - sin(2 pi t)/(2 pi): columns j >= N2/2 - Ncut2 carry
+    w((N2/2 - j)/Ncut2), rows N1/2 - a and N1/2 + 1 + a (0 <= a <= Ncut1) carry w(a/Ncut1),
+    Ncut = floor(N/2 * width); every other entry is 1 (so the Nyquist column and rows
+    N1/2, N1/2 + 1 vanish)."""
+    w = lambda t: t - np.sin(2 * np.pi * t) / (2 * np.pi)  # noqa: E731
+    for N1, N2, width in [(64, 48, 0.25), (40, 40, 0.25), (100, 64, 0.3)]:
+        h1, h2 = N1 // 2, N2 // 2
+        out = np.zeros((N1, h2 + 1, 2))
+        out[..., 0] = 1.0
+        out[..., 1] = -2.0
+        O.lib().orc_window_2d(out.ctypes.data_as(O.c_double_p), width, h1, h2)
+        c1, c2 = int(h1 * width), int(h2 * width)
+        wr = np.ones(N1)
+        for a in range(c1 + 1):
+            wr[h1 - a] = w(a / c1)
+            wr[h1 + 1 + a] = w(a / c1)
+        wc = np.ones(h2 + 1)
+        for j in range(h2 + 1):
+            if h2 - j <= c2:
+                wc[j] = w((h2 - j) / c2)
+        exp = wr[:, None] * wc[None, :]
+        assert np.allclose(out[..., 0], exp, rtol=0, atol=1e-15)
+        assert np.allclose(out[..., 1], -2.0 * exp, rtol=0, atol=1e-15)
+        assert np.all(out[:, h2, :] == 0) and np.all(out[h1, :, :] == 0) and np.all(out[h1 + 1, :, :] == 0)
