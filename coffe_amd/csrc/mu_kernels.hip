@@ -19,6 +19,7 @@
 #include "ce_internal.h"
 #include "coffe_engine.h"
 #include "ce_device.h"
+#include "fullsky_forms.h"
 
 namespace {
 
@@ -34,12 +35,7 @@ __device__ __forceinline__ double legendre_p(int l, double x)
     return p1;
 }
 
-// integer powers of the local terms (functions.c writes them as pow(x, n)): x^2 is the
-// correctly rounded square either way; x^3 and x^4 by products differ from a library pow
-// in the last bits, far below the FP64 cancellation noise of the full-sky terms
 __device__ __forceinline__ double sq(double x) { return x * x; }
-__device__ __forceinline__ double cube(double x) { return x * x * x; }
-__device__ __forceinline__ double p4(double x) { return sq(sq(x)); }
 
 #define CE_MU_MAXL 4
 #define CE_MU_NCOEF 8
@@ -171,14 +167,15 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
     for (int s = 0; s < NSTD; ++s) c[s] = 0.0;
     const double chi1 = chi_mean - sep * mu / 2.;
     const double chi2 = chi_mean + sep * mu / 2.;
-    const double costheta = (2. * chi_mean * chi_mean - sep * sep + mu * mu * sep * sep / 2.) /
-                            (2. * chi_mean * chi_mean - mu * mu * sep * sep / 2.);
-    const double ct2 = costheta * costheta;
-    // the flat-sky forms use the mean's background only
+    // the flat-sky forms use the mean's background only; the full-sky ones the end
+    // points' and the geometric coefficients of fullsky_forms.h (the reference's
+    // polynomials in cos theta with their (chi/s)^n cancellations done exactly)
     endpoint e1, e2;
+    ce_fullsky_geom g;
     if (!a.flatsky) {
         e1 = background_at(a, chi1, jbar, chib, dchi);
         e2 = background_at(a, chi2, jbar, chib, dchi);
+        g = ce_fullsky_geometry(chi_mean, sep, mu);
     } else {
         e1.z = e1.f = e1.H = e1.G1 = e1.G2 = e1.D1 = e1.b1 = e1.b2 = 0.0;
         e2 = e1;
@@ -196,12 +193,9 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
             c[S02] += -4 * fmean * fmean / 7. * P2;
             c[S04] += 8 * fmean * fmean / 35. * P4;
         } else {
-            c[S00] += f1 * f2 * (1 + 2 * ct2) / 15;
-            c[S02] += -f1 * f2 / 21. * ((1 + 11. * ct2) + 18 * costheta * (ct2 - 1) * chi1 * chi2 / sep / sep);
-            c[S04] += f1 * f2 *
-                      (4 * (3 * ct2 - 1) * (p4(chi1) + p4(chi2)) / 35. / p4(sep) +
-                       chi1 * chi2 * (3 + ct2) * (3 * (3 + ct2) * chi1 * chi2 - 8 * (chi1 * chi1 + chi2 * chi2) * costheta) /
-                           35. / p4(sep));
+            c[S00] += f1 * f2 * g.rr0;
+            c[S02] += f1 * f2 * g.rr2;
+            c[S04] += f1 * f2 * g.rr4;
         }
     }
     if (a.terms & COFFE_TERM_D1_D1) {   // functions.c:363-573
@@ -209,9 +203,8 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
             c[S20] += Hmean * Hmean * fmean * fmean * Gm1 * Gm2 / 3.;
             c[S22] += 2 * Hmean * Hmean * fmean * fmean * Gm1 * Gm2 * sq(sep) * P2 / 3.;
         } else {
-            c[S20] += H1 * H2 * f1 * f2 * G1 * G2 * costheta / 3.;
-            c[S22] += -H1 * H2 * f1 * f2 * G1 * G2 *
-                      ((chi2 - chi1 * costheta) * (chi1 - chi2 * costheta) + sq(sep) * costheta / 3.);
+            c[S20] += H1 * H2 * f1 * f2 * G1 * G2 * g.dd0;
+            c[S22] += H1 * H2 * f1 * f2 * G1 * G2 * g.dd2;
         }
     }
     if (a.terms & COFFE_TERM_DEN_RSD) {   // functions.c:670-771
@@ -220,15 +213,14 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
             c[S02] += -(2 * bm1 * fmean / 3. + 2 * bm2 * fmean / 3.) * P2;
         } else {
             c[S00] += b1 * f2 / 3. + b2 * f1 / 3.;
-            c[S02] += -(b1 * f2 * (2. / 3. - (1. - ct2) * sq(chi1 / sep)) +
-                        b2 * f1 * (2. / 3. - (1. - ct2) * sq(chi2 / sep)));
+            c[S02] += -(b1 * f2 * g.dr_a + b2 * f1 * g.dr_b);
         }
     }
     if (a.terms & COFFE_TERM_DEN_D1) {   // functions.c:772-855
         if (a.flatsky) {
             c[S11] += (bm1 * fmean * Hmean * Gm2 * mu + bm2 * fmean * Hmean * Gm1 * (-mu)) * sep;
         } else {
-            c[S11] += -(b1 * f2 * H2 * G2 * (chi1 * costheta - chi2) + b2 * f1 * H1 * G1 * (chi2 * costheta - chi1));
+            c[S11] += -(b1 * f2 * H2 * G2 * g.dd1_a + b2 * f1 * H1 * G1 * g.dd1_b);
         }
     }
     if (a.terms & COFFE_TERM_RSD_D1) {   // functions.c:904-1143
@@ -237,36 +229,31 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
             c[S11] += -3. / 5. * (fmean * fmean * Hmean * Gm2 * (-mu) + fmean * fmean * Hmean * Gm1 * mu) * sep;
             c[S13] += (fmean * fmean * Hmean * Gm2 * 2. / 5. * P3m + fmean * fmean * Hmean * Gm1 * 2. / 5. * P3p) * sep;
         } else {
-            c[S11] += f1 * f2 * H2 * G2 * ((1. + 2 * ct2) * chi2 - 3 * chi1 * costheta) / 5. +
-                      f2 * f1 * H1 * G1 * ((1. + 2 * ct2) * chi1 - 3 * chi2 * costheta) / 5.;
-            c[S13] += (f1 * f2 * H2 * G2 *
-                           ((1. - 3 * ct2) * cube(chi2) + costheta * (5. + ct2) * sq(chi2) * chi1 -
-                            2 * (2. + ct2) * chi2 * sq(chi1) + 2 * cube(chi1) * costheta) /
-                           5 +
-                       f2 * f1 * H1 * G1 *
-                           ((1. - 3 * ct2) * cube(chi1) + costheta * (5. + ct2) * sq(chi1) * chi2 -
-                            2 * (2. + ct2) * chi1 * sq(chi2) + 2 * cube(chi2) * costheta) /
-                           5) /
-                      sq(sep);
+            c[S11] += f1 * f2 * H2 * G2 * g.rd1_a + f2 * f1 * H1 * G1 * g.rd1_b;
+            c[S13] += f1 * f2 * H2 * G2 * g.rd3_a + f2 * f1 * H1 * G1 * g.rd3_b;
         }
     }
     const double G = a.flatsky ? Dmean * Dmean : e1.D1 * e2.D1;
     for (int s = 0; s < NSTD; ++s) c[s] *= G;
 }
 
-// one wavefront per (z, r); blockDim = 256 (four points)
-// held to 128 VGPRs (4 waves per SIMD, every (z, r) wavefront of the 8 x 512 grid
-// resident at once): the few values that spill cost less than the second round of
-// wavefronts that 224 registers (2 waves per SIMD) needed (measured -8 %)
-// NL = the number of multipoles the partial sums are kept for (>= a.nl), a template
-// argument so that the sums of absent multipoles take no registers
+// one wavefront per (z, r); blockDim = 256 (four points).  The lanes take 64 mu nodes
+// at a time; after each batch every (multipole, slot) sum is reduced across the
+// wavefront and added into the wavefront's LDS accumulators, so no per-lane partial
+// sums stay live over the node loop and the kernel fits 128 VGPRs (4 waves per SIMD,
+// every (z, r) wavefront of the 8 x 512 grid resident at once) without spills.
+// NL = the number of multipoles kept (>= a.nl), a template argument so that absent
+// multipoles cost nothing
 template <int NL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_mu_weights(mu_args a)
 {
+    __shared__ double sacc[4][CE_MU_MAXL * NSTD];
     const int lane = threadIdx.x & 63;
     const int point = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int npoint = a.nz * a.nsep;
     if (point >= npoint) return;   // whole wavefronts exit together
+    double *acc = sacc[threadIdx.x >> 6];
+    if (lane < NL * NSTD) acc[lane] = 0.0;
     const int iz = point / a.nsep, ir = point % a.nsep;
     const double zm = a.zmean[iz], sep = a.sep[ir];
     const double chi_mean = ce::akima_general(a.bg_z, a.bg_chi, a.nbg, zm);
@@ -279,37 +266,39 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     // slope of chi, from which the end points' chi intervals are extrapolated
     const int jbar = ce::interval_bisect(a.bg_z, 0, a.nbg - 1, zm);
     const double chib = a.bg_chi[jbar], dchi = a.bg_chi[jbar + 1] - a.bg_chi[jbar];
-    // partial sums per (multipole, standard table slot); fixed bounds keep them in registers
-    double acc[NL][NSTD];
-#pragma unroll
-    for (int l = 0; l < NL; ++l)
-#pragma unroll
-        for (int s = 0; s < NSTD; ++s) acc[l][s] = 0.0;
-    for (int q = lane; q < a.nmu; q += 64) {
-        const double mu = a.mu[q], w = a.wmu[q];
+    for (int q0 = 0; q0 < a.nmu; q0 += 64) {
+        const int q = q0 + lane;
         double c[NSTD];
-        local_coefficients(a, zm, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, jbar, chib, dchi, c);
+        double mu = 0.0, w = 0.0;
+        if (q < a.nmu) {
+            mu = a.mu[q];
+            w = a.wmu[q];
+            local_coefficients(a, zm, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, jbar, chib, dchi, c);
+        } else {
+#pragma unroll
+            for (int s = 0; s < NSTD; ++s) c[s] = 0.0;
+        }
 #pragma unroll
         for (int il = 0; il < NL; ++il) {
-            if (il < a.nl) {
-                const double pl = w * legendre_p(a.ls[il], mu);
+            if (il >= a.nl) break;
+            const double pl = w * legendre_p(a.ls[il], mu);
 #pragma unroll
-                for (int s = 0; s < NSTD; ++s) acc[il][s] += pl * c[s];
+            for (int s = 0; s < NSTD; ++s) {
+                double v = pl * c[s];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                if (lane == 0) acc[il * NSTD + s] += v;
             }
         }
     }
-#pragma unroll
-    for (int il = 0; il < NL; ++il) {
-        if (il >= a.nl) break;
-#pragma unroll
-        for (int s = 0; s < NSTD; ++s) {
-            double v = acc[il][s];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) {
-                for (int t = 0; t < a.ntab; ++t)
-                    if (a.slot[t] == s)
-                        a.A[(((int64_t)iz * a.nl + il) * a.ntab + t) * a.nsep + ir] = (2 * a.ls[il] + 1) * 0.5 * v;
-            }
+    // lane 0 wrote the sums; the other lanes read them after the wavefront's LDS writes
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): the LDS stores have completed
+    if (lane < NL * NSTD) {
+        const int il = lane / NSTD, s = lane % NSTD;
+        if (il < a.nl) {
+            const double v = acc[lane];
+            for (int t = 0; t < a.ntab; ++t)
+                if (a.slot[t] == s) a.A[(((int64_t)iz * a.nl + il) * a.ntab + t) * a.nsep + ir] = (2 * a.ls[il] + 1) * 0.5 * v;
         }
     }
 }

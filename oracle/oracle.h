@@ -97,6 +97,12 @@ int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const
                    const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int quad,
                    double epsrel, double *out);
 
+int orc_mu_weights_hp(size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                      const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                      const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                      const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, size_t nmu,
+                      const double *mu, const double *wmu, double *out);
+
 int orc_reference_batch(int nspec, const long long *off, const double *k, const double *p, const long long *nfft,
                         size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
                         const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,

@@ -82,6 +82,8 @@ def _declare(L):
     L.orc_tables_at_sep_jvp.argtypes = [i, ll, p, p, p, ll, i, ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int), s, p, p, p]
     L.orc_akima_jvp.argtypes = [s, p, p, p, s, p, p, p]
+    L.orc_mu_weights_hp.argtypes = [s, p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
+                                    ctypes.POINTER(ctypes.c_int), s, p, s, p, p, p]
     L.orc_qag_test.argtypes = [i, d, d, d, ctypes.POINTER(ctypes.c_int)]
     L.orc_qag_test.restype = d
 
@@ -389,3 +391,25 @@ def tables_at_sep_jvp(specs, nfft, tables, sep, dirs):
                                        _p(dout)), "tables_at_sep_jvp")
     shape = (len(specs), len(tables), len(sp))
     return out.reshape(shape), dout.reshape(shape)
+
+
+def mu_weights_hp(bg, bias_z, bias1, bias2, z_mean, ls, sep, den=True, rsd=True, d1=False, flatsky=False, nmu=256,
+                  nodes=None):
+    """The reference's local-term weights A[z][l][t][r] (t: the 8 standard tables)
+    evaluated in binary128 under an nmu-point Gauss-Legendre rule (or the given
+    (mu, w) nodes); see orc_mu_weights_hp."""
+    arrs = {kk: _f64(bg[kk]) for kk in ("z", "chi", "D1", "f", "cH", "G1", "G2")}
+    bz, b1, b2 = _f64(bias_z), _f64(bias1), _f64(bias2)
+    zm, sp = _f64(np.atleast_1d(z_mean)), _f64(np.atleast_1d(sep))
+    if nodes is None:
+        mu, w = np.polynomial.legendre.leggauss(nmu)
+    else:
+        mu, w = nodes
+    mu, w = _f64(mu), _f64(w)
+    lsa = (ctypes.c_int * len(ls))(*ls)
+    out = np.empty(len(zm) * len(ls) * 8 * len(sp))
+    _check(lib().orc_mu_weights_hp(len(arrs["z"]), _p(arrs["z"]), _p(arrs["chi"]), _p(arrs["D1"]), _p(arrs["f"]),
+                                   _p(arrs["cH"]), _p(arrs["G1"]), _p(arrs["G2"]), len(bz), _p(bz), _p(b1), _p(b2),
+                                   int(den), int(rsd), int(d1), int(flatsky), len(zm), _p(zm), len(ls), lsa, len(sp),
+                                   _p(sp), len(mu), _p(mu), _p(w), _p(out)), "mu_weights_hp")
+    return out.reshape(len(zm), len(ls), 8, len(sp))

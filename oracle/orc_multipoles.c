@@ -280,3 +280,162 @@ int orc_reference_batch(int nspec, const long long *off, const double *k, const 
     }
     return status;
 }
+
+/* ---- high-precision weights ---------------------------------------------------- */
+/*
+ * The mu-integrated weights of the local terms with the reference's formulas
+ * (functions.c:129-1143, full and flat sky, written as in the reference: polynomials in
+ * cos theta with their (chi/s)^n factors) evaluated in binary128 (__float128, 113-bit
+ * significand), so that the cancellations of those terms at s << chi (up to (chi/s)^4 ~
+ * 1e10) leave ~1e-24 of error instead of FP64's ~1e-6.  The background and bias values
+ * at the end points are the FP64 Akima values of the tables (smooth, no cancellation),
+ * then carried in binary128.  For the Gauss-Legendre rule (mu_q, w_q) on [-1, 1]:
+ *   A[z][l][t][r] = (2l+1)/2 sum_q w_q P_l(mu_q) G(mu_q) c_t(mu_q),
+ * t over the 8 standard tables (I00, I02, I04, I11, I13, I20, I22, I31), G = D1(z1)
+ * D1(z2) (full sky) or D1(z_mean)^2 (flat sky), functions.c:1457-1467.  terms: the
+ * den/rsd/d1 auto and cross terms as flags (den, rsd, d1 as in orc_multipoles).
+ */
+typedef __float128 qf;
+
+static qf qpl(int l, qf x)
+{
+    if (l == 0) return 1;
+    qf p0 = 1, p1 = x;
+    for (int k = 2; k <= l; ++k) {
+        const qf p2 = ((qf)(2 * k - 1) * x * p1 - (qf)(k - 1) * p0) / (qf)k;
+        p0 = p1;
+        p1 = p2;
+    }
+    return p1;
+}
+
+static void hp_coefficients(const mp_ctx *c, double z_mean, double mu_d, double sep_d, qf *out)
+{
+    for (int t = 0; t < NTAB; ++t) out[t] = 0;
+    const double chi_mean_d = ak(&c->chi, z_mean);
+    const qf chi_mean = chi_mean_d, mu = mu_d, sep = sep_d;
+    const qf chi1 = chi_mean - sep * mu / 2;
+    const qf chi2 = chi_mean + sep * mu / 2;
+    const qf costheta = (2 * chi_mean * chi_mean - sep * sep + mu * mu * sep * sep / 2) /
+                        (2 * chi_mean * chi_mean - mu * mu * sep * sep / 2);
+    /* the end points' redshifts from the FP64 chi1, chi2 the reference computes */
+    const double chi1_d = chi_mean_d - sep_d * mu_d / 2., chi2_d = chi_mean_d + sep_d * mu_d / 2.;
+    const double z1 = ak(&c->z_of_chi, chi1_d), z2 = ak(&c->z_of_chi, chi2_d);
+    const qf f1 = ak(&c->f, z1), f2 = ak(&c->f, z2), fmean = ak(&c->f, z_mean);
+    const qf H1 = ak(&c->cH, z1), H2 = ak(&c->cH, z2), Hmean = ak(&c->cH, z_mean);
+    const qf b1 = ak(&c->b1, z1), b2 = ak(&c->b2, z2);
+    const qf bm1 = ak(&c->b1, z_mean), bm2 = ak(&c->b2, z_mean);
+    const qf G1 = ak(&c->G1, z1), G2 = ak(&c->G2, z2);
+    const qf Gm1 = ak(&c->G1, z_mean), Gm2 = ak(&c->G2, z_mean);
+    const qf P2 = qpl(2, mu), P4 = qpl(4, mu);
+    const qf ct2 = costheta * costheta;
+    const qf s2 = sep * sep, s4 = s2 * s2;
+    if (c->den) {
+        if (c->flatsky) out[T00] += bm1 * bm2;
+        else out[T00] += b1 * b2;
+    }
+    if (c->rsd) {
+        if (c->flatsky) {
+            out[T00] += fmean * fmean / 5;
+            out[T02] += -4 * fmean * fmean / 7 * P2;
+            out[T04] += 8 * fmean * fmean / 35 * P4;
+        } else {
+            out[T00] += f1 * f2 * (1 + 2 * ct2) / 15;
+            out[T02] += -f1 * f2 / 21 * ((1 + 11 * ct2) + 18 * costheta * (ct2 - 1) * chi1 * chi2 / sep / sep);
+            out[T04] += f1 * f2 *
+                        (4 * (3 * ct2 - 1) * (chi1 * chi1 * chi1 * chi1 + chi2 * chi2 * chi2 * chi2) / 35 / s4 +
+                         chi1 * chi2 * (3 + ct2) * (3 * (3 + ct2) * chi1 * chi2 - 8 * (chi1 * chi1 + chi2 * chi2) * costheta) /
+                             35 / s4);
+        }
+    }
+    if (c->d1) {
+        if (c->flatsky) {
+            out[T20] += Hmean * Hmean * fmean * fmean * Gm1 * Gm2 / 3;
+            out[T22] += 2 * Hmean * Hmean * fmean * fmean * Gm1 * Gm2 * s2 * P2 / 3;
+        } else {
+            out[T20] += H1 * H2 * f1 * f2 * G1 * G2 * costheta / 3;
+            out[T22] += -H1 * H2 * f1 * f2 * G1 * G2 *
+                        ((chi2 - chi1 * costheta) * (chi1 - chi2 * costheta) + s2 * costheta / 3);
+        }
+    }
+    if (c->den && c->rsd) {
+        if (c->flatsky) {
+            out[T00] += bm1 * fmean / 3 + bm2 * fmean / 3;
+            out[T02] += -(2 * bm1 * fmean / 3 + 2 * bm2 * fmean / 3) * P2;
+        } else {
+            out[T00] += b1 * f2 / 3 + b2 * f1 / 3;
+            out[T02] += -(b1 * f2 * ((qf)2 / 3 - (1 - ct2) * (chi1 / sep) * (chi1 / sep)) +
+                          b2 * f1 * ((qf)2 / 3 - (1 - ct2) * (chi2 / sep) * (chi2 / sep)));
+        }
+    }
+    if (c->den && c->d1) {
+        if (c->flatsky) {
+            out[T11] += (bm1 * fmean * Hmean * Gm2 * mu + bm2 * fmean * Hmean * Gm1 * (-mu)) * sep;
+        } else {
+            out[T11] += -(b1 * f2 * H2 * G2 * (chi1 * costheta - chi2) + b2 * f1 * H1 * G1 * (chi2 * costheta - chi1));
+        }
+    }
+    if (c->rsd && c->d1) {
+        if (c->flatsky) {
+            const qf P3p = qpl(3, mu), P3m = qpl(3, -mu);
+            out[T11] += -(qf)3 / 5 * (fmean * fmean * Hmean * Gm2 * (-mu) + fmean * fmean * Hmean * Gm1 * mu) * sep;
+            out[T13] += (fmean * fmean * Hmean * Gm2 * 2 / 5 * P3m + fmean * fmean * Hmean * Gm1 * 2 / 5 * P3p) * sep;
+        } else {
+            out[T11] += f1 * f2 * H2 * G2 * ((1 + 2 * ct2) * chi2 - 3 * chi1 * costheta) / 5 +
+                        f2 * f1 * H1 * G1 * ((1 + 2 * ct2) * chi1 - 3 * chi2 * costheta) / 5;
+            out[T13] += (f1 * f2 * H2 * G2 *
+                             ((1 - 3 * ct2) * chi2 * chi2 * chi2 + costheta * (5 + ct2) * chi2 * chi2 * chi1 -
+                              2 * (2 + ct2) * chi2 * chi1 * chi1 + 2 * chi1 * chi1 * chi1 * costheta) /
+                             5 +
+                         f2 * f1 * H1 * G1 *
+                             ((1 - 3 * ct2) * chi1 * chi1 * chi1 + costheta * (5 + ct2) * chi1 * chi1 * chi2 -
+                              2 * (2 + ct2) * chi1 * chi2 * chi2 + 2 * chi2 * chi2 * chi2 * costheta) /
+                             5) /
+                        s2;
+        }
+    }
+    const qf G = c->flatsky ? (qf)ak(&c->D1, z_mean) * (qf)ak(&c->D1, z_mean) : (qf)ak(&c->D1, z1) * (qf)ak(&c->D1, z2);
+    for (int t = 0; t < NTAB; ++t) out[t] *= G;
+}
+
+int orc_mu_weights_hp(size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                      const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                      const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                      const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, size_t nmu,
+                      const double *mu, const double *wmu, double *out)
+{
+    mp_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.den = den; c.rsd = rsd; c.d1 = d1; c.flatsky = flatsky;
+    int st = ak_init(&c.chi, nbg, bg_z, bg_chi);
+    if (st == ORC_OK) st = ak_init(&c.z_of_chi, nbg, bg_chi, bg_z);
+    if (st == ORC_OK) st = ak_init(&c.D1, nbg, bg_z, bg_D1);
+    if (st == ORC_OK) st = ak_init(&c.f, nbg, bg_z, bg_f);
+    if (st == ORC_OK) st = ak_init(&c.cH, nbg, bg_z, bg_cH);
+    if (st == ORC_OK) st = ak_init(&c.G1, nbg, bg_z, bg_G1);
+    if (st == ORC_OK) st = ak_init(&c.G2, nbg, bg_z, bg_G2);
+    if (st == ORC_OK) st = ak_init(&c.b1, nb, zb, bias1);
+    if (st == ORC_OK) st = ak_init(&c.b2, nb, zb, bias2);
+    if (st == ORC_OK) {
+        #pragma omp parallel for schedule(dynamic)
+        for (size_t idx = 0; idx < nz * nsep; ++idx) {
+            const size_t iz = idx / nsep, ir = idx % nsep;
+            qf acc[4][NTAB];
+            memset(acc, 0, sizeof(acc));
+            for (size_t q = 0; q < nmu; ++q) {
+                qf cq[NTAB];
+                hp_coefficients(&c, z_mean[iz], mu[q], sep[ir], cq);
+                for (size_t il = 0; il < nl && il < 4; ++il) {
+                    const qf pw = (qf)wmu[q] * qpl(ls[il], (qf)mu[q]);
+                    for (int t = 0; t < NTAB; ++t) acc[il][t] += pw * cq[t];
+                }
+            }
+            for (size_t il = 0; il < nl && il < 4; ++il)
+                for (int t = 0; t < NTAB; ++t)
+                    out[((iz * nl + il) * NTAB + (size_t)t) * nsep + ir] = (double)((qf)(2 * ls[il] + 1) / 2 * acc[il][t]);
+        }
+    }
+    free(c.chi.b); free(c.z_of_chi.b); free(c.D1.b); free(c.f.b); free(c.cH.b); free(c.G1.b); free(c.G2.b);
+    free(c.b1.b); free(c.b2.b);
+    return st;
+}
