@@ -76,6 +76,8 @@ def load_library():
                                    ip, i, dp, i, vp, vp]
     L.coffe_mu_weights_terms.argtypes = [vp, i, dp, dp, dp, dp, dp, dp, dp, i, dp, dp, dp, ctypes.c_uint, i, i, dp, i,
                                          ip, i, ip, ip, i, dp, i, vp, vp]
+    L.coffe_mu_weights_ex.argtypes = [vp, vp, ctypes.c_uint, ctypes.c_uint, i, i, i, dp, i, ip, i, ip, ip, i, dp, i,
+                                      vp, vp, vp]
     L.coffe_integral_table.argtypes = [i, i, sz, dp, dp, sz, d, d, dp, dp]
     L.coffe_divergent_tables.argtypes = [dp, dp, sz, d, d, dp, sz, dp, sz, dp, dp, dp, ctypes.POINTER(i),
                                          ctypes.POINTER(i)]
@@ -115,7 +117,7 @@ EXPORTED = [
     "coffe_batch_vjp", "coffe_batch_tables", "coffe_batch_prepare", "coffe_batch_tables_stage",
     "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute", "coffe_background_jvp",
     "coffe_batch_resample", "coffe_batch_fft_prepare", "coffe_batch_adj_c2r", "coffe_batch_spline_adj",
-    "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_terms_of", "coffe_integral_table", "coffe_divergent_tables", "coffe_divergent_tables_device", "coffe_batch_set_radix", "coffe_batch_small_r",
+    "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_mu_weights_ex", "coffe_terms_of", "coffe_integral_table", "coffe_divergent_tables", "coffe_divergent_tables_device", "coffe_batch_set_radix", "coffe_batch_small_r",
     "coffe_cov_plan_create", "coffe_cov_plan_destroy", "coffe_cov_plan_rows", "coffe_cov_plan_pairs",
     "coffe_cov_integrals", "coffe_cov_integrals_vjp", "coffe_cov_assemble", "coffe_cov_assemble_vjp", "two_sph_bessel", "two_sph_bessel_status",
 ]

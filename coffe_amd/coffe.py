@@ -8,11 +8,10 @@ are validated as the reference validates them (TypeError / ValueError), lazily
 computed pieces are dropped when a parameter they depend on changes
 (coffe.pyx:948-993), and results come back as Multipoles / Covariance records in the
 reference's order (multipoles.c:76-86: z, l, r; covariance.c:1046-1060: z, l1, r1,
-l2, r2).  Contributions beyond density, RSD and Doppler-1 (lensing, d2, g1-g5) and
-the CLASS / corrfunc / redshift-averaged paths are not part of this engine and raise
-NotImplementedError; the divergent table those terms read, I_0^4, is available as
-integral(r=..., n=4, l=0), and its renormalisation grid through
-coffe_amd.engine.divergent_tables.
+l2, r2).  The local terms are density, RSD, Doppler-1, d2 and g1-g3 (with their cross
+terms); redshift-averaged multipoles (compute_average_multipoles_bulk) are computed for
+them too.  Lensing, g4, g5, the CLASS / corrfunc paths and the covariance of the averaged
+multipoles are not part of this engine and raise NotImplementedError.
 """
 import numpy as np
 
@@ -20,7 +19,9 @@ from . import background as _background
 from .representation import AverageMultipoles, Covariance, Multipoles
 
 TABLES = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]   # integrals.c:599-608
-_UNSUPPORTED = ("has_lensing", "has_d2", "has_g1", "has_g2", "has_g3", "has_g4", "has_g5")
+_UNSUPPORTED = ("has_lensing", "has_g4", "has_g5")
+_DG = ("d2", "g1", "g2", "g3")
+COFFE_H0 = 1 / 2997.92458   # h/Mpc (common.h)
 
 
 def _check_parameter(name, value, kind, xmin=None, xmax=None):
@@ -67,7 +68,8 @@ class Coffe:
         sep=5.0 + 3.0 * np.arange(100), l=[0, 2, 4], z_mean=[1.0], deltaz=[0.2], fsky=[1.0], pixelsize=[10.0],
         z_min=[0.9], z_max=[1.1],
         number_density1=[1e-3], number_density2=[1e-3],
-        has_density=True, has_rsd=False, has_d1=False, has_flatsky_local=False,
+        has_density=True, has_rsd=False, has_d1=False, has_d2=False, has_g1=False, has_g2=False, has_g3=False,
+        has_flatsky_local=False,
         integrals_fftlog_sampling=10000, covariance_integration_sampling=8000,
         covariance_cosmic=True, covariance_mixed=True, covariance_poisson=True,
         has_binned_covariance=False, covariance_populations=[1, 1, 1, 1],
@@ -78,6 +80,8 @@ class Coffe:
         self._p = {k: (np.array(v) if isinstance(v, (list, np.ndarray)) else v) for k, v in self._defaults.items()}
         zb = np.linspace(0.0, 15.0, 16)
         self._bias = {i: _Spline(zb, np.ones_like(zb)) for i in (1, 2, 3, 4)}
+        # magnification and evolution biases (parser.c:900-940: 0 by default)
+        self._mbias = {k: _Spline(zb, np.zeros_like(zb)) for k in ("s1", "s2", "fevo1", "fevo2")}
         self._pk = None
         self._engine = None
         self._free_all()
@@ -89,6 +93,7 @@ class Coffe:
         self._bg = None
         self._tables = None
         self._divergent = None
+        self._ren = None
         self._multipoles = None
         self._average_multipoles = None
         self._covariance = None
@@ -234,6 +239,10 @@ class Coffe:
     has_density = property(lambda s: s._p["has_density"], lambda s, v: s._set_flag("has_density", v))
     has_rsd = property(lambda s: s._p["has_rsd"], lambda s, v: s._set_flag("has_rsd", v))
     has_d1 = property(lambda s: s._p["has_d1"], lambda s, v: s._set_flag("has_d1", v))
+    has_d2 = property(lambda s: s._p["has_d2"], lambda s, v: s._set_flag("has_d2", v))
+    has_g1 = property(lambda s: s._p["has_g1"], lambda s, v: s._set_flag("has_g1", v))
+    has_g2 = property(lambda s: s._p["has_g2"], lambda s, v: s._set_flag("has_g2", v))
+    has_g3 = property(lambda s: s._p["has_g3"], lambda s, v: s._set_flag("has_g3", v))
     has_flatsky_local = property(lambda s: s._p["has_flatsky_local"],
                                  lambda s, v: s._set_flag("has_flatsky_local", v))
     covariance_cosmic = property(lambda s: s._p["covariance_cosmic"], lambda s, v: s._set_flag("covariance_cosmic", v))
@@ -252,13 +261,13 @@ class Coffe:
     def __setattr__(self, name, value):
         if name in _UNSUPPORTED:
             if value:
-                raise NotImplementedError(f"{name}: only density, RSD and Doppler-1 are computed by this engine")
+                raise NotImplementedError(f"{name}: lensing, g4 and g5 are not computed by this engine")
             return
         object.__setattr__(self, name, value)
 
     def reset_contributions(self):
         """Sets every has_* contribution to False."""
-        for name in ("has_density", "has_rsd", "has_d1"):
+        for name in ("has_density", "has_rsd", "has_d1", "has_d2", "has_g1", "has_g2", "has_g3"):
             setattr(self, name, False)
 
     # ---- inputs ----------------------------------------------------------------------
@@ -276,6 +285,7 @@ class Coffe:
         self._pk = (np.ascontiguousarray(k), np.ascontiguousarray(pk))
         self._tables = None
         self._divergent = None
+        self._ren = None
         self._free_results()
 
     @property
@@ -335,12 +345,65 @@ class Coffe:
     def galaxy_bias4(self, z):
         return self._eval_bias(4, z)
 
+    def _set_mbias(self, key, x, y):
+        x = np.asarray(x, dtype=np.float64)
+        y = np.asarray(y, dtype=np.float64)
+        if not np.allclose(x, np.sort(x)):
+            raise ValueError("The input array must be sorted")
+        if len(x) != len(y):
+            raise ValueError(f"Mismatching lengths for x ({len(x)}) and y ({len(y)})")
+        if len(x) < 5:
+            raise ValueError("At least 5 samples are needed")
+        self._mbias[key] = _Spline(x, y)
+        # the background's G1 / G2 read them (background.c:301-697)
+        self._bg = None
+        self._free_results()
+
+    def _eval_mbias(self, key, z):
+        s = self._mbias[key]
+        _check_parameter("x", z, (int, float), float(s.x[0]), float(s.x[-1]))
+        return float(s(float(z))[0])
+
+    def set_magnification_bias1(self, x_sampling, y_sampling):
+        """Magnification bias s of the first population (coffe.pyx:1837-1857)."""
+        self._set_mbias("s1", x_sampling, y_sampling)
+
+    def set_magnification_bias2(self, x_sampling, y_sampling):
+        self._set_mbias("s2", x_sampling, y_sampling)
+
+    def set_evolution_bias1(self, x_sampling, y_sampling):
+        """Evolution bias f_evo of the first population (coffe.pyx:1892-1957)."""
+        self._set_mbias("fevo1", x_sampling, y_sampling)
+
+    def set_evolution_bias2(self, x_sampling, y_sampling):
+        self._set_mbias("fevo2", x_sampling, y_sampling)
+
+    def magnification_bias1(self, z):
+        return self._eval_mbias("s1", z)
+
+    def magnification_bias2(self, z):
+        return self._eval_mbias("s2", z)
+
+    def evolution_bias1(self, z):
+        return self._eval_mbias("fevo1", z)
+
+    def evolution_bias2(self, z):
+        return self._eval_mbias("fevo2", z)
+
+    def _mbias_grid(self):
+        """The four magnification / evolution biases on their common redshift samples."""
+        x = self._mbias["s1"].x
+        if not all(np.array_equal(x, self._mbias[k].x) for k in ("s2", "fevo1", "fevo2")):
+            raise ValueError("the magnification and evolution biases must share their redshift sampling")
+        return x, [self._mbias[k].y for k in ("s1", "s2", "fevo1", "fevo2")]
+
     # ---- background -------------------------------------------------------------
     def _background(self):
         if self._bg is None:
+            mz, (s1, s2, e1, e2) = self._mbias_grid()
             self._bg = _background.compute(omega_cdm=self._p["omega_cdm"], omega_baryon=self._p["omega_baryon"],
                                            omega_gamma=self._p["omega_gamma"], w0=self._p["w0"], wa=self._p["wa"],
-                                           h=self._p["h"])
+                                           h=self._p["h"], bias_z=mz, s1=s1, s2=s2, fevo1=e1, fevo2=e2)
         return self._bg
 
     def _bg_eval(self, key, z):
@@ -372,7 +435,7 @@ class Coffe:
     def _bg_tables(self):
         bg = self._background()
         return dict(z=bg["z"], chi=bg["chi"], D1=bg["D1"], f=bg["f"], cH=bg["conformal_Hz"], G1=bg["G1"],
-                    G2=bg["G2"])
+                    G2=bg["G2"], a=bg["a"])
 
     # ---- integrals ----------------------------------------------------------------
     def _need_pk(self):
@@ -423,23 +486,66 @@ class Coffe:
             raise ValueError("At least one of the following parameters is empty: l, sep, z_mean")
 
     def _check_contributions(self):
-        if not (self._p["has_density"] or self._p["has_rsd"] or self._p["has_d1"]):
+        if not any(self._p[k] for k in ("has_density", "has_rsd", "has_d1", "has_d2", "has_g1", "has_g2", "has_g3")):
             raise ValueError("No contributions specified, you need to specify at least one of: "
-                             "'has_density', 'has_rsd', 'has_d1'")
+                             "'has_density', 'has_rsd', 'has_d1', 'has_d2', 'has_g1', 'has_g2', 'has_g3'")
 
-    def _multipole_values(self, zs, seps, ls):
-        """[z][l][r] multipoles: weights on the GPU (mu quadrature), tables at the separations."""
-        from .engine import mu_weights
+    def _dg_sources(self):
+        return {k for k in _DG if self._p["has_" + k]}
+
+    def _ren_grid(self, chi_max):
+        """The 201-node renormalisation grid of I_0^4 on [0, chi_max] (integrals.c:848-907:
+        chi_j = j / 200 chi_max), computed on the GPU for the current spectrum; cached."""
+        from .engine import divergent_tables
+        self._need_pk()
+        if self._ren is None or self._ren[0] != chi_max:
+            k, pk = self._pk
+            chi = np.array([j / 200 * chi_max for j in range(201)])
+            _, _, _, ren = divergent_tables(k, pk, sep=np.zeros(0), chi=chi)
+            self._ren = (chi_max, chi, ren)
+        return self._ren[1], self._ren[2]
+
+    def _weights_setup(self, chi_max):
+        """The tables the weights use and the d2 / g inputs (None without those terms)."""
+        srcs = self._dg_sources()
+        if not srcs:
+            return list(TABLES), None
+        mz, (s1, s2, e1, e2) = self._mbias_grid()
+        chi, ren = self._ren_grid(chi_max)
+        dg = dict(sources=srcs, H0h=COFFE_H0 * self._p["h"], Omega_m=self.omega_m, mb_z=mz, s1=s1, s2=s2, fevo1=e1,
+                  fevo2=e2, ren_chi=chi, ren=ren)
+        return list(TABLES) + [(4, 0)], dg
+
+    def _table_values(self, tables, seps):
+        """I_t(sep) for the tables (the divergent (4, 0) one from its own table)."""
         from .engine import akima as _ak
         tabs = self._integral_tables()
+        rows = []
+        for t in tables:
+            x, y = self._divergent_table() if t == (4, 0) else tabs[t]
+            rows.append(_ak(x, y, seps))
+        return np.array(rows)
+
+    def _multipole_values(self, zs, seps, ls):
+        """[z][l][r] multipoles: weights on the GPU (mu quadrature), tables at the separations
+        (with the d2 / g terms also the renormalisation grid's part, on the GPU)."""
+        import torch
+        from .engine import mu_weights
         bz = self._bias[1].x
         if not np.array_equal(bz, self._bias[2].x):
             raise ValueError("galaxy_bias1 and galaxy_bias2 must share their redshift sampling")
+        # the grid of the d2 / g terms spans chi(max(z_mean + deltaz)) (integrals.c:862-871)
+        zmax = max(float(z) + float(dz) for z, dz in zip(self._p["z_mean"], self._p["deltaz"]))
+        tables, dg = self._weights_setup(self.comoving_distance(zmax))
+        ren = torch.empty((len(zs), len(ls), len(seps)), dtype=torch.float64, device="cuda") if dg else None
         A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, zs, list(ls),
-                       TABLES, seps, den=self._p["has_density"], rsd=self._p["has_rsd"], d1=self._p["has_d1"],
-                       flatsky=self._p["has_flatsky_local"], nmu=self._p["multipole_quadrature_nodes"]).cpu().numpy()
-        I = np.array([_ak(tabs[t][0], tabs[t][1], seps) for t in TABLES])
-        return np.einsum("zltr,tr->zlr", A, I)
+                       tables, seps, den=self._p["has_density"], rsd=self._p["has_rsd"], d1=self._p["has_d1"],
+                       flatsky=self._p["has_flatsky_local"], nmu=self._p["multipole_quadrature_nodes"], dg=dg,
+                       ren_out=ren).cpu().numpy()
+        out = np.einsum("zltr,tr->zlr", A, self._table_values(tables, seps))
+        if ren is not None:
+            out += ren.cpu().numpy()
+        return out
 
     def compute_multipole(self, *, z, r, l, recompute=False):  # noqa: E741
         """The multipole at (z_mean, r, l) (coffe.pyx:3232-3300)."""
@@ -476,16 +582,14 @@ class Coffe:
         if np.any(np.asarray(zmin) >= np.asarray(zmax)):
             raise ValueError("every z_min must be below its z_max")
 
-    def _average_values(self, zmin, zmax, seps, ls, nodes=32, chunk=8):
+    def _average_values(self, zmin, zmax, seps, ls, nodes=32):
         """[l][r] multipoles averaged over the redshift bin (signal.c:249-311, 543-550): for
         each separation r, z runs over [z1, z2] with z1 = z(chi(zmin) + r/2) and
         z2 = z(chi(zmax) - r/2), and the average is int_0^1 dv M_l(z(v), r) h H0 / H(z(v)),
         z(v) = z1 + (z2 - z1) v (the reference's integrand, no further normalisation); the
         v integral is a Gauss-Legendre rule of `nodes` points, the M_l(z, r) the engine's
-        weights (GPU mu quadrature) at the (z, r) node pairs times the tables.  The weights
-        are computed for `chunk` separations at a time (all node redshifts of the chunk
-        times its separations, of which the matching pairs are kept)."""
-        from .engine import akima as _ak
+        weights (GPU mu quadrature) at exactly the (z(v), r) node pairs (paired mode)."""
+        import torch
         from .engine import mu_weights
         bg = self._background()
         z_of_chi = _Spline(bg["chi"], bg["z"])
@@ -499,25 +603,24 @@ class Coffe:
         if np.any(z2 <= z1):
             raise ValueError("a separation exceeds the comoving depth of the redshift bin")
         zn = z1[:, None] + (z2 - z1)[:, None] * v[None, :]                  # [r][v]
-        hH0 = self._p["h"] / 2997.92458
+        hH0 = self._p["h"] * COFFE_H0
         wz = w[None, :] * hH0 / hz(zn.ravel()).reshape(zn.shape)           # [r][v]
-        tabs = self._integral_tables()
         bz = self._bias[1].x
         if not np.array_equal(bz, self._bias[2].x):
             raise ValueError("galaxy_bias1 and galaxy_bias2 must share their redshift sampling")
-        Aavg = np.empty((len(ls), len(TABLES), len(seps)))
-        for c0 in range(0, len(seps), chunk):
-            sl = slice(c0, min(c0 + chunk, len(seps)))
-            nr = sl.stop - sl.start
-            A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y,
-                           zn[sl].ravel(), list(ls), TABLES, seps[sl], den=self._p["has_density"],
-                           rsd=self._p["has_rsd"], d1=self._p["has_d1"], flatsky=self._p["has_flatsky_local"],
-                           nmu=self._p["multipole_quadrature_nodes"]).cpu().numpy()   # [r*v][l][t][r']
-            A = A.reshape(nr, nodes, len(ls), len(TABLES), nr)
-            Ad = A[np.arange(nr), :, :, :, np.arange(nr)]                      # [r][v][l][t]
-            Aavg[:, :, sl] = np.einsum("rv,rvlt->ltr", wz[sl], Ad)
-        I = np.array([_ak(tabs[t][0], tabs[t][1], seps) for t in TABLES])
-        return np.einsum("ltr,tr->lr", Aavg, I)
+        # the d2 / g grid spans chi(max(z_max)) for averaged multipoles (integrals.c:873-879)
+        tables, dg = self._weights_setup(self.comoving_distance(float(max(self._p["z_max"]))))
+        npair = len(seps) * nodes
+        ren = torch.empty((npair, len(ls)), dtype=torch.float64, device="cuda") if dg else None
+        A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, zn.ravel(),
+                       list(ls), tables, np.repeat(seps, nodes), den=self._p["has_density"], rsd=self._p["has_rsd"],
+                       d1=self._p["has_d1"], flatsky=self._p["has_flatsky_local"],
+                       nmu=self._p["multipole_quadrature_nodes"], dg=dg, ren_out=ren, paired=True).cpu().numpy()
+        A = A.reshape(len(seps), nodes, len(ls), len(tables))                 # [r][v][l][t]
+        out = np.einsum("rv,rvlt,tr->lr", wz, A, self._table_values(tables, seps))
+        if ren is not None:
+            out += np.einsum("rv,rvl->lr", wz, ren.cpu().numpy().reshape(len(seps), nodes, len(ls)))
+        return out
 
     def compute_average_multipoles_bulk(self, recompute=False):
         """Redshift-averaged multipoles at every (z_min/z_max bin, l, sep), bin-major then

@@ -166,12 +166,28 @@ int ce_launch_cov_assemble_vjp(void *stream, int nspec, int nz, int nl, int R, i
                                const double *d_gout, double *d_ginteg, double *d_gcoef);
 
 /* ---- mu-quadrature weights (mu_kernels.hip) ---- */
-/* coef: workspace of 3 (6 nbg + 2 nb) doubles for the end-point Akima coefficients */
-int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
-                         const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, int nb,
-                         const double *bias_z, const double *bias1, const double *bias2, unsigned terms,
-                         int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *slot,
-                         int nsep, const double *sep, int nmu, const double *mu, const double *wmu, double *A,
+/* device inputs: background tables on bg_z (a: scale factor, d2 / g terms only), galaxy
+   biases on bias_z, magnification and evolution biases on mb_z (d2 / g terms), and the
+   renormalisation grid of I_0^4 with the bicubic's node derivatives ([b nren + a] at
+   (ren_chi[a], ren_chi[b]); d2 / g auto terms) */
+typedef struct ce_mu_dev {
+    int nbg;
+    const double *bg_z, *bg_chi, *bg_D1, *bg_f, *bg_cH, *bg_G1, *bg_G2, *bg_a;
+    int nb;
+    const double *bias_z, *bias1, *bias2;
+    int nmb;
+    const double *mb_z, *s1, *s2, *fevo1, *fevo2;
+    double H0h, Om;
+    int nren;
+    const double *ren_chi, *ren_v, *ren_x, *ren_y, *ren_xy;
+} ce_mu_dev;
+/* coef: workspace of 3 (7 nbg + 2 nb + 4 nmb) doubles for the Akima coefficients; dg: the
+   d2 / g sources (bit 0 d2, 1 g1, 2 g2, 3 g3); paired: point i = (zmean[i], sep[i]) with
+   nz == nsep, A [nz][nl][ntab] and ren_out [nz][nl]; else A [nz][nl][ntab][nsep] and
+   ren_out [nz][nl][nsep] (ren_out may be NULL) */
+int ce_launch_mu_weights(void *stream, const ce_mu_dev *in, unsigned terms, unsigned dg, int flatsky, int paired,
+                         int nz, const double *zmean, int nl, const int *ls, int ntab, const int *slot, int nsep,
+                         const double *sep, int nmu, const double *mu, const double *wmu, double *A, double *ren_out,
                          double *coef);
 
 #ifdef __cplusplus

@@ -1042,47 +1042,118 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
                            const int *tab_n, const int *tab_l, int nsep, const double *sep, int nmu, double *d_A,
                            void *stream)
 {
-    if (!e || nbg < 5 || nb < 5 || nz <= 0 || nl <= 0 || nl > 4 || ntab <= 0 || ntab > CE_MAX_TABLES || nsep <= 0 ||
-        nmu < 2 || terms == 0 || terms > 63 || !d_A || !bg_z || !bg_chi || !bg_D1 || !bg_f || !bg_cH || !bg_G1 || !bg_G2 || !bias_z || !bias1 ||
-        !bias2 || !zmean || !ls || !tab_n || !tab_l || !sep)
+    if (terms > 63) return COFFE_EINVAL;   /* the d2 / g terms need coffe_mu_weights_ex */
+    coffe_mu_inputs in;
+    memset(&in, 0, sizeof(in));
+    in.nbg = nbg; in.bg_z = bg_z; in.bg_chi = bg_chi; in.bg_D1 = bg_D1; in.bg_f = bg_f; in.bg_cH = bg_cH;
+    in.bg_G1 = bg_G1; in.bg_G2 = bg_G2;
+    in.nb = nb; in.bias_z = bias_z; in.bias1 = bias1; in.bias2 = bias2;
+    return coffe_mu_weights_ex(e, &in, terms, 0, flatsky, 0, nz, zmean, nl, ls, ntab, tab_n, tab_l, nsep, sep, nmu,
+                               d_A, NULL, stream);
+}
+
+/* node slopes of the natural cubic spline through (x_i, y_i), i < n (the derivative
+   tables of GSL's bicubic): second derivatives M from the tridiagonal system with M_0 =
+   M_{n-1} = 0 (Thomas elimination), then y'_i from the interval formulas; y and out
+   are strided, work holds 2 n doubles */
+static void spline_node_slopes(int n, const double *x, const double *y, int ys, double *out, int os, double *work)
+{
+    double *M = work, *cp = work + n;
+    M[0] = 0.0;
+    cp[0] = 0.0;
+    /* forward elimination of h_{i-1} M_{i-1} + 2 (h_{i-1} + h_i) M_i + h_i M_{i+1} = r_i */
+    for (int i = 1; i < n - 1; ++i) {
+        const double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+        const double r = 6.0 * ((y[(i + 1) * ys] - y[i * ys]) / h1 - (y[i * ys] - y[(i - 1) * ys]) / h0);
+        const double den = 2.0 * (h0 + h1) - h0 * cp[i - 1];
+        cp[i] = h1 / den;
+        M[i] = (r - h0 * M[i - 1]) / den;
+    }
+    M[n - 1] = 0.0;
+    for (int i = n - 2; i >= 1; --i) M[i] -= cp[i] * M[i + 1];
+    for (int i = 0; i < n - 1; ++i) {
+        const double h = x[i + 1] - x[i];
+        out[i * os] = (y[(i + 1) * ys] - y[i * ys]) / h - h * (2.0 * M[i] + M[i + 1]) / 6.0;
+    }
+    const double h = x[n - 1] - x[n - 2];
+    out[(n - 1) * os] = (y[(n - 1) * ys] - y[(n - 2) * ys]) / h + h * (M[n - 2] + 2.0 * M[n - 1]) / 6.0;
+}
+
+int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned terms, unsigned sources, int flatsky,
+                        int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
+                        const int *tab_l, int nsep, const double *sep, int nmu, double *d_A, double *d_ren,
+                        void *stream)
+{
+    if (!e || !in || nz <= 0 || nl <= 0 || nl > 4 || ntab <= 0 || ntab > CE_MAX_TABLES + 1 || nsep <= 0 || nmu < 2 ||
+        terms == 0 || terms > 1023 || !d_A || !zmean || !ls || !tab_n || !tab_l || !sep || (paired && nz != nsep))
         return COFFE_EINVAL;
-    int slot[CE_MAX_TABLES], have[8] = {0};
+    const int nbg = in->nbg, nb = in->nb;
+    if (nbg < 5 || nb < 5 || !in->bg_z || !in->bg_chi || !in->bg_D1 || !in->bg_f || !in->bg_cH || !in->bg_G1 ||
+        !in->bg_G2 || !in->bias_z || !in->bias1 || !in->bias2)
+        return COFFE_EINVAL;
+    const unsigned dgterms = COFFE_TERM_DG_DG | COFFE_TERM_DEN_DG | COFFE_TERM_RSD_DG | COFFE_TERM_D1_DG;
+    const int dg = (terms & dgterms) != 0;
+    if (dg != (sources != 0) || sources > 15) return COFFE_EINVAL;
+    const int nmb = dg ? in->nmb : 0;
+    if (dg && (nmb < 5 || !in->bg_a || !in->mb_z || !in->s1 || !in->s2 || !in->fevo1 || !in->fevo2 ||
+               !(in->H0h > 0)))
+        return COFFE_EINVAL;
+    const int ren = (terms & COFFE_TERM_DG_DG) != 0;
+    const int nren = ren ? in->nren : 0;
+    if (ren && (nren < 5 || !in->ren_chi || !in->ren || !d_ren)) return COFFE_EINVAL;
+    int slot[CE_MAX_TABLES + 1], have[9] = {0};
     for (int t = 0; t < ntab; ++t) {
-        slot[t] = std_slot(tab_n[t], tab_l[t]);
+        slot[t] = (tab_n[t] == 4 && tab_l[t] == 0) ? 8 : std_slot(tab_n[t], tab_l[t]);
         if (slot[t] < 0) return COFFE_EINVAL;
         have[slot[t]] = 1;
     }
-    /* the tables each contribution needs (functions.c:129-1143) */
+    /* the tables each contribution needs (functions.c:129-1428) */
     if ((terms & COFFE_TERM_DEN_DEN) && !have[0]) return COFFE_EINVAL;
     if ((terms & COFFE_TERM_RSD_RSD) && !(have[0] && have[1] && have[2])) return COFFE_EINVAL;
     if ((terms & COFFE_TERM_D1_D1) && !(have[5] && have[6])) return COFFE_EINVAL;
     if ((terms & COFFE_TERM_DEN_RSD) && !(have[0] && have[1])) return COFFE_EINVAL;
     if ((terms & COFFE_TERM_DEN_D1) && !have[3]) return COFFE_EINVAL;
     if ((terms & COFFE_TERM_RSD_D1) && !(have[3] && have[4])) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_DG_DG) && !have[8]) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_DEN_DG) && !have[5]) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_RSD_DG) && !(have[5] && have[6])) return COFFE_EINVAL;
+    if ((terms & COFFE_TERM_D1_DG) && !have[7]) return COFFE_EINVAL;
     for (int i = 0; i < nl; ++i)
         if (ls[i] < 0) return COFFE_EINVAL;
-    /* chi(z_mean) -/+ sep/2 must stay inside the chi table at every |mu| <= 1 and
-       z_mean inside the bias table (the reference stops on a GSL domain error) */
+    /* chi(z_mean) -/+ sep/2 must stay inside the chi table (and the renormalisation grid)
+       at every |mu| <= 1 and z_mean inside the bias tables (the reference stops on a GSL
+       domain error) */
     for (int iz = 0; iz < nz; ++iz)
-        if (zmean[iz] < bg_z[0] || zmean[iz] > bg_z[nbg - 1] || zmean[iz] < bias_z[0] || zmean[iz] > bias_z[nb - 1])
+        if (zmean[iz] < in->bg_z[0] || zmean[iz] > in->bg_z[nbg - 1] || zmean[iz] < in->bias_z[0] ||
+            zmean[iz] > in->bias_z[nb - 1])
             return COFFE_EDOMAIN;
+    for (int r = 0; r < nsep; ++r)
+        if (!(sep[r] > 0.0)) return COFFE_EDOMAIN;
     double *chim = (double *)malloc(sizeof(double) * (size_t)nz);
     if (!chim) return COFFE_ENOMEM;
-    if (coffe_akima_eval((size_t)nbg, bg_z, bg_chi, (size_t)nz, zmean, chim) != COFFE_OK) { free(chim); return COFFE_EINVAL; }
-    double smax = 0.0;
-    for (int r = 0; r < nsep; ++r) {
-        if (!(sep[r] > 0.0)) { free(chim); return COFFE_EDOMAIN; }
-        if (sep[r] > smax) smax = sep[r];
+    if (coffe_akima_eval((size_t)nbg, in->bg_z, in->bg_chi, (size_t)nz, zmean, chim) != COFFE_OK) {
+        free(chim);
+        return COFFE_EINVAL;
     }
-    for (int iz = 0; iz < nz; ++iz)
-        if (!(chim[iz] - smax / 2 >= bg_chi[0] && chim[iz] + smax / 2 <= bg_chi[nbg - 1])) { free(chim); return COFFE_EDOMAIN; }
+    double smax = 0.0;
+    for (int r = 0; r < nsep; ++r) smax = sep[r] > smax ? sep[r] : smax;
+    const double lo = ren ? fmax(in->bg_chi[0], in->ren_chi[0]) : in->bg_chi[0];
+    const double hi = ren ? fmin(in->bg_chi[nbg - 1], in->ren_chi[nren - 1]) : in->bg_chi[nbg - 1];
+    for (int iz = 0; iz < nz; ++iz) {
+        const double s = paired ? sep[iz] : smax;
+        if (!(chim[iz] - s / 2 >= lo && chim[iz] + s / 2 <= hi)) { free(chim); return COFFE_EDOMAIN; }
+    }
     free(chim);
     hipSetDevice(e->device);
     hipStream_t s = stream ? (hipStream_t)stream : e->stream;
-    const size_t nbytes_bg = sizeof(double) * (size_t)nbg, nbytes_b = sizeof(double) * (size_t)nb;
-    const size_t ncoef = 3 * (6 * (size_t)nbg + 2 * (size_t)nb);
-    const size_t total = 7 * nbytes_bg + 3 * nbytes_b + sizeof(double) * ((size_t)nz + (size_t)nsep + 2 * (size_t)nmu) +
-                         sizeof(double) * ncoef;
+    const size_t nnode = (size_t)nren * (size_t)nren;
+    /* host arrays staged in one pinned buffer: 8 background tables, 3 bias tables, the 5
+       magnification / evolution tables, zmean, sep, the rule (2 nmu), the renormalisation
+       chi grid and its values with their three derivative tables */
+    const size_t nstage = 8 * (size_t)nbg + 3 * (size_t)nb + 5 * (size_t)nmb + (size_t)nz + (size_t)nsep +
+                          2 * (size_t)nmu + (size_t)nren + 4 * nnode;
+    const size_t ncoef = 3 * (7 * (size_t)nbg + 2 * (size_t)nb + 4 * (size_t)nmb);
+    const size_t total = sizeof(double) * (nstage + ncoef);
     int owned = 0;
     const double *h_mu = gl_nodes(nmu, &owned);
     if (!h_mu) return COFFE_ENOMEM;
@@ -1099,8 +1170,6 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
         st = hip_ok(hipEventSynchronize(e->ev_stage[sb]));
         e->stage_pending[sb] = 0;
     }
-    /* every host input goes through one pinned staging buffer and one copy */
-    const size_t nstage = total / sizeof(double) - ncoef;   /* everything but the coefficient workspace */
     if (st == COFFE_OK && e->pinned_bytes[sb] < sizeof(double) * nstage) {
         hipHostFree(e->h_pinned[sb]);
         e->h_pinned[sb] = NULL;
@@ -1110,29 +1179,80 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
         else
             st = COFFE_ENOMEM;
     }
-    double *p = (double *)e->d_scratch;
-    const double *src[10] = {bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, bias_z, bias1, bias2};
-    double *dst[10];
-    double *hp = (double *)e->h_pinned[sb];
-    for (int i = 0; i < 10 && st == COFFE_OK; ++i) {
-        const size_t nv = (i < 7 ? nbytes_bg : nbytes_b) / sizeof(double);
-        dst[i] = p;
-        memcpy(hp, src[i], sizeof(double) * nv);
-        p += nv;
-        hp += nv;
-    }
-    double *d_z = p, *d_sep = p + nz, *d_mu = d_sep + nsep, *d_w = d_mu + nmu;
+    ce_mu_dev d;
+    memset(&d, 0, sizeof(d));
+    double *dp = (double *)e->d_scratch, *hp = (double *)e->h_pinned[sb];
+    /* stage n values of src (or zeros when src is NULL), return the device copy */
+#define CE_STAGE(src, n)                                                                 \
+    ({                                                                                   \
+        const size_t nn_ = (size_t)(n);                                                  \
+        if ((src)) memcpy(hp, (src), sizeof(double) * nn_);                              \
+        else memset(hp, 0, sizeof(double) * nn_);                                        \
+        const double *r_ = dp;                                                           \
+        dp += nn_;                                                                       \
+        hp += nn_;                                                                       \
+        r_;                                                                              \
+    })
+    double *d_z = NULL, *d_sep = NULL, *d_mu = NULL;
     if (st == COFFE_OK) {
-        memcpy(hp, zmean, sizeof(double) * (size_t)nz);
-        memcpy(hp + nz, sep, sizeof(double) * (size_t)nsep);
-        memcpy(hp + nz + nsep, h_mu, sizeof(double) * 2 * (size_t)nmu);
+        d.nbg = nbg;
+        d.bg_z = CE_STAGE(in->bg_z, nbg);
+        d.bg_chi = CE_STAGE(in->bg_chi, nbg);
+        d.bg_D1 = CE_STAGE(in->bg_D1, nbg);
+        d.bg_f = CE_STAGE(in->bg_f, nbg);
+        d.bg_cH = CE_STAGE(in->bg_cH, nbg);
+        d.bg_G1 = CE_STAGE(in->bg_G1, nbg);
+        d.bg_G2 = CE_STAGE(in->bg_G2, nbg);
+        d.bg_a = CE_STAGE(dg ? in->bg_a : NULL, nbg);
+        d.nb = nb;
+        d.bias_z = CE_STAGE(in->bias_z, nb);
+        d.bias1 = CE_STAGE(in->bias1, nb);
+        d.bias2 = CE_STAGE(in->bias2, nb);
+        d.nmb = nmb;
+        if (dg) {
+            d.mb_z = CE_STAGE(in->mb_z, nmb);
+            d.s1 = CE_STAGE(in->s1, nmb);
+            d.s2 = CE_STAGE(in->s2, nmb);
+            d.fevo1 = CE_STAGE(in->fevo1, nmb);
+            d.fevo2 = CE_STAGE(in->fevo2, nmb);
+        }
+        d.H0h = in->H0h;
+        d.Om = in->Omega_m;
+        d_z = (double *)CE_STAGE(zmean, nz);
+        d_sep = (double *)CE_STAGE(sep, nsep);
+        d_mu = (double *)CE_STAGE(h_mu, 2 * (size_t)nmu);
+        if (ren) {
+            d.nren = nren;
+            d.ren_chi = CE_STAGE(in->ren_chi, nren);
+            /* the values and GSL bicubic's derivative tables: natural-spline slopes along
+               x (each row), along y (each column), and along y of the x slopes */
+            double *hv = hp, *hx = hp + nnode, *hy = hp + 2 * nnode, *hxy = hp + 3 * nnode;
+            double *work = (double *)malloc(sizeof(double) * 2 * (size_t)nren);
+            if (!work) st = COFFE_ENOMEM;
+            if (st == COFFE_OK) {
+                memcpy(hv, in->ren, sizeof(double) * nnode);
+                for (int j = 0; j < nren; ++j) spline_node_slopes(nren, in->ren_chi, hv + (size_t)j * nren, 1, hx + (size_t)j * nren, 1, work);
+                for (int i = 0; i < nren; ++i) spline_node_slopes(nren, in->ren_chi, hv + i, nren, hy + i, nren, work);
+                for (int i = 0; i < nren; ++i) spline_node_slopes(nren, in->ren_chi, hx + i, nren, hxy + i, nren, work);
+            }
+            free(work);
+            d.ren_v = dp;
+            d.ren_x = dp + nnode;
+            d.ren_y = dp + 2 * nnode;
+            d.ren_xy = dp + 3 * nnode;
+            dp += 4 * nnode;
+            hp += 4 * nnode;
+        }
+    }
+#undef CE_STAGE
+    if (st == COFFE_OK) {
         st = hip_ok(hipMemcpyAsync(e->d_scratch, e->h_pinned[sb], sizeof(double) * nstage, hipMemcpyHostToDevice, s));
         if (st == COFFE_OK && hip_ok(hipEventRecord(e->ev_stage[sb], s)) == COFFE_OK) e->stage_pending[sb] = 1;
     }
+    double *d_coef = (double *)e->d_scratch + nstage;
     if (st == COFFE_OK &&
-        ce_launch_mu_weights(s, nbg, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], nb, dst[7], dst[8], dst[9],
-                             terms, flatsky, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu, d_mu, d_w, d_A,
-                             d_w + nmu))
+        ce_launch_mu_weights(s, &d, terms, sources, flatsky, paired, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu,
+                             d_mu, d_mu + nmu, d_A, ren ? d_ren : NULL, d_coef))
         st = COFFE_EDEVICE;
     if (st == COFFE_OK) {
         st = hip_ok(hipEventRecord(e->ev_scratch, s));

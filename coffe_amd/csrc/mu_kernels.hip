@@ -16,6 +16,8 @@
 // around the interval with GSL's end-slope extension.
 #include <hip/hip_runtime.h>
 
+#include <string.h>
+
 #include "ce_internal.h"
 #include "coffe_engine.h"
 #include "ce_device.h"
@@ -38,31 +40,44 @@ __device__ __forceinline__ double legendre_p(int l, double x)
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
 #define CE_MU_MAXL 4
-#define CE_MU_NCOEF 8
+// Akima coefficient tables: z(chi); f, conformal H, G1, G2, D1, a on the z grid; bias1,
+// bias2 on bias_z; s1, s2, f_evo1, f_evo2 on the magnification grid (the last five are
+// built only for the d2 / g terms)
+#define CE_MU_NCOEF 13
+enum { K_ZCHI = 0, K_F, K_H, K_G1, K_G2, K_D1, K_B1, K_B2, K_A, K_S1, K_S2, K_E1, K_E2 };
 
-// standard table slots (integrals.c:599-608)
-enum { S00 = 0, S02, S04, S11, S13, S20, S22, S31, NSTD };
+// table slots: the 8 standard tables (integrals.c:599-608), the divergent I_0^4
+// (integrals.c:772-834), then the accumulator of the renormalisation term
+enum { S00 = 0, S02, S04, S11, S13, S20, S22, S31, S40, NSTD, SREN = NSTD, NACC };
 
 struct mu_args {
     int nbg;
-    const double *bg_z, *bg_chi, *bg_D1, *bg_f, *bg_cH, *bg_G1, *bg_G2;
+    const double *bg_z, *bg_chi, *bg_D1, *bg_f, *bg_cH, *bg_G1, *bg_G2, *bg_a;
     int nb;
     const double *bias_z, *bias1, *bias2;
+    int nmb;                   // magnification / evolution biases (d2 / g terms)
+    const double *mb_z, *s1, *s2, *fevo1, *fevo2;
+    double H0h, Om;            // COFFE_H0 h (1/Mpc), Omega0_cdm + Omega0_baryon
     unsigned terms;            // COFFE_TERM_* mask
+    unsigned dg;               // sources of the d2 / g terms: bit 0 d2, 1 g1, 2 g2, 3 g3
     int flatsky;
+    int paired;                // point i is (zmean[i], sep[i]) instead of the z x r grid
     int nz;
     const double *zmean;
     int nl;
     int ls[CE_MAX_TABLES];
     int ntab;
-    int slot[CE_MAX_TABLES];   // standard slot of output table t
+    int slot[CE_MAX_TABLES + 1];   // slot of output table t (the 8 standard ones and I_0^4)
     int nsep;
     const double *sep;
     int nmu;
     const double *mu, *wmu;
-    double *A;                 // [nz][nl][ntab][nsep]
-    // Akima coefficients (b, c, d per interval) of the end-point tables: z(chi), then
-    // f, conformal H, G1, G2, D1 on the z grid, then bias1, bias2 on bias_z
+    double *A;                 // [nz][nl][ntab][nsep] (paired: [npoint][nl][ntab])
+    // the renormalisation grid of I_0^4 (integrals.c:848-907): values and the bicubic's
+    // node derivatives on ren_chi x ren_chi, [b nren + a] at (chi_a, chi_b)
+    int nren;
+    const double *ren_chi, *ren_v, *ren_x, *ren_y, *ren_xy;
+    double *ren_out;           // [nz][nl][nsep] (paired: [npoint][nl]) or null
     const double *cb[CE_MU_NCOEF], *cc[CE_MU_NCOEF], *cd_[CE_MU_NCOEF];
 };
 
@@ -126,62 +141,124 @@ __global__ void __launch_bounds__(256) k_akima_coef(coef_args a)
     a.d[t][j] = d;
 }
 
-// coefficients c_s (standard slots) of the local terms at one (z, mu, r), times G
-// the background at the end point chi of a pair: z(chi) and f, conformal H, G, D1 at
-// that z (one interval search on the chi table, started from a linear extrapolation of
-// the mean's interval, and one on the z grid shared by the four lookups), galaxy bias
+// the background at the end point chi of population p (1 or 2): z(chi) and f, conformal
+// H, G_p, D1 at that z (one interval search on the chi table, started from a linear
+// extrapolation of the mean's interval, one on the z grid shared by the lookups), the
+// galaxy bias b_p, and for the d2 / g terms a(z), s_p and f_evo,p
 struct endpoint {
-    double z, f, H, G1, G2, D1, b1, b2;
+    double z, f, H, G, D1, b, a, s, fevo;
 };
 
-__device__ __forceinline__ endpoint background_at(const mu_args &a, double chi, int jbar, double chib, double dchi)
+template <bool DG>
+__device__ __forceinline__ endpoint background_at(const mu_args &a, double chi, int p, int jbar, double chib,
+                                                  double dchi)
 {
     endpoint e;
     const int n = a.nbg;
+    const double nan = __builtin_nan("");
+    e.a = e.s = e.fevo = 0.0;
     if (!(chi >= a.bg_chi[0] && chi <= a.bg_chi[n - 1])) {
-        e.z = e.f = e.H = e.G1 = e.G2 = e.D1 = e.b1 = e.b2 = __builtin_nan("");
+        e.z = e.f = e.H = e.G = e.D1 = e.b = e.a = e.s = e.fevo = nan;
         return e;
     }
     const int jc = ce::interval_from(a.bg_chi, n, chi, ce::clamp_index(jbar + floor((chi - chib) / dchi), n));
-    e.z = akima_coef(a.bg_chi, a.bg_z, a.cb[0], a.cc[0], a.cd_[0], jc, chi);
+    e.z = akima_coef(a.bg_chi, a.bg_z, a.cb[K_ZCHI], a.cc[K_ZCHI], a.cd_[K_ZCHI], jc, chi);
     const int k = ce::interval_from(a.bg_z, n, e.z, ce::interval_guess(a.bg_z, n, e.z));
-    e.f = akima_coef(a.bg_z, a.bg_f, a.cb[1], a.cc[1], a.cd_[1], k, e.z);
-    e.H = akima_coef(a.bg_z, a.bg_cH, a.cb[2], a.cc[2], a.cd_[2], k, e.z);
-    e.G1 = akima_coef(a.bg_z, a.bg_G1, a.cb[3], a.cc[3], a.cd_[3], k, e.z);
-    e.G2 = akima_coef(a.bg_z, a.bg_G2, a.cb[4], a.cc[4], a.cd_[4], k, e.z);
-    e.D1 = akima_coef(a.bg_z, a.bg_D1, a.cb[5], a.cc[5], a.cd_[5], k, e.z);
+    e.f = akima_coef(a.bg_z, a.bg_f, a.cb[K_F], a.cc[K_F], a.cd_[K_F], k, e.z);
+    e.H = akima_coef(a.bg_z, a.bg_cH, a.cb[K_H], a.cc[K_H], a.cd_[K_H], k, e.z);
+    const int kg = p == 1 ? K_G1 : K_G2;
+    e.G = akima_coef(a.bg_z, p == 1 ? a.bg_G1 : a.bg_G2, a.cb[kg], a.cc[kg], a.cd_[kg], k, e.z);
+    e.D1 = akima_coef(a.bg_z, a.bg_D1, a.cb[K_D1], a.cc[K_D1], a.cd_[K_D1], k, e.z);
     if (!(e.z >= a.bias_z[0] && e.z <= a.bias_z[a.nb - 1])) {
-        e.b1 = e.b2 = __builtin_nan("");
+        e.b = nan;
     } else {
         const int kb = ce::interval_from(a.bias_z, a.nb, e.z, ce::interval_guess(a.bias_z, a.nb, e.z));
-        e.b1 = akima_coef(a.bias_z, a.bias1, a.cb[6], a.cc[6], a.cd_[6], kb, e.z);
-        e.b2 = akima_coef(a.bias_z, a.bias2, a.cb[7], a.cc[7], a.cd_[7], kb, e.z);
+        const int kt = p == 1 ? K_B1 : K_B2;
+        e.b = akima_coef(a.bias_z, p == 1 ? a.bias1 : a.bias2, a.cb[kt], a.cc[kt], a.cd_[kt], kb, e.z);
+    }
+    if (DG) {
+        e.a = akima_coef(a.bg_z, a.bg_a, a.cb[K_A], a.cc[K_A], a.cd_[K_A], k, e.z);
+        if (!(e.z >= a.mb_z[0] && e.z <= a.mb_z[a.nmb - 1])) {
+            e.s = e.fevo = nan;
+        } else {
+            const int km = ce::interval_from(a.mb_z, a.nmb, e.z, ce::interval_guess(a.mb_z, a.nmb, e.z));
+            const int ks = p == 1 ? K_S1 : K_S2, ke = p == 1 ? K_E1 : K_E2;
+            e.s = akima_coef(a.mb_z, p == 1 ? a.s1 : a.s2, a.cb[ks], a.cc[ks], a.cd_[ks], km, e.z);
+            e.fevo = akima_coef(a.mb_z, p == 1 ? a.fevo1 : a.fevo2, a.cb[ke], a.cc[ke], a.cd_[ke], km, e.z);
+        }
     }
     return e;
 }
 
-__device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, double mu, double sep, double chi_mean,
-                                   double fmean, double Hmean, double bm1, double bm2, double Gm1, double Gm2,
-                                   double Dmean, int jbar, double chib, double dchi, double *c)
+// the d2 / g source amplitude of one end point: the d2 and g1-g3 contributions of
+// functions.c:574-669, 856-903, 1144-1428 are sums of products of one factor per end
+// point, S(1) S(2) for the auto and mutual terms and b, f, H f times S for the crosses:
+//   S = [d2] (3 - f_evo) H^2 f - sum_g [g] (H0 h)^2 3 Omega_m / (2 a) {1 + G, 5 s - 2, f - 1}
+__device__ __forceinline__ double dg_source(const mu_args &a, const endpoint &e)
 {
-    for (int s = 0; s < NSTD; ++s) c[s] = 0.0;
+    double S = 0.0;
+    if (a.dg & 1u) S += (3 - e.fevo) * e.H * e.H * e.f;
+    const double g0 = a.H0h * a.H0h * 3 * a.Om / 2 / e.a;
+    if (a.dg & 2u) S -= g0 * (1 + e.G);
+    if (a.dg & 4u) S -= g0 * (5 * e.s - 2);
+    if (a.dg & 8u) S -= g0 * (e.f - 1);
+    return S;
+}
+
+// the renormalisation grid's bicubic (GSL interp2d bicubic: node derivatives of natural
+// cubic splines, Hermite patches; coffe_interp_spline2d, common.c:712-719) at (x, y)
+__device__ __forceinline__ double ren_bicubic(const mu_args &a, double x, double y)
+{
+    const int n = a.nren;
+    if (!(x >= a.ren_chi[0] && x <= a.ren_chi[n - 1] && y >= a.ren_chi[0] && y <= a.ren_chi[n - 1]))
+        return __builtin_nan("");
+    const int i = ce::interval_from(a.ren_chi, n, x, ce::interval_guess(a.ren_chi, n, x));
+    const int j = ce::interval_from(a.ren_chi, n, y, ce::interval_guess(a.ren_chi, n, y));
+    const double dx = a.ren_chi[i + 1] - a.ren_chi[i], dy = a.ren_chi[j + 1] - a.ren_chi[j];
+    const double t = (x - a.ren_chi[i]) / dx, u = (y - a.ren_chi[j]) / dy;
+    const double ht[4] = {(1 + 2 * t) * (1 - t) * (1 - t), t * t * (3 - 2 * t), t * (1 - t) * (1 - t), t * t * (t - 1)};
+    const double hu[4] = {(1 + 2 * u) * (1 - u) * (1 - u), u * u * (3 - 2 * u), u * (1 - u) * (1 - u), u * u * (u - 1)};
+    double r = 0.0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int o = (j + q) * n + i + p;
+            r += a.ren_v[o] * ht[p] * hu[q] + a.ren_x[o] * dx * ht[2 + p] * hu[q] + a.ren_y[o] * dy * ht[p] * hu[2 + q] +
+                 a.ren_xy[o] * dx * dy * ht[2 + p] * hu[2 + q];
+        }
+    return r;
+}
+
+// coefficients c_s of the local terms at one (z, mu, r), times G = D1(z1) D1(z2) (full
+// sky) or D1(z_mean)^2 (flat sky), functions.c:1457-1467; c[SREN] is minus the
+// coefficient of the renormalisation grid's value at (chi1, chi2)
+template <bool DG>
+__device__ __forceinline__ void local_coefficients(const mu_args &a, double mu, double sep, double chi_mean,
+                                                   double fmean, double Hmean, double bm1, double bm2, double Gm1,
+                                                   double Gm2, double Dmean, int jbar, double chib, double dchi,
+                                                   double *c)
+{
+    for (int s = 0; s < (DG ? NACC : S40); ++s) c[s] = 0.0;
     const double chi1 = chi_mean - sep * mu / 2.;
     const double chi2 = chi_mean + sep * mu / 2.;
-    // the flat-sky forms use the mean's background only; the full-sky ones the end
-    // points' and the geometric coefficients of fullsky_forms.h (the reference's
+    // the flat-sky forms of the den / rsd / d1 terms use the mean's background only; the
+    // full-sky ones and every d2 / g term the end points' (the reference reads them in
+    // flat sky too), with the geometric coefficients of fullsky_forms.h (the reference's
     // polynomials in cos theta with their (chi/s)^n cancellations done exactly)
+    const bool ends = !a.flatsky || DG;
     endpoint e1, e2;
     ce_fullsky_geom g;
-    if (!a.flatsky) {
-        e1 = background_at(a, chi1, jbar, chib, dchi);
-        e2 = background_at(a, chi2, jbar, chib, dchi);
+    if (ends) {
+        e1 = background_at<DG>(a, chi1, 1, jbar, chib, dchi);
+        e2 = background_at<DG>(a, chi2, 2, jbar, chib, dchi);
         g = ce_fullsky_geometry(chi_mean, sep, mu);
     } else {
-        e1.z = e1.f = e1.H = e1.G1 = e1.G2 = e1.D1 = e1.b1 = e1.b2 = 0.0;
+        e1.z = e1.f = e1.H = e1.G = e1.D1 = e1.b = e1.a = e1.s = e1.fevo = 0.0;
         e2 = e1;
     }
-    const double f1 = e1.f, f2 = e2.f, H1 = e1.H, H2 = e2.H, b1 = e1.b1, b2 = e2.b2;
-    const double G1 = e1.G1, G2 = e2.G2;
+    const double f1 = e1.f, f2 = e2.f, H1 = e1.H, H2 = e2.H, b1 = e1.b, b2 = e2.b;
+    const double G1 = e1.G, G2 = e2.G;
     const double P2 = legendre_p(2, mu), P4 = legendre_p(4, mu);
     if (a.terms & COFFE_TERM_DEN_DEN) {   // functions.c:129-229
         if (a.flatsky) c[S00] += bm1 * bm2;
@@ -233,28 +310,41 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double zm, 
             c[S13] += f1 * f2 * H2 * G2 * g.rd3_a + f2 * f1 * H1 * G1 * g.rd3_b;
         }
     }
+    if (DG) {
+        const double Sg1 = dg_source(a, e1), Sg2 = dg_source(a, e2);
+        if (a.terms & COFFE_TERM_DG_DG) {     // I_0^4 minus its renormalisation at (chi1, chi2)
+            c[S40] += Sg1 * Sg2;
+            c[SREN] -= Sg1 * Sg2 * ren_bicubic(a, chi1, chi2);
+        }
+        if (a.terms & COFFE_TERM_DEN_DG) c[S20] += b1 * Sg2 + b2 * Sg1;
+        if (a.terms & COFFE_TERM_RSD_DG) {
+            c[S20] += (f1 * Sg2 + f2 * Sg1) / 3.;
+            c[S22] += -(f1 * Sg2 * g.dr_b + f2 * Sg1 * g.dr_a) * sq(sep);
+        }
+        if (a.terms & COFFE_TERM_D1_DG) c[S31] += -(H1 * f1 * Sg2 * g.dd1_b + H2 * f2 * Sg1 * g.dd1_a);
+    }
     const double G = a.flatsky ? Dmean * Dmean : e1.D1 * e2.D1;
-    for (int s = 0; s < NSTD; ++s) c[s] *= G;
+    for (int s = 0; s < (DG ? NACC : S40); ++s) c[s] *= G;
 }
 
-// one wavefront per (z, r); blockDim = 256 (four points).  The lanes take 64 mu nodes
-// at a time; after each batch every (multipole, slot) sum is reduced across the
-// wavefront and added into the wavefront's LDS accumulators, so no per-lane partial
-// sums stay live over the node loop and the kernel fits 128 VGPRs (4 waves per SIMD,
-// every (z, r) wavefront of the 8 x 512 grid resident at once) without spills.
-// NL = the number of multipoles kept (>= a.nl), a template argument so that absent
-// multipoles cost nothing
-template <int NL>
+// one wavefront per (z, r) point (paired: per (zmean[i], sep[i])); blockDim = 256 (four
+// points).  The lanes take 64 mu nodes at a time; after each batch every (multipole,
+// slot) sum is reduced across the wavefront and added into the wavefront's LDS
+// accumulators, so that no per-lane partial sums stay live over the node loop.
+// NL = the number of multipoles kept (>= a.nl) and DG = the d2 / g terms, template
+// arguments so that absent multipoles and terms cost no registers
+template <int NL, bool DG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_mu_weights(mu_args a)
 {
-    __shared__ double sacc[4][CE_MU_MAXL * NSTD];
+    constexpr int NC = DG ? NACC : S40;   // slots in use
+    __shared__ double sacc[4][CE_MU_MAXL * NACC];
     const int lane = threadIdx.x & 63;
     const int point = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int npoint = a.nz * a.nsep;
+    const int npoint = a.paired ? a.nz : a.nz * a.nsep;
     if (point >= npoint) return;   // whole wavefronts exit together
     double *acc = sacc[threadIdx.x >> 6];
-    if (lane < NL * NSTD) acc[lane] = 0.0;
-    const int iz = point / a.nsep, ir = point % a.nsep;
+    if (lane < NL * NACC) acc[lane] = 0.0;
+    const int iz = a.paired ? point : point / a.nsep, ir = a.paired ? point : point % a.nsep;
     const double zm = a.zmean[iz], sep = a.sep[ir];
     const double chi_mean = ce::akima_general(a.bg_z, a.bg_chi, a.nbg, zm);
     const double fmean = ce::akima_general(a.bg_z, a.bg_f, a.nbg, zm);
@@ -268,83 +358,106 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const double chib = a.bg_chi[jbar], dchi = a.bg_chi[jbar + 1] - a.bg_chi[jbar];
     for (int q0 = 0; q0 < a.nmu; q0 += 64) {
         const int q = q0 + lane;
-        double c[NSTD];
+        double c[NACC];
         double mu = 0.0, w = 0.0;
         if (q < a.nmu) {
             mu = a.mu[q];
             w = a.wmu[q];
-            local_coefficients(a, zm, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, jbar, chib, dchi, c);
+            local_coefficients<DG>(a, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, jbar, chib, dchi, c);
         } else {
 #pragma unroll
-            for (int s = 0; s < NSTD; ++s) c[s] = 0.0;
+            for (int s = 0; s < NC; ++s) c[s] = 0.0;
         }
 #pragma unroll
         for (int il = 0; il < NL; ++il) {
             if (il >= a.nl) break;
             const double pl = w * legendre_p(a.ls[il], mu);
 #pragma unroll
-            for (int s = 0; s < NSTD; ++s) {
+            for (int s = 0; s < NC; ++s) {
                 double v = pl * c[s];
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-                if (lane == 0) acc[il * NSTD + s] += v;
+                if (lane == 0) acc[il * NACC + s] += v;
             }
         }
     }
     // lane 0 wrote the sums; the other lanes read them after the wavefront's LDS writes
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): the LDS stores have completed
-    if (lane < NL * NSTD) {
-        const int il = lane / NSTD, s = lane % NSTD;
-        if (il < a.nl) {
-            const double v = acc[lane];
-            for (int t = 0; t < a.ntab; ++t)
-                if (a.slot[t] == s) a.A[(((int64_t)iz * a.nl + il) * a.ntab + t) * a.nsep + ir] = (2 * a.ls[il] + 1) * 0.5 * v;
+    if (lane < NL * NACC) {
+        const int il = lane / NACC, s = lane % NACC;
+        if (il < a.nl && s < NC) {
+            const double v = (2 * a.ls[il] + 1) * 0.5 * acc[lane];
+            if (s == SREN) {
+                if (a.ren_out)
+                    a.ren_out[a.paired ? (int64_t)point * a.nl + il : ((int64_t)iz * a.nl + il) * a.nsep + ir] = v;
+            } else {
+                for (int t = 0; t < a.ntab; ++t)
+                    if (a.slot[t] == s)
+                        a.A[a.paired ? ((int64_t)point * a.nl + il) * a.ntab + t
+                                     : (((int64_t)iz * a.nl + il) * a.ntab + t) * a.nsep + ir] = v;
+            }
         }
     }
 }
 
 }  // namespace
 
-extern "C" int ce_launch_mu_weights(void *stream, int nbg, const double *bg_z, const double *bg_chi,
-                                    const double *bg_D1, const double *bg_f, const double *bg_cH, const double *bg_G1,
-                                    const double *bg_G2, int nb, const double *bias_z, const double *bias1,
-                                    const double *bias2, unsigned terms, int flatsky, int nz,
-                                    const double *zmean, int nl, const int *ls, int ntab, const int *slot, int nsep,
-                                    const double *sep, int nmu, const double *mu, const double *wmu, double *A,
-                                    double *coef)
+extern "C" int ce_launch_mu_weights(void *stream, const ce_mu_dev *in, unsigned terms, unsigned dg, int flatsky,
+                                    int paired, int nz, const double *zmean, int nl, const int *ls, int ntab,
+                                    const int *slot, int nsep, const double *sep, int nmu, const double *mu,
+                                    const double *wmu, double *A, double *ren_out, double *coef)
 {
-    if (nl > CE_MU_MAXL || ntab > CE_MAX_TABLES) return (int)hipErrorInvalidValue;
+    if (nl > CE_MU_MAXL || ntab > CE_MAX_TABLES + 1) return (int)hipErrorInvalidValue;
     mu_args a;
+    memset(&a, 0, sizeof(a));
     coef_args ca;
+    memset(&ca, 0, sizeof(ca));
     {
-        const double *xs[CE_MU_NCOEF] = {bg_chi, bg_z, bg_z, bg_z, bg_z, bg_z, bias_z, bias_z};
-        const double *ys[CE_MU_NCOEF] = {bg_z, bg_f, bg_cH, bg_G1, bg_G2, bg_D1, bias1, bias2};
+        const double *xs[CE_MU_NCOEF] = {in->bg_chi, in->bg_z, in->bg_z, in->bg_z, in->bg_z, in->bg_z, in->bias_z,
+                                         in->bias_z, in->bg_z, in->mb_z, in->mb_z, in->mb_z, in->mb_z};
+        const double *ys[CE_MU_NCOEF] = {in->bg_z, in->bg_f, in->bg_cH, in->bg_G1, in->bg_G2, in->bg_D1, in->bias1,
+                                         in->bias2, in->bg_a, in->s1, in->s2, in->fevo1, in->fevo2};
+        const int ns[CE_MU_NCOEF] = {in->nbg, in->nbg, in->nbg, in->nbg, in->nbg, in->nbg, in->nb, in->nb, in->nbg,
+                                     in->nmb, in->nmb, in->nmb, in->nmb};
+        const int ntabs = dg ? CE_MU_NCOEF : K_A;   // the d2 / g tables only when asked for
         double *p = coef;
         int nmax = 0;
-        for (int t = 0; t < CE_MU_NCOEF; ++t) {
-            const int n = t < 6 ? nbg : nb;
+        for (int t = 0; t < ntabs; ++t) {
+            const int n = ns[t];
             ca.x[t] = xs[t]; ca.y[t] = ys[t]; ca.n[t] = n;
             ca.b[t] = p; ca.c[t] = p + n; ca.d[t] = p + 2 * n;
             a.cb[t] = ca.b[t]; a.cc[t] = ca.c[t]; a.cd_[t] = ca.d[t];
             p += 3 * n;
             if (n > nmax) nmax = n;
         }
-        hipLaunchKernelGGL(k_akima_coef, dim3((nmax + 255) / 256, CE_MU_NCOEF), dim3(256), 0, (hipStream_t)stream, ca);
+        hipLaunchKernelGGL(k_akima_coef, dim3((nmax + 255) / 256, ntabs), dim3(256), 0, (hipStream_t)stream, ca);
     }
-    a.nbg = nbg; a.bg_z = bg_z; a.bg_chi = bg_chi; a.bg_D1 = bg_D1; a.bg_f = bg_f; a.bg_cH = bg_cH;
-    a.bg_G1 = bg_G1; a.bg_G2 = bg_G2;
-    a.nb = nb; a.bias_z = bias_z; a.bias1 = bias1; a.bias2 = bias2;
-    a.terms = terms; a.flatsky = flatsky;
+    a.nbg = in->nbg; a.bg_z = in->bg_z; a.bg_chi = in->bg_chi; a.bg_D1 = in->bg_D1; a.bg_f = in->bg_f;
+    a.bg_cH = in->bg_cH; a.bg_G1 = in->bg_G1; a.bg_G2 = in->bg_G2; a.bg_a = in->bg_a;
+    a.nb = in->nb; a.bias_z = in->bias_z; a.bias1 = in->bias1; a.bias2 = in->bias2;
+    a.nmb = in->nmb; a.mb_z = in->mb_z; a.s1 = in->s1; a.s2 = in->s2; a.fevo1 = in->fevo1; a.fevo2 = in->fevo2;
+    a.H0h = in->H0h; a.Om = in->Om;
+    a.nren = in->nren; a.ren_chi = in->ren_chi; a.ren_v = in->ren_v; a.ren_x = in->ren_x; a.ren_y = in->ren_y;
+    a.ren_xy = in->ren_xy;
+    a.terms = terms; a.dg = dg; a.flatsky = flatsky; a.paired = paired;
     a.nz = nz; a.zmean = zmean; a.nl = nl;
     for (int i = 0; i < nl; ++i) a.ls[i] = ls[i];
     a.ntab = ntab;
     for (int t = 0; t < ntab; ++t) a.slot[t] = slot[t];
-    a.nsep = nsep; a.sep = sep; a.nmu = nmu; a.mu = mu; a.wmu = wmu; a.A = A;
-    const int npoint = nz * nsep;
+    a.nsep = nsep; a.sep = sep; a.nmu = nmu; a.mu = mu; a.wmu = wmu; a.A = A; a.ren_out = ren_out;
+    const int npoint = paired ? nz : nz * nsep;
     const int blocks = (npoint + 3) / 4;
-    if (nl <= 1) hipLaunchKernelGGL(k_mu_weights<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
-    else if (nl == 2) hipLaunchKernelGGL(k_mu_weights<2>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
-    else if (nl == 3) hipLaunchKernelGGL(k_mu_weights<3>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
-    else hipLaunchKernelGGL(k_mu_weights<CE_MU_MAXL>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
+    hipStream_t s = (hipStream_t)stream;
+    if (dg) {
+        if (nl <= 1) hipLaunchKernelGGL((k_mu_weights<1, true>), dim3(blocks), dim3(256), 0, s, a);
+        else if (nl == 2) hipLaunchKernelGGL((k_mu_weights<2, true>), dim3(blocks), dim3(256), 0, s, a);
+        else if (nl == 3) hipLaunchKernelGGL((k_mu_weights<3, true>), dim3(blocks), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_mu_weights<CE_MU_MAXL, true>), dim3(blocks), dim3(256), 0, s, a);
+    } else {
+        if (nl <= 1) hipLaunchKernelGGL((k_mu_weights<1, false>), dim3(blocks), dim3(256), 0, s, a);
+        else if (nl == 2) hipLaunchKernelGGL((k_mu_weights<2, false>), dim3(blocks), dim3(256), 0, s, a);
+        else if (nl == 3) hipLaunchKernelGGL((k_mu_weights<3, false>), dim3(blocks), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_mu_weights<CE_MU_MAXL, false>), dim3(blocks), dim3(256), 0, s, a);
+    }
     return (int)hipGetLastError();
 }

@@ -279,10 +279,12 @@ def akima(x, y, xq):
 
 
 TERM_DEN_DEN, TERM_RSD_RSD, TERM_D1_D1, TERM_DEN_RSD, TERM_DEN_D1, TERM_RSD_D1 = 1, 2, 4, 8, 16, 32
+TERM_DG_DG, TERM_DEN_DG, TERM_RSD_DG, TERM_D1_DG = 64, 128, 256, 512
 
 
-def terms_of(den, rsd, d1):
-    """COFFE_TERM_* mask of the contribution flags (auto terms and cross terms)."""
+def terms_of(den, rsd, d1, dg=False):
+    """COFFE_TERM_* mask of the contribution flags (auto terms and cross terms); dg: any of
+    d2, g1-g3 (their mutual terms and their crosses with den, rsd, d1)."""
     t = 0
     t |= TERM_DEN_DEN if den else 0
     t |= TERM_RSD_RSD if rsd else 0
@@ -290,32 +292,87 @@ def terms_of(den, rsd, d1):
     t |= TERM_DEN_RSD if den and rsd else 0
     t |= TERM_DEN_D1 if den and d1 else 0
     t |= TERM_RSD_D1 if rsd and d1 else 0
+    if dg:
+        t |= TERM_DG_DG
+        t |= TERM_DEN_DG if den else 0
+        t |= TERM_RSD_DG if rsd else 0
+        t |= TERM_D1_DG if d1 else 0
     return t
 
 
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+class MuInputs(ctypes.Structure):
+    """coffe_mu_inputs (include/coffe_engine.h)."""
+    _fields_ = [("nbg", ctypes.c_int)] + [(n, _DP) for n in ("bg_z", "bg_chi", "bg_D1", "bg_f", "bg_cH", "bg_G1",
+                                                              "bg_G2", "bg_a")] + \
+        [("nb", ctypes.c_int)] + [(n, _DP) for n in ("bias_z", "bias1", "bias2")] + \
+        [("nmb", ctypes.c_int)] + [(n, _DP) for n in ("mb_z", "s1", "s2", "fevo1", "fevo2")] + \
+        [("H0h", ctypes.c_double), ("Omega_m", ctypes.c_double), ("nren", ctypes.c_int), ("ren_chi", _DP),
+         ("ren", _DP)]
+
+
 def mu_weights(engine, bg, bias_z, bias1, bias2, zmean, ls, tables, sep, den=True, rsd=True, d1=False,
-               flatsky=False, nmu=256, terms=None, out=None, stream=None):
-    """Spectrum-independent mu-quadrature weights A[z][l][t][r] (coffe_mu_weights_terms), so that
-    multipoles = sum_t A I_t(sep).  bg: dict of host arrays z, chi, D1, f, cH (conformal Hubble rate),
-    G1, G2.  terms: COFFE_TERM_* mask, default from den/rsd/d1."""
+               flatsky=False, nmu=256, terms=None, out=None, stream=None, dg=None, ren_out=None, paired=False):
+    """Spectrum-independent mu-quadrature weights A[z][l][t][r] (coffe_mu_weights_ex), so that
+    multipoles = sum_t A I_t(sep) (+ the renormalisation part with the d2 / g terms).
+    bg: dict of host arrays z, chi, D1, f, cH (conformal Hubble rate), G1, G2 (and a with
+    dg).  terms: COFFE_TERM_* mask, default from den/rsd/d1 and dg.
+    dg: None or a dict with sources (a set of "d2", "g1", "g2", "g3"), H0h, Omega_m, the
+    magnification / evolution biases mb_z, s1, s2, fevo1, fevo2, and for the auto terms the
+    renormalisation grid ren_chi, ren (I_0^4's table (4, 0) must be among tables); then
+    ren_out (device, (nz, nl, nsep), or (nz, nl) paired) receives the grid's part.
+    paired: point i is (zmean[i], sep[i]); A is (nz, nl, ntab)."""
     import torch
     L = engine.L
     f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
-    arrs = [f64(bg[k]) for k in ("z", "chi", "D1", "f", "cH", "G1", "G2")]
-    bz, b1, b2 = f64(bias_z), f64(bias1), f64(bias2)
-    zm, sp = f64(np.atleast_1d(zmean)), f64(np.atleast_1d(sep))
     dp = ctypes.POINTER(ctypes.c_double)
+    keep = []
+
+    def arr(a):
+        v = f64(a)
+        keep.append(v)
+        return v.ctypes.data_as(dp), len(v)
+
+    inp = MuInputs()
+    for name, key in (("bg_z", "z"), ("bg_chi", "chi"), ("bg_D1", "D1"), ("bg_f", "f"), ("bg_cH", "cH"),
+                      ("bg_G1", "G1"), ("bg_G2", "G2")):
+        ptr, n = arr(bg[key])
+        setattr(inp, name, ptr)
+    inp.nbg = n
+    inp.bias_z, inp.nb = arr(bias_z)
+    inp.bias1, _ = arr(bias1)
+    inp.bias2, _ = arr(bias2)
+    sources = 0
+    if dg is not None:
+        src = dg["sources"]
+        sources = (1 if "d2" in src else 0) | (2 if "g1" in src else 0) | (4 if "g2" in src else 0) | \
+            (8 if "g3" in src else 0)
+        inp.bg_a, _ = arr(bg["a"])
+        inp.mb_z, inp.nmb = arr(dg["mb_z"])
+        for name in ("s1", "s2", "fevo1", "fevo2"):
+            setattr(inp, name, arr(dg[name])[0])
+        inp.H0h, inp.Omega_m = float(dg["H0h"]), float(dg["Omega_m"])
+        if "ren" in dg:
+            inp.ren_chi, inp.nren = arr(dg["ren_chi"])
+            inp.ren, _ = arr(np.asarray(dg["ren"]).reshape(-1))
+    zm, sp = f64(np.atleast_1d(zmean)), f64(np.atleast_1d(sep))
+    if paired and len(zm) != len(sp):
+        raise ValueError("paired weights need as many redshifts as separations")
     ls_c = (ctypes.c_int * len(ls))(*ls)
     tn = (ctypes.c_int * len(tables))(*[t[0] for t in tables])
     tl = (ctypes.c_int * len(tables))(*[t[1] for t in tables])
     if terms is None:
-        terms = terms_of(den, rsd, d1)
-    A = out if out is not None else torch.empty((len(zm), len(ls), len(tables), len(sp)), dtype=torch.float64,
-                                                device="cuda")
-    _need(A, (len(zm), len(ls), len(tables), len(sp)), "A")
+        terms = terms_of(den, rsd, d1, dg is not None)
+    shape = (len(zm), len(ls), len(tables)) if paired else (len(zm), len(ls), len(tables), len(sp))
+    A = out if out is not None else torch.empty(shape, dtype=torch.float64, device="cuda")
+    _need(A, shape, "A")
+    if ren_out is not None:
+        _need(ren_out, shape[:2] if paired else (len(zm), len(ls), len(sp)), "ren_out")
     st = stream if stream is not None else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    check(L.coffe_mu_weights_terms(engine.h, len(arrs[0]), *[a.ctypes.data_as(dp) for a in arrs], len(bz),
-                                   bz.ctypes.data_as(dp), b1.ctypes.data_as(dp), b2.ctypes.data_as(dp), int(terms),
-                                   int(flatsky), len(zm), zm.ctypes.data_as(dp), len(ls), ls_c, len(tables), tn, tl,
-                                   len(sp), sp.ctypes.data_as(dp), int(nmu), _ptr(A), st), "coffe_mu_weights")
+    check(L.coffe_mu_weights_ex(engine.h, ctypes.byref(inp), int(terms), int(sources), int(flatsky), int(paired),
+                                len(zm), zm.ctypes.data_as(dp), len(ls), ls_c, len(tables), tn, tl, len(sp),
+                                sp.ctypes.data_as(dp), int(nmu), _ptr(A),
+                                _ptr(ren_out) if ren_out is not None else None, st), "coffe_mu_weights")
     return A

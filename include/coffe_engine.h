@@ -168,8 +168,18 @@ enum {
     COFFE_TERM_D1_D1 = 4,
     COFFE_TERM_DEN_RSD = 8,
     COFFE_TERM_DEN_D1 = 16,
-    COFFE_TERM_RSD_D1 = 32
+    COFFE_TERM_RSD_D1 = 32,
+    /* the d2 and g1-g3 terms (functions.c:574-669, 856-903, 1144-1428): among the chosen
+       d2 / g sources (their auto and mutual terms, on I_0^4 minus its renormalisation),
+       and their crosses with density (I_0^2), RSD (I_0^2, I_2^2) and Doppler-1 (I_1^3) */
+    COFFE_TERM_DG_DG = 64,
+    COFFE_TERM_DEN_DG = 128,
+    COFFE_TERM_RSD_DG = 256,
+    COFFE_TERM_D1_DG = 512
 };
+
+/* the d2 / g sources of coffe_mu_weights_ex */
+enum { COFFE_SOURCE_D2 = 1, COFFE_SOURCE_G1 = 2, COFFE_SOURCE_G2 = 4, COFFE_SOURCE_G3 = 8 };
 
 /* the term mask of the reference's contribution flags: auto terms of each flag set
    and cross terms of each pair set (functions.c:129-1143) */
@@ -204,6 +214,46 @@ int coffe_mu_weights_terms(coffe_engine *e, int nbg, const double *bg_z, const d
                            int flatsky, int nz, const double *zmean, int nl, const int *ls, int ntab,
                            const int *tab_n, const int *tab_l, int nsep, const double *sep, int nmu, double *d_A,
                            void *stream);
+
+/*
+ * Host inputs of coffe_mu_weights_ex.  Background tables on bg_z (as coffe_mu_weights;
+ * bg_a = the scale factor, read by the d2 / g terms only), the galaxy biases 1 and 2 on
+ * bias_z, the magnification biases s1, s2 and evolution biases f_evo,1/2 on mb_z (d2 / g
+ * terms only; Akima like every table, parser.c:807), H0h = COFFE_H0 h in 1/Mpc and
+ * Omega_m = Omega0_cdm + Omega0_baryon (g terms), and the renormalisation grid of I_0^4
+ * (integrals.c:848-907): nren chi nodes ren_chi and ren[b nren + a] at (ren_chi[a],
+ * ren_chi[b]), interpolated by GSL's bicubic (natural-spline node derivatives, the
+ * reference's default covariance_interpolation_method, parser.c:810).
+ */
+typedef struct coffe_mu_inputs {
+    int nbg;
+    const double *bg_z, *bg_chi, *bg_D1, *bg_f, *bg_cH, *bg_G1, *bg_G2, *bg_a;
+    int nb;
+    const double *bias_z, *bias1, *bias2;
+    int nmb;
+    const double *mb_z, *s1, *s2, *fevo1, *fevo2;
+    double H0h, Omega_m;
+    int nren;
+    const double *ren_chi, *ren;
+} coffe_mu_inputs;
+
+/*
+ * coffe_mu_weights_terms with the d2 and g1-g3 terms and a paired mode (functions.c:
+ * 89-1484; no reference counterpart for the pairing).  ntab <= 9 (the 8 standard tables
+ * and I_0^4, each at most once).  terms: any COFFE_TERM_* mask;
+ * sources: the COFFE_SOURCE_* of the d2 / g terms (0 without them).  Table (n, l) = (4,
+ * 0) is the divergent I_0^4 (its r^4 I_0^4 table, integrals.c:772-834) and is needed by
+ * COFFE_TERM_DG_DG, which also integrates minus the renormalisation grid at (chi1,
+ * chi2) against the same mu weights into d_ren [nz][nl][nsep] (device, required then):
+ *   multipoles = sum_t A[z][l][t][r] I_t(sep_r) + ren[z][l][r].
+ * paired = 1: point i is (zmean[i], sep[i]), nz == nsep, d_A [nz][nl][ntab] and d_ren
+ * [nz][nl] (the redshift-averaged multipoles' node pairs).  Asynchronous like
+ * coffe_mu_weights; COFFE_EDOMAIN when an end point leaves the chi table or the grid.
+ */
+int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned terms, unsigned sources, int flatsky,
+                        int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
+                        const int *tab_l, int nsep, const double *sep, int nmu, double *d_A, double *d_ren,
+                        void *stream);
 
 /*
  * The reference's full table (coffe_integrals_renormalizable, integrals.c:459-573):

@@ -89,6 +89,19 @@ int orc_tables_at_sep_batch(int nspec, const long long *off, const double *k, co
                             int ntab, const int *tab_n, const int *tab_l, size_t nsep, const double *sep, double *out);
 
 /* multipoles (orc_multipoles.c) */
+/* the d2 and g1-g3 contributions (functions.c:574-669, 856-903, 1144-1428) */
+typedef struct orc_dg {
+    int d2, g1, g2, g3;
+    double H0h;          /* COFFE_H0 h in 1/Mpc */
+    double Omega_m;      /* Omega0_cdm + Omega0_baryon */
+    const double *bg_a;  /* scale factor on the background z grid */
+    size_t nmb;          /* magnification and evolution biases on mb_z (Akima) */
+    const double *mb_z, *s1, *s2, *fevo1, *fevo2;
+    size_t ndiv;         /* the divergent table r^4 I_0^4 (Akima) */
+    const double *div_x, *div_y;
+    size_t nren;         /* renormalisation grid on ren_chi: ren[b nren + a] at (chi_a, chi_b) (bicubic) */
+    const double *ren_chi, *ren;
+} orc_dg;
 double orc_functions_nonintegrated(const void *ctx, double z_mean, double mu, double sep);
 int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
                    const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
@@ -102,6 +115,20 @@ int orc_mu_weights_hp(size_t nbg, const double *bg_z, const double *bg_chi, cons
                       const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
                       const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, size_t nmu,
                       const double *mu, const double *wmu, double *out);
+
+int orc_multipoles_dg(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
+                      const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                      const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                      const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                      const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int quad,
+                      double epsrel, const orc_dg *dg, double *out);
+
+int orc_mu_weights_hp_dg(size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                         const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb,
+                         const double *zb, const double *bias1, const double *bias2, int den, int rsd, int d1,
+                         int flatsky, size_t nz, const double *z_mean, size_t nl, const int *ls, size_t nsep,
+                         const double *sep, size_t nmu, const double *mu, const double *wmu, const orc_dg *dg,
+                         double *out, double *out_ren);
 
 int orc_reference_batch(int nspec, const long long *off, const double *k, const double *p, const long long *nfft,
                         size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,

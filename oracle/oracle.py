@@ -57,6 +57,9 @@ def _declare(L):
     L.orc_multipoles.argtypes = [ctypes.POINTER(c_double_p), ctypes.POINTER(c_double_p), ctypes.POINTER(s), s,
                                  p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
                                  ctypes.POINTER(ctypes.c_int), s, p, i, d, p]
+    L.orc_multipoles_dg.argtypes = [ctypes.POINTER(c_double_p), ctypes.POINTER(c_double_p), ctypes.POINTER(s), s,
+                                    p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
+                                    ctypes.POINTER(ctypes.c_int), s, p, i, d, ctypes.c_void_p, p]
     L.orc_g_l.argtypes = [d, d, s, p, p]
     L.orc_g_l.restype = None
     L.orc_window_2d.argtypes = [p, d, ctypes.c_long, ctypes.c_long]
@@ -84,6 +87,8 @@ def _declare(L):
     L.orc_akima_jvp.argtypes = [s, p, p, p, s, p, p, p]
     L.orc_mu_weights_hp.argtypes = [s, p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
                                     ctypes.POINTER(ctypes.c_int), s, p, s, p, p, p]
+    L.orc_mu_weights_hp_dg.argtypes = [s, p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
+                                       ctypes.POINTER(ctypes.c_int), s, p, s, p, p, ctypes.c_void_p, p, p]
     L.orc_qag_test.argtypes = [i, d, d, d, ctypes.POINTER(ctypes.c_int)]
     L.orc_qag_test.restype = d
 
@@ -237,11 +242,32 @@ def divergent_tables(k, pk, sep, chi, kmin=None, kmax=None):
 TABLE_ORDER = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]  # (n, l), integrals.c:599-608
 
 
+class DG(ctypes.Structure):
+    """orc_dg: the d2 / g1-g3 inputs (functions.c:574-669, 856-903, 1144-1428)."""
+    _fields_ = [("d2", ctypes.c_int), ("g1", ctypes.c_int), ("g2", ctypes.c_int), ("g3", ctypes.c_int),
+                ("H0h", ctypes.c_double), ("Omega_m", ctypes.c_double), ("bg_a", c_double_p),
+                ("nmb", c_size), ("mb_z", c_double_p), ("s1", c_double_p), ("s2", c_double_p),
+                ("fevo1", c_double_p), ("fevo2", c_double_p), ("ndiv", c_size), ("div_x", c_double_p),
+                ("div_y", c_double_p), ("nren", c_size), ("ren_chi", c_double_p), ("ren", c_double_p)]
+
+
+def make_dg(flags, H0h, Omega_m, bg_a, mb_z, s1, s2, fevo1, fevo2, div_x, div_y, ren_chi, ren):
+    """An orc_dg for the flags dict (d2, g1, g2, g3); returns (struct, the arrays it points
+    into, which the caller keeps alive)."""
+    keep = [_f64(a) for a in (bg_a, mb_z, s1, s2, fevo1, fevo2, div_x, div_y, ren_chi, ren)]
+    st = DG(int(flags.get("d2", 0)), int(flags.get("g1", 0)), int(flags.get("g2", 0)), int(flags.get("g3", 0)),
+            float(H0h), float(Omega_m), _p(keep[0]), len(keep[1]), _p(keep[1]), _p(keep[2]), _p(keep[3]),
+            _p(keep[4]), _p(keep[5]), len(keep[6]), _p(keep[6]), _p(keep[7]), len(keep[8]), _p(keep[8]),
+            _p(keep[9]))
+    return st, keep
+
+
 def multipoles(tables, bg, bias_z, bias1, bias2, z_mean, ls, sep, den=True, rsd=True, d1=False, flatsky=False,
-               quad=0, epsrel=1e-5):
+               quad=0, epsrel=1e-5, dg=None):
     """functions.c:89-1484 + signal.c:83-111,521-542 + multipoles.c:48-136.
 
     tables: dict (n, l) -> (x, y); bg: dict of arrays on bg['z'] (chi, D1, f, cH, G1, G2).
+    dg: an orc_dg (make_dg) for the d2 / g1-g3 terms, or None.
     Returns array (len(z_mean), len(ls), len(sep)).
     """
     T = len(TABLE_ORDER)
@@ -261,10 +287,11 @@ def multipoles(tables, bg, bias_z, bias1, bias2, z_mean, ls, sep, den=True, rsd=
     zm, sp = _f64(np.atleast_1d(z_mean)), _f64(np.atleast_1d(sep))
     lsa = (ctypes.c_int * len(ls))(*ls)
     out = np.empty(len(zm) * len(ls) * len(sp))
-    _check(lib().orc_multipoles(xs, ys, ns, len(arrs["z"]), _p(arrs["z"]), _p(arrs["chi"]), _p(arrs["D1"]),
-                                _p(arrs["f"]), _p(arrs["cH"]), _p(arrs["G1"]), _p(arrs["G2"]), len(bz), _p(bz),
-                                _p(b1), _p(b2), int(den), int(rsd), int(d1), int(flatsky), len(zm), _p(zm), len(ls),
-                                lsa, len(sp), _p(sp), int(quad), float(epsrel), _p(out)), "multipoles")
+    _check(lib().orc_multipoles_dg(xs, ys, ns, len(arrs["z"]), _p(arrs["z"]), _p(arrs["chi"]), _p(arrs["D1"]),
+                                   _p(arrs["f"]), _p(arrs["cH"]), _p(arrs["G1"]), _p(arrs["G2"]), len(bz), _p(bz),
+                                   _p(b1), _p(b2), int(den), int(rsd), int(d1), int(flatsky), len(zm), _p(zm),
+                                   len(ls), lsa, len(sp), _p(sp), int(quad), float(epsrel),
+                                   ctypes.byref(dg) if dg is not None else None, _p(out)), "multipoles")
     return out.reshape(len(zm), len(ls), len(sp))
 
 
@@ -394,10 +421,11 @@ def tables_at_sep_jvp(specs, nfft, tables, sep, dirs):
 
 
 def mu_weights_hp(bg, bias_z, bias1, bias2, z_mean, ls, sep, den=True, rsd=True, d1=False, flatsky=False, nmu=256,
-                  nodes=None):
-    """The reference's local-term weights A[z][l][t][r] (t: the 8 standard tables)
-    evaluated in binary128 under an nmu-point Gauss-Legendre rule (or the given
-    (mu, w) nodes); see orc_mu_weights_hp."""
+                  nodes=None, dg=None):
+    """The reference's local-term weights A[z][l][t][r] (t: the 8 standard tables, and I_0^4
+    as a ninth with dg) evaluated in binary128 under an nmu-point Gauss-Legendre rule (or
+    the given (mu, w) nodes); see orc_mu_weights_hp_dg.  With dg (make_dg) returns (A, ren),
+    ren [z][l][r] the renormalisation grid's part of the multipoles."""
     arrs = {kk: _f64(bg[kk]) for kk in ("z", "chi", "D1", "f", "cH", "G1", "G2")}
     bz, b1, b2 = _f64(bias_z), _f64(bias1), _f64(bias2)
     zm, sp = _f64(np.atleast_1d(z_mean)), _f64(np.atleast_1d(sep))
@@ -407,9 +435,16 @@ def mu_weights_hp(bg, bias_z, bias1, bias2, z_mean, ls, sep, den=True, rsd=True,
         mu, w = nodes
     mu, w = _f64(mu), _f64(w)
     lsa = (ctypes.c_int * len(ls))(*ls)
-    out = np.empty(len(zm) * len(ls) * 8 * len(sp))
-    _check(lib().orc_mu_weights_hp(len(arrs["z"]), _p(arrs["z"]), _p(arrs["chi"]), _p(arrs["D1"]), _p(arrs["f"]),
-                                   _p(arrs["cH"]), _p(arrs["G1"]), _p(arrs["G2"]), len(bz), _p(bz), _p(b1), _p(b2),
-                                   int(den), int(rsd), int(d1), int(flatsky), len(zm), _p(zm), len(ls), lsa, len(sp),
-                                   _p(sp), len(mu), _p(mu), _p(w), _p(out)), "mu_weights_hp")
-    return out.reshape(len(zm), len(ls), 8, len(sp))
+    nt = 9 if dg is not None else 8
+    out = np.empty(len(zm) * len(ls) * nt * len(sp))
+    ren = np.zeros(len(zm) * len(ls) * len(sp))
+    _check(lib().orc_mu_weights_hp_dg(len(arrs["z"]), _p(arrs["z"]), _p(arrs["chi"]), _p(arrs["D1"]), _p(arrs["f"]),
+                                      _p(arrs["cH"]), _p(arrs["G1"]), _p(arrs["G2"]), len(bz), _p(bz), _p(b1),
+                                      _p(b2), int(den), int(rsd), int(d1), int(flatsky), len(zm), _p(zm), len(ls),
+                                      lsa, len(sp), _p(sp), len(mu), _p(mu), _p(w),
+                                      ctypes.byref(dg) if dg is not None else None, _p(out), _p(ren)),
+           "mu_weights_hp")
+    A = out.reshape(len(zm), len(ls), nt, len(sp))
+    if dg is None:
+        return A
+    return A, ren.reshape(len(zm), len(ls), len(sp))

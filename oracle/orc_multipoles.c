@@ -44,6 +44,11 @@ typedef struct {
     int have[NTAB];
     akima_t chi, z_of_chi, D1, f, cH, G1, G2, b1, b2;
     int den, rsd, d1, flatsky;
+    /* d2 and g1-g3 (orc_dg): flags, end-point tables, divergent table and its grid */
+    int d2, g1, g2, g3;
+    double H0h, Omega_m;
+    akima_t a, s1, s2, fevo1, fevo2, div;
+    orc_bicubic *ren;
 } mp_ctx;
 
 typedef struct {
@@ -148,6 +153,54 @@ double orc_functions_nonintegrated(const void *vctx, double z_mean, double mu, d
                            I(c, T13, sep) / pow(sep, 2));
         }
     }
+    if (c->d2 || c->g1 || c->g2 || c->g3) {
+        /* functions.c:574-669 (auto), 856-903 (den), 1144-1212 (rsd), 1213-1254 (d1),
+           1255-1428 (among d2, g1-g3); the end points' values in flat sky too, as there */
+        const double a1 = ak(&c->a, z1), a2 = ak(&c->a, z2);
+        const double s1 = ak(&c->s1, z1), s2 = ak(&c->s2, z2);
+        const double fevo1 = ak(&c->fevo1, z1), fevo2 = ak(&c->fevo2, z2);
+        const double Om = c->Omega_m, h2 = pow(c->H0h, 2), h4 = pow(c->H0h, 4);
+        const double I40 = ak(&c->div, sep) - orc_bicubic_eval(c->ren, chi1, chi2);
+        const double I20 = I(c, T20, sep), I22 = I(c, T22, sep), I31 = I(c, T31, sep);
+        const double wr1 = 2. / 3 * pow(sep, 2) - (1 - pow(costheta, 2)) * pow(chi1, 2);
+        const double wr2 = 2. / 3 * pow(sep, 2) - (1 - pow(costheta, 2)) * pow(chi2, 2);
+        /* the g sources' end-point factors (1 + G), (5 s - 2), (f - 1) */
+        const int gon[3] = {c->g1, c->g2, c->g3};
+        const double g_1[3] = {1 + G1, 5 * s1 - 2, f1 - 1}, g_2[3] = {1 + G2, 5 * s2 - 2, f2 - 1};
+        if (c->d2) result += (3 - fevo1) * (3 - fevo2) * pow(H1, 2) * pow(H2, 2) * f1 * f2 * I40;
+        for (int i = 0; i < 3; ++i)
+            if (gon[i]) result += h4 * (9 * pow(Om, 2) * g_1[i] * g_2[i] / 4 / a1 / a2 * I40);
+        if (c->den && c->d2)
+            result += ((3 - fevo2) * b1 * f2 * pow(H2, 2) + (3 - fevo1) * b2 * f1 * pow(H1, 2)) * I20;
+        for (int i = 0; i < 3; ++i)
+            if (c->den && gon[i])
+                result += h2 * (-(b1 * 3 * Om / 2 / a2 * g_2[i] + b2 * 3 * Om / 2 / a1 * g_1[i]) * I20);
+        if (c->rsd && c->d2)
+            result += ((3 - fevo2) / 3 * f1 * f2 * pow(H2, 2) + (3 - fevo1) / 3 * f2 * f1 * pow(H1, 2)) * I20 -
+                      ((3 - fevo2) * f1 * f2 * pow(H2, 2) * wr2 + (3 - fevo1) * f2 * f1 * pow(H1, 2) * wr1) * I22;
+        for (int i = 0; i < 3; ++i)
+            if (c->rsd && gon[i])
+                result += h2 * (-(Om / 2. / a2 * f1 * g_2[i] + Om / 2. / a1 * f2 * g_1[i]) * I20 +
+                                (3 * Om / 2. / a2 * f1 * g_2[i] * wr2 + 3 * Om / 2. / a1 * f2 * g_1[i] * wr1) * I22);
+        if (c->d1 && c->d2)
+            result += -((3 - fevo2) * H1 * pow(H2, 2) * f1 * f2 * (chi2 * costheta - chi1) +
+                        (3 - fevo1) * H2 * pow(H1, 2) * f2 * f1 * (chi1 * costheta - chi2)) *
+                      I31;
+        for (int i = 0; i < 3; ++i)
+            if (c->d1 && gon[i])
+                result += h2 * ((3 * Om / 2. / a2 * H1 * f1 * g_2[i] * (chi2 * costheta - chi1) +
+                                 3 * Om / 2. / a1 * H2 * f2 * g_1[i] * (chi1 * costheta - chi2)) *
+                                I31);
+        for (int i = 0; i < 3; ++i)
+            if (c->d2 && gon[i])
+                result += h2 * (-(3 * (3 - fevo1) * Om / 2. / a2 * pow(H1, 2) * f1 * g_2[i] +
+                                  3 * (3 - fevo2) * Om / 2. / a1 * pow(H2, 2) * f2 * g_1[i]) *
+                                I40);
+        for (int i = 0; i < 3; ++i)
+            for (int j = i + 1; j < 3; ++j)
+                if (gon[i] && gon[j])
+                    result += h4 * (9 * pow(Om, 2) / 4. * (g_1[i] * g_2[j] + g_2[i] * g_1[j]) / a1 / a2 * I40);
+    }
     if (!isfinite(result)) return NAN;
     if (c->flatsky) return result * pow(ak(&c->D1, z_mean), 2);
     return result * ak(&c->D1, z1) * ak(&c->D1, z2);
@@ -166,12 +219,49 @@ static double mp_integrand(double x, void *vp)
  * bg: z grid (nbg) with chi, D1, f, conformal H, G1, G2; bias tables on zb (nb).
  * quad: 0 = QAG-GK61 with epsrel, M > 0 = M-point Gauss-Legendre in x.
  */
+/* the d2 / g1-g3 part of the context (dg may be NULL: none) */
+static int dg_init(mp_ctx *c, size_t nbg, const double *bg_z, const orc_dg *dg)
+{
+    if (!dg || !(dg->d2 || dg->g1 || dg->g2 || dg->g3)) return ORC_OK;
+    c->d2 = dg->d2; c->g1 = dg->g1; c->g2 = dg->g2; c->g3 = dg->g3;
+    c->H0h = dg->H0h;
+    c->Omega_m = dg->Omega_m;
+    int st = ak_init(&c->a, nbg, bg_z, dg->bg_a);
+    if (st == ORC_OK) st = ak_init(&c->s1, dg->nmb, dg->mb_z, dg->s1);
+    if (st == ORC_OK) st = ak_init(&c->s2, dg->nmb, dg->mb_z, dg->s2);
+    if (st == ORC_OK) st = ak_init(&c->fevo1, dg->nmb, dg->mb_z, dg->fevo1);
+    if (st == ORC_OK) st = ak_init(&c->fevo2, dg->nmb, dg->mb_z, dg->fevo2);
+    if (st == ORC_OK) st = ak_init(&c->div, dg->ndiv, dg->div_x, dg->div_y);
+    if (st == ORC_OK) {
+        c->ren = orc_bicubic_new(dg->nren, dg->nren, dg->ren_chi, dg->ren_chi, dg->ren);
+        if (!c->ren) st = ORC_ENOMEM;
+    }
+    return st;
+}
+
+static void dg_free(mp_ctx *c)
+{
+    free(c->a.b); free(c->s1.b); free(c->s2.b); free(c->fevo1.b); free(c->fevo2.b); free(c->div.b);
+    orc_bicubic_free(c->ren);
+}
+
 int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
                    const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
                    const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
                    const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
                    const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int quad,
                    double epsrel, double *out)
+{
+    return orc_multipoles_dg(tab_x, tab_y, tab_n, nbg, bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, nb, zb, bias1,
+                             bias2, den, rsd, d1, flatsky, nz, z_mean, nl, ls, nsep, sep, quad, epsrel, NULL, out);
+}
+
+int orc_multipoles_dg(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
+                      const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                      const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                      const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                      const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int quad,
+                      double epsrel, const orc_dg *dg, double *out)
 {
     mp_ctx c;
     memset(&c, 0, sizeof(c));
@@ -189,6 +279,7 @@ int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const
     if (st == ORC_OK) st = ak_init(&c.G2, nbg, bg_z, bg_G2);
     if (st == ORC_OK) st = ak_init(&c.b1, nb, zb, bias1);
     if (st == ORC_OK) st = ak_init(&c.b2, nb, zb, bias2);
+    if (st == ORC_OK) st = dg_init(&c, nbg, bg_z, dg);
     double *glx = NULL, *glw = NULL;
     if (st == ORC_OK && quad > 0) {
         glx = (double *)malloc(sizeof(double) * (size_t)quad);
@@ -216,6 +307,7 @@ int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const
     for (int t = 0; t < NTAB; ++t) if (c.have[t]) free(c.tab[t].b);
     free(c.chi.b); free(c.z_of_chi.b); free(c.D1.b); free(c.f.b); free(c.cH.b); free(c.G1.b); free(c.G2.b);
     free(c.b1.b); free(c.b2.b);
+    dg_free(&c);
     return st;
 }
 
@@ -309,9 +401,12 @@ static qf qpl(int l, qf x)
     return p1;
 }
 
+/* out[0..NTAB) the standard tables' coefficients, out[NTAB] that of I_0^4 and
+   out[NTAB + 1] minus that of the renormalisation grid's value at (chi1, chi2) (the d2 /
+   g terms), all times G */
 static void hp_coefficients(const mp_ctx *c, double z_mean, double mu_d, double sep_d, qf *out)
 {
-    for (int t = 0; t < NTAB; ++t) out[t] = 0;
+    for (int t = 0; t < NTAB + 2; ++t) out[t] = 0;
     const double chi_mean_d = ak(&c->chi, z_mean);
     const qf chi_mean = chi_mean_d, mu = mu_d, sep = sep_d;
     const qf chi1 = chi_mean - sep * mu / 2;
@@ -394,8 +489,52 @@ static void hp_coefficients(const mp_ctx *c, double z_mean, double mu_d, double 
                         s2;
         }
     }
+    if (c->d2 || c->g1 || c->g2 || c->g3) {
+        /* functions.c:574-669, 856-903, 1144-1428 as in orc_functions_nonintegrated */
+        const qf a1 = ak(&c->a, z1), a2 = ak(&c->a, z2);
+        const qf mb1 = ak(&c->s1, z1), mb2 = ak(&c->s2, z2);   /* magnification biases */
+        const qf fevo1 = ak(&c->fevo1, z1), fevo2 = ak(&c->fevo2, z2);
+        const qf Om = c->Omega_m, h2 = (qf)c->H0h * (qf)c->H0h, h4 = h2 * h2;
+        const qf ren = orc_bicubic_eval(c->ren, chi1_d, chi2_d);
+        const qf wr1 = (qf)2 / 3 * s2 - (1 - ct2) * chi1 * chi1;   /* s2 = sep^2 */
+        const qf wr2 = (qf)2 / 3 * s2 - (1 - ct2) * chi2 * chi2;
+        const int gon[3] = {c->g1, c->g2, c->g3};
+        const qf g_1[3] = {1 + G1, 5 * mb1 - 2, f1 - 1}, g_2[3] = {1 + G2, 5 * mb2 - 2, f2 - 1};
+        qf k40 = 0;   /* the coefficient of I_0^4 - ren */
+        if (c->d2) k40 += (3 - fevo1) * (3 - fevo2) * H1 * H1 * H2 * H2 * f1 * f2;
+        for (int i = 0; i < 3; ++i)
+            if (gon[i]) k40 += h4 * (9 * Om * Om * g_1[i] * g_2[i] / 4 / a1 / a2);
+        for (int i = 0; i < 3; ++i)
+            if (c->d2 && gon[i])
+                k40 += h2 * (-(3 * (3 - fevo1) * Om / 2 / a2 * H1 * H1 * f1 * g_2[i] +
+                               3 * (3 - fevo2) * Om / 2 / a1 * H2 * H2 * f2 * g_1[i]));
+        for (int i = 0; i < 3; ++i)
+            for (int j = i + 1; j < 3; ++j)
+                if (gon[i] && gon[j]) k40 += h4 * (9 * Om * Om / 4 * (g_1[i] * g_2[j] + g_2[i] * g_1[j]) / a1 / a2);
+        out[NTAB] += k40;
+        out[NTAB + 1] -= k40 * ren;
+        if (c->den && c->d2) out[T20] += (3 - fevo2) * b1 * f2 * H2 * H2 + (3 - fevo1) * b2 * f1 * H1 * H1;
+        for (int i = 0; i < 3; ++i)
+            if (c->den && gon[i]) out[T20] += h2 * (-(b1 * 3 * Om / 2 / a2 * g_2[i] + b2 * 3 * Om / 2 / a1 * g_1[i]));
+        if (c->rsd && c->d2) {
+            out[T20] += (3 - fevo2) / 3 * f1 * f2 * H2 * H2 + (3 - fevo1) / 3 * f2 * f1 * H1 * H1;
+            out[T22] -= (3 - fevo2) * f1 * f2 * H2 * H2 * wr2 + (3 - fevo1) * f2 * f1 * H1 * H1 * wr1;
+        }
+        for (int i = 0; i < 3; ++i)
+            if (c->rsd && gon[i]) {
+                out[T20] += h2 * (-(Om / 2 / a2 * f1 * g_2[i] + Om / 2 / a1 * f2 * g_1[i]));
+                out[T22] += h2 * (3 * Om / 2 / a2 * f1 * g_2[i] * wr2 + 3 * Om / 2 / a1 * f2 * g_1[i] * wr1);
+            }
+        if (c->d1 && c->d2)
+            out[T31] += -((3 - fevo2) * H1 * H2 * H2 * f1 * f2 * (chi2 * costheta - chi1) +
+                          (3 - fevo1) * H2 * H1 * H1 * f2 * f1 * (chi1 * costheta - chi2));
+        for (int i = 0; i < 3; ++i)
+            if (c->d1 && gon[i])
+                out[T31] += h2 * (3 * Om / 2 / a2 * H1 * f1 * g_2[i] * (chi2 * costheta - chi1) +
+                                  3 * Om / 2 / a1 * H2 * f2 * g_1[i] * (chi1 * costheta - chi2));
+    }
     const qf G = c->flatsky ? (qf)ak(&c->D1, z_mean) * (qf)ak(&c->D1, z_mean) : (qf)ak(&c->D1, z1) * (qf)ak(&c->D1, z2);
-    for (int t = 0; t < NTAB; ++t) out[t] *= G;
+    for (int t = 0; t < NTAB + 2; ++t) out[t] *= G;
 }
 
 int orc_mu_weights_hp(size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
@@ -403,6 +542,20 @@ int orc_mu_weights_hp(size_t nbg, const double *bg_z, const double *bg_chi, cons
                       const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
                       const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, size_t nmu,
                       const double *mu, const double *wmu, double *out)
+{
+    return orc_mu_weights_hp_dg(nbg, bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, nb, zb, bias1, bias2, den, rsd,
+                                d1, flatsky, nz, z_mean, nl, ls, nsep, sep, nmu, mu, wmu, NULL, out, NULL);
+}
+
+/* with the d2 / g terms of dg (NULL: none): out [nz][nl][8 or 9][nsep], the ninth table
+   I_0^4 when dg is given, and out_ren [nz][nl][nsep] (with dg) = the renormalisation
+   grid's part of the multipoles */
+int orc_mu_weights_hp_dg(size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                         const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb,
+                         const double *zb, const double *bias1, const double *bias2, int den, int rsd, int d1,
+                         int flatsky, size_t nz, const double *z_mean, size_t nl, const int *ls, size_t nsep,
+                         const double *sep, size_t nmu, const double *mu, const double *wmu, const orc_dg *dg,
+                         double *out, double *out_ren)
 {
     mp_ctx c;
     memset(&c, 0, sizeof(c));
@@ -416,26 +569,32 @@ int orc_mu_weights_hp(size_t nbg, const double *bg_z, const double *bg_chi, cons
     if (st == ORC_OK) st = ak_init(&c.G2, nbg, bg_z, bg_G2);
     if (st == ORC_OK) st = ak_init(&c.b1, nb, zb, bias1);
     if (st == ORC_OK) st = ak_init(&c.b2, nb, zb, bias2);
+    if (st == ORC_OK) st = dg_init(&c, nbg, bg_z, dg);
+    const int with_dg = c.d2 || c.g1 || c.g2 || c.g3;
+    const size_t nt = with_dg ? NTAB + 1 : NTAB;
     if (st == ORC_OK) {
         #pragma omp parallel for schedule(dynamic)
         for (size_t idx = 0; idx < nz * nsep; ++idx) {
             const size_t iz = idx / nsep, ir = idx % nsep;
-            qf acc[4][NTAB];
+            qf acc[4][NTAB + 2];
             memset(acc, 0, sizeof(acc));
             for (size_t q = 0; q < nmu; ++q) {
-                qf cq[NTAB];
+                qf cq[NTAB + 2];
                 hp_coefficients(&c, z_mean[iz], mu[q], sep[ir], cq);
                 for (size_t il = 0; il < nl && il < 4; ++il) {
                     const qf pw = (qf)wmu[q] * qpl(ls[il], (qf)mu[q]);
-                    for (int t = 0; t < NTAB; ++t) acc[il][t] += pw * cq[t];
+                    for (int t = 0; t < NTAB + 2; ++t) acc[il][t] += pw * cq[t];
                 }
             }
-            for (size_t il = 0; il < nl && il < 4; ++il)
-                for (int t = 0; t < NTAB; ++t)
-                    out[((iz * nl + il) * NTAB + (size_t)t) * nsep + ir] = (double)((qf)(2 * ls[il] + 1) / 2 * acc[il][t]);
+            for (size_t il = 0; il < nl && il < 4; ++il) {
+                const qf norm = (qf)(2 * ls[il] + 1) / 2;
+                for (size_t t = 0; t < nt; ++t) out[((iz * nl + il) * nt + t) * nsep + ir] = (double)(norm * acc[il][t]);
+                if (with_dg && out_ren) out_ren[(iz * nl + il) * nsep + ir] = (double)(norm * acc[il][NTAB + 1]);
+            }
         }
     }
     free(c.chi.b); free(c.z_of_chi.b); free(c.D1.b); free(c.f.b); free(c.cH.b); free(c.G1.b); free(c.G2.b);
     free(c.b1.b); free(c.b2.b);
+    dg_free(&c);
     return st;
 }
