@@ -6,10 +6,13 @@ degree <= 30 with respect to the weight P_30(x) on [-1, 1].  The weights follow 
 exactness on even monomials up to degree 60 (odd ones vanish by symmetry), and the
 rule is then checked for exactness up to degree 91.  Output order follows
 QUADPACK's dqk61: xgk[0] is the largest node, odd indices are the Gauss nodes,
-xgk[30] = 0.  The same script also prints Gauss-Legendre rules used elsewhere.
+xgk[30] = 0.
 
-Usage: python oracle/gen_gk61.py > oracle/gk61_table.h
+Usage: python gen_gk61.py [output]  (default: gk61_nodes.h beside this file)
 """
+import os
+import sys
+
 import mpmath as mp
 
 mp.mp.dps = 50
@@ -43,7 +46,7 @@ def gauss_legendre(n):
     return xs, ws  # descending
 
 
-def main():
+def main(out_path):
     gx, gw = gauss_legendre(N)
     gx_pos = [x for x in gx if x > 0]
     gw_pos = [w for x, w in zip(gx, gw) if x > 0]
@@ -95,14 +98,26 @@ def main():
     for j in range(31, 46):
         s = mp.fsum(2 * w * x ** (2 * j) for x, w in zip(xgk[:30], wgk[:30]))
         assert abs(s - mp.mpf(2) / (2 * j + 1)) < mp.mpf(10) ** -40, j
-    print("/* 61-point Gauss-Kronrod rule in QUADPACK dqk61 layout (xgk descending, odd")
-    print("   indices are the 30-point Gauss nodes).  Generated by oracle/gen_gk61.py. */")
-    for name, arr in (("GK61_XGK", xgk), ("GK61_WGK", list(wgk)), ("GK61_WG", gw_pos)):
-        print("static const double %s[%d] = {" % (name, len(arr)))
-        for v in arr:
-            print("    %s," % mp.nstr(v, 25, strip_zeros=False, min_fixed=-1, max_fixed=1))
-        print("};")
+    # the header the engine includes: the 31 Kronrod nodes xgk[0..30] with their weights,
+    # and the 15 Gauss weights of the nodes xgk[1], xgk[3], ..., xgk[29]
+    def fmt(v):
+        """25 significant digits in d.ddd...eN form"""
+        if v == 0:
+            return "0.0"
+        e10 = int(mp.floor(mp.log10(abs(v))))
+        return "%se%d" % (mp.nstr(v / mp.mpf(10) ** e10, 25, strip_zeros=False, min_fixed=-1, max_fixed=2), e10)
 
+    out = [
+        "/* Kronrod nodes (descending, last = 0) and weights of the 61-point Gauss-Kronrod",
+        "   rule; same values as QUADPACK dqk61 (generated at 50 digits, see gen_gk61.py). */",
+        "static const double GK61K_X[31] = {%s};" % ", ".join(fmt(x) for x in xgk),
+        "static const double GK61K_W[31] = {%s};" % ", ".join(fmt(w) for w in wgk),
+        "/* 30-point Gauss weights of the nodes GK61K_X[1], GK61K_X[3], ..., GK61K_X[29] */",
+        "static const double GK61G_W[15] = {%s};" % ", ".join(fmt(w) for w in gw_pos),
+    ]
+    with open(out_path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
 
 if __name__ == "__main__":
-    main()
+    here = os.path.dirname(os.path.abspath(__file__))
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(here, "gk61_nodes.h"))
