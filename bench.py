@@ -90,21 +90,41 @@ def hbm_bytes(n_fft, n_in, nsep, ntab, nspec, nz, nl, nq=1):
     )
 
 
-def cpu_baseline(specs, sep, bg, phys, sample):
+def cpu_baseline(specs, sep, bg, phys, sample, runs=5, sample_1t=None):
+    """the oracle's reference sequence on the host: the median of `runs` timings at the
+    OpenMP thread count of the box (OMP_NUM_THREADS) on `sample` spectra, and the median
+    of `runs` one-thread timings on `sample_1t` spectra (default sample / 16)"""
     import oracle as O
-    sub = specs[:sample]
     bgo = dict(z=bg["z"], chi=bg["chi"], D1=bg["D1"], f=bg["f"], cH=bg["conformal_Hz"], G1=bg["G1"], G2=bg["G2"])
     zb = np.arange(11.0)
     b = np.full(11, phys["b"])
-    t0 = time.perf_counter()
-    O.reference_batch([(s[0], s[1]) for s in sub], [len(s[0]) for s in sub], bgo, zb, b, b, [phys["zbar"]], LS, sep,
-                      den=True, rsd=True, d1=False, flatsky=True, small_r=True)
-    dt = time.perf_counter() - t0
-    points = len(sub) * len(LS) * len(sep)
+
+    def rate(sub):
+        t0 = time.perf_counter()
+        O.reference_batch([(s[0], s[1]) for s in sub], [len(s[0]) for s in sub], bgo, zb, b, b, [phys["zbar"]], LS,
+                          sep, den=True, rsd=True, d1=False, flatsky=True, small_r=True)
+        dt = time.perf_counter() - t0
+        return len(sub) * len(LS) * len(sep) / dt, dt
+
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=points / dt, unit="points/s", cores=cores, kind="port",
+    prev = O.set_threads(cores)
+    try:
+        sub = specs[:sample]
+        multi = sorted(rate(sub) for _ in range(runs))
+        sub1 = specs[:sample_1t or max(1, sample // 16)]
+        O.set_threads(1)
+        single = sorted(rate(sub1) for _ in range(runs))
+    finally:
+        O.set_threads(prev)
+    med, med1 = multi[runs // 2], single[runs // 2]
+    return dict(value=med[0], unit="points/s", cores=cores, kind="port",
                 sample=f"{len(sub)} spectra x {len(LS)} l x {len(sep)} seps, reference sequence (8 FFTLog tables with "
-                       f"small-r QAG nodes, QAG-GK61 mu-integral per point), forward only, {dt:.2f} s wall")
+                       f"small-r QAG nodes, QAG-GK61 mu-integral per point), forward only; median of {runs} runs "
+                       f"({med[1]:.2f} s wall each) at {cores} threads",
+                runs=[r[0] for r in multi],
+                one_thread={"value": med1[0], "unit": "points/s", "cores": 1,
+                            "sample": f"{len(sub1)} spectra, median of {runs} runs ({med1[1]:.2f} s wall each)",
+                            "runs": [r[0] for r in single]})
 
 
 def plain_path(specs, sep, A, sample):
@@ -578,7 +598,7 @@ def main():
                    "launch": "hipGraph replay of the captured step" if args.graph else "eager (one C call)",
                    "autotune": tuned,
                    "other_configs": "bench.py --workload fullsky | covariance"},
-        "roofline": {"bound": "mfma", "kernel": dom,
+        "roofline": {"bound": "mfma", "kernel": dom, "unit_kind": "FP64 VALU",
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
                      "note": "FP64 VALU bound (no MFMA on this path); peak = MI355X FP64 78.6 TFLOP/s; "

@@ -130,6 +130,7 @@ int orc_mu_weights_hp_dg(size_t nbg, const double *bg_z, const double *bg_chi, c
                          const double *sep, size_t nmu, const double *mu, const double *wmu, const orc_dg *dg,
                          double *out, double *out_ren);
 
+int orc_set_threads(int n);
 int orc_reference_batch(int nspec, const long long *off, const double *k, const double *p, const long long *nfft,
                         size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
                         const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,

@@ -78,6 +78,7 @@ def _declare(L):
     L.orc_covariance_assemble.argtypes = [s, s, ctypes.POINTER(ctypes.c_int), s, p, ctypes.POINTER(ctypes.c_int),
                                           p, p, p, p, p, p, p, i, i, i, i, p, p, p]
     ll = ctypes.POINTER(ctypes.c_longlong)
+    L.orc_set_threads.argtypes = [i]
     L.orc_reference_batch.argtypes = [i, ll, p, p, ll, s, p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
                                       ctypes.POINTER(ctypes.c_int), s, p, i, d, p]
     L.orc_tables_at_sep_batch.argtypes = [i, ll, p, p, ll, i, ctypes.POINTER(ctypes.c_int),
@@ -351,6 +352,11 @@ def covariance_assemble(ls, sep, bias, f, D1, nbar, vol, pix, pk_int, pk2_int, p
                                          int(cosmic), int(window), _p(pk_int), _p(pk2_int), _p(out)),
            "covariance_assemble")
     return out.reshape(nz, nl, R, nl, R)
+
+
+def set_threads(n):
+    """OpenMP thread count of the oracle's parallel loops; returns the previous one."""
+    return int(lib().orc_set_threads(int(n)))
 
 
 def reference_batch(specs, nfft, bg, bias_z, bias1, bias2, z_mean, ls, sep, den=True, rsd=True, d1=False,

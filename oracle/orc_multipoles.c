@@ -11,6 +11,7 @@
  * reference builds them with the default `interp_method`).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -597,4 +598,13 @@ int orc_mu_weights_hp_dg(size_t nbg, const double *bg_z, const double *bg_chi, c
     free(c.b1.b); free(c.b2.b);
     dg_free(&c);
     return st;
+}
+
+/* OpenMP threads of the oracle's parallel loops (the CPU baseline is timed at the box's
+   thread count and at one thread); returns the previous setting */
+int orc_set_threads(int n)
+{
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
 }
