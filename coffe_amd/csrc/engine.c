@@ -1079,12 +1079,14 @@ static void spline_node_slopes(int n, const double *x, const double *y, int ys, 
     out[(n - 1) * os] = (y[(n - 1) * ys] - y[(n - 2) * ys]) / h + h * (M[n - 2] + 2.0 * M[n - 1]) / 6.0;
 }
 
-int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned terms, unsigned sources, int flatsky,
-                        int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
-                        const int *tab_l, int nsep, const double *sep, int nmu, double *d_A, double *d_ren,
-                        void *stream)
+/* the mu rule: rule_mu / rule_w (nmu nodes) or, when NULL, the nmu-node Gauss-Legendre rule */
+static int mu_weights_impl(coffe_engine *e, const coffe_mu_inputs *in, unsigned terms, unsigned sources, int flatsky,
+                           int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
+                           const int *tab_l, int nsep, const double *sep, int nmu, const double *rule_mu,
+                           const double *rule_w, double *d_A, double *d_ren, void *stream)
 {
-    if (!e || !in || nz <= 0 || nl <= 0 || nl > 4 || ntab <= 0 || ntab > CE_MAX_TABLES + 1 || nsep <= 0 || nmu < 2 ||
+    if (!e || !in || nz <= 0 || nl <= 0 || nl > 4 || ntab <= 0 || ntab > CE_MAX_TABLES + 1 || nsep <= 0 ||
+        nmu < (rule_mu ? 1 : 2) || nmu > (1 << 20) || (!rule_mu) != (!rule_w) ||
         terms == 0 || terms > 1023 || !d_A || !zmean || !ls || !tab_n || !tab_l || !sep || (paired && nz != nsep))
         return COFFE_EINVAL;
     const int nbg = in->nbg, nb = in->nb;
@@ -1155,7 +1157,18 @@ int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned ter
     const size_t ncoef = 3 * (7 * (size_t)nbg + 2 * (size_t)nb + 4 * (size_t)nmb);
     const size_t total = sizeof(double) * (nstage + ncoef);
     int owned = 0;
-    const double *h_mu = gl_nodes(nmu, &owned);
+    const double *h_mu = NULL;
+    if (rule_mu) {
+        double *r = (double *)malloc(sizeof(double) * 2 * (size_t)nmu);
+        if (r) {
+            memcpy(r, rule_mu, sizeof(double) * (size_t)nmu);
+            memcpy(r + nmu, rule_w, sizeof(double) * (size_t)nmu);
+            owned = 1;
+        }
+        h_mu = r;
+    } else {
+        h_mu = gl_nodes(nmu, &owned);
+    }
     if (!h_mu) return COFFE_ENOMEM;
     /* the inputs are staged in the engine's scratch (the drop-in calls share it under the
        lock).  The call is asynchronous on the stream: the next user of the scratch waits
@@ -1186,7 +1199,8 @@ int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned ter
 #define CE_STAGE(src, n)                                                                 \
     ({                                                                                   \
         const size_t nn_ = (size_t)(n);                                                  \
-        if ((src)) memcpy(hp, (src), sizeof(double) * nn_);                              \
+        const double *src_ = (src);                                                      \
+        if (src_) memcpy(hp, src_, sizeof(double) * nn_);                                \
         else memset(hp, 0, sizeof(double) * nn_);                                        \
         const double *r_ = dp;                                                           \
         dp += nn_;                                                                       \
@@ -1262,6 +1276,27 @@ int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned ter
     pthread_mutex_unlock(&e->lock);
     if (owned) free((void *)h_mu);
     return st;
+}
+
+int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned terms, unsigned sources, int flatsky,
+                        int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
+                        const int *tab_l, int nsep, const double *sep, int nmu, double *d_A, double *d_ren,
+                        void *stream)
+{
+    return mu_weights_impl(e, in, terms, sources, flatsky, paired, nz, zmean, nl, ls, ntab, tab_n, tab_l, nsep, sep,
+                           nmu, NULL, NULL, d_A, d_ren, stream);
+}
+
+int coffe_mu_weights_rule(coffe_engine *e, const coffe_mu_inputs *in, unsigned terms, unsigned sources, int flatsky,
+                          int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
+                          const int *tab_l, int nsep, const double *sep, int nmu, const double *mu,
+                          const double *wmu, double *d_A, double *d_ren, void *stream)
+{
+    if (!mu || !wmu) return COFFE_EINVAL;
+    for (int q = 0; q < nmu; ++q)
+        if (!(mu[q] >= -1.0 && mu[q] <= 1.0) || !isfinite(wmu[q])) return COFFE_EDOMAIN;
+    return mu_weights_impl(e, in, terms, sources, flatsky, paired, nz, zmean, nl, ls, ntab, tab_n, tab_l, nsep, sep,
+                           nmu, mu, wmu, d_A, d_ren, stream);
 }
 
 /* ---- small-separation nodes --------------------------------------------------- */

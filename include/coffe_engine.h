@@ -256,6 +256,17 @@ int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned ter
                         void *stream);
 
 /*
+ * coffe_mu_weights_ex with an explicit mu rule of nmu >= 1 nodes mu[q] in [-1, 1] and
+ * weights wmu[q] in place of the Gauss-Legendre one (A = (2l+1)/2 sum_q wmu_q P_l(mu_q)
+ * G c_t): one node mu with weight 2 and l = 0 gives the local correlation function at
+ * mu (functions_nonintegrated, functions.c:89-1484, the reference's CORRFUNC output).
+ */
+int coffe_mu_weights_rule(coffe_engine *e, const coffe_mu_inputs *in, unsigned terms, unsigned sources, int flatsky,
+                          int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
+                          const int *tab_l, int nsep, const double *sep, int nmu, const double *mu,
+                          const double *wmu, double *d_A, double *d_ren, void *stream);
+
+/*
  * The reference's full table (coffe_integrals_renormalizable, integrals.c:459-573):
  * out_x/out_y (caller-allocated, N + 43 entries) = [r = 0, the 42 min_sep values
  * (integrals.c:111-126), the N FFTLog nodes], y = I_l^n(r), times r^(n-l) when n >= l.
