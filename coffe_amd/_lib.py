@@ -78,6 +78,8 @@ def load_library():
                                          ip, i, ip, ip, i, dp, i, vp, vp]
     L.coffe_mu_weights_ex.argtypes = [vp, vp, ctypes.c_uint, ctypes.c_uint, i, i, i, dp, i, ip, i, ip, ip, i, dp, i,
                                       vp, vp, vp]
+    L.coffe_mu_weights_rule.argtypes = [vp, vp, ctypes.c_uint, ctypes.c_uint, i, i, i, dp, i, ip, i, ip, ip, i, dp,
+                                        i, dp, dp, vp, vp, vp]
     L.coffe_integral_table.argtypes = [i, i, sz, dp, dp, sz, d, d, dp, dp]
     L.coffe_divergent_tables.argtypes = [dp, dp, sz, d, d, dp, sz, dp, sz, dp, dp, dp, ctypes.POINTER(i),
                                          ctypes.POINTER(i)]

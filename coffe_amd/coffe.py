@@ -16,7 +16,7 @@ multipoles are not part of this engine and raise NotImplementedError.
 import numpy as np
 
 from . import background as _background
-from .representation import AverageMultipoles, Covariance, Multipoles
+from .representation import AverageCovariance, AverageMultipoles, Corrfunc, Covariance, Multipoles
 
 TABLES = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]   # integrals.c:599-608
 _UNSUPPORTED = ("has_lensing", "has_g4", "has_g5")
@@ -66,10 +66,10 @@ class Coffe:
     _defaults = dict(
         omega_cdm=0.25, omega_baryon=0.05, omega_gamma=9e-5, h=0.67, w0=-1.0, wa=0.0,
         sep=5.0 + 3.0 * np.arange(100), l=[0, 2, 4], z_mean=[1.0], deltaz=[0.2], fsky=[1.0], pixelsize=[10.0],
-        z_min=[0.9], z_max=[1.1],
+        z_min=[0.9], z_max=[1.1], mu=[0.5],
         number_density1=[1e-3], number_density2=[1e-3],
         has_density=True, has_rsd=False, has_d1=False, has_d2=False, has_g1=False, has_g2=False, has_g3=False,
-        has_flatsky_local=False,
+        has_flatsky_local=False, has_only_cross_correlations=False,
         integrals_fftlog_sampling=10000, covariance_integration_sampling=8000,
         covariance_cosmic=True, covariance_mixed=True, covariance_poisson=True,
         has_binned_covariance=False, covariance_populations=[1, 1, 1, 1],
@@ -94,14 +94,14 @@ class Coffe:
         self._tables = None
         self._divergent = None
         self._ren = None
-        self._multipoles = None
-        self._average_multipoles = None
-        self._covariance = None
+        self._free_results()
 
     def _free_results(self):
         self._multipoles = None
         self._average_multipoles = None
         self._covariance = None
+        self._average_covariance = None
+        self._corrfunc = None
 
     def _engine_handle(self):
         if self._engine is None:
@@ -194,6 +194,9 @@ class Coffe:
                      lambda s, v: s._set_list("z_min", v, lambda x: 0 <= x < 10, s._free_results))
     z_max = property(lambda s: s._p["z_max"].copy(),
                      lambda s, v: s._set_list("z_max", v, lambda x: 0 <= x < 10, s._free_results))
+    # the angles of the correlation function, in (-1, 1) (coffe.pyx:1589-1627)
+    mu = property(lambda s: s._p["mu"].copy(),
+                  lambda s, v: s._set_list("mu", v, lambda x: -1 < x < 1, s._free_results))
     l = property(lambda s: s._p["l"].copy(),  # noqa: E741
                  lambda s, v: s._set_list("l", v, lambda x: 0 <= x < 10, s._free_results, as_int=True))
     deltaz = property(lambda s: s._p["deltaz"].copy(),
@@ -245,6 +248,9 @@ class Coffe:
     has_g3 = property(lambda s: s._p["has_g3"], lambda s, v: s._set_flag("has_g3", v))
     has_flatsky_local = property(lambda s: s._p["has_flatsky_local"],
                                  lambda s, v: s._set_flag("has_flatsky_local", v))
+    # only the cross terms of the contributions (coffe.pyx:1121-1133, functions.c:129-669)
+    has_only_cross_correlations = property(lambda s: s._p["has_only_cross_correlations"],
+                                           lambda s, v: s._set_flag("has_only_cross_correlations", v))
     covariance_cosmic = property(lambda s: s._p["covariance_cosmic"], lambda s, v: s._set_flag("covariance_cosmic", v))
     covariance_mixed = property(lambda s: s._p["covariance_mixed"], lambda s, v: s._set_flag("covariance_mixed", v))
     covariance_poisson = property(lambda s: s._p["covariance_poisson"],
@@ -493,6 +499,18 @@ class Coffe:
     def _dg_sources(self):
         return {k for k in _DG if self._p["has_" + k]}
 
+    def _terms(self, dg):
+        """COFFE_TERM_* mask of the contribution flags; has_only_cross_correlations drops
+        the auto terms (functions.c:129-669)."""
+        from .engine import TERM_D1_D1, TERM_DEN_DEN, TERM_RSD_RSD, terms_of
+        t = terms_of(self._p["has_density"], self._p["has_rsd"], self._p["has_d1"], dg is not None)
+        if self._p["has_only_cross_correlations"]:
+            if dg is not None:
+                raise NotImplementedError("has_only_cross_correlations with the d2 / g1-g3 terms: their mutual "
+                                          "terms are not separable from their auto terms in the engine")
+            t &= ~(TERM_DEN_DEN | TERM_RSD_RSD | TERM_D1_D1)
+        return t
+
     def _ren_grid(self, chi_max):
         """The 201-node renormalisation grid of I_0^4 on [0, chi_max] (integrals.c:848-907:
         chi_j = j / 200 chi_max), computed on the GPU for the current spectrum; cached."""
@@ -526,9 +544,10 @@ class Coffe:
             rows.append(_ak(x, y, seps))
         return np.array(rows)
 
-    def _multipole_values(self, zs, seps, ls):
+    def _multipole_values(self, zs, seps, ls, rule=None):
         """[z][l][r] multipoles: weights on the GPU (mu quadrature), tables at the separations
-        (with the d2 / g terms also the renormalisation grid's part, on the GPU)."""
+        (with the d2 / g terms also the renormalisation grid's part, on the GPU).  rule:
+        an explicit (mu nodes, weights) rule in place of the Gauss-Legendre one."""
         import torch
         from .engine import mu_weights
         bz = self._bias[1].x
@@ -537,14 +556,22 @@ class Coffe:
         # the grid of the d2 / g terms spans chi(max(z_mean + deltaz)) (integrals.c:862-871)
         zmax = max(float(z) + float(dz) for z, dz in zip(self._p["z_mean"], self._p["deltaz"]))
         tables, dg = self._weights_setup(self.comoving_distance(zmax))
-        ren = torch.empty((len(zs), len(ls), len(seps)), dtype=torch.float64, device="cuda") if dg else None
-        A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, zs, list(ls),
-                       tables, seps, den=self._p["has_density"], rsd=self._p["has_rsd"], d1=self._p["has_d1"],
-                       flatsky=self._p["has_flatsky_local"], nmu=self._p["multipole_quadrature_nodes"], dg=dg,
-                       ren_out=ren).cpu().numpy()
-        out = np.einsum("zltr,tr->zlr", A, self._table_values(tables, seps))
-        if ren is not None:
-            out += ren.cpu().numpy()
+        terms = self._terms(dg)
+        if not terms:
+            return np.zeros((len(zs), len(ls), len(seps)))
+        tv = self._table_values(tables, seps)
+        out = np.empty((len(zs), len(ls), len(seps)))
+        # the weights kernel takes at most 4 multipoles per launch
+        for l0 in range(0, len(ls), 4):
+            lc = [int(v) for v in ls[l0:l0 + 4]]
+            ren = torch.empty((len(zs), len(lc), len(seps)), dtype=torch.float64, device="cuda") if dg else None
+            A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, zs, lc,
+                           tables, seps, flatsky=self._p["has_flatsky_local"],
+                           nmu=self._p["multipole_quadrature_nodes"], terms=terms, dg=dg, ren_out=ren,
+                           rule=rule).cpu().numpy()
+            out[:, l0:l0 + len(lc)] = np.einsum("zltr,tr->zlr", A, tv)
+            if ren is not None:
+                out[:, l0:l0 + len(lc)] += ren.cpu().numpy()
         return out
 
     def compute_multipole(self, *, z, r, l, recompute=False):  # noqa: E741
@@ -571,6 +598,52 @@ class Coffe:
                                          for iz, z in enumerate(zs) for il, li in enumerate(ls)
                                          for ir, r in enumerate(seps)])
         return self._multipoles
+
+    # ---- correlation function -----------------------------------------------------
+    def _check_coords_corrfunc(self):
+        if not (len(self._p["mu"]) and len(self._p["sep"]) and len(self._p["z_mean"])):
+            raise ValueError("At least one of the following parameters is empty: mu, sep, z_mean")
+
+    def _corrfunc_values(self, zs, seps, mus):
+        """[z][mu][r] local correlation function (functions_nonintegrated, functions.c:
+        89-1484, at each mu): the weights of a one-node rule (the node mu, weight 2, l = 0)."""
+        out = np.empty((len(zs), len(mus), len(seps)))
+        for j, mu in enumerate(mus):
+            out[:, j, :] = self._multipole_values(zs, seps, [0], rule=([float(mu)], [2.0]))[:, 0, :]
+        return out
+
+    def compute_corrfunc(self, *, z, r, mu, recompute=False):
+        """The correlation function at (z_mean, r, mu) (coffe.pyx:3159-3230)."""
+        self._check_coords_corrfunc()
+        self._check_contributions()
+        _check_parameter("z", z, (int, float), 0, 15)
+        _check_parameter("r", r, (int, float), 0, 1500)
+        _check_parameter("mu", mu, (int, float), -1, 1)
+        if recompute:
+            self._free_all()
+        return float(self._corrfunc_values([float(z)], [float(r)], [float(mu)])[0, 0, 0])
+
+    def compute_corrfunc_bulk(self, recompute=False):
+        """The correlation function at every (z_mean, mu, sep), z-major then mu then r
+        (corrfunc.c:83-90)."""
+        self._check_coords_corrfunc()
+        self._check_contributions()
+        if recompute:
+            self._free_all()
+        if self._corrfunc is None:
+            zs, mus, seps = self._p["z_mean"], self._p["mu"], self._p["sep"]
+            vals = self._corrfunc_values(zs, seps, mus)
+            self._corrfunc = np.array([Corrfunc(z=float(z), r=float(r), mu=float(mu), value=float(vals[iz, im, ir]))
+                                       for iz, z in enumerate(zs) for im, mu in enumerate(mus)
+                                       for ir, r in enumerate(seps)])
+        return self._corrfunc
+
+    def maximum_separation(self, z_mean, deltaz):
+        """The largest separation of the bin [z_mean - deltaz, z_mean + deltaz]: 2 (chi(z_mean
+        + deltaz) - chi(z_mean)) (coffe.pyx:2764-2792)."""
+        _check_parameter("z_mean", z_mean, (int, float), 0, 15)
+        _check_parameter("deltaz", deltaz, (int, float), 0, z_mean)
+        return 2 * (self.comoving_distance(z_mean + deltaz) - self.comoving_distance(z_mean))
 
     # ---- redshift-averaged multipoles ----------------------------------------------
     def _check_coords_average_multipoles(self):
@@ -611,15 +684,23 @@ class Coffe:
         # the d2 / g grid spans chi(max(z_max)) for averaged multipoles (integrals.c:873-879)
         tables, dg = self._weights_setup(self.comoving_distance(float(max(self._p["z_max"]))))
         npair = len(seps) * nodes
-        ren = torch.empty((npair, len(ls)), dtype=torch.float64, device="cuda") if dg else None
-        A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, zn.ravel(),
-                       list(ls), tables, np.repeat(seps, nodes), den=self._p["has_density"], rsd=self._p["has_rsd"],
-                       d1=self._p["has_d1"], flatsky=self._p["has_flatsky_local"],
-                       nmu=self._p["multipole_quadrature_nodes"], dg=dg, ren_out=ren, paired=True).cpu().numpy()
-        A = A.reshape(len(seps), nodes, len(ls), len(tables))                 # [r][v][l][t]
-        out = np.einsum("rv,rvlt,tr->lr", wz, A, self._table_values(tables, seps))
-        if ren is not None:
-            out += np.einsum("rv,rvl->lr", wz, ren.cpu().numpy().reshape(len(seps), nodes, len(ls)))
+        terms = self._terms(dg)
+        if not terms:
+            return np.zeros((len(ls), len(seps)))
+        tv = self._table_values(tables, seps)
+        out = np.empty((len(ls), len(seps)))
+        for l0 in range(0, len(ls), 4):   # at most 4 multipoles per weights launch
+            lc = [int(v) for v in ls[l0:l0 + 4]]
+            ren = torch.empty((npair, len(lc)), dtype=torch.float64, device="cuda") if dg else None
+            A = mu_weights(self._engine_handle(), self._bg_tables(), bz, self._bias[1].y, self._bias[2].y,
+                           zn.ravel(), lc, tables, np.repeat(seps, nodes), flatsky=self._p["has_flatsky_local"],
+                           nmu=self._p["multipole_quadrature_nodes"], terms=terms, dg=dg, ren_out=ren,
+                           paired=True).cpu().numpy()
+            A = A.reshape(len(seps), nodes, len(lc), len(tables))                 # [r][v][l][t]
+            out[l0:l0 + len(lc)] = np.einsum("rv,rvlt,tr->lr", wz, A, tv)
+            if ren is not None:
+                out[l0:l0 + len(lc)] += np.einsum("rv,rvl->lr", wz, ren.cpu().numpy().reshape(len(seps), nodes,
+                                                                                              len(lc)))
         return out
 
     def compute_average_multipoles_bulk(self, recompute=False):
@@ -641,11 +722,39 @@ class Coffe:
         return self._average_multipoles
 
     # ---- covariance ---------------------------------------------------------------
-    def compute_covariance_bulk(self, recompute=False):
-        """Gaussian flat-sky covariance of the multipoles with the 2D FFTLog
-        (covariance.c:477-1103); records ordered z, l1, r1, l2, r2."""
+    def _covariance_values(self, zs, vols):
+        """cov[bin][l1][r1][l2][r2]: the 2D FFTLog integrals of P and P^2 and the Gaussian
+        assembly (covariance.c:477-1103) with the biases, f and D1 at each bin's redshift
+        zs[bin] and the volumes vols[bin]."""
         import torch
         from .covariance import CovPlan
+        k, pk = self._pk
+        ls, seps = [int(v) for v in self._p["l"]], self._p["sep"]
+        n1 = np.asarray(self._p["number_density1"], dtype=np.float64)
+        n2 = np.asarray(self._p["number_density2"], dtype=np.float64)
+        if np.any(n1 <= 0) or np.any(n2 <= 0):
+            raise ValueError("number_density1 and number_density2 must be positive for the covariance")
+        plan = CovPlan(self._engine_handle(), self._p["covariance_integration_sampling"], k[0], k[-1], ls, seps)
+        try:
+            integ = plan.integrals(torch.tensor(k, device="cuda"), torch.tensor(pk, device="cuda"), [len(k)])
+            # galaxy biases 1..4 only with a density contribution, f only with RSD
+            # (covariance.c:842-860); the populations pick which of them each field reads
+            bias = [[self._eval_bias(i, float(z)) if self._p["has_density"] else 0.0 for i in (1, 2, 3, 4)]
+                    for z in zs]
+            f = [self.growth_rate(float(z)) if self._p["has_rsd"] else 0.0 for z in zs]
+            D1 = [self.growth_factor(float(z)) for z in zs]
+            return plan.assemble(integ, np.array(bias), f, D1, n1, list(vols), self._p["pixelsize"],
+                                 poisson=self._p["covariance_poisson"], mixed=self._p["covariance_mixed"],
+                                 cosmic=self._p["covariance_cosmic"], pops=self._p["covariance_populations"],
+                                 nbar2=n2, window=self._p["has_binned_covariance"]).cpu().numpy()[0]
+        finally:
+            plan.close()
+
+    def compute_covariance_bulk(self, recompute=False):
+        """Gaussian flat-sky covariance of the multipoles with the 2D FFTLog
+        (covariance.c:477-1103); records ordered z, l1, r1, l2, r2.  The volume of bin k is
+        the shell 4 pi fsky (chi(z + deltaz)^3 - chi(z - deltaz)^3) / 3 (covariance.c:
+        793-806)."""
         self._check_coords_multipoles()
         self._check_contributions()
         self._need_pk()
@@ -657,36 +766,59 @@ class Coffe:
         if recompute:
             self._free_all()
         if self._covariance is None:
-            k, pk = self._pk
             ls, seps = [int(v) for v in self._p["l"]], self._p["sep"]
-            n1 = np.asarray(self._p["number_density1"], dtype=np.float64)
-            n2 = np.asarray(self._p["number_density2"], dtype=np.float64)
-            if np.any(n1 <= 0) or np.any(n2 <= 0):
-                raise ValueError("number_density1 and number_density2 must be positive for the covariance")
-            plan = CovPlan(self._engine_handle(), self._p["covariance_integration_sampling"], k[0], k[-1], ls, seps)
-            kt = torch.tensor(k, device="cuda")
-            pt = torch.tensor(pk, device="cuda")
-            integ = plan.integrals(kt, pt, [len(k)])
-            bias, f, D1, vol = [], [], [], []
+            vol = []
             for iz, z in enumerate(zs):
                 dz, fsky = self._p["deltaz"][iz], self._p["fsky"][iz]
-                # galaxy biases 1..4 only with a density contribution, f only with RSD
-                # (covariance.c:842-860); the populations pick which of them each field reads
-                b = [self._eval_bias(i, float(z)) if self._p["has_density"] else 0.0 for i in (1, 2, 3, 4)]
-                bias.append(b)
-                f.append(self.growth_rate(float(z)) if self._p["has_rsd"] else 0.0)
-                D1.append(self.growth_factor(float(z)))
                 chi_lo = self.comoving_distance(float(z - dz))
                 chi_hi = self.comoving_distance(float(z + dz))
                 vol.append(4 * np.pi * fsky * (chi_hi ** 3 - chi_lo ** 3) / 3.)
-            cov = plan.assemble(integ, np.array(bias), f, D1, n1, vol, self._p["pixelsize"],
-                                poisson=self._p["covariance_poisson"], mixed=self._p["covariance_mixed"],
-                                cosmic=self._p["covariance_cosmic"], pops=self._p["covariance_populations"],
-                                nbar2=n2, window=self._p["has_binned_covariance"]).cpu().numpy()[0]
-            plan.close()
+            cov = self._covariance_values(zs, vol)
             self._covariance = np.array([
                 Covariance(r1=float(seps[m]), r2=float(seps[n]), l1=ls[i], l2=ls[j], z=float(z),
                            value=float(cov[iz, i, m, j, n]))
                 for iz, z in enumerate(zs) for i in range(len(ls)) for m in range(len(seps))
                 for j in range(len(ls)) for n in range(len(seps))])
         return self._covariance
+
+    def _average_volume(self, zmin, zmax, fsky):
+        """4 pi fsky / (int_zmin^zmax dz / (H chi^2 (1 + z)) (H0 h)^2), H the conformal
+        Hubble rate (covariance.c:150-162, 808-837)."""
+        from scipy.integrate import quad
+        bg = self._background()
+        chi, cH = _Spline(bg["z"], bg["chi"]), _Spline(bg["z"], bg["conformal_Hz"])
+        integ = quad(lambda z: 1.0 / cH(z)[0] / chi(z)[0] ** 2 / (1 + z), zmin, zmax, epsabs=0, epsrel=1e-12,
+                     limit=200)[0]
+        hH0 = self._p["h"] * COFFE_H0
+        return 4 * np.pi * fsky / integ / hH0 ** 2
+
+    def compute_average_covariance_bulk(self, recompute=False):
+        """Gaussian covariance of the redshift-averaged multipoles (covariance.c:477-1103,
+        COVARIANCE_AVERAGE_MULTIPOLES; coffe.pyx:3482-3535): per bin [z_min, z_max] the
+        flat-sky covariance at z = (z_min + z_max)/2 with the volume of _average_volume;
+        records ordered bin, l1, r1, l2, r2."""
+        self._check_coords_multipoles()
+        self._check_contributions()
+        self._need_pk()
+        zmin, zmax = self._p["z_min"], self._p["z_max"]
+        per_bin = [self._p[k] for k in ("deltaz", "z_max", "number_density1", "number_density2", "pixelsize",
+                                        "fsky")]
+        if any(len(v) != len(zmin) for v in per_bin):
+            raise ValueError("Mismatching lengths for average covariance parameters (`z_min`, `z_max`, `deltaz`, "
+                             "`number_density1`, `number_density2`, `pixelsize`, `fsky`")
+        for a, b in zip(zmin, zmax):
+            self.maximum_separation((a + b) / 2, (b - a) / 2)
+        if recompute:
+            self._free_all()
+        if self._average_covariance is None:
+            ls, seps = [int(v) for v in self._p["l"]], self._p["sep"]
+            zs = [(a + b) / 2 for a, b in zip(zmin, zmax)]
+            vol = [self._average_volume(float(a), float(b), float(fs))
+                   for a, b, fs in zip(zmin, zmax, self._p["fsky"])]
+            cov = self._covariance_values(zs, vol)
+            self._average_covariance = np.array([
+                AverageCovariance(r1=float(seps[m]), r2=float(seps[n]), l1=ls[i], l2=ls[j], z_min=float(a),
+                                  z_max=float(b), value=float(cov[iz, i, m, j, n]))
+                for iz, (a, b) in enumerate(zip(zmin, zmax)) for i in range(len(ls)) for m in range(len(seps))
+                for j in range(len(ls)) for n in range(len(seps))])
+        return self._average_covariance

@@ -314,7 +314,8 @@ class MuInputs(ctypes.Structure):
 
 
 def mu_weights(engine, bg, bias_z, bias1, bias2, zmean, ls, tables, sep, den=True, rsd=True, d1=False,
-               flatsky=False, nmu=256, terms=None, out=None, stream=None, dg=None, ren_out=None, paired=False):
+               flatsky=False, nmu=256, terms=None, out=None, stream=None, dg=None, ren_out=None, paired=False,
+               rule=None):
     """Spectrum-independent mu-quadrature weights A[z][l][t][r] (coffe_mu_weights_ex), so that
     multipoles = sum_t A I_t(sep) (+ the renormalisation part with the d2 / g terms).
     bg: dict of host arrays z, chi, D1, f, cH (conformal Hubble rate), G1, G2 (and a with
@@ -323,7 +324,10 @@ def mu_weights(engine, bg, bias_z, bias1, bias2, zmean, ls, tables, sep, den=Tru
     magnification / evolution biases mb_z, s1, s2, fevo1, fevo2, and for the auto terms the
     renormalisation grid ren_chi, ren (I_0^4's table (4, 0) must be among tables); then
     ren_out (device, (nz, nl, nsep), or (nz, nl) paired) receives the grid's part.
-    paired: point i is (zmean[i], sep[i]); A is (nz, nl, ntab)."""
+    paired: point i is (zmean[i], sep[i]); A is (nz, nl, ntab).
+    rule: None (the nmu-node Gauss-Legendre rule) or (mu nodes, weights) through
+    coffe_mu_weights_rule; one node mu with weight 2 and l = 0 gives the correlation
+    function at mu."""
     import torch
     L = engine.L
     f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
@@ -371,8 +375,17 @@ def mu_weights(engine, bg, bias_z, bias1, bias2, zmean, ls, tables, sep, den=Tru
     if ren_out is not None:
         _need(ren_out, shape[:2] if paired else (len(zm), len(ls), len(sp)), "ren_out")
     st = stream if stream is not None else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ren_p = _ptr(ren_out) if ren_out is not None else None
+    if rule is not None:
+        rmu, rw = f64(np.atleast_1d(rule[0])), f64(np.atleast_1d(rule[1]))
+        if len(rmu) != len(rw):
+            raise ValueError("the rule needs one weight per node")
+        check(L.coffe_mu_weights_rule(engine.h, ctypes.byref(inp), int(terms), int(sources), int(flatsky),
+                                      int(paired), len(zm), zm.ctypes.data_as(dp), len(ls), ls_c, len(tables), tn,
+                                      tl, len(sp), sp.ctypes.data_as(dp), len(rmu), rmu.ctypes.data_as(dp),
+                                      rw.ctypes.data_as(dp), _ptr(A), ren_p, st), "coffe_mu_weights_rule")
+        return A
     check(L.coffe_mu_weights_ex(engine.h, ctypes.byref(inp), int(terms), int(sources), int(flatsky), int(paired),
                                 len(zm), zm.ctypes.data_as(dp), len(ls), ls_c, len(tables), tn, tl, len(sp),
-                                sp.ctypes.data_as(dp), int(nmu), _ptr(A),
-                                _ptr(ren_out) if ren_out is not None else None, st), "coffe_mu_weights")
+                                sp.ctypes.data_as(dp), int(nmu), _ptr(A), ren_p, st), "coffe_mu_weights")
     return A

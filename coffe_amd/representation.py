@@ -46,3 +46,16 @@ class Covariance(Representation):
     """One covariance entry between (l1, r1) and (l2, r2) at mean redshift z."""
     _fields = ("r1", "r2", "l1", "l2", "z", "value")
     r1, r2, l1, l2, z, value = _ro("r1"), _ro("r2"), _ro("l1"), _ro("l2"), _ro("z"), _ro("value")
+
+
+class AverageCovariance(Representation):
+    """One covariance entry of the redshift-averaged multipoles of bin [z_min, z_max]."""
+    _fields = ("r1", "r2", "l1", "l2", "z_min", "z_max", "value")
+    r1, r2, l1, l2 = _ro("r1"), _ro("r2"), _ro("l1"), _ro("l2")
+    z_min, z_max, value = _ro("z_min"), _ro("z_max"), _ro("value")
+
+
+class Corrfunc(Representation):
+    """One point of the correlation function: mean redshift z, separation r (Mpc), mu."""
+    _fields = ("r", "mu", "z", "value")
+    r, mu, z, value = _ro("r"), _ro("mu"), _ro("z"), _ro("value")

@@ -122,6 +122,13 @@ int orc_multipoles_dg(const double *const *tab_x, const double *const *tab_y, co
                       const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
                       const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int quad,
                       double epsrel, const orc_dg *dg, double *out);
+/* the local correlation function at every (z_mean, mu, sep): out[z][mu][r] */
+int orc_corrfunc_dg(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
+                    const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                    const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                    const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                    const double *z_mean, size_t nmu, const double *mu, size_t nsep, const double *sep,
+                    const orc_dg *dg, double *out);
 
 int orc_mu_weights_hp_dg(size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1,
                          const double *bg_f, const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb,

@@ -60,6 +60,8 @@ def _declare(L):
     L.orc_multipoles_dg.argtypes = [ctypes.POINTER(c_double_p), ctypes.POINTER(c_double_p), ctypes.POINTER(s), s,
                                     p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s,
                                     ctypes.POINTER(ctypes.c_int), s, p, i, d, ctypes.c_void_p, p]
+    L.orc_corrfunc_dg.argtypes = [ctypes.POINTER(c_double_p), ctypes.POINTER(c_double_p), ctypes.POINTER(s), s,
+                                  p, p, p, p, p, p, p, s, p, p, p, i, i, i, i, s, p, s, p, s, p, ctypes.c_void_p, p]
     L.orc_g_l.argtypes = [d, d, s, p, p]
     L.orc_g_l.restype = None
     L.orc_window_2d.argtypes = [p, d, ctypes.c_long, ctypes.c_long]
@@ -294,6 +296,34 @@ def multipoles(tables, bg, bias_z, bias1, bias2, z_mean, ls, sep, den=True, rsd=
                                    len(ls), lsa, len(sp), _p(sp), int(quad), float(epsrel),
                                    ctypes.byref(dg) if dg is not None else None, _p(out)), "multipoles")
     return out.reshape(len(zm), len(ls), len(sp))
+
+
+def corrfunc(tables, bg, bias_z, bias1, bias2, z_mean, mu, sep, den=True, rsd=True, d1=False, flatsky=False,
+             dg=None):
+    """The local correlation function (functions.c:89-1484) at every (z_mean, mu, sep), the
+    inputs of multipoles(); returns array (len(z_mean), len(mu), len(sep))."""
+    T = len(TABLE_ORDER)
+    keep = []
+    xs = (c_double_p * T)()
+    ys = (c_double_p * T)()
+    ns = (c_size * T)()
+    for t, key in enumerate(TABLE_ORDER):
+        if key in tables:
+            x, y = _f64(tables[key][0]), _f64(tables[key][1])
+            keep += [x, y]
+            xs[t], ys[t], ns[t] = _p(x), _p(y), len(x)
+        else:
+            xs[t], ys[t], ns[t] = None, None, 0
+    arrs = {kk: _f64(bg[kk]) for kk in ("z", "chi", "D1", "f", "cH", "G1", "G2")}
+    bz, b1, b2 = _f64(bias_z), _f64(bias1), _f64(bias2)
+    zm, mu_a, sp = _f64(np.atleast_1d(z_mean)), _f64(np.atleast_1d(mu)), _f64(np.atleast_1d(sep))
+    out = np.empty(len(zm) * len(mu_a) * len(sp))
+    _check(lib().orc_corrfunc_dg(xs, ys, ns, len(arrs["z"]), _p(arrs["z"]), _p(arrs["chi"]), _p(arrs["D1"]),
+                                 _p(arrs["f"]), _p(arrs["cH"]), _p(arrs["G1"]), _p(arrs["G2"]), len(bz), _p(bz),
+                                 _p(b1), _p(b2), int(den), int(rsd), int(d1), int(flatsky), len(zm), _p(zm),
+                                 len(mu_a), _p(mu_a), len(sp), _p(sp), ctypes.byref(dg) if dg is not None else None,
+                                 _p(out)), "corrfunc")
+    return out.reshape(len(zm), len(mu_a), len(sp))
 
 
 def g_l(l, nu, eta):

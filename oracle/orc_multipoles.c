@@ -257,12 +257,11 @@ int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const
                              bias2, den, rsd, d1, flatsky, nz, z_mean, nl, ls, nsep, sep, quad, epsrel, NULL, out);
 }
 
-int orc_multipoles_dg(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
-                      const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
-                      const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
-                      const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
-                      const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int quad,
-                      double epsrel, const orc_dg *dg, double *out)
+static int mp_ctx_init(mp_ctx *cp, const double *const *tab_x, const double *const *tab_y, const size_t *tab_n,
+                       size_t nbg, const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                       const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                       const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky,
+                       const orc_dg *dg)
 {
     mp_ctx c;
     memset(&c, 0, sizeof(c));
@@ -281,6 +280,28 @@ int orc_multipoles_dg(const double *const *tab_x, const double *const *tab_y, co
     if (st == ORC_OK) st = ak_init(&c.b1, nb, zb, bias1);
     if (st == ORC_OK) st = ak_init(&c.b2, nb, zb, bias2);
     if (st == ORC_OK) st = dg_init(&c, nbg, bg_z, dg);
+    *cp = c;
+    return st;
+}
+
+static void mp_ctx_free(mp_ctx *c)
+{
+    for (int t = 0; t < NTAB; ++t) if (c->have[t]) free(c->tab[t].b);
+    free(c->chi.b); free(c->z_of_chi.b); free(c->D1.b); free(c->f.b); free(c->cH.b); free(c->G1.b); free(c->G2.b);
+    free(c->b1.b); free(c->b2.b);
+    dg_free(c);
+}
+
+int orc_multipoles_dg(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
+                      const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                      const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                      const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                      const double *z_mean, size_t nl, const int *ls, size_t nsep, const double *sep, int quad,
+                      double epsrel, const orc_dg *dg, double *out)
+{
+    mp_ctx c;
+    int st = mp_ctx_init(&c, tab_x, tab_y, tab_n, nbg, bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, nb, zb, bias1,
+                         bias2, den, rsd, d1, flatsky, dg);
     double *glx = NULL, *glw = NULL;
     if (st == ORC_OK && quad > 0) {
         glx = (double *)malloc(sizeof(double) * (size_t)quad);
@@ -305,10 +326,31 @@ int orc_multipoles_dg(const double *const *tab_x, const double *const *tab_y, co
         }
     }
     free(glx); free(glw);
-    for (int t = 0; t < NTAB; ++t) if (c.have[t]) free(c.tab[t].b);
-    free(c.chi.b); free(c.z_of_chi.b); free(c.D1.b); free(c.f.b); free(c.cH.b); free(c.G1.b); free(c.G2.b);
-    free(c.b1.b); free(c.b2.b);
-    dg_free(&c);
+    mp_ctx_free(&c);
+    return st;
+}
+
+/*
+ * The local correlation function (functions_nonintegrated, functions.c:89-1484, the
+ * reference's CORRFUNC output of signal.c:504-509) at every (z_mean, mu, sep):
+ * out[z][mu][r].  Same inputs as orc_multipoles_dg.
+ */
+int orc_corrfunc_dg(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,
+                    const double *bg_z, const double *bg_chi, const double *bg_D1, const double *bg_f,
+                    const double *bg_cH, const double *bg_G1, const double *bg_G2, size_t nb, const double *zb,
+                    const double *bias1, const double *bias2, int den, int rsd, int d1, int flatsky, size_t nz,
+                    const double *z_mean, size_t nmu, const double *mu, size_t nsep, const double *sep,
+                    const orc_dg *dg, double *out)
+{
+    mp_ctx c;
+    int st = mp_ctx_init(&c, tab_x, tab_y, tab_n, nbg, bg_z, bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2, nb, zb, bias1,
+                         bias2, den, rsd, d1, flatsky, dg);
+    if (st == ORC_OK)
+        for (size_t iz = 0; iz < nz; ++iz)
+            for (size_t im = 0; im < nmu; ++im)
+                for (size_t ir = 0; ir < nsep; ++ir)
+                    out[(iz * nmu + im) * nsep + ir] = orc_functions_nonintegrated(&c, z_mean[iz], mu[im], sep[ir]);
+    mp_ctx_free(&c);
     return st;
 }
 

@@ -38,10 +38,6 @@ class FitArray(ctypes.Structure):
     _fields_ = [("array", VP), ("size", SZ)]
 
 
-def _names(*n):
-    return list(n)
-
-
 class Parameters(ctypes.Structure):
     """coffe_parameters_t of the C build (common.h:340-550)."""
     _fields_ = (
@@ -333,6 +329,8 @@ class Program:
         par.covariance_interpolation_method = 2
         par.covariance_window = int(p["has_binned_covariance"])
         par.flatsky_local = int(p["has_flatsky_local"])
+        par.only_cross_correlations = int(p["has_only_cross_correlations"])
+        par.mu, par.mu_len = self._arr(p["mu"])
         par.integration_1d_type, par.integration_1d_prec = 1, 1e-5
         self.par = par
         bg = cosmo._background()

@@ -262,3 +262,27 @@ def test_hp_weights_dg_vs_fp64(setup, dg_inputs):
     for il in range(3):
         err = np.abs(hp[0, il] - ref[0, il])
         assert np.all(err <= 1e-9 * np.abs(ref[0, il]) + 1e-12 * np.abs(ref[0, 0])), (il, err / np.abs(ref[0, il]))
+
+
+def test_corrfunc_projects_onto_multipoles():
+    """The oracle's correlation function (functions_nonintegrated at each mu) projected on
+    P_l with a 256-node Gauss-Legendre rule over [-1, 1] equals its multipoles (the mu
+    integral of signal.c:83-111 on 256 nodes over [0, 1]): exactly for flat sky (a
+    polynomial of degree <= 4 in mu), to the two rules' difference (1e-9) for full sky
+    with its odd multipoles."""
+    k = np.logspace(-4, 1, 800)
+    pk = k / (1 + (k / 0.02) ** 2.5)
+    tabs = {key: O.integral_table(key[0], key[1], k, pk, 2048, small_r=False) for key in O.TABLE_ORDER}
+    bg = B.compute(omega_cdm=0.25, omega_baryon=0.05, h=0.67)
+    bgo = dict(z=bg["z"], chi=bg["chi"], D1=bg["D1"], f=bg["f"], cH=bg["conformal_Hz"], G1=bg["G1"], G2=bg["G2"])
+    zb = np.linspace(0, 10, 11)
+    b = 1.3 + 0.2 * zb
+    x, w = np.polynomial.legendre.leggauss(256)
+    ls = [0, 1, 2, 3, 4]
+    for flat, tol in ((True, 1e-13), (False, 1e-9)):
+        xi = O.corrfunc(tabs, bgo, zb, b, b, [1.0], x, [30.0, 90.0], den=True, rsd=True, d1=True, flatsky=flat)[0]
+        mp = O.multipoles(tabs, bgo, zb, b, b, [1.0], ls, [30.0, 90.0], den=True, rsd=True, d1=True,
+                          flatsky=flat, quad=256)[0]
+        for il, l in enumerate(ls):
+            proj = (2 * l + 1) / 2 * (w * np.polynomial.legendre.Legendre.basis(l)(x)) @ xi
+            assert np.all(np.abs(proj - mp[il]) <= tol * np.max(np.abs(mp[0]))), (flat, l)
