@@ -161,7 +161,7 @@ class Coffe:
     wa = property(lambda s: s._p["wa"], lambda s, v: s._set_scalar("wa", v, (int, float), -100, 100, s._free_all))
     integrals_fftlog_sampling = property(
         lambda s: s._p["integrals_fftlog_sampling"],
-        lambda s, v: s._set_scalar("integrals_fftlog_sampling", v, int, 4, 16384, s._free_all))
+        lambda s, v: s._set_scalar("integrals_fftlog_sampling", v, int, 4, 1000000, s._free_all))
     covariance_integration_sampling = property(
         lambda s: s._p["covariance_integration_sampling"],
         lambda s, v: s._set_scalar("covariance_integration_sampling", v, int, 8, 8192, s._free_results))

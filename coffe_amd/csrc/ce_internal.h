@@ -94,6 +94,21 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
    pack (0: lds_data_len(max_M)) */
 /* d_ranges (4 ints per chunk id, may be NULL): the resample index ranges of the
    chunk's owned and staged intervals, written for ce_launch_spline_adj */
+/* complex FFTs of any length on device memory (anyfft.hip): forward, unnormalised,
+   in place on d_x (L complex values); direct LDS lines up to 8192 points, four-step for
+   longer 2^a 3^b 5^c lengths, Bluestein otherwise */
+#define CE_ANY_MAX_N (1 << 21)
+typedef struct ce_anyfft ce_anyfft;
+int ce_anyfft_create(int64_t L, ce_anyfft **out);
+void ce_anyfft_destroy(ce_anyfft *p);
+int ce_anyfft_forward(ce_anyfft *p, void *d_x, void *stream);
+int ce_anyfft_kind(const ce_anyfft *p);
+int64_t ce_anyfft_padded(const ce_anyfft *p);
+/* the FFTLog stages around the two length-N transforms of the general path */
+int ce_launch_tf_in(void *stream, const double *d_S, const double *d_bw, void *d_c, int N);
+int ce_launch_tf_mid(void *stream, void *d_c, const double *d_U, int N);
+int ce_launch_tf_out(void *stream, const void *d_c, const double *d_os, double *d_out, int N);
+
 int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
                        const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws,
                        int32_t *d_ranges);

@@ -41,7 +41,8 @@ const char *coffe_engine_strerror(int code)
     case COFFE_EINVAL: return "invalid argument";
     case COFFE_ENOMEM: return "out of memory";
     case COFFE_EDEVICE: return "HIP device error";
-    case COFFE_EUNSUPPORTED: return "unsupported size (FFT length must be even, N/2 = 2^a 3^b 5^c, N <= 16384)";
+    case COFFE_EUNSUPPORTED: return "unsupported size or case (an odd or too long FFT length, a batch length "
+                                    "outside the LDS transforms, or an unsupported option)";
     case COFFE_EDOMAIN: return "argument outside the domain (window limits or separation outside the table)";
     case COFFE_ENONFINITE: return "non-finite value in input or kernel";
     case COFFE_ENODEVICE: return "no HIP device available";
@@ -65,7 +66,8 @@ typedef struct ce_grid {
     double *h_p;
     /* device */
     double *d_p, *d_tw, *d_bw[2], *d_U[CE_MAX_TABLES], *d_os[CE_MAX_TABLES];
-    ce_radix_plan rp;
+    ce_radix_plan rp;   /* npass 0: no LDS schedule (N > 16384 or N/2 not 2^a 3^b 5^c) */
+    ce_anyfft *fft;     /* the length-N transform of the general path, made on first use */
 } ce_grid;
 
 static int hip_ok(hipError_t e) { return e == hipSuccess ? COFFE_OK : COFFE_EDEVICE; }
@@ -175,6 +177,7 @@ static void grid_free(ce_grid *g)
     hipFree(g->d_tw);
     for (int q = 0; q < 2; ++q) hipFree(g->d_bw[q]);
     for (int t = 0; t < CE_MAX_TABLES; ++t) { hipFree(g->d_U[t]); hipFree(g->d_os[t]); }
+    ce_anyfft_destroy(g->fft);
     free(g);
 }
 
@@ -210,7 +213,7 @@ static int assign_q(int ntab, ce_table_desc *tabs, double *qnu, int *nq, double 
 static int grid_get(coffe_engine *e, int N, double kmin, double kmax, double k0, double r0, int ntab,
                     const ce_table_desc *tabs_in, ce_grid **out)
 {
-    if (N < 4 || (N & 1) || N > 16384) return COFFE_EUNSUPPORTED;
+    if (N < 4 || (N & 1) || N > CE_ANY_MAX_N) return COFFE_EUNSUPPORTED;
     if (!(kmin > 0) || !(kmax > kmin) || !(k0 > 0) || !(r0 > 0) || !isfinite(kmax)) return COFFE_EINVAL;
     if (ntab < 1 || ntab > CE_MAX_TABLES) return COFFE_EINVAL;
     pthread_mutex_lock(&e->lock);
@@ -229,7 +232,8 @@ static int grid_get(coffe_engine *e, int N, double kmin, double kmax, double k0,
     memcpy(g->tabs, tabs_in, sizeof(ce_table_desc) * (size_t)ntab);
     double qnu[CE_MAX_TABLES];
     int st = assign_q(ntab, g->tabs, qnu, &g->nq, g->qprime);
-    if (st == COFFE_OK) st = ce_radix_schedule(N / 2, &g->rp);
+    /* the fused LDS kernels' schedule; lengths without one take the general path */
+    if (st == COFFE_OK && (N > 16384 || ce_radix_schedule(N / 2, &g->rp) != COFFE_OK)) memset(&g->rp, 0, sizeof(g->rp));
     /* window limits of twofast_fft_input (twofast.c:283-289) */
     const double xmax = k0 * pow(kmax / kmin, (double)(N - 1) / N);
     const double xl = exp(0.46) * kmin, xr = exp(-0.46) * xmax;
@@ -475,6 +479,7 @@ int coffe_batch_create(coffe_engine *e, int nspec, const int64_t *n_in, const in
         if (n_in[s] < 5 || n_fft[s] < 4 || n_fft[s] > 16384) { st = n_in[s] < 5 ? COFFE_EINVAL : COFFE_EUNSUPPORTED; break; }
         ce_grid *g = NULL;
         st = grid_get(e, (int)n_fft[s], kmin[s], kmax[s], kmin[s], 1.0, ntab, tabs, &g);
+        if (st == COFFE_OK && g->rp.npass == 0) st = COFFE_EUNSUPPORTED;   /* the batch runs the LDS kernels */
         if (st != COFFE_OK) break;
         int gi = -1;
         for (int q = 0; q < b->ngrid; ++q) if (b->grids[q] == g) gi = q;
@@ -848,7 +853,7 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
                            double kmin, double kmax)
 {
     if (!output_x || !output_y || !input_x || !input_y || input_len < 5) return COFFE_EINVAL;
-    if (output_len < 4 || (output_len & 1) || output_len > 16384) return COFFE_EUNSUPPORTED;
+    if (output_len < 4 || (output_len & 1) || output_len > CE_ANY_MAX_N) return COFFE_EUNSUPPORTED;
     for (size_t i = 0; i < input_len; ++i)
         if (!isfinite(input_x[i]) || !isfinite(input_y[i])) return COFFE_ENONFINITE;
     coffe_engine *e = coffe_engine_default();
@@ -864,10 +869,19 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     if (st != COFFE_OK) return st;
     const int N = (int)output_len, M = N / 2;
     const size_t n = input_len;
-    /* scratch layout: k, P | ws (2N) | xf (M+1 cplx) | out (N) | spec, grid, tab, unit, chunks */
+    /* the general path: a length-N transform of the grid, made once */
+    const int general = g->rp.npass == 0;
+    if (general) {
+        pthread_mutex_lock(&e->lock);
+        if (!g->fft) st = ce_anyfft_create(N, &g->fft);
+        pthread_mutex_unlock(&e->lock);
+        if (st != COFFE_OK) return st;
+    }
+    /* scratch layout: k, P | ws (2N) | xf (M+1 cplx; N cplx on the general path) | out (N) |
+       spec, grid, tab, unit, chunks */
     const int nchunk = (int)((n - 1 + CE_CHUNK - 1) / CE_CHUNK);
     const size_t off_k = 0, off_p = n, off_ws = 2 * n, off_xf = off_ws + 2 * (size_t)N;
-    const size_t off_out = off_xf + 2 * (size_t)(M + 1), off_meta = off_out + (size_t)N;
+    const size_t off_out = off_xf + 2 * (size_t)(general ? N : M + 1), off_meta = off_out + (size_t)N;
     const size_t meta_bytes = sizeof(ce_spec_dev) + sizeof(ce_grid_dev) + sizeof(ce_table_desc) +
                               sizeof(ce_unit_dev) + sizeof(ce_chunk_dev) * (size_t)nchunk + 64;
     pthread_mutex_lock(&e->lock);
@@ -917,12 +931,23 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     if (st == COFFE_OK && ce_launch_resample(s, nchunk, d_chunks, d_spec, d_grid, base + off_k, base + off_p,
                                              base + off_ws, NULL))
         st = COFFE_EDEVICE;
-    if (st == COFFE_OK && ce_launch_prepare(s, 1, d_spec, d_grid, 1, base + off_ws, base + off_xf, threads, M))
-        st = COFFE_EDEVICE;
-    if (st == COFFE_OK && ce_launch_tables(s, 1, d_unit, d_spec, d_grid, d_tab, 1, base + off_xf, 0, NULL, NULL,
-                                           NULL, base + off_out, N, NULL, NULL, 1, 1, NULL, NULL, threads, M, NULL, 0,
-                                           0))
-        st = COFFE_EDEVICE;
+    if (general) {
+        /* bias * window * S -> FFT_N -> kernel multiply, Hermitian extension -> FFT_N ->
+           real part times the output prefactor (twofast.c:362-483) */
+        const int qi = g->tabs[0].qidx;
+        if (st == COFFE_OK && ce_launch_tf_in(s, base + off_ws, g->d_bw[qi], base + off_xf, N)) st = COFFE_EDEVICE;
+        if (st == COFFE_OK) st = ce_anyfft_forward(g->fft, base + off_xf, s);
+        if (st == COFFE_OK && ce_launch_tf_mid(s, base + off_xf, g->d_U[0], N)) st = COFFE_EDEVICE;
+        if (st == COFFE_OK) st = ce_anyfft_forward(g->fft, base + off_xf, s);
+        if (st == COFFE_OK && ce_launch_tf_out(s, base + off_xf, g->d_os[0], base + off_out, N)) st = COFFE_EDEVICE;
+    } else {
+        if (st == COFFE_OK && ce_launch_prepare(s, 1, d_spec, d_grid, 1, base + off_ws, base + off_xf, threads, M))
+            st = COFFE_EDEVICE;
+        if (st == COFFE_OK && ce_launch_tables(s, 1, d_unit, d_spec, d_grid, d_tab, 1, base + off_xf, 0, NULL, NULL,
+                                               NULL, base + off_out, N, NULL, NULL, 1, 1, NULL, NULL, threads, M,
+                                               NULL, 0, 0))
+            st = COFFE_EDEVICE;
+    }
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(output_y, base + off_out, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, s));
     if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
     pthread_mutex_unlock(&e->lock);
@@ -930,6 +955,26 @@ int twofast_1bessel_status(double *output_x, double *output_y, size_t output_len
     if (st != COFFE_OK) return st;
     for (int i = 0; i < N; ++i) output_x[i] = r0 * g->h_p[i];
     return COFFE_OK;
+}
+
+int coffe_fft_any(const double *in, double *out, int64_t L)
+{
+    if (!in || !out || L < 1) return COFFE_EINVAL;
+    coffe_engine *e = coffe_engine_default();
+    if (!e) return COFFE_ENODEVICE;
+    hipSetDevice(e->device);
+    ce_anyfft *p = NULL;
+    int st = ce_anyfft_create(L, &p);
+    if (st != COFFE_OK) return st;
+    void *d = NULL;
+    st = dev_alloc(&d, sizeof(double) * 2 * (size_t)L);
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d, in, sizeof(double) * 2 * (size_t)L, hipMemcpyHostToDevice, e->stream));
+    if (st == COFFE_OK) st = ce_anyfft_forward(p, d, e->stream);
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(out, d, sizeof(double) * 2 * (size_t)L, hipMemcpyDeviceToHost, e->stream));
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(e->stream));
+    hipFree(d);
+    ce_anyfft_destroy(p);
+    return st;
 }
 
 void twofast_1bessel(double *output_x, double *output_y, size_t output_len, double *input_x, double *input_y,

@@ -64,7 +64,9 @@ coffe_engine *coffe_engine_default(void);
 /* ---- reference drop-ins -------------------------------------------------- */
 
 /* twofast.c:362-483: 1D FFTLog of an input P(k) table (natural cubic spline
-   resampled onto output_len log points); host buffers, flag ignored (FFTW effort). */
+   resampled onto output_len log points); host buffers, flag ignored (FFTW effort).
+   Any even output_len up to 2^21: lengths with N/2 = 2^a 3^b 5^c <= 8192 run the fused
+   LDS kernels, the others the general device transform of coffe_fft_any. */
 void twofast_1bessel(double *output_x, double *output_y, size_t output_len, double *input_x, double *input_y,
                      size_t input_len, double l, double nu, double r0, double k0, double kmin, double kmax,
                      unsigned flag);
@@ -317,6 +319,16 @@ int coffe_integral_table(int n, int l, size_t N, const double *k, const double *
  * stream and returns COFFE_ENOCONV when an integral did not converge.
  */
 int coffe_batch_small_r(coffe_batch *b, const double *d_k, const double *d_pk, double *d_out, void *stream);
+
+/*
+ * Forward complex FFT of any length L (no reference counterpart: the reference calls
+ * FFTW): out_k = sum_n in_n e^{-2 pi i k n / L}, unnormalised, interleaved (re, im)
+ * host arrays of 2L doubles.  The device transform behind the FFTLog drop-in for
+ * lengths outside the fused LDS kernels: one LDS transform for 2^a 3^b 5^c <= 8192,
+ * four-step beyond, Bluestein for other factors.  L <= 2^26; synchronous, default
+ * engine.
+ */
+int coffe_fft_any(const double *in, double *out, int64_t L);
 
 /* synchronise the engine's stream */
 int coffe_engine_synchronize(coffe_engine *e);

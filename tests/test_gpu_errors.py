@@ -38,8 +38,7 @@ def test_twofast_void_error_codes(engine):
     assert _void_twofast(L, k, p, 1024)[0] == 0
     print("void twofast: odd N", flush=True)
     assert _void_twofast(L, k, p, 1023)[0] == EUNSUPPORTED                 # the reference's FFT needs even N
-    assert _void_twofast(L, k, p, 32768)[0] == EUNSUPPORTED                # above the LDS transform size
-    assert _void_twofast(L, k, p, 2 * 7 * 64)[0] == EUNSUPPORTED           # N/2 with the prime factor 7
+    assert _void_twofast(L, k, p, 1 << 22)[0] == EUNSUPPORTED              # above the general path's 2^21
     bad = p.copy()
     bad[17] = np.nan
     assert _void_twofast(L, k, bad, 1024)[0] == ENONFINITE                 # twofast.c:122 exits
