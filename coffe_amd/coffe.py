@@ -164,7 +164,7 @@ class Coffe:
         lambda s, v: s._set_scalar("integrals_fftlog_sampling", v, int, 4, 1000000, s._free_all))
     covariance_integration_sampling = property(
         lambda s: s._p["covariance_integration_sampling"],
-        lambda s, v: s._set_scalar("covariance_integration_sampling", v, int, 8, 8192, s._free_results))
+        lambda s, v: s._set_scalar("covariance_integration_sampling", v, int, 8, 1000000, s._free_results))
     multipole_quadrature_nodes = property(
         lambda s: s._p["multipole_quadrature_nodes"],
         lambda s, v: s._set_scalar("multipole_quadrature_nodes", v, int, 2, 65536, s._free_results))

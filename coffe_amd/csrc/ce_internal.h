@@ -102,12 +102,21 @@ typedef struct ce_anyfft ce_anyfft;
 int ce_anyfft_create(int64_t L, ce_anyfft **out);
 void ce_anyfft_destroy(ce_anyfft *p);
 int ce_anyfft_forward(ce_anyfft *p, void *d_x, void *stream);
+/* nb transforms of L points one after another (nb <= 65535) */
+int ce_anyfft_forward_batch(ce_anyfft *p, void *d_x, int64_t nb, void *stream);
 int ce_anyfft_kind(const ce_anyfft *p);
 int64_t ce_anyfft_padded(const ce_anyfft *p);
 /* the FFTLog stages around the two length-N transforms of the general path */
 int ce_launch_tf_in(void *stream, const double *d_S, const double *d_bw, void *d_c, int N);
 int ce_launch_tf_mid(void *stream, void *d_c, const double *d_U, int N);
 int ce_launch_tf_out(void *stream, const void *d_c, const double *d_os, double *d_out, int N);
+/* the general-length counterparts of ce_launch_cov_ahat and ce_launch_cov_rows (same
+   arrays; the row transform synchronises the stream) */
+int ce_general_cov_ahat(ce_anyfft *fft, void *stream, int nl, int N, const double *d_A, double *d_ahat);
+int ce_general_cov_rows(ce_anyfft *fft, void *stream, int nblk, int N, const double *d_ahat, const double *d_B,
+                        const double *d_d, int npair, const int *d_pl1, const int *d_pl2, int nrows,
+                        const int *d_rows, const double *d_prn1, int R, const int *d_flo, const int *d_fcnt,
+                        const int *d_foff, const double *d_fw, double *d_W);
 
 int ce_launch_resample(void *stream, int nchunk, const ce_chunk_dev *d_chunks, const ce_spec_dev *d_spec,
                        const ce_grid_dev *d_grids, const double *d_k, const double *d_pk, double *d_ws,

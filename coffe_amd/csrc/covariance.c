@@ -98,13 +98,25 @@ static void axis_arrays(int N, double k0r0, double dlnk, double l, double nu, do
     free(wr); free(wc); free(H);
 }
 
-static int cov_length_ok(int N, double width)
+/* the window must fit the row (twobessel.c:76-92); lengths the fused LDS kernels take
+   (N <= 8192, N and N/2 of 2, 3, 5) run there, other even lengths up to 2^20 on the
+   general device FFT */
+static int cov_window_ok(int N, double width)
+{
+    const long ncut = (long)((N / 2) * width);
+    return ncut >= 1 && N / 2 + 1 + ncut <= N - 1;
+}
+
+static int cov_lds_ok(int N)
 {
     ce_radix_plan rp;
     if (N < 8 || N > 8192 || (N & 1)) return 0;
-    if (ce_radix_schedule_ex(N, &rp, 1) != COFFE_OK || ce_radix_schedule_ex(N / 2, &rp, 1) != COFFE_OK) return 0;
-    const long ncut = (long)((N / 2) * width);
-    return ncut >= 1 && N / 2 + 1 + ncut <= N - 1;
+    return ce_radix_schedule_ex(N, &rp, 1) == COFFE_OK && ce_radix_schedule_ex(N / 2, &rp, 1) == COFFE_OK;
+}
+
+static int cov_length_ok(int N, double width)
+{
+    return N >= 8 && !(N & 1) && N <= (1 << 20) && cov_window_ok(N, width);
 }
 
 /* ---- bicubic functionals ------------------------------------------------------- */
@@ -252,6 +264,7 @@ struct coffe_cov_plan {
     int pl1[CV_MAXPAIR], pl2[CV_MAXPAIR];
     double kmin, kmax, step;
     ce_radix_plan rpN, rpM;
+    ce_anyfft *fft;    /* the general path's length-N transform (lengths outside the LDS kernels) */
     double *h_sep;
     /* device: spectrum-independent */
     double *d_tw, *d_A, *d_ahat, *d_B, *d_kg, *d_k3, *d_kn1, *d_kn2, *d_prn1, *d_prn2, *d_fac, *d_fw, *d_sep;
@@ -282,6 +295,7 @@ void coffe_cov_plan_destroy(coffe_cov_plan *p)
     hipFree(p->d_n_in);
     hipFree(p->d_Wbar);
     hipFree(p->d_dbar);
+    ce_anyfft_destroy(p->fft);
     free(p);
 }
 
@@ -308,8 +322,13 @@ int coffe_cov_plan_create(coffe_engine *e, int N, double kmin, double kmax, int 
     p->e = e; p->N = N; p->M = N / 2; p->nl = nl; p->R = R; p->kmin = kmin; p->kmax = kmax;
     memcpy(p->ls, ls, sizeof(int) * (size_t)nl);
     p->step = log(kmax / kmin) / N;
-    int st = ce_radix_schedule_ex(N, &p->rpN, 1);
-    if (st == COFFE_OK) st = ce_radix_schedule_ex(N / 2, &p->rpM, 1);
+    int st = COFFE_OK;
+    if (cov_lds_ok(N)) {
+        st = ce_radix_schedule_ex(N, &p->rpN, 1);
+        if (st == COFFE_OK) st = ce_radix_schedule_ex(N / 2, &p->rpM, 1);
+    } else {
+        st = ce_anyfft_create(N, &p->fft);
+    }
     int pair_of[CV_MAXL * CV_MAXL];
     double fac[2 * CV_MAXPAIR];
     p->npair = 0;
@@ -417,7 +436,10 @@ int coffe_cov_plan_create(coffe_engine *e, int N, double kmin, double kmax, int 
     if (st == COFFE_OK) st = upload((void **)&p->d_ls, ls, sizeof(int) * (size_t)nl, s);
     if (st == COFFE_OK) st = upload((void **)&p->d_pair_of, pair_of, sizeof(int) * (size_t)nl * (size_t)nl, s);
     if (st == COFFE_OK) st = ce_dev_alloc((void **)&p->d_status, sizeof(int));
-    if (st == COFFE_OK && ce_launch_cov_ahat(s, nl, N, &p->rpN, p->d_tw, p->d_A, p->d_ahat)) st = COFFE_EDEVICE;
+    if (st == COFFE_OK) {
+        if (p->fft) st = ce_general_cov_ahat(p->fft, s, nl, N, p->d_A, p->d_ahat);
+        else if (ce_launch_cov_ahat(s, nl, N, &p->rpN, p->d_tw, p->d_A, p->d_ahat)) st = COFFE_EDEVICE;
+    }
     if (st == COFFE_OK) st = ce_hip_ok(hipStreamSynchronize(s));
     free(k); free(r); free(k3); free(kn); free(prn); free(tw); free(A); free(B);
     free(mark); free(pos); free(rows); free(upos);
@@ -470,10 +492,14 @@ int coffe_cov_integrals(coffe_cov_plan *p, int nspec, const int64_t *n_in, const
     if (st == COFFE_OK && ce_launch_cov_diag(s, nspec, p->N, p->d_in_off, p->d_n_in, d_k, d_pk, p->d_kg, p->d_k3,
                                              p->d_kn1, p->d_kn2, p->step, p->d_d, p->d_status))
         st = COFFE_EDEVICE;
-    if (st == COFFE_OK &&
-        ce_launch_cov_rows(s, 2 * nspec, p->N, &p->rpN, &p->rpM, p->d_tw, p->d_ahat, p->d_B, p->d_d, p->npair,
-                           p->d_pl1, p->d_pl2, p->nrows, p->d_rows, p->d_prn1, p->d_prn2, p->R, p->d_flo, p->d_fcnt,
-                           p->d_foff, p->d_fws, p->d_W))
+    if (st == COFFE_OK && p->fft)
+        st = ce_general_cov_rows(p->fft, s, 2 * nspec, p->N, p->d_ahat, p->d_B, p->d_d, p->npair, p->d_pl1, p->d_pl2,
+                                 p->nrows, p->d_rows, p->d_prn1, p->R, p->d_flo, p->d_fcnt, p->d_foff, p->d_fws,
+                                 p->d_W);
+    else if (st == COFFE_OK &&
+             ce_launch_cov_rows(s, 2 * nspec, p->N, &p->rpN, &p->rpM, p->d_tw, p->d_ahat, p->d_B, p->d_d, p->npair,
+                                p->d_pl1, p->d_pl2, p->nrows, p->d_rows, p->d_prn1, p->d_prn2, p->R, p->d_flo,
+                                p->d_fcnt, p->d_foff, p->d_fws, p->d_W))
         st = COFFE_EDEVICE;
     if (st == COFFE_OK && ce_launch_cov_cols(s, nspec, p->npair, p->nrows, p->R, p->d_upos, p->d_fcnt, p->d_foff,
                                              p->d_fw, p->d_fac, p->d_W, d_out))
@@ -492,6 +518,7 @@ int coffe_cov_integrals_vjp(coffe_cov_plan *p, int nspec, const int64_t *n_in, c
                             const double *d_gint, double *d_gpk, void *stream)
 {
     if (!p || nspec <= 0 || !n_in || !d_k || !d_pk || !d_gint || !d_gpk) return COFFE_EINVAL;
+    if (p->fft) return COFFE_EUNSUPPORTED;   /* the adjoint runs on the fused LDS kernels only */
     hipSetDevice(p->e->device);
     hipStream_t s = stream ? (hipStream_t)stream : p->e->stream;
     int st = COFFE_OK;
@@ -792,8 +819,14 @@ int two_sph_bessel_status(const double *k1, const double *k2, double **fk1k2, lo
     for (int i = 0; i < N; ++i) r1[i] = 1. / k1[N - 1 - i];
     for (int i = 0; i < N; ++i) r2[i] = 1. / k2[N - 1 - i];
     ce_radix_plan rpN, rpM;
-    ce_radix_schedule_ex(N, &rpN, 1);
-    ce_radix_schedule_ex(M, &rpM, 1);
+    ce_anyfft *fft = NULL;
+    if (cov_lds_ok(N)) {
+        ce_radix_schedule_ex(N, &rpN, 1);
+        ce_radix_schedule_ex(M, &rpM, 1);
+    } else {
+        const int fst = ce_anyfft_create(N, &fft);
+        if (fst != COFFE_OK) return fst;
+    }
     double *tw = (double *)malloc(sizeof(double) * 2 * (size_t)N);
     double *A = (double *)malloc(sizeof(double) * 2 * (size_t)N);
     double *B = (double *)malloc(sizeof(double) * 2 * (size_t)(M + 1));
@@ -840,10 +873,16 @@ int two_sph_bessel_status(const double *k1, const double *k2, double **fk1k2, lo
     if (st == COFFE_OK) st = upload((void **)&d_w, wone, sizeof(double) * (size_t)N, s);
     if (st == COFFE_OK) st = upload((void **)&d_zero, &zero, sizeof(int), s);
     if (st == COFFE_OK) st = ce_dev_alloc((void **)&d_W, sizeof(double) * (size_t)N * (size_t)N);
-    if (st == COFFE_OK && ce_launch_cov_ahat(s, 1, N, &rpN, d_tw, d_A, d_ahat)) st = COFFE_EDEVICE;
+    if (st == COFFE_OK) {
+        if (fft) st = ce_general_cov_ahat(fft, s, 1, N, d_A, d_ahat);
+        else if (ce_launch_cov_ahat(s, 1, N, &rpN, d_tw, d_A, d_ahat)) st = COFFE_EDEVICE;
+    }
     /* every row, identity column functionals: W is the N x N result */
-    if (st == COFFE_OK && ce_launch_cov_rows(s, 1, N, &rpN, &rpM, d_tw, d_ahat, d_B, d_d, 1, d_zero, d_zero, N, d_idx,
-                                             d_p1, d_p2, N, d_idx, d_one, d_idx, d_w, d_W))
+    if (st == COFFE_OK && fft)
+        st = ce_general_cov_rows(fft, s, 1, N, d_ahat, d_B, d_d, 1, d_zero, d_zero, N, d_idx, d_p1, N, d_idx, d_one,
+                                 d_idx, d_w, d_W);
+    else if (st == COFFE_OK && ce_launch_cov_rows(s, 1, N, &rpN, &rpM, d_tw, d_ahat, d_B, d_d, 1, d_zero, d_zero, N,
+                                                  d_idx, d_p1, d_p2, N, d_idx, d_one, d_idx, d_w, d_W))
         st = COFFE_EDEVICE;
     if (st == COFFE_OK)
         st = ce_hip_ok(hipMemcpyAsync(res, d_W, sizeof(double) * (size_t)N * (size_t)N, hipMemcpyDeviceToHost, s));
@@ -854,6 +893,7 @@ int two_sph_bessel_status(const double *k1, const double *k2, double **fk1k2, lo
     double *dd[] = {d_tw, d_A, d_ahat, d_B, d_d, d_p1, d_p2, d_w, d_W};
     for (size_t i = 0; i < sizeof(dd) / sizeof(dd[0]); ++i) hipFree(dd[i]);
     hipFree(d_idx); hipFree(d_one); hipFree(d_zero);
+    ce_anyfft_destroy(fft);
     free(tw); free(A); free(B); free(dg); free(p1); free(p2); free(idx); free(one); free(wone); free(res);
     return st;
 }

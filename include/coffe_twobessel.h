@@ -27,9 +27,10 @@ typedef struct config {
  * r2 likewise, result[i][j] at (r1[i], r2[j]).  The engine computes the transform of
  * a DIAGONAL fk1k2 on a square grid (k1 == k2), the form covariance_integrate_fftlog
  * calls (covariance.c:391-420); other inputs record COFFE_EUNSUPPORTED.  Odd lengths
- * (where the reference calls exit) record COFFE_EINVAL.  N <= 8192, N and N/2 with
- * prime factors 2, 3, 5 only.  The void form records the status for
- * coffe_engine_last_error(); the _status form returns it.
+ * (where the reference calls exit) record COFFE_EINVAL.  Even N up to 2^20: N <= 8192
+ * with N and N/2 of prime factors 2, 3, 5 runs the fused LDS row kernel, other lengths
+ * the general device FFT (LDS lines, four-step, Bluestein).  The void form records the
+ * status for coffe_engine_last_error(); the _status form returns it.
  */
 void two_sph_bessel(double *k1, double *k2, double **fk1k2, long N1, long N2, config *config, double *r1, double *r2,
                     double **result);
@@ -37,7 +38,9 @@ int two_sph_bessel_status(const double *k1, const double *k2, double **fk1k2, lo
                           double *r1, double *r2, double **result);
 
 /*
- * Covariance plan: N sampling points (covariance_integration_bins), k_i = kmin
+ * Covariance plan: N sampling points (covariance_integration_bins, even, up to 2^20;
+ * lengths outside the fused LDS kernels take the general device FFT, and their
+ * coffe_cov_integrals_vjp returns COFFE_EUNSUPPORTED), k_i = kmin
  * (kmax/kmin)^(i/N) with kmin/kmax the spectrum's k_min_norm/k_max_norm, multipoles
  * ls[nl], separations sep[R] (the bicubic queries).  Spectrum-independent tables are
  * built once (covariance.c:357-470 with nu1 = nu2 = 1.01, window width 0.25).
