@@ -574,6 +574,42 @@ class Coffe:
                 out[:, l0:l0 + len(lc)] += ren.cpu().numpy()
         return out
 
+    def multipoles_cosmo_jvp(self, direction):
+        """d/d theta of compute_multipoles_bulk's values along `direction`, a dict over
+        the background parameters omega_cdm, omega_baryon, omega_gamma, w0, wa, h (no
+        reference counterpart): the exact tangents of the background tables
+        (coffe_background_jvp) through the engine's forward-mode weights
+        (coffe_mu_weights_jvp), full or flat sky, density / RSD / Doppler-1, at fixed P(k)
+        and separations.  Returns an array in the records' order (z, l, r)."""
+        from . import background as _bgm
+        from .engine import mu_weights_jvp
+        self._check_coords_multipoles()
+        self._check_contributions()
+        if self._dg_sources():
+            raise NotImplementedError("the cosmological tangent of the d2 / g1-g3 terms is not built")
+        bz = self._bias[1].x
+        if not np.array_equal(bz, self._bias[2].x):
+            raise ValueError("galaxy_bias1 and galaxy_bias2 must share their redshift sampling")
+        mz, (s1, s2, e1, e2) = self._mbias_grid()
+        bg, tg = _bgm.jvp(direction, omega_cdm=self._p["omega_cdm"], omega_baryon=self._p["omega_baryon"],
+                          omega_gamma=self._p["omega_gamma"], w0=self._p["w0"], wa=self._p["wa"], h=self._p["h"],
+                          bias_z=mz, s1=s1, s2=s2, fevo1=e1, fevo2=e2)
+        tabs = dict(z=bg["z"], chi=bg["chi"], D1=bg["D1"], f=bg["f"], cH=bg["conformal_Hz"], G1=bg["G1"], G2=bg["G2"])
+        tans = dict(chi=tg["chi"], D1=tg["D1"], f=tg["f"], cH=tg["conformal_Hz"], G1=tg["G1"], G2=tg["G2"])
+        zs, ls, seps = self._p["z_mean"], [int(v) for v in self._p["l"]], self._p["sep"]
+        tables = list(TABLES)
+        terms = self._terms(None)
+        out = np.zeros((len(zs), len(ls), len(seps)))
+        if terms:
+            tv = self._table_values(tables, seps)
+            for l0 in range(0, len(ls), 4):
+                lc = ls[l0:l0 + 4]
+                _, At = mu_weights_jvp(self._engine_handle(), tabs, tans, bz, self._bias[1].y, self._bias[2].y, zs, lc,
+                                       tables, seps, flatsky=self._p["has_flatsky_local"],
+                                       nmu=self._p["multipole_quadrature_nodes"], terms=terms)
+                out[:, l0:l0 + len(lc)] = np.einsum("zltr,tr->zlr", At.cpu().numpy(), tv)
+        return out.reshape(-1)
+
     def compute_multipole(self, *, z, r, l, recompute=False):  # noqa: E741
         """The multipole at (z_mean, r, l) (coffe.pyx:3232-3300)."""
         self._check_coords_multipoles()

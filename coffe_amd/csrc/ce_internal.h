@@ -213,6 +213,16 @@ int ce_launch_mu_weights(void *stream, const ce_mu_dev *in, unsigned terms, unsi
                          int nz, const double *zmean, int nl, const int *ls, int ntab, const int *slot, int nsep,
                          const double *sep, int nmu, const double *mu, const double *wmu, double *A, double *ren_out,
                          double *coef);
+/* the tangents of the background tables along one direction (device, on the z grid) */
+typedef struct ce_mu_tan {
+    const double *chi, *f, *cH, *G1, *G2, *D1;
+} ce_mu_tan;
+/* the weights and their tangents At (layout of A) along tan, density / RSD / Doppler-1
+   terms only (no d2 / g); coef: 3 (7 nbg + 2 nb) + 18 nbg doubles */
+int ce_launch_mu_weights_jvp(void *stream, const ce_mu_dev *in, const ce_mu_tan *tan, unsigned terms, int flatsky,
+                             int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *slot,
+                             int nsep, const double *sep, int nmu, const double *mu, const double *wmu, double *A,
+                             double *At, double *coef);
 
 #ifdef __cplusplus
 }

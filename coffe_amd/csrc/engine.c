@@ -1128,7 +1128,8 @@ static void spline_node_slopes(int n, const double *x, const double *y, int ys, 
 static int mu_weights_impl(coffe_engine *e, const coffe_mu_inputs *in, unsigned terms, unsigned sources, int flatsky,
                            int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *tab_n,
                            const int *tab_l, int nsep, const double *sep, int nmu, const double *rule_mu,
-                           const double *rule_w, double *d_A, double *d_ren, void *stream)
+                           const double *rule_w, double *d_A, double *d_ren, const coffe_mu_inputs *tan,
+                           double *d_At, void *stream)
 {
     if (!e || !in || nz <= 0 || nl <= 0 || nl > 4 || ntab <= 0 || ntab > CE_MAX_TABLES + 1 || nsep <= 0 ||
         nmu < (rule_mu ? 1 : 2) || nmu > (1 << 20) || (!rule_mu) != (!rule_w) ||
@@ -1141,6 +1142,10 @@ static int mu_weights_impl(coffe_engine *e, const coffe_mu_inputs *in, unsigned 
     const unsigned dgterms = COFFE_TERM_DG_DG | COFFE_TERM_DEN_DG | COFFE_TERM_RSD_DG | COFFE_TERM_D1_DG;
     const int dg = (terms & dgterms) != 0;
     if (dg != (sources != 0) || sources > 15) return COFFE_EINVAL;
+    /* the tangent instance: density / RSD / Doppler-1 terms, the six background tangents */
+    if (tan && (dg || !d_At || !tan->bg_chi || !tan->bg_D1 || !tan->bg_f || !tan->bg_cH || !tan->bg_G1 ||
+                !tan->bg_G2 || ntab > CE_MAX_TABLES))
+        return dg ? COFFE_EUNSUPPORTED : COFFE_EINVAL;
     const int nmb = dg ? in->nmb : 0;
     if (dg && (nmb < 5 || !in->bg_a || !in->mb_z || !in->s1 || !in->s2 || !in->fevo1 || !in->fevo2 ||
                !(in->H0h > 0)))
@@ -1197,9 +1202,10 @@ static int mu_weights_impl(coffe_engine *e, const coffe_mu_inputs *in, unsigned 
     /* host arrays staged in one pinned buffer: 8 background tables, 3 bias tables, the 5
        magnification / evolution tables, zmean, sep, the rule (2 nmu), the renormalisation
        chi grid and its values with their three derivative tables */
+    const size_t ntan = tan ? 6 * (size_t)nbg : 0;
     const size_t nstage = 8 * (size_t)nbg + 3 * (size_t)nb + 5 * (size_t)nmb + (size_t)nz + (size_t)nsep +
-                          2 * (size_t)nmu + (size_t)nren + 4 * nnode;
-    const size_t ncoef = 3 * (7 * (size_t)nbg + 2 * (size_t)nb + 4 * (size_t)nmb);
+                          2 * (size_t)nmu + (size_t)nren + 4 * nnode + ntan;
+    const size_t ncoef = 3 * (7 * (size_t)nbg + 2 * (size_t)nb + 4 * (size_t)nmb) + 3 * ntan;
     const size_t total = sizeof(double) * (nstage + ncoef);
     int owned = 0;
     const double *h_mu = NULL;
@@ -1253,6 +1259,8 @@ static int mu_weights_impl(coffe_engine *e, const coffe_mu_inputs *in, unsigned 
         r_;                                                                              \
     })
     double *d_z = NULL, *d_sep = NULL, *d_mu = NULL;
+    ce_mu_tan dt;
+    memset(&dt, 0, sizeof(dt));
     if (st == COFFE_OK) {
         d.nbg = nbg;
         d.bg_z = CE_STAGE(in->bg_z, nbg);
@@ -1277,6 +1285,14 @@ static int mu_weights_impl(coffe_engine *e, const coffe_mu_inputs *in, unsigned 
         }
         d.H0h = in->H0h;
         d.Om = in->Omega_m;
+        if (tan) {
+            dt.chi = CE_STAGE(tan->bg_chi, nbg);
+            dt.D1 = CE_STAGE(tan->bg_D1, nbg);
+            dt.f = CE_STAGE(tan->bg_f, nbg);
+            dt.cH = CE_STAGE(tan->bg_cH, nbg);
+            dt.G1 = CE_STAGE(tan->bg_G1, nbg);
+            dt.G2 = CE_STAGE(tan->bg_G2, nbg);
+        }
         d_z = (double *)CE_STAGE(zmean, nz);
         d_sep = (double *)CE_STAGE(sep, nsep);
         d_mu = (double *)CE_STAGE(h_mu, 2 * (size_t)nmu);
@@ -1309,10 +1325,15 @@ static int mu_weights_impl(coffe_engine *e, const coffe_mu_inputs *in, unsigned 
         if (st == COFFE_OK && hip_ok(hipEventRecord(e->ev_stage[sb], s)) == COFFE_OK) e->stage_pending[sb] = 1;
     }
     double *d_coef = (double *)e->d_scratch + nstage;
-    if (st == COFFE_OK &&
-        ce_launch_mu_weights(s, &d, terms, sources, flatsky, paired, nz, d_z, nl, ls, ntab, slot, nsep, d_sep, nmu,
-                             d_mu, d_mu + nmu, d_A, ren ? d_ren : NULL, d_coef))
+    if (st == COFFE_OK && tan) {
+        if (ce_launch_mu_weights_jvp(s, &d, &dt, terms, flatsky, paired, nz, d_z, nl, ls, ntab, slot, nsep, d_sep,
+                                     nmu, d_mu, d_mu + nmu, d_A, d_At, d_coef))
+            st = COFFE_EDEVICE;
+    } else if (st == COFFE_OK &&
+               ce_launch_mu_weights(s, &d, terms, sources, flatsky, paired, nz, d_z, nl, ls, ntab, slot, nsep, d_sep,
+                                    nmu, d_mu, d_mu + nmu, d_A, ren ? d_ren : NULL, d_coef)) {
         st = COFFE_EDEVICE;
+    }
     if (st == COFFE_OK) {
         st = hip_ok(hipEventRecord(e->ev_scratch, s));
         e->scratch_pending = (st == COFFE_OK);
@@ -1329,7 +1350,17 @@ int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned ter
                         void *stream)
 {
     return mu_weights_impl(e, in, terms, sources, flatsky, paired, nz, zmean, nl, ls, ntab, tab_n, tab_l, nsep, sep,
-                           nmu, NULL, NULL, d_A, d_ren, stream);
+                           nmu, NULL, NULL, d_A, d_ren, NULL, NULL, stream);
+}
+
+int coffe_mu_weights_jvp(coffe_engine *e, const coffe_mu_inputs *in, const coffe_mu_inputs *tangent, unsigned terms,
+                         int flatsky, int paired, int nz, const double *zmean, int nl, const int *ls, int ntab,
+                         const int *tab_n, const int *tab_l, int nsep, const double *sep, int nmu, double *d_A,
+                         double *d_At, void *stream)
+{
+    if (!tangent || !d_At) return COFFE_EINVAL;
+    return mu_weights_impl(e, in, terms, 0, flatsky, paired, nz, zmean, nl, ls, ntab, tab_n, tab_l, nsep, sep, nmu,
+                           NULL, NULL, d_A, NULL, tangent, d_At, stream);
 }
 
 int coffe_mu_weights_rule(coffe_engine *e, const coffe_mu_inputs *in, unsigned terms, unsigned sources, int flatsky,
@@ -1341,7 +1372,7 @@ int coffe_mu_weights_rule(coffe_engine *e, const coffe_mu_inputs *in, unsigned t
     for (int q = 0; q < nmu; ++q)
         if (!(mu[q] >= -1.0 && mu[q] <= 1.0) || !isfinite(wmu[q])) return COFFE_EDOMAIN;
     return mu_weights_impl(e, in, terms, sources, flatsky, paired, nz, zmean, nl, ls, ntab, tab_n, tab_l, nsep, sep,
-                           nmu, mu, wmu, d_A, d_ren, stream);
+                           nmu, mu, wmu, d_A, d_ren, NULL, NULL, stream);
 }
 
 /* ---- small-separation nodes --------------------------------------------------- */

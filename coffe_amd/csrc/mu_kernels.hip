@@ -23,6 +23,8 @@
 #include "ce_device.h"
 #include "fullsky_forms.h"
 
+#include <type_traits>
+
 namespace {
 
 __device__ __forceinline__ double legendre_p(int l, double x)
@@ -79,6 +81,12 @@ struct mu_args {
     const double *ren_chi, *ren_v, *ren_x, *ren_y, *ren_xy;
     double *ren_out;           // [nz][nl][nsep] (paired: [npoint][nl]) or null
     const double *cb[CE_MU_NCOEF], *cc[CE_MU_NCOEF], *cd_[CE_MU_NCOEF];
+    // the JVP instance: the tangents of chi, f, conformal H, G1, G2 and D1 on the z grid
+    // along one direction, the tangents of their Akima coefficients (null: zero), and
+    // the tangent weights, laid out as A
+    const double *tg_chi, *tg_f, *tg_cH, *tg_G1, *tg_G2, *tg_D1;
+    const double *cbt[CE_MU_NCOEF], *cct[CE_MU_NCOEF], *cdt[CE_MU_NCOEF];
+    double *At;
 };
 
 // y(xq) in interval j from precomputed Akima coefficients: the value akima_at returns
@@ -92,91 +100,94 @@ __device__ __forceinline__ double akima_coef(const double *__restrict__ x, const
 
 struct coef_args {
     const double *x[CE_MU_NCOEF], *y[CE_MU_NCOEF];
+    const double *xt[CE_MU_NCOEF], *yt[CE_MU_NCOEF];   // node tangents (dual instance; null: 0)
     int n[CE_MU_NCOEF];
     double *b[CE_MU_NCOEF], *c[CE_MU_NCOEF], *d[CE_MU_NCOEF];
+    double *bt[CE_MU_NCOEF], *ct[CE_MU_NCOEF], *dt[CE_MU_NCOEF];   // their tangents (dual instance)
 };
 
+__device__ __forceinline__ double tangent_of(double) { return 0.0; }
+__device__ __forceinline__ double tangent_of(ce::dd x) { return x.t; }
+
 // Akima (GSL, non-rounded ends) coefficients of every interval of the tables: one thread
-// per (interval, table); the same slopes and formulas as akima_at
+// per (interval, table), ce::akima_coefs; T = dd also writes their tangents along the
+// nodes' tangents
+template <typename T>
 __global__ void __launch_bounds__(256) k_akima_coef(coef_args a)
 {
     const int t = blockIdx.y;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = a.n[t];
     if (j >= n - 1) return;
-    const double *x = a.x[t], *y = a.y[t];
-    auto slope = [&](int i) -> double {
-        if (i >= 0 && i <= n - 2) return (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
-        const double m0 = (y[1] - y[0]) / (x[1] - x[0]);
-        const double m1 = (y[2] - y[1]) / (x[2] - x[1]);
-        const double mn2 = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
-        const double mn3 = (y[n - 2] - y[n - 3]) / (x[n - 2] - x[n - 3]);
-        if (i == -1) return 2.0 * m0 - m1;
-        if (i == -2) return 3.0 * m0 - 2.0 * m1;
-        if (i == n - 1) return 2.0 * mn2 - mn3;
-        return 3.0 * mn2 - 2.0 * mn3;
-    };
-    const double mm2 = slope(j - 2), mm1 = slope(j - 1), m0 = slope(j), mp1 = slope(j + 1), mp2 = slope(j + 2);
-    const double ne = fabs(mp1 - m0) + fabs(mm1 - mm2);
-    double b, c, d;
-    if (ne == 0.0) {
-        b = m0; c = 0.0; d = 0.0;
-    } else {
-        const double h = x[j + 1] - x[j];
-        const double nen = fabs(mp2 - mp1) + fabs(m0 - mm1);
-        const double alpha = fabs(mm1 - mm2) / ne;
-        double tn;
-        if (nen == 0.0) {
-            tn = m0;
-        } else {
-            const double an = fabs(m0 - mm1) / nen;
-            tn = (1.0 - an) * m0 + an * mp1;
-        }
-        b = (1.0 - alpha) * mm1 + alpha * m0;
-        c = (3.0 * m0 - 2.0 * b - tn) / h;
-        d = (b + tn - 2.0 * m0) / (h * h);
+    const ce::akima_cf<T> cf = ce::akima_coefs<T>(a.x[t], a.xt[t], a.y[t], a.yt[t], n, j);
+    a.b[t][j] = ce::val(cf.b);
+    a.c[t][j] = ce::val(cf.c);
+    a.d[t][j] = ce::val(cf.d);
+    if (a.bt[t]) {
+        a.bt[t][j] = tangent_of(cf.b);
+        a.ct[t][j] = tangent_of(cf.c);
+        a.dt[t][j] = tangent_of(cf.d);
     }
-    a.b[t][j] = b;
-    a.c[t][j] = c;
-    a.d[t][j] = d;
+}
+
+// a table value by its Akima coefficients in interval j (akima_coef); the dual instance
+// also carries the tangents of the nodes (xt, yt) and of the coefficients of table K
+template <typename T>
+__device__ __forceinline__ T lookup(const mu_args &a, int K, const double *__restrict__ x,
+                                    const double *__restrict__ xt, const double *__restrict__ y,
+                                    const double *__restrict__ yt, int j, T q)
+{
+    if constexpr (std::is_same<T, double>::value) {
+        return akima_coef(x, y, a.cb[K], a.cc[K], a.cd_[K], j, q);
+    } else {
+        auto dual = [&](const double *v, const double *vt) { return T(v[j], vt ? vt[j] : 0.0); };
+        const T delx = q - dual(x, xt);
+        return dual(y, yt) + delx * (dual(a.cb[K], a.cbt[K]) + delx * (dual(a.cc[K], a.cct[K]) +
+                                                                       dual(a.cd_[K], a.cdt[K]) * delx));
+    }
 }
 
 // the background at the end point chi of population p (1 or 2): z(chi) and f, conformal
 // H, G_p, D1 at that z (one interval search on the chi table, started from a linear
 // extrapolation of the mean's interval, one on the z grid shared by the lookups), the
 // galaxy bias b_p, and for the d2 / g terms a(z), s_p and f_evo,p
-struct endpoint {
-    double z, f, H, G, D1, b, a, s, fevo;
+template <typename T>
+struct endpoint_t {
+    T z, f, H, G, D1, b, a, s, fevo;
 };
+using endpoint = endpoint_t<double>;
 
-template <bool DG>
-__device__ __forceinline__ endpoint background_at(const mu_args &a, double chi, int p, int jbar, double chib,
-                                                  double dchi)
+template <typename T, bool DG>
+__device__ __forceinline__ endpoint_t<T> background_at(const mu_args &a, T chi, int p, int jbar, double chib,
+                                                       double dchi)
 {
-    endpoint e;
+    endpoint_t<T> e;
     const int n = a.nbg;
     const double nan = __builtin_nan("");
     e.a = e.s = e.fevo = 0.0;
-    if (!(chi >= a.bg_chi[0] && chi <= a.bg_chi[n - 1])) {
+    const double chiv = ce::val(chi);
+    if (!(chiv >= a.bg_chi[0] && chiv <= a.bg_chi[n - 1])) {
         e.z = e.f = e.H = e.G = e.D1 = e.b = e.a = e.s = e.fevo = nan;
         return e;
     }
-    const int jc = ce::interval_from(a.bg_chi, n, chi, ce::clamp_index(jbar + floor((chi - chib) / dchi), n));
-    e.z = akima_coef(a.bg_chi, a.bg_z, a.cb[K_ZCHI], a.cc[K_ZCHI], a.cd_[K_ZCHI], jc, chi);
-    const int k = ce::interval_from(a.bg_z, n, e.z, ce::interval_guess(a.bg_z, n, e.z));
-    e.f = akima_coef(a.bg_z, a.bg_f, a.cb[K_F], a.cc[K_F], a.cd_[K_F], k, e.z);
-    e.H = akima_coef(a.bg_z, a.bg_cH, a.cb[K_H], a.cc[K_H], a.cd_[K_H], k, e.z);
+    const int jc = ce::interval_from(a.bg_chi, n, chiv, ce::clamp_index(jbar + floor((chiv - chib) / dchi), n));
+    // z(chi): the chi nodes move with the background, the z nodes do not
+    e.z = lookup<T>(a, K_ZCHI, a.bg_chi, a.tg_chi, a.bg_z, nullptr, jc, chi);
+    const double zv = ce::val(e.z);
+    const int k = ce::interval_from(a.bg_z, n, zv, ce::interval_guess(a.bg_z, n, zv));
+    e.f = lookup<T>(a, K_F, a.bg_z, nullptr, a.bg_f, a.tg_f, k, e.z);
+    e.H = lookup<T>(a, K_H, a.bg_z, nullptr, a.bg_cH, a.tg_cH, k, e.z);
     const int kg = p == 1 ? K_G1 : K_G2;
-    e.G = akima_coef(a.bg_z, p == 1 ? a.bg_G1 : a.bg_G2, a.cb[kg], a.cc[kg], a.cd_[kg], k, e.z);
-    e.D1 = akima_coef(a.bg_z, a.bg_D1, a.cb[K_D1], a.cc[K_D1], a.cd_[K_D1], k, e.z);
-    if (!(e.z >= a.bias_z[0] && e.z <= a.bias_z[a.nb - 1])) {
+    e.G = lookup<T>(a, kg, a.bg_z, nullptr, p == 1 ? a.bg_G1 : a.bg_G2, p == 1 ? a.tg_G1 : a.tg_G2, k, e.z);
+    e.D1 = lookup<T>(a, K_D1, a.bg_z, nullptr, a.bg_D1, a.tg_D1, k, e.z);
+    if (!(zv >= a.bias_z[0] && zv <= a.bias_z[a.nb - 1])) {
         e.b = nan;
     } else {
-        const int kb = ce::interval_from(a.bias_z, a.nb, e.z, ce::interval_guess(a.bias_z, a.nb, e.z));
+        const int kb = ce::interval_from(a.bias_z, a.nb, zv, ce::interval_guess(a.bias_z, a.nb, zv));
         const int kt = p == 1 ? K_B1 : K_B2;
-        e.b = akima_coef(a.bias_z, p == 1 ? a.bias1 : a.bias2, a.cb[kt], a.cc[kt], a.cd_[kt], kb, e.z);
+        e.b = lookup<T>(a, kt, a.bias_z, nullptr, p == 1 ? a.bias1 : a.bias2, nullptr, kb, e.z);
     }
-    if (DG) {
+    if constexpr (DG) {
         e.a = akima_coef(a.bg_z, a.bg_a, a.cb[K_A], a.cc[K_A], a.cd_[K_A], k, e.z);
         if (!(e.z >= a.mb_z[0] && e.z <= a.mb_z[a.nmb - 1])) {
             e.s = e.fevo = nan;
@@ -233,32 +244,31 @@ __device__ __forceinline__ double ren_bicubic(const mu_args &a, double x, double
 // coefficients c_s of the local terms at one (z, mu, r), times G = D1(z1) D1(z2) (full
 // sky) or D1(z_mean)^2 (flat sky), functions.c:1457-1467; c[SREN] is minus the
 // coefficient of the renormalisation grid's value at (chi1, chi2)
-template <bool DG>
-__device__ __forceinline__ void local_coefficients(const mu_args &a, double mu, double sep, double chi_mean,
-                                                   double fmean, double Hmean, double bm1, double bm2, double Gm1,
-                                                   double Gm2, double Dmean, int jbar, double chib, double dchi,
-                                                   double *c)
+template <typename T, bool DG>
+__device__ __forceinline__ void local_coefficients(const mu_args &a, double mu, double sep, T chi_mean, T fmean,
+                                                   T Hmean, T bm1, T bm2, T Gm1, T Gm2, T Dmean, int jbar,
+                                                   double chib, double dchi, T *c)
 {
     for (int s = 0; s < (DG ? NACC : S40); ++s) c[s] = 0.0;
-    const double chi1 = chi_mean - sep * mu / 2.;
-    const double chi2 = chi_mean + sep * mu / 2.;
+    const T chi1 = chi_mean - sep * mu / 2.;
+    const T chi2 = chi_mean + sep * mu / 2.;
     // the flat-sky forms of the den / rsd / d1 terms use the mean's background only; the
     // full-sky ones and every d2 / g term the end points' (the reference reads them in
     // flat sky too), with the geometric coefficients of fullsky_forms.h (the reference's
     // polynomials in cos theta with their (chi/s)^n cancellations done exactly)
     const bool ends = !a.flatsky || DG;
-    endpoint e1, e2;
-    ce_fullsky_geom g;
+    endpoint_t<T> e1, e2;
+    ce_fullsky_geom_t<T> g;
     if (ends) {
-        e1 = background_at<DG>(a, chi1, 1, jbar, chib, dchi);
-        e2 = background_at<DG>(a, chi2, 2, jbar, chib, dchi);
+        e1 = background_at<T, DG>(a, chi1, 1, jbar, chib, dchi);
+        e2 = background_at<T, DG>(a, chi2, 2, jbar, chib, dchi);
         g = ce_fullsky_geometry(chi_mean, sep, mu);
     } else {
         e1.z = e1.f = e1.H = e1.G = e1.D1 = e1.b = e1.a = e1.s = e1.fevo = 0.0;
         e2 = e1;
     }
-    const double f1 = e1.f, f2 = e2.f, H1 = e1.H, H2 = e2.H, b1 = e1.b, b2 = e2.b;
-    const double G1 = e1.G, G2 = e2.G;
+    const T f1 = e1.f, f2 = e2.f, H1 = e1.H, H2 = e2.H, b1 = e1.b, b2 = e2.b;
+    const T G1 = e1.G, G2 = e2.G;
     const double P2 = legendre_p(2, mu), P4 = legendre_p(4, mu);
     if (a.terms & COFFE_TERM_DEN_DEN) {   // functions.c:129-229
         if (a.flatsky) c[S00] += bm1 * bm2;
@@ -310,7 +320,7 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double mu, 
             c[S13] += f1 * f2 * H2 * G2 * g.rd3_a + f2 * f1 * H1 * G1 * g.rd3_b;
         }
     }
-    if (DG) {
+    if constexpr (DG) {
         const double Sg1 = dg_source(a, e1), Sg2 = dg_source(a, e2);
         if (a.terms & COFFE_TERM_DG_DG) {     // I_0^4 minus its renormalisation at (chi1, chi2)
             c[S40] += Sg1 * Sg2;
@@ -323,7 +333,7 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double mu, 
         }
         if (a.terms & COFFE_TERM_D1_DG) c[S31] += -(H1 * f1 * Sg2 * g.dd1_b + H2 * f2 * Sg1 * g.dd1_a);
     }
-    const double G = a.flatsky ? Dmean * Dmean : e1.D1 * e2.D1;
+    const T G = a.flatsky ? Dmean * Dmean : e1.D1 * e2.D1;
     for (int s = 0; s < (DG ? NACC : S40); ++s) c[s] *= G;
 }
 
@@ -363,7 +373,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         if (q < a.nmu) {
             mu = a.mu[q];
             w = a.wmu[q];
-            local_coefficients<DG>(a, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, jbar, chib, dchi, c);
+            local_coefficients<double, DG>(a, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, jbar,
+                                           chib, dchi, c);
         } else {
 #pragma unroll
             for (int s = 0; s < NC; ++s) c[s] = 0.0;
@@ -400,15 +411,96 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
 }
 
+// the forward-mode instance (density, RSD, Doppler-1; no d2 / g): the weights and their
+// tangents along the background tables' tangents (chi and the z(chi) nodes, f, conformal
+// H, G1, G2, D1), the same node loop with dual numbers and two accumulators per slot
+template <int NL>
+__global__ void __launch_bounds__(256) k_mu_weights_jvp(mu_args a)
+{
+    using ce::dd;
+    __shared__ double sacc[4][2 * CE_MU_MAXL * S40];
+    const int lane = threadIdx.x & 63;
+    const int point = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int npoint = a.paired ? a.nz : a.nz * a.nsep;
+    if (point >= npoint) return;   // whole wavefronts exit together
+    double *acc = sacc[threadIdx.x >> 6];
+    for (int i = lane; i < 2 * NL * S40; i += 64) acc[i] = 0.0;
+    const int iz = a.paired ? point : point / a.nsep, ir = a.paired ? point : point % a.nsep;
+    const double zm = a.zmean[iz], sep = a.sep[ir];
+    const int n = a.nbg;
+    const double nan = __builtin_nan("");
+    const bool inside = zm >= a.bg_z[0] && zm <= a.bg_z[n - 1];
+    const int jbar = inside ? ce::interval_bisect(a.bg_z, 0, n - 1, zm) : 0;
+    // the mean's background as duals: Akima of the tables with their tangents at zm
+    auto mean = [&](const double *y, const double *yt) -> dd {
+        if (!inside) return dd(nan, nan);
+        const ce::akima_cf<dd> cf = ce::akima_coefs<dd>(a.bg_z, nullptr, y, yt, n, jbar);
+        const double delx = zm - a.bg_z[jbar];
+        return cf.y0 + delx * (cf.b + delx * (cf.c + cf.d * delx));
+    };
+    const dd chi_mean = mean(a.bg_chi, a.tg_chi), fmean = mean(a.bg_f, a.tg_f), Hmean = mean(a.bg_cH, a.tg_cH);
+    const dd Gm1 = mean(a.bg_G1, a.tg_G1), Gm2 = mean(a.bg_G2, a.tg_G2), Dmean = mean(a.bg_D1, a.tg_D1);
+    const dd bm1 = ce::akima_general(a.bias_z, a.bias1, a.nb, zm), bm2 = ce::akima_general(a.bias_z, a.bias2, a.nb, zm);
+    const double chib = a.bg_chi[jbar], dchi = a.bg_chi[jbar + 1] - a.bg_chi[jbar];
+    for (int q0 = 0; q0 < a.nmu; q0 += 64) {
+        const int q = q0 + lane;
+        dd c[S40];
+        double mu = 0.0, w = 0.0;
+        if (q < a.nmu) {
+            mu = a.mu[q];
+            w = a.wmu[q];
+            local_coefficients<dd, false>(a, mu, sep, chi_mean, fmean, Hmean, bm1, bm2, Gm1, Gm2, Dmean, jbar, chib,
+                                          dchi, c);
+        } else {
+#pragma unroll
+            for (int s = 0; s < S40; ++s) c[s] = 0.0;
+        }
+#pragma unroll
+        for (int il = 0; il < NL; ++il) {
+            if (il >= a.nl) break;
+            const double pl = w * legendre_p(a.ls[il], mu);
+#pragma unroll
+            for (int s = 0; s < S40; ++s) {
+                double v = pl * c[s].v, t = pl * c[s].t;
+                for (int off = 32; off > 0; off >>= 1) {
+                    v += __shfl_xor(v, off, 64);
+                    t += __shfl_xor(t, off, 64);
+                }
+                if (lane == 0) {
+                    acc[2 * (il * S40 + s)] += v;
+                    acc[2 * (il * S40 + s) + 1] += t;
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): the LDS stores have completed
+    if (lane < NL * S40) {
+        const int il = lane / S40, s = lane % S40;
+        if (il < a.nl) {
+            const double f = (2 * a.ls[il] + 1) * 0.5;
+            for (int t = 0; t < a.ntab; ++t)
+                if (a.slot[t] == s) {
+                    const int64_t o = a.paired ? ((int64_t)point * a.nl + il) * a.ntab + t
+                                               : (((int64_t)iz * a.nl + il) * a.ntab + t) * a.nsep + ir;
+                    a.A[o] = f * acc[2 * lane];
+                    a.At[o] = f * acc[2 * lane + 1];
+                }
+        }
+    }
+}
+
 }  // namespace
 
-extern "C" int ce_launch_mu_weights(void *stream, const ce_mu_dev *in, unsigned terms, unsigned dg, int flatsky,
-                                    int paired, int nz, const double *zmean, int nl, const int *ls, int ntab,
-                                    const int *slot, int nsep, const double *sep, int nmu, const double *mu,
-                                    const double *wmu, double *A, double *ren_out, double *coef)
+namespace {
+
+// the Akima coefficient tables (coef: 3 per node of each table; with tangents a further 3
+// per node of the six background tables that carry them) and the kernel arguments
+void prepare_args(void *stream, const ce_mu_dev *in, const ce_mu_tan *tan, unsigned terms, unsigned dg, int flatsky,
+                  int paired, int nz, const double *zmean, int nl, const int *ls, int ntab, const int *slot, int nsep,
+                  const double *sep, int nmu, const double *mu, const double *wmu, double *A, double *ren_out,
+                  double *coef, mu_args &a)
 {
-    if (nl > CE_MU_MAXL || ntab > CE_MAX_TABLES + 1) return (int)hipErrorInvalidValue;
-    mu_args a;
     memset(&a, 0, sizeof(a));
     coef_args ca;
     memset(&ca, 0, sizeof(ca));
@@ -430,7 +522,28 @@ extern "C" int ce_launch_mu_weights(void *stream, const ce_mu_dev *in, unsigned 
             p += 3 * n;
             if (n > nmax) nmax = n;
         }
-        hipLaunchKernelGGL(k_akima_coef, dim3((nmax + 255) / 256, ntabs), dim3(256), 0, (hipStream_t)stream, ca);
+        if (tan) {
+            // the z(chi) table's nodes move with chi; the others' values with their tangents
+            ca.xt[K_ZCHI] = tan->chi;
+            ca.yt[K_F] = tan->f;
+            ca.yt[K_H] = tan->cH;
+            ca.yt[K_G1] = tan->G1;
+            ca.yt[K_G2] = tan->G2;
+            ca.yt[K_D1] = tan->D1;
+            for (int t = K_ZCHI; t <= K_D1; ++t) {
+                const int n = ns[t];
+                ca.bt[t] = p; ca.ct[t] = p + n; ca.dt[t] = p + 2 * n;
+                a.cbt[t] = ca.bt[t]; a.cct[t] = ca.ct[t]; a.cdt[t] = ca.dt[t];
+                p += 3 * n;
+            }
+            hipLaunchKernelGGL(k_akima_coef<ce::dd>, dim3((nmax + 255) / 256, ntabs), dim3(256), 0,
+                               (hipStream_t)stream, ca);
+            a.tg_chi = tan->chi; a.tg_f = tan->f; a.tg_cH = tan->cH; a.tg_G1 = tan->G1; a.tg_G2 = tan->G2;
+            a.tg_D1 = tan->D1;
+        } else {
+            hipLaunchKernelGGL(k_akima_coef<double>, dim3((nmax + 255) / 256, ntabs), dim3(256), 0,
+                               (hipStream_t)stream, ca);
+        }
     }
     a.nbg = in->nbg; a.bg_z = in->bg_z; a.bg_chi = in->bg_chi; a.bg_D1 = in->bg_D1; a.bg_f = in->bg_f;
     a.bg_cH = in->bg_cH; a.bg_G1 = in->bg_G1; a.bg_G2 = in->bg_G2; a.bg_a = in->bg_a;
@@ -445,6 +558,39 @@ extern "C" int ce_launch_mu_weights(void *stream, const ce_mu_dev *in, unsigned 
     a.ntab = ntab;
     for (int t = 0; t < ntab; ++t) a.slot[t] = slot[t];
     a.nsep = nsep; a.sep = sep; a.nmu = nmu; a.mu = mu; a.wmu = wmu; a.A = A; a.ren_out = ren_out;
+}
+
+}  // namespace
+
+extern "C" int ce_launch_mu_weights_jvp(void *stream, const ce_mu_dev *in, const ce_mu_tan *tan, unsigned terms,
+                                        int flatsky, int paired, int nz, const double *zmean, int nl, const int *ls,
+                                        int ntab, const int *slot, int nsep, const double *sep, int nmu,
+                                        const double *mu, const double *wmu, double *A, double *At, double *coef)
+{
+    if (nl > CE_MU_MAXL || ntab > CE_MAX_TABLES || !tan) return (int)hipErrorInvalidValue;
+    mu_args a;
+    prepare_args(stream, in, tan, terms, 0, flatsky, paired, nz, zmean, nl, ls, ntab, slot, nsep, sep, nmu, mu, wmu,
+                 A, nullptr, coef, a);
+    a.At = At;
+    const int npoint = paired ? nz : nz * nsep;
+    const int blocks = (npoint + 3) / 4;
+    hipStream_t s = (hipStream_t)stream;
+    if (nl <= 1) hipLaunchKernelGGL((k_mu_weights_jvp<1>), dim3(blocks), dim3(256), 0, s, a);
+    else if (nl == 2) hipLaunchKernelGGL((k_mu_weights_jvp<2>), dim3(blocks), dim3(256), 0, s, a);
+    else if (nl == 3) hipLaunchKernelGGL((k_mu_weights_jvp<3>), dim3(blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_mu_weights_jvp<CE_MU_MAXL>), dim3(blocks), dim3(256), 0, s, a);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ce_launch_mu_weights(void *stream, const ce_mu_dev *in, unsigned terms, unsigned dg, int flatsky,
+                                    int paired, int nz, const double *zmean, int nl, const int *ls, int ntab,
+                                    const int *slot, int nsep, const double *sep, int nmu, const double *mu,
+                                    const double *wmu, double *A, double *ren_out, double *coef)
+{
+    if (nl > CE_MU_MAXL || ntab > CE_MAX_TABLES + 1) return (int)hipErrorInvalidValue;
+    mu_args a;
+    prepare_args(stream, in, nullptr, terms, dg, flatsky, paired, nz, zmean, nl, ls, ntab, slot, nsep, sep, nmu, mu,
+                 wmu, A, ren_out, coef, a);
     const int npoint = paired ? nz : nz * nsep;
     const int blocks = (npoint + 3) / 4;
     hipStream_t s = (hipStream_t)stream;

@@ -265,6 +265,54 @@ def divergent_tables(k, pk, sep=None, chi=None, kmin=None, kmax=None, return_sta
     return sep, i40, r0, ren
 
 
+def mu_weights_jvp(engine, bg, tangent, bias_z, bias1, bias2, zmean, ls, tables, sep, den=True, rsd=True, d1=False,
+                   flatsky=False, nmu=256, terms=None, stream=None):
+    """(A, At): the weights of mu_weights (density / RSD / Doppler-1 terms) and their
+    tangents along one direction of the background (coffe_mu_weights_jvp).  tangent: the
+    derivatives of the tables chi, D1, f, cH, G1, G2 at the nodes bg["z"] (from
+    coffe_amd.background.jvp); the biases have none.  Both (nz, nl, ntab, nsep) device
+    tensors; nl <= 4."""
+    import torch
+    L = engine.L
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    dp = ctypes.POINTER(ctypes.c_double)
+    keep = []
+
+    def arr(a):
+        v = f64(a)
+        keep.append(v)
+        return v.ctypes.data_as(dp), len(v)
+
+    inp, tan = MuInputs(), MuInputs()
+    for name, key in (("bg_z", "z"), ("bg_chi", "chi"), ("bg_D1", "D1"), ("bg_f", "f"), ("bg_cH", "cH"),
+                      ("bg_G1", "G1"), ("bg_G2", "G2")):
+        ptr, n = arr(bg[key])
+        setattr(inp, name, ptr)
+        if key != "z":
+            tp, tn = arr(tangent[key])
+            if tn != n:
+                raise ValueError(f"the tangent of {key} must have one value per node")
+            setattr(tan, name, tp)
+    inp.nbg = tan.nbg = n
+    inp.bias_z, inp.nb = arr(bias_z)
+    inp.bias1, _ = arr(bias1)
+    inp.bias2, _ = arr(bias2)
+    zm, sp = f64(np.atleast_1d(zmean)), f64(np.atleast_1d(sep))
+    ls_c = (ctypes.c_int * len(ls))(*ls)
+    tn_c = (ctypes.c_int * len(tables))(*[t[0] for t in tables])
+    tl_c = (ctypes.c_int * len(tables))(*[t[1] for t in tables])
+    if terms is None:
+        terms = terms_of(den, rsd, d1)
+    shape = (len(zm), len(ls), len(tables), len(sp))
+    A = torch.empty(shape, dtype=torch.float64, device="cuda")
+    At = torch.empty(shape, dtype=torch.float64, device="cuda")
+    st = stream if stream is not None else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(L.coffe_mu_weights_jvp(engine.h, ctypes.byref(inp), ctypes.byref(tan), int(terms), int(flatsky), 0,
+                                 len(zm), zm.ctypes.data_as(dp), len(ls), ls_c, len(tables), tn_c, tl_c, len(sp),
+                                 sp.ctypes.data_as(dp), int(nmu), _ptr(A), _ptr(At), st), "coffe_mu_weights_jvp")
+    return A, At
+
+
 def akima(x, y, xq):
     """Host Akima interpolation through the engine library (coffe_akima_eval)."""
     L = load_library()

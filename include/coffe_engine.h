@@ -258,6 +258,24 @@ int coffe_mu_weights_ex(coffe_engine *e, const coffe_mu_inputs *in, unsigned ter
                         void *stream);
 
 /*
+ * The weights of coffe_mu_weights_ex (density, RSD and Doppler-1 terms, full or flat sky,
+ * GL nmu) and their tangents d_At (layout of d_A) along one direction of the background
+ * (no reference counterpart: the derivative of the multipoles with respect to the
+ * cosmological parameters through the background, functions.c:98-127 lookups, for
+ * d M / d theta = sum_t At I_t): tangent->bg_chi, bg_D1, bg_f, bg_cH, bg_G1, bg_G2 are the
+ * derivatives of those tables at the nodes in->bg_z (the z grid is fixed; chi's tangent
+ * also moves the nodes of the z(chi) inversion), from coffe_background_jvp; the other
+ * fields of tangent are ignored and the biases have no tangent.  Forward-mode dual
+ * numbers through the Akima lookups (their slopes' |.| included) and the full-sky
+ * geometry.  The d2 / g terms return COFFE_EUNSUPPORTED.  Asynchronous like
+ * coffe_mu_weights_ex.
+ */
+int coffe_mu_weights_jvp(coffe_engine *e, const coffe_mu_inputs *in, const coffe_mu_inputs *tangent, unsigned terms,
+                         int flatsky, int paired, int nz, const double *zmean, int nl, const int *ls, int ntab,
+                         const int *tab_n, const int *tab_l, int nsep, const double *sep, int nmu, double *d_A,
+                         double *d_At, void *stream);
+
+/*
  * coffe_mu_weights_ex with an explicit mu rule of nmu >= 1 nodes mu[q] in [-1, 1] and
  * weights wmu[q] in place of the Gauss-Legendre one (A = (2l+1)/2 sum_q wmu_q P_l(mu_q)
  * G c_t): one node mu with weight 2 and l = 0 gives the local correlation function at
