@@ -1,27 +1,108 @@
-"""Drop-in subset of COFFE's Python API (python/coffe/coffe.pyx) for the spherical-
-Bessel path: the Coffe object with its parameters, set_power_spectrum_linear,
-galaxy biases, background lookups, integral(r, n, l), compute_multipole,
-compute_multipoles_bulk and compute_covariance_bulk, computed by the MI355X engine.
+"""Drop-in for COFFE's Python API (python/coffe/coffe.pyx) on the spherical-Bessel
+path: the Coffe object with its parameters, settings files (from_file / to_file),
+set_power_spectrum_linear, biases, background lookups, integral(r, n, l), the
+correlation function, the multipoles, their redshift averages and the Gaussian
+covariance of both, computed by the MI355X engine.
 
 Units follow the reference: k in 1/Mpc, P in Mpc^3, separations in Mpc.  Parameters
 are validated as the reference validates them (TypeError / ValueError), lazily
 computed pieces are dropped when a parameter they depend on changes
-(coffe.pyx:948-993), and results come back as Multipoles / Covariance records in the
-reference's order (multipoles.c:76-86: z, l, r; covariance.c:1046-1060: z, l1, r1,
-l2, r2).  The local terms are density, RSD, Doppler-1, d2 and g1-g3 (with their cross
-terms); redshift-averaged multipoles (compute_average_multipoles_bulk) are computed for
-them too.  Lensing, g4, g5, the CLASS / corrfunc paths and the covariance of the averaged
-multipoles are not part of this engine and raise NotImplementedError.
+(coffe.pyx:948-993), and results come back as Multipoles / Covariance / ... records in
+the reference's order (multipoles.c:76-86: z, l, r; covariance.c:1046-1060: z, l1, r1,
+l2, r2).  The terms are density, RSD, Doppler-1, d2 and g1-g3 with their cross terms.
+Lensing, g4, g5, the non-local flat-sky switches, massive neutrinos and the CLASS /
+nonlinear spectra are not part of this engine and raise NotImplementedError; the
+parameters of the CLASS interface and of the reference's CPU integrators are kept and
+written to settings files but change nothing here.
 """
+import warnings
+from configparser import ConfigParser
+from pathlib import Path
+
 import numpy as np
 
 from . import background as _background
 from .representation import AverageCovariance, AverageMultipoles, Corrfunc, Covariance, Multipoles
 
 TABLES = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]   # integrals.c:599-608
-_UNSUPPORTED = ("has_lensing", "has_g4", "has_g5")
+_UNSUPPORTED = ("has_lensing", "has_g4", "has_g5", "has_flatsky_nonlocal", "has_flatsky_local_nonlocal")
 _DG = ("d2", "g1", "g2", "g3")
 COFFE_H0 = 1 / 2997.92458   # h/Mpc (common.h)
+
+# the named choices of the enum-valued parameters (coffe.pyx:213-233, common.h:283-310)
+_CHOICES = {
+    "pk_type": {0: "linear", 1: "linear_class", 2: "halofit", 3: "hmcode"},
+    "covariance_integration_method": {1: "standard", 2: "fftlog"},
+    "covariance_interpolation_method": {1: "bilinear", 2: "bicubic"},
+    "integration_1d_type": {1: "gsl", 2: "exponential"},
+}
+
+_MAX_SIZE = 2 ** 63 - 1
+# Parameters of the reference's CLASS interface and of its CPU integrators
+# (coffe.pyx:1306-1545, 2439-2520): checked as the reference checks them, kept, listed
+# by `parameters` and written by to_file, with no effect on what this engine computes.
+_KEPT = {
+    "n_s": ((int, float), -100, 100), "sigma8": ((int, float), -100, 100), "A_s": ((int, float), -100, 100),
+    "T_cmb": ((int, float), -100, 100), "N_ur": ((int, float), -100, 100), "YHe": ((int, float), -100, 100),
+    "N_ncdm": (int, -100, 100), "integration_1d_prec": (float, 0, 1), "integration_sampling": (int, 0, _MAX_SIZE),
+    "k_per_decade_for_bao": ((int, float), None, None), "k_per_decade_for_pk": ((int, float), None, None),
+    "start_large_k_at_tau_h_over_tau_k": ((int, float), None, None), "l_max_g": (int, None, None),
+    "l_max_ur": (int, None, None), "tol_perturb_integration": ((int, float), None, None),
+    "radiation_streaming_trigger_tau_over_tau_k": ((int, float), None, None),
+    "ur_fluid_trigger_tau_over_tau_k": ((int, float), None, None),
+}
+
+
+class CoffeConfigParser(ConfigParser):
+    """The settings-file parser (coffe.pyx:59-82): arrays are written "[a, b, ...]"."""
+
+    def getfloat_array(self, section, option, *args, **kwargs):
+        return [float(v) for v in self.get(section, option).strip("[]").split(",")]
+
+    def getint_array(self, section, option, *args, **kwargs):
+        return [int(v) for v in self.get(section, option).strip("[]").split(",")]
+
+    def getstr_array(self, section, option, *args, **kwargs):
+        return [str(v.strip(' "')) for v in self.get(section, option).strip("[]").split(",")]
+
+
+# Options of older settings files and what replaces them (coffe.pyx:571-589); None:
+# read and ignored, with a DeprecationWarning either way.
+_LEGACY = [
+    ("only_cross_correlations", "int", "has_only_cross_correlations"),
+    ("covariance_z_mean", "float_array", "z_mean"),
+    ("covariance_deltaz", "float_array", "deltaz"),
+    ("covariance_fsky", "float_array", "fsky"),
+    ("covariance_pixelsize", "float_array", "pixelsize"),
+    ("covariance_step_size", "float", None),
+    ("covariance_zmin", "float_array", "z_min"),
+    ("covariance_zmax", "float_array", "z_max"),
+    ("covariance_minimum_separation", "float", None),
+    ("covariance_window", "int", "has_binned_covariance"),
+    ("covariance_density", "float_array", ["number_density1", "number_density2"]),
+    ("covariance_integration_bins", "int", None),
+    ("multipoles", "int_array", "l"),
+    ("flatsky_local", "int", "has_flatsky_local"),
+    ("flatsky_local_nonlocal", "int", "has_flatsky_local_nonlocal"),
+    ("flatsky_nonlocal", "int", "has_flatsky_nonlocal"),
+]
+_CONTRIBUTION_NAMES = {"den": "has_density", "rsd": "has_rsd", "len": "has_lensing", "d1": "has_d1",
+                       "d2": "has_d2", "g1": "has_g1", "g2": "has_g2", "g3": "has_g3", "g4": "has_g4",
+                       "g5": "has_g5"}
+_FILE_FLOAT_ARRAYS = ("sep", "z_mean", "deltaz", "z_min", "z_max", "mu", "number_density1", "number_density2",
+                      "pixelsize", "fsky")
+_FILE_INT_ARRAYS = ("l", "covariance_populations")
+_FILE_FLOATS = ("omega_m", "omega_baryon", "omega_gamma", "w0", "wa", "h", "sigma8", "n_s", "T_cmb", "A_s",
+                "k_per_decade_for_bao", "k_per_decade_for_pk", "start_large_k_at_tau_h_over_tau_k",
+                "tol_perturb_integration", "radiation_streaming_trigger_tau_over_tau_k",
+                "ur_fluid_trigger_tau_over_tau_k", "N_ur", "m_ncdm", "k_min", "k_max", "YHe", "integration_1d_prec")
+_FILE_INTS = ("has_density", "has_lensing", "has_rsd", "has_d1", "has_d2", "has_g1", "has_g2", "has_g3", "has_g4",
+              "has_g5", "integration_sampling", "background_sampling", "has_binned_covariance", "has_flatsky_local",
+              "has_flatsky_nonlocal", "has_flatsky_local_nonlocal", "has_only_cross_correlations",
+              "covariance_cosmic", "covariance_mixed", "covariance_poisson", "covariance_integration_sampling",
+              "integrals_fftlog_sampling", "N_ncdm", "l_max_g", "l_max_ur", "multipole_quadrature_nodes", "verbose")
+_FILE_INT_OR_STR = ("covariance_integration_method", "pk_type", "covariance_interpolation_method",
+                    "integration_1d_type")
 
 
 def _check_parameter(name, value, kind, xmin=None, xmax=None):
@@ -74,6 +155,16 @@ class Coffe:
         covariance_cosmic=True, covariance_mixed=True, covariance_poisson=True,
         has_binned_covariance=False, covariance_populations=[1, 1, 1, 1],
         multipole_quadrature_nodes=256,
+        # the window of P(k) the integrals use (parser.c:840-841); set_power_spectrum_linear
+        # moves it to the ends of the samples (coffe.pyx:2974-2977)
+        k_min=1e-5, k_max=300.0, background_sampling=10000, verbose=False,
+        pk_type=0, covariance_integration_method=1, covariance_interpolation_method=2, integration_1d_type=1,
+        # parser.c:750-799, 994: values of the CLASS interface and the CPU integrators
+        integration_1d_prec=1e-5, integration_sampling=750000,
+        n_s=0.96, sigma8=0.8156, A_s=0.0, T_cmb=2.726, N_ur=2.0328, YHe=0.0, N_ncdm=1, m_ncdm=0.0,
+        k_per_decade_for_bao=70.0, k_per_decade_for_pk=10.0, start_large_k_at_tau_h_over_tau_k=0.07,
+        l_max_g=12, l_max_ur=17, tol_perturb_integration=1e-5,
+        radiation_streaming_trigger_tau_over_tau_k=45.0, ur_fluid_trigger_tau_over_tau_k=30.0,
     )
 
     def __init__(self, **kwargs):
@@ -96,6 +187,13 @@ class Coffe:
         self._ren = None
         self._free_results()
 
+    def _free_spectrum(self):
+        """Drops what depends on P(k) or its window."""
+        self._tables = None
+        self._divergent = None
+        self._ren = None
+        self._free_results()
+
     def _free_results(self):
         self._multipoles = None
         self._average_multipoles = None
@@ -112,7 +210,13 @@ class Coffe:
     # ---- parameters --------------------------------------------------------------
     @property
     def parameters(self):
-        return {k: (list(v) if isinstance(v, np.ndarray) else v) for k, v in self._p.items()}
+        """The settable parameters and their values, as set_parameters takes them
+        (coffe.pyx:1136-1154)."""
+        out = {}
+        for k in list(self._p) + ["omega_m"] + list(_UNSUPPORTED):
+            v = getattr(self, k)
+            out[k] = list(v) if isinstance(v, np.ndarray) else v
+        return out
 
     def set_parameters(self, **kwargs):
         for k, v in kwargs.items():
@@ -123,32 +227,49 @@ class Coffe:
                 raise AttributeError(f"unknown parameter {k}")
             setattr(self, k, v)
 
-    def _set_omega(self, name, value):
+    def _set_omegas(self, name, value, others, cdm, gamma):
+        """The reference's bookkeeping (coffe.pyx:1178-1256): Omega_m = Omega_cdm +
+        Omega_baryon; setting omega_baryon or omega_m keeps the other of the two and moves
+        Omega_cdm; Omega_de closes the budget and must stay >= 0 for the values checked."""
         _check_parameter(name, value, (int, float), -100, 100)
-        trial = dict(self._p)
-        trial[name] = float(value)
-        if trial["omega_cdm"] + trial["omega_baryon"] + trial["omega_gamma"] > 1:
+        if np.allclose(value, getattr(self, name)):
+            return
+        if sum([float(value)] + list(others)) > 1:
             raise ValueError("Cannot satisfy the condition Omega_de >= 0")
-        if self._p[name] != float(value):
-            self._p[name] = float(value)
-            self._free_all()
+        self._p["omega_cdm"], self._p["omega_gamma"] = float(cdm), float(gamma)
+        if name == "omega_baryon":
+            self._p["omega_baryon"] = float(value)
+        self._free_all()
 
-    omega_cdm = property(lambda s: s._p["omega_cdm"], lambda s, v: s._set_omega("omega_cdm", v))
-    omega_baryon = property(lambda s: s._p["omega_baryon"], lambda s, v: s._set_omega("omega_baryon", v))
-    omega_gamma = property(lambda s: s._p["omega_gamma"], lambda s, v: s._set_omega("omega_gamma", v))
-
-    @property
-    def omega_m(self):
-        return self._p["omega_cdm"] + self._p["omega_baryon"]
-
-    @omega_m.setter
-    def omega_m(self, value):
-        _check_parameter("omega_m", value, (int, float), -100, 100)
-        self._set_omega("omega_cdm", float(value) - self._p["omega_baryon"])
+    omega_cdm = property(
+        lambda s: s._p["omega_cdm"],
+        lambda s, v: s._set_omegas("omega_cdm", v, (s.omega_baryon, s.omega_gamma), v, s.omega_gamma))
+    omega_baryon = property(
+        lambda s: s._p["omega_baryon"],
+        lambda s, v: s._set_omegas("omega_baryon", v, (s.omega_cdm, s.omega_gamma), s.omega_m - v, s.omega_gamma))
+    omega_gamma = property(
+        lambda s: s._p["omega_gamma"],
+        lambda s, v: s._set_omegas("omega_gamma", v, (s.omega_m,), s.omega_cdm, v))
+    omega_m = property(
+        lambda s: s._p["omega_cdm"] + s._p["omega_baryon"],
+        lambda s, v: s._set_omegas("omega_m", v, (s.omega_gamma,), v - s.omega_baryon, s.omega_gamma))
 
     @property
     def omega_de(self):
         return 1 - self.omega_m - self._p["omega_gamma"]
+
+    # only big Omegas here (use_little_omega is False in the reference, coffe.pyx:1014-1053)
+    use_little_omega = property(lambda s: False)
+    big_omega_m = property(lambda s: s.omega_m)
+    big_omega_de = property(lambda s: s.omega_de)
+    big_omega_baryon = property(lambda s: s.omega_baryon)
+    big_omega_cdm = property(lambda s: s.omega_cdm)
+    big_omega_gamma = property(lambda s: s.omega_gamma)
+    big_omega_nu = property(lambda s: 0.0)
+    omega_nu = property(lambda s: 0.0)
+    # no CLASS and no Cuba here (coffe.pyx:1023-1029)
+    has_class = property(lambda s: False)
+    has_cuba = property(lambda s: False)
 
     def _set_scalar(self, name, value, kind, lo, hi, frees):
         _check_parameter(name, value, kind, lo, hi)
@@ -258,6 +379,56 @@ class Coffe:
     # the bin-averaged Poisson term (covariance_window, coffe.pyx:2525-2545)
     has_binned_covariance = property(lambda s: s._p["has_binned_covariance"],
                                      lambda s, v: s._set_flag("has_binned_covariance", v))
+    verbose = property(lambda s: s._p["verbose"], lambda s, v: s._p.__setitem__("verbose", bool(v)))
+
+    # the window of P(k) the integral tables use (coffe.pyx:2895-2926)
+    k_min = property(lambda s: s._p["k_min"],
+                     lambda s, v: s._set_scalar("k_min", v, (int, float), 0, 1e-3, s._free_spectrum))
+    k_max = property(lambda s: s._p["k_max"],
+                     lambda s, v: s._set_scalar("k_max", v, (int, float), 0.1, 1e3, s._free_spectrum))
+    # the number of background samples on [0, 15] (coffe.pyx:2505-2518)
+    background_sampling = property(
+        lambda s: s._p["background_sampling"],
+        lambda s, v: s._set_scalar("background_sampling", v, int, 0, _MAX_SIZE, s._free_all))
+
+    def _set_choice(self, name, value):
+        """An enum-valued parameter, set by code or by name (coffe.pyx:236-245)."""
+        table = _CHOICES[name]
+        if value not in list(table) + list(table.values()):
+            raise ValueError(f"The value `{name}={value}` is not one of `{list(table) + list(table.values())}`")
+        code = value if value in table else {v: c for c, v in table.items()}[value]
+        if name == "pk_type" and code != 0:
+            raise NotImplementedError("only the linear power spectrum (pk_type 'linear') is computed by this "
+                                      "engine: the CLASS and nonlinear spectra are not part of it")
+        self._p[name] = code
+        self._free_results()
+
+    # The reference chooses among its CPU methods with these (coffe.pyx:2242-2325,
+    # 2438-2460); the engine computes the covariance integrals by 2D FFTLog and the table
+    # nodes by adaptive Gauss-Kronrod whatever they say.
+    pk_type = property(lambda s: _CHOICES["pk_type"][s._p["pk_type"]],
+                       lambda s, v: s._set_choice("pk_type", v))
+    covariance_integration_method = property(
+        lambda s: _CHOICES["covariance_integration_method"][s._p["covariance_integration_method"]],
+        lambda s, v: s._set_choice("covariance_integration_method", v))
+    covariance_interpolation_method = property(
+        lambda s: _CHOICES["covariance_interpolation_method"][s._p["covariance_interpolation_method"]],
+        lambda s, v: s._set_choice("covariance_interpolation_method", v))
+    integration_1d_type = property(
+        lambda s: _CHOICES["integration_1d_type"][s._p["integration_1d_type"]],
+        lambda s, v: s._set_choice("integration_1d_type", v))
+
+    @property
+    def m_ncdm(self):
+        """Mass of the non-CDM species in eV (coffe.pyx:1496-1513); only 0 here."""
+        return self._p["m_ncdm"]
+
+    @m_ncdm.setter
+    def m_ncdm(self, value):
+        _check_parameter("m_ncdm", value, (int, float), -100, 100)
+        if value != 0:
+            raise NotImplementedError("massive neutrinos (m_ncdm != 0) are not part of this engine")
+        self._p["m_ncdm"] = float(value)
 
     def __getattr__(self, name):
         if name in _UNSUPPORTED:
@@ -289,24 +460,32 @@ class Coffe:
         if len(k) < 5:
             raise ValueError("The power spectrum needs at least 5 samples")
         self._pk = (np.ascontiguousarray(k), np.ascontiguousarray(pk))
-        self._tables = None
-        self._divergent = None
-        self._ren = None
-        self._free_results()
+        self._p["k_min"], self._p["k_max"] = float(k[0]), float(k[-1])
+        self._free_spectrum()
 
-    @property
-    def k_min(self):
-        return None if self._pk is None else float(self._pk[0][0])
-
-    @property
-    def k_max(self):
-        return None if self._pk is None else float(self._pk[0][-1])
+    def _kwin(self):
+        """The window [k_min, k_max] of the integrals, inside the P(k) samples."""
+        self._need_pk()
+        k = self._pk[0]
+        kmin, kmax = float(self._p["k_min"]), float(self._p["k_max"])
+        if not (k[0] <= kmin < kmax <= k[-1]):
+            raise ValueError(f"[k_min, k_max] = [{kmin}, {kmax}] is not inside the power spectrum's samples "
+                             f"[{k[0]}, {k[-1]}]")
+        return kmin, kmax
 
     def power_spectrum(self, k, z):
         """P(k, z) = D1(z)^2 P(k) (Akima of the input samples)."""
         self._need_pk()
         _check_parameter("k", k, (int, float), self.k_min, self.k_max)
         return float(_Spline(*self._pk)(float(k))[0] * self.growth_factor(z) ** 2)
+
+    def cross_spectrum(self, k, z1, z2, approximation="geometric"):
+        """P(k) D1(z1) D1(z2) (coffe.pyx:2822-2860)."""
+        self._need_pk()
+        _check_parameter("k", k, (int, float), self.k_min, self.k_max)
+        _check_parameter("z1", z1, (int, float), 0, 15)
+        _check_parameter("z2", z2, (int, float), 0, 15)
+        return float(_Spline(*self._pk)(float(k))[0] * self.growth_factor(z1) * self.growth_factor(z2))
 
     def _set_bias(self, i, x, y):
         x = np.asarray(x, dtype=np.float64)
@@ -396,6 +575,137 @@ class Coffe:
     def evolution_bias2(self, z):
         return self._eval_mbias("fevo2", z)
 
+    # The sampled biases as to_file writes them: the spline's size evenly spaced points
+    # over its range and the spline's values there (coffe.pyx:845-945).
+    def _samples(self, s):
+        return np.linspace(float(s.x[0]), float(s.x[-1]), len(s.x))
+
+    def galaxy_bias1_redshifts(self):
+        return self._samples(self._bias[1])
+
+    def galaxy_bias1_values(self):
+        return np.array([self.galaxy_bias1(float(z)) for z in self.galaxy_bias1_redshifts()])
+
+    def galaxy_bias2_redshifts(self):
+        return self._samples(self._bias[2])
+
+    def galaxy_bias2_values(self):
+        return np.array([self.galaxy_bias2(float(z)) for z in self.galaxy_bias2_redshifts()])
+
+    def galaxy_bias3_redshifts(self):
+        return self._samples(self._bias[3])
+
+    def galaxy_bias3_values(self):
+        return np.array([self.galaxy_bias3(float(z)) for z in self.galaxy_bias3_redshifts()])
+
+    def galaxy_bias4_redshifts(self):
+        return self._samples(self._bias[4])
+
+    def galaxy_bias4_values(self):
+        return np.array([self.galaxy_bias4(float(z)) for z in self.galaxy_bias4_redshifts()])
+
+    def magnification_bias1_redshifts(self):
+        return self._samples(self._mbias["s1"])
+
+    def magnification_bias1_values(self):
+        return np.array([self.magnification_bias1(float(z)) for z in self.magnification_bias1_redshifts()])
+
+    def magnification_bias2_redshifts(self):
+        return self._samples(self._mbias["s2"])
+
+    def magnification_bias2_values(self):
+        return np.array([self.magnification_bias2(float(z)) for z in self.magnification_bias2_redshifts()])
+
+    def evolution_bias1_redshifts(self):
+        return self._samples(self._mbias["fevo1"])
+
+    def evolution_bias1_values(self):
+        return np.array([self.evolution_bias1(float(z)) for z in self.evolution_bias1_redshifts()])
+
+    def evolution_bias2_redshifts(self):
+        return self._samples(self._mbias["fevo2"])
+
+    def evolution_bias2_values(self):
+        return np.array([self.evolution_bias2(float(z)) for z in self.evolution_bias2_redshifts()])
+
+    # ---- settings files ---------------------------------------------------------------
+    def to_file(self, filename):
+        """Writes `key = value` for every parameter (booleans as 0 / 1, arrays as
+        "[a, b, ...]") and then the sampled biases (coffe.pyx:794-843)."""
+        lines = []
+        for key, value in self.parameters.items():
+            if isinstance(value, bool):
+                value = int(value)
+            elif isinstance(value, (list, tuple, np.ndarray)):
+                value = [v.item() if isinstance(v, np.generic) else v for v in value]
+            lines.append(f"{key} = {value}")
+        for kind, pops in (("galaxy", (1, 2, 3, 4)), ("magnification", (1, 2)), ("evolution", (1, 2))):
+            for i in pops:
+                name = f"{kind}_bias{i}"
+                lines.append(f"{name}_redshifts = {getattr(self, name + '_redshifts')().tolist()}")
+                lines.append(f"{name}_values = {getattr(self, name + '_values')().tolist()}")
+        Path(filename).write_text("\n".join(lines) + "\n", encoding="utf-8")
+
+    @staticmethod
+    def from_file(filename):
+        """A Coffe object with the parameters of a settings file (coffe.pyx:536-791):
+        the reference's `key = value;` files, including the older option names, which
+        are mapped to the current ones with a DeprecationWarning.  The options of the
+        reference's readers are read (also has_g2, has_only_cross_correlations,
+        covariance_populations, l_max_g / l_max_ur as integers and this engine's
+        multipole_quadrature_nodes); other keys are ignored."""
+        section = "default"
+        text = f"[{section}]\n" + "\n".join(line.strip(";") for line in Path(filename).read_text().split("\n"))
+        parser = CoffeConfigParser()
+        parser.read_string(text)
+        config = parser[section]
+        cosmo = Coffe()
+
+        def read(kind, option):
+            return getattr(config, f"get{kind}")(option)
+
+        for name, kind, replaced_by in _LEGACY:
+            if name not in config:
+                continue
+            if replaced_by is None:
+                warnings.warn(f"Option '{name}' is no longer supported and will be ignored.", DeprecationWarning)
+                continue
+            warnings.warn(f"Option '{name}' is no longer supported; it may be replaced by '{replaced_by}' "
+                          "(if found).", DeprecationWarning)
+            for target in (replaced_by if isinstance(replaced_by, list) else [replaced_by]):
+                setattr(cosmo, target, read(kind, name))
+        if "correlation_contributions" in config:
+            warnings.warn("Option 'correlation_contributions' is no longer supported; it may be replaced by one "
+                          f"of {sorted(set(_CONTRIBUTION_NAMES.values()))} (if found).", DeprecationWarning)
+            contributions = config.getstr_array("correlation_contributions")
+            for key, target in _CONTRIBUTION_NAMES.items():
+                if key in contributions:
+                    setattr(cosmo, target, True)
+        for options, kind in ((_FILE_FLOAT_ARRAYS, "float_array"), (_FILE_INT_ARRAYS, "int_array"),
+                              (_FILE_FLOATS, "float"), (_FILE_INTS, "int")):
+            for option in options:
+                if option in config:
+                    setattr(cosmo, option, read(kind, option))
+        for option in _FILE_INT_OR_STR:
+            if option in config:
+                try:
+                    value = config.getint(option)
+                except ValueError:
+                    value = config.get(option)
+                setattr(cosmo, option, value)
+        # biases: a file (read_* = 1), a constant (input_* present), or sampled values
+        for kind, pops in (("galaxy", (1, 2, 3, 4)), ("magnification", (1, 2)), ("evolution", (1, 2))):
+            for i in pops:
+                name = f"{kind}_bias{i}"
+                setter = getattr(cosmo, f"set_{name}")
+                if f"read_{name}" in config and config.getint(f"read_{name}") == 1:
+                    setter(*np.transpose(np.loadtxt(config.get(f"input_{name}").strip('" '))))
+                elif f"input_{name}" in config:
+                    setter(np.linspace(0, 10, 100), [config.getfloat(name)] * 100)
+                elif f"{name}_redshifts" in config and f"{name}_values" in config:
+                    setter(config.getfloat_array(f"{name}_redshifts"), config.getfloat_array(f"{name}_values"))
+        return cosmo
+
     def _mbias_grid(self):
         """The four magnification / evolution biases on their common redshift samples."""
         x = self._mbias["s1"].x
@@ -409,13 +719,19 @@ class Coffe:
             mz, (s1, s2, e1, e2) = self._mbias_grid()
             self._bg = _background.compute(omega_cdm=self._p["omega_cdm"], omega_baryon=self._p["omega_baryon"],
                                            omega_gamma=self._p["omega_gamma"], w0=self._p["w0"], wa=self._p["wa"],
-                                           h=self._p["h"], bias_z=mz, s1=s1, s2=s2, fevo1=e1, fevo2=e2)
+                                           h=self._p["h"], bins=self._p["background_sampling"], bias_z=mz,
+                                           s1=s1, s2=s2, fevo1=e1, fevo2=e2)
         return self._bg
 
     def _bg_eval(self, key, z):
         _check_parameter("z", z, (int, float), 0, 15)
         bg = self._background()
         return float(_Spline(bg["z"], bg[key])(float(z))[0])
+
+    def _bg_column(self, key, zs):
+        """A background table at the redshifts zs in [0, 15] (one Akima pass)."""
+        bg = self._background()
+        return _Spline(bg["z"], bg[key])(np.asarray(zs, dtype=np.float64))
 
     def comoving_distance(self, z):
         return self._bg_eval("chi", z)
@@ -433,6 +749,10 @@ class Coffe:
         return self._bg_eval("conformal_Hz", z)
 
     def hubble_rate_conformal_prime(self, z):
+        return self._bg_eval("conformal_Hz_prime", z)
+
+    def hubble_rate_conformal_derivative(self, z):
+        """d(aH)/d tau at z, in 1/Mpc^2 (coffe.pyx:3090-3110)."""
         return self._bg_eval("conformal_Hz_prime", z)
 
     def scale_factor(self, z):
@@ -455,7 +775,8 @@ class Coffe:
             from .engine import integral_table
             k, pk = self._pk
             N = self._p["integrals_fftlog_sampling"]
-            self._tables = {t: integral_table(t[0], t[1], k, pk, N) for t in TABLES}
+            kmin, kmax = self._kwin()
+            self._tables = {t: integral_table(t[0], t[1], k, pk, N, kmin, kmax) for t in TABLES}
         return self._tables
 
     def _divergent_table(self):
@@ -465,7 +786,8 @@ class Coffe:
         if self._divergent is None:
             from .engine import divergent_tables
             k, pk = self._pk
-            sep, i40, _, _, status, _ = divergent_tables(k, pk, return_status=True)
+            kmin, kmax = self._kwin()
+            sep, i40, _, _, status, _ = divergent_tables(k, pk, kmin=kmin, kmax=kmax, return_status=True)
             if np.any(status[:len(sep)] == 1):
                 raise RuntimeError("I_0^4: an integral reached the interval cap")
             self._divergent = (sep, i40)
@@ -519,7 +841,8 @@ class Coffe:
         if self._ren is None or self._ren[0] != chi_max:
             k, pk = self._pk
             chi = np.array([j / 200 * chi_max for j in range(201)])
-            _, _, _, ren = divergent_tables(k, pk, sep=np.zeros(0), chi=chi)
+            kmin, kmax = self._kwin()
+            _, _, _, ren = divergent_tables(k, pk, sep=np.zeros(0), chi=chi, kmin=kmin, kmax=kmax)
             self._ren = (chi_max, chi, ren)
         return self._ren[1], self._ren[2]
 
@@ -580,7 +903,9 @@ class Coffe:
         reference counterpart): the exact tangents of the background tables
         (coffe_background_jvp) through the engine's forward-mode weights
         (coffe_mu_weights_jvp), full or flat sky, density / RSD / Doppler-1, at fixed P(k)
-        and separations.  Returns an array in the records' order (z, l, r)."""
+        and separations.  The six parameters are independent coordinates here: a step in
+        omega_baryon holds omega_cdm (the omega_baryon setter holds omega_m instead).
+        Returns an array in the records' order (z, l, r)."""
         from . import background as _bgm
         from .engine import mu_weights_jvp
         self._check_coords_multipoles()
@@ -770,7 +1095,8 @@ class Coffe:
         n2 = np.asarray(self._p["number_density2"], dtype=np.float64)
         if np.any(n1 <= 0) or np.any(n2 <= 0):
             raise ValueError("number_density1 and number_density2 must be positive for the covariance")
-        plan = CovPlan(self._engine_handle(), self._p["covariance_integration_sampling"], k[0], k[-1], ls, seps)
+        kmin, kmax = self._kwin()
+        plan = CovPlan(self._engine_handle(), self._p["covariance_integration_sampling"], kmin, kmax, ls, seps)
         try:
             integ = plan.integrals(torch.tensor(k, device="cuda"), torch.tensor(pk, device="cuda"), [len(k)])
             # galaxy biases 1..4 only with a density contribution, f only with RSD
@@ -858,3 +1184,14 @@ class Coffe:
                 for iz, (a, b) in enumerate(zip(zmin, zmax)) for i in range(len(ls)) for m in range(len(seps))
                 for j in range(len(ls)) for n in range(len(seps))])
         return self._average_covariance
+
+
+def _kept_property(name, kind, lo, hi):
+    def setter(self, value):
+        _check_parameter(name, value, kind, lo, hi)
+        self._p[name] = value
+    return property(lambda s: s._p[name], setter)
+
+
+for _name, (_kind, _lo, _hi) in _KEPT.items():
+    setattr(Coffe, _name, _kept_property(_name, _kind, _lo, _hi))

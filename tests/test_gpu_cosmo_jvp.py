@@ -22,9 +22,16 @@ def _pk():
 
 
 def _cosmo(flat, **theta):
+    """A Coffe object at theta = (omega_cdm, omega_baryon, ...).  Its omega_baryon setter
+    holds omega_m (coffe.pyx:1229-1236), so omega_baryon goes first and omega_m = cdm + b
+    after it, which leaves omega_cdm at theta's value."""
     from coffe_amd.coffe import Coffe
+    rest = dict(theta)
+    cdm, baryon = rest.pop("omega_cdm"), rest.pop("omega_baryon")
+    theta = dict(omega_baryon=baryon, omega_m=cdm + baryon, **rest)
     c = Coffe(sep=np.array([15.0, 45.0, 110.0, 220.0]), l=[0, 1, 2, 3, 4], z_mean=[0.6, 1.4], has_density=True,
               has_rsd=True, has_d1=True, has_flatsky_local=flat, integrals_fftlog_sampling=4096, **theta)
+    assert c.omega_cdm == pytest.approx(cdm, rel=1e-15, abs=1e-15)
     c.set_power_spectrum_linear(*_pk())
     zb = np.linspace(0, 15, 16)
     c.set_galaxy_bias1(zb, 1.1 + 0.4 * zb)
