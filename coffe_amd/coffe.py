@@ -404,8 +404,10 @@ class Coffe:
         self._free_results()
 
     # The reference chooses among its CPU methods with these (coffe.pyx:2242-2325,
-    # 2438-2460); the engine computes the covariance integrals by 2D FFTLog and the table
-    # nodes by adaptive Gauss-Kronrod whatever they say.
+    # 2438-2460).  The engine computes the covariance integrals by 2D FFTLog for either
+    # method (the reference's tests hold its default, 'standard', to the same fixtures at
+    # rtol 5e-4) and the table nodes by adaptive Gauss-Kronrod for either 1D type; the
+    # covariance is interpolated bicubically only ('bilinear' raises at compute time).
     pk_type = property(lambda s: _CHOICES["pk_type"][s._p["pk_type"]],
                        lambda s, v: s._set_choice("pk_type", v))
     covariance_integration_method = property(
@@ -1089,6 +1091,9 @@ class Coffe:
         zs[bin] and the volumes vols[bin]."""
         import torch
         from .covariance import CovPlan
+        if self._p["covariance_interpolation_method"] != 2:
+            raise NotImplementedError("the covariance is interpolated bicubically here "
+                                      "(covariance_interpolation_method 'bicubic')")
         k, pk = self._pk
         ls, seps = [int(v) for v in self._p["l"]], self._p["sep"]
         n1 = np.asarray(self._p["number_density1"], dtype=np.float64)

@@ -797,7 +797,10 @@ int coffe_covariance_init(coffe_parameters_t *par, coffe_background_t *bg, coffe
     if (!bg || !cov_mp || !cov_ramp) return done(COFFE_EINVAL);
     coffe_covariance_free(cov_mp);
     coffe_average_covariance_free(cov_ramp);
-    if (par->covariance_integration_method != COFFE_REF_COVARIANCE_FFTLOG ||
+    /* One method here: the STANDARD one (the reference's default) is computed by FFTLog
+     * too; the reference's tests hold both to the same fixtures (rtol 5e-4). */
+    if ((par->covariance_integration_method != COFFE_REF_COVARIANCE_FFTLOG &&
+         par->covariance_integration_method != COFFE_REF_COVARIANCE_STANDARD) ||
         par->covariance_interpolation_method != COFFE_REF_INTERP2D_BICUBIC || par->pk_type != 0)
         return done(COFFE_EUNSUPPORTED);
     const size_t nbin = avg ? par->zmin_len : par->z_mean_len;

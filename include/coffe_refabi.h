@@ -316,7 +316,8 @@ int coffe_average_multipoles_free(coffe_average_multipoles_array_t *ramp);
 
 /*
  * coffe_covariance_init (covariance.h:48-53, covariance.c:477-1103), the FFTLog method
- * (covariance_integration_method = 2, bicubic interpolation): the Gaussian covariance of
+ * (covariance_integration_method 2, or 1, the STANDARD quadrature, which is computed by
+ * FFTLog as well; bicubic interpolation): the Gaussian covariance of
  * the multipoles per z_mean bin (COVARIANCE_MULTIPOLES, records z, l1, r1, l2, r2) or
  * per [zmin, zmax] bin (COVARIANCE_AVERAGE_MULTIPOLES: z_mean = (zmin + zmax)/2 and the
  * volume 4 pi fsky / (int dz / (H chi^2 (1 + z)) (H0 h)^2), covariance.c:808-837), with
@@ -324,7 +325,7 @@ int coffe_average_multipoles_free(coffe_average_multipoles_array_t *ramp);
  * and P^2 run on the GPU (coffe_cov_integrals over power_spectrum_norm, k_min_norm,
  * k_max_norm, covariance_integration_bins points); the biases, f and D1 at each bin
  * come from the host's splines.  Other output types leave both arrays untouched; the
- * STANDARD method, bilinear interpolation and CLASS spectra return COFFE_EUNSUPPORTED.
+ * bilinear interpolation and CLASS spectra return COFFE_EUNSUPPORTED.
  */
 int coffe_covariance_init(coffe_parameters_t *par, coffe_background_t *bg, coffe_covariance_array_t *cov_mp,
                           coffe_average_covariance_array_t *cov_ramp);

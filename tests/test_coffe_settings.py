@@ -163,3 +163,11 @@ def test_omega_bookkeeping():
     assert c.omega_nu == 0.0 and not c.use_little_omega and not c.has_class
     with pytest.raises(ValueError):
         c.omega_gamma = 0.8
+
+
+def test_bilinear_covariance_is_refused():
+    c = Coffe(covariance_interpolation_method="bilinear")
+    k = np.logspace(-4, 1, 50)
+    c.set_power_spectrum_linear(k, k ** -1)
+    with pytest.raises(NotImplementedError):
+        c.compute_covariance_bulk()

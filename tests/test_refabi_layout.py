@@ -147,7 +147,7 @@ def test_integrate_flag_gating(lib):
         par.b_derivative = 1
         assert np.isnan(lib.coffe_integrate(ctypes.byref(par), *args, R.NONINTEGRATED, R.MULTIPOLES))
         assert lib.coffe_engine_last_error() == EUNSUPPORTED
-        # lensing in a multipoles run, the flat-sky nonlocal tables, the STANDARD covariance
+        # lensing in a multipoles run, the flat-sky nonlocal tables, bilinear covariance interpolation
         par.b_derivative = 0
         par.correlation_contrib.g4 = 1
         z = np.array([1.0])
@@ -164,7 +164,8 @@ def test_integrate_flag_gating(lib):
         assert lib.coffe_integrals_init(ctypes.byref(par), None, ctypes.byref(ia)) == EUNSUPPORTED
         assert ia.size == 0 and not ia.array
         par.output_type = R.COVARIANCE_MULTIPOLES
-        par.covariance_integration_method = 1
+        par.covariance_integration_method = 1        # STANDARD: computed by FFTLog, accepted
+        par.covariance_interpolation_method = 1      # bilinear: not offered
         cm, ca = R.CovarianceArray(), R.AverageCovarianceArray()
         assert lib.coffe_covariance_init(ctypes.byref(par), ctypes.byref(R.Background()), ctypes.byref(cm),
                                          ctypes.byref(ca)) == EUNSUPPORTED
