@@ -10,8 +10,10 @@ computed pieces are dropped when a parameter they depend on changes
 (coffe.pyx:948-993), and results come back as Multipoles / Covariance / ... records in
 the reference's order (multipoles.c:76-86: z, l, r; covariance.c:1046-1060: z, l1, r1,
 l2, r2).  The terms are density, RSD, Doppler-1, d2 and g1-g3 with their cross terms.
-Lensing, g4, g5, the non-local flat-sky switches, massive neutrinos and the CLASS /
-nonlinear spectra are not part of this engine and raise NotImplementedError; the
+With has_lensing the flat-sky lensing-lensing (has_flatsky_nonlocal) and
+density-lensing (has_flatsky_local_nonlocal) multipoles are added; the full-sky
+lensing integrals, g4, g5, massive neutrinos and the CLASS / nonlinear spectra are not
+part of this engine and raise NotImplementedError; the
 parameters of the CLASS interface and of the reference's CPU integrators are kept and
 written to settings files but change nothing here.
 """
@@ -25,7 +27,7 @@ from . import background as _background
 from .representation import AverageCovariance, AverageMultipoles, Corrfunc, Covariance, Multipoles
 
 TABLES = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]   # integrals.c:599-608
-_UNSUPPORTED = ("has_lensing", "has_g4", "has_g5", "has_flatsky_nonlocal", "has_flatsky_local_nonlocal")
+_UNSUPPORTED = ("has_g4", "has_g5")
 _DG = ("d2", "g1", "g2", "g3")
 COFFE_H0 = 1 / 2997.92458   # h/Mpc (common.h)
 
@@ -151,6 +153,7 @@ class Coffe:
         number_density1=[1e-3], number_density2=[1e-3],
         has_density=True, has_rsd=False, has_d1=False, has_d2=False, has_g1=False, has_g2=False, has_g3=False,
         has_flatsky_local=False, has_only_cross_correlations=False,
+        has_lensing=False, has_flatsky_nonlocal=False, has_flatsky_local_nonlocal=False,
         integrals_fftlog_sampling=10000, covariance_integration_sampling=8000,
         covariance_cosmic=True, covariance_mixed=True, covariance_poisson=True,
         has_binned_covariance=False, covariance_populations=[1, 1, 1, 1],
@@ -182,6 +185,7 @@ class Coffe:
     # ---- lazily computed state -------------------------------------------------
     def _free_all(self):
         self._bg = None
+        self._lens = None
         self._tables = None
         self._divergent = None
         self._ren = None
@@ -189,6 +193,7 @@ class Coffe:
 
     def _free_spectrum(self):
         """Drops what depends on P(k) or its window."""
+        self._lens = None
         self._tables = None
         self._divergent = None
         self._ren = None
@@ -369,6 +374,15 @@ class Coffe:
     has_g3 = property(lambda s: s._p["has_g3"], lambda s, v: s._set_flag("has_g3", v))
     has_flatsky_local = property(lambda s: s._p["has_flatsky_local"],
                                  lambda s, v: s._set_flag("has_flatsky_local", v))
+    # lensing (signal.c:556-771): the flat-sky lensing-lensing term with
+    # has_flatsky_nonlocal and the flat-sky density-lensing term with
+    # has_flatsky_local_nonlocal are computed; the full-sky lensing integrals (Monte
+    # Carlo in the reference) raise NotImplementedError
+    has_lensing = property(lambda s: s._p["has_lensing"], lambda s, v: s._set_flag("has_lensing", v))
+    has_flatsky_nonlocal = property(lambda s: s._p["has_flatsky_nonlocal"],
+                                    lambda s, v: s._set_flag("has_flatsky_nonlocal", v))
+    has_flatsky_local_nonlocal = property(lambda s: s._p["has_flatsky_local_nonlocal"],
+                                          lambda s, v: s._set_flag("has_flatsky_local_nonlocal", v))
     # only the cross terms of the contributions (coffe.pyx:1121-1133, functions.c:129-669)
     has_only_cross_correlations = property(lambda s: s._p["has_only_cross_correlations"],
                                            lambda s, v: s._set_flag("has_only_cross_correlations", v))
@@ -816,9 +830,15 @@ class Coffe:
             raise ValueError("At least one of the following parameters is empty: l, sep, z_mean")
 
     def _check_contributions(self):
-        if not any(self._p[k] for k in ("has_density", "has_rsd", "has_d1", "has_d2", "has_g1", "has_g2", "has_g3")):
+        if not any(self._p[k] for k in ("has_density", "has_rsd", "has_d1", "has_d2", "has_g1", "has_g2", "has_g3",
+                                        "has_lensing")):
             raise ValueError("No contributions specified, you need to specify at least one of: "
-                             "'has_density', 'has_rsd', 'has_d1', 'has_d2', 'has_g1', 'has_g2', 'has_g3'")
+                             "'has_density', 'has_rsd', 'has_d1', 'has_d2', 'has_g1', 'has_g2', 'has_g3', "
+                             "'has_lensing'")
+
+    def _no_lensing(self, what):
+        if self._p["has_lensing"]:
+            raise NotImplementedError(f"lensing in {what} is not built (the reference integrates it by Monte Carlo)")
 
     def _dg_sources(self):
         return {k for k in _DG if self._p["has_" + k]}
@@ -912,6 +932,7 @@ class Coffe:
         from .engine import mu_weights_jvp
         self._check_coords_multipoles()
         self._check_contributions()
+        self._no_lensing("the cosmological tangent")
         if self._dg_sources():
             raise NotImplementedError("the cosmological tangent of the d2 / g1-g3 terms is not built")
         bz = self._bias[1].x
@@ -937,6 +958,74 @@ class Coffe:
                 out[:, l0:l0 + len(lc)] = np.einsum("zltr,tr->zlr", At.cpu().numpy(), tv)
         return out.reshape(-1)
 
+    def _lensing_tables(self, ls, want_ll, want_dl):
+        """The (1, l) tables of the lensing-lensing term (integrals.c:1082-1130) and the
+        half-integer tables ((l - 2k + 3)/2, (l - 2k + 1)/2) of the density-lensing term
+        (integrals.c:1246-1300), for the even multipoles; cached."""
+        from .engine import FL_MAXK, half_integer_table, integral_table
+        key = (tuple(ls), bool(want_ll), bool(want_dl))
+        if self._lens is None or self._lens[0] != key:
+            self._need_pk()
+            k, pk = self._pk
+            N = self._p["integrals_fftlog_sampling"]
+            kmin, kmax = self._kwin()
+            even = sorted({int(v) for v in ls if int(v) % 2 == 0})
+            if want_ll and even and even[-1] > 4:
+                raise ValueError("the flat-sky lensing-lensing multipoles go up to l = 4 here")
+            if want_dl and even and even[-1] > 2 * (FL_MAXK - 1):
+                raise ValueError(f"the flat-sky density-lensing multipoles go up to l = {2 * (FL_MAXK - 1)} here")
+            ll = {l: integral_table(1, l, k, pk, N, kmin, kmax) for l in even} if want_ll else {}
+            dl = ({(l, j): half_integer_table(l - 2 * j + 3, l - 2 * j + 1, k, pk, N, kmin, kmax)
+                   for l in even for j in range(l // 2 + 1)} if want_dl else {})
+            self._lens = (key, ll, dl)
+        return self._lens[1], self._lens[2]
+
+    def _lensing_plan(self):
+        """(want lensing-lensing, want density-lensing), or None without has_lensing
+        (signal.c:612-628, 704-736): lensing-lensing unless has_only_cross_correlations
+        (it is an auto term), needing has_flatsky_nonlocal; with density, RSD or
+        Doppler-1 on, their cross terms with lensing, needing has_flatsky_local_nonlocal,
+        of which only density-lensing is nonzero in flat sky.  The full-sky integrals
+        raise NotImplementedError."""
+        if not self._p["has_lensing"]:
+            return None
+        if self._dg_sources():
+            raise NotImplementedError("lensing with d2 / g1-g3 (single-integrated terms, Monte Carlo in the "
+                                      "reference) is not built")
+        want_ll = not self._p["has_only_cross_correlations"]
+        local = any(self._p[k] for k in ("has_density", "has_rsd", "has_d1"))
+        if want_ll and not self._p["has_flatsky_nonlocal"]:
+            raise NotImplementedError("full-sky lensing-lensing (double-integrated, Monte Carlo in the reference) "
+                                      "is not built: set has_flatsky_nonlocal")
+        if local and not self._p["has_flatsky_local_nonlocal"]:
+            raise NotImplementedError("full-sky density/RSD/Doppler-lensing (single-integrated, Monte Carlo in the "
+                                      "reference) is not built: set has_flatsky_local_nonlocal")
+        return want_ll, bool(self._p["has_density"])
+
+    def _lensing_values(self, zs, seps, ls):
+        """[z][l][r] flat-sky lensing multipoles on the GPU (coffe_flat_lensing_multipoles),
+        or None without has_lensing."""
+        from .engine import flat_lensing
+        plan = self._lensing_plan()
+        if plan is None:
+            return None
+        want_ll, want_dl = plan
+        if not (want_ll or want_dl):
+            return np.zeros((len(zs), len(ls), len(seps)))
+        ll, dl = self._lensing_tables(ls, want_ll, want_dl)
+        zpar = [[float(z), self.comoving_distance(float(z)), self.growth_factor(float(z)),
+                 self._eval_mbias("s1", float(z)), self._eval_mbias("s2", float(z)),
+                 self._eval_bias(1, float(z)), self._eval_bias(2, float(z))] for z in zs]
+        return flat_lensing(zpar, [int(v) for v in ls], seps, self._background(), ll, dl, self.omega_m,
+                            COFFE_H0 * self._p["h"], want_ll, want_dl)
+
+    def _all_multipoles(self, zs, seps, ls):
+        """The local terms and, with has_lensing, the flat-sky lensing terms."""
+        self._lensing_plan()   # refuses what is not built before any device work
+        vals = self._multipole_values(zs, seps, ls)
+        lens = self._lensing_values(zs, seps, ls)
+        return vals if lens is None else vals + lens
+
     def compute_multipole(self, *, z, r, l, recompute=False):  # noqa: E741
         """The multipole at (z_mean, r, l) (coffe.pyx:3232-3300)."""
         self._check_coords_multipoles()
@@ -946,7 +1035,7 @@ class Coffe:
         _check_parameter("l", l, int, 0, 10)
         if recompute:
             self._free_all()
-        return float(self._multipole_values([float(z)], [float(r)], [int(l)])[0, 0, 0])
+        return float(self._all_multipoles([float(z)], [float(r)], [int(l)])[0, 0, 0])
 
     def compute_multipoles_bulk(self, recompute=False):
         """Multipoles at every (z_mean, l, sep), z-major then l then r (multipoles.c:76-86)."""
@@ -956,7 +1045,7 @@ class Coffe:
             self._free_all()
         if self._multipoles is None:
             zs, ls, seps = self._p["z_mean"], self._p["l"], self._p["sep"]
-            vals = self._multipole_values(zs, seps, ls)
+            vals = self._all_multipoles(zs, seps, ls)
             self._multipoles = np.array([Multipoles(z=float(z), r=float(r), l=int(li), value=float(vals[iz, il, ir]))
                                          for iz, z in enumerate(zs) for il, li in enumerate(ls)
                                          for ir, r in enumerate(seps)])
@@ -979,6 +1068,7 @@ class Coffe:
         """The correlation function at (z_mean, r, mu) (coffe.pyx:3159-3230)."""
         self._check_coords_corrfunc()
         self._check_contributions()
+        self._no_lensing("the correlation function")
         _check_parameter("z", z, (int, float), 0, 15)
         _check_parameter("r", r, (int, float), 0, 1500)
         _check_parameter("mu", mu, (int, float), -1, 1)
@@ -991,6 +1081,7 @@ class Coffe:
         (corrfunc.c:83-90)."""
         self._check_coords_corrfunc()
         self._check_contributions()
+        self._no_lensing("the correlation function")
         if recompute:
             self._free_all()
         if self._corrfunc is None:
@@ -1071,6 +1162,7 @@ class Coffe:
         l then r (average_multipoles.c:90-146)."""
         self._check_coords_average_multipoles()
         self._check_contributions()
+        self._no_lensing("the redshift-averaged multipoles")
         if recompute:
             self._free_all()
         if self._average_multipoles is None:

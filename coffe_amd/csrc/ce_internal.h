@@ -15,6 +15,9 @@ extern "C" {
 #define CE_MAX_PASSES 16
 #define CE_MAX_TABLES 8
 #define CE_MIN_SEP_LEN 42
+/* half-integer tables per multipole of the flat-sky density-lensing term: k <= l/2,
+   so multipoles up to l = 2 (CE_FL_MAXK - 1) */
+#define CE_FL_MAXK 5
 /* intervals an adaptive QAG keeps in LDS (the reference allows 50000; the small-r
    integrands converge within a few hundred) */
 #define CE_QAG_CAP 1024
@@ -94,6 +97,12 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
    pack (0: lds_data_len(max_M)) */
 /* d_ranges (4 ints per chunk id, may be NULL): the resample index ranges of the
    chunk's owned and staged intervals, written for ce_launch_spline_adj */
+/* flat-sky lensing multipoles (lensing_kernels.hip), device pointers */
+int ce_launch_flat_lensing(void *stream, int nz, int nl, int nsep, int nb, int ngl, int want_ll, int want_dl,
+                           const double *d_zpar, const int *d_ls, const double *d_sep, const double *d_bg_z,
+                           const double *d_bg_chi, const double *d_bg_D1, const double *d_tab_x, const double *d_tab_y,
+                           const int64_t *d_ll_off, const int *d_ll_len, const int64_t *d_dl_off, const int *d_dl_len,
+                           const double *d_glx, const double *d_glw, double Omega_m, double H0h, double *d_out);
 /* complex FFTs of any length on device memory (anyfft.hip): forward, unnormalised,
    in place on d_x (L complex values); direct LDS lines up to 8192 points, four-step for
    longer 2^a 3^b 5^c lengths, Bluestein otherwise */

@@ -1596,3 +1596,101 @@ int coffe_integral_table(int n, int l, size_t N, const double *k, const double *
     for (size_t i = 0; i < CE_MIN_SEP_LEN; ++i) out_x[1 + i] = min_sep[i];
     return COFFE_OK;
 }
+
+/* ---- flat-sky lensing multipoles ------------------------------------------------ */
+
+static int flat_table_ok(int64_t off, int len, int64_t ntab_values)
+{
+    return len >= 5 && off >= 0 && off + len <= ntab_values;
+}
+
+int coffe_flat_lensing_multipoles(int nz, const double *zpar, int nl, const int *ls, int nsep, const double *sep,
+                                  int nb, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                                  int64_t ntab_values, const double *tab_x, const double *tab_y, const int64_t *ll_off,
+                                  const int *ll_len, const int64_t *dl_off, const int *dl_len, double Omega_m,
+                                  double H0h, int want_ll, int want_dl, int panels, double *out)
+{
+    if (nz <= 0 || nl <= 0 || nsep <= 0 || nb < 5 || !zpar || !ls || !sep || !bg_z || !bg_chi || !bg_D1 || !out ||
+        !ll_off || !ll_len || !dl_off || !dl_len || panels < 1 || panels > 4096 || ntab_values < 0 ||
+        (ntab_values > 0 && (!tab_x || !tab_y)) || (int64_t)nz * nl * nsep > 0x7fffffff)
+        return COFFE_EINVAL;
+    /* every table a wanted term reads must be there; the others must be empty or whole */
+    for (int il = 0; il < nl; ++il) {
+        const int l = ls[il];
+        if (l < 0) return COFFE_EINVAL;
+        const int even = (l % 2) == 0;
+        if (want_ll && even && !flat_table_ok(ll_off[il], ll_len[il], ntab_values)) return COFFE_EINVAL;
+        if (ll_len[il] != 0 && !flat_table_ok(ll_off[il], ll_len[il], ntab_values)) return COFFE_EINVAL;
+        if (want_dl && even && l > 2 * (CE_FL_MAXK - 1)) return COFFE_EINVAL;
+        for (int k = 0; k < CE_FL_MAXK; ++k) {
+            const int t = il * CE_FL_MAXK + k;
+            if (want_dl && even && k <= l / 2 && !flat_table_ok(dl_off[t], dl_len[t], ntab_values))
+                return COFFE_EINVAL;
+            if (dl_len[t] != 0 && !flat_table_ok(dl_off[t], dl_len[t], ntab_values)) return COFFE_EINVAL;
+        }
+    }
+    coffe_engine *e = coffe_engine_default();
+    if (!e) return COFFE_ENODEVICE;
+    hipSetDevice(e->device);
+    /* the x rule: panels of 32 Gauss-Legendre nodes on [0, 1] */
+    const int ngl = 32 * panels;
+    double t32[64];
+    gauss_legendre(32, t32, t32 + 32);
+    double *gl = (double *)malloc(sizeof(double) * 2 * (size_t)ngl);
+    if (!gl) return COFFE_ENOMEM;
+    for (int p = 0; p < panels; ++p) {
+        const double a = (double)p / panels, b = (double)(p + 1) / panels;
+        for (int i = 0; i < 32; ++i) {
+            gl[32 * p + i] = a + 0.5 * (b - a) * (t32[i] + 1.0);
+            gl[ngl + 32 * p + i] = 0.5 * (b - a) * t32[32 + i];
+        }
+    }
+    /* one device buffer, 8-byte slots */
+    const size_t nzp = 7 * (size_t)nz, nd = (size_t)nl * CE_FL_MAXK, nout = (size_t)nz * nl * nsep;
+    const size_t o_zpar = 0, o_ls = o_zpar + nzp, o_sep = o_ls + (size_t)nl, o_bz = o_sep + (size_t)nsep,
+                 o_bc = o_bz + (size_t)nb, o_bd = o_bc + (size_t)nb, o_tx = o_bd + (size_t)nb,
+                 o_ty = o_tx + (size_t)ntab_values, o_llo = o_ty + (size_t)ntab_values, o_lll = o_llo + (size_t)nl,
+                 o_dlo = o_lll + (size_t)nl, o_dll = o_dlo + nd, o_gl = o_dll + nd, o_out = o_gl + 2 * (size_t)ngl,
+                 total = o_out + nout;
+    int64_t *hbuf = (int64_t *)calloc(2 * (size_t)nl + 2 * nd + (size_t)nl, sizeof(int64_t));
+    if (!hbuf) { free(gl); return COFFE_ENOMEM; }
+    int64_t *h_ls = hbuf, *h_lll = hbuf + nl, *h_dll = hbuf + 2 * (size_t)nl;
+    /* the int arrays go to 8-byte slots as int pairs: pack them as int32 at the slot start */
+    int32_t *ls32 = (int32_t *)h_ls, *lll32 = (int32_t *)h_lll, *dll32 = (int32_t *)h_dll;
+    for (int il = 0; il < nl; ++il) {
+        ls32[il] = ls[il];
+        lll32[il] = ll_len[il];
+    }
+    for (size_t t = 0; t < nd; ++t) dll32[t] = dl_len[t];
+    double *d = NULL;
+    int st = dev_alloc((void **)&d, sizeof(double) * total);
+    hipStream_t s = e->stream;
+#define FL_PUT(off, src, bytes) \
+    if (st == COFFE_OK && (bytes) > 0) st = hip_ok(hipMemcpyAsync(d + (off), (src), (bytes), hipMemcpyHostToDevice, s))
+    FL_PUT(o_zpar, zpar, sizeof(double) * nzp);
+    FL_PUT(o_ls, ls32, sizeof(int32_t) * (size_t)nl);
+    FL_PUT(o_sep, sep, sizeof(double) * (size_t)nsep);
+    FL_PUT(o_bz, bg_z, sizeof(double) * (size_t)nb);
+    FL_PUT(o_bc, bg_chi, sizeof(double) * (size_t)nb);
+    FL_PUT(o_bd, bg_D1, sizeof(double) * (size_t)nb);
+    FL_PUT(o_tx, tab_x, sizeof(double) * (size_t)ntab_values);
+    FL_PUT(o_ty, tab_y, sizeof(double) * (size_t)ntab_values);
+    FL_PUT(o_llo, ll_off, sizeof(int64_t) * (size_t)nl);
+    FL_PUT(o_lll, lll32, sizeof(int32_t) * (size_t)nl);
+    FL_PUT(o_dlo, dl_off, sizeof(int64_t) * nd);
+    FL_PUT(o_dll, dll32, sizeof(int32_t) * nd);
+    FL_PUT(o_gl, gl, sizeof(double) * 2 * (size_t)ngl);
+#undef FL_PUT
+    if (st == COFFE_OK &&
+        ce_launch_flat_lensing(s, nz, nl, nsep, nb, ngl, want_ll != 0, want_dl != 0, d + o_zpar,
+                               (const int *)(d + o_ls), d + o_sep, d + o_bz, d + o_bc, d + o_bd, d + o_tx, d + o_ty,
+                               (const int64_t *)(d + o_llo), (const int *)(d + o_lll), (const int64_t *)(d + o_dlo),
+                               (const int *)(d + o_dll), d + o_gl, d + o_gl + ngl, Omega_m, H0h, d + o_out))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(out, d + o_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    hipFree(d);
+    free(hbuf);
+    free(gl);
+    return st;
+}

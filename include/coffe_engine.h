@@ -348,6 +348,34 @@ int coffe_batch_small_r(coffe_batch *b, const double *d_k, const double *d_pk, d
  */
 int coffe_fft_any(const double *in, double *out, int64_t L);
 
+/*
+ * Flat-sky lensing multipoles, the two lensing terms the reference computes without
+ * Monte Carlo when its flat-sky switches are on (signal.c:612-628, 704-736):
+ * lensing-lensing (functions.c:4020-4133, flatsky_nonlocal; the x integral over
+ * [0, 1] by a composite Gauss-Legendre rule of `panels` panels of 32 nodes in place
+ * of the reference's QAG) and density-lensing (functions.c:4136-4257,
+ * flatsky_local_nonlocal), each already multiplied by 2 pi^2 as signal.c adds them,
+ * summed into out[nz][nl][nsep] when want_ll / want_dl is set.  Odd multipoles give 0.
+ *   zpar[nz][7]   z_mean, chi(z_mean), D1(z_mean), s1, s2 (magnification biases) and
+ *                 b1, b2 (galaxy biases) at z_mean;
+ *   ls[nl], sep[nsep] (Mpc); bg_z, bg_chi, bg_D1[nb] the background samples (chi
+ *                 increasing: the reference's z_as_chi spline is Akima of z against chi);
+ *   tab_x, tab_y  the tables, concatenated: the (n, l) = (1, l) table of every even
+ *                 multipole at ll_off[il] (length ll_len[il], 0 for none) and the
+ *                 half-integer tables ((l - 2k + 3)/2, (l - 2k + 1)/2) at
+ *                 dl_off[il * 5 + k] (lengths dl_len, k <= l/2), as
+ *                 coffe_integral_table / twofast_1bessel give them (r^(n-l) I_l^n for
+ *                 n >= l), at least 5 nodes each;
+ *   Omega_m, H0h  Omega_cdm + Omega_baryon and H0 h in 1/Mpc.
+ * Multipoles up to 8 with want_dl.  Synchronous on the default engine; COFFE_EINVAL
+ * for bad sizes, NaN where a lookup falls outside its table.
+ */
+int coffe_flat_lensing_multipoles(int nz, const double *zpar, int nl, const int *ls, int nsep, const double *sep,
+                                  int nb, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                                  int64_t ntab_values, const double *tab_x, const double *tab_y, const int64_t *ll_off,
+                                  const int *ll_len, const int64_t *dl_off, const int *dl_len, double Omega_m,
+                                  double H0h, int want_ll, int want_dl, int panels, double *out);
+
 /* synchronise the engine's stream */
 int coffe_engine_synchronize(coffe_engine *e);
 

@@ -1,0 +1,77 @@
+"""CPU restatement of the reference's flat-sky lensing multipoles (test
+infrastructure only: tests and the bench's CPU baseline use it as the checker).
+
+  flat_lensing_lensing  functions.c:4020-4133 (the integrand) with signal.c:704-736
+                        (2 pi^2 times its integral over x in [0, 1]), where the
+                        reference runs GSL QAG to integration_1d_prec = 1e-5; here by
+                        brute force, 2048 panels of 32 Gauss-Legendre nodes: the
+                        Akima lookups make the integrand piecewise cubic, which leaves
+                        adaptive error estimates unreliable below ~1e-9, while the
+                        panel rule converges (1024 and 2048 panels agree to ~1e-12)
+  flat_density_lensing  functions.c:4136-4257 with signal.c:612-628 (2 pi^2 times the
+                        closed form over the half-integer tables)
+  half_table            integrals.c:1246-1300 with 459-573: the FFT nodes of the
+                        ((l - 2k + 3)/2, (l - 2k + 1)/2) table, r^(n - l) I_l^n(r)
+
+Every lookup is the oracle's Akima (GSL non-rounded ends), as the reference's splines.
+Pinned by the reference's fixtures tests/benchmarks/benchmark_flatsky_{lensing_lensing,
+density_lensing}_multipoles{0,2,4}.dat (the reference's own tests hold them to rtol
+5e-4), which tests/test_oracle_lensing.py checks.
+"""
+from math import comb, factorial, pi, sqrt
+
+import numpy as np
+
+from . import oracle as O
+
+
+def half_table(n2, l2, k, pk, N, kmin=None, kmax=None):
+    """FFT nodes of the table (n, l) = (n2/2, l2/2): twofast_1bessel with l and nu
+    half-integers, times r^(n - l) when n >= l (integrals.c:496-522)."""
+    x, y = O.twofast_1bessel(k, pk, N, l2 / 2.0, n2 / 2.0, 1.0, None, kmin, kmax)
+    if n2 >= l2:
+        y = y * x ** ((n2 - l2) / 2.0)
+    return x, y
+
+
+def _lookup(table, v):
+    x, y = table
+    return float(O.akima(x, y, [v])[0])
+
+
+def flat_lensing_lensing(table_1l, bg, z_mean, l, sep, s1, s2, Omega_m, H0h, panels=2048):
+    """2 pi^2 int_0^1 F(x) dx of functions.c:4020-4133 at one (z_mean, l, sep); table_1l
+    is the (1, l) table (x, y); bg a dict of the background samples z, chi, D1; s1, s2
+    the magnification biases at z_mean.  Odd l: 0 (signal.c:712)."""
+    if l % 2:
+        return 0.0
+    chi = _lookup((bg["z"], bg["chi"]), z_mean)
+    t, w = np.polynomial.legendre.leggauss(32)
+    edges = np.linspace(0.0, 1.0, panels + 1)
+    a, b = edges[:-1, None], edges[1:, None]
+    x = (a + 0.5 * (b - a) * (t + 1)).ravel()
+    wx = (0.5 * (b - a) * w).ravel()
+    zl = O.akima(bg["chi"], bg["z"], chi * x)           # z_as_chi (background.c:611-617)
+    D = O.akima(bg["z"], bg["D1"], zl)
+    tv = O.akima(table_1l[0], table_1l[1], x * sep)
+    if l > 0:
+        tv = tv * x * sep
+    val = float(np.sum(wx * tv * D * D * (1 + zl) ** 2 * (x * (1 - x)) ** 2))
+    pre = ((2 * l + 1) * factorial(l) / 2 ** l / factorial(l // 2) ** 2 * 9 * Omega_m ** 2 * (2 - 5 * s1) *
+           (2 - 5 * s2) * chi ** 3 / 8 / pi * H0h ** 4)
+    return 2 * pi * pi * pre * val
+
+
+def flat_density_lensing(tables_k, bg, z_mean, l, sep, s1, s2, b1, b2, Omega_m, H0h):
+    """2 pi^2 times functions.c:4136-4257 at one (z_mean, l, sep); tables_k[k] is the
+    half-integer table ((l - 2k + 3)/2, (l - 2k + 1)/2), k <= l/2.  Odd l: 0."""
+    if l % 2:
+        return 0.0
+    s = 0.0
+    for k in range(l // 2 + 1):
+        s += ((-1) ** k / 2 ** k * comb(l, k) * comb(2 * l - 2 * k, l) * factorial(l // 2 - k) *
+              _lookup(tables_k[k], sep))
+    D1 = _lookup((bg["z"], bg["D1"]), z_mean)
+    s *= (-3 * Omega_m / pi / 8 * (2 * l + 1) * D1 * D1 * (1 + z_mean) * sep *
+          ((2 - 5 * s1) * b2 + (2 - 5 * s2) * b1) * 2 ** (1 - l // 2))
+    return 2 * pi * pi * s / sqrt(2 * pi) * H0h ** 2

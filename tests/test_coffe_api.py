@@ -60,8 +60,30 @@ def test_parameters():
     with pytest.raises(AttributeError):
         c.set_parameters(no_such_parameter=1)
     with pytest.raises(NotImplementedError):
-        c.has_lensing = True
-    c.has_lensing = False
+        c.has_g4 = True
+    c.has_g5 = False
+
+
+def test_lensing_paths_not_built_are_refused():
+    """Flat-sky lensing is computed; the full-sky lensing integrals (Monte Carlo in the
+    reference, signal.c:556-771) raise before any device work."""
+    c = Coffe(has_lensing=True)
+    c.set_power_spectrum_linear(np.logspace(-4, 1, 50), np.logspace(-4, 1, 50) ** -1)
+    with pytest.raises(NotImplementedError):          # lensing-lensing needs has_flatsky_nonlocal
+        c.compute_multipoles_bulk()
+    c.has_flatsky_nonlocal = True
+    c.has_density = True
+    with pytest.raises(NotImplementedError):          # density-lensing needs has_flatsky_local_nonlocal
+        c.compute_multipoles_bulk()
+    c.has_flatsky_local_nonlocal = True
+    c.has_d2 = True
+    with pytest.raises(NotImplementedError):          # d2-lensing: single-integrated
+        c.compute_multipoles_bulk()
+    c.has_d2 = False
+    with pytest.raises(NotImplementedError):
+        c.compute_corrfunc_bulk()
+    with pytest.raises(NotImplementedError):
+        c.compute_average_multipoles_bulk()
 
 
 def test_coordinates_and_contributions_checked_first():

@@ -113,9 +113,9 @@ def test_choice_parameters():
         c.pk_type = "halofit"
     with pytest.raises(NotImplementedError):
         c.m_ncdm = 0.06
-    with pytest.raises(NotImplementedError):
-        c.has_flatsky_nonlocal = True
+    c.has_flatsky_nonlocal = True
     c.has_flatsky_local_nonlocal = False
+    assert c.has_flatsky_nonlocal and not c.has_flatsky_local_nonlocal
 
 
 def test_kept_parameters_are_checked():
