@@ -82,10 +82,12 @@ def load_library():
                                         i, dp, dp, vp, vp, vp]
     L.coffe_mu_weights_jvp.argtypes = [vp, vp, vp, ctypes.c_uint, i, i, i, dp, i, ip, i, ip, ip, i, dp, i, vp, vp, vp]
     L.coffe_integral_table.argtypes = [i, i, sz, dp, dp, sz, d, d, dp, dp]
+    L.coffe_integral_table_state.argtypes = [i, i, i, i, sz, dp, dp, sz, d, d, dp, dp]
     L.coffe_fft_any.argtypes = [dp, dp, ctypes.c_int64]
     i64ptr = ctypes.POINTER(ctypes.c_int64)
     L.coffe_flat_lensing_multipoles.argtypes = [i, dp, i, ip, i, dp, i, dp, dp, dp, ctypes.c_int64, dp, dp, i64ptr, ip,
                                                 i64ptr, ip, d, d, i, i, i, dp]
+    L.coffe_flat_lensing_corrfunc.argtypes = [i, dp, i, dp, i, dp, i, dp, dp, dp, i, dp, dp, d, d, i, i, i, dp]
     L.coffe_divergent_tables.argtypes = [dp, dp, sz, d, d, dp, sz, dp, sz, dp, dp, dp, ctypes.POINTER(i),
                                          ctypes.POINTER(i)]
     L.coffe_batch_set_radix.argtypes = [vp, i, i, ctypes.POINTER(i)]
@@ -124,10 +126,10 @@ EXPORTED = [
     "coffe_batch_vjp", "coffe_batch_tables", "coffe_batch_prepare", "coffe_batch_tables_stage",
     "coffe_batch_project", "coffe_batch_prepare_adj", "coffe_akima_eval", "coffe_background_compute", "coffe_background_jvp",
     "coffe_batch_resample", "coffe_batch_fft_prepare", "coffe_batch_adj_c2r", "coffe_batch_spline_adj",
-    "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_mu_weights_ex", "coffe_terms_of", "coffe_integral_table", "coffe_divergent_tables", "coffe_divergent_tables_device", "coffe_batch_set_radix", "coffe_batch_small_r",
+    "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_mu_weights_ex", "coffe_terms_of", "coffe_integral_table", "coffe_integral_table_state", "coffe_divergent_tables", "coffe_divergent_tables_device", "coffe_batch_set_radix", "coffe_batch_small_r",
     "coffe_cov_plan_create", "coffe_cov_plan_destroy", "coffe_cov_plan_rows", "coffe_cov_plan_pairs",
     "coffe_cov_integrals", "coffe_cov_integrals_vjp", "coffe_cov_assemble", "coffe_cov_assemble_vjp", "two_sph_bessel", "two_sph_bessel_status",
-    "coffe_mu_weights_rule", "coffe_mu_weights_jvp", "coffe_fft_any", "coffe_flat_lensing_multipoles", "coffe_refabi_set_spline_ops", "coffe_find_integral", "coffe_integrals_init",
+    "coffe_mu_weights_rule", "coffe_mu_weights_jvp", "coffe_fft_any", "coffe_flat_lensing_multipoles", "coffe_flat_lensing_corrfunc", "coffe_refabi_set_spline_ops", "coffe_find_integral", "coffe_integrals_init",
     "coffe_integrals_free", "coffe_integrate", "coffe_multipoles_init", "coffe_multipoles_free",
     "coffe_average_multipoles_init", "coffe_average_multipoles_free", "coffe_covariance_init",
     "coffe_covariance_free", "coffe_average_covariance_free", "coffe_covariance_find", "covariance_coefficient",

@@ -186,6 +186,7 @@ class Coffe:
     def _free_all(self):
         self._bg = None
         self._lens = None
+        self._lens_cf = None
         self._tables = None
         self._divergent = None
         self._ren = None
@@ -194,6 +195,7 @@ class Coffe:
     def _free_spectrum(self):
         """Drops what depends on P(k) or its window."""
         self._lens = None
+        self._lens_cf = None
         self._tables = None
         self._divergent = None
         self._ren = None
@@ -1057,18 +1059,33 @@ class Coffe:
             raise ValueError("At least one of the following parameters is empty: mu, sep, z_mean")
 
     def _corrfunc_values(self, zs, seps, mus):
-        """[z][mu][r] local correlation function (functions_nonintegrated, functions.c:
-        89-1484, at each mu): the weights of a one-node rule (the node mu, weight 2, l = 0)."""
+        """[z][mu][r] correlation function: the local terms (functions_nonintegrated,
+        functions.c:89-1484, at each mu: the weights of a one-node rule, the node mu with
+        weight 2 and l = 0) and, with has_lensing, the flat-sky lensing terms
+        (coffe_flat_lensing_corrfunc)."""
+        from .engine import flat_lensing_corrfunc, half_integer_table
+        plan = self._lensing_plan()
         out = np.empty((len(zs), len(mus), len(seps)))
         for j, mu in enumerate(mus):
             out[:, j, :] = self._multipole_values(zs, seps, [0], rule=([float(mu)], [2.0]))[:, 0, :]
+        if plan is not None and (plan[0] or plan[1]):
+            # the (1/2, -1/2) table (integrals.c:917-962), cached with the other lensing tables
+            if self._lens_cf is None:
+                k, pk = self._pk
+                kmin, kmax = self._kwin()
+                self._lens_cf = half_integer_table(1, -1, k, pk, self._p["integrals_fftlog_sampling"], kmin, kmax)
+            zpar = [[float(z), self.comoving_distance(float(z)), self.growth_factor(float(z)),
+                     self._eval_mbias("s1", float(z)), self._eval_mbias("s2", float(z)),
+                     self._eval_bias(1, float(z)), self._eval_bias(2, float(z))] for z in zs]
+            out += flat_lensing_corrfunc(zpar, [float(m) for m in mus], seps, self._background(), self._lens_cf,
+                                         self.omega_m, COFFE_H0 * self._p["h"], plan[0], plan[1])
         return out
 
     def compute_corrfunc(self, *, z, r, mu, recompute=False):
         """The correlation function at (z_mean, r, mu) (coffe.pyx:3159-3230)."""
         self._check_coords_corrfunc()
         self._check_contributions()
-        self._no_lensing("the correlation function")
+        self._lensing_plan()
         _check_parameter("z", z, (int, float), 0, 15)
         _check_parameter("r", r, (int, float), 0, 1500)
         _check_parameter("mu", mu, (int, float), -1, 1)
@@ -1081,7 +1098,7 @@ class Coffe:
         (corrfunc.c:83-90)."""
         self._check_coords_corrfunc()
         self._check_contributions()
-        self._no_lensing("the correlation function")
+        self._lensing_plan()
         if recompute:
             self._free_all()
         if self._corrfunc is None:

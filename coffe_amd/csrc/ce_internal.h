@@ -98,8 +98,9 @@ int ce_launch_grid_tables(void *stream, int N, double kmin, double kmax, double 
 /* d_ranges (4 ints per chunk id, may be NULL): the resample index ranges of the
    chunk's owned and staged intervals, written for ce_launch_spline_adj */
 /* flat-sky lensing multipoles (lensing_kernels.hip), device pointers */
-int ce_launch_flat_lensing(void *stream, int nz, int nl, int nsep, int nb, int ngl, int want_ll, int want_dl,
-                           const double *d_zpar, const int *d_ls, const double *d_sep, const double *d_bg_z,
+int ce_launch_flat_lensing(void *stream, int mode, const double *d_mus, int nz, int nl, int nsep, int nb, int ngl,
+                           int want_ll, int want_dl, const double *d_zpar, const int *d_ls, const double *d_sep,
+                           const double *d_bg_z,
                            const double *d_bg_chi, const double *d_bg_D1, const double *d_tab_x, const double *d_tab_y,
                            const int64_t *d_ll_off, const int *d_ll_len, const int64_t *d_dl_off, const int *d_dl_len,
                            const double *d_glx, const double *d_glw, double Omega_m, double H0h, double *d_out);

@@ -1424,7 +1424,6 @@ int coffe_integrals_renormalizable(double **output_x, double **output_y, const s
                                    const double x_max)
 {
     if (!output_x || !output_y || !real_output_len || !spectrum) return COFFE_EINVAL;
-    if (state_n != 0 || state_l != 0) return COFFE_EUNSUPPORTED;
     const ce_interpolation_view *sp = (const ce_interpolation_view *)spectrum;
     if (!sp->spline || !sp->spline->x || !sp->spline->y) return COFFE_EINVAL;
     const size_t len = output_len + 1 + CE_MIN_SEP_LEN;
@@ -1435,8 +1434,8 @@ int coffe_integrals_renormalizable(double **output_x, double **output_y, const s
         free(y);
         return COFFE_ENOMEM;
     }
-    const int st = coffe_integral_table(n, l, output_len, sp->spline->x, sp->spline->y, sp->spline->size, x_min,
-                                        x_max, x, y);
+    const int st = coffe_integral_table_state(n, l, state_n, state_l, output_len, sp->spline->x, sp->spline->y,
+                                              sp->spline->size, x_min, x_max, x, y);
     if (st != COFFE_OK) {
         free(x);
         free(y);
@@ -1532,18 +1531,17 @@ int coffe_divergent_tables(const double *k, const double *pk, size_t nin, double
     return st;
 }
 
-int coffe_integral_table(int n, int l, size_t N, const double *k, const double *pk, size_t nin, double kmin,
-                         double kmax, double *out_x, double *out_y)
+/* the table of (n, l), both integers or both half-integers (integrals.c:186-207) */
+static int integral_table_any(double n, double l, size_t N, const double *k, const double *pk, size_t nin,
+                              double kmin, double kmax, double *out_x, double *out_y)
 {
     const size_t nodes = 1 + CE_MIN_SEP_LEN;
-    if (!k || !pk || !out_x || !out_y || nin < 5 || n < -2 || n > 4 || l < 0 || l > 4) return COFFE_EINVAL;
     if (!(kmin >= k[0]) || !(kmax <= k[nin - 1]) || !(kmin < kmax)) return COFFE_EDOMAIN;
     /* FFT nodes (integrals.c:496-522) */
-    int st = twofast_1bessel_status(out_x + nodes, out_y + nodes, N, k, pk, nin, (double)l, (double)n, 1.0, kmin,
-                                    kmin, kmax);
+    int st = twofast_1bessel_status(out_x + nodes, out_y + nodes, N, k, pk, nin, l, n, 1.0, kmin, kmin, kmax);
     if (st != COFFE_OK) return st;
     if (n >= l)
-        for (size_t i = 0; i < N; ++i) out_y[nodes + i] *= pow(out_x[nodes + i], (double)(n - l));
+        for (size_t i = 0; i < N; ++i) out_y[nodes + i] *= pow(out_x[nodes + i], n - l);
     /* small-r nodes (integrals.c:524-557) */
     coffe_engine *e = coffe_engine_default();
     if (!e) return COFFE_ENODEVICE;
@@ -1597,6 +1595,26 @@ int coffe_integral_table(int n, int l, size_t N, const double *k, const double *
     return COFFE_OK;
 }
 
+int coffe_integral_table(int n, int l, size_t N, const double *k, const double *pk, size_t nin, double kmin,
+                         double kmax, double *out_x, double *out_y)
+{
+    if (!k || !pk || !out_x || !out_y || nin < 5 || n < -2 || n > 4 || l < 0 || l > 4) return COFFE_EINVAL;
+    return integral_table_any((double)n, (double)l, N, k, pk, nin, kmin, kmax, out_x, out_y);
+}
+
+int coffe_integral_table_state(int n, int l, int state_n, int state_l, size_t N, const double *k, const double *pk,
+                               size_t nin, double kmin, double kmax, double *out_x, double *out_y)
+{
+    if (state_n < 0 || state_n > 1 || state_l < 0 || state_l > 1) return COFFE_EINVAL;
+    if (state_n != state_l) return COFFE_EUNSUPPORTED;
+    if (state_n == 0) return coffe_integral_table(n, l, N, k, pk, nin, kmin, kmax, out_x, out_y);
+    /* half-integers n/2, l/2: odd raw values, l/2 >= -1/2 (J_{l/2 + 1/2} of the small-r
+       nodes has a non-negative integer order) */
+    if (!k || !pk || !out_x || !out_y || nin < 5 || !(n & 1) || !(l & 1) || l < -1 || l > 9 || n < -3 || n > 9)
+        return COFFE_EINVAL;
+    return integral_table_any(n / 2.0, l / 2.0, N, k, pk, nin, kmin, kmax, out_x, out_y);
+}
+
 /* ---- flat-sky lensing multipoles ------------------------------------------------ */
 
 static int flat_table_ok(int64_t off, int len, int64_t ntab_values)
@@ -1604,31 +1622,14 @@ static int flat_table_ok(int64_t off, int len, int64_t ntab_values)
     return len >= 5 && off >= 0 && off + len <= ntab_values;
 }
 
-int coffe_flat_lensing_multipoles(int nz, const double *zpar, int nl, const int *ls, int nsep, const double *sep,
-                                  int nb, const double *bg_z, const double *bg_chi, const double *bg_D1,
-                                  int64_t ntab_values, const double *tab_x, const double *tab_y, const int64_t *ll_off,
-                                  const int *ll_len, const int64_t *dl_off, const int *dl_len, double Omega_m,
-                                  double H0h, int want_ll, int want_dl, int panels, double *out)
+/* mode 0: multipoles over ls (tables per multipole); mode 1: correlation function over
+   mus (the (1/2, -1/2) table in slot 0 of ll_off / ll_len) */
+static int flat_lensing_run(int mode, int nz, const double *zpar, int nl, const int *ls, const double *mus, int nsep,
+                            const double *sep, int nb, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                            int64_t ntab_values, const double *tab_x, const double *tab_y, const int64_t *ll_off,
+                            const int *ll_len, const int64_t *dl_off, const int *dl_len, double Omega_m, double H0h,
+                            int want_ll, int want_dl, int panels, double *out)
 {
-    if (nz <= 0 || nl <= 0 || nsep <= 0 || nb < 5 || !zpar || !ls || !sep || !bg_z || !bg_chi || !bg_D1 || !out ||
-        !ll_off || !ll_len || !dl_off || !dl_len || panels < 1 || panels > 4096 || ntab_values < 0 ||
-        (ntab_values > 0 && (!tab_x || !tab_y)) || (int64_t)nz * nl * nsep > 0x7fffffff)
-        return COFFE_EINVAL;
-    /* every table a wanted term reads must be there; the others must be empty or whole */
-    for (int il = 0; il < nl; ++il) {
-        const int l = ls[il];
-        if (l < 0) return COFFE_EINVAL;
-        const int even = (l % 2) == 0;
-        if (want_ll && even && !flat_table_ok(ll_off[il], ll_len[il], ntab_values)) return COFFE_EINVAL;
-        if (ll_len[il] != 0 && !flat_table_ok(ll_off[il], ll_len[il], ntab_values)) return COFFE_EINVAL;
-        if (want_dl && even && l > 2 * (CE_FL_MAXK - 1)) return COFFE_EINVAL;
-        for (int k = 0; k < CE_FL_MAXK; ++k) {
-            const int t = il * CE_FL_MAXK + k;
-            if (want_dl && even && k <= l / 2 && !flat_table_ok(dl_off[t], dl_len[t], ntab_values))
-                return COFFE_EINVAL;
-            if (dl_len[t] != 0 && !flat_table_ok(dl_off[t], dl_len[t], ntab_values)) return COFFE_EINVAL;
-        }
-    }
     coffe_engine *e = coffe_engine_default();
     if (!e) return COFFE_ENODEVICE;
     hipSetDevice(e->device);
@@ -1645,44 +1646,58 @@ int coffe_flat_lensing_multipoles(int nz, const double *zpar, int nl, const int 
             gl[ngl + 32 * p + i] = 0.5 * (b - a) * t32[32 + i];
         }
     }
-    /* one device buffer, 8-byte slots */
+    /* one device buffer, 8-byte slots; int arrays as int32 at the start of their slots */
     const size_t nzp = 7 * (size_t)nz, nd = (size_t)nl * CE_FL_MAXK, nout = (size_t)nz * nl * nsep;
     const size_t o_zpar = 0, o_ls = o_zpar + nzp, o_sep = o_ls + (size_t)nl, o_bz = o_sep + (size_t)nsep,
                  o_bc = o_bz + (size_t)nb, o_bd = o_bc + (size_t)nb, o_tx = o_bd + (size_t)nb,
                  o_ty = o_tx + (size_t)ntab_values, o_llo = o_ty + (size_t)ntab_values, o_lll = o_llo + (size_t)nl,
                  o_dlo = o_lll + (size_t)nl, o_dll = o_dlo + nd, o_gl = o_dll + nd, o_out = o_gl + 2 * (size_t)ngl,
                  total = o_out + nout;
-    int64_t *hbuf = (int64_t *)calloc(2 * (size_t)nl + 2 * nd + (size_t)nl, sizeof(int64_t));
-    if (!hbuf) { free(gl); return COFFE_ENOMEM; }
-    int64_t *h_ls = hbuf, *h_lll = hbuf + nl, *h_dll = hbuf + 2 * (size_t)nl;
-    /* the int arrays go to 8-byte slots as int pairs: pack them as int32 at the slot start */
-    int32_t *ls32 = (int32_t *)h_ls, *lll32 = (int32_t *)h_lll, *dll32 = (int32_t *)h_dll;
-    for (int il = 0; il < nl; ++il) {
-        ls32[il] = ls[il];
-        lll32[il] = ll_len[il];
+    int32_t *i32 = (int32_t *)calloc(2 * (size_t)nl + nd, sizeof(int32_t));
+    int64_t *i64 = (int64_t *)calloc((size_t)nl + nd, sizeof(int64_t));
+    if (!i32 || !i64) {
+        free(gl);
+        free(i32);
+        free(i64);
+        return COFFE_ENOMEM;
     }
-    for (size_t t = 0; t < nd; ++t) dll32[t] = dl_len[t];
+    int32_t *ls32 = i32, *lll32 = i32 + nl, *dll32 = i32 + 2 * (size_t)nl;
+    int64_t *llo64 = i64, *dlo64 = i64 + nl;
+    for (int il = 0; il < nl; ++il) {
+        ls32[il] = mode == 0 ? ls[il] : 0;
+        lll32[il] = ll_len[mode == 0 ? il : 0];
+        llo64[il] = ll_off[mode == 0 ? il : 0];
+    }
+    if (mode == 0)
+        for (size_t t = 0; t < nd; ++t) {
+            dll32[t] = dl_len[t];
+            dlo64[t] = dl_off[t];
+        }
     double *d = NULL;
     int st = dev_alloc((void **)&d, sizeof(double) * total);
     hipStream_t s = e->stream;
 #define FL_PUT(off, src, bytes) \
     if (st == COFFE_OK && (bytes) > 0) st = hip_ok(hipMemcpyAsync(d + (off), (src), (bytes), hipMemcpyHostToDevice, s))
     FL_PUT(o_zpar, zpar, sizeof(double) * nzp);
-    FL_PUT(o_ls, ls32, sizeof(int32_t) * (size_t)nl);
+    if (mode == 0) {
+        FL_PUT(o_ls, ls32, sizeof(int32_t) * (size_t)nl);
+    } else {
+        FL_PUT(o_ls, mus, sizeof(double) * (size_t)nl);
+    }
     FL_PUT(o_sep, sep, sizeof(double) * (size_t)nsep);
     FL_PUT(o_bz, bg_z, sizeof(double) * (size_t)nb);
     FL_PUT(o_bc, bg_chi, sizeof(double) * (size_t)nb);
     FL_PUT(o_bd, bg_D1, sizeof(double) * (size_t)nb);
     FL_PUT(o_tx, tab_x, sizeof(double) * (size_t)ntab_values);
     FL_PUT(o_ty, tab_y, sizeof(double) * (size_t)ntab_values);
-    FL_PUT(o_llo, ll_off, sizeof(int64_t) * (size_t)nl);
+    FL_PUT(o_llo, llo64, sizeof(int64_t) * (size_t)nl);
     FL_PUT(o_lll, lll32, sizeof(int32_t) * (size_t)nl);
-    FL_PUT(o_dlo, dl_off, sizeof(int64_t) * nd);
+    FL_PUT(o_dlo, dlo64, sizeof(int64_t) * nd);
     FL_PUT(o_dll, dll32, sizeof(int32_t) * nd);
     FL_PUT(o_gl, gl, sizeof(double) * 2 * (size_t)ngl);
 #undef FL_PUT
     if (st == COFFE_OK &&
-        ce_launch_flat_lensing(s, nz, nl, nsep, nb, ngl, want_ll != 0, want_dl != 0, d + o_zpar,
+        ce_launch_flat_lensing(s, mode, d + o_ls, nz, nl, nsep, nb, ngl, want_ll != 0, want_dl != 0, d + o_zpar,
                                (const int *)(d + o_ls), d + o_sep, d + o_bz, d + o_bc, d + o_bd, d + o_tx, d + o_ty,
                                (const int64_t *)(d + o_llo), (const int *)(d + o_lll), (const int64_t *)(d + o_dlo),
                                (const int *)(d + o_dll), d + o_gl, d + o_gl + ngl, Omega_m, H0h, d + o_out))
@@ -1690,7 +1705,61 @@ int coffe_flat_lensing_multipoles(int nz, const double *zpar, int nl, const int 
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(out, d + o_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
     if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
     hipFree(d);
-    free(hbuf);
+    free(i32);
+    free(i64);
     free(gl);
     return st;
+}
+
+static int flat_common_ok(int nz, const double *zpar, int nl, int nsep, const double *sep, int nb, const double *bg_z,
+                          const double *bg_chi, const double *bg_D1, int64_t ntab_values, const double *tab_x,
+                          const double *tab_y, int panels, const double *out)
+{
+    return !(nz <= 0 || nl <= 0 || nsep <= 0 || nb < 5 || !zpar || !sep || !bg_z || !bg_chi || !bg_D1 || !out ||
+             panels < 1 || panels > 4096 || ntab_values < 0 || (ntab_values > 0 && (!tab_x || !tab_y)) ||
+             (int64_t)nz * nl * nsep > 0x7fffffff);
+}
+
+int coffe_flat_lensing_multipoles(int nz, const double *zpar, int nl, const int *ls, int nsep, const double *sep,
+                                  int nb, const double *bg_z, const double *bg_chi, const double *bg_D1,
+                                  int64_t ntab_values, const double *tab_x, const double *tab_y, const int64_t *ll_off,
+                                  const int *ll_len, const int64_t *dl_off, const int *dl_len, double Omega_m,
+                                  double H0h, int want_ll, int want_dl, int panels, double *out)
+{
+    if (!flat_common_ok(nz, zpar, nl, nsep, sep, nb, bg_z, bg_chi, bg_D1, ntab_values, tab_x, tab_y, panels, out) ||
+        !ls || !ll_off || !ll_len || !dl_off || !dl_len)
+        return COFFE_EINVAL;
+    /* every table a wanted term reads must be there; the others must be empty or whole */
+    for (int il = 0; il < nl; ++il) {
+        const int l = ls[il];
+        if (l < 0) return COFFE_EINVAL;
+        const int even = (l % 2) == 0;
+        if (want_ll && even && !flat_table_ok(ll_off[il], ll_len[il], ntab_values)) return COFFE_EINVAL;
+        if (ll_len[il] != 0 && !flat_table_ok(ll_off[il], ll_len[il], ntab_values)) return COFFE_EINVAL;
+        if (want_dl && even && l > 2 * (CE_FL_MAXK - 1)) return COFFE_EINVAL;
+        for (int k = 0; k < CE_FL_MAXK; ++k) {
+            const int t = il * CE_FL_MAXK + k;
+            if (want_dl && even && k <= l / 2 && !flat_table_ok(dl_off[t], dl_len[t], ntab_values))
+                return COFFE_EINVAL;
+            if (dl_len[t] != 0 && !flat_table_ok(dl_off[t], dl_len[t], ntab_values)) return COFFE_EINVAL;
+        }
+    }
+    return flat_lensing_run(0, nz, zpar, nl, ls, NULL, nsep, sep, nb, bg_z, bg_chi, bg_D1, ntab_values, tab_x, tab_y,
+                            ll_off, ll_len, dl_off, dl_len, Omega_m, H0h, want_ll, want_dl, panels, out);
+}
+
+int coffe_flat_lensing_corrfunc(int nz, const double *zpar, int nmu, const double *mu, int nsep, const double *sep,
+                                int nb, const double *bg_z, const double *bg_chi, const double *bg_D1, int ntab,
+                                const double *tab_x, const double *tab_y, double Omega_m, double H0h, int want_ll,
+                                int want_dl, int panels, double *out)
+{
+    if (!flat_common_ok(nz, zpar, nmu, nsep, sep, nb, bg_z, bg_chi, bg_D1, ntab, tab_x, tab_y, panels, out) || !mu ||
+        ((want_ll || want_dl) && ntab < 5))
+        return COFFE_EINVAL;
+    for (int i = 0; i < nmu; ++i)
+        if (!(mu[i] >= -1.0 && mu[i] <= 1.0)) return COFFE_EINVAL;
+    const int64_t off0 = 0;
+    const int len0 = ntab;
+    return flat_lensing_run(1, nz, zpar, nmu, NULL, mu, nsep, sep, nb, bg_z, bg_chi, bg_D1, ntab, tab_x, tab_y, &off0,
+                            &len0, NULL, NULL, Omega_m, H0h, want_ll, want_dl, panels, out);
 }

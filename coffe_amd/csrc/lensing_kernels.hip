@@ -28,6 +28,8 @@
 namespace {
 
 struct flatlens_args {
+    int mode;                            // 0: multipoles over ls; 1: correlation function over mus
+    const double *mus;                   // [nl] in mode 1
     int nz, nl, nsep, nb, ngl, want_ll, want_dl;
     const double *zpar;                  // [nz][7]: z, chi, D1, s1, s2, b1, b2 at z_mean
     const int *ls;                       // [nl]
@@ -55,6 +57,54 @@ __device__ inline double binomial(int n, int k)
     double c = 1.0;
     for (int i = 1; i <= k; ++i) c = c * (n - k + i) / i;
     return c;
+}
+
+// Mode 1, the correlation function at (z, mu, r) (functions.c:3515-3558 and 1563-1622,
+// the branches for flatsky_nonlocal / flatsky_local_nonlocal outside MULTIPOLES): with
+// T the (1/2, -1/2) half-integer table (r I(r)) times sqrt(2/pi) 2 pi^2 (integrals.c:956-
+// 962) and s = sqrt(1 - mu^2),
+//   lensing-lensing = int_0^1 dx 9 Omega_m^2 (2 - 5 s1)(2 - 5 s2) chi^3 / (8 pi)
+//                     T(x r s) D1(z(x chi))^2 (1 + z(x chi))^2 (x (1 - x))^2 (H0 h)^4
+//     (the reference's second line-of-sight variable does not enter: its integral is 1),
+//   density-lensing = (H0 h)^2 (-3 Omega_m / (8 pi)) D1^2 (1 + z) |mu| r
+//                     ((2 - 5 s1) b2 + (2 - 5 s2) b1) T(r s).
+__global__ void __launch_bounds__(64) k_flat_lensing_cf(flatlens_args a)
+{
+    const int idx = blockIdx.x;   // (z, mu, r), r fastest
+    const int ir = idx % a.nsep;
+    const int im = (idx / a.nsep) % a.nl;
+    const int iz = idx / (a.nsep * a.nl);
+    const int lane = threadIdx.x;
+    const double mu = a.mus[im];
+    const double r = a.sep[ir];
+    const double sm = sqrt(fmax(1.0 - mu * mu, 0.0));
+    const double *zp = a.zpar + 7 * iz;
+    const double z = zp[0], chi = zp[1], D1 = zp[2], s1 = zp[3], s2 = zp[4], b1 = zp[5], b2 = zp[6];
+    const double pi = 3.14159265358979323846;
+    const double *tx = a.tab_x + a.ll_off[0], *ty = a.tab_y + a.ll_off[0];
+    const int tn = a.ll_len[0];
+    const double tscale = sqrt(2.0 / pi) * 2.0 * pi * pi;
+    double acc = 0.0;
+    if (a.want_ll) {
+        for (int q = lane; q < a.ngl; q += 64) {
+            const double x = a.glx[q];
+            const double zl = ce::akima_general(a.bg_chi, a.bg_z, a.nb, chi * x);
+            const double Dl = ce::akima_general(a.bg_z, a.bg_D1, a.nb, zl);
+            const double t = tscale * ce::akima_general(tx, ty, tn, x * r * sm);
+            const double xx = x * (1.0 - x);
+            acc += a.glw[q] * t * Dl * Dl * (1.0 + zl) * (1.0 + zl) * xx * xx;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+        const double h2 = a.H0h * a.H0h;
+        acc *= 9.0 * a.Omega_m * a.Omega_m * (2.0 - 5.0 * s1) * (2.0 - 5.0 * s2) * chi * chi * chi / 8.0 / pi * h2 * h2;
+    }
+    if (lane != 0) return;
+    if (a.want_dl)
+        acc += a.H0h * a.H0h * (-3.0 * a.Omega_m / pi / 8.0 * D1 * D1 * (1.0 + z) * fabs(mu) * r *
+                                ((2.0 - 5.0 * s1) * b2 + (2.0 - 5.0 * s2) * b1) *
+                                (tscale * ce::akima_general(tx, ty, tn, r * sm)));
+    a.out[idx] = acc;
 }
 
 __global__ void __launch_bounds__(64) k_flat_lensing(flatlens_args a)
@@ -111,8 +161,9 @@ __global__ void __launch_bounds__(64) k_flat_lensing(flatlens_args a)
 
 }  // namespace
 
-extern "C" int ce_launch_flat_lensing(void *stream, int nz, int nl, int nsep, int nb, int ngl, int want_ll,
-                                      int want_dl, const double *d_zpar, const int *d_ls, const double *d_sep,
+extern "C" int ce_launch_flat_lensing(void *stream, int mode, const double *d_mus, int nz, int nl, int nsep, int nb,
+                                      int ngl, int want_ll, int want_dl, const double *d_zpar, const int *d_ls,
+                                      const double *d_sep,
                                       const double *d_bg_z, const double *d_bg_chi, const double *d_bg_D1,
                                       const double *d_tab_x, const double *d_tab_y, const int64_t *d_ll_off,
                                       const int *d_ll_len, const int64_t *d_dl_off, const int *d_dl_len,
@@ -120,6 +171,8 @@ extern "C" int ce_launch_flat_lensing(void *stream, int nz, int nl, int nsep, in
                                       double *d_out)
 {
     flatlens_args a;
+    a.mode = mode;
+    a.mus = d_mus;
     a.nz = nz;
     a.nl = nl;
     a.nsep = nsep;
@@ -146,6 +199,9 @@ extern "C" int ce_launch_flat_lensing(void *stream, int nz, int nl, int nsep, in
     a.out = d_out;
     const int64_t blocks = (int64_t)nz * nl * nsep;
     if (blocks <= 0 || blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_flat_lensing, dim3((unsigned)blocks), dim3(64), 0, (hipStream_t)stream, a);
+    if (mode == 1)
+        hipLaunchKernelGGL(k_flat_lensing_cf, dim3((unsigned)blocks), dim3(64), 0, (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(k_flat_lensing, dim3((unsigned)blocks), dim3(64), 0, (hipStream_t)stream, a);
     return (int)hipGetLastError();
 }

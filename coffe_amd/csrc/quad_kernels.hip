@@ -86,6 +86,63 @@ __device__ double sph_jl(int l, double x)
     return fl * scale;
 }
 
+// cylindrical Bessel J_m(x), integer m >= 0, x >= 0 (the half-integer tables' small-r
+// integrand calls GSL's sf_bessel_Jn, integrals.c:296-306): power series below 2,
+// Hankel's asymptotic expansion from 25 on (the terms of P and Q fall below 1e-17
+// there for the orders used), otherwise Miller's downward recurrence normalised with
+// J_0 + 2 sum_k J_2k = 1, started 30 orders above max(m, x)
+__device__ double cyl_jn(int m, double x)
+{
+    if (x == 0.0) return m == 0 ? 1.0 : 0.0;
+    if (x < 2.0) {
+        double pre = 1.0;
+        for (int i = 1; i <= m; ++i) pre *= 0.5 * x / i;
+        const double y = -0.25 * x * x;
+        double term = 1.0, sum = 1.0;
+        for (int k = 1; k < 40; ++k) {
+            term *= y / (k * (double)(k + m));
+            sum += term;
+            if (fabs(term) < 1e-18 * fabs(sum)) break;
+        }
+        return pre * sum;
+    }
+    const double pi = 3.14159265358979323846;
+    if (x >= 25.0) {
+        const double mu = 4.0 * m * m;
+        double P = 1.0, Q = 0.0, ak = 1.0;
+        for (int k = 1; k < 60; ++k) {
+            ak *= (mu - (2.0 * k - 1.0) * (2.0 * k - 1.0)) / (k * 8.0 * x);
+            if (k & 1)
+                Q += (((k - 1) / 2) & 1) ? -ak : ak;
+            else
+                P += ((k / 2) & 1) ? -ak : ak;
+            if (fabs(ak) < 1e-17) break;
+        }
+        double sn, cs;
+        sincos(x - (0.5 * m + 0.25) * pi, &sn, &cs);
+        return sqrt(2.0 / (pi * x)) * (P * cs - Q * sn);
+    }
+    const int top = m > (int)x ? m : (int)x;
+    const int N = 2 * ((top + 30) / 2 + 1);
+    double fp = 0.0, fc = 1e-30, fm = 0.0, norm = 0.0;
+    for (int k = N; k > 0; --k) {
+        if (k == m) fm = fc;
+        if ((k & 1) == 0) norm += 2.0 * fc;
+        const double fn = 2.0 * k / x * fc - fp;
+        fp = fc;
+        fc = fn;
+        if (fabs(fc) > 1e250) {
+            fc *= 1e-250;
+            fp *= 1e-250;
+            fm *= 1e-250;
+            norm *= 1e-250;
+        }
+    }
+    if (m == 0) fm = fc;
+    norm += fc;
+    return fm / norm;
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -107,11 +164,21 @@ struct sr_unit {
     int n, l;
     double r, r2;
     int kind;
+    // half-integer table (n, l both half-integers, integrals.c:186-207): nd, ld are
+    // n and l, and the kernel is sqrt(pi/2) J_{l+1/2}(kr) / (kr)^(n+1/2) = j_l(kr)/(kr)^n
+    int half;
+    double nd, ld;
 };
 
 __device__ __forceinline__ double integrand(const sr_unit &u, double k)
 {
     const double pk = ce::akima_general(u.k, u.p, u.n_in, k);
+    const double sqrt_pi_2 = 1.25331413731550025121;
+    if (u.half && u.kind == SR_PREFACTOR) return sqrt_pi_2 * pow(k, 2.0 + u.ld - u.nd) * pk;
+    if (u.half && u.kind == SR_BESSEL) {
+        const double x = k * u.r;
+        return k * k * pk * sqrt_pi_2 * cyl_jn((int)(u.ld + 0.5), x) / pow(x, u.nd + 0.5);
+    }
     switch (u.kind) {
     case SR_PREFACTOR: return pow(k, (double)(2 + u.l - u.n)) * pk;
     case SR_RENORM0: {
@@ -274,11 +341,14 @@ __global__ void __launch_bounds__(64) k_small_r(sr_args a)
     u.n_in = sp.n_in;
     u.n = (int)a.tabs[t].n;
     u.l = (int)a.tabs[t].l;
+    u.nd = a.tabs[t].n;
+    u.ld = a.tabs[t].l;
+    u.half = u.ld != floor(u.ld);
     u.kind = node == 0 ? SR_PREFACTOR : SR_BESSEL;
     u.r = node == 0 ? 0.0 : c_min_sep[node - 1];
     u.r2 = 0.0;
     const int64_t o = (int64_t)unit;
-    if (node == 0 && u.n < u.l) {   // I_l^n(0) = 0 for n < l
+    if (node == 0 && (u.half ? u.nd < u.ld : u.n < u.l)) {   // I_l^n(0) = 0 for n < l
         if (lane == 0) {
             a.out[o] = 0.0;
             a.status[o] = 0;
@@ -290,12 +360,18 @@ __global__ void __launch_bounds__(64) k_small_r(sr_args a)
     if (lane == 0) {
         const double pi = 3.14159265358979323846;
         double v = result / 2. / pi / pi;
-        if (node == 0) {
+        if (node == 0 && u.half) {
+            // 2^(l+1/2) Gamma(l+3/2) = 2^m m!, m = l + 1/2 (integrals.c:215-231)
+            const int m = (int)(u.ld + 0.5);
+            double c = 1.0;
+            for (int i = 1; i <= m; ++i) c *= 2.0 * i;
+            v /= c;
+        } else if (node == 0) {
             double df = 1.0;   // (2l+1)!! = 2^(l+1/2) Gamma(l+3/2) / sqrt(pi/2)
             for (int i = 3; i <= 2 * u.l + 1; i += 2) df *= i;
             v /= df;
-        } else if (u.n >= u.l) {
-            v *= pow(u.r, (double)(u.n - u.l));
+        } else if (u.half ? u.nd >= u.ld : u.n >= u.l) {
+            v *= u.half ? pow(u.r, u.nd - u.ld) : pow(u.r, (double)(u.n - u.l));
         }
         if (!isfinite(v) && st == 0) st = 3;
         a.out[o] = v;
@@ -332,6 +408,9 @@ __global__ void __launch_bounds__(64) k_divergent(div_args a)
     u.n_in = a.n_in;
     u.n = 4;
     u.l = 0;
+    u.half = 0;
+    u.nd = 4.0;
+    u.ld = 0.0;
     u.r2 = 0.0;
     if (unit < 2 * a.nsep) {
         u.kind = unit < a.nsep ? SR_BESSEL : SR_RENORM0;

@@ -222,16 +222,28 @@ def integral_table(n, l, k, pk, N, kmin=None, kmax=None):
     return ox, oy
 
 
+def integral_table_state(n, l, state_n, state_l, k, pk, N, kmin=None, kmax=None):
+    """The full (N + 43)-point table with the reference's integer states
+    (coffe_integral_table_state; state 1 = half-integer: the table of (n/2, l/2)); host
+    arrays."""
+    L = load_library()
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    pk = np.ascontiguousarray(pk, dtype=np.float64)
+    kmin = float(k[0]) if kmin is None else kmin
+    kmax = float(k[-1]) if kmax is None else kmax
+    ox, oy = np.empty(N + 43), np.empty(N + 43)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(L.coffe_integral_table_state(int(n), int(l), int(state_n), int(state_l), int(N), k.ctypes.data_as(dp),
+                                       pk.ctypes.data_as(dp), len(k), kmin, kmax, ox.ctypes.data_as(dp),
+                                       oy.ctypes.data_as(dp)), "coffe_integral_table_state")
+    return ox, oy
+
+
 def half_integer_table(n2, l2, k, pk, N, kmin=None, kmax=None):
-    """The FFT nodes of the half-integer table (n, l) = (n2/2, l2/2) of the flat-sky
-    density-lensing term (integrals.c:1246-1300 with 459-573): twofast_1bessel with
-    half-integer l and nu, times r^(n - l) when n >= l.  The reference's table adds r = 0
-    and 42 smaller separations by quadrature; they lie below the FFT grid's first node
-    (1/kmax) and are not built here, so lookups start at that node."""
-    x, y = twofast_1bessel(k, pk, N, l2 / 2.0, n2 / 2.0, 1.0, None, kmin, kmax)
-    if n2 >= l2:
-        y = y * x ** ((n2 - l2) / 2.0)
-    return x, y
+    """The half-integer table (n, l) = (n2/2, l2/2) of the flat-sky lensing terms
+    (integrals.c:917-960, 1246-1300 with 459-573): r = 0, the 42 small separations and
+    the N FFTLog nodes, r^(n - l) I_l^n(r) for n >= l."""
+    return integral_table_state(n2, l2, 1, 1, k, pk, N, kmin, kmax)
 
 
 FL_MAXK = 5   # half-integer tables per multipole of the density-lensing term (l <= 8)
@@ -290,6 +302,26 @@ def flat_lensing(zpar, ls, sep, bg, ll_tables, dl_tables, Omega_m, H0h, want_ll,
                                           dl_len.ctypes.data_as(ip), float(Omega_m), float(H0h), int(bool(want_ll)),
                                           int(bool(want_dl)), int(panels), out.ctypes.data_as(dp)),
           "coffe_flat_lensing_multipoles")
+    return out
+
+
+def flat_lensing_corrfunc(zpar, mus, sep, bg, table, Omega_m, H0h, want_ll, want_dl, panels=128):
+    """coffe_flat_lensing_corrfunc: the flat-sky lensing-lensing and density-lensing
+    correlation function out[nz][nmu][nsep]; table = the (1/2, -1/2) half-integer table
+    (half_integer_table(1, -1, ...)); zpar as for flat_lensing."""
+    L = load_library()
+    zpar = np.ascontiguousarray(zpar, dtype=np.float64).reshape(-1, 7)
+    mus = np.ascontiguousarray(mus, dtype=np.float64)
+    sep = np.ascontiguousarray(sep, dtype=np.float64)
+    tx, ty = (np.ascontiguousarray(v, dtype=np.float64) for v in table)
+    bz, bc, bd = (np.ascontiguousarray(bg[key], dtype=np.float64) for key in ("z", "chi", "D1"))
+    out = np.empty((len(zpar), len(mus), len(sep)))
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(L.coffe_flat_lensing_corrfunc(len(zpar), zpar.ctypes.data_as(dp), len(mus), mus.ctypes.data_as(dp), len(sep),
+                                        sep.ctypes.data_as(dp), len(bz), bz.ctypes.data_as(dp), bc.ctypes.data_as(dp),
+                                        bd.ctypes.data_as(dp), len(tx), tx.ctypes.data_as(dp), ty.ctypes.data_as(dp),
+                                        float(Omega_m), float(H0h), int(bool(want_ll)), int(bool(want_dl)),
+                                        int(panels), out.ctypes.data_as(dp)), "coffe_flat_lensing_corrfunc")
     return out
 
 

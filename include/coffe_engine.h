@@ -332,6 +332,19 @@ int coffe_integral_table(int n, int l, size_t N, const double *k, const double *
                          double kmax, double *out_x, double *out_y);
 
 /*
+ * The same table with the reference's integer states (integrals.c:186-207, enum
+ * coffe_integer_state of integrals.h:23: 0 COFFE_INTEGER, 1 COFFE_HALF_INTEGER): with
+ * both states 1 the table is that of (n/2, l/2) for odd n in [-3, 9], l in [-1, 9], the
+ * half-integer tables of the flat-sky lensing terms (integrals.c:917-960, 1246-1300):
+ * FFT nodes from twofast_1bessel with half-integer l and nu, small-r nodes from the
+ * kernel sqrt(pi/2) J_{l/2+1/2}(kr) / (kr)^{n/2+1/2} (integrals.c:296-306) and
+ * I(0) = sqrt(pi/2) int k^{2+l/2-n/2} P / (2^m m!) / (2 pi^2), m = l/2 + 1/2
+ * (integrals.c:215-231).  Mixed states return COFFE_EUNSUPPORTED.
+ */
+int coffe_integral_table_state(int n, int l, int state_n, int state_l, size_t N, const double *k, const double *pk,
+                               size_t nin, double kmin, double kmax, double *out_x, double *out_y);
+
+/*
  * The 43 small-separation nodes (r = 0 and the 42 min_sep values) of every table of
  * a batch, for its spectra: d_out[nspec][ntab][43] (device).  Synchronises the
  * stream and returns COFFE_ENOCONV when an integral did not converge.
@@ -375,6 +388,27 @@ int coffe_flat_lensing_multipoles(int nz, const double *zpar, int nl, const int 
                                   int64_t ntab_values, const double *tab_x, const double *tab_y, const int64_t *ll_off,
                                   const int *ll_len, const int64_t *dl_off, const int *dl_len, double Omega_m,
                                   double H0h, int want_ll, int want_dl, int panels, double *out);
+
+/*
+ * The flat-sky lensing correlation function at every (z_mean, mu, separation), the
+ * branches the reference takes outside MULTIPOLES with its flat-sky switches
+ * (functions.c:3515-3558, lensing-lensing with flatsky_nonlocal, whose second
+ * line-of-sight variable does not enter; functions.c:1563-1622, density-lensing with
+ * flatsky_local_nonlocal): with T(r) the (n, l) = (1/2, -1/2) table times
+ * sqrt(2/pi) 2 pi^2 (integrals.c:956-962) and s = sqrt(1 - mu^2),
+ *   lensing-lensing = int_0^1 dx 9 Omega_m^2 (2 - 5 s1)(2 - 5 s2) chi^3 / (8 pi) T(x r s)
+ *                     D1(z(x chi))^2 (1 + z(x chi))^2 (x (1 - x))^2 (H0 h)^4,
+ *   density-lensing = (H0 h)^2 (-3 Omega_m / (8 pi)) D1^2 (1 + z) |mu| r
+ *                     ((2 - 5 s1) b2 + (2 - 5 s2) b1) T(r s),
+ * summed into out[nz][nmu][nsep] when want_ll / want_dl is set.  tab_x, tab_y is the
+ * table as coffe_integral_table_state(1, -1, 1, 1) gives it (ntab >= 5 nodes, r = 0
+ * first); zpar, bg_*, Omega_m, H0h and panels as for coffe_flat_lensing_multipoles;
+ * mu in [-1, 1].  Synchronous on the default engine.
+ */
+int coffe_flat_lensing_corrfunc(int nz, const double *zpar, int nmu, const double *mu, int nsep, const double *sep,
+                                int nb, const double *bg_z, const double *bg_chi, const double *bg_D1, int ntab,
+                                const double *tab_x, const double *tab_y, double Omega_m, double H0h, int want_ll,
+                                int want_dl, int panels, double *out);
 
 /* synchronise the engine's stream */
 int coffe_engine_synchronize(coffe_engine *e);

@@ -25,9 +25,9 @@ struct coffe_interpolation;
  * times r^(n-l) when n >= l.  The FFTLog runs on the spectrum's spline samples
  * (twofast_1bessel with nu = n, r0 = 1, k0 = kmin = x_min, kmax = x_max), the
  * small-r nodes are QAG-GK61 integrals over [x_min, x_max] of their Akima spline
- * (the reference's default spline type).  Integer states only (state_n = state_l =
- * 0, COFFE_INTEGER): the half-integer ones (flat-sky lensing) return
- * COFFE_EUNSUPPORTED.  Returns 0 or a COFFE_E* code (the reference exits instead);
+ * (the reference's default spline type).  Both states COFFE_INTEGER (0) or both
+ * COFFE_HALF_INTEGER (1, the flat-sky lensing tables: coffe_integral_table_state);
+ * mixed states return COFFE_EUNSUPPORTED.  Returns 0 or a COFFE_E* code (the reference exits instead);
  * on error nothing is allocated.
  */
 int coffe_integrals_renormalizable(double **output_x, double **output_y, const size_t output_len,

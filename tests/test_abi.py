@@ -55,21 +55,26 @@ class _GslSplineView:
         return Interp(ctypes.pointer(sp), None), sp
 
 
-def test_integrals_renormalizable_rejects_half_integer_states():
-    """The drop-in of integrals.c:459-573 returns COFFE_EUNSUPPORTED for half-integer
-    states (flat-sky lensing) before any device work, and allocates nothing."""
+def test_integrals_renormalizable_rejects_bad_states():
+    """The drop-in of integrals.c:459-573 takes both states integer or both half-integer
+    (the flat-sky lensing tables); mixed states return COFFE_EUNSUPPORTED and a
+    half-integer state with an even raw value COFFE_EINVAL, before any device work and
+    with nothing allocated."""
     import numpy as np
     from coffe_amd._lib import load_library
     L = load_library()
     k = np.logspace(-4, 1, 64)
     pk = k / (1 + k ** 2)
     interp, keep = _GslSplineView.make(k, pk)
-    ox, oy = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_double)()
-    n_out = ctypes.c_size_t(0)
     f = L.coffe_integrals_renormalizable
     f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
-    rc = f(ctypes.byref(ox), ctypes.byref(oy), 1024, ctypes.byref(n_out), ctypes.byref(interp), 1, -1, 1, 1,
-           float(k[0]), float(k[-1]))
-    assert rc == 4
-    assert not ox and not oy and n_out.value == 0
+    EINVAL, EUNSUPPORTED = 1, 4
+    for n, l, sn, sl, want in ((1, -1, 1, 0, EUNSUPPORTED), (1, 1, 0, 1, EUNSUPPORTED), (2, -1, 1, 1, EINVAL),
+                               (1, 2, 1, 1, EINVAL), (1, -3, 1, 1, EINVAL)):
+        ox, oy = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_double)()
+        n_out = ctypes.c_size_t(0)
+        rc = f(ctypes.byref(ox), ctypes.byref(oy), 1024, ctypes.byref(n_out), ctypes.byref(interp), n, l, sn, sl,
+               float(k[0]), float(k[-1]))
+        assert rc == want, (n, l, sn, sl, rc)
+        assert not ox and not oy and n_out.value == 0

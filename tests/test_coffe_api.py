@@ -79,9 +79,9 @@ def test_lensing_paths_not_built_are_refused():
     c.has_d2 = True
     with pytest.raises(NotImplementedError):          # d2-lensing: single-integrated
         c.compute_multipoles_bulk()
-    c.has_d2 = False
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(NotImplementedError):          # the same refusals for the correlation function
         c.compute_corrfunc_bulk()
+    c.has_d2 = False
     with pytest.raises(NotImplementedError):
         c.compute_average_multipoles_bulk()
 

@@ -1,8 +1,8 @@
 """Flat-sky lensing multipoles on the GPU (coffe_flat_lensing_multipoles and the
 Coffe API), against the reference's fixtures the way its own tests hold them
 (tests/test_coffe.py:328-380: rtol 5e-4) and against the oracle (oracle/lensing.py)
-on the same tables: the kernel's 128-panel Gauss-Legendre x integral against the
-oracle's 2048-panel one to 1e-9,
+on the same tables (multipoles and correlation function): the kernel's 128-panel
+Gauss-Legendre x integral against the oracle's 2048-panel one to 1e-9,
 the closed-form density-lensing term to 1e-12, the half-integer tables of the GPU
 FFTLog against the oracle's to 1e-9."""
 import os
@@ -82,7 +82,8 @@ def test_kernel_vs_oracle_on_shared_tables(engine):
     zs = [0.4, 1.0, 1.7]
     s1, s2, b1, b2 = 0.3, -0.1, 1.4, 0.9
     ll = {l: O.integral_table(1, l, k, pk, N) for l in (0, 2, 4)}
-    dl = {(l, j): OL.half_table(l - 2 * j + 3, l - 2 * j + 1, k, pk, N) for l in (0, 2, 4) for j in range(l // 2 + 1)}
+    dl = {(l, j): OL.half_table(l - 2 * j + 3, l - 2 * j + 1, k, pk, N, small_r=False) for l in (0, 2, 4)
+          for j in range(l // 2 + 1)}
     zpar = [[z, float(akima(bg["z"], bg["chi"], [z])[0]), float(akima(bg["z"], bg["D1"], [z])[0]), s1, s2, b1, b2]
             for z in zs]
     Om = 0.26 + 0.048
@@ -108,20 +109,74 @@ def test_kernel_vs_oracle_on_shared_tables(engine):
 
 
 def test_tables_vs_oracle(engine):
-    """The GPU FFTLog's half-integer tables and the (1, l) tables against the oracle's."""
+    """The GPU's half-integer tables (FFTLog with half-integer l and nu, small-r nodes by
+    QAG-GK61 of J_{l+1/2}) and the (1, l) tables against the oracle's: the FFT nodes to
+    1e-9, the 43 small-r nodes to 1e-6 of their largest (the GPU's quadrature runs to
+    the reference's epsrel 1e-5, the oracle's to 1e-9)."""
     from coffe_amd.engine import half_integer_table, integral_table
     k, pk = _pk()
     N = 2048
-    for n2, l2 in ((3, 1), (5, 3), (7, 5)):
+    for n2, l2 in ((1, -1), (3, 1), (5, 3), (7, 5)):
+        print(f"half table ({n2}/2, {l2}/2)", flush=True)
         x, y = half_integer_table(n2, l2, k, pk, N)
         xo, yo = OL.half_table(n2, l2, k, pk, N)
         assert np.allclose(x, xo, rtol=1e-14, atol=0)
-        assert np.max(np.abs(y - yo)) <= 1e-9 * np.max(np.abs(yo)), (n2, l2)
+        assert np.max(np.abs(y[43:] - yo[43:])) <= 1e-9 * np.max(np.abs(yo[43:])), (n2, l2)
+        assert np.max(np.abs(y[:43] - yo[:43])) <= 1e-6 * np.max(np.abs(yo[:43])), (n2, l2, y[:43] - yo[:43])
     for l in (0, 2, 4):
         x, y = integral_table(1, l, k, pk, N)
         xo, yo = O.integral_table(1, l, k, pk, N)
         assert np.allclose(x, xo, rtol=1e-14, atol=0)
         assert np.max(np.abs(y - yo)) <= 1e-9 * np.max(np.abs(yo)), l
+
+
+MUS = [0.0, 0.2, 0.5, 0.8, 0.95]
+
+
+def test_lensing_lensing_corrfunc_vs_reference(engine):
+    """tests/test_corrfunc.c:360-420 of the reference through the Python API: the
+    flat-sky lensing-lensing correlation function at five mu."""
+    from coffe_amd.coffe import Coffe
+    x0, _ = np.loadtxt(os.path.join(GOLD, "benchmark_flatsky_lensing_lensing_corrfunc0.dat")).T
+    cosmo = Coffe(sep=x0 / H, mu=MUS, has_density=False, has_lensing=True, has_flatsky_nonlocal=True)
+    cosmo.set_power_spectrum_linear(*_pk())
+    print("flat lensing-lensing corrfunc via Coffe", flush=True)
+    result = cosmo.compute_corrfunc_bulk()
+    for i, mu in enumerate(MUS):
+        x, y = np.loadtxt(os.path.join(GOLD, f"benchmark_flatsky_lensing_lensing_corrfunc{i}.dat")).T
+        rows = [r for r in result if r.mu == mu]
+        assert np.allclose([r.r * H for r in rows], x)
+        assert np.allclose([r.value for r in rows], y, rtol=5e-4), (mu, [r.value for r in rows], y)
+
+
+def test_corrfunc_kernel_vs_oracle(engine):
+    """Both correlation-function terms on the oracle's (1/2, -1/2) table, 2 redshifts,
+    mu including 0 and +-1 (the table's r = 0 node), with unequal biases."""
+    from coffe_amd import background as B
+    from coffe_amd.engine import akima, flat_lensing_corrfunc
+    k, pk = _pk()
+    bg = B.compute(omega_cdm=0.26, omega_baryon=0.048, h=H)
+    table = OL.half_table(1, -1, k, pk, 4096)
+    mus = [-1.0, -0.6, 0.0, 0.3, 0.9, 1.0]
+    seps = np.array([5.0, 40.0, 220.0])
+    zs = [0.5, 1.3]
+    s1, s2, b1, b2 = 0.3, -0.1, 1.4, 0.9
+    zpar = [[z, float(akima(bg["z"], bg["chi"], [z])[0]), float(akima(bg["z"], bg["D1"], [z])[0]), s1, s2, b1, b2]
+            for z in zs]
+    Om = 0.26 + 0.048
+    print("flat_lensing_corrfunc kernel", flush=True)
+    got_ll = flat_lensing_corrfunc(zpar, mus, seps, bg, table, Om, H0h, True, False)
+    got_dl = flat_lensing_corrfunc(zpar, mus, seps, bg, table, Om, H0h, False, True)
+    for iz, z in enumerate(zs):
+        for im, mu in enumerate(mus):
+            for ir, r in enumerate(seps):
+                want_ll = OL.flat_lensing_lensing_cf(table, bg, z, mu, r, s1, s2, Om, H0h)
+                want_dl = OL.flat_density_lensing_cf(table, bg, z, mu, r, s1, s2, b1, b2, Om, H0h)
+                assert abs(got_ll[iz, im, ir] - want_ll) <= 1e-9 * abs(want_ll), (z, mu, r)
+                if mu == 0.0:
+                    assert got_dl[iz, im, ir] == 0.0
+                else:
+                    assert abs(got_dl[iz, im, ir] - want_dl) <= 1e-12 * abs(want_dl), (z, mu, r)
 
 
 def test_local_and_lensing_terms_add(engine):
