@@ -116,6 +116,8 @@ static int done(int code)
 /* ---- integrals.c ------------------------------------------------------------------ */
 
 static const int STD_N[8] = {0, 0, 0, 1, 1, 2, 2, 3};
+/* the scale of the (1/2, -1/2) table (integrals.c:956-962) */
+#define REF_FLAT_TSCALE (sqrt(2.0 / M_PI) * 2.0 * M_PI * M_PI)
 static const int STD_L[8] = {0, 2, 4, 1, 3, 0, 2, 1};
 
 coffe_integral_t *coffe_find_integral(const coffe_integral_array_t *integral, int n, int l, int state_n,
@@ -200,12 +202,31 @@ int coffe_integrals_init(const coffe_parameters_t *par, const coffe_background_t
     if (st != COFFE_OK) return done(st);
     st = coffe_integrals_free(integral);
     if (st != COFFE_OK) return done(st);
-    if (par->flatsky_local_nonlocal || par->flatsky_nonlocal) return done(COFFE_EUNSUPPORTED);
     if (par->pk_type != 0 && par->midpoint_approximation) return done(COFFE_EUNSUPPORTED);
+    const int div = par->divergent != 0;
+    /* the flat-sky lensing tables (integrals.c:915-962, 1082-1130, 1246-1300): the
+       (1/2, -1/2) one, and for the multipoles the (1, l) and ((l-2k+3)/2, (l-2k+1)/2)
+       ones of the even multipoles (the odd ones' terms are zero and their tables unused) */
+    const int flat = par->flatsky_local_nonlocal || par->flatsky_nonlocal;
+    const int flat_mp = par->output_type == COFFE_REF_MULTIPOLES;
+    size_t extra = flat ? 1 : 0;
+    if (flat && flat_mp && (par->multipole_values_len && !par->multipole_values)) return done(COFFE_EINVAL);
+    for (size_t i = 0; flat && flat_mp && i < par->multipole_values_len; ++i) {
+        const int l = par->multipole_values[i];
+        if (l < 0) return done(COFFE_EINVAL);
+        if (l % 2) continue;
+        if (par->flatsky_nonlocal) {
+            if (l > 4) return done(COFFE_EUNSUPPORTED);   /* the engine's (n, l) tables stop at l = 4 */
+            ++extra;
+        }
+        if (par->flatsky_local_nonlocal) {
+            if (l + 3 > 9) return done(COFFE_EUNSUPPORTED);
+            extra += (size_t)(l / 2 + 1);
+        }
+    }
     nodes pk;
     if (nodes_of(&par->power_spectrum_norm, &pk) != COFFE_OK || par->bessel_bins <= 0) return done(COFFE_EINVAL);
-    const int div = par->divergent != 0;
-    const size_t count = 8 + (size_t)div;
+    const size_t count = 8 + (size_t)div + extra;
     const size_t N = (size_t)par->bessel_bins, len = N + 43;
     coffe_integral_t *arr = (coffe_integral_t *)calloc(count, sizeof(coffe_integral_t));
     double *x = (double *)malloc(sizeof(double) * len), *y = (double *)malloc(sizeof(double) * len);
@@ -268,6 +289,40 @@ int coffe_integrals_init(const coffe_parameters_t *par, const coffe_background_t
         free(r0);
         free(chi);
         free(ren);
+    }
+    if (st == COFFE_OK && flat) {
+        double *fx = (double *)malloc(sizeof(double) * len), *fy = (double *)malloc(sizeof(double) * len);
+        /* the tables (n, l, state) to add, each unless the array holds it already */
+        int *tn = (int *)malloc(sizeof(int) * 3 * extra), *tl = tn ? tn + extra : NULL, *ts = tn ? tn + 2 * extra : NULL;
+        size_t nt = 0;
+        if (!fx || !fy || !tn) st = COFFE_ENOMEM;
+        if (st == COFFE_OK) {
+            tn[nt] = 1, tl[nt] = -1, ts[nt] = 1, ++nt;
+            for (size_t i = 0; flat_mp && i < par->multipole_values_len; ++i) {
+                const int l = par->multipole_values[i];
+                if (l % 2) continue;
+                if (par->flatsky_nonlocal) tn[nt] = 1, tl[nt] = l, ts[nt] = 0, ++nt;
+                for (int k = 0; par->flatsky_local_nonlocal && k <= l / 2; ++k)
+                    tn[nt] = l - 2 * k + 3, tl[nt] = l - 2 * k + 1, ts[nt] = 1, ++nt;
+            }
+        }
+        for (size_t t = 0; t < nt && st == COFFE_OK; ++t) {
+            if (coffe_find_integral(integral, tn[t], tl[t], ts[t], ts[t])) continue;
+            st = coffe_integral_table_state(tn[t], tl[t], ts[t], ts[t], N, pk.x, pk.y, pk.n, par->k_min_norm,
+                                            par->k_max_norm, fx, fy);
+            if (st != COFFE_OK) break;
+            /* the (1/2, -1/2) table is kept as sqrt(2/pi) 2 pi^2 r I(r) (integrals.c:956-962) */
+            if (tn[t] == 1 && tl[t] == -1 && ts[t] == 1)
+                for (size_t i = 0; i < len; ++i) fy[i] *= REF_FLAT_TSCALE;
+            coffe_integral_t *I = &arr[integral->size++];
+            I->n = tn[t];
+            I->l = tl[t];
+            I->state_n = I->state_l = ts[t];
+            if (ops.init_spline(&I->result, fx, fy, len, par->interp_method) != 0) st = COFFE_EINVAL;
+        }
+        free(fx);
+        free(fy);
+        free(tn);
     }
     if (st != COFFE_OK) {
         coffe_integrals_free(integral);
@@ -548,6 +603,126 @@ static int average_eval(const local_ctx *lc, const coffe_spline_ops *ops, const 
     return st;
 }
 
+/* ---- the flat-sky lensing terms (signal.c:556-771) ------------------------------------ */
+
+/* the terms' values from the integral array's own tables, on the GPU
+   (coffe_flat_lensing_multipoles / coffe_flat_lensing_corrfunc): lensing-lensing
+   (want_ll) and density-lensing (want_dl), as signal.c adds them (the multipoles times
+   2 pi^2); corrfunc: at mus[nl] in place of the multipoles ls[nl] */
+static int flat_eval(const coffe_parameters_t *par, const coffe_background_t *bg,
+                     const coffe_integral_array_t *integral, int corrfunc, int want_ll, int want_dl, int nz,
+                     const double *zs, int nl, const int *ls, const double *mus, int nsep, const double *seps,
+                     double *out)
+{
+    if (!bg || !integral) return COFFE_EINVAL;
+    if (par->interp_method != COFFE_REF_INTERP_AKIMA || par->pk_type != 0) return COFFE_EUNSUPPORTED;
+    coffe_spline_ops ops;
+    int st = spline_ops(&ops);
+    if (st != COFFE_OK) return st;
+    nodes bgc, bgd;
+    if (nodes_of(&bg->comoving_distance, &bgc) != COFFE_OK || nodes_of(&bg->D1, &bgd) != COFFE_OK) return COFFE_EINVAL;
+    if (!same_nodes(&bgc, &bgd)) return COFFE_EUNSUPPORTED;
+    double *zpar = (double *)malloc(sizeof(double) * 7 * (size_t)nz);
+    if (!zpar) return COFFE_ENOMEM;
+    for (int i = 0; i < nz; ++i) {
+        double *zp = zpar + 7 * i;
+        zp[0] = zs[i];
+        zp[1] = ops.interp_spline(&bg->comoving_distance, zs[i]);
+        zp[2] = ops.interp_spline(&bg->D1, zs[i]);
+        zp[3] = ops.interp_spline(&par->magnification_bias1, zs[i]);
+        zp[4] = ops.interp_spline(&par->magnification_bias2, zs[i]);
+        zp[5] = ops.interp_spline(&par->galaxy_bias1, zs[i]);
+        zp[6] = ops.interp_spline(&par->galaxy_bias2, zs[i]);
+    }
+    const double Om = par->Omega0_cdm + par->Omega0_baryon, H0h = REF_H0 * par->h;
+    if (corrfunc) {
+        /* the (1/2, -1/2) table, kept scaled by sqrt(2/pi) 2 pi^2 in the array */
+        const coffe_integral_t *I = coffe_find_integral(integral, 1, -1, 1, 1);
+        nodes t;
+        if (!I || nodes_of(&I->result, &t) != COFFE_OK) {
+            free(zpar);
+            return COFFE_EINVAL;
+        }
+        double *ty = (double *)malloc(sizeof(double) * t.n);
+        if (!ty) st = COFFE_ENOMEM;
+        for (size_t i = 0; st == COFFE_OK && i < t.n; ++i) ty[i] = t.y[i] / REF_FLAT_TSCALE;
+        if (st == COFFE_OK)
+            st = coffe_flat_lensing_corrfunc(nz, zpar, nl, mus, nsep, seps, (int)bgc.n, bgc.x, bgc.y, bgd.y, (int)t.n,
+                                             t.x, ty, Om, H0h, want_ll, want_dl, 128, out);
+        free(ty);
+        free(zpar);
+        return st;
+    }
+    /* the multipoles: the tables of every even multipole, concatenated */
+    int64_t *off = (int64_t *)calloc((size_t)nl * (1 + CE_FL_MAXK), sizeof(int64_t));
+    int *lens = (int *)calloc((size_t)nl * (1 + CE_FL_MAXK), sizeof(int));
+    const nodes **tabs = (const nodes **)calloc((size_t)nl * (1 + CE_FL_MAXK), sizeof(nodes *));
+    nodes *store = (nodes *)calloc((size_t)nl * (1 + CE_FL_MAXK), sizeof(nodes));
+    if (!off || !lens || !tabs || !store) st = COFFE_ENOMEM;
+    int64_t total = 0;
+    for (int il = 0; il < nl && st == COFFE_OK; ++il) {
+        const int l = ls[il];
+        if (l % 2) continue;
+        for (int slot = 0; slot < 1 + CE_FL_MAXK && st == COFFE_OK; ++slot) {
+            const int k = slot - 1;
+            const coffe_integral_t *I = NULL;
+            if (slot == 0 && want_ll) I = coffe_find_integral(integral, 1, l, 0, 0);
+            else if (slot > 0 && want_dl && k <= l / 2) I = coffe_find_integral(integral, l - 2 * k + 3, l - 2 * k + 1, 1, 1);
+            else continue;
+            const size_t q = (size_t)il * (1 + CE_FL_MAXK) + (size_t)slot;
+            if (!I || nodes_of(&I->result, &store[q]) != COFFE_OK) {
+                st = COFFE_EINVAL;
+                break;
+            }
+            tabs[q] = &store[q];
+            off[q] = total;
+            lens[q] = (int)store[q].n;
+            total += (int64_t)store[q].n;
+        }
+    }
+    double *tx = NULL, *ty = NULL;
+    int64_t *ll_off = NULL, *dl_off = NULL;
+    int *ll_len = NULL, *dl_len = NULL;
+    if (st == COFFE_OK) {
+        tx = (double *)malloc(sizeof(double) * (size_t)(total > 0 ? total : 1));
+        ty = (double *)malloc(sizeof(double) * (size_t)(total > 0 ? total : 1));
+        ll_off = (int64_t *)calloc((size_t)nl, sizeof(int64_t));
+        ll_len = (int *)calloc((size_t)nl, sizeof(int));
+        dl_off = (int64_t *)calloc((size_t)nl * CE_FL_MAXK, sizeof(int64_t));
+        dl_len = (int *)calloc((size_t)nl * CE_FL_MAXK, sizeof(int));
+        if (!tx || !ty || !ll_off || !ll_len || !dl_off || !dl_len) st = COFFE_ENOMEM;
+    }
+    for (int il = 0; il < nl && st == COFFE_OK; ++il)
+        for (int slot = 0; slot < 1 + CE_FL_MAXK; ++slot) {
+            const size_t q = (size_t)il * (1 + CE_FL_MAXK) + (size_t)slot;
+            if (!tabs[q]) continue;
+            memcpy(tx + off[q], tabs[q]->x, sizeof(double) * tabs[q]->n);
+            memcpy(ty + off[q], tabs[q]->y, sizeof(double) * tabs[q]->n);
+            if (slot == 0) {
+                ll_off[il] = off[q];
+                ll_len[il] = lens[q];
+            } else {
+                dl_off[(size_t)il * CE_FL_MAXK + slot - 1] = off[q];
+                dl_len[(size_t)il * CE_FL_MAXK + slot - 1] = lens[q];
+            }
+        }
+    if (st == COFFE_OK)
+        st = coffe_flat_lensing_multipoles(nz, zpar, nl, ls, nsep, seps, (int)bgc.n, bgc.x, bgc.y, bgd.y, total, tx, ty,
+                                           ll_off, ll_len, dl_off, dl_len, Om, H0h, want_ll, want_dl, 128, out);
+    free(off);
+    free(lens);
+    free(tabs);
+    free(store);
+    free(tx);
+    free(ty);
+    free(ll_off);
+    free(ll_len);
+    free(dl_off);
+    free(dl_len);
+    free(zpar);
+    return st;
+}
+
 /* ---- signal.c ------------------------------------------------------------------------ */
 
 double coffe_integrate(coffe_parameters_t *par, coffe_background_t *bg, coffe_integral_array_t *integral,
@@ -585,14 +760,41 @@ double coffe_integrate(coffe_parameters_t *par, coffe_background_t *bg, coffe_in
         }
         break;
     }
-    case COFFE_REF_SINGLE_INTEGRATED:
-        /* signal.c:556-582: lensing, g4 or g5 with a local term */
-        if (nonlocal && local) st = COFFE_EUNSUPPORTED;
+    case COFFE_REF_SINGLE_INTEGRATED: {
+        /* signal.c:556-683: lensing, g4 or g5 with a local term.  Only density-lensing in
+           flat sky (flatsky_local_nonlocal) is nonzero without Monte Carlo; lensing with
+           d2 / g1-g3, g4 / g5 with a local term and the full-sky terms are not built */
+        if (!(nonlocal && local)) break;
+        const int mc = (c->len && (c->d2 || c->g1 || c->g2 || c->g3)) || ((c->g4 || c->g5) && local) ||
+                       (c->len && !par->flatsky_local_nonlocal);
+        if (mc || flag_output == COFFE_REF_AVERAGE_MULTIPOLES) {
+            st = COFFE_EUNSUPPORTED;
+            break;
+        }
+        if (!c->den) break;   /* RSD- and Doppler-lensing are zero in flat sky */
+        if (flag_output == COFFE_REF_CORRFUNC)
+            st = flat_eval(par, bg, integral, 1, 0, 1, 1, &z_mean, 1, NULL, &mu, 1, &sep, &result);
+        else if (flag_output == COFFE_REF_MULTIPOLES)
+            st = flat_eval(par, bg, integral, 0, 0, 1, 1, &z_mean, 1, &l, NULL, 1, &sep, &result);
         break;
-    case COFFE_REF_DOUBLE_INTEGRATED:
-        /* signal.c:685-692 */
-        if (nonlocal) st = COFFE_EUNSUPPORTED;
+    }
+    case COFFE_REF_DOUBLE_INTEGRATED: {
+        /* signal.c:685-771: lensing-lensing in flat sky (flatsky_nonlocal, not with only
+           the cross terms: an auto term) without Monte Carlo; g4, g5 and full-sky lensing
+           are not built */
+        if (!nonlocal) break;
+        if (c->g4 || c->g5 || (c->len && !par->flatsky_nonlocal && !par->only_cross_correlations) ||
+            flag_output == COFFE_REF_AVERAGE_MULTIPOLES) {
+            st = COFFE_EUNSUPPORTED;
+            break;
+        }
+        if (!c->len || par->only_cross_correlations) break;
+        if (flag_output == COFFE_REF_CORRFUNC)
+            st = flat_eval(par, bg, integral, 1, 1, 0, 1, &z_mean, 1, NULL, &mu, 1, &sep, &result);
+        else if (flag_output == COFFE_REF_MULTIPOLES)
+            st = flat_eval(par, bg, integral, 0, 1, 0, 1, &z_mean, 1, &l, NULL, 1, &sep, &result);
         break;
+    }
     default:
         st = COFFE_EINVAL;
     }
@@ -620,12 +822,24 @@ int coffe_average_multipoles_free(coffe_average_multipoles_array_t *ramp)
     return done(COFFE_OK);
 }
 
-/* the checks shared by the two multipole outputs */
+/* the checks shared by the two multipole outputs; *ll / *dl (may be NULL: refused)
+   receive the flat-sky lensing terms to add (signal.c:612-628, 704-736) */
 static int multipole_setup(const coffe_parameters_t *par, const coffe_background_t *bg,
-                           const coffe_integral_array_t *integral, coffe_spline_ops *ops, local_ctx *lc)
+                           const coffe_integral_array_t *integral, coffe_spline_ops *ops, local_ctx *lc, int *ll,
+                           int *dl)
 {
     const coffe_correlation_contributions *c = &par->correlation_contrib;
-    if (c->len || c->g4 || c->g5) return COFFE_EUNSUPPORTED;   /* integrated terms */
+    if (ll) *ll = 0;
+    if (dl) *dl = 0;
+    if (c->g4 || c->g5) return COFFE_EUNSUPPORTED;   /* integrated terms */
+    if (c->len) {
+        const int local = c->den || c->rsd || c->d1;
+        if (!ll || c->d2 || c->g1 || c->g2 || c->g3 || (local && !par->flatsky_local_nonlocal) ||
+            (!par->only_cross_correlations && !par->flatsky_nonlocal))
+            return COFFE_EUNSUPPORTED;
+        *ll = !par->only_cross_correlations;
+        *dl = c->den != 0;
+    }
     if (!par->sep || !par->multipole_values) return COFFE_EINVAL;
     int st = spline_ops(ops);
     if (st != COFFE_OK) return st;
@@ -643,16 +857,26 @@ int coffe_multipoles_init(coffe_parameters_t *par, coffe_background_t *bg, coffe
     if (!par->z_mean || nz > INT32_MAX || ns > INT32_MAX || nl > INT32_MAX) return done(COFFE_EINVAL);
     coffe_spline_ops ops;
     local_ctx lc;
-    int st = multipole_setup(par, bg, integral, &ops, &lc);
+    int want_ll = 0, want_dl = 0;
+    int st = multipole_setup(par, bg, integral, &ops, &lc, &want_ll, &want_dl);
     if (st != COFFE_OK) return done(st);
     coffe_multipoles_t *arr = (coffe_multipoles_t *)malloc(sizeof(coffe_multipoles_t) * size);
     double *vals = (double *)malloc(sizeof(double) * size);
-    if (!arr || !vals) {
+    double *lens = (want_ll || want_dl) ? (double *)malloc(sizeof(double) * size) : NULL;
+    if (!arr || !vals || ((want_ll || want_dl) && !lens)) {
         free(arr);
         free(vals);
+        free(lens);
         return done(COFFE_ENOMEM);
     }
     st = local_eval_ls(&lc, &ops, 0, (int)nz, par->z_mean, (int)nl, par->multipole_values, (int)ns, par->sep, vals);
+    /* the flat-sky lensing terms, as multipoles.c adds the integrated outputs */
+    if (st == COFFE_OK && lens) {
+        st = flat_eval(par, bg, integral, 0, want_ll, want_dl, (int)nz, par->z_mean, (int)nl, par->multipole_values,
+                       NULL, (int)ns, par->sep, lens);
+        for (size_t q = 0; st == COFFE_OK && q < size; ++q) vals[q] += lens[q];
+    }
+    free(lens);
     if (st != COFFE_OK) {
         free(arr);
         free(vals);
@@ -686,7 +910,7 @@ int coffe_average_multipoles_init(coffe_parameters_t *par, coffe_background_t *b
         return done(COFFE_EINVAL);
     coffe_spline_ops ops;
     local_ctx lc;
-    int st = multipole_setup(par, bg, integral, &ops, &lc);
+    int st = multipole_setup(par, bg, integral, &ops, &lc, NULL, NULL);
     if (st == COFFE_OK && !bg) st = COFFE_EINVAL;
     if (st != COFFE_OK) return done(st);
     coffe_average_multipoles_t *arr = (coffe_average_multipoles_t *)malloc(sizeof(*arr) * size);

@@ -269,9 +269,15 @@ coffe_integral_t *coffe_find_integral(const coffe_integral_array_t *integral, in
  * grid of the renormalisation on [0, chi_max] (renormalization,
  * covariance_interpolation_method), chi_max = chi(max(z_mean + deltaz)) for CORRFUNC / MULTIPOLES output and
  * chi(max(zmax)) for AVERAGE_MULTIPOLES; other outputs build no grid (the reference's
- * grid would be degenerate there).  The flat-sky nonlocal tables (half-integer
- * states, flatsky_local_nonlocal / flatsky_nonlocal) and CLASS spectra (pk_type != 0
- * with midpoint_approximation) return COFFE_EUNSUPPORTED.  The array is freed first.
+ * grid would be degenerate there).  With flatsky_local_nonlocal or flatsky_nonlocal
+ * also the flat-sky lensing tables (integrals.c:915-962, 1082-1130, 1246-1300): the
+ * half-integer (1/2, -1/2) one, kept times sqrt(2/pi) 2 pi^2 as there, and for
+ * MULTIPOLES output the (1, l) tables (flatsky_nonlocal) and the half-integer
+ * ((l - 2k + 3)/2, (l - 2k + 1)/2) tables, k <= l/2 (flatsky_local_nonlocal), of the
+ * even multipoles (the odd multipoles' terms are zero and their tables unused; tables
+ * the array holds already are not repeated); lensing-lensing multipoles above l = 4
+ * and CLASS spectra (pk_type != 0 with midpoint_approximation) return
+ * COFFE_EUNSUPPORTED.  The array is freed first.
  */
 int coffe_integrals_init(const coffe_parameters_t *par, const coffe_background_t *bg,
                          coffe_integral_array_t *integral);
@@ -284,8 +290,14 @@ int coffe_integrals_free(coffe_integral_array_t *integral);
  * sep, mu)), MULTIPOLES (the multipole l at (z_mean, sep), a 256-node Gauss-Legendre mu
  * rule in place of the reference's adaptive one) or AVERAGE_MULTIPOLES (the multipole
  * averaged over [z_min, z_max], a 32-node Gauss-Legendre rule in redshift).  The
- * integrated terms: SINGLE_INTEGRATED / DOUBLE_INTEGRATED return 0 when the reference
- * would (no len / g4 / g5 contribution) and NaN with COFFE_EUNSUPPORTED otherwise.
+ * integrated terms (signal.c:556-771) for CORRFUNC and MULTIPOLES: SINGLE_INTEGRATED
+ * gives the flat-sky density-lensing term (len with den and flatsky_local_nonlocal;
+ * RSD- and Doppler-lensing are zero there), DOUBLE_INTEGRATED the flat-sky
+ * lensing-lensing term (len with flatsky_nonlocal, not with only_cross_correlations),
+ * both from the integral array's tables on the GPU (coffe_flat_lensing_multipoles /
+ * coffe_flat_lensing_corrfunc); they return 0 when the reference would (no len / g4 /
+ * g5 contribution) and NaN with COFFE_EUNSUPPORTED where the reference integrates by
+ * Monte Carlo (full-sky lensing, g4, g5, lensing with d2 / g1-g3, AVERAGE_MULTIPOLES).
  * Errors return NaN; coffe_engine_last_error() gives the code.
  */
 double coffe_integrate(coffe_parameters_t *par, coffe_background_t *bg, coffe_integral_array_t *integral,
@@ -295,9 +307,10 @@ double coffe_integrate(coffe_parameters_t *par, coffe_background_t *bg, coffe_in
 /*
  * coffe_multipoles_init (multipoles.h:47-52, multipoles.c:48-136): mp->array of
  * z_mean_len x multipole_values_len x sep_len records, z-major then l then separation,
- * each the local multipole (coffe_integrate NONINTEGRATED / MULTIPOLES) computed in one
- * batch on the GPU.  len, g4 or g5 (integrated terms) return COFFE_EUNSUPPORTED.  The
- * array is freed first.
+ * each the local multipole (coffe_integrate NONINTEGRATED / MULTIPOLES) plus, with len,
+ * the flat-sky lensing terms (SINGLE_ and DOUBLE_INTEGRATED as coffe_integrate gives
+ * them), computed in one batch on the GPU.  g4, g5 and the lensing terms that need Monte
+ * Carlo return COFFE_EUNSUPPORTED.  The array is freed first.
  */
 int coffe_multipoles_init(coffe_parameters_t *par, coffe_background_t *bg, coffe_integral_array_t *integral,
                           coffe_multipoles_array_t *mp);

@@ -288,6 +288,7 @@ class Program:
         c = par.correlation_contrib
         c.den, c.rsd, c.d1 = int(p["has_density"]), int(p["has_rsd"]), int(p["has_d1"])
         c.d2, c.g1, c.g2, c.g3 = (int(p[k]) for k in ("has_d2", "has_g1", "has_g2", "has_g3"))
+        c.len = int(p["has_lensing"])
         par.bessel_bins = int(p["integrals_fftlog_sampling"] if bessel_bins is None else bessel_bins)
         par.Omega0_cdm, par.Omega0_baryon, par.Omega0_gamma = p["omega_cdm"], p["omega_baryon"], p["omega_gamma"]
         par.Omega0_m = p["omega_cdm"] + p["omega_baryon"]
@@ -329,6 +330,8 @@ class Program:
         par.covariance_interpolation_method = 2
         par.covariance_window = int(p["has_binned_covariance"])
         par.flatsky_local = int(p["has_flatsky_local"])
+        par.flatsky_nonlocal = int(p["has_flatsky_nonlocal"])
+        par.flatsky_local_nonlocal = int(p["has_flatsky_local_nonlocal"])
         par.only_cross_correlations = int(p["has_only_cross_correlations"])
         par.mu, par.mu_len = self._arr(p["mu"])
         par.integration_1d_type, par.integration_1d_prec = 1, 1e-5

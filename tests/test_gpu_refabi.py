@@ -254,3 +254,77 @@ def test_average_covariance_init(lib):
         assert abs(a.value - b.value * v_mp / v_avg) <= 1e-10 * abs(b.value * v_mp / v_avg) + 1e-300
     L.coffe_covariance_free(ctypes.byref(cm))
     L.coffe_average_covariance_free(ctypes.byref(ca2))
+
+
+@pytest.mark.parametrize("name,flags", [
+    ("lensing_lensing", dict(has_lensing=True, has_flatsky_nonlocal=True)),
+    ("density_lensing", dict(has_density=True, has_lensing=True, has_flatsky_nonlocal=True,
+                             has_flatsky_local_nonlocal=True, has_only_cross_correlations=True)),
+])
+def test_flat_lensing_multipoles_init(lib, name, flags):
+    """The reference's flat-sky lensing runs (test_multipoles.c:393-501) through
+    coffe_integrals_init (which adds the (1, l), the half-integer and the (1/2, -1/2)
+    tables to the host's splines) and coffe_multipoles_init: the Python API to 1e-12, the
+    reference's fixtures to 5e-4; one point of coffe_integrate (SINGLE_INTEGRATED for
+    density-lensing, DOUBLE_INTEGRATED for lensing-lensing) equals the batch."""
+    from coffe_amd.coffe import Coffe
+    L, host = lib
+    cosmo = Coffe(sep=np.array([20.0, 40.0, 100.0, 150.0]) / H, l=[0, 2, 4])
+    cosmo.set_power_spectrum_linear(*_pk())
+    cosmo.reset_contributions()
+    cosmo.set_parameters(**flags)
+    prog = R.Program(host, cosmo)
+    print(f"coffe_multipoles_init flat {name}", flush=True)
+    ia = _integrals(L, prog)
+    mp = R.MultipolesArray()
+    assert L.coffe_multipoles_init(ctypes.byref(prog.par), ctypes.byref(prog.bg), ctypes.byref(ia),
+                                   ctypes.byref(mp)) == 0
+    recs = [mp.array[i] for i in range(mp.size)]
+    py = cosmo.compute_multipoles_bulk()
+    assert len(recs) == len(py) == 12
+    for a, b in zip(recs, py):
+        assert (a.coords.z_mean, a.coords.separation, a.coords.l) == (b.z, b.r, b.l)
+        assert abs(a.value - b.value) <= 1e-12 * abs(b.value), (name, b.l, b.r, a.value, b.value)
+    for mpl in (0, 2, 4):
+        x, y = np.loadtxt(os.path.join(GOLD, f"benchmark_flatsky_{name}_multipoles{mpl}.dat")).T
+        got = np.array([r.value for r in recs if r.coords.l == mpl])
+        assert np.allclose(got, y, rtol=5e-4), (name, mpl)
+    kind = R.DOUBLE_INTEGRATED if name == "lensing_lensing" else R.SINGLE_INTEGRATED
+    for rec in (recs[1], recs[6], recs[11]):
+        one = L.coffe_integrate(ctypes.byref(prog.par), ctypes.byref(prog.bg), ctypes.byref(ia), rec.coords.z_mean,
+                                0.0, 0.0, rec.coords.separation, 0.0, rec.coords.l, kind, R.MULTIPOLES)
+        assert abs(one - rec.value) <= 1e-12 * abs(rec.value), (name, rec.coords.l, one, rec.value)
+    # the lensing terms are not in the other integration kind
+    other = R.SINGLE_INTEGRATED if kind == R.DOUBLE_INTEGRATED else R.DOUBLE_INTEGRATED
+    rec = recs[1]
+    assert L.coffe_integrate(ctypes.byref(prog.par), ctypes.byref(prog.bg), ctypes.byref(ia), rec.coords.z_mean, 0.0,
+                             0.0, rec.coords.separation, 0.0, rec.coords.l, other, R.MULTIPOLES) == 0.0
+    L.coffe_multipoles_free(ctypes.byref(mp))
+    L.coffe_integrals_free(ctypes.byref(ia))
+
+
+def test_flat_lensing_corrfunc_integrate(lib):
+    """The reference's flat-sky lensing-lensing correlation function test
+    (test_corrfunc.c:360-420): NONINTEGRATED + SINGLE_INTEGRATED + DOUBLE_INTEGRATED of
+    coffe_integrate at five mu against its fixtures (5e-4) and the Python API (1e-12)."""
+    from coffe_amd.coffe import Coffe
+    L, host = lib
+    mus = [0.0, 0.2, 0.5, 0.8, 0.95]
+    x0, _ = np.loadtxt(os.path.join(GOLD, "benchmark_flatsky_lensing_lensing_corrfunc0.dat")).T
+    cosmo = Coffe(sep=x0 / H, mu=mus, has_density=False, has_lensing=True, has_flatsky_nonlocal=True)
+    cosmo.set_power_spectrum_linear(*_pk())
+    prog = R.Program(host, cosmo, output_type=R.CORRFUNC)
+    print("coffe_integrate flat lensing corrfunc", flush=True)
+    ia = _integrals(L, prog)
+    py = {(round(r.mu, 6), round(r.r, 9)): r.value for r in cosmo.compute_corrfunc_bulk()}
+    for i, mu in enumerate(mus):
+        x, y = np.loadtxt(os.path.join(GOLD, f"benchmark_flatsky_lensing_lensing_corrfunc{i}.dat")).T
+        got = []
+        for r in x / H:
+            got.append(sum(L.coffe_integrate(ctypes.byref(prog.par), ctypes.byref(prog.bg), ctypes.byref(ia), 1.0,
+                                             0.0, 0.0, float(r), mu, 0, kind, R.CORRFUNC)
+                           for kind in (R.NONINTEGRATED, R.SINGLE_INTEGRATED, R.DOUBLE_INTEGRATED)))
+            want = py[(round(mu, 6), round(float(r), 9))]
+            assert abs(got[-1] - want) <= 1e-12 * abs(want), (mu, r, got[-1], want)
+        assert np.allclose(got, y, rtol=5e-4), (mu, got, y)
+    L.coffe_integrals_free(ctypes.byref(ia))

@@ -147,7 +147,8 @@ def test_integrate_flag_gating(lib):
         par.b_derivative = 1
         assert np.isnan(lib.coffe_integrate(ctypes.byref(par), *args, R.NONINTEGRATED, R.MULTIPOLES))
         assert lib.coffe_engine_last_error() == EUNSUPPORTED
-        # lensing in a multipoles run, the flat-sky nonlocal tables, bilinear covariance interpolation
+        # g4 in a multipoles run, flat-sky lensing-lensing tables above l = 4, bilinear covariance
+        # interpolation
         par.b_derivative = 0
         par.correlation_contrib.g4 = 1
         z = np.array([1.0])
@@ -160,9 +161,13 @@ def test_integrate_flag_gating(lib):
         assert lib.coffe_multipoles_init(ctypes.byref(par), None, None, ctypes.byref(mp)) == EUNSUPPORTED
         assert mp.size == 0 and not mp.array
         par.flatsky_nonlocal = 1
+        par.output_type = R.MULTIPOLES
+        l6 = np.array([0, 6], dtype=np.int32)
+        par.multipole_values, par.multipole_values_len = l6.ctypes.data_as(R.IP), 2
         ia = R.IntegralArray()
         assert lib.coffe_integrals_init(ctypes.byref(par), None, ctypes.byref(ia)) == EUNSUPPORTED
         assert ia.size == 0 and not ia.array
+        par.multipole_values, par.multipole_values_len = ls.ctypes.data_as(R.IP), 1
         par.output_type = R.COVARIANCE_MULTIPOLES
         par.covariance_integration_method = 1        # STANDARD: computed by FFTLog, accepted
         par.covariance_interpolation_method = 1      # bilinear: not offered
