@@ -31,6 +31,14 @@ def _close(a, b, tol):
     return np.max(np.abs(a - b)) <= tol * np.max(np.abs(b))
 
 
+# Tolerances of the three vjp outputs (tables, multipoles, pk gradient) relative to
+# their largest element.  A different radix order changes every FFT's rounding; the
+# pk gradient sums the adjoint transform against spectra spanning twelve decades, and
+# over all orders of the 1024-point transform it moves by up to 1.0e-12 of its maximum
+# (the tables and multipoles by at most 2.7e-15).
+TOLS = (1e-12, 1e-12, 4e-12)
+
+
 def test_set_radix_orders(engine):
     from coffe_amd import EngineError
     from coffe_amd.autotune import set_radix
@@ -42,8 +50,8 @@ def test_set_radix_orders(engine):
     set_radix(b, 512, (8, 8, 8))
     set_radix(b, 1024, (2, 8, 8, 8))
     got = [x.cpu().numpy() for x in b.vjp(k, p, A, gout)]
-    for x, y in zip(got, ref):
-        assert _close(x, y, 1e-12)
+    for x, y, tol in zip(got, ref, TOLS):
+        assert _close(x, y, tol)
     with pytest.raises(EngineError):
         set_radix(b, 1024, (8, 8, 8, 4))        # product 2048
     with pytest.raises(EngineError):
@@ -64,6 +72,6 @@ def test_autotune_keeps_results(engine):
     assert set(rep["radix"]) == {512, 1024, 4096}
     assert rep["ms"] > 0 and len(rep["layouts"]) == 3
     got = [x.cpu().numpy() for x in b.vjp(k, p, A, gout)]
-    for x, y in zip(got, ref):
-        assert _close(x, y, 1e-12)
+    for x, y, tol in zip(got, ref, TOLS):
+        assert _close(x, y, tol)
     b.close()
