@@ -88,6 +88,8 @@ def load_library():
     L.coffe_flat_lensing_multipoles.argtypes = [i, dp, i, ip, i, dp, i, dp, dp, dp, ctypes.c_int64, dp, dp, i64ptr, ip,
                                                 i64ptr, ip, d, d, i, i, i, dp]
     L.coffe_flat_lensing_corrfunc.argtypes = [i, dp, i, dp, i, dp, i, dp, dp, dp, i, dp, dp, d, d, i, i, i, dp]
+    L.coffe_integrated_corrfunc.argtypes = [vp, ctypes.c_uint, i, ip, ip, ip, dp, dp, i, dp, dp, i, dp, i, dp, i, dp,
+                                            ip, dp]
     L.coffe_divergent_tables.argtypes = [dp, dp, sz, d, d, dp, sz, dp, sz, dp, dp, dp, ctypes.POINTER(i),
                                          ctypes.POINTER(i)]
     L.coffe_batch_set_radix.argtypes = [vp, i, i, ctypes.POINTER(i)]
@@ -129,7 +131,7 @@ EXPORTED = [
     "coffe_mu_weights", "coffe_mu_weights_terms", "coffe_mu_weights_ex", "coffe_terms_of", "coffe_integral_table", "coffe_integral_table_state", "coffe_divergent_tables", "coffe_divergent_tables_device", "coffe_batch_set_radix", "coffe_batch_small_r",
     "coffe_cov_plan_create", "coffe_cov_plan_destroy", "coffe_cov_plan_rows", "coffe_cov_plan_pairs",
     "coffe_cov_integrals", "coffe_cov_integrals_vjp", "coffe_cov_assemble", "coffe_cov_assemble_vjp", "two_sph_bessel", "two_sph_bessel_status",
-    "coffe_mu_weights_rule", "coffe_mu_weights_jvp", "coffe_fft_any", "coffe_flat_lensing_multipoles", "coffe_flat_lensing_corrfunc", "coffe_refabi_set_spline_ops", "coffe_find_integral", "coffe_integrals_init",
+    "coffe_mu_weights_rule", "coffe_mu_weights_jvp", "coffe_fft_any", "coffe_flat_lensing_multipoles", "coffe_flat_lensing_corrfunc", "coffe_integrated_corrfunc", "coffe_refabi_set_spline_ops", "coffe_find_integral", "coffe_integrals_init",
     "coffe_integrals_free", "coffe_integrate", "coffe_multipoles_init", "coffe_multipoles_free",
     "coffe_average_multipoles_init", "coffe_average_multipoles_free", "coffe_covariance_init",
     "coffe_covariance_free", "coffe_average_covariance_free", "coffe_covariance_find", "covariance_coefficient",

@@ -12,7 +12,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("COFFE_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["fftlog_kernels.hip", "mu_kernels.hip", "quad_kernels.hip", "cov_kernels.hip", "noalloc.hip",
-               "anyfft.hip", "lensing_kernels.hip"]
+               "anyfft.hip", "lensing_kernels.hip", "integrated_kernels.hip"]
 C_SOURCES = ["engine.c", "background.c", "interp.c", "covariance.c", "refabi.c"]
 
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")

@@ -9,11 +9,11 @@ are validated as the reference validates them (TypeError / ValueError), lazily
 computed pieces are dropped when a parameter they depend on changes
 (coffe.pyx:948-993), and results come back as Multipoles / Covariance / ... records in
 the reference's order (multipoles.c:76-86: z, l, r; covariance.c:1046-1060: z, l1, r1,
-l2, r2).  The terms are density, RSD, Doppler-1, d2 and g1-g3 with their cross terms.
-With has_lensing the flat-sky lensing-lensing (has_flatsky_nonlocal) and
-density-lensing (has_flatsky_local_nonlocal) multipoles are added; the full-sky
-lensing integrals, g4, g5, massive neutrinos and the CLASS / nonlinear spectra are not
-part of this engine and raise NotImplementedError; the
+l2, r2).  The terms are density, RSD, Doppler-1, d2 and g1-g3 with their cross terms, and the
+integrated ones: lensing, g4 and g5 with each other and with the local terms (full
+sky, or the flat-sky lensing forms with has_flatsky_nonlocal /
+has_flatsky_local_nonlocal).  Massive neutrinos and the CLASS / nonlinear spectra are
+not part of this engine and raise NotImplementedError; the
 parameters of the CLASS interface and of the reference's CPU integrators are kept and
 written to settings files but change nothing here.
 """
@@ -27,7 +27,7 @@ from . import background as _background
 from .representation import AverageCovariance, AverageMultipoles, Corrfunc, Covariance, Multipoles
 
 TABLES = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]   # integrals.c:599-608
-_UNSUPPORTED = ("has_g4", "has_g5")
+_UNSUPPORTED = ()
 _DG = ("d2", "g1", "g2", "g3")
 COFFE_H0 = 1 / 2997.92458   # h/Mpc (common.h)
 
@@ -151,7 +151,7 @@ class Coffe:
         sep=5.0 + 3.0 * np.arange(100), l=[0, 2, 4], z_mean=[1.0], deltaz=[0.2], fsky=[1.0], pixelsize=[10.0],
         z_min=[0.9], z_max=[1.1], mu=[0.5],
         number_density1=[1e-3], number_density2=[1e-3],
-        has_density=True, has_rsd=False, has_d1=False, has_d2=False, has_g1=False, has_g2=False, has_g3=False,
+        has_density=True, has_rsd=False, has_d1=False, has_d2=False, has_g1=False, has_g2=False, has_g3=False, has_g4=False, has_g5=False,
         has_flatsky_local=False, has_only_cross_correlations=False,
         has_lensing=False, has_flatsky_nonlocal=False, has_flatsky_local_nonlocal=False,
         integrals_fftlog_sampling=10000, covariance_integration_sampling=8000,
@@ -189,6 +189,7 @@ class Coffe:
         self._lens_cf = None
         self._tables = None
         self._divergent = None
+        self._divergent_r0 = None
         self._ren = None
         self._free_results()
 
@@ -198,6 +199,7 @@ class Coffe:
         self._lens_cf = None
         self._tables = None
         self._divergent = None
+        self._divergent_r0 = None
         self._ren = None
         self._free_results()
 
@@ -374,12 +376,15 @@ class Coffe:
     has_g1 = property(lambda s: s._p["has_g1"], lambda s, v: s._set_flag("has_g1", v))
     has_g2 = property(lambda s: s._p["has_g2"], lambda s, v: s._set_flag("has_g2", v))
     has_g3 = property(lambda s: s._p["has_g3"], lambda s, v: s._set_flag("has_g3", v))
+    # the integrated potential terms g4 (functions.c:3837-3851) and g5 (3853-3874) with
+    # their cross terms
+    has_g4 = property(lambda s: s._p["has_g4"], lambda s, v: s._set_flag("has_g4", v))
+    has_g5 = property(lambda s: s._p["has_g5"], lambda s, v: s._set_flag("has_g5", v))
     has_flatsky_local = property(lambda s: s._p["has_flatsky_local"],
                                  lambda s, v: s._set_flag("has_flatsky_local", v))
-    # lensing (signal.c:556-771): the flat-sky lensing-lensing term with
-    # has_flatsky_nonlocal and the flat-sky density-lensing term with
-    # has_flatsky_local_nonlocal are computed; the full-sky lensing integrals (Monte
-    # Carlo in the reference) raise NotImplementedError
+    # lensing (signal.c:556-771): full sky by the integrated-terms kernel, or the flat-sky
+    # lensing-lensing term with has_flatsky_nonlocal and the flat-sky density-lensing
+    # term with has_flatsky_local_nonlocal
     has_lensing = property(lambda s: s._p["has_lensing"], lambda s, v: s._set_flag("has_lensing", v))
     has_flatsky_nonlocal = property(lambda s: s._p["has_flatsky_nonlocal"],
                                     lambda s, v: s._set_flag("has_flatsky_nonlocal", v))
@@ -461,9 +466,10 @@ class Coffe:
         object.__setattr__(self, name, value)
 
     def reset_contributions(self):
-        """Sets every has_* contribution to False."""
-        for name in ("has_density", "has_rsd", "has_d1", "has_d2", "has_g1", "has_g2", "has_g3"):
-            setattr(self, name, False)
+        """Sets every writable has_* parameter to False (coffe.pyx:2702-2708)."""
+        for name in self._p:
+            if name.startswith("has_"):
+                setattr(self, name, False)
 
     # ---- inputs ----------------------------------------------------------------------
     def set_power_spectrum_linear(self, k, pk, z=0):
@@ -805,10 +811,14 @@ class Coffe:
             from .engine import divergent_tables
             k, pk = self._pk
             kmin, kmax = self._kwin()
-            sep, i40, _, _, status, _ = divergent_tables(k, pk, kmin=kmin, kmax=kmax, return_status=True)
+            sep, i40, r0, _, status, _ = divergent_tables(k, pk, kmin=kmin, kmax=kmax, return_status=True)
             if np.any(status[:len(sep)] == 1):
                 raise RuntimeError("I_0^4: an integral reached the interval cap")
             self._divergent = (sep, i40)
+            # the r = 0 renormalisation against separation, first node (0, 0) (integrals.c:829-835)
+            x0, y0 = np.array(sep, dtype=np.float64), np.array(r0, dtype=np.float64)
+            x0[0], y0[0] = 0.0, 0.0
+            self._divergent_r0 = (x0, y0)
         return self._divergent
 
     def integral(self, *, r, n, l):  # noqa: E741
@@ -833,14 +843,14 @@ class Coffe:
 
     def _check_contributions(self):
         if not any(self._p[k] for k in ("has_density", "has_rsd", "has_d1", "has_d2", "has_g1", "has_g2", "has_g3",
-                                        "has_lensing")):
+                                        "has_g4", "has_g5", "has_lensing")):
             raise ValueError("No contributions specified, you need to specify at least one of: "
-                             "'has_density', 'has_rsd', 'has_d1', 'has_d2', 'has_g1', 'has_g2', 'has_g3', "
-                             "'has_lensing'")
+                             "'has_density', 'has_rsd', 'has_lensing, 'has_d1', 'has_d2', 'has_g1', 'has_g2', "
+                             "'has_g3', 'has_g4', 'has_g5'")
 
     def _no_lensing(self, what):
-        if self._p["has_lensing"]:
-            raise NotImplementedError(f"lensing in {what} is not built (the reference integrates it by Monte Carlo)")
+        if self._p["has_lensing"] or self._p["has_g4"] or self._p["has_g5"]:
+            raise NotImplementedError(f"the integrated terms (lensing, g4, g5) in {what} are not built")
 
     def _dg_sources(self):
         return {k for k in _DG if self._p["has_" + k]}
@@ -983,37 +993,51 @@ class Coffe:
         return self._lens[1], self._lens[2]
 
     def _lensing_plan(self):
-        """(want lensing-lensing, want density-lensing), or None without has_lensing
-        (signal.c:612-628, 704-736): lensing-lensing unless has_only_cross_correlations
-        (it is an auto term), needing has_flatsky_nonlocal; with density, RSD or
-        Doppler-1 on, their cross terms with lensing, needing has_flatsky_local_nonlocal,
-        of which only density-lensing is nonzero in flat sky.  The full-sky integrals
-        raise NotImplementedError."""
-        if not self._p["has_lensing"]:
+        """How the integrated terms are computed (signal.c:556-771), or None without
+        lensing, g4 and g5: a dict with
+          flat_ll  the flat-sky lensing-lensing term (has_flatsky_nonlocal; an auto term,
+                   so not with has_only_cross_correlations),
+          flat_dl  the flat-sky density-lensing term (has_flatsky_local_nonlocal; RSD- and
+                   Doppler-lensing vanish in flat sky),
+          flags    the contributions of the full-sky integrated terms (coffe_integrated_
+                   corrfunc), with flat_ll / flat_dl marking what the flat-sky forms
+                   replace, or None when none remains."""
+        p = self._p
+        lens, g4, g5 = p["has_lensing"], p["has_g4"], p["has_g5"]
+        if not (lens or g4 or g5):
             return None
-        if self._dg_sources():
-            raise NotImplementedError("lensing with d2 / g1-g3 (single-integrated terms, Monte Carlo in the "
-                                      "reference) is not built")
-        want_ll = not self._p["has_only_cross_correlations"]
-        local = any(self._p[k] for k in ("has_density", "has_rsd", "has_d1"))
-        if want_ll and not self._p["has_flatsky_nonlocal"]:
-            raise NotImplementedError("full-sky lensing-lensing (double-integrated, Monte Carlo in the reference) "
-                                      "is not built: set has_flatsky_nonlocal")
-        if local and not self._p["has_flatsky_local_nonlocal"]:
-            raise NotImplementedError("full-sky density/RSD/Doppler-lensing (single-integrated, Monte Carlo in the "
-                                      "reference) is not built: set has_flatsky_local_nonlocal")
-        return want_ll, bool(self._p["has_density"])
+        oc = p["has_only_cross_correlations"]
+        local = [k for k in ("density", "rsd", "d1", "d2", "g1", "g2", "g3") if p["has_" + k]]
+        names = {"density": "den"}
+        flags = {names.get(k, k) for k in local}
+        if lens:
+            flags.add("len")
+        if g4:
+            flags.add("g4")
+        if g5:
+            flags.add("g5")
+        if oc:
+            flags.add("only_cross")
+        if p["has_flatsky_nonlocal"]:
+            flags.add("flat_ll")
+        if p["has_flatsky_local_nonlocal"]:
+            flags.add("flat_dl")
+        dl_local = bool(set(local) & {"density", "rsd", "d1"}) and not p["has_flatsky_local_nonlocal"]
+        double = (lens and not oc and not p["has_flatsky_nonlocal"]) or ((g4 or g5) and not oc) or \
+            (lens and (g4 or g5)) or (g4 and g5)
+        single = (lens and (dl_local or bool(set(local) & {"d2", "g1", "g2", "g3"}))) or ((g4 or g5) and bool(local))
+        return dict(flat_ll=bool(lens and not oc and p["has_flatsky_nonlocal"]),
+                    flat_dl=bool(lens and p["has_density"] and p["has_flatsky_local_nonlocal"]),
+                    flags=flags if (double or single) else None)
 
     def _lensing_values(self, zs, seps, ls):
         """[z][l][r] flat-sky lensing multipoles on the GPU (coffe_flat_lensing_multipoles),
-        or None without has_lensing."""
+        or None when neither flat-sky term is on."""
         from .engine import flat_lensing
         plan = self._lensing_plan()
-        if plan is None:
+        if plan is None or not (plan["flat_ll"] or plan["flat_dl"]):
             return None
-        want_ll, want_dl = plan
-        if not (want_ll or want_dl):
-            return np.zeros((len(zs), len(ls), len(seps)))
+        want_ll, want_dl = plan["flat_ll"], plan["flat_dl"]
         ll, dl = self._lensing_tables(ls, want_ll, want_dl)
         zpar = [[float(z), self.comoving_distance(float(z)), self.growth_factor(float(z)),
                  self._eval_mbias("s1", float(z)), self._eval_mbias("s2", float(z)),
@@ -1021,12 +1045,61 @@ class Coffe:
         return flat_lensing(zpar, [int(v) for v in ls], seps, self._background(), ll, dl, self.omega_m,
                             COFFE_H0 * self._p["h"], want_ll, want_dl)
 
+    # theta-Gauss-Legendre nodes (mu = cos(theta)) of the integrated terms' multipoles
+    INTEGRATED_MU_NODES = 64
+
+    def _integrated_corrfunc(self, zs, seps, mus):
+        """[z][mu][r] single- plus double-integrated correlation function, full sky
+        (coffe_integrated_corrfunc), or None when the plan has no full-sky terms."""
+        from .engine import integrated_corrfunc
+        plan = self._lensing_plan()
+        if plan is None or plan["flags"] is None:
+            return None
+        flags = plan["flags"]
+        bz = self._bias[1].x
+        if not np.array_equal(bz, self._bias[2].x):
+            raise ValueError("galaxy_bias1 and galaxy_bias2 must share their redshift sampling")
+        mz, (s1, s2, e1, e2) = self._mbias_grid()
+        tables = dict(self._integral_tables())
+        ren = ren0 = None
+        if flags & {"g4", "g5"}:
+            tables[(4, 0)] = self._divergent_table()
+            ren0 = self._divergent_r0
+            # the grid spans chi(max(z_mean + deltaz)) (integrals.c:862-871)
+            zmax = max(float(z) + float(dz) for z, dz in zip(self._p["z_mean"], self._p["deltaz"]))
+            ren = self._ren_grid(self.comoving_distance(zmax))
+        single, double = integrated_corrfunc(self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, mz, s1, s2, e1,
+                                             e2, self.omega_m, COFFE_H0 * self._p["h"], tables, flags, zs, mus, seps,
+                                             ren=ren, ren0=ren0)
+        return single + double
+
+    def _integrated_multipoles(self, zs, seps, ls):
+        """[z][l][r] multipoles of the full-sky integrated terms: (2 l + 1) / 2 int dmu
+        P_l(mu) xi(mu) (signal.c:143-230, 664-672, 743-752), mu = cos(theta) with theta on
+        an INTEGRATED_MU_NODES-point Gauss-Legendre rule; None without such terms."""
+        t, w = np.polynomial.legendre.leggauss(self.INTEGRATED_MU_NODES)
+        theta = np.pi / 2 * (t + 1)
+        mus, wm = np.cos(theta), np.pi / 2 * w * np.sin(theta)
+        xi = self._integrated_corrfunc(zs, seps, mus)
+        if xi is None:
+            return None
+        out = np.empty((len(zs), len(ls), len(seps)))
+        for il, l in enumerate(ls):
+            pl = np.polynomial.legendre.legval(mus, [0] * int(l) + [1])
+            out[:, il, :] = (2 * int(l) + 1) / 2 * np.einsum("m,zmr->zr", wm * pl, xi)
+        return out
+
     def _all_multipoles(self, zs, seps, ls):
-        """The local terms and, with has_lensing, the flat-sky lensing terms."""
-        self._lensing_plan()   # refuses what is not built before any device work
-        vals = self._multipole_values(zs, seps, ls)
-        lens = self._lensing_values(zs, seps, ls)
-        return vals if lens is None else vals + lens
+        """The local terms and the integrated ones (flat-sky forms and full-sky integrals)."""
+        self._lensing_plan()
+        vals = self._multipole_values(zs, seps, ls) if self._has_local() else np.zeros((len(zs), len(ls), len(seps)))
+        for extra in (self._lensing_values(zs, seps, ls), self._integrated_multipoles(zs, seps, ls)):
+            if extra is not None:
+                vals = vals + extra
+        return vals
+
+    def _has_local(self):
+        return any(self._p[k] for k in ("has_density", "has_rsd", "has_d1", "has_d2", "has_g1", "has_g2", "has_g3"))
 
     def compute_multipole(self, *, z, r, l, recompute=False):  # noqa: E741
         """The multipole at (z_mean, r, l) (coffe.pyx:3232-3300)."""
@@ -1061,14 +1134,15 @@ class Coffe:
     def _corrfunc_values(self, zs, seps, mus):
         """[z][mu][r] correlation function: the local terms (functions_nonintegrated,
         functions.c:89-1484, at each mu: the weights of a one-node rule, the node mu with
-        weight 2 and l = 0) and, with has_lensing, the flat-sky lensing terms
-        (coffe_flat_lensing_corrfunc)."""
+        weight 2 and l = 0), the flat-sky lensing terms (coffe_flat_lensing_corrfunc) and
+        the full-sky integrated ones (coffe_integrated_corrfunc)."""
         from .engine import flat_lensing_corrfunc, half_integer_table
         plan = self._lensing_plan()
-        out = np.empty((len(zs), len(mus), len(seps)))
-        for j, mu in enumerate(mus):
-            out[:, j, :] = self._multipole_values(zs, seps, [0], rule=([float(mu)], [2.0]))[:, 0, :]
-        if plan is not None and (plan[0] or plan[1]):
+        out = np.zeros((len(zs), len(mus), len(seps)))
+        if self._has_local():
+            for j, mu in enumerate(mus):
+                out[:, j, :] = self._multipole_values(zs, seps, [0], rule=([float(mu)], [2.0]))[:, 0, :]
+        if plan is not None and (plan["flat_ll"] or plan["flat_dl"]):
             # the (1/2, -1/2) table (integrals.c:917-962), cached with the other lensing tables
             if self._lens_cf is None:
                 k, pk = self._pk
@@ -1078,7 +1152,11 @@ class Coffe:
                      self._eval_mbias("s1", float(z)), self._eval_mbias("s2", float(z)),
                      self._eval_bias(1, float(z)), self._eval_bias(2, float(z))] for z in zs]
             out += flat_lensing_corrfunc(zpar, [float(m) for m in mus], seps, self._background(), self._lens_cf,
-                                         self.omega_m, COFFE_H0 * self._p["h"], plan[0], plan[1])
+                                         self.omega_m, COFFE_H0 * self._p["h"], plan["flat_ll"], plan["flat_dl"])
+        if plan is not None:
+            xi = self._integrated_corrfunc(zs, seps, [float(m) for m in mus])
+            if xi is not None:
+                out += xi
         return out
 
     def compute_corrfunc(self, *, z, r, mu, recompute=False):

@@ -234,6 +234,35 @@ int ce_launch_mu_weights_jvp(void *stream, const ce_mu_dev *in, const ce_mu_tan 
                              int nsep, const double *sep, int nmu, const double *mu, const double *wmu, double *A,
                              double *At, double *coef);
 
+/* ---- integrated terms (integrated_kernels.hip) ---- */
+/* device inputs: the background on bg_z, galaxy biases on bias_z, magnification and
+   evolution biases on mb_z, the tables concatenated (slots (0,0), (0,2), (0,4), (1,1),
+   (1,3), (2,0), (2,2), (3,1), (4,0): offset and length, 0 when absent), the r = 0
+   renormalisation of I_0^4 (ren0) and the renormalisation grid with its bicubic's node
+   derivatives ([b nren + a] at (ren_chi[a], ren_chi[b])) */
+#define CE_INTG_NSLOT 9
+typedef struct ce_intg_dev {
+    int nbg;
+    const double *bg_z, *bg_chi, *bg_D1, *bg_a, *bg_f, *bg_cH, *bg_G1, *bg_G2;
+    int nb;
+    const double *bias_z, *b1, *b2;
+    int nmb;
+    const double *mb_z, *s1, *s2, *fe1, *fe2;
+    const double *tab_x, *tab_y;
+    int tab_off[CE_INTG_NSLOT], tab_len[CE_INTG_NSLOT];
+    int nren0;
+    const double *ren0_x, *ren0_y;
+    int nren;
+    const double *ren_chi, *ren_v, *ren_x, *ren_y, *ren_xy;
+    double Om, H0h;
+} ce_intg_dev;
+/* one workgroup per (z, mu, sep): out [nz][nmu][nsep][2] = (single, double); rule =
+   {p1, n1, pu, nu, ps, ns}, d_gl = the n1-, nu- and ns-point Gauss-Legendre rules on
+   [-1, 1] (nodes then weights, in that order) */
+int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigned flags, int nz, const double *d_z, int nmu,
+                         const double *d_mu, int nsep, const double *d_sep, const int *rule, const double *d_gl,
+                         double *d_out);
+
 #ifdef __cplusplus
 }
 #endif

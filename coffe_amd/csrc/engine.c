@@ -1763,3 +1763,178 @@ int coffe_flat_lensing_corrfunc(int nz, const double *zpar, int nmu, const doubl
     return flat_lensing_run(1, nz, zpar, nmu, NULL, mu, nsep, sep, nb, bg_z, bg_chi, bg_D1, ntab, tab_x, tab_y, &off0,
                             &len0, NULL, NULL, Omega_m, H0h, want_ll, want_dl, panels, out);
 }
+
+/* ---- integrated terms, full sky ---------------------------------------------- */
+
+/* the table slots of the integrated kernel, in the order of integrated_kernels.hip */
+static const int intg_slot_n[CE_INTG_NSLOT] = {0, 0, 0, 1, 1, 2, 2, 3, 4};
+static const int intg_slot_l[CE_INTG_NSLOT] = {0, 2, 4, 1, 3, 0, 2, 1, 0};
+
+/* the slots the terms of `flags` read (bit t: slot t) */
+static unsigned intg_needed_slots(unsigned f)
+{
+    const unsigned loc = COFFE_INTEGRATED_DEN | COFFE_INTEGRATED_RSD | COFFE_INTEGRATED_D1 | COFFE_INTEGRATED_D2 |
+                         COFFE_INTEGRATED_G1 | COFFE_INTEGRATED_G2 | COFFE_INTEGRATED_G3;
+    const int len = (f & COFFE_INTEGRATED_LEN) != 0, g45 = (f & (COFFE_INTEGRATED_G4 | COFFE_INTEGRATED_G5)) != 0;
+    const int flat_dl = (f & COFFE_INTEGRATED_FLAT_DL) != 0;
+    unsigned m = 0;
+    enum { S00, S02, S04, S11, S13, S20, S22, S31, S40 };
+#define B(s) (1u << (s))
+    if (len && !(f & COFFE_INTEGRATED_ONLY_CROSS) && !(f & COFFE_INTEGRATED_FLAT_LL))
+        m |= B(S00) | B(S02) | B(S04) | B(S11) | B(S13) | B(S20);
+    if (len && g45) m |= B(S31) | B(S22);
+    if (g45) m |= B(S40);
+    if (len && !flat_dl && (f & COFFE_INTEGRATED_DEN)) m |= B(S11) | B(S02);
+    if (len && !flat_dl && (f & COFFE_INTEGRATED_RSD)) m |= B(S00) | B(S02) | B(S04);
+    if (len && !flat_dl && (f & COFFE_INTEGRATED_D1)) m |= B(S11) | B(S20) | B(S13);
+    if (len && (f & (COFFE_INTEGRATED_D2 | COFFE_INTEGRATED_G1 | COFFE_INTEGRATED_G2 | COFFE_INTEGRATED_G3)))
+        m |= B(S31) | B(S22);
+    if (g45 && (f & COFFE_INTEGRATED_DEN)) m |= B(S20);
+    if (g45 && (f & COFFE_INTEGRATED_RSD)) m |= B(S22) | B(S20);
+    if (g45 && (f & COFFE_INTEGRATED_D1)) m |= B(S31);
+    if (g45 && (f & loc)) m |= B(S40);
+#undef B
+    return m;
+}
+
+int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int ntab, const int *tab_n, const int *tab_l,
+                              const int *tab_len, const double *tab_x, const double *tab_y, int nren0,
+                              const double *ren0_x, const double *ren0_y, int nz, const double *zmean, int nmu,
+                              const double *mu, int nsep, const double *sep, const int *rule, double *out)
+{
+    const unsigned all = (COFFE_INTEGRATED_FLAT_DL << 1) - 1;
+    if (!in || (flags & ~all) || nz <= 0 || nmu <= 0 || nsep <= 0 || !zmean || !mu || !sep || !out || ntab < 0 ||
+        ntab > 64 || (ntab > 0 && (!tab_n || !tab_l || !tab_len || !tab_x || !tab_y)) ||
+        (int64_t)nz * nmu * nsep > 0x7fffffff)
+        return COFFE_EINVAL;
+    if (in->nbg < 5 || !in->bg_z || !in->bg_chi || !in->bg_D1 || !in->bg_a || !in->bg_f || !in->bg_cH ||
+        !in->bg_G1 || !in->bg_G2 || in->nb < 5 || !in->bias_z || !in->bias1 || !in->bias2 || in->nmb < 5 ||
+        !in->mb_z || !in->s1 || !in->s2 || !in->fevo1 || !in->fevo2)
+        return COFFE_EINVAL;
+    int r[6] = {8, 16, 12, 16, 32, 16};
+    if (rule)
+        for (int i = 0; i < 6; ++i) r[i] = rule[i];
+    for (int i = 0; i < 6; i += 2)
+        if (r[i] < 1 || r[i] > 256 || r[i + 1] < 2 || r[i + 1] > 64) return COFFE_EINVAL;
+    for (int i = 0; i < nmu; ++i)
+        if (!(mu[i] >= -1.0 && mu[i] <= 1.0)) return COFFE_EINVAL;
+    for (int i = 0; i < nsep; ++i)
+        if (!(sep[i] > 0.0) || !isfinite(sep[i])) return COFFE_EINVAL;
+    /* the tables by slot; a needed one must be there with 5 nodes or more */
+    int off[CE_INTG_NSLOT], len[CE_INTG_NSLOT];
+    int64_t total_tab = 0;
+    for (int t = 0; t < CE_INTG_NSLOT; ++t) off[t] = len[t] = 0;
+    for (int i = 0; i < ntab; ++i) {
+        if (tab_len[i] < 0 || total_tab + tab_len[i] > 0x7fffffff) return COFFE_EINVAL;
+        for (int t = 0; t < CE_INTG_NSLOT; ++t)
+            if (tab_n[i] == intg_slot_n[t] && tab_l[i] == intg_slot_l[t]) {
+                off[t] = (int)total_tab;
+                len[t] = tab_len[i];
+            }
+        total_tab += tab_len[i];
+    }
+    const unsigned need = intg_needed_slots(flags);
+    for (int t = 0; t < CE_INTG_NSLOT; ++t)
+        if ((need >> t & 1u) && len[t] < 5) return COFFE_EINVAL;
+    const int want_ren = (need >> 8) & 1u;
+    if (want_ren && (in->nren < 5 || !in->ren_chi || !in->ren || nren0 < 5 || !ren0_x || !ren0_y))
+        return COFFE_EINVAL;
+    coffe_engine *e = coffe_engine_default();
+    if (!e) return COFFE_ENODEVICE;
+    hipSetDevice(e->device);
+    const size_t nbg = (size_t)in->nbg, nb = (size_t)in->nb, nmb = (size_t)in->nmb;
+    const size_t nren = want_ren ? (size_t)in->nren : 0, nnode = nren * nren, n0 = want_ren ? (size_t)nren0 : 0;
+    const size_t ngl = 2 * (size_t)(r[1] + r[3] + r[5]), nout = 2 * (size_t)nz * nmu * nsep;
+    const size_t total = 8 * nbg + 3 * nb + 5 * nmb + 2 * (size_t)total_tab + 2 * n0 + nren + 4 * nnode +
+                         (size_t)nz + (size_t)nmu + (size_t)nsep + ngl + nout;
+    double *h = (double *)malloc(sizeof(double) * (total - nout));
+    double *work = (double *)malloc(sizeof(double) * (2 * nren + 2));
+    if (!h || !work) {
+        free(h);
+        free(work);
+        return COFFE_ENOMEM;
+    }
+    double *d = NULL;
+    int st = dev_alloc((void **)&d, sizeof(double) * total);
+    ce_intg_dev dv;
+    memset(&dv, 0, sizeof(dv));
+    size_t o = 0;
+    /* place n values of src in the staging buffer; the device pointer at the same offset */
+#define IG_PUT(src, n)                                                       \
+    ({                                                                       \
+        const size_t n_ = (size_t)(n);                                       \
+        if (n_) memcpy(h + o, (src), sizeof(double) * n_);                   \
+        const double *r_ = d + o;                                            \
+        o += n_;                                                             \
+        r_;                                                                  \
+    })
+    dv.nbg = in->nbg;
+    dv.bg_z = IG_PUT(in->bg_z, nbg);
+    dv.bg_chi = IG_PUT(in->bg_chi, nbg);
+    dv.bg_D1 = IG_PUT(in->bg_D1, nbg);
+    dv.bg_a = IG_PUT(in->bg_a, nbg);
+    dv.bg_f = IG_PUT(in->bg_f, nbg);
+    dv.bg_cH = IG_PUT(in->bg_cH, nbg);
+    dv.bg_G1 = IG_PUT(in->bg_G1, nbg);
+    dv.bg_G2 = IG_PUT(in->bg_G2, nbg);
+    dv.nb = in->nb;
+    dv.bias_z = IG_PUT(in->bias_z, nb);
+    dv.b1 = IG_PUT(in->bias1, nb);
+    dv.b2 = IG_PUT(in->bias2, nb);
+    dv.nmb = in->nmb;
+    dv.mb_z = IG_PUT(in->mb_z, nmb);
+    dv.s1 = IG_PUT(in->s1, nmb);
+    dv.s2 = IG_PUT(in->s2, nmb);
+    dv.fe1 = IG_PUT(in->fevo1, nmb);
+    dv.fe2 = IG_PUT(in->fevo2, nmb);
+    dv.tab_x = IG_PUT(tab_x, total_tab);
+    dv.tab_y = IG_PUT(tab_y, total_tab);
+    for (int t = 0; t < CE_INTG_NSLOT; ++t) {
+        dv.tab_off[t] = off[t];
+        dv.tab_len[t] = len[t];
+    }
+    if (want_ren) {
+        dv.nren0 = nren0;
+        dv.ren0_x = IG_PUT(ren0_x, n0);
+        dv.ren0_y = IG_PUT(ren0_y, n0);
+        dv.nren = in->nren;
+        dv.ren_chi = IG_PUT(in->ren_chi, nren);
+        /* values and the bicubic's node-derivative tables (as coffe_mu_weights_ex) */
+        double *hv = h + o, *hx = hv + nnode, *hy = hx + nnode, *hxy = hy + nnode;
+        memcpy(hv, in->ren, sizeof(double) * nnode);
+        for (size_t j = 0; j < nren; ++j) spline_node_slopes((int)nren, in->ren_chi, hv + j * nren, 1, hx + j * nren, 1, work);
+        for (size_t i = 0; i < nren; ++i) spline_node_slopes((int)nren, in->ren_chi, hv + i, (int)nren, hy + i, (int)nren, work);
+        for (size_t i = 0; i < nren; ++i) spline_node_slopes((int)nren, in->ren_chi, hx + i, (int)nren, hxy + i, (int)nren, work);
+        dv.ren_v = d + o;
+        dv.ren_x = d + o + nnode;
+        dv.ren_y = d + o + 2 * nnode;
+        dv.ren_xy = d + o + 3 * nnode;
+        o += 4 * nnode;
+    }
+    dv.Om = in->Omega_m;
+    dv.H0h = in->H0h;
+    const double *d_z = IG_PUT(zmean, nz);
+    const double *d_mu = IG_PUT(mu, nmu);
+    const double *d_sep = IG_PUT(sep, nsep);
+    const double *d_gl = d + o;
+    {
+        const int ns[3] = {r[1], r[3], r[5]};
+        for (int k = 0; k < 3; ++k) {
+            gauss_legendre(ns[k], h + o, h + o + ns[k]);
+            o += 2 * (size_t)ns[k];
+        }
+    }
+#undef IG_PUT
+    double *d_out = d + o;
+    hipStream_t s = e->stream;
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d, h, sizeof(double) * o, hipMemcpyHostToDevice, s));
+    if (st == COFFE_OK && ce_launch_integrated(s, &dv, flags, nz, d_z, nmu, d_mu, nsep, d_sep, r, d_gl, d_out))
+        st = COFFE_EDEVICE;
+    if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(out, d_out, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+    if (st == COFFE_OK) st = hip_ok(hipStreamSynchronize(s));
+    else if (d) hipStreamSynchronize(s);
+    hipFree(d);
+    free(h);
+    free(work);
+    return st;
+}

@@ -1,0 +1,541 @@
+// Integrated (lensing and potential) terms of the correlation function, full sky, on
+// the GPU (gfx950, FP64).
+//
+// The reference evaluates them with CUBA's Cuhre or GSL Monte Carlo over [0, 1]^2
+// (double-integrated, functions.c:3460-4017, signal.c:735-741) and with QAG or a
+// double-exponential rule over [0, 1] (single-integrated, functions.c:1486-3458,
+// signal.c:578-600).  Here both are deterministic product rules, one workgroup per
+// (z_mean, mu, separation):
+//   - double: x1 a composite Gauss-Legendre rule on [0, 1] (cut where lambda1 cos(theta)
+//     = chi2), and for each x1 the substitution lambda2 = lambda1 c + s sinh(u),
+//     s = lambda1 sin(theta), under which r = s cosh(u) and the ridge of closest
+//     approach along lambda2 is smooth in u; u a composite Gauss-Legendre rule;
+//   - single: the two mixed separations are closest at x = chi1 c / chi2 and chi2 c /
+//     chi1; [0, 1] is cut there (and halfway between), each piece mapped by x = centre
+//     +- s sinh(u) about its nearer centre (s the ridge width), u Gauss-Legendre.
+// Every lookup is an Akima interpolation of raw nodes (ce::akima_general) and the
+// renormalisation grid a GSL-style bicubic, as the reference's splines.
+
+#include <hip/hip_runtime.h>
+
+#include "ce_device.h"
+#include "ce_internal.h"
+
+namespace {
+
+enum { S00, S02, S04, S11, S13, S20, S22, S31, S40, NSLOT };
+
+constexpr unsigned F_DEN = 1u << 0, F_RSD = 1u << 1, F_D1 = 1u << 2, F_D2 = 1u << 3, F_G1 = 1u << 4,
+                   F_G2 = 1u << 5, F_G3 = 1u << 6, F_LEN = 1u << 7, F_G4 = 1u << 8, F_G5 = 1u << 9,
+                   F_ONLY_CROSS = 1u << 10, F_FLAT_LL = 1u << 11, F_FLAT_DL = 1u << 12;
+
+struct intg_args {
+    int nbg;
+    const double *bg_z, *bg_chi, *bg_D1, *bg_a, *bg_f, *bg_cH, *bg_G1, *bg_G2;
+    int nb;
+    const double *bias_z, *b1, *b2;
+    int nmb;
+    const double *mb_z, *s1, *s2, *fe1, *fe2;
+    const double *tab_x, *tab_y;
+    int tab_off[NSLOT], tab_len[NSLOT];
+    int nren0;
+    const double *ren0_x, *ren0_y;
+    int nren;
+    const double *ren_chi, *ren_v, *ren_x, *ren_y, *ren_xy;
+    double Om, H0h;
+    unsigned flags;
+    int nz, nmu, nsep;
+    const double *zmean, *mu, *sep;
+    int p1, n1, pu, nu, ps, ns;
+    const double *g1x, *g1w, *gux, *guw, *gsx, *gsw;   // Gauss-Legendre rules on [-1, 1]
+    double *out;                                       // [nz][nmu][nsep][2]: single, double
+};
+
+__device__ inline double ak(const double *x, const double *y, int n, double v) { return ce::akima_general(x, y, n, v); }
+
+struct lk {
+    const intg_args &a;
+    __device__ lk(const intg_args &a_) : a(a_) {}
+    __device__ double bg(const double *col, double z) const { return ak(a.bg_z, col, a.nbg, z); }
+    __device__ double z_of(double chi) const { return ak(a.bg_chi, a.bg_z, a.nbg, chi); }
+    __device__ double tab(int s, double r) const
+    {
+        return ak(a.tab_x + a.tab_off[s], a.tab_y + a.tab_off[s], a.tab_len[s], r);
+    }
+    // GSL interp2d bicubic of the renormalisation grid at (x, y) (node derivatives of
+    // natural cubic splines, Hermite patches; common.c:712-719)
+    __device__ double bicubic(double x, double y) const
+    {
+        const int n = a.nren;
+        const double *c = a.ren_chi;
+        if (!(x >= c[0] && x <= c[n - 1] && y >= c[0] && y <= c[n - 1])) return __builtin_nan("");
+        const int i = ce::interval_from(c, n, x, ce::interval_guess(c, n, x));
+        const int j = ce::interval_from(c, n, y, ce::interval_guess(c, n, y));
+        const double dx = c[i + 1] - c[i], dy = c[j + 1] - c[j];
+        const double t = (x - c[i]) / dx, u = (y - c[j]) / dy;
+        const double ht[4] = {(1 + 2 * t) * (1 - t) * (1 - t), t * t * (3 - 2 * t), t * (1 - t) * (1 - t),
+                              t * t * (t - 1)};
+        const double hu[4] = {(1 + 2 * u) * (1 - u) * (1 - u), u * u * (3 - 2 * u), u * (1 - u) * (1 - u),
+                              u * u * (u - 1)};
+        double r = 0.0;
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int o = (j + q) * n + i + p;
+                r += a.ren_v[o] * ht[p] * hu[q] + a.ren_x[o] * dx * ht[2 + p] * hu[q] +
+                     a.ren_y[o] * dy * ht[p] * hu[2 + q] + a.ren_xy[o] * dx * dy * ht[2 + p] * hu[2 + q];
+            }
+        return r;
+    }
+    // I_0^4 r^4 at r minus the grid at (x, y); the r = 0 renormalisation at `at0` where
+    // `small` holds (functions.c:1524-1553, 3494-3512)
+    __device__ double ren(double r2, double x, double y, bool small, double at0) const
+    {
+        if (small) return ak(a.ren0_x, a.ren0_y, a.nren0, at0);
+        return tab(S40, sqrt(r2)) - bicubic(x, y);
+    }
+};
+
+// geometry of the pair (functions.c:1498-1503, 3472-3479) and the values at the two
+// end points (z(chi_i) and the biases there)
+struct pt {
+    double mu, chi1, chi2, c, omc, sint;
+    double s1, s2, b1, b2, fe1, fe2;
+    double Dc1, Dc2, fc1, fc2, cHc1, cHc2, ac1, ac2, G1c, G2c, G1c2;
+};
+
+__device__ pt make_pt(const intg_args &a, double zm, double mu, double sep)
+{
+    const lk L(a);
+    pt p;
+    p.mu = mu;
+    const double cm = ak(a.bg_z, a.bg_chi, a.nbg, zm);
+    p.chi1 = cm - sep * mu / 2;
+    p.chi2 = cm + sep * mu / 2;
+    const double den = 2 * cm * cm - mu * mu * sep * sep / 2;
+    p.c = (2 * cm * cm - sep * sep + mu * mu * sep * sep / 2) / den;
+    // 1 - cos(theta) without cancellation
+    p.omc = sep * sep * (1 - mu * mu) / den;
+    p.sint = sqrt(fmax(p.omc * (1 + p.c), 0.0));
+    const double z1c = L.z_of(p.chi1), z2c = L.z_of(p.chi2);
+    p.s1 = ak(a.mb_z, a.s1, a.nmb, z1c);
+    p.s2 = ak(a.mb_z, a.s2, a.nmb, z2c);
+    p.fe1 = ak(a.mb_z, a.fe1, a.nmb, z1c);
+    p.fe2 = ak(a.mb_z, a.fe2, a.nmb, z2c);
+    p.b1 = ak(a.bias_z, a.b1, a.nb, z1c);
+    p.b2 = ak(a.bias_z, a.b2, a.nb, z2c);
+    p.Dc1 = L.bg(a.bg_D1, z1c);
+    p.Dc2 = L.bg(a.bg_D1, z2c);
+    p.fc1 = L.bg(a.bg_f, z1c);
+    p.fc2 = L.bg(a.bg_f, z2c);
+    p.cHc1 = L.bg(a.bg_cH, z1c);
+    p.cHc2 = L.bg(a.bg_cH, z2c);
+    p.ac1 = L.bg(a.bg_a, z1c);
+    p.ac2 = L.bg(a.bg_a, z2c);
+    p.G1c = L.bg(a.bg_G1, z1c);
+    p.G2c = L.bg(a.bg_G2, z2c);
+    p.G1c2 = L.bg(a.bg_G1, z2c);
+    return p;
+}
+
+// functions_double_integrated (functions.c:3460-4017) at (x1, x2), times (H0 h)^4
+__device__ double double_point(const intg_args &a, const pt &p, double x1, double x2)
+{
+    const lk L(a);
+    const unsigned f = a.flags;
+    const double c = p.c, Om2 = a.Om * a.Om;
+    const double l1 = p.chi1 * x1, l2 = p.chi2 * x2;
+    const double r2 = fmax(l1 * l1 + l2 * l2 - 2 * l1 * l2 * c, 0.0);
+    const double r = sqrt(r2);
+    const double z1 = L.z_of(l1), z2 = L.z_of(l2);
+    const double D1 = L.bg(a.bg_D1, z1), D2 = L.bg(a.bg_D1, z2);
+    const double DD = D1 * D2 / L.bg(a.bg_a, z1) / L.bg(a.bg_a, z2);
+    const double LL = l1 * l2;
+    const double q2 = (2 - 5 * p.s1) * (2 - 5 * p.s2);
+    const bool lens = f & F_LEN, g4 = f & F_G4, g5 = f & F_G5, oc = f & F_ONLY_CROSS;
+    double res = 0.0;
+    double ren = 0.0;
+    if (g4 || g5) ren = L.ren(r2, l1, l2, r2 <= 1e-12, l1);
+    if (lens && !oc && !(f & F_FLAT_LL)) {
+        double T;
+        if (r2 > 1e-20) {
+            const double cc1 = c * c - 1;
+            T = 2 * cc1 * LL * L.tab(S00, r) / 5 + 4 * c * L.tab(S20, r) / 3 +
+                4 * c * (r2 + 6 * c * LL) * L.tab(S11, r) / 15 +
+                2 * cc1 * LL * (2 * r2 + 3 * c * LL) * L.tab(S02, r) / 7 / r2 +
+                2 * c * (2 * r2 * r2 + 12 * c * r2 * LL + 15 * cc1 * LL * LL) * L.tab(S13, r) / 15 / r2 +
+                cc1 * LL * (6 * r2 * r2 + 30 * c * r2 * LL + 35 * cc1 * LL * LL) * L.tab(S04, r) / 35 / r2 / r2;
+        } else {
+            T = 4 + 24 * LL * L.tab(S11, 0.0) / 15;
+        }
+        res += 9 * Om2 * q2 / 4 * p.chi1 * p.chi2 * DD * (1 - x1) * (1 - x2) * T;
+    }
+    double cH1 = 0, cH2 = 0, f1 = 0, f2 = 0;
+    if (g5) {
+        cH1 = L.bg(a.bg_cH, z1);
+        cH2 = L.bg(a.bg_cH, z2);
+        f1 = L.bg(a.bg_f, z1);
+        f2 = L.bg(a.bg_f, z2);
+    }
+    if (g4 && !oc) res += 9 * Om2 * q2 * DD * ren;
+    if (g5 && !oc) res += 9 * Om2 * p.G1c * p.G2c * p.chi1 * p.chi2 * DD * cH1 * cH2 * (f1 - 1) * (f2 - 1) * ren;
+    if ((g4 || g5) && lens) {
+        const double P = r2 != 0.0 ? 2 * LL * c * L.tab(S31, r) - LL * LL * (1 - c * c) * L.tab(S22, r)
+                                   : 2 * LL * L.tab(S31, 0.0);
+        const double w2 = (1 - x2) / x2, w1 = (1 - x1) / x1;
+        if (g4) res += 9 * Om2 / 2 * q2 * DD * (w2 + w1) * P;
+        // functions.c:3927-3940: conformal_Hz (conformal_Hz - 1) as written there
+        if (g5)
+            res += 9 * Om2 / 2 * DD * P *
+                   ((2 - 5 * p.s2) * p.G1c * p.chi1 * cH1 * (cH1 - 1) * w2 +
+                    (2 - 5 * p.s1) * p.G2c * p.chi2 * cH2 * (cH2 - 1) * w1);
+    }
+    if (g4 && g5)
+        res += 9 * Om2 * DD * ren *
+               (p.G2c * (2 - 5 * p.s1) * p.chi2 * cH2 * (f2 - 1) + p.G1c * (2 - 5 * p.s2) * p.chi1 * cH1 * (f1 - 1));
+    const double h2 = a.H0h * a.H0h;
+    return res * h2 * h2;
+}
+
+// The numerators of the single-integrated lensing terms that lose digits near the
+// line of closest approach (functions.c:2183-2187, 2296-2303, 2638-2644; chi the local
+// end point's distance, lambda the integration point's), in d = chi - lambda and
+// e = 1 - cos(theta), where the squared separation is d^2 + 2 chi lambda e; each is the
+// reference's polynomial identically, without its FP64 cancellation.
+__device__ inline double num_rsd_i02(double ch, double lam, double d, double e)
+{
+    return 2 * d * d * (4 * d - ch) + 2 * e * (ch * ch * ch - 11 * ch * ch * d + 19 * ch * d * d - 12 * d * d * d) -
+           3 * e * e * lam * (ch * ch - 2 * ch * d + 4 * d * d) - 6 * ch * e * e * e * lam * lam;
+}
+
+__device__ inline double num_rsd_i04(double ch, double lam, double d, double e)
+{
+    const double d2 = d * d, ch2 = ch * ch, e2 = e * e, lm2 = lam * lam;
+    return 4 * d2 * d2 * (d - 2 * ch) + 4 * d2 * e * (12 * ch2 * ch - 23 * ch2 * d + 15 * ch * d2 - 3 * d2 * d) +
+           6 * e2 * lm2 * (-2 * ch2 * ch + 6 * ch2 * d - 6 * ch * d2 + d2 * d) +
+           2 * ch * e2 * e * lm2 * (2 * ch2 - 4 * ch * d + 3 * d2) + ch2 * e2 * e2 * lm2 * lam;
+}
+
+__device__ inline double num_d1_i13(double ch, double lam, double d, double e)
+{
+    const double d2 = d * d, ch2 = ch * ch;
+    return 2 * d2 * d * (3 * ch - d) + 2 * d * e * (-9 * ch2 * ch + 14 * ch2 * d - 8 * ch * d2 + d2 * d) -
+           ch * e * e * lam * (2 * ch2 - 5 * ch * d + 4 * d2) - ch2 * e * e * e * lam * lam;
+}
+
+// functions_single_integrated (functions.c:1486-3458) at x: point 1 at chi1 with the
+// integrated source along line of sight 2 (lambda2 = x chi2, squared separation r21)
+// and the other way round (r22); the branches where neither is exactly zero
+__device__ double single_point(const intg_args &a, const pt &p, double x)
+{
+    const lk L(a);
+    const unsigned f = a.flags;
+    const double c = p.c, Om = a.Om, H2 = a.H0h * a.H0h, H4 = H2 * H2;
+    const double chi1 = p.chi1, chi2 = p.chi2;
+    const double l1 = chi1 * x, l2 = chi2 * x;
+    const double r21 = fmax(l2 * l2 + chi1 * chi1 - 2 * chi1 * l2 * c, 0.0);
+    const double r22 = fmax(l1 * l1 + chi2 * chi2 - 2 * chi2 * l1 * c, 0.0);
+    const double rA = sqrt(r21), rB = sqrt(r22);
+    const double z1 = L.z_of(l1), z2 = L.z_of(l2);
+    const double D1x = L.bg(a.bg_D1, z1), D2x = L.bg(a.bg_D1, z2);
+    const double a1x = L.bg(a.bg_a, z1), a2x = L.bg(a.bg_a, z2);
+    const double VA = D2x / a2x, VB = D1x / a1x;
+    const double WA = (1 - x) * VA, WB = (1 - x) * VB;
+    const double s1 = p.s1, s2 = p.s2, c2 = 2 * c * c - 1, e = p.omc;
+    const bool lens = f & F_LEN, g4 = f & F_G4, g5 = f & F_G5, full_dl = !(f & F_FLAT_DL);
+    double res = 0.0;
+    if (lens && (f & F_DEN) && full_dl)   // functions.c:1707-1781
+        res += H2 * (-3 * Om / 2) *
+               (p.b1 * (2 - 5 * s2) * p.Dc1 * chi2 * WA *
+                    (2 * chi1 * c * L.tab(S11, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tab(S02, rA) / r21) +
+                p.b2 * (2 - 5 * s1) * p.Dc2 * chi1 * WB *
+                    (2 * chi2 * c * L.tab(S11, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tab(S02, rB) / r22));
+    if (lens && (f & F_RSD) && full_dl) {   // functions.c:2167-2198, then 2291-2324 or 2377-2390
+        const bool wide = fabs(p.mu) < 0.999;
+        auto rsd = [&](double ch, double lam, double rr, double rv) {
+            (void)rr;
+            const double d = ch - lam, rs = d * d + 2 * ch * lam * e;
+            double v = (lam - 6 * ch * c + 3 * lam * c2) * L.tab(S00, rv) / 15 -
+                       num_rsd_i02(ch, lam, d, e) * L.tab(S02, rv) / rs / 21;
+            if (wide) v -= num_rsd_i04(ch, lam, d, e) * L.tab(S04, rv) / rs / rs / 35;
+            else v += 4 * (lam + ch) * L.tab(S04, rv) / 35;
+            return v;
+        };
+        res += H2 * 3 * Om / 2 *
+               (chi2 * p.fc1 * (2 - 5 * s2) * p.Dc1 * WA * rsd(chi1, l2, r21, rA) +
+                chi1 * p.fc2 * (2 - 5 * s1) * p.Dc2 * WB * rsd(chi2, l1, r22, rB));
+    }
+    if (lens && (f & F_D1) && full_dl) {   // functions.c:2624-2661 (G1 at z(chi2), 2 - 5 s2 in the second half)
+        auto d1k = [&](double ch, double lam, double rr, double rv) {
+            (void)rr;
+            const double d = ch - lam, rs = d * d + 2 * ch * lam * e;
+            return 2 * (c * (lam * lam - 2 * ch * ch) + ch * lam * (2 * c2 - 1)) * L.tab(S11, rv) / 15 +
+                   2 * c * L.tab(S20, rv) / 3 - num_d1_i13(ch, lam, d, e) * L.tab(S13, rv) / rs / 15;
+        };
+        res += H2 * 3 * Om / 2 *
+               (chi2 * p.cHc1 * p.fc1 * p.G1c * (2 - 5 * s2) * p.Dc1 * WA * d1k(chi1, l2, r21, rA) +
+                chi1 * p.cHc2 * p.fc2 * p.G1c2 * (2 - 5 * s2) * p.Dc2 * WB * d1k(chi2, l1, r22, rB));
+    }
+    if (lens && (f & (F_D2 | F_G1 | F_G2 | F_G3))) {
+        const double PA = 2 * chi1 * c * L.tab(S31, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tab(S22, rA);
+        const double PB = 2 * chi2 * c * L.tab(S31, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tab(S22, rB);
+        const double A = chi2 * p.Dc1 * WA * PA, B = chi1 * p.Dc2 * WB * PB;
+        const double q = 9 * Om * Om / 4 * H4;
+        if (f & F_D2)   // functions.c:3000-3030
+            res += H2 * (-3 * Om / 2) *
+                   ((3 - p.fe1) * p.fc1 * p.cHc1 * p.cHc1 * (2 - 5 * s2) * A +
+                    (3 - p.fe2) * p.fc2 * p.cHc2 * p.cHc2 * (2 - 5 * s1) * B);
+        if (f & F_G1) res += q * ((1 + p.G1c) * (2 - 5 * s2) * A + (1 + p.G2c) * (2 - 5 * s1) * B);   // 3032-3060
+        if (f & F_G2) res += q * ((5 * s1 - 2) * (2 - 5 * s2) * A + (5 * s2 - 2) * (2 - 5 * s1) * B);  // 3062-3090
+        if (f & F_G3) res += q * ((p.fc1 - 1) * (2 - 5 * s2) * A + (p.fc2 - 1) * (2 - 5 * s1) * B);   // 3092-3120
+    }
+    if (g4 || g5) {
+        const bool div = f & (F_D2 | F_G1 | F_G2 | F_G3 | F_G4 | F_G5);
+        const double ren1 = div ? L.ren(r21, l2, chi1, r21 == 0.0, l2) : 0.0;
+        const double ren2 = div ? L.ren(r22, l1, chi2, r22 == 0.0, l1) : 0.0;
+        const double E1 = L.bg(a.bg_cH, z1) * (L.bg(a.bg_f, z1) - 1), E2 = L.bg(a.bg_cH, z2) * (L.bg(a.bg_f, z2) - 1);
+        auto rsdk = [&](double rr, double lam, double rv) {
+            return (2 * rr / 3 + (c * c - 1) * lam * lam) * L.tab(S22, rv) - L.tab(S20, rv) / 3;
+        };
+        const double q = 9 * Om * Om / 2 * H4;
+        const double cH2c1 = p.cHc1 * p.cHc1, cH2c2 = p.cHc2 * p.cHc2;
+        if (g4) {
+            if (f & F_DEN)   // functions.c:3122-3140
+                res += H2 * (-3 * Om) *
+                       (p.b1 * (2 - 5 * s2) * p.Dc1 * VA * L.tab(S20, rA) + p.b2 * (2 - 5 * s1) * p.Dc2 * VB * L.tab(S20, rB));
+            if (f & F_RSD)   // functions.c:3164-3189
+                res += H2 * 3 * Om *
+                       (p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * rsdk(r21, l2, rA) +
+                        p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * rsdk(r22, l1, rB));
+            if (f & F_D1)    // functions.c:3220-3235
+                res += H2 * 3 * Om *
+                       (p.cHc1 * p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * (l2 * c - chi1) * L.tab(S31, rA) +
+                        p.cHc2 * p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * (l1 * c - chi2) * L.tab(S31, rB));
+            if (f & F_D2)    // functions.c:3258-3275
+                res += H2 * (-3 * Om) *
+                       ((3 - p.fe1) * p.fc1 * cH2c1 * (2 - 5 * s2) * p.Dc1 * VA * ren1 +
+                        (3 - p.fe2) * p.fc2 * cH2c2 * (2 - 5 * s1) * p.Dc2 * VB * ren2);
+            const double u1 = p.Dc1 / p.ac1 * VA * ren1, u2 = p.Dc2 / p.ac2 * VB * ren2;
+            if (f & F_G1) res += q * ((1 + p.G1c) * (2 - 5 * s2) * u1 + (1 + p.G2c) * (2 - 5 * s1) * u2);   // 3298-3315
+            if (f & F_G2) res += q * ((5 * s1 - 2) * (2 - 5 * s2) * u1 + (5 * s2 - 2) * (2 - 5 * s1) * u2);  // 3340-3357
+            if (f & F_G3) res += q * ((p.fc1 - 1) * (2 - 5 * s2) * u1 + (p.fc2 - 1) * (2 - 5 * s1) * u2);   // 3382-3399
+        }
+        if (g5) {
+            if (f & F_DEN)   // functions.c:3142-3162 (D1 a along the line of sight, as written there)
+                res += H2 * (-3 * Om) *
+                       (chi2 * p.b1 * p.G2c * p.Dc1 * E2 * D2x * a2x * L.tab(S20, rA) +
+                        chi1 * p.b2 * p.G1c * p.Dc2 * E1 * D1x * a1x * L.tab(S20, rB));
+            if (f & F_RSD)   // functions.c:3191-3218
+                res += H2 * 3 * Om *
+                       (chi2 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * rsdk(r21, l2, rA) +
+                        chi1 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * rsdk(r22, l1, rB));
+            if (f & F_D1)    // functions.c:3237-3256
+                res += H2 * 3 * Om *
+                       (chi2 * p.cHc1 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * (l2 * c - chi1) * L.tab(S31, rA) +
+                        chi1 * p.cHc2 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * (l1 * c - chi2) * L.tab(S31, rB));
+            if (f & F_D2)    // functions.c:3277-3296
+                res += H2 * (-3 * Om) *
+                       (chi2 * (3 - p.fe1) * p.fc1 * cH2c1 * p.G2c * p.Dc1 * E2 * VA * ren1 +
+                        chi1 * (3 - p.fe2) * p.fc2 * cH2c2 * p.G1c * p.Dc2 * E1 * VB * ren2);
+            const double u1 = chi2 * p.Dc1 / p.ac1 * E2 * VA * ren1, u2 = chi1 * p.Dc2 / p.ac2 * E1 * VB * ren2;
+            if (f & F_G1) res += q * ((1 + p.G1c) * p.G2c * u1 + (1 + p.G2c) * p.G1c * u2);   // 3317-3338
+            if (f & F_G2) res += q * ((5 * s1 - 2) * p.G2c * u1 + (5 * s2 - 2) * p.G1c * u2);  // 3359-3380
+            if (f & F_G3) res += q * ((p.fc1 - 1) * p.G2c * u1 + (p.fc2 - 1) * p.G1c * u2);   // 3401-3422
+        }
+    }
+    return res;
+}
+
+// node q of the composite rule on [lo, hi]: `panels` equal panels of the n-point
+// Gauss-Legendre rule (gx, gw on [-1, 1]); returns the node, weight in *w
+__device__ inline double composite(double lo, double hi, int panels, int n, const double *gx, const double *gw, int q,
+                                   double *w)
+{
+    const int k = q / n, j = q - k * n;
+    const double step = (hi - lo) / panels;
+    const double e0 = lo + k * step, e1 = (k + 1 == panels) ? hi : lo + (k + 1) * step;
+    const double h = (e1 - e0) / 2;
+    *w = h * gw[j];
+    return (e0 + e1) / 2 + h * gx[j];
+}
+
+// the pieces of the single-integrated rule: [a, b] mapped about centre cc with width s
+struct piece {
+    double a, b, cc, s;
+};
+
+__device__ int single_pieces(const pt &p, piece *pc)
+{
+    const double cA = p.chi1 * p.c / p.chi2, sA = fmax(p.chi1 * p.sint / p.chi2, 1e-12);
+    const double cB = p.chi2 * p.c / p.chi1, sB = fmax(p.chi2 * p.sint / p.chi1, 1e-12);
+    double in[2];
+    int nin = 0;
+    if (cA > 0.0 && cA < 1.0) in[nin++] = cA;
+    if (cB > 0.0 && cB < 1.0 && !(nin == 1 && cB == cA)) in[nin++] = cB;
+    if (nin == 2 && in[1] < in[0]) {
+        const double t = in[0];
+        in[0] = in[1];
+        in[1] = t;
+    }
+    double cuts[5];
+    int nc = 0;
+    cuts[nc++] = 0.0;
+    if (nin == 2) {
+        cuts[nc++] = in[0];
+        cuts[nc++] = (in[0] + in[1]) / 2;
+        cuts[nc++] = in[1];
+    } else if (nin == 1) {
+        cuts[nc++] = in[0];
+    }
+    cuts[nc++] = 1.0;
+    int np = 0;
+    for (int i = 0; i + 1 < nc; ++i) {
+        const double a = cuts[i], b = cuts[i + 1];
+        if (b <= a) continue;
+        const double m = (a + b) / 2;
+        const double dA = fmin(fmin(fabs(cA - m), fabs(cA - a)), fabs(cA - b));
+        const double dB = fmin(fmin(fabs(cB - m), fabs(cB - a)), fabs(cB - b));
+        double cc = dA <= dB ? cA : cB;
+        const double s = dA <= dB ? sA : sB;
+        if (a < cc && cc < b) cc = fabs(cc - a) < fabs(cc - b) ? a : b;
+        pc[np++] = piece{a, b, cc, s};
+    }
+    return np;
+}
+
+__global__ void __launch_bounds__(256) k_integrated(intg_args a)
+{
+    const int idx = blockIdx.x;   // (z, mu, r), r fastest
+    const int ir = idx % a.nsep;
+    const int im = (idx / a.nsep) % a.nmu;
+    const int iz = idx / (a.nsep * a.nmu);
+    const pt p = make_pt(a, a.zmean[iz], a.mu[im], a.sep[ir]);
+    const unsigned f = a.flags;
+    double acc_s = 0.0, acc_d = 0.0;
+    const bool want_single =
+        ((f & F_LEN) && (f & (F_DEN | F_RSD | F_D1 | F_D2 | F_G1 | F_G2 | F_G3))) ||
+        ((f & (F_G4 | F_G5)) && (f & (F_DEN | F_RSD | F_D1 | F_D2 | F_G1 | F_G2 | F_G3)));
+    if (want_single) {
+        piece pc[4];
+        const int np = single_pieces(p, pc);
+        const int per = a.ps * a.ns;
+        for (int q = threadIdx.x; q < np * per; q += blockDim.x) {
+            const piece &P = pc[q / per];
+            const double sgn = P.a >= P.cc ? 1.0 : -1.0;
+            const double ua = asinh(sgn * (P.a - P.cc) / P.s), ub = asinh(sgn * (P.b - P.cc) / P.s);
+            double w;
+            const double u = composite(fmin(ua, ub), fmax(ua, ub), a.ps, a.ns, a.gsx, a.gsw, q % per, &w);
+            const double x = P.cc + sgn * P.s * sinh(u);
+            acc_s += w * P.s * cosh(u) * single_point(a, p, x);
+        }
+    }
+    if (f & (F_LEN | F_G4 | F_G5)) {
+        const double xb = p.chi1 * p.c > p.chi2 ? p.chi2 / (p.chi1 * p.c) : 1.0;
+        const int pA = a.p1, pB = xb < 1.0 ? (a.p1 / 2 > 1 ? a.p1 / 2 : 1) : 0;
+        const int n1 = (pA + pB) * a.n1, nuu = a.pu * a.nu;
+        for (int q = threadIdx.x; q < n1 * nuu; q += blockDim.x) {
+            const int i1 = q / nuu, iu = q - i1 * nuu;
+            double w1, wu;
+            const double x1 = i1 < pA * a.n1 ? composite(0.0, xb, pA, a.n1, a.g1x, a.g1w, i1, &w1)
+                                             : composite(xb, 1.0, pB, a.n1, a.g1x, a.g1w, i1 - pA * a.n1, &w1);
+            const double l1 = p.chi1 * x1, centre = l1 * p.c;
+            const double s = fmax(l1 * p.sint, 1e-9 * p.chi2);
+            const double u = composite(asinh((0.0 - centre) / s), asinh((p.chi2 - centre) / s), a.pu, a.nu, a.gux,
+                                       a.guw, iu, &wu);
+            const double lam2 = centre + s * sinh(u);
+            acc_d += w1 * wu * s * cosh(u) / p.chi2 * double_point(a, p, x1, lam2 / p.chi2);
+        }
+    }
+    // workgroup sums: wavefront shuffles, then one slot per wavefront in LDS
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        acc_s += __shfl_down(acc_s, o);
+        acc_d += __shfl_down(acc_d, o);
+    }
+    __shared__ double red[2][4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[0][wv] = acc_s;
+        red[1][wv] = acc_d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ss = 0.0, dd = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            ss += red[0][k];
+            dd += red[1][k];
+        }
+        a.out[2 * (int64_t)idx] = ss;
+        a.out[2 * (int64_t)idx + 1] = dd;
+    }
+}
+
+}  // namespace
+
+extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigned flags, int nz, const double *d_z,
+                                    int nmu, const double *d_mu, int nsep, const double *d_sep, const int *rule,
+                                    const double *d_gl, double *d_out)
+{
+    intg_args a;
+    a.nbg = in->nbg;
+    a.bg_z = in->bg_z;
+    a.bg_chi = in->bg_chi;
+    a.bg_D1 = in->bg_D1;
+    a.bg_a = in->bg_a;
+    a.bg_f = in->bg_f;
+    a.bg_cH = in->bg_cH;
+    a.bg_G1 = in->bg_G1;
+    a.bg_G2 = in->bg_G2;
+    a.nb = in->nb;
+    a.bias_z = in->bias_z;
+    a.b1 = in->b1;
+    a.b2 = in->b2;
+    a.nmb = in->nmb;
+    a.mb_z = in->mb_z;
+    a.s1 = in->s1;
+    a.s2 = in->s2;
+    a.fe1 = in->fe1;
+    a.fe2 = in->fe2;
+    a.tab_x = in->tab_x;
+    a.tab_y = in->tab_y;
+    for (int t = 0; t < NSLOT; ++t) {
+        a.tab_off[t] = in->tab_off[t];
+        a.tab_len[t] = in->tab_len[t];
+    }
+    a.nren0 = in->nren0;
+    a.ren0_x = in->ren0_x;
+    a.ren0_y = in->ren0_y;
+    a.nren = in->nren;
+    a.ren_chi = in->ren_chi;
+    a.ren_v = in->ren_v;
+    a.ren_x = in->ren_x;
+    a.ren_y = in->ren_y;
+    a.ren_xy = in->ren_xy;
+    a.Om = in->Om;
+    a.H0h = in->H0h;
+    a.flags = flags;
+    a.nz = nz;
+    a.nmu = nmu;
+    a.nsep = nsep;
+    a.zmean = d_z;
+    a.mu = d_mu;
+    a.sep = d_sep;
+    a.p1 = rule[0];
+    a.n1 = rule[1];
+    a.pu = rule[2];
+    a.nu = rule[3];
+    a.ps = rule[4];
+    a.ns = rule[5];
+    a.g1x = d_gl;
+    a.g1w = d_gl + a.n1;
+    a.gux = d_gl + 2 * a.n1;
+    a.guw = a.gux + a.nu;
+    a.gsx = a.guw + a.nu;
+    a.gsw = a.gsx + a.ns;
+    a.out = d_out;
+    const int64_t blocks = (int64_t)nz * nmu * nsep;
+    if (blocks <= 0 || blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_integrated, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}

@@ -325,6 +325,71 @@ def flat_lensing_corrfunc(zpar, mus, sep, bg, table, Omega_m, H0h, want_ll, want
     return out
 
 
+# the contributions of coffe_integrated_corrfunc (COFFE_INTEGRATED_*)
+INTEGRATED_FLAGS = {"den": 1, "rsd": 2, "d1": 4, "d2": 8, "g1": 16, "g2": 32, "g3": 64, "len": 128, "g4": 256,
+                    "g5": 512, "only_cross": 1024, "flat_ll": 2048, "flat_dl": 4096}
+
+
+def integrated_corrfunc(bg, bias_z, bias1, bias2, mb_z, s1, s2, fevo1, fevo2, Omega_m, H0h, tables, flags, zmean,
+                        mus, sep, ren=None, ren0=None, rule=None):
+    """coffe_integrated_corrfunc: the single- and double-integrated correlation function,
+    full sky, at every (z_mean, mu, sep); returns (single, double), each (nz, nmu, nsep).
+    bg: dict of host arrays z, chi, D1, a, f, cH, G1, G2; tables: {(n, l): (x, y)} (the
+    standard tables and (4, 0), the divergent r^4 I_0^4); flags: names of INTEGRATED_FLAGS;
+    ren: (chi, grid) the renormalisation grid and ren0: (sep, values) the r = 0
+    renormalisation (with g4 / g5); rule: None or (p1, n1, pu, nu, ps, ns)."""
+    L = load_library()
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    keep = []
+
+    def arr(a):
+        v = f64(a)
+        keep.append(v)
+        return v.ctypes.data_as(dp), len(v)
+
+    unknown = set(flags) - set(INTEGRATED_FLAGS)
+    if unknown:
+        raise ValueError(f"unknown contributions {sorted(unknown)}")
+    mask = 0
+    for name in flags:
+        mask |= INTEGRATED_FLAGS[name]
+    inp = MuInputs()
+    for name, key in (("bg_z", "z"), ("bg_chi", "chi"), ("bg_D1", "D1"), ("bg_f", "f"), ("bg_cH", "cH"),
+                      ("bg_G1", "G1"), ("bg_G2", "G2"), ("bg_a", "a")):
+        ptr, n = arr(bg[key])
+        setattr(inp, name, ptr)
+    inp.nbg = n
+    inp.bias_z, inp.nb = arr(bias_z)
+    inp.bias1, _ = arr(bias1)
+    inp.bias2, _ = arr(bias2)
+    inp.mb_z, inp.nmb = arr(mb_z)
+    for name, v in (("s1", s1), ("s2", s2), ("fevo1", fevo1), ("fevo2", fevo2)):
+        setattr(inp, name, arr(v)[0])
+    inp.H0h, inp.Omega_m = float(H0h), float(Omega_m)
+    n0, x0, y0 = 0, None, None
+    if ren is not None:
+        inp.ren_chi, inp.nren = arr(ren[0])
+        inp.ren, _ = arr(np.asarray(ren[1]).reshape(-1))
+    if ren0 is not None:
+        x0, n0 = arr(ren0[0])
+        y0, _ = arr(ren0[1])
+    keys = list(tables)
+    tn = (ctypes.c_int * len(keys))(*[int(t[0]) for t in keys])
+    tl = (ctypes.c_int * len(keys))(*[int(t[1]) for t in keys])
+    tlen = (ctypes.c_int * len(keys))(*[len(tables[t][0]) for t in keys])
+    tx, _ = arr(np.concatenate([f64(tables[t][0]) for t in keys]) if keys else np.zeros(1))
+    ty, _ = arr(np.concatenate([f64(tables[t][1]) for t in keys]) if keys else np.zeros(1))
+    zm, mu, sp = f64(np.atleast_1d(zmean)), f64(np.atleast_1d(mus)), f64(np.atleast_1d(sep))
+    r = (ctypes.c_int * 6)(*rule) if rule is not None else None
+    out = np.empty((len(zm), len(mu), len(sp), 2))
+    check(L.coffe_integrated_corrfunc(ctypes.byref(inp), ctypes.c_uint(mask), len(keys), tn, tl, tlen, tx, ty, n0, x0,
+                                      y0, len(zm), zm.ctypes.data_as(dp), len(mu), mu.ctypes.data_as(dp), len(sp),
+                                      sp.ctypes.data_as(dp), r, out.ctypes.data_as(dp)), "coffe_integrated_corrfunc")
+    return out[..., 0], out[..., 1]
+
+
 # separations of the divergent table I_0^4, in Mpc (integrals.c:76-99)
 COFFE_SEP = np.array(
     [0.000001, 0.000005, 0.0001, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.4, 0.8, 1., 1.5]

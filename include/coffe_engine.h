@@ -410,6 +410,54 @@ int coffe_flat_lensing_corrfunc(int nz, const double *zpar, int nmu, const doubl
                                 const double *tab_x, const double *tab_y, double Omega_m, double H0h, int want_ll,
                                 int want_dl, int panels, double *out);
 
+/* contributions of coffe_integrated_corrfunc (the reference's correlation_contrib
+   switches, common.h, and its flat-sky switches) */
+enum {
+    COFFE_INTEGRATED_DEN = 1u << 0,
+    COFFE_INTEGRATED_RSD = 1u << 1,
+    COFFE_INTEGRATED_D1 = 1u << 2,
+    COFFE_INTEGRATED_D2 = 1u << 3,
+    COFFE_INTEGRATED_G1 = 1u << 4,
+    COFFE_INTEGRATED_G2 = 1u << 5,
+    COFFE_INTEGRATED_G3 = 1u << 6,
+    COFFE_INTEGRATED_LEN = 1u << 7,
+    COFFE_INTEGRATED_G4 = 1u << 8,
+    COFFE_INTEGRATED_G5 = 1u << 9,
+    COFFE_INTEGRATED_ONLY_CROSS = 1u << 10,   /* only_cross_correlations: no auto terms */
+    COFFE_INTEGRATED_FLAT_LL = 1u << 11,      /* flatsky_nonlocal: lensing-lensing left to the flat-sky forms */
+    COFFE_INTEGRATED_FLAT_DL = 1u << 12       /* flatsky_local_nonlocal: density/RSD/d1-lensing likewise */
+};
+
+/*
+ * The integrated terms of the correlation function, full sky, at every (z_mean, mu,
+ * separation): the single-integrated terms (functions_single_integrated, functions.c:
+ * 1486-3458: density, RSD, d1, d2, g1-g3 against lensing, g4, g5) and the double-
+ * integrated ones (functions_double_integrated, functions.c:3460-4017: lensing, g4 and
+ * g5 with each other), integrated over the line-of-sight variables as coffe_integrate
+ * does for CORRFUNC (signal.c:556-741), with deterministic product rules in place of
+ * the reference's QAG / double-exponential and CUBA / Monte Carlo integrators.
+ * out[((iz nmu + im) nsep + ir) 2 + {0, 1}] = (single, double).
+ *   in        background (bg_a required), galaxy biases, magnification and evolution
+ *             biases, Omega_m, H0 h, and with g4 / g5 the renormalisation grid of I_0^4
+ *             (integrals.c:848-907) as for coffe_mu_weights_ex;
+ *   flags     COFFE_INTEGRATED_* (the contributions present);
+ *   tables    ntab tables (tab_n, tab_l), concatenated in tab_x / tab_y with lengths
+ *             tab_len: the (n, l) of (0,0), (0,2), (0,4), (1,1), (1,3), (2,0), (2,2),
+ *             (3,1) as coffe_integral_table gives them and (4, 0) the divergent r^4
+ *             I_0^4 table; those the flags need must be present (5 nodes or more);
+ *   ren0      with g4 / g5: the r = 0 renormalisation of I_0^4 against separation
+ *             (integrals.c:829-835, first node (0, 0));
+ *   rule      {p1, n1, pu, nu, ps, ns} or NULL for {8, 16, 12, 16, 32, 16}: x1 as p1
+ *             panels of n1 Gauss-Legendre nodes, the sinh-mapped x2 as pu panels of
+ *             nu, each single-integrated piece as ps panels of ns.
+ * Synchronous on the default engine; COFFE_EINVAL for bad sizes, flags or a missing
+ * table, NaN where a lookup leaves its table.
+ */
+int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int ntab, const int *tab_n, const int *tab_l,
+                              const int *tab_len, const double *tab_x, const double *tab_y, int nren0,
+                              const double *ren0_x, const double *ren0_y, int nz, const double *zmean, int nmu,
+                              const double *mu, int nsep, const double *sep, const int *rule, double *out);
+
 /* synchronise the engine's stream */
 int coffe_engine_synchronize(coffe_engine *e);
 

@@ -53,6 +53,8 @@ typedef struct orc_bicubic orc_bicubic;
 orc_bicubic *orc_bicubic_new(size_t nx, size_t ny, const double *x, const double *y, const double *z);
 void orc_bicubic_free(orc_bicubic *s);
 double orc_bicubic_eval(const orc_bicubic *s, double xq, double yq);
+int orc_bicubic_interp(size_t nx, size_t ny, const double *x, const double *y, const double *z, size_t nq,
+                       const double *xq, const double *yq, double *out);
 
 /* quadrature and special functions (orc_quad.c) */
 typedef double (*orc_fn)(double, void *);

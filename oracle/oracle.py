@@ -40,6 +40,7 @@ def _declare(L):
     L.orc_rfft2_c2r.argtypes = [s, s, p, p]
     L.orc_cspline_interp.argtypes = [s, p, p, s, p, p]
     L.orc_akima_interp.argtypes = [s, p, p, s, p, p]
+    L.orc_bicubic_interp.argtypes = [s, s, p, p, p, s, p, p, p]
     L.orc_gauss_legendre.argtypes = [s, p, p]
     L.orc_legendre_pl.argtypes = [i, d]
     L.orc_legendre_pl.restype = d
@@ -163,6 +164,18 @@ def akima(x, y, xq):
     x, y, xq = _f64(x), _f64(y), _f64(np.atleast_1d(xq))
     out = np.empty(len(xq))
     _check(lib().orc_akima_interp(len(x), _p(x), _p(y), len(xq), _p(xq), _p(out)), "akima")
+    return out
+
+
+def bicubic(x, y, z, xq, yq):
+    """GSL's bicubic interpolant of z[j, i] = f(x_i, y_j) (natural-spline node
+    derivatives) at the points (xq, yq); NaN outside the grid."""
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    xq, yq = np.broadcast_arrays(_f64(np.atleast_1d(xq)), _f64(np.atleast_1d(yq)))
+    xq, yq = _f64(xq.ravel()), _f64(yq.ravel())
+    out = np.empty(len(xq))
+    _check(lib().orc_bicubic_interp(len(x), len(y), _p(x), _p(y), _p(z), len(xq), _p(xq), _p(yq), _p(out)),
+           "bicubic")
     return out
 
 

@@ -353,3 +353,14 @@ double orc_bicubic_eval(const orc_bicubic *s, double xq, double yq)
         }
     return r;
 }
+
+/* the bicubic interpolant of z (z[j nx + i] = f(x_i, y_j)) at nq points (NaN outside) */
+int orc_bicubic_interp(size_t nx, size_t ny, const double *x, const double *y, const double *z, size_t nq,
+                       const double *xq, const double *yq, double *out)
+{
+    orc_bicubic *s = orc_bicubic_new(nx, ny, x, y, z);
+    if (!s) return ORC_ENOMEM;
+    for (size_t q = 0; q < nq; ++q) out[q] = orc_bicubic_eval(s, xq[q], yq[q]);
+    orc_bicubic_free(s);
+    return ORC_OK;
+}

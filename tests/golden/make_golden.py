@@ -29,9 +29,13 @@ def main(ref):
     d = np.loadtxt(os.path.join(bench, "benchmark_background.dat"))
     np.savetxt(os.path.join(out, "benchmark_background.dat"), d[::50], fmt="%.10e",
                header="decimated copy of tests/benchmarks/benchmark_background.dat: z a H cH cH' D1 f chi")
-    for pre in ("den", "rsd", "d1", "std", "std_d1", "d2", "g1", "g2", "g3"):
+    for pre in ("den", "rsd", "d1", "std", "std_d1", "d2", "g1", "g2", "g3", "len", "g4", "g5", "std_len", "all",
+                "flatsky_lensing_lensing", "flatsky_density_lensing"):
         for l in (0, 2, 4):
             shutil.copy(os.path.join(bench, f"benchmark_{pre}_multipoles{l}.dat"), out)
+    for pre in ("len", "g4", "g5", "std_len", "all", "flatsky_lensing_lensing"):
+        for i in range(5):
+            shutil.copy(os.path.join(bench, f"benchmark_{pre}_corrfunc{i}.dat"), out)
     for l1 in (0, 2, 4):
         for l2 in (0, 2, 4):
             shutil.copy(os.path.join(bench, f"benchmark_multipoles_covariance_{l1}{l2}.dat"), out)

@@ -59,29 +59,37 @@ def test_parameters():
         c.integral(r=100, n=0.5, l=0)
     with pytest.raises(AttributeError):
         c.set_parameters(no_such_parameter=1)
-    with pytest.raises(NotImplementedError):
-        c.has_g4 = True
+    c.has_g4 = True
     c.has_g5 = False
+    assert c.has_g4 and not c.has_g5 and c.parameters["has_g4"]
+    c.reset_contributions()
+    assert not any(v for k, v in c.parameters.items() if k.startswith("has_"))
 
 
-def test_lensing_paths_not_built_are_refused():
-    """Flat-sky lensing is computed; the full-sky lensing integrals (Monte Carlo in the
-    reference, signal.c:556-771) raise before any device work."""
-    c = Coffe(has_lensing=True)
+def test_lensing_plan():
+    """Which integrated terms go where (signal.c:556-771): the flat-sky forms with the
+    flat-sky switches, the rest to the full-sky integrals; the redshift-averaged
+    multipoles and the cosmological tangent refuse the integrated terms."""
+    c = Coffe(has_lensing=True, has_density=False)
     c.set_power_spectrum_linear(np.logspace(-4, 1, 50), np.logspace(-4, 1, 50) ** -1)
-    with pytest.raises(NotImplementedError):          # lensing-lensing needs has_flatsky_nonlocal
-        c.compute_multipoles_bulk()
+    assert c._lensing_plan() == dict(flat_ll=False, flat_dl=False, flags={"len"})
     c.has_flatsky_nonlocal = True
+    assert c._lensing_plan() == dict(flat_ll=True, flat_dl=False, flags=None)
     c.has_density = True
-    with pytest.raises(NotImplementedError):          # density-lensing needs has_flatsky_local_nonlocal
-        c.compute_multipoles_bulk()
+    plan = c._lensing_plan()
+    assert plan["flat_ll"] and not plan["flat_dl"] and plan["flags"] == {"den", "len", "flat_ll"}
     c.has_flatsky_local_nonlocal = True
-    c.has_d2 = True
-    with pytest.raises(NotImplementedError):          # d2-lensing: single-integrated
-        c.compute_multipoles_bulk()
-    with pytest.raises(NotImplementedError):          # the same refusals for the correlation function
-        c.compute_corrfunc_bulk()
+    plan = c._lensing_plan()
+    assert plan["flat_ll"] and plan["flat_dl"] and plan["flags"] is None
+    c.has_d2 = True          # d2-lensing stays full sky
+    assert c._lensing_plan()["flags"] == {"den", "d2", "len", "flat_ll", "flat_dl"}
     c.has_d2 = False
+    c.has_only_cross_correlations = True
+    plan = c._lensing_plan()
+    assert not plan["flat_ll"] and plan["flat_dl"]
+    c.reset_contributions()
+    c.has_g4 = True
+    assert c._lensing_plan() == dict(flat_ll=False, flat_dl=False, flags={"g4"})
     with pytest.raises(NotImplementedError):
         c.compute_average_multipoles_bulk()
 
