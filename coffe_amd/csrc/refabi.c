@@ -723,6 +723,169 @@ static int flat_eval(const coffe_parameters_t *par, const coffe_background_t *bg
     return st;
 }
 
+/* ---- the full-sky integrated terms (signal.c:556-771) -------------------------------- */
+
+#define REF_INTG_MU_NODES 64   /* theta nodes (mu = cos theta) of the integrated multipoles */
+
+/* the contributions of the integrated kernel and whether any full-sky term remains
+   (lensing-lensing and density/RSD/d1-lensing left to the flat-sky forms under the
+   flat-sky switches, signal.c:612-628, 704-736) */
+static unsigned intg_flags(const coffe_parameters_t *par, int *any)
+{
+    const coffe_correlation_contributions *c = &par->correlation_contrib;
+    unsigned f = (c->den ? COFFE_INTEGRATED_DEN : 0u) | (c->rsd ? COFFE_INTEGRATED_RSD : 0u) |
+                 (c->d1 ? COFFE_INTEGRATED_D1 : 0u) | (c->d2 ? COFFE_INTEGRATED_D2 : 0u) |
+                 (c->g1 ? COFFE_INTEGRATED_G1 : 0u) | (c->g2 ? COFFE_INTEGRATED_G2 : 0u) |
+                 (c->g3 ? COFFE_INTEGRATED_G3 : 0u) | (c->len ? COFFE_INTEGRATED_LEN : 0u) |
+                 (c->g4 ? COFFE_INTEGRATED_G4 : 0u) | (c->g5 ? COFFE_INTEGRATED_G5 : 0u);
+    const int oc = par->only_cross_correlations != 0, fl = par->flatsky_nonlocal != 0,
+              fdl = par->flatsky_local_nonlocal != 0;
+    if (oc) f |= COFFE_INTEGRATED_ONLY_CROSS;
+    if (fl) f |= COFFE_INTEGRATED_FLAT_LL;
+    if (fdl) f |= COFFE_INTEGRATED_FLAT_DL;
+    const int local = local_flags(par), g45 = c->g4 || c->g5;
+    const int dl_local = (c->den || c->rsd || c->d1) && !fdl;
+    const int dbl = (c->len && !oc && !fl) || (g45 && !oc) || (c->len && g45) || (c->g4 && c->g5);
+    const int sgl = (c->len && (dl_local || c->d2 || c->g1 || c->g2 || c->g3)) || (g45 && local);
+    *any = dbl || sgl;
+    return f;
+}
+
+/* the single- and double-integrated correlation function at every (z, mu, sep) from the
+   integral array's own tables (coffe_integrated_corrfunc): out[z][mu][r][2] */
+static int intg_corrfunc(const coffe_parameters_t *par, const coffe_background_t *bg,
+                         const coffe_integral_array_t *integral, unsigned flags, int nz, const double *zs, int nmu,
+                         const double *mus, int nsep, const double *seps, double *out)
+{
+    if (!bg || !integral) return COFFE_EINVAL;
+    if (par->interp_method != COFFE_REF_INTERP_AKIMA || par->pk_type != 0 || par->b_derivative ||
+        par->f_derivative || par->b_tilde_derivative || par->f_tilde_derivative)
+        return COFFE_EUNSUPPORTED;
+    coffe_mu_inputs in;
+    memset(&in, 0, sizeof(in));
+    nodes chi, v;
+    if (nodes_of(&bg->comoving_distance, &chi) != COFFE_OK) return COFFE_EINVAL;
+    in.nbg = (int)chi.n;
+    in.bg_z = chi.x;
+    in.bg_chi = chi.y;
+    const coffe_interpolation *bgs[6] = {&bg->D1, &bg->f, &bg->conformal_Hz, &bg->G1, &bg->G2, &bg->a};
+    const double **dst[6] = {&in.bg_D1, &in.bg_f, &in.bg_cH, &in.bg_G1, &in.bg_G2, &in.bg_a};
+    for (int i = 0; i < 6; ++i) {
+        if (nodes_of(bgs[i], &v) != COFFE_OK) return COFFE_EINVAL;
+        if (!same_nodes(&v, &chi)) return COFFE_EUNSUPPORTED;
+        *dst[i] = v.y;
+    }
+    nodes b1, b2, m[4];
+    if (nodes_of(&par->galaxy_bias1, &b1) != COFFE_OK || nodes_of(&par->galaxy_bias2, &b2) != COFFE_OK)
+        return COFFE_EINVAL;
+    if (!same_nodes(&b1, &b2)) return COFFE_EUNSUPPORTED;
+    in.nb = (int)b1.n;
+    in.bias_z = b1.x;
+    in.bias1 = b1.y;
+    in.bias2 = b2.y;
+    const coffe_interpolation *ms[4] = {&par->magnification_bias1, &par->magnification_bias2, &par->evolution_bias1,
+                                        &par->evolution_bias2};
+    for (int i = 0; i < 4; ++i) {
+        if (nodes_of(ms[i], &m[i]) != COFFE_OK) return COFFE_EINVAL;
+        if (!same_nodes(&m[i], &m[0])) return COFFE_EUNSUPPORTED;
+    }
+    in.nmb = (int)m[0].n;
+    in.mb_z = m[0].x;
+    in.s1 = m[0].y;
+    in.s2 = m[1].y;
+    in.fevo1 = m[2].y;
+    in.fevo2 = m[3].y;
+    in.H0h = REF_H0 * par->h;
+    in.Omega_m = par->Omega0_cdm + par->Omega0_baryon;
+    /* the 8 tables and, with g4 / g5, the divergent one and its renormalisations */
+    int tn[9], tl[9], tlen[9], nt = 0;
+    nodes tabs[9];
+    for (int k = 0; k < 8; ++k) {
+        const coffe_integral_t *I = coffe_find_integral(integral, STD_N[k], STD_L[k], 0, 0);
+        if (!I || nodes_of(&I->result, &tabs[nt]) != COFFE_OK) return COFFE_EINVAL;
+        tn[nt] = STD_N[k];
+        tl[nt] = STD_L[k];
+        tlen[nt] = (int)tabs[nt].n;
+        ++nt;
+    }
+    nodes r0 = {NULL, NULL, 0};
+    if (flags & (COFFE_INTEGRATED_G4 | COFFE_INTEGRATED_G5)) {
+        const coffe_integral_t *I = coffe_find_integral(integral, 4, 0, 0, 0);
+        if (!I || nodes_of(&I->result, &tabs[nt]) != COFFE_OK ||
+            nodes_of(&I->renormalization_zero_separation, &r0) != COFFE_OK)
+            return COFFE_EINVAL;
+        const gsl_spline2d_view *r = (const gsl_spline2d_view *)I->renormalization.spline;
+        if (!r || !r->xarr || !r->yarr || !r->zarr) return COFFE_EINVAL;
+        if (r->obj.xsize != r->obj.ysize || memcmp(r->xarr, r->yarr, sizeof(double) * r->obj.xsize) != 0 ||
+            par->covariance_interpolation_method != COFFE_REF_INTERP2D_BICUBIC)
+            return COFFE_EUNSUPPORTED;
+        in.nren = (int)r->obj.xsize;
+        in.ren_chi = r->xarr;
+        in.ren = r->zarr;
+        tn[nt] = 4;
+        tl[nt] = 0;
+        tlen[nt] = (int)tabs[nt].n;
+        ++nt;
+    }
+    size_t total = 0;
+    for (int t = 0; t < nt; ++t) total += tabs[t].n;
+    double *tx = (double *)malloc(sizeof(double) * total), *ty = (double *)malloc(sizeof(double) * total);
+    int st = (tx && ty) ? COFFE_OK : COFFE_ENOMEM;
+    for (int t = 0, o = 0; st == COFFE_OK && t < nt; o += tlen[t], ++t) {
+        memcpy(tx + o, tabs[t].x, sizeof(double) * tabs[t].n);
+        memcpy(ty + o, tabs[t].y, sizeof(double) * tabs[t].n);
+    }
+    if (st == COFFE_OK)
+        st = coffe_integrated_corrfunc(&in, flags, nt, tn, tl, tlen, tx, ty, (int)r0.n, r0.x, r0.y, nz, zs, nmu, mus,
+                                       nsep, seps, NULL, out);
+    free(tx);
+    free(ty);
+    return st;
+}
+
+/* one component (0 single, 1 double, -1 both) of the integrated multipoles: out[z][l][r]
+   = (2 l + 1) / 2 int dmu P_l(mu) xi(mu), theta (mu = cos theta) on a REF_INTG_MU_NODES
+   Gauss-Legendre rule */
+static int intg_multipoles(const coffe_parameters_t *par, const coffe_background_t *bg,
+                           const coffe_integral_array_t *integral, unsigned flags, int which, int nz, const double *zs,
+                           int nl, const int *ls, int nsep, const double *seps, double *out)
+{
+    const int nm = REF_INTG_MU_NODES;
+    double t[REF_INTG_MU_NODES], w[REF_INTG_MU_NODES], mu[REF_INTG_MU_NODES], wm[REF_INTG_MU_NODES];
+    gauss_legendre01(nm, t, w);
+    for (int j = 0; j < nm; ++j) {
+        const double th = M_PI * t[j];
+        mu[j] = cos(th);
+        wm[j] = M_PI * w[j] * sin(th);
+    }
+    double *xi = (double *)malloc(sizeof(double) * 2 * (size_t)nz * nm * nsep);
+    if (!xi) return COFFE_ENOMEM;
+    int st = intg_corrfunc(par, bg, integral, flags, nz, zs, nm, mu, nsep, seps, xi);
+    for (int z = 0; st == COFFE_OK && z < nz; ++z)
+        for (int il = 0; il < nl; ++il) {
+            const int l = ls[il];
+            for (int r = 0; r < nsep; ++r) {
+                double acc = 0.0;
+                for (int j = 0; j < nm; ++j) {
+                    /* P_l(mu) by the three-term recurrence */
+                    double p0 = 1.0, p1 = mu[j];
+                    double pl = l == 0 ? 1.0 : mu[j];
+                    for (int k = 2; k <= l; ++k) {
+                        pl = ((2.0 * k - 1.0) * mu[j] * p1 - (k - 1.0) * p0) / k;
+                        p0 = p1;
+                        p1 = pl;
+                    }
+                    const double *q = xi + 2 * (((size_t)z * nm + j) * nsep + r);
+                    const double val = which < 0 ? q[0] + q[1] : q[which];
+                    acc += wm[j] * pl * val;
+                }
+                out[((size_t)z * nl + il) * nsep + r] = (2.0 * l + 1.0) / 2.0 * acc;
+            }
+        }
+    free(xi);
+    return st;
+}
+
 /* ---- signal.c ------------------------------------------------------------------------ */
 
 double coffe_integrate(coffe_parameters_t *par, coffe_background_t *bg, coffe_integral_array_t *integral,
@@ -761,38 +924,58 @@ double coffe_integrate(coffe_parameters_t *par, coffe_background_t *bg, coffe_in
         break;
     }
     case COFFE_REF_SINGLE_INTEGRATED: {
-        /* signal.c:556-683: lensing, g4 or g5 with a local term.  Only density-lensing in
-           flat sky (flatsky_local_nonlocal) is nonzero without Monte Carlo; lensing with
-           d2 / g1-g3, g4 / g5 with a local term and the full-sky terms are not built */
+        /* signal.c:556-683: lensing, g4 or g5 with a local term.  Density-lensing takes its
+           flat-sky form with flatsky_local_nonlocal (RSD- and Doppler-lensing vanish
+           there); the rest is the full-sky single integral */
         if (!(nonlocal && local)) break;
-        const int mc = (c->len && (c->d2 || c->g1 || c->g2 || c->g3)) || ((c->g4 || c->g5) && local) ||
-                       (c->len && !par->flatsky_local_nonlocal);
-        if (mc || flag_output == COFFE_REF_AVERAGE_MULTIPOLES) {
+        if (flag_output == COFFE_REF_AVERAGE_MULTIPOLES) {
             st = COFFE_EUNSUPPORTED;
             break;
         }
-        if (!c->den) break;   /* RSD- and Doppler-lensing are zero in flat sky */
-        if (flag_output == COFFE_REF_CORRFUNC)
-            st = flat_eval(par, bg, integral, 1, 0, 1, 1, &z_mean, 1, NULL, &mu, 1, &sep, &result);
-        else if (flag_output == COFFE_REF_MULTIPOLES)
-            st = flat_eval(par, bg, integral, 0, 0, 1, 1, &z_mean, 1, &l, NULL, 1, &sep, &result);
+        if (flag_output != COFFE_REF_CORRFUNC && flag_output != COFFE_REF_MULTIPOLES) break;
+        const int cf = flag_output == COFFE_REF_CORRFUNC;
+        if (c->len && c->den && par->flatsky_local_nonlocal)
+            st = cf ? flat_eval(par, bg, integral, 1, 0, 1, 1, &z_mean, 1, NULL, &mu, 1, &sep, &result)
+                    : flat_eval(par, bg, integral, 0, 0, 1, 1, &z_mean, 1, &l, NULL, 1, &sep, &result);
+        int any = 0;
+        const unsigned f = intg_flags(par, &any);
+        double part = 0.0, both[2];
+        if (st == COFFE_OK && any) {
+            if (cf) {
+                st = intg_corrfunc(par, bg, integral, f, 1, &z_mean, 1, &mu, 1, &sep, both);
+                part = both[0];
+            } else {
+                st = intg_multipoles(par, bg, integral, f, 0, 1, &z_mean, 1, &l, 1, &sep, &part);
+            }
+        }
+        result += part;
         break;
     }
     case COFFE_REF_DOUBLE_INTEGRATED: {
-        /* signal.c:685-771: lensing-lensing in flat sky (flatsky_nonlocal, not with only
-           the cross terms: an auto term) without Monte Carlo; g4, g5 and full-sky lensing
-           are not built */
+        /* signal.c:685-771: lensing, g4 and g5 with each other; lensing-lensing takes its
+           flat-sky form with flatsky_nonlocal (not with only the cross terms: an auto term) */
         if (!nonlocal) break;
-        if (c->g4 || c->g5 || (c->len && !par->flatsky_nonlocal && !par->only_cross_correlations) ||
-            flag_output == COFFE_REF_AVERAGE_MULTIPOLES) {
+        if (flag_output == COFFE_REF_AVERAGE_MULTIPOLES) {
             st = COFFE_EUNSUPPORTED;
             break;
         }
-        if (!c->len || par->only_cross_correlations) break;
-        if (flag_output == COFFE_REF_CORRFUNC)
-            st = flat_eval(par, bg, integral, 1, 1, 0, 1, &z_mean, 1, NULL, &mu, 1, &sep, &result);
-        else if (flag_output == COFFE_REF_MULTIPOLES)
-            st = flat_eval(par, bg, integral, 0, 1, 0, 1, &z_mean, 1, &l, NULL, 1, &sep, &result);
+        if (flag_output != COFFE_REF_CORRFUNC && flag_output != COFFE_REF_MULTIPOLES) break;
+        const int cf = flag_output == COFFE_REF_CORRFUNC;
+        if (c->len && par->flatsky_nonlocal && !par->only_cross_correlations)
+            st = cf ? flat_eval(par, bg, integral, 1, 1, 0, 1, &z_mean, 1, NULL, &mu, 1, &sep, &result)
+                    : flat_eval(par, bg, integral, 0, 1, 0, 1, &z_mean, 1, &l, NULL, 1, &sep, &result);
+        int any = 0;
+        const unsigned f = intg_flags(par, &any);
+        double part = 0.0, both[2];
+        if (st == COFFE_OK && any) {
+            if (cf) {
+                st = intg_corrfunc(par, bg, integral, f, 1, &z_mean, 1, &mu, 1, &sep, both);
+                part = both[1];
+            } else {
+                st = intg_multipoles(par, bg, integral, f, 1, 1, &z_mean, 1, &l, 1, &sep, &part);
+            }
+        }
+        result += part;
         break;
     }
     default:
@@ -831,14 +1014,10 @@ static int multipole_setup(const coffe_parameters_t *par, const coffe_background
     const coffe_correlation_contributions *c = &par->correlation_contrib;
     if (ll) *ll = 0;
     if (dl) *dl = 0;
-    if (c->g4 || c->g5) return COFFE_EUNSUPPORTED;   /* integrated terms */
-    if (c->len) {
-        const int local = c->den || c->rsd || c->d1;
-        if (!ll || c->d2 || c->g1 || c->g2 || c->g3 || (local && !par->flatsky_local_nonlocal) ||
-            (!par->only_cross_correlations && !par->flatsky_nonlocal))
-            return COFFE_EUNSUPPORTED;
-        *ll = !par->only_cross_correlations;
-        *dl = c->den != 0;
+    if (c->len || c->g4 || c->g5) {
+        if (!ll) return COFFE_EUNSUPPORTED;   /* the integrated terms of the averaged multipoles */
+        *ll = c->len && !par->only_cross_correlations && par->flatsky_nonlocal;
+        *dl = c->len && c->den && par->flatsky_local_nonlocal;
     }
     if (!par->sep || !par->multipole_values) return COFFE_EINVAL;
     int st = spline_ops(ops);
@@ -870,13 +1049,26 @@ int coffe_multipoles_init(coffe_parameters_t *par, coffe_background_t *bg, coffe
         return done(COFFE_ENOMEM);
     }
     st = local_eval_ls(&lc, &ops, 0, (int)nz, par->z_mean, (int)nl, par->multipole_values, (int)ns, par->sep, vals);
-    /* the flat-sky lensing terms, as multipoles.c adds the integrated outputs */
+    /* the flat-sky lensing terms and the full-sky integrated ones, as multipoles.c adds
+       the integrated outputs */
     if (st == COFFE_OK && lens) {
         st = flat_eval(par, bg, integral, 0, want_ll, want_dl, (int)nz, par->z_mean, (int)nl, par->multipole_values,
                        NULL, (int)ns, par->sep, lens);
         for (size_t q = 0; st == COFFE_OK && q < size; ++q) vals[q] += lens[q];
     }
     free(lens);
+    int any = 0;
+    const unsigned f = intg_flags(par, &any);
+    if (st == COFFE_OK && any && (par->correlation_contrib.len || par->correlation_contrib.g4 ||
+                                  par->correlation_contrib.g5)) {
+        double *part = (double *)malloc(sizeof(double) * size);
+        if (!part) st = COFFE_ENOMEM;
+        if (st == COFFE_OK)
+            st = intg_multipoles(par, bg, integral, f, -1, (int)nz, par->z_mean, (int)nl, par->multipole_values,
+                                 (int)ns, par->sep, part);
+        for (size_t q = 0; st == COFFE_OK && q < size; ++q) vals[q] += part[q];
+        free(part);
+    }
     if (st != COFFE_OK) {
         free(arr);
         free(vals);

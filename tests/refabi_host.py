@@ -289,6 +289,7 @@ class Program:
         c.den, c.rsd, c.d1 = int(p["has_density"]), int(p["has_rsd"]), int(p["has_d1"])
         c.d2, c.g1, c.g2, c.g3 = (int(p[k]) for k in ("has_d2", "has_g1", "has_g2", "has_g3"))
         c.len = int(p["has_lensing"])
+        c.g4, c.g5 = int(p["has_g4"]), int(p["has_g5"])
         par.bessel_bins = int(p["integrals_fftlog_sampling"] if bessel_bins is None else bessel_bins)
         par.Omega0_cdm, par.Omega0_baryon, par.Omega0_gamma = p["omega_cdm"], p["omega_baryon"], p["omega_gamma"]
         par.Omega0_m = p["omega_cdm"] + p["omega_baryon"]
@@ -311,7 +312,7 @@ class Program:
                           ("evolution_bias1", "fevo1"), ("evolution_bias2", "fevo2")):
             b = cosmo._mbias[key]
             setattr(par, name, host.spline(b.x, b.y))
-        dg = any(p[k] for k in ("has_d2", "has_g1", "has_g2", "has_g3"))
+        dg = any(p[k] for k in ("has_d2", "has_g1", "has_g2", "has_g3", "has_g4", "has_g5"))
         par.divergent = int(dg if divergent is None else divergent)
         par.interp_method = INTERP_AKIMA
         par.zmin, par.zmin_len = self._arr(p["z_min"])

@@ -328,3 +328,68 @@ def test_flat_lensing_corrfunc_integrate(lib):
             assert abs(got[-1] - want) <= 1e-12 * abs(want), (mu, r, got[-1], want)
         assert np.allclose(got, y, rtol=5e-4), (mu, got, y)
     L.coffe_integrals_free(ctypes.byref(ia))
+
+
+@pytest.mark.parametrize("flags", [
+    dict(has_lensing=True),
+    dict(has_g4=True, has_g5=True),
+    dict(has_density=True, has_rsd=True, has_lensing=True, has_g4=True),
+])
+def test_fullsky_integrated_multipoles_init(lib, flags):
+    """Full-sky lensing, g4 and g5 (signal.c:556-771) through coffe_integrals_init (the
+    divergent table and its renormalisations for g4 / g5) and coffe_multipoles_init:
+    the Python API to 1e-10 (the same kernel and mu rule); NONINTEGRATED +
+    SINGLE_INTEGRATED + DOUBLE_INTEGRATED of coffe_integrate equal each record."""
+    from coffe_amd.coffe import Coffe
+    L, host = lib
+    cosmo = Coffe(sep=np.array([20.0, 100.0]) / H, l=[0, 2])
+    cosmo.set_power_spectrum_linear(*_pk())
+    cosmo.reset_contributions()
+    cosmo.set_parameters(**flags)
+    prog = R.Program(host, cosmo)
+    print("coffe_multipoles_init full sky", flags, flush=True)
+    ia = _integrals(L, prog)
+    mp = R.MultipolesArray()
+    assert L.coffe_multipoles_init(ctypes.byref(prog.par), ctypes.byref(prog.bg), ctypes.byref(ia),
+                                   ctypes.byref(mp)) == 0
+    recs = [mp.array[i] for i in range(mp.size)]
+    py = cosmo.compute_multipoles_bulk()
+    assert len(recs) == len(py) == 4
+    for a, b in zip(recs, py):
+        assert (a.coords.z_mean, a.coords.separation, a.coords.l) == (b.z, b.r, b.l)
+        assert abs(a.value - b.value) <= 1e-10 * abs(b.value), (flags, b.l, b.r, a.value, b.value)
+    for rec in (recs[0], recs[3]):
+        one = sum(L.coffe_integrate(ctypes.byref(prog.par), ctypes.byref(prog.bg), ctypes.byref(ia),
+                                    rec.coords.z_mean, 0.0, 0.0, rec.coords.separation, 0.0, rec.coords.l, kind,
+                                    R.MULTIPOLES)
+                  for kind in (R.NONINTEGRATED, R.SINGLE_INTEGRATED, R.DOUBLE_INTEGRATED))
+        assert abs(one - rec.value) <= 1e-10 * abs(rec.value), (flags, rec.coords.l, one, rec.value)
+    L.coffe_multipoles_free(ctypes.byref(mp))
+    L.coffe_integrals_free(ctypes.byref(ia))
+
+
+def test_fullsky_lensing_corrfunc_integrate(lib):
+    """Density + RSD + lensing, full sky (test_corrfunc.c:239-280): NONINTEGRATED +
+    SINGLE_INTEGRATED + DOUBLE_INTEGRATED of coffe_integrate at five mu against the
+    reference's fixtures (5e-5) and the Python API (1e-10)."""
+    from coffe_amd.coffe import Coffe
+    L, host = lib
+    mus = [0.0, 0.2, 0.5, 0.8, 0.95]
+    seps = np.array([10, 20, 40, 100, 150]) / H
+    cosmo = Coffe(sep=seps, mu=mus, has_density=True, has_rsd=True, has_lensing=True)
+    cosmo.set_power_spectrum_linear(*_pk())
+    prog = R.Program(host, cosmo, output_type=R.CORRFUNC)
+    print("coffe_integrate full-sky lensing corrfunc", flush=True)
+    ia = _integrals(L, prog)
+    py = {(round(r.mu, 6), round(r.r, 9)): r.value for r in cosmo.compute_corrfunc_bulk()}
+    for i, mu in enumerate(mus):
+        x, y = np.loadtxt(os.path.join(GOLD, f"benchmark_std_len_corrfunc{i}.dat")).T
+        got = []
+        for r in seps:
+            got.append(sum(L.coffe_integrate(ctypes.byref(prog.par), ctypes.byref(prog.bg), ctypes.byref(ia), 1.0,
+                                             0.0, 0.0, float(r), mu, 0, kind, R.CORRFUNC)
+                           for kind in (R.NONINTEGRATED, R.SINGLE_INTEGRATED, R.DOUBLE_INTEGRATED)))
+            want = py[(round(mu, 6), round(float(r), 9))]
+            assert abs(got[-1] - want) <= 1e-10 * abs(want), (mu, r, got[-1], want)
+        assert np.allclose(got, y, rtol=5e-5, atol=0), (mu, got, y)
+    L.coffe_integrals_free(ctypes.byref(ia))

@@ -126,8 +126,10 @@ def test_covariance_find(lib):
 
 
 def test_integrate_flag_gating(lib):
-    """coffe_integrate returns 0 where the reference does (no contribution of the kind),
-    and NaN with EUNSUPPORTED for the integrated terms, before any device work."""
+    """coffe_integrate returns 0 where the reference does (no contribution of the kind);
+    the integrated terms need the background and tables (EINVAL without them) and are
+    not offered for the redshift-averaged multipoles (EUNSUPPORTED), before any device
+    work."""
     host = R.Host(lib)
     try:
         par = R.Parameters()
@@ -140,8 +142,11 @@ def test_integrate_flag_gating(lib):
         assert lib.coffe_integrate(ctypes.byref(par), *args, R.NONINTEGRATED, R.COVARIANCE_MULTIPOLES) == 0.0
         par.correlation_contrib.len = 1
         assert np.isnan(lib.coffe_integrate(ctypes.byref(par), *args, R.SINGLE_INTEGRATED, R.MULTIPOLES))
-        assert lib.coffe_engine_last_error() == EUNSUPPORTED
+        assert lib.coffe_engine_last_error() == 1
         assert np.isnan(lib.coffe_integrate(ctypes.byref(par), *args, R.DOUBLE_INTEGRATED, R.CORRFUNC))
+        assert lib.coffe_engine_last_error() == 1
+        assert np.isnan(lib.coffe_integrate(ctypes.byref(par), *args, R.DOUBLE_INTEGRATED, R.AVERAGE_MULTIPOLES))
+        assert lib.coffe_engine_last_error() == EUNSUPPORTED
         # the derivative outputs change the local terms' form
         par.correlation_contrib.len = 0
         par.b_derivative = 1
@@ -158,8 +163,12 @@ def test_integrate_flag_gating(lib):
         par.sep, par.sep_len = s.ctypes.data_as(R.DP), 1
         par.multipole_values, par.multipole_values_len = ls.ctypes.data_as(R.IP), 1
         mp = R.MultipolesArray()
-        assert lib.coffe_multipoles_init(ctypes.byref(par), None, None, ctypes.byref(mp)) == EUNSUPPORTED
+        assert lib.coffe_multipoles_init(ctypes.byref(par), None, None, ctypes.byref(mp)) == 1
         assert mp.size == 0 and not mp.array
+        par.zmin, par.zmin_len = z.ctypes.data_as(R.DP), 1
+        par.zmax, par.zmax_len = z.ctypes.data_as(R.DP), 1
+        ram = R.AverageMultipolesArray()
+        assert lib.coffe_average_multipoles_init(ctypes.byref(par), None, None, ctypes.byref(ram)) == EUNSUPPORTED
         par.flatsky_nonlocal = 1
         par.output_type = R.MULTIPOLES
         l6 = np.array([0, 6], dtype=np.int32)
