@@ -1048,9 +1048,10 @@ class Coffe:
     # theta-Gauss-Legendre nodes (mu = cos(theta)) of the integrated terms' multipoles
     INTEGRATED_MU_NODES = 64
 
-    def _integrated_corrfunc(self, zs, seps, mus):
+    def _integrated_corrfunc(self, zs, seps, mus, chi_max=None):
         """[z][mu][r] single- plus double-integrated correlation function, full sky
-        (coffe_integrated_corrfunc), or None when the plan has no full-sky terms."""
+        (coffe_integrated_corrfunc), or None when the plan has no full-sky terms.  chi_max:
+        the renormalisation grid's extent (default chi(max(z_mean + deltaz)))."""
         from .engine import integrated_corrfunc
         plan = self._lensing_plan()
         if plan is None or plan["flags"] is None:
@@ -1065,13 +1066,40 @@ class Coffe:
         if flags & {"g4", "g5"}:
             tables[(4, 0)] = self._divergent_table()
             ren0 = self._divergent_r0
-            # the grid spans chi(max(z_mean + deltaz)) (integrals.c:862-871)
-            zmax = max(float(z) + float(dz) for z, dz in zip(self._p["z_mean"], self._p["deltaz"]))
-            ren = self._ren_grid(self.comoving_distance(zmax))
+            if chi_max is None:
+                # the grid spans chi(max(z_mean + deltaz)) (integrals.c:862-871)
+                zmax = max(float(z) + float(dz) for z, dz in zip(self._p["z_mean"], self._p["deltaz"]))
+                chi_max = self.comoving_distance(zmax)
+            ren = self._ren_grid(chi_max)
         single, double = integrated_corrfunc(self._bg_tables(), bz, self._bias[1].y, self._bias[2].y, mz, s1, s2, e1,
                                              e2, self.omega_m, COFFE_H0 * self._p["h"], tables, flags, zs, mus, seps,
                                              ren=ren, ren0=ren0)
         return single + double
+
+    def _integrated_xi(self, zs, seps, mus, chi_max=None):
+        """[z][mu][r] every integrated term of the correlation function (the flat-sky
+        lensing forms, coffe_flat_lensing_corrfunc, and the full-sky integrals), or None
+        without lensing, g4 and g5."""
+        from .engine import flat_lensing_corrfunc, half_integer_table
+        plan = self._lensing_plan()
+        if plan is None:
+            return None
+        out = np.zeros((len(zs), len(mus), len(seps)))
+        if plan["flat_ll"] or plan["flat_dl"]:
+            # the (1/2, -1/2) table (integrals.c:917-962), cached with the other lensing tables
+            if self._lens_cf is None:
+                k, pk = self._pk
+                kmin, kmax = self._kwin()
+                self._lens_cf = half_integer_table(1, -1, k, pk, self._p["integrals_fftlog_sampling"], kmin, kmax)
+            zpar = [[float(z), self.comoving_distance(float(z)), self.growth_factor(float(z)),
+                     self._eval_mbias("s1", float(z)), self._eval_mbias("s2", float(z)),
+                     self._eval_bias(1, float(z)), self._eval_bias(2, float(z))] for z in zs]
+            out += flat_lensing_corrfunc(zpar, [float(m) for m in mus], seps, self._background(), self._lens_cf,
+                                         self.omega_m, COFFE_H0 * self._p["h"], plan["flat_ll"], plan["flat_dl"])
+        xi = self._integrated_corrfunc(zs, seps, [float(m) for m in mus], chi_max)
+        if xi is not None:
+            out += xi
+        return out
 
     def _integrated_multipoles(self, zs, seps, ls):
         """[z][l][r] multipoles of the full-sky integrated terms: (2 l + 1) / 2 int dmu
@@ -1136,27 +1164,14 @@ class Coffe:
         functions.c:89-1484, at each mu: the weights of a one-node rule, the node mu with
         weight 2 and l = 0), the flat-sky lensing terms (coffe_flat_lensing_corrfunc) and
         the full-sky integrated ones (coffe_integrated_corrfunc)."""
-        from .engine import flat_lensing_corrfunc, half_integer_table
-        plan = self._lensing_plan()
+        self._lensing_plan()
         out = np.zeros((len(zs), len(mus), len(seps)))
         if self._has_local():
             for j, mu in enumerate(mus):
                 out[:, j, :] = self._multipole_values(zs, seps, [0], rule=([float(mu)], [2.0]))[:, 0, :]
-        if plan is not None and (plan["flat_ll"] or plan["flat_dl"]):
-            # the (1/2, -1/2) table (integrals.c:917-962), cached with the other lensing tables
-            if self._lens_cf is None:
-                k, pk = self._pk
-                kmin, kmax = self._kwin()
-                self._lens_cf = half_integer_table(1, -1, k, pk, self._p["integrals_fftlog_sampling"], kmin, kmax)
-            zpar = [[float(z), self.comoving_distance(float(z)), self.growth_factor(float(z)),
-                     self._eval_mbias("s1", float(z)), self._eval_mbias("s2", float(z)),
-                     self._eval_bias(1, float(z)), self._eval_bias(2, float(z))] for z in zs]
-            out += flat_lensing_corrfunc(zpar, [float(m) for m in mus], seps, self._background(), self._lens_cf,
-                                         self.omega_m, COFFE_H0 * self._p["h"], plan["flat_ll"], plan["flat_dl"])
-        if plan is not None:
-            xi = self._integrated_corrfunc(zs, seps, [float(m) for m in mus])
-            if xi is not None:
-                out += xi
+        xi = self._integrated_xi(zs, seps, [float(m) for m in mus])
+        if xi is not None:
+            out += xi
         return out
 
     def compute_corrfunc(self, *, z, r, mu, recompute=False):
@@ -1231,13 +1246,28 @@ class Coffe:
         if not np.array_equal(bz, self._bias[2].x):
             raise ValueError("galaxy_bias1 and galaxy_bias2 must share their redshift sampling")
         # the d2 / g grid spans chi(max(z_max)) for averaged multipoles (integrals.c:873-879)
-        tables, dg = self._weights_setup(self.comoving_distance(float(max(self._p["z_max"]))))
+        chi_max = self.comoving_distance(float(max(self._p["z_max"])))
+        out = np.zeros((len(ls), len(seps)))
+        # the integrated terms (signal.c:312-470, 674-682, 759-767): their correlation
+        # function at the (z(v), mu) nodes of every separation, projected on P_l and
+        # averaged with the same weights
+        if self._lensing_plan() is not None:
+            t_, w_ = np.polynomial.legendre.leggauss(self.INTEGRATED_MU_NODES)
+            theta = np.pi / 2 * (t_ + 1)
+            mus, wm = np.cos(theta), np.pi / 2 * w_ * np.sin(theta)
+            pls = np.array([np.polynomial.legendre.legval(mus, [0] * int(l) + [1]) * (2 * int(l) + 1) / 2
+                            for l in ls])
+            for ir, r in enumerate(seps):
+                xi = self._integrated_xi(zn[ir], [float(r)], mus, chi_max)[:, :, 0]    # [v][mu]
+                out[:, ir] += np.einsum("v,vm,lm->l", wz[ir], xi * wm[None, :], pls)
+        if not self._has_local():
+            return out
+        tables, dg = self._weights_setup(chi_max)
         npair = len(seps) * nodes
         terms = self._terms(dg)
         if not terms:
-            return np.zeros((len(ls), len(seps)))
+            return out
         tv = self._table_values(tables, seps)
-        out = np.empty((len(ls), len(seps)))
         for l0 in range(0, len(ls), 4):   # at most 4 multipoles per weights launch
             lc = [int(v) for v in ls[l0:l0 + 4]]
             ren = torch.empty((npair, len(lc)), dtype=torch.float64, device="cuda") if dg else None
@@ -1246,7 +1276,7 @@ class Coffe:
                            nmu=self._p["multipole_quadrature_nodes"], terms=terms, dg=dg, ren_out=ren,
                            paired=True).cpu().numpy()
             A = A.reshape(len(seps), nodes, len(lc), len(tables))                 # [r][v][l][t]
-            out[l0:l0 + len(lc)] = np.einsum("rv,rvlt,tr->lr", wz, A, tv)
+            out[l0:l0 + len(lc)] += np.einsum("rv,rvlt,tr->lr", wz, A, tv)
             if ren is not None:
                 out[l0:l0 + len(lc)] += np.einsum("rv,rvl->lr", wz, ren.cpu().numpy().reshape(len(seps), nodes,
                                                                                               len(lc)))
@@ -1257,7 +1287,6 @@ class Coffe:
         l then r (average_multipoles.c:90-146)."""
         self._check_coords_average_multipoles()
         self._check_contributions()
-        self._no_lensing("the redshift-averaged multipoles")
         if recompute:
             self._free_all()
         if self._average_multipoles is None:

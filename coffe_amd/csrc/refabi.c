@@ -886,6 +886,78 @@ static int intg_multipoles(const coffe_parameters_t *par, const coffe_background
     return st;
 }
 
+/* the integrated terms of the multipoles averaged over [zmin, zmax] (signal.c:312-470,
+   674-682, 759-767): their correlation function (the flat-sky forms flat_ll / flat_dl
+   and the full-sky kernel's component `which`: 0 single, 1 double, -1 both) at the
+   redshift nodes of average_eval and the theta nodes of intg_multipoles, projected on
+   P_l and averaged with average_eval's weights; out[l][r] */
+static int intg_average(const coffe_parameters_t *par, const coffe_background_t *bg,
+                        const coffe_integral_array_t *integral, const coffe_spline_ops *ops, int which, int flat_ll,
+                        int flat_dl, double zmin, double zmax, int nsep, const double *seps, int nl, const int *ls,
+                        double *out)
+{
+    const int nv = REF_Z_NODES, nm = REF_INTG_MU_NODES;
+    double v[REF_Z_NODES], w[REF_Z_NODES], t[REF_INTG_MU_NODES], wt[REF_INTG_MU_NODES];
+    double mu[REF_INTG_MU_NODES], wm[REF_INTG_MU_NODES], zn[REF_Z_NODES], wz[REF_Z_NODES];
+    gauss_legendre01(nv, v, w);
+    gauss_legendre01(nm, t, wt);
+    for (int j = 0; j < nm; ++j) {
+        mu[j] = cos(M_PI * t[j]);
+        wm[j] = M_PI * wt[j] * sin(M_PI * t[j]);
+    }
+    int any = 0;
+    const unsigned f = intg_flags(par, &any);
+    double *xi = (double *)malloc(sizeof(double) * 2 * (size_t)nv * nm);
+    double *fl = (double *)malloc(sizeof(double) * (size_t)nv * nm);
+    double *tot = (double *)malloc(sizeof(double) * (size_t)nv * nm);
+    int st = (xi && fl && tot) ? COFFE_OK : COFFE_ENOMEM;
+    const double hH0 = par->h * REF_H0;
+    const double chi_lo = ops->interp_spline(&bg->comoving_distance, zmin);
+    const double chi_hi = ops->interp_spline(&bg->comoving_distance, zmax);
+    for (int r = 0; r < nsep && st == COFFE_OK; ++r) {
+        const double z1 = ops->interp_spline(&bg->z_as_chi, chi_lo + seps[r] / 2);
+        const double z2 = ops->interp_spline(&bg->z_as_chi, chi_hi - seps[r] / 2);
+        if (!(z2 > z1)) {
+            st = COFFE_EDOMAIN;
+            break;
+        }
+        for (int j = 0; j < nv; ++j) {
+            zn[j] = z1 + (z2 - z1) * v[j];
+            wz[j] = w[j] * hH0 / ops->interp_spline(&bg->Hz, zn[j]);
+        }
+        memset(tot, 0, sizeof(double) * (size_t)nv * nm);
+        if (flat_ll || flat_dl) {
+            st = flat_eval(par, bg, integral, 1, flat_ll, flat_dl, nv, zn, nm, NULL, mu, 1, &seps[r], fl);
+            for (size_t q = 0; st == COFFE_OK && q < (size_t)nv * nm; ++q) tot[q] += fl[q];
+        }
+        if (st == COFFE_OK && any) {
+            st = intg_corrfunc(par, bg, integral, f, nv, zn, nm, mu, 1, &seps[r], xi);
+            for (size_t q = 0; st == COFFE_OK && q < (size_t)nv * nm; ++q)
+                tot[q] += which < 0 ? xi[2 * q] + xi[2 * q + 1] : xi[2 * q + which];
+        }
+        for (int il = 0; st == COFFE_OK && il < nl; ++il) {
+            const int l = ls[il];
+            double acc = 0.0;
+            for (int m = 0; m < nm; ++m) {
+                double p0 = 1.0, p1 = mu[m], pl = l == 0 ? 1.0 : mu[m];
+                for (int k = 2; k <= l; ++k) {
+                    pl = ((2.0 * k - 1.0) * mu[m] * p1 - (k - 1.0) * p0) / k;
+                    p0 = p1;
+                    p1 = pl;
+                }
+                double s_ = 0.0;
+                for (int j = 0; j < nv; ++j) s_ += wz[j] * tot[(size_t)j * nm + m];
+                acc += wm[m] * pl * s_;
+            }
+            out[(size_t)il * nsep + r] = (2.0 * l + 1.0) / 2.0 * acc;
+        }
+    }
+    free(xi);
+    free(fl);
+    free(tot);
+    return st;
+}
+
 /* ---- signal.c ------------------------------------------------------------------------ */
 
 double coffe_integrate(coffe_parameters_t *par, coffe_background_t *bg, coffe_integral_array_t *integral,
@@ -929,7 +1001,12 @@ double coffe_integrate(coffe_parameters_t *par, coffe_background_t *bg, coffe_in
            there); the rest is the full-sky single integral */
         if (!(nonlocal && local)) break;
         if (flag_output == COFFE_REF_AVERAGE_MULTIPOLES) {
-            st = COFFE_EUNSUPPORTED;
+            coffe_spline_ops ops;
+            st = spline_ops(&ops);
+            if (st == COFFE_OK && !bg) st = COFFE_EINVAL;
+            if (st == COFFE_OK)
+                st = intg_average(par, bg, integral, &ops, 0, 0, c->len && c->den && par->flatsky_local_nonlocal,
+                                  z_min, z_max, 1, &sep, 1, &l, &result);
             break;
         }
         if (flag_output != COFFE_REF_CORRFUNC && flag_output != COFFE_REF_MULTIPOLES) break;
@@ -956,7 +1033,13 @@ double coffe_integrate(coffe_parameters_t *par, coffe_background_t *bg, coffe_in
            flat-sky form with flatsky_nonlocal (not with only the cross terms: an auto term) */
         if (!nonlocal) break;
         if (flag_output == COFFE_REF_AVERAGE_MULTIPOLES) {
-            st = COFFE_EUNSUPPORTED;
+            coffe_spline_ops ops;
+            st = spline_ops(&ops);
+            if (st == COFFE_OK && !bg) st = COFFE_EINVAL;
+            if (st == COFFE_OK)
+                st = intg_average(par, bg, integral, &ops, 1,
+                                  c->len && par->flatsky_nonlocal && !par->only_cross_correlations, 0, z_min, z_max, 1,
+                                  &sep, 1, &l, &result);
             break;
         }
         if (flag_output != COFFE_REF_CORRFUNC && flag_output != COFFE_REF_MULTIPOLES) break;
@@ -1014,8 +1097,7 @@ static int multipole_setup(const coffe_parameters_t *par, const coffe_background
     const coffe_correlation_contributions *c = &par->correlation_contrib;
     if (ll) *ll = 0;
     if (dl) *dl = 0;
-    if (c->len || c->g4 || c->g5) {
-        if (!ll) return COFFE_EUNSUPPORTED;   /* the integrated terms of the averaged multipoles */
+    if ((c->len || c->g4 || c->g5) && ll && dl) {
         *ll = c->len && !par->only_cross_correlations && par->flatsky_nonlocal;
         *dl = c->len && c->den && par->flatsky_local_nonlocal;
     }
@@ -1102,17 +1184,26 @@ int coffe_average_multipoles_init(coffe_parameters_t *par, coffe_background_t *b
         return done(COFFE_EINVAL);
     coffe_spline_ops ops;
     local_ctx lc;
-    int st = multipole_setup(par, bg, integral, &ops, &lc, NULL, NULL);
+    int want_ll = 0, want_dl = 0;
+    int st = multipole_setup(par, bg, integral, &ops, &lc, &want_ll, &want_dl);
     if (st == COFFE_OK && !bg) st = COFFE_EINVAL;
     if (st != COFFE_OK) return done(st);
+    const coffe_correlation_contributions *c = &par->correlation_contrib;
+    const int intg = c->len || c->g4 || c->g5;
     coffe_average_multipoles_t *arr = (coffe_average_multipoles_t *)malloc(sizeof(*arr) * size);
     double *vals = (double *)malloc(sizeof(double) * nl * ns);
-    if (!arr || !vals) st = COFFE_ENOMEM;
+    double *part = intg ? (double *)malloc(sizeof(double) * nl * ns) : NULL;
+    if (!arr || !vals || (intg && !part)) st = COFFE_ENOMEM;
     /* bin-major, then l, then separation (average_multipoles.c:90-104) */
     size_t i = 0;
     for (size_t b = 0; b < nb && st == COFFE_OK; ++b) {
         st = average_eval(&lc, &ops, par, bg, par->zmin[b], par->zmax[b], (int)ns, par->sep, (int)nl,
                           par->multipole_values, vals);
+        if (st == COFFE_OK && intg) {
+            st = intg_average(par, bg, integral, &ops, -1, want_ll, want_dl, par->zmin[b], par->zmax[b], (int)ns,
+                              par->sep, (int)nl, par->multipole_values, part);
+            for (size_t q = 0; st == COFFE_OK && q < nl * ns; ++q) vals[q] += part[q];
+        }
         for (size_t l = 0; l < nl && st == COFFE_OK; ++l)
             for (size_t r = 0; r < ns; ++r, ++i) {
                 arr[i].coords.z_min = par->zmin[b];
@@ -1123,6 +1214,7 @@ int coffe_average_multipoles_init(coffe_parameters_t *par, coffe_background_t *b
             }
     }
     free(vals);
+    free(part);
     if (st != COFFE_OK) {
         free(arr);
         return done(st);

@@ -68,8 +68,8 @@ def test_parameters():
 
 def test_lensing_plan():
     """Which integrated terms go where (signal.c:556-771): the flat-sky forms with the
-    flat-sky switches, the rest to the full-sky integrals; the redshift-averaged
-    multipoles and the cosmological tangent refuse the integrated terms."""
+    flat-sky switches, the rest to the full-sky integrals; the cosmological tangent
+    refuses the integrated terms."""
     c = Coffe(has_lensing=True, has_density=False)
     c.set_power_spectrum_linear(np.logspace(-4, 1, 50), np.logspace(-4, 1, 50) ** -1)
     assert c._lensing_plan() == dict(flat_ll=False, flat_dl=False, flags={"len"})
@@ -91,7 +91,7 @@ def test_lensing_plan():
     c.has_g4 = True
     assert c._lensing_plan() == dict(flat_ll=False, flat_dl=False, flags={"g4"})
     with pytest.raises(NotImplementedError):
-        c.compute_average_multipoles_bulk()
+        c.multipoles_cosmo_jvp({"h": 1.0})
 
 
 def test_coordinates_and_contributions_checked_first():

@@ -128,9 +128,7 @@ def test_density_rsd_lensing_vs_reference(engine):
 
 def test_lensing_multipoles(engine):
     """Full-sky lensing multipoles: the flat-sky lensing fixtures at 20 and 40 Mpc/h
-    within 1e-3, the full-sky monopole fixtures at 100 and 150 Mpc/h within 5e-4; the
-    same through the correlation function's Legendre projection of the oracle's rule
-    (64 theta nodes, as Coffe's)."""
+    within 1e-3, the full-sky monopole fixtures at 100 and 150 Mpc/h within 5e-4."""
     c = _coffe(sep=SEPS, l=[0, 2, 4], has_density=False, has_lensing=True)
     print("Coffe lensing multipoles", flush=True)
     mp = c.compute_multipoles_bulk()
@@ -180,3 +178,43 @@ def test_flat_switches_keep_full_sky_potential_terms(engine):
     assert plan["flat_ll"] and plan["flat_dl"] and {"g4", "flat_ll", "flat_dl"} <= plan["flags"]
     extra = both._integrated_corrfunc(both.z_mean, both.sep, both.mu).reshape(-1)
     assert np.allclose(a, b + extra, rtol=1e-12, atol=1e-20)
+
+
+def test_average_multipoles_with_integrated_terms(engine):
+    """Redshift-averaged multipoles with lensing and g4 (signal.c:312-470, 674-682,
+    759-767): the local part equals the averaged multipoles without them; the integrated
+    part of a bin of width 0.1 equals h H0 / H(z) times the multipoles at its centre
+    within 3e-3 (the bin's curvature: a sharp z weighting reduces to the value there);
+    the reference ABI (coffe_average_multipoles_init) equals the Python API to 1e-10."""
+    import ctypes
+    from coffe_amd.coffe import COFFE_H0
+    from tests import refabi_host as R
+    seps = np.array([30.0, 100.0])
+    kw = dict(sep=seps, l=[0, 2], z_min=[0.95], z_max=[1.05], z_mean=[1.0], has_density=True)
+    full = _coffe(**kw, has_lensing=True, has_g4=True)
+    local = _coffe(**kw)
+    print("averaged multipoles with lensing and g4", flush=True)
+    a = np.array([r.value for r in full.compute_average_multipoles_bulk()])
+    b = np.array([r.value for r in local.compute_average_multipoles_bulk()])
+    part = (a - b).reshape(2, 2)                                  # [l][r]
+    mp = (np.array([r.value for r in full.compute_multipoles_bulk()]) -
+          np.array([r.value for r in local.compute_multipoles_bulk()])).reshape(2, 2)
+    w = H * COFFE_H0 / full.hubble_rate(1.0)
+    assert np.allclose(part, w * mp, rtol=3e-3, atol=0), part / (w * mp) - 1
+    # the reference ABI
+    from coffe_amd._lib import load_library
+    L = R.declare(load_library())
+    host = R.Host(L)
+    try:
+        prog = R.Program(host, full, output_type=R.AVERAGE_MULTIPOLES)
+        ia = R.IntegralArray()
+        assert L.coffe_integrals_init(ctypes.byref(prog.par), ctypes.byref(prog.bg), ctypes.byref(ia)) == 0
+        ram = R.AverageMultipolesArray()
+        assert L.coffe_average_multipoles_init(ctypes.byref(prog.par), ctypes.byref(prog.bg), ctypes.byref(ia),
+                                               ctypes.byref(ram)) == 0
+        got = np.array([ram.array[i].value for i in range(ram.size)])
+        assert np.allclose(got, a, rtol=1e-10, atol=0), got / a - 1
+        L.coffe_average_multipoles_free(ctypes.byref(ram))
+        L.coffe_integrals_free(ctypes.byref(ia))
+    finally:
+        host.close()

@@ -127,9 +127,8 @@ def test_covariance_find(lib):
 
 def test_integrate_flag_gating(lib):
     """coffe_integrate returns 0 where the reference does (no contribution of the kind);
-    the integrated terms need the background and tables (EINVAL without them) and are
-    not offered for the redshift-averaged multipoles (EUNSUPPORTED), before any device
-    work."""
+    the integrated terms need the background and tables (EINVAL without them, before
+    any device work)."""
     host = R.Host(lib)
     try:
         par = R.Parameters()
@@ -146,7 +145,7 @@ def test_integrate_flag_gating(lib):
         assert np.isnan(lib.coffe_integrate(ctypes.byref(par), *args, R.DOUBLE_INTEGRATED, R.CORRFUNC))
         assert lib.coffe_engine_last_error() == 1
         assert np.isnan(lib.coffe_integrate(ctypes.byref(par), *args, R.DOUBLE_INTEGRATED, R.AVERAGE_MULTIPOLES))
-        assert lib.coffe_engine_last_error() == EUNSUPPORTED
+        assert lib.coffe_engine_last_error() == 1
         # the derivative outputs change the local terms' form
         par.correlation_contrib.len = 0
         par.b_derivative = 1
@@ -168,7 +167,7 @@ def test_integrate_flag_gating(lib):
         par.zmin, par.zmin_len = z.ctypes.data_as(R.DP), 1
         par.zmax, par.zmax_len = z.ctypes.data_as(R.DP), 1
         ram = R.AverageMultipolesArray()
-        assert lib.coffe_average_multipoles_init(ctypes.byref(par), None, None, ctypes.byref(ram)) == EUNSUPPORTED
+        assert lib.coffe_average_multipoles_init(ctypes.byref(par), None, None, ctypes.byref(ram)) == 1
         par.flatsky_nonlocal = 1
         par.output_type = R.MULTIPOLES
         l6 = np.array([0, 6], dtype=np.int32)
