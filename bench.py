@@ -421,6 +421,114 @@ def run_divergent(args):
     print(json.dumps(res))
 
 
+# FP64 operations per integrand evaluation of k_integrated with every contribution on,
+# counted from the source: ~19 Akima lookups of ~40 flop (five slopes and the cubic; the
+# bisections are comparisons), the bicubic renormalisation ~60, the terms ~200
+INTG_FLOP_PER_NODE = 1000.0
+
+
+def run_integrated(args):
+    """The integrated terms of the reference's all-contributions configuration (density,
+    RSD, Doppler, d2, g1-g3, lensing, g4, g5, full sky; functions.c:1486-4017): the
+    single- and double-integrated correlation function at z = 1 on the 64 theta nodes of
+    the multipoles and 64 separations 5-300 Mpc/h, projected on l = 0, 2, 4
+    (coffe_integrated_corrfunc, one k_integrated workgroup per (mu, separation))."""
+    import torch
+    import oracle as O
+    from oracle import integrated as OI
+    from coffe_amd import background as B
+    from coffe_amd.engine import COFFE_SEP, Engine, integrated_corrfunc
+    H = 0.67
+    k, pk = np.loadtxt(os.path.join(ROOT, "tests", "golden", "PkL_CLASS.dat")).T
+    k, pk = k * H, pk / H ** 3
+    bg = B.compute(omega_cdm=0.25, omega_baryon=0.05, h=H)
+    Engine(0)
+    tabs = {key: O.integral_table(key[0], key[1], k, pk, 10000, small_r=True) for key in O.TABLE_ORDER}
+    chi = np.arange(201) / 200 * O.akima(bg["z"], bg["chi"], [1.2])[0]
+    i40, r0, ren = O.divergent_tables(k, pk, COFFE_SEP, chi)
+    tabs[(4, 0)] = (COFFE_SEP, i40)
+    x0, y0 = COFFE_SEP.copy(), r0.copy()
+    x0[0], y0[0] = 0.0, 0.0
+    zb, one, zero = np.arange(11.0), np.ones(11), np.zeros(11)
+    bgd = dict(z=bg["z"], chi=bg["chi"], D1=bg["D1"], a=bg["a"], f=bg["f"], cH=bg["conformal_Hz"], G1=bg["G1"],
+               G2=bg["G2"])
+    flags = ["den", "rsd", "d1", "d2", "g1", "g2", "g3", "len", "g4", "g5"]
+    t_, w_ = np.polynomial.legendre.leggauss(64)
+    theta = np.pi / 2 * (t_ + 1)
+    mus, wm = np.cos(theta), np.pi / 2 * w_ * np.sin(theta)
+    seps = np.linspace(5, 300, 64) / H
+    ls = [0, 2, 4]
+    pls = np.array([np.polynomial.legendre.legval(mus, [0] * l + [1]) * (2 * l + 1) / 2 for l in ls])
+    args_c = (bgd, zb, one, one, zb, zero, zero, zero, zero, 0.30, H / 2997.92458, tabs, flags)
+
+    def step():
+        s_, d_ = integrated_corrfunc(*args_c, [1.0], mus, seps, ren=(chi, ren), ren0=(x0, y0))
+        return np.einsum("lm,m,mr->lr", pls, wm, (s_ + d_)[0])
+
+    for _ in range(args.warmup):
+        out = step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    points = len(ls) * len(seps)
+    # nodes per (mu, r): the double rule (8 or 12 x 16 x1 nodes x 192 u nodes) and the single
+    # rule (up to 4 pieces x 512): counted from the cut positions of each point
+    nodes = 0
+    for m in mus:
+        for r in seps:
+            cm = float(O.akima(bg["z"], bg["chi"], [1.0])[0])
+            chi1, chi2 = cm - r * m / 2, cm + r * m / 2
+            c = (2 * cm * cm - r * r + m * m * r * r / 2) / (2 * cm * cm - m * m * r * r / 2)
+            nodes += (12 if chi1 * c > chi2 else 8) * 16 * 192
+            inside = {round(v, 15) for v in (chi1 * c / chi2, chi2 * c / chi1) if 0 < v < 1}
+            nodes += (1 + (2 * len(inside) if len(inside) == 2 else len(inside))) * 512
+    flops = INTG_FLOP_PER_NODE * nodes
+    # parity: three points against the oracle on the same rules
+    chk = []
+    for im, ir in ((5, 3), (32, 20), (60, 50)):
+        fl = {f_: 1 for f_ in flags}
+        inp = OI.Inputs(tabs, bg, 0.30, H / 2997.92458, zb, one, one, zb, zero, zero, zero, zero,
+                        div=(COFFE_SEP, i40, r0), ren=(chi, ren))
+        s_, d_ = integrated_corrfunc(*args_c, [1.0], [mus[im]], [seps[ir]], ren=(chi, ren), ren0=(x0, y0))
+        want = OI.corrfunc_single(inp, fl, 1.0, mus[im], seps[ir]) + OI.corrfunc_double(inp, fl, 1.0, mus[im], seps[ir])
+        chk.append(abs(float(s_[0, 0, 0] + d_[0, 0, 0]) / want - 1))
+    res = {
+        "metric": "integrated-term multipole points/sec (all contributions, full sky), 1 MI355X",
+        "value": points / (ms / 1e3), "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "reference test spectrum (PkL_CLASS.dat)",
+        "config": {"workload": "single- and double-integrated correlation function, every contribution, z = 1, "
+                               "64 theta nodes x 64 separations 5-300 Mpc/h, projected on l = 0, 2, 4",
+                   "points_per_step": points, "integrand_nodes_per_step": nodes},
+        "roofline": {"bound": "mfma", "kernel": "k_integrated", "unit": "TFLOP/s",
+                     "achieved": flops / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "frac": flops / (ms * 1e-3) / PEAK_FP64, "traffic": None,
+                     "note": f"{INTG_FLOP_PER_NODE:.0f} flop per integrand node (estimated from the source), "
+                             "divided by the whole synchronous call (uploads, allocation, the kernel, download): "
+                             "a lower bound of the kernel's own rate, whose duration the rocprof summary gives; "
+                             "FP64 VALU with L2-latency-bound table bisections"},
+        "parity": {"vs_oracle_max_rel": max(chk), "points": len(chk)},
+    }
+    if not args.no_cpu:
+        inp = OI.Inputs(tabs, bg, 0.30, H / 2997.92458, zb, one, one, zb, zero, zero, zero, zero,
+                        div=(COFFE_SEP, i40, r0), ren=(chi, ren))
+        fl = {f_: 1 for f_ in flags}
+        t1 = time.perf_counter()
+        n = 0
+        for m in mus[::8]:
+            OI.corrfunc_single(inp, fl, 1.0, m, seps[10])
+            OI.corrfunc_double(inp, fl, 1.0, m, seps[10])
+            n += 1
+        dt = time.perf_counter() - t1
+        per_point = dt / n * len(mus) / len(ls)      # 64 (mu) evaluations make 3 multipoles
+        res["cpu_baseline"] = {"value": 1.0 / per_point, "unit": "points/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle (numpy, same rules) at 8 mu nodes of one separation "
+                                         f"({dt:.1f} s), scaled to the 64-node projection"}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -438,7 +546,8 @@ def main():
                          "call: 0.167 against 0.163 ms)")
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=16)
-    ap.add_argument("--workload", choices=["batched", "fullsky", "covariance", "divergent"], default="batched",
+    ap.add_argument("--workload", choices=["batched", "fullsky", "covariance", "divergent", "integrated"],
+                    default="batched",
                     help="batched: the headline forward+adjoint batch; fullsky / covariance: the other "
                          "configurations, reported on their own lines")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -464,6 +573,8 @@ def main():
         return run_covariance(args)
     if args.workload == "divergent":
         return run_divergent(args)
+    if args.workload == "integrated":
+        return run_integrated(args)
     import torch
     from coffe_amd.engine import Batch, Engine
     import oracle as O

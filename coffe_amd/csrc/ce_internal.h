@@ -241,6 +241,8 @@ int ce_launch_mu_weights_jvp(void *stream, const ce_mu_dev *in, const ce_mu_tan 
    renormalisation of I_0^4 (ren0) and the renormalisation grid with its bicubic's node
    derivatives ([b nren + a] at (ren_chi[a], ren_chi[b])) */
 #define CE_INTG_NSLOT 9
+#define CE_INTG_ARGS_BYTES 2048
+#define CE_INTG_MAX_X1 512    /* x1 nodes of the double rule: (p1 + p1 / 2) n1 at most */
 typedef struct ce_intg_dev {
     int nbg;
     const double *bg_z, *bg_chi, *bg_D1, *bg_a, *bg_f, *bg_cH, *bg_G1, *bg_G2;
@@ -250,6 +252,19 @@ typedef struct ce_intg_dev {
     const double *mb_z, *s1, *s2, *fe1, *fe2;
     const double *tab_x, *tab_y;
     int tab_off[CE_INTG_NSLOT], tab_len[CE_INTG_NSLOT];
+    /* lookup guesses: the first node of each table's log-uniform part (-1: none) with
+       log x there and nodes per unit of log x; nguess + 1 interval indices of bg_chi at
+       chi = k chi_max / nguess (stored as doubles) */
+    int tab_f0[CE_INTG_NSLOT];
+    double tab_lx0[CE_INTG_NSLOT], tab_lsc[CE_INTG_NSLOT];
+    int tab_ref;   /* a slot whose nodes every present standard table shares, or -1 */
+    int nguess;
+    const double *chi_guess;
+    /* workspace of 3 ntab_values + 15 nbg doubles for the Akima coefficients, and
+       CE_INTG_ARGS_BYTES of device memory for the kernel's arguments */
+    int ntab_values;
+    double *coef;
+    void *args_dev;
     int nren0;
     const double *ren0_x, *ren0_y;
     int nren;

@@ -1816,6 +1816,7 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
         for (int i = 0; i < 6; ++i) r[i] = rule[i];
     for (int i = 0; i < 6; i += 2)
         if (r[i] < 1 || r[i] > 256 || r[i + 1] < 2 || r[i + 1] > 64) return COFFE_EINVAL;
+    if ((r[0] + (r[0] / 2 > 1 ? r[0] / 2 : 1)) * r[1] > CE_INTG_MAX_X1) return COFFE_EINVAL;
     for (int i = 0; i < nmu; ++i)
         if (!(mu[i] >= -1.0 && mu[i] <= 1.0)) return COFFE_EINVAL;
     for (int i = 0; i < nsep; ++i)
@@ -1845,9 +1846,12 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
     const size_t nbg = (size_t)in->nbg, nb = (size_t)in->nb, nmb = (size_t)in->nmb;
     const size_t nren = want_ren ? (size_t)in->nren : 0, nnode = nren * nren, n0 = want_ren ? (size_t)nren0 : 0;
     const size_t ngl = 2 * (size_t)(r[1] + r[3] + r[5]), nout = 2 * (size_t)nz * nmu * nsep;
+    const int nguess = 4096;
     const size_t total = 8 * nbg + 3 * nb + 5 * nmb + 2 * (size_t)total_tab + 2 * n0 + nren + 4 * nnode +
-                         (size_t)nz + (size_t)nmu + (size_t)nsep + ngl + nout;
-    double *h = (double *)malloc(sizeof(double) * (total - nout));
+                         (size_t)nz + (size_t)nmu + (size_t)nsep + ngl + (size_t)nguess + 1 + nout +
+                         3 * (size_t)total_tab + 15 * nbg + CE_INTG_ARGS_BYTES / sizeof(double);
+    double *h = (double *)malloc(sizeof(double) * (total - nout - 3 * (size_t)total_tab - 15 * nbg -
+                                                    CE_INTG_ARGS_BYTES / sizeof(double)));
     double *work = (double *)malloc(sizeof(double) * (2 * nren + 2));
     if (!h || !work) {
         free(h);
@@ -1892,6 +1896,43 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
     for (int t = 0; t < CE_INTG_NSLOT; ++t) {
         dv.tab_off[t] = off[t];
         dv.tab_len[t] = len[t];
+        /* the tables of coffe_integral_table: r = 0, the 42 small separations, then N
+           log-uniform FFT nodes from index 43 */
+        const double *x = tab_x + off[t];
+        const int n = len[t], f0 = 1 + CE_MIN_SEP_LEN;
+        dv.tab_f0[t] = -1;
+        if (n > f0 + 4 && x[0] == 0.0 && x[f0] > 0.0 && x[n - 1] > x[f0]) {
+            dv.tab_f0[t] = f0;
+            dv.tab_lx0[t] = log(x[f0]);
+            dv.tab_lsc[t] = (double)(n - 1 - f0) / (log(x[n - 1]) - log(x[f0]));
+        }
+    }
+    /* one interval serves every standard table when they share their nodes (the tables of
+       one spectrum and one N do) */
+    dv.tab_ref = -1;
+    for (int t = 0; t < CE_INTG_NSLOT - 1; ++t) {
+        if (len[t] < 5) continue;
+        if (dv.tab_ref < 0) {
+            dv.tab_ref = t;
+        } else if (len[t] != len[dv.tab_ref] ||
+                   memcmp(tab_x + off[t], tab_x + off[dv.tab_ref], sizeof(double) * (size_t)len[t]) != 0) {
+            dv.tab_ref = -1;
+            break;
+        }
+    }
+    /* guesses of bg_chi's interval on a uniform chi grid */
+    {
+        double *g = h + o;
+        const double cmax = in->bg_chi[nbg - 1];
+        size_t j = 0;
+        for (int k = 0; k <= nguess; ++k) {
+            const double c = cmax * k / nguess;
+            while (j + 2 < nbg && in->bg_chi[j + 1] <= c) ++j;
+            g[k] = (double)j;
+        }
+        dv.nguess = nguess;
+        dv.chi_guess = d + o;
+        o += (size_t)nguess + 1;
     }
     if (want_ren) {
         dv.nren0 = nren0;
@@ -1926,6 +1967,9 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
     }
 #undef IG_PUT
     double *d_out = d + o;
+    dv.ntab_values = (int)total_tab;
+    dv.coef = d_out + nout;
+    dv.args_dev = d_out + nout + 3 * (size_t)total_tab + 15 * nbg;
     hipStream_t s = e->stream;
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d, h, sizeof(double) * o, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK && ce_launch_integrated(s, &dv, flags, nz, d_z, nmu, d_mu, nsep, d_sep, r, d_gl, d_out))

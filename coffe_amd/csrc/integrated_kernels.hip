@@ -24,6 +24,9 @@
 namespace {
 
 enum { S00, S02, S04, S11, S13, S20, S22, S31, S40, NSLOT };
+// background curves with precomputed Akima coefficients: D1, a, f, conformal H against z,
+// and z against chi
+enum { C_D1, C_A, C_F, C_CH, C_ZOF, NBGC };
 
 constexpr unsigned F_DEN = 1u << 0, F_RSD = 1u << 1, F_D1 = 1u << 2, F_D2 = 1u << 3, F_G1 = 1u << 4,
                    F_G2 = 1u << 5, F_G3 = 1u << 6, F_LEN = 1u << 7, F_G4 = 1u << 8, F_G5 = 1u << 9,
@@ -38,6 +41,14 @@ struct intg_args {
     const double *mb_z, *s1, *s2, *fe1, *fe2;
     const double *tab_x, *tab_y;
     int tab_off[NSLOT], tab_len[NSLOT];
+    int tab_f0[NSLOT];                 // first node of the log-uniform part (-1: bisection)
+    int tab_ref;                       // a slot whose nodes every present standard table shares (-1: none)
+    double tab_lx0[NSLOT], tab_lsc[NSLOT];   // its log(x[f0]) and nodes per unit of log x
+    int nguess;                        // chi -> interval guesses on [0, chi_max], nguess + 1 of them
+    const double *chi_guess;
+    // Akima coefficients (b, c, d per node, the last node's unused): the tables' in the
+    // layout of tab_y, the background curves' [NBGC][nbg]
+    const double *tab_b, *tab_c, *tab_d, *bgc_b, *bgc_c, *bgc_d;
     int nren0;
     const double *ren0_x, *ren0_y;
     int nren;
@@ -53,14 +64,80 @@ struct intg_args {
 
 __device__ inline double ak(const double *x, const double *y, int n, double v) { return ce::akima_general(x, y, n, v); }
 
+// y(xq) in interval j from the precomputed Akima coefficients (the value akima_at gives)
+__device__ __forceinline__ double akc(const double *__restrict__ x, const double *__restrict__ y,
+                                      const double *__restrict__ b, const double *__restrict__ c,
+                                      const double *__restrict__ d, int j, double xq)
+{
+    const double dx = xq - x[j];
+    return y[j] + dx * (b[j] + dx * (c[j] + d[j] * dx));
+}
+
+// The lookups: Akima of raw nodes as ce::akima_general, with the interval found from a
+// guess (the background's redshift grid is uniform; chi(z) through a table of interval
+// guesses; the integral tables' FFT nodes are log-uniform) and a short walk, falling
+// back to bisection: the same interval, so the same value, in a few dependent loads
+// where bisection over 10^4 nodes takes 14.
 struct lk {
     const intg_args &a;
     __device__ lk(const intg_args &a_) : a(a_) {}
-    __device__ double bg(const double *col, double z) const { return ak(a.bg_z, col, a.nbg, z); }
-    __device__ double z_of(double chi) const { return ak(a.bg_chi, a.bg_z, a.nbg, chi); }
+    __device__ double bg(const double *col, double z) const
+    {
+        const double *x = a.bg_z;
+        const int n = a.nbg;
+        if (!(z >= x[0] && z <= x[n - 1])) return __builtin_nan("");
+        return ce::akima_at(x, col, n, ce::interval_from(x, n, z, ce::interval_guess(x, n, z)), z);
+    }
+    // D1, a, f or conformal H (curve k, values col) by the precomputed coefficients
+    __device__ double bgc(int k, const double *col, double z) const
+    {
+        const double *x = a.bg_z;
+        const int n = a.nbg;
+        if (!(z >= x[0] && z <= x[n - 1])) return __builtin_nan("");
+        const int j = ce::interval_from(x, n, z, ce::interval_guess(x, n, z));
+        const size_t o = (size_t)k * n;
+        return akc(x, col, a.bgc_b + o, a.bgc_c + o, a.bgc_d + o, j, z);
+    }
+    __device__ double z_of(double chi) const
+    {
+        const double *x = a.bg_chi;
+        const int n = a.nbg;
+        if (!(chi >= x[0] && chi <= x[n - 1])) return __builtin_nan("");
+        const int k = ce::clamp_index(chi / x[n - 1] * a.nguess, a.nguess + 1);
+        const int j = ce::interval_from(x, n, chi, (int)a.chi_guess[k]);
+        const size_t o = (size_t)C_ZOF * n;
+        return akc(x, a.bg_z, a.bgc_b + o, a.bgc_c + o, a.bgc_d + o, j, chi);
+    }
     __device__ double tab(int s, double r) const
     {
-        return ak(a.tab_x + a.tab_off[s], a.tab_y + a.tab_off[s], a.tab_len[s], r);
+        const double *x = a.tab_x + a.tab_off[s], *y = a.tab_y + a.tab_off[s];
+        const int n = a.tab_len[s], f0 = a.tab_f0[s];
+        if (!(r >= x[0] && r <= x[n - 1])) return __builtin_nan("");
+        int j;
+        if (f0 < 0) j = ce::interval_bisect(x, 0, n - 1, r);
+        else if (r < x[f0]) j = ce::interval_bisect(x, 0, f0, r);
+        else j = ce::interval_from(x, n, r, f0 + ce::clamp_index((log(r) - a.tab_lx0[s]) * a.tab_lsc[s], n - f0));
+        const int o = a.tab_off[s];
+        return akc(x, y, a.tab_b + o, a.tab_c + o, a.tab_d + o, j, r);
+    }
+    // the interval of r on the standard tables' shared nodes (-1: none shared, or r
+    // outside them), and a standard table's value there
+    __device__ int tab_interval(double r) const
+    {
+        const int s = a.tab_ref;
+        if (s < 0) return -1;
+        const double *x = a.tab_x + a.tab_off[s];
+        const int n = a.tab_len[s], f0 = a.tab_f0[s];
+        if (!(r >= x[0] && r <= x[n - 1])) return -1;
+        if (f0 < 0) return ce::interval_bisect(x, 0, n - 1, r);
+        if (r < x[f0]) return ce::interval_bisect(x, 0, f0, r);
+        return ce::interval_from(x, n, r, f0 + ce::clamp_index((log(r) - a.tab_lx0[s]) * a.tab_lsc[s], n - f0));
+    }
+    __device__ double tabj(int s, int j, double r) const
+    {
+        if (j < 0) return tab(s, r);
+        const int o = a.tab_off[s];
+        return akc(a.tab_x + o, a.tab_y + o, a.tab_b + o, a.tab_c + o, a.tab_d + o, j, r);
     }
     // GSL interp2d bicubic of the renormalisation grid at (x, y) (node derivatives of
     // natural cubic splines, Hermite patches; common.c:712-719)
@@ -125,14 +202,14 @@ __device__ pt make_pt(const intg_args &a, double zm, double mu, double sep)
     p.fe2 = ak(a.mb_z, a.fe2, a.nmb, z2c);
     p.b1 = ak(a.bias_z, a.b1, a.nb, z1c);
     p.b2 = ak(a.bias_z, a.b2, a.nb, z2c);
-    p.Dc1 = L.bg(a.bg_D1, z1c);
-    p.Dc2 = L.bg(a.bg_D1, z2c);
-    p.fc1 = L.bg(a.bg_f, z1c);
-    p.fc2 = L.bg(a.bg_f, z2c);
-    p.cHc1 = L.bg(a.bg_cH, z1c);
-    p.cHc2 = L.bg(a.bg_cH, z2c);
-    p.ac1 = L.bg(a.bg_a, z1c);
-    p.ac2 = L.bg(a.bg_a, z2c);
+    p.Dc1 = L.bgc(C_D1, a.bg_D1, z1c);
+    p.Dc2 = L.bgc(C_D1, a.bg_D1, z2c);
+    p.fc1 = L.bgc(C_F, a.bg_f, z1c);
+    p.fc2 = L.bgc(C_F, a.bg_f, z2c);
+    p.cHc1 = L.bgc(C_CH, a.bg_cH, z1c);
+    p.cHc2 = L.bgc(C_CH, a.bg_cH, z2c);
+    p.ac1 = L.bgc(C_A, a.bg_a, z1c);
+    p.ac2 = L.bgc(C_A, a.bg_a, z2c);
     p.G1c = L.bg(a.bg_G1, z1c);
     p.G2c = L.bg(a.bg_G2, z2c);
     p.G1c2 = L.bg(a.bg_G1, z2c);
@@ -149,11 +226,12 @@ __device__ double double_point(const intg_args &a, const pt &p, double x1, doubl
     const double r2 = fmax(l1 * l1 + l2 * l2 - 2 * l1 * l2 * c, 0.0);
     const double r = sqrt(r2);
     const double z1 = L.z_of(l1), z2 = L.z_of(l2);
-    const double D1 = L.bg(a.bg_D1, z1), D2 = L.bg(a.bg_D1, z2);
-    const double DD = D1 * D2 / L.bg(a.bg_a, z1) / L.bg(a.bg_a, z2);
+    const double D1 = L.bgc(C_D1, a.bg_D1, z1), D2 = L.bgc(C_D1, a.bg_D1, z2);
+    const double DD = D1 * D2 / L.bgc(C_A, a.bg_a, z1) / L.bgc(C_A, a.bg_a, z2);
     const double LL = l1 * l2;
     const double q2 = (2 - 5 * p.s1) * (2 - 5 * p.s2);
     const bool lens = f & F_LEN, g4 = f & F_G4, g5 = f & F_G5, oc = f & F_ONLY_CROSS;
+    const int jr = L.tab_interval(r);
     double res = 0.0;
     double ren = 0.0;
     if (g4 || g5) ren = L.ren(r2, l1, l2, r2 <= 1e-12, l1);
@@ -161,11 +239,11 @@ __device__ double double_point(const intg_args &a, const pt &p, double x1, doubl
         double T;
         if (r2 > 1e-20) {
             const double cc1 = c * c - 1;
-            T = 2 * cc1 * LL * L.tab(S00, r) / 5 + 4 * c * L.tab(S20, r) / 3 +
-                4 * c * (r2 + 6 * c * LL) * L.tab(S11, r) / 15 +
-                2 * cc1 * LL * (2 * r2 + 3 * c * LL) * L.tab(S02, r) / 7 / r2 +
-                2 * c * (2 * r2 * r2 + 12 * c * r2 * LL + 15 * cc1 * LL * LL) * L.tab(S13, r) / 15 / r2 +
-                cc1 * LL * (6 * r2 * r2 + 30 * c * r2 * LL + 35 * cc1 * LL * LL) * L.tab(S04, r) / 35 / r2 / r2;
+            T = 2 * cc1 * LL * L.tabj(S00, jr, r) / 5 + 4 * c * L.tabj(S20, jr, r) / 3 +
+                4 * c * (r2 + 6 * c * LL) * L.tabj(S11, jr, r) / 15 +
+                2 * cc1 * LL * (2 * r2 + 3 * c * LL) * L.tabj(S02, jr, r) / 7 / r2 +
+                2 * c * (2 * r2 * r2 + 12 * c * r2 * LL + 15 * cc1 * LL * LL) * L.tabj(S13, jr, r) / 15 / r2 +
+                cc1 * LL * (6 * r2 * r2 + 30 * c * r2 * LL + 35 * cc1 * LL * LL) * L.tabj(S04, jr, r) / 35 / r2 / r2;
         } else {
             T = 4 + 24 * LL * L.tab(S11, 0.0) / 15;
         }
@@ -173,15 +251,15 @@ __device__ double double_point(const intg_args &a, const pt &p, double x1, doubl
     }
     double cH1 = 0, cH2 = 0, f1 = 0, f2 = 0;
     if (g5) {
-        cH1 = L.bg(a.bg_cH, z1);
-        cH2 = L.bg(a.bg_cH, z2);
-        f1 = L.bg(a.bg_f, z1);
-        f2 = L.bg(a.bg_f, z2);
+        cH1 = L.bgc(C_CH, a.bg_cH, z1);
+        cH2 = L.bgc(C_CH, a.bg_cH, z2);
+        f1 = L.bgc(C_F, a.bg_f, z1);
+        f2 = L.bgc(C_F, a.bg_f, z2);
     }
     if (g4 && !oc) res += 9 * Om2 * q2 * DD * ren;
     if (g5 && !oc) res += 9 * Om2 * p.G1c * p.G2c * p.chi1 * p.chi2 * DD * cH1 * cH2 * (f1 - 1) * (f2 - 1) * ren;
     if ((g4 || g5) && lens) {
-        const double P = r2 != 0.0 ? 2 * LL * c * L.tab(S31, r) - LL * LL * (1 - c * c) * L.tab(S22, r)
+        const double P = r2 != 0.0 ? 2 * LL * c * L.tabj(S31, jr, r) - LL * LL * (1 - c * c) * L.tabj(S22, jr, r)
                                    : 2 * LL * L.tab(S31, 0.0);
         const double w2 = (1 - x2) / x2, w1 = (1 - x1) / x1;
         if (g4) res += 9 * Om2 / 2 * q2 * DD * (w2 + w1) * P;
@@ -237,9 +315,10 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
     const double r21 = fmax(l2 * l2 + chi1 * chi1 - 2 * chi1 * l2 * c, 0.0);
     const double r22 = fmax(l1 * l1 + chi2 * chi2 - 2 * chi2 * l1 * c, 0.0);
     const double rA = sqrt(r21), rB = sqrt(r22);
+    const int jA = L.tab_interval(rA), jB = L.tab_interval(rB);
     const double z1 = L.z_of(l1), z2 = L.z_of(l2);
-    const double D1x = L.bg(a.bg_D1, z1), D2x = L.bg(a.bg_D1, z2);
-    const double a1x = L.bg(a.bg_a, z1), a2x = L.bg(a.bg_a, z2);
+    const double D1x = L.bgc(C_D1, a.bg_D1, z1), D2x = L.bgc(C_D1, a.bg_D1, z2);
+    const double a1x = L.bgc(C_A, a.bg_a, z1), a2x = L.bgc(C_A, a.bg_a, z2);
     const double VA = D2x / a2x, VB = D1x / a1x;
     const double WA = (1 - x) * VA, WB = (1 - x) * VB;
     const double s1 = p.s1, s2 = p.s2, c2 = 2 * c * c - 1, e = p.omc;
@@ -248,38 +327,38 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
     if (lens && (f & F_DEN) && full_dl)   // functions.c:1707-1781
         res += H2 * (-3 * Om / 2) *
                (p.b1 * (2 - 5 * s2) * p.Dc1 * chi2 * WA *
-                    (2 * chi1 * c * L.tab(S11, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tab(S02, rA) / r21) +
+                    (2 * chi1 * c * L.tabj(S11, jA, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tabj(S02, jA, rA) / r21) +
                 p.b2 * (2 - 5 * s1) * p.Dc2 * chi1 * WB *
-                    (2 * chi2 * c * L.tab(S11, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tab(S02, rB) / r22));
+                    (2 * chi2 * c * L.tabj(S11, jB, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tabj(S02, jB, rB) / r22));
     if (lens && (f & F_RSD) && full_dl) {   // functions.c:2167-2198, then 2291-2324 or 2377-2390
         const bool wide = fabs(p.mu) < 0.999;
-        auto rsd = [&](double ch, double lam, double rr, double rv) {
+        auto rsd = [&](double ch, double lam, double rr, double rv, int jv) {
             (void)rr;
             const double d = ch - lam, rs = d * d + 2 * ch * lam * e;
-            double v = (lam - 6 * ch * c + 3 * lam * c2) * L.tab(S00, rv) / 15 -
-                       num_rsd_i02(ch, lam, d, e) * L.tab(S02, rv) / rs / 21;
-            if (wide) v -= num_rsd_i04(ch, lam, d, e) * L.tab(S04, rv) / rs / rs / 35;
-            else v += 4 * (lam + ch) * L.tab(S04, rv) / 35;
+            double v = (lam - 6 * ch * c + 3 * lam * c2) * L.tabj(S00, jv, rv) / 15 -
+                       num_rsd_i02(ch, lam, d, e) * L.tabj(S02, jv, rv) / rs / 21;
+            if (wide) v -= num_rsd_i04(ch, lam, d, e) * L.tabj(S04, jv, rv) / rs / rs / 35;
+            else v += 4 * (lam + ch) * L.tabj(S04, jv, rv) / 35;
             return v;
         };
         res += H2 * 3 * Om / 2 *
-               (chi2 * p.fc1 * (2 - 5 * s2) * p.Dc1 * WA * rsd(chi1, l2, r21, rA) +
-                chi1 * p.fc2 * (2 - 5 * s1) * p.Dc2 * WB * rsd(chi2, l1, r22, rB));
+               (chi2 * p.fc1 * (2 - 5 * s2) * p.Dc1 * WA * rsd(chi1, l2, r21, rA, jA) +
+                chi1 * p.fc2 * (2 - 5 * s1) * p.Dc2 * WB * rsd(chi2, l1, r22, rB, jB));
     }
     if (lens && (f & F_D1) && full_dl) {   // functions.c:2624-2661 (G1 at z(chi2), 2 - 5 s2 in the second half)
-        auto d1k = [&](double ch, double lam, double rr, double rv) {
+        auto d1k = [&](double ch, double lam, double rr, double rv, int jv) {
             (void)rr;
             const double d = ch - lam, rs = d * d + 2 * ch * lam * e;
-            return 2 * (c * (lam * lam - 2 * ch * ch) + ch * lam * (2 * c2 - 1)) * L.tab(S11, rv) / 15 +
-                   2 * c * L.tab(S20, rv) / 3 - num_d1_i13(ch, lam, d, e) * L.tab(S13, rv) / rs / 15;
+            return 2 * (c * (lam * lam - 2 * ch * ch) + ch * lam * (2 * c2 - 1)) * L.tabj(S11, jv, rv) / 15 +
+                   2 * c * L.tabj(S20, jv, rv) / 3 - num_d1_i13(ch, lam, d, e) * L.tabj(S13, jv, rv) / rs / 15;
         };
         res += H2 * 3 * Om / 2 *
-               (chi2 * p.cHc1 * p.fc1 * p.G1c * (2 - 5 * s2) * p.Dc1 * WA * d1k(chi1, l2, r21, rA) +
-                chi1 * p.cHc2 * p.fc2 * p.G1c2 * (2 - 5 * s2) * p.Dc2 * WB * d1k(chi2, l1, r22, rB));
+               (chi2 * p.cHc1 * p.fc1 * p.G1c * (2 - 5 * s2) * p.Dc1 * WA * d1k(chi1, l2, r21, rA, jA) +
+                chi1 * p.cHc2 * p.fc2 * p.G1c2 * (2 - 5 * s2) * p.Dc2 * WB * d1k(chi2, l1, r22, rB, jB));
     }
     if (lens && (f & (F_D2 | F_G1 | F_G2 | F_G3))) {
-        const double PA = 2 * chi1 * c * L.tab(S31, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tab(S22, rA);
-        const double PB = 2 * chi2 * c * L.tab(S31, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tab(S22, rB);
+        const double PA = 2 * chi1 * c * L.tabj(S31, jA, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tabj(S22, jA, rA);
+        const double PB = 2 * chi2 * c * L.tabj(S31, jB, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tabj(S22, jB, rB);
         const double A = chi2 * p.Dc1 * WA * PA, B = chi1 * p.Dc2 * WB * PB;
         const double q = 9 * Om * Om / 4 * H4;
         if (f & F_D2)   // functions.c:3000-3030
@@ -294,24 +373,24 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
         const bool div = f & (F_D2 | F_G1 | F_G2 | F_G3 | F_G4 | F_G5);
         const double ren1 = div ? L.ren(r21, l2, chi1, r21 == 0.0, l2) : 0.0;
         const double ren2 = div ? L.ren(r22, l1, chi2, r22 == 0.0, l1) : 0.0;
-        const double E1 = L.bg(a.bg_cH, z1) * (L.bg(a.bg_f, z1) - 1), E2 = L.bg(a.bg_cH, z2) * (L.bg(a.bg_f, z2) - 1);
-        auto rsdk = [&](double rr, double lam, double rv) {
-            return (2 * rr / 3 + (c * c - 1) * lam * lam) * L.tab(S22, rv) - L.tab(S20, rv) / 3;
+        const double E1 = L.bgc(C_CH, a.bg_cH, z1) * (L.bgc(C_F, a.bg_f, z1) - 1), E2 = L.bgc(C_CH, a.bg_cH, z2) * (L.bgc(C_F, a.bg_f, z2) - 1);
+        auto rsdk = [&](double rr, double lam, double rv, int jv) {
+            return (2 * rr / 3 + (c * c - 1) * lam * lam) * L.tabj(S22, jv, rv) - L.tabj(S20, jv, rv) / 3;
         };
         const double q = 9 * Om * Om / 2 * H4;
         const double cH2c1 = p.cHc1 * p.cHc1, cH2c2 = p.cHc2 * p.cHc2;
         if (g4) {
             if (f & F_DEN)   // functions.c:3122-3140
                 res += H2 * (-3 * Om) *
-                       (p.b1 * (2 - 5 * s2) * p.Dc1 * VA * L.tab(S20, rA) + p.b2 * (2 - 5 * s1) * p.Dc2 * VB * L.tab(S20, rB));
+                       (p.b1 * (2 - 5 * s2) * p.Dc1 * VA * L.tabj(S20, jA, rA) + p.b2 * (2 - 5 * s1) * p.Dc2 * VB * L.tabj(S20, jB, rB));
             if (f & F_RSD)   // functions.c:3164-3189
                 res += H2 * 3 * Om *
-                       (p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * rsdk(r21, l2, rA) +
-                        p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * rsdk(r22, l1, rB));
+                       (p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * rsdk(r21, l2, rA, jA) +
+                        p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * rsdk(r22, l1, rB, jB));
             if (f & F_D1)    // functions.c:3220-3235
                 res += H2 * 3 * Om *
-                       (p.cHc1 * p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * (l2 * c - chi1) * L.tab(S31, rA) +
-                        p.cHc2 * p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * (l1 * c - chi2) * L.tab(S31, rB));
+                       (p.cHc1 * p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * (l2 * c - chi1) * L.tabj(S31, jA, rA) +
+                        p.cHc2 * p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * (l1 * c - chi2) * L.tabj(S31, jB, rB));
             if (f & F_D2)    // functions.c:3258-3275
                 res += H2 * (-3 * Om) *
                        ((3 - p.fe1) * p.fc1 * cH2c1 * (2 - 5 * s2) * p.Dc1 * VA * ren1 +
@@ -324,16 +403,16 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
         if (g5) {
             if (f & F_DEN)   // functions.c:3142-3162 (D1 a along the line of sight, as written there)
                 res += H2 * (-3 * Om) *
-                       (chi2 * p.b1 * p.G2c * p.Dc1 * E2 * D2x * a2x * L.tab(S20, rA) +
-                        chi1 * p.b2 * p.G1c * p.Dc2 * E1 * D1x * a1x * L.tab(S20, rB));
+                       (chi2 * p.b1 * p.G2c * p.Dc1 * E2 * D2x * a2x * L.tabj(S20, jA, rA) +
+                        chi1 * p.b2 * p.G1c * p.Dc2 * E1 * D1x * a1x * L.tabj(S20, jB, rB));
             if (f & F_RSD)   // functions.c:3191-3218
                 res += H2 * 3 * Om *
-                       (chi2 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * rsdk(r21, l2, rA) +
-                        chi1 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * rsdk(r22, l1, rB));
+                       (chi2 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * rsdk(r21, l2, rA, jA) +
+                        chi1 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * rsdk(r22, l1, rB, jB));
             if (f & F_D1)    // functions.c:3237-3256
                 res += H2 * 3 * Om *
-                       (chi2 * p.cHc1 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * (l2 * c - chi1) * L.tab(S31, rA) +
-                        chi1 * p.cHc2 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * (l1 * c - chi2) * L.tab(S31, rB));
+                       (chi2 * p.cHc1 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * (l2 * c - chi1) * L.tabj(S31, jA, rA) +
+                        chi1 * p.cHc2 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * (l1 * c - chi2) * L.tabj(S31, jB, rB));
             if (f & F_D2)    // functions.c:3277-3296
                 res += H2 * (-3 * Om) *
                        (chi2 * (3 - p.fe1) * p.fc1 * cH2c1 * p.G2c * p.Dc1 * E2 * VA * ren1 +
@@ -404,8 +483,31 @@ __device__ int single_pieces(const pt &p, piece *pc)
     return np;
 }
 
-__global__ void __launch_bounds__(256) k_integrated(intg_args a)
+// the Akima coefficients of up to 16 curves, one thread per (node, curve)
+struct coef_list {
+    int ncurve;
+    const double *x[16], *y[16];
+    int n[16];
+    double *b[16], *c[16], *d[16];
+};
+
+__global__ void __launch_bounds__(256) k_intg_coef(coef_list L)
 {
+    const int t = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = L.n[t];
+    if (j >= n - 1) return;
+    const ce::akima_cf<double> cf = ce::akima_coefs<double>(L.x[t], nullptr, L.y[t], nullptr, n, j);
+    L.b[t][j] = cf.b;
+    L.c[t][j] = cf.c;
+    L.d[t][j] = cf.d;
+}
+
+// the arguments live in device memory (a pointer, not a by-value struct, whose address
+// the lookups would otherwise take in private memory)
+__global__ void __launch_bounds__(256) k_integrated(const intg_args *__restrict__ pa)
+{
+    const intg_args &a = *pa;
     const int idx = blockIdx.x;   // (z, mu, r), r fastest
     const int ir = idx % a.nsep;
     const int im = (idx / a.nsep) % a.nmu;
@@ -417,8 +519,16 @@ __global__ void __launch_bounds__(256) k_integrated(intg_args a)
         ((f & F_LEN) && (f & (F_DEN | F_RSD | F_D1 | F_D2 | F_G1 | F_G2 | F_G3))) ||
         ((f & (F_G4 | F_G5)) && (f & (F_DEN | F_RSD | F_D1 | F_D2 | F_G1 | F_G2 | F_G3)));
     if (want_single) {
-        piece pc[4];
-        const int np = single_pieces(p, pc);
+        // the pieces in LDS (a private array indexed by node would live in scratch)
+        __shared__ piece pc[4];
+        __shared__ int npc;
+        if (threadIdx.x == 0) {
+            piece t[4];
+            npc = single_pieces(p, t);
+            for (int i = 0; i < 4; ++i) pc[i] = t[i];
+        }
+        __syncthreads();
+        const int np = npc;
         const int per = a.ps * a.ns;
         for (int q = threadIdx.x; q < np * per; q += blockDim.x) {
             const piece &P = pc[q / per];
@@ -434,17 +544,31 @@ __global__ void __launch_bounds__(256) k_integrated(intg_args a)
         const double xb = p.chi1 * p.c > p.chi2 ? p.chi2 / (p.chi1 * p.c) : 1.0;
         const int pA = a.p1, pB = xb < 1.0 ? (a.p1 / 2 > 1 ? a.p1 / 2 : 1) : 0;
         const int n1 = (pA + pB) * a.n1, nuu = a.pu * a.nu;
-        for (int q = threadIdx.x; q < n1 * nuu; q += blockDim.x) {
-            const int i1 = q / nuu, iu = q - i1 * nuu;
-            double w1, wu;
+        // per x1 node (CE_INTG_MAX_X1 at most, checked by the host): the node, its weight,
+        // the sinh scale and the u interval, once per workgroup
+        __shared__ double sx[CE_INTG_MAX_X1][5];
+        for (int i1 = threadIdx.x; i1 < n1; i1 += blockDim.x) {
+            double w1;
             const double x1 = i1 < pA * a.n1 ? composite(0.0, xb, pA, a.n1, a.g1x, a.g1w, i1, &w1)
                                              : composite(xb, 1.0, pB, a.n1, a.g1x, a.g1w, i1 - pA * a.n1, &w1);
             const double l1 = p.chi1 * x1, centre = l1 * p.c;
-            const double s = fmax(l1 * p.sint, 1e-9 * p.chi2);
-            const double u = composite(asinh((0.0 - centre) / s), asinh((p.chi2 - centre) / s), a.pu, a.nu, a.gux,
-                                       a.guw, iu, &wu);
-            const double lam2 = centre + s * sinh(u);
-            acc_d += w1 * wu * s * cosh(u) / p.chi2 * double_point(a, p, x1, lam2 / p.chi2);
+            const double sc = fmax(l1 * p.sint, 1e-9 * p.chi2);
+            sx[i1][0] = x1;
+            sx[i1][1] = w1;
+            sx[i1][2] = sc;
+            sx[i1][3] = asinh((0.0 - centre) / sc);
+            sx[i1][4] = asinh((p.chi2 - centre) / sc);
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < n1 * nuu; q += blockDim.x) {
+            const int i1 = q / nuu, iu = q - i1 * nuu;
+            const double x1 = sx[i1][0], w1 = sx[i1][1], sc = sx[i1][2];
+            double wu;
+            const double u = composite(sx[i1][3], sx[i1][4], a.pu, a.nu, a.gux, a.guw, iu, &wu);
+            // sinh and cosh from one exponential (lambda2's absolute error stays ~eps sc)
+            const double eu = exp(u), ei = 1.0 / eu;
+            const double lam2 = p.chi1 * x1 * p.c + sc * (0.5 * (eu - ei));
+            acc_d += w1 * wu * sc * (0.5 * (eu + ei)) / p.chi2 * double_point(a, p, x1, lam2 / p.chi2);
         }
     }
     // workgroup sums: wavefront shuffles, then one slot per wavefront in LDS
@@ -502,7 +626,50 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
     for (int t = 0; t < NSLOT; ++t) {
         a.tab_off[t] = in->tab_off[t];
         a.tab_len[t] = in->tab_len[t];
+        a.tab_f0[t] = in->tab_f0[t];
+        a.tab_lx0[t] = in->tab_lx0[t];
+        a.tab_lsc[t] = in->tab_lsc[t];
     }
+    a.nguess = in->nguess;
+    a.chi_guess = in->chi_guess;
+    a.tab_ref = in->tab_ref;
+    // the coefficients: the tables' in tab_y's layout, then the background curves
+    double *tb = in->coef, *tc = tb + in->ntab_values, *td = tc + in->ntab_values;
+    double *gb = td + in->ntab_values, *gc = gb + (size_t)NBGC * in->nbg, *gd = gc + (size_t)NBGC * in->nbg;
+    a.tab_b = tb;
+    a.tab_c = tc;
+    a.tab_d = td;
+    a.bgc_b = gb;
+    a.bgc_c = gc;
+    a.bgc_d = gd;
+    coef_list cl;
+    cl.ncurve = 0;
+    int nmax = 0;
+    for (int t = 0; t < NSLOT; ++t) {
+        if (in->tab_len[t] < 5) continue;
+        const int q = cl.ncurve++;
+        cl.x[q] = in->tab_x + in->tab_off[t];
+        cl.y[q] = in->tab_y + in->tab_off[t];
+        cl.n[q] = in->tab_len[t];
+        cl.b[q] = tb + in->tab_off[t];
+        cl.c[q] = tc + in->tab_off[t];
+        cl.d[q] = td + in->tab_off[t];
+        nmax = cl.n[q] > nmax ? cl.n[q] : nmax;
+    }
+    const double *bx[NBGC] = {in->bg_z, in->bg_z, in->bg_z, in->bg_z, in->bg_chi};
+    const double *by[NBGC] = {in->bg_D1, in->bg_a, in->bg_f, in->bg_cH, in->bg_z};
+    for (int k = 0; k < NBGC; ++k) {
+        const int q = cl.ncurve++;
+        cl.x[q] = bx[k];
+        cl.y[q] = by[k];
+        cl.n[q] = in->nbg;
+        cl.b[q] = gb + (size_t)k * in->nbg;
+        cl.c[q] = gc + (size_t)k * in->nbg;
+        cl.d[q] = gd + (size_t)k * in->nbg;
+        nmax = in->nbg > nmax ? in->nbg : nmax;
+    }
+    hipLaunchKernelGGL(k_intg_coef, dim3((unsigned)((nmax + 255) / 256), (unsigned)cl.ncurve), dim3(256), 0,
+                       (hipStream_t)stream, cl);
     a.nren0 = in->nren0;
     a.ren0_x = in->ren0_x;
     a.ren0_y = in->ren0_y;
@@ -536,6 +703,11 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
     a.out = d_out;
     const int64_t blocks = (int64_t)nz * nmu * nsep;
     if (blocks <= 0 || blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_integrated, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, a);
+    static_assert(sizeof(intg_args) <= CE_INTG_ARGS_BYTES, "intg_args outgrew its device slot");
+    hipError_t e = hipMemcpyAsync(in->args_dev, &a, sizeof(a), hipMemcpyHostToDevice, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);   /* `a` is on this stack */
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_integrated, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const intg_args *)in->args_dev);
     return (int)hipGetLastError();
 }
