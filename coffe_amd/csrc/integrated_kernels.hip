@@ -217,7 +217,7 @@ __device__ pt make_pt(const intg_args &a, double zm, double mu, double sep)
 }
 
 // functions_double_integrated (functions.c:3460-4017) at (x1, x2), times (H0 h)^4
-__device__ double double_point(const intg_args &a, const pt &p, double x1, double x2)
+__device__ __forceinline__ double double_point(const intg_args &a, const pt &p, double x1, double x2)
 {
     const lk L(a);
     const unsigned f = a.flags;
