@@ -76,6 +76,19 @@ def test_legacy_contributions_and_ignored_options(tmp_path):
     assert any("covariance_step_size" in str(w.message) and "ignored" in str(w.message) for w in record)
 
 
+def test_integrated_contributions_round_trip(tmp_path):
+    """The lensing and potential contributions of a settings file (parser.c's
+    correlation_contributions names len, g4, g5) survive to_file / from_file."""
+    cfg = tmp_path / "intg.cfg"
+    cfg.write_text('correlation_contributions = ["den", "len", "g4", "g5"];\nflatsky_nonlocal = 1;\n')
+    with pytest.warns(DeprecationWarning):
+        c = Coffe.from_file(cfg)
+    assert c.has_density and c.has_lensing and c.has_g4 and c.has_g5 and c.has_flatsky_nonlocal
+    c.to_file(tmp_path / "back.cfg")
+    d = Coffe.from_file(tmp_path / "back.cfg")
+    assert d.has_lensing and d.has_g4 and d.has_g5 and not d.has_rsd
+
+
 def test_bias_file_is_read(tmp_path):
     zb = np.linspace(0, 4, 9)
     np.savetxt(tmp_path / "b.dat", np.transpose([zb, 2 - 0.1 * zb]))
