@@ -1821,6 +1821,11 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
         if (!(mu[i] >= -1.0 && mu[i] <= 1.0)) return COFFE_EINVAL;
     for (int i = 0; i < nsep; ++i)
         if (!(sep[i] > 0.0) || !isfinite(sep[i])) return COFFE_EINVAL;
+    /* along the line of sight (|mu| -> 1) the separations between the integration points
+       reach FP64 rounding noise and the terms' r = 0 branches; 1 - 1e-10 keeps the
+       closest approach above it */
+    for (int i = 0; i < nmu; ++i)
+        if (fabs(mu[i]) > 1.0 - 1e-10) return COFFE_EDOMAIN;
     /* the tables by slot; a needed one must be there with 5 nodes or more */
     int off[CE_INTG_NSLOT], len[CE_INTG_NSLOT];
     int64_t total_tab = 0;

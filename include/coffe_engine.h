@@ -451,7 +451,9 @@ enum {
  *             panels of n1 Gauss-Legendre nodes, the sinh-mapped x2 as pu panels of
  *             nu, each single-integrated piece as ps panels of ns.
  * Synchronous on the default engine; COFFE_EINVAL for bad sizes, flags or a missing
- * table, NaN where a lookup leaves its table.
+ * table, COFFE_EDOMAIN for |mu| > 1 - 1e-10 (along the line of sight the separations
+ * of the integration points fall to FP64 rounding noise and into the terms' r = 0
+ * branches), NaN where a lookup leaves its table.
  */
 int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int ntab, const int *tab_n, const int *tab_l,
                               const int *tab_len, const double *tab_x, const double *tab_y, int nren0,

@@ -70,7 +70,8 @@ def test_kernel_vs_oracle(engine, shared, flags):
 
 
 def test_kernel_rejects_bad_inputs(engine, shared):
-    """A table the flags need, an unknown flag bit or a mu outside [-1, 1] is refused."""
+    """A table the flags need, an unknown flag bit, a mu outside [-1, 1] or along the line
+    of sight (|mu| > 1 - 1e-10) is refused."""
     from coffe_amd import EngineError
     from coffe_amd.engine import integrated_corrfunc
     inp, tabs, bgd, pops, ren, ren0 = shared
@@ -82,6 +83,11 @@ def test_kernel_rejects_bad_inputs(engine, shared):
         integrated_corrfunc(*args, tabs, ["g4"], [1.0], [0.3], [50.0])       # no renormalisation grid
     with pytest.raises(EngineError):
         integrated_corrfunc(*args, tabs, ["len"], [1.0], [1.5], [50.0])
+    with pytest.raises(EngineError) as e:          # along the line of sight: COFFE_EDOMAIN
+        integrated_corrfunc(*args, tabs, ["len"], [1.0], [1.0], [50.0])
+    assert e.value.code == 5
+    s_, d_ = integrated_corrfunc(*args, tabs, ["len"], [1.0], [1.0 - 2e-10], [50.0])
+    assert np.isfinite(d_).all()
     with pytest.raises(ValueError):
         integrated_corrfunc(*args, tabs, ["len", "g6"], [1.0], [0.3], [50.0])
 
