@@ -858,13 +858,12 @@ class Coffe:
     def _terms(self, dg):
         """COFFE_TERM_* mask of the contribution flags; has_only_cross_correlations drops
         the auto terms (functions.c:129-669)."""
-        from .engine import TERM_D1_D1, TERM_DEN_DEN, TERM_RSD_RSD, terms_of
+        from .engine import TERM_D1_D1, TERM_DEN_DEN, TERM_DG_MUTUAL, TERM_RSD_RSD, terms_of
         t = terms_of(self._p["has_density"], self._p["has_rsd"], self._p["has_d1"], dg is not None)
         if self._p["has_only_cross_correlations"]:
-            if dg is not None:
-                raise NotImplementedError("has_only_cross_correlations with the d2 / g1-g3 terms: their mutual "
-                                          "terms are not separable from their auto terms in the engine")
             t &= ~(TERM_DEN_DEN | TERM_RSD_RSD | TERM_D1_D1)
+            if dg is not None:
+                t |= TERM_DG_MUTUAL   # the d2-d2, g1-g1, ... terms left out, their mutual ones kept
         return t
 
     def _ren_grid(self, chi_max):

@@ -1133,8 +1133,10 @@ static int mu_weights_impl(coffe_engine *e, const coffe_mu_inputs *in, unsigned 
 {
     if (!e || !in || nz <= 0 || nl <= 0 || nl > 4 || ntab <= 0 || ntab > CE_MAX_TABLES + 1 || nsep <= 0 ||
         nmu < (rule_mu ? 1 : 2) || nmu > (1 << 20) || (!rule_mu) != (!rule_w) ||
-        terms == 0 || terms > 1023 || !d_A || !zmean || !ls || !tab_n || !tab_l || !sep || (paired && nz != nsep))
+        terms == 0 || terms > 2047 || !d_A || !zmean || !ls || !tab_n || !tab_l || !sep || (paired && nz != nsep))
         return COFFE_EINVAL;
+    /* the mutual-only d2 / g terms qualify COFFE_TERM_DG_DG */
+    if ((terms & COFFE_TERM_DG_MUTUAL) && !(terms & COFFE_TERM_DG_DG)) return COFFE_EINVAL;
     const int nbg = in->nbg, nb = in->nb;
     if (nbg < 5 || nb < 5 || !in->bg_z || !in->bg_chi || !in->bg_D1 || !in->bg_f || !in->bg_cH || !in->bg_G1 ||
         !in->bg_G2 || !in->bias_z || !in->bias1 || !in->bias2)

@@ -216,6 +216,19 @@ __device__ __forceinline__ double dg_source(const mu_args &a, const endpoint &e)
     return S;
 }
 
+// the auto part of S(1) S(2): the sum over the chosen sources of each one's factor at
+// the two end points (the d2-d2, g1-g1, g2-g2, g3-g3 terms of functions.c:574-669)
+__device__ __forceinline__ double dg_auto(const mu_args &a, const endpoint &e1, const endpoint &e2)
+{
+    double D = 0.0;
+    if (a.dg & 1u) D += (3 - e1.fevo) * e1.H * e1.H * e1.f * ((3 - e2.fevo) * e2.H * e2.H * e2.f);
+    const double g1 = a.H0h * a.H0h * 3 * a.Om / 2 / e1.a, g2 = a.H0h * a.H0h * 3 * a.Om / 2 / e2.a;
+    if (a.dg & 2u) D += g1 * (1 + e1.G) * (g2 * (1 + e2.G));
+    if (a.dg & 4u) D += g1 * (5 * e1.s - 2) * (g2 * (5 * e2.s - 2));
+    if (a.dg & 8u) D += g1 * (e1.f - 1) * (g2 * (e2.f - 1));
+    return D;
+}
+
 // the renormalisation grid's bicubic (GSL interp2d bicubic: node derivatives of natural
 // cubic splines, Hermite patches; coffe_interp_spline2d, common.c:712-719) at (x, y)
 __device__ __forceinline__ double ren_bicubic(const mu_args &a, double x, double y)
@@ -323,8 +336,10 @@ __device__ __forceinline__ void local_coefficients(const mu_args &a, double mu, 
     if constexpr (DG) {
         const double Sg1 = dg_source(a, e1), Sg2 = dg_source(a, e2);
         if (a.terms & COFFE_TERM_DG_DG) {     // I_0^4 minus its renormalisation at (chi1, chi2)
-            c[S40] += Sg1 * Sg2;
-            c[SREN] -= Sg1 * Sg2 * ren_bicubic(a, chi1, chi2);
+            double P = Sg1 * Sg2;
+            if (a.terms & COFFE_TERM_DG_MUTUAL) P -= dg_auto(a, e1, e2);
+            c[S40] += P;
+            c[SREN] -= P * ren_bicubic(a, chi1, chi2);
         }
         if (a.terms & COFFE_TERM_DEN_DG) c[S20] += b1 * Sg2 + b2 * Sg1;
         if (a.terms & COFFE_TERM_RSD_DG) {

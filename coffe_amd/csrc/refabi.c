@@ -366,10 +366,10 @@ static int local_setup(const coffe_parameters_t *par, const coffe_background_t *
         if (c->d1) t |= COFFE_TERM_D1_DG;
     }
     if (par->only_cross_correlations) {
-        /* the reference skips the auto terms (functions.c:129-669); the d2 / g sources'
-           mutual terms are not separable from their autos in the term mask */
-        if (src) return COFFE_EUNSUPPORTED;
+        /* the reference skips the auto terms (functions.c:129-669), those of the d2 / g
+           sources among them */
         t &= ~(unsigned)(COFFE_TERM_DEN_DEN | COFFE_TERM_RSD_RSD | COFFE_TERM_D1_D1);
+        if (src) t |= COFFE_TERM_DG_MUTUAL;
     }
     lc->terms = t;
     lc->sources = src;

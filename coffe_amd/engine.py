@@ -496,6 +496,7 @@ def akima(x, y, xq):
 
 TERM_DEN_DEN, TERM_RSD_RSD, TERM_D1_D1, TERM_DEN_RSD, TERM_DEN_D1, TERM_RSD_D1 = 1, 2, 4, 8, 16, 32
 TERM_DG_DG, TERM_DEN_DG, TERM_RSD_DG, TERM_D1_DG = 64, 128, 256, 512
+TERM_DG_MUTUAL = 1024   # with TERM_DG_DG: the mutual d2 / g terms only, their autos left out
 
 
 def terms_of(den, rsd, d1, dg=False):

@@ -177,7 +177,10 @@ enum {
     COFFE_TERM_DG_DG = 64,
     COFFE_TERM_DEN_DG = 128,
     COFFE_TERM_RSD_DG = 256,
-    COFFE_TERM_D1_DG = 512
+    COFFE_TERM_D1_DG = 512,
+    /* with COFFE_TERM_DG_DG: only the mutual terms among the d2 / g sources, their autos
+       left out (only_cross_correlations, functions.c:574-669) */
+    COFFE_TERM_DG_MUTUAL = 1024
 };
 
 /* the d2 / g sources of coffe_mu_weights_ex */

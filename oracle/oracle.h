@@ -103,6 +103,7 @@ typedef struct orc_dg {
     const double *div_x, *div_y;
     size_t nren;         /* renormalisation grid on ren_chi: ren[b nren + a] at (chi_a, chi_b) (bicubic) */
     const double *ren_chi, *ren;
+    int only_cross;      /* only_cross_correlations: the d2-d2, g1-g1, g2-g2, g3-g3 terms left out */
 } orc_dg;
 double orc_functions_nonintegrated(const void *ctx, double z_mean, double mu, double sep);
 int orc_multipoles(const double *const *tab_x, const double *const *tab_y, const size_t *tab_n, size_t nbg,

@@ -264,17 +264,19 @@ class DG(ctypes.Structure):
                 ("H0h", ctypes.c_double), ("Omega_m", ctypes.c_double), ("bg_a", c_double_p),
                 ("nmb", c_size), ("mb_z", c_double_p), ("s1", c_double_p), ("s2", c_double_p),
                 ("fevo1", c_double_p), ("fevo2", c_double_p), ("ndiv", c_size), ("div_x", c_double_p),
-                ("div_y", c_double_p), ("nren", c_size), ("ren_chi", c_double_p), ("ren", c_double_p)]
+                ("div_y", c_double_p), ("nren", c_size), ("ren_chi", c_double_p), ("ren", c_double_p),
+                ("only_cross", ctypes.c_int)]
 
 
-def make_dg(flags, H0h, Omega_m, bg_a, mb_z, s1, s2, fevo1, fevo2, div_x, div_y, ren_chi, ren):
-    """An orc_dg for the flags dict (d2, g1, g2, g3); returns (struct, the arrays it points
-    into, which the caller keeps alive)."""
+def make_dg(flags, H0h, Omega_m, bg_a, mb_z, s1, s2, fevo1, fevo2, div_x, div_y, ren_chi, ren, only_cross=0):
+    """An orc_dg for the flags dict (d2, g1, g2, g3); only_cross leaves their autos out
+    (only_cross_correlations); returns (struct, the arrays it points into, which the
+    caller keeps alive)."""
     keep = [_f64(a) for a in (bg_a, mb_z, s1, s2, fevo1, fevo2, div_x, div_y, ren_chi, ren)]
     st = DG(int(flags.get("d2", 0)), int(flags.get("g1", 0)), int(flags.get("g2", 0)), int(flags.get("g3", 0)),
             float(H0h), float(Omega_m), _p(keep[0]), len(keep[1]), _p(keep[1]), _p(keep[2]), _p(keep[3]),
             _p(keep[4]), _p(keep[5]), len(keep[6]), _p(keep[6]), _p(keep[7]), len(keep[8]), _p(keep[8]),
-            _p(keep[9]))
+            _p(keep[9]), int(only_cross))
     return st, keep
 
 

@@ -46,7 +46,7 @@ typedef struct {
     akima_t chi, z_of_chi, D1, f, cH, G1, G2, b1, b2;
     int den, rsd, d1, flatsky;
     /* d2 and g1-g3 (orc_dg): flags, end-point tables, divergent table and its grid */
-    int d2, g1, g2, g3;
+    int d2, g1, g2, g3, dg_only_cross;
     double H0h, Omega_m;
     akima_t a, s1, s2, fevo1, fevo2, div;
     orc_bicubic *ren;
@@ -168,9 +168,10 @@ double orc_functions_nonintegrated(const void *vctx, double z_mean, double mu, d
         /* the g sources' end-point factors (1 + G), (5 s - 2), (f - 1) */
         const int gon[3] = {c->g1, c->g2, c->g3};
         const double g_1[3] = {1 + G1, 5 * s1 - 2, f1 - 1}, g_2[3] = {1 + G2, 5 * s2 - 2, f2 - 1};
-        if (c->d2) result += (3 - fevo1) * (3 - fevo2) * pow(H1, 2) * pow(H2, 2) * f1 * f2 * I40;
+        /* the autos, skipped with only_cross_correlations (functions.c:574-669) */
+        if (c->d2 && !c->dg_only_cross) result += (3 - fevo1) * (3 - fevo2) * pow(H1, 2) * pow(H2, 2) * f1 * f2 * I40;
         for (int i = 0; i < 3; ++i)
-            if (gon[i]) result += h4 * (9 * pow(Om, 2) * g_1[i] * g_2[i] / 4 / a1 / a2 * I40);
+            if (gon[i] && !c->dg_only_cross) result += h4 * (9 * pow(Om, 2) * g_1[i] * g_2[i] / 4 / a1 / a2 * I40);
         if (c->den && c->d2)
             result += ((3 - fevo2) * b1 * f2 * pow(H2, 2) + (3 - fevo1) * b2 * f1 * pow(H1, 2)) * I20;
         for (int i = 0; i < 3; ++i)
@@ -225,6 +226,7 @@ static int dg_init(mp_ctx *c, size_t nbg, const double *bg_z, const orc_dg *dg)
 {
     if (!dg || !(dg->d2 || dg->g1 || dg->g2 || dg->g3)) return ORC_OK;
     c->d2 = dg->d2; c->g1 = dg->g1; c->g2 = dg->g2; c->g3 = dg->g3;
+    c->dg_only_cross = dg->only_cross;
     c->H0h = dg->H0h;
     c->Omega_m = dg->Omega_m;
     int st = ak_init(&c->a, nbg, bg_z, dg->bg_a);

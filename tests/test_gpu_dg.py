@@ -136,3 +136,56 @@ def test_dg_coffe_magnification_and_evolution(engine, setup):
     for il in range(3):
         err = np.abs(got[il] - ref[il])
         assert np.all(err <= 1e-9 * np.abs(ref[il]) + 1e-12 * np.abs(ref[0])), (il, err / np.abs(ref[il]))
+
+
+@pytest.mark.parametrize("flat", [False, True])
+def test_dg_mutual_only_vs_oracle(engine, setup, flat):
+    """COFFE_TERM_DG_MUTUAL (only_cross_correlations with d2 / g sources): the products of
+    the chosen sources without their autos (functions.c:574-669 skip d2-d2, g1-g1, ...),
+    against the oracle with those autos left out; full sky with the sources alone, flat
+    sky next to density, RSD and Doppler-1 (whose own terms stay), 1e-9."""
+    import torch
+    from coffe_amd.engine import TERM_DG_MUTUAL, mu_weights, terms_of
+    S = setup
+    srcs = {"d2", "g1", "g3"}
+    loc = dict(den=1, rsd=1, d1=1) if flat else dict(den=0, rsd=0, d1=0)
+    sep = np.array([20.0, 80.0, 200.0]) / H
+    zm, ls = [0.7, 1.5], [0, 2]
+    b1, b2 = 1.2 + 0.3 * S["zb"], 1.5 + 0.1 * S["zb"]
+    s1, s2, e1, e2 = S["mb"]
+    Om, H0h = 0.30, H / 2997.92458
+    dg = dict(sources=srcs, H0h=H0h, Omega_m=Om, mb_z=S["zb"], s1=s1, s2=s2, fevo1=e1, fevo2=e2, ren_chi=S["chi"],
+              ren=S["ren"])
+    tables = TABLES + [(4, 0)]
+    terms = terms_of(loc["den"], loc["rsd"], loc["d1"], True) | TERM_DG_MUTUAL
+    print(f"d2/g mutual only, flat={flat}", flush=True)
+    ren = torch.empty((len(zm), len(ls), len(sep)), dtype=torch.float64, device="cuda")
+    A = mu_weights(engine, S["bgo"], S["zb"], b1, b2, zm, ls, tables, sep, flatsky=flat, nmu=256, dg=dg, ren_out=ren,
+                   terms=terms).cpu().numpy()
+    I = np.array([O.akima(*(S["tabs"][t] if t != (4, 0) else S["div"]), sep) for t in tables])
+    got = np.einsum("zltr,tr->zlr", A, I) + ren.cpu().numpy()
+    odg, keep = O.make_dg({k_: 1 for k_ in srcs}, H0h, Om, S["bg"]["a"], S["zb"], s1, s2, e1, e2, *S["div"], S["chi"],
+                          S["ren"], only_cross=1)
+    ref = O.multipoles(S["tabs"], S["bgo"], S["zb"], b1, b2, zm, ls, sep, flatsky=flat, quad=256, dg=odg, **loc)
+    for il in range(len(ls)):
+        err = np.abs(got[:, il] - ref[:, il])
+        assert np.all(err <= 1e-9 * np.abs(ref[:, il]) + 1e-12 * np.abs(ref[:, 0])), (flat, il, err / np.abs(ref[:, il]))
+
+
+def test_dg_only_cross_coffe(engine):
+    """has_only_cross_correlations with density, d2 and g1 through the Coffe object: the
+    full result minus each contribution alone (their autos), to 1e-10 of the monopole."""
+    from coffe_amd.coffe import Coffe
+    k, pk = np.loadtxt(os.path.join(GOLD, "PkL_CLASS.dat")).T
+
+    def run(**flags):
+        c = Coffe(sep=np.array([20.0, 60.0, 150.0]) / H, l=[0, 2], has_density=False)
+        c.set_power_spectrum_linear(k * H, pk / H ** 3)
+        c.set_parameters(**flags)
+        return np.array([r.value for r in c.compute_multipoles_bulk()])
+    print("only_cross with d2 / g1 via Coffe", flush=True)
+    both = dict(has_density=True, has_d2=True, has_g1=True)
+    cross = run(**both, has_only_cross_correlations=True)
+    want = run(**both) - run(has_density=True) - run(has_d2=True) - run(has_g1=True)
+    mono = np.abs(run(**both)[:3]).max()
+    assert np.allclose(cross, want, rtol=0, atol=1e-10 * mono), (cross, want)
