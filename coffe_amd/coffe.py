@@ -27,7 +27,6 @@ from . import background as _background
 from .representation import AverageCovariance, AverageMultipoles, Corrfunc, Covariance, Multipoles
 
 TABLES = [(0, 0), (0, 2), (0, 4), (1, 1), (1, 3), (2, 0), (2, 2), (3, 1)]   # integrals.c:599-608
-_UNSUPPORTED = ()
 _DG = ("d2", "g1", "g2", "g3")
 COFFE_H0 = 1 / 2997.92458   # h/Mpc (common.h)
 
@@ -222,16 +221,13 @@ class Coffe:
         """The settable parameters and their values, as set_parameters takes them
         (coffe.pyx:1136-1154)."""
         out = {}
-        for k in list(self._p) + ["omega_m"] + list(_UNSUPPORTED):
+        for k in list(self._p) + ["omega_m"]:
             v = getattr(self, k)
             out[k] = list(v) if isinstance(v, np.ndarray) else v
         return out
 
     def set_parameters(self, **kwargs):
         for k, v in kwargs.items():
-            if k in _UNSUPPORTED:
-                setattr(self, k, v)
-                continue
             if not hasattr(type(self), k) or not isinstance(getattr(type(self), k), property):
                 raise AttributeError(f"unknown parameter {k}")
             setattr(self, k, v)
@@ -452,18 +448,6 @@ class Coffe:
         if value != 0:
             raise NotImplementedError("massive neutrinos (m_ncdm != 0) are not part of this engine")
         self._p["m_ncdm"] = float(value)
-
-    def __getattr__(self, name):
-        if name in _UNSUPPORTED:
-            return False
-        raise AttributeError(name)
-
-    def __setattr__(self, name, value):
-        if name in _UNSUPPORTED:
-            if value:
-                raise NotImplementedError(f"{name}: lensing, g4 and g5 are not computed by this engine")
-            return
-        object.__setattr__(self, name, value)
 
     def reset_contributions(self):
         """Sets every writable has_* parameter to False (coffe.pyx:2702-2708)."""
