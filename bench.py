@@ -708,7 +708,7 @@ def main():
                    "threads_per_wg": int(os.environ.get("COFFE_ENGINE_THREADS", 512)),
                    "launch": "hipGraph replay of the captured step" if args.graph else "eager (one C call)",
                    "autotune": tuned,
-                   "other_configs": "bench.py --workload fullsky | covariance"},
+                   "other_configs": "bench.py --workload fullsky | covariance | divergent | integrated"},
         "roofline": {"bound": "mfma", "kernel": dom, "unit_kind": "FP64 VALU",
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
