@@ -104,9 +104,9 @@ def test_only_cross_correlations(engine):
     scale = np.max(np.abs(full))
     assert np.max(np.abs(cross - (full - autos))) <= 1e-12 * scale
     assert np.max(np.abs(cross)) > 1e-3 * scale
-    with pytest.raises(NotImplementedError):
-        c.has_d2 = True
-        c.compute_multipoles_bulk()
+    c.has_d2 = True          # d2 joins as cross terms only (its auto left out; tests/test_gpu_dg.py)
+    with_d2 = np.array([r.value for r in c.compute_multipoles_bulk()])
+    assert np.isfinite(with_d2).all() and np.max(np.abs(with_d2 - cross)) > 0
     c.has_d2 = False
     from coffe_amd._lib import load_library
     L = R.declare(load_library())
