@@ -243,6 +243,8 @@ int ce_launch_mu_weights_jvp(void *stream, const ce_mu_dev *in, const ce_mu_tan 
 #define CE_INTG_NSLOT 9
 #define CE_INTG_ARGS_BYTES 2048
 #define CE_INTG_MAX_X1 512    /* x1 nodes of the double rule: (p1 + p1 / 2) n1 at most */
+#define CE_INTG_SEG 8         /* doubles per Akima segment record (64 bytes) */
+#define CE_INTG_NBGC 5        /* background curves with segment records */
 typedef struct ce_intg_dev {
     int nbg;
     const double *bg_z, *bg_chi, *bg_D1, *bg_a, *bg_f, *bg_cH, *bg_G1, *bg_G2;
@@ -260,8 +262,9 @@ typedef struct ce_intg_dev {
     int tab_ref;   /* a slot whose nodes every present standard table shares, or -1 */
     int nguess;
     const double *chi_guess;
-    /* workspace of 3 ntab_values + 15 nbg doubles for the Akima coefficients, and
-       CE_INTG_ARGS_BYTES of device memory for the kernel's arguments */
+    /* workspace of CE_INTG_SEG (ntab_values + CE_INTG_NBGC nbg) doubles, 64-byte
+       aligned, for the Akima segment records, and CE_INTG_ARGS_BYTES of device memory
+       for the kernel's arguments */
     int ntab_values;
     double *coef;
     void *args_dev;

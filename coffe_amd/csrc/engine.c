@@ -1854,11 +1854,12 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
     const size_t nren = want_ren ? (size_t)in->nren : 0, nnode = nren * nren, n0 = want_ren ? (size_t)nren0 : 0;
     const size_t ngl = 2 * (size_t)(r[1] + r[3] + r[5]), nout = 2 * (size_t)nz * nmu * nsep;
     const int nguess = 4096;
+    /* the segment records, and 7 doubles of slack to align them to 64 bytes */
+    const size_t nrec = CE_INTG_SEG * ((size_t)total_tab + CE_INTG_NBGC * nbg) + 7;
     const size_t total = 8 * nbg + 3 * nb + 5 * nmb + 2 * (size_t)total_tab + 2 * n0 + nren + 4 * nnode +
-                         (size_t)nz + (size_t)nmu + (size_t)nsep + ngl + (size_t)nguess + 1 + nout +
-                         3 * (size_t)total_tab + 15 * nbg + CE_INTG_ARGS_BYTES / sizeof(double);
-    double *h = (double *)malloc(sizeof(double) * (total - nout - 3 * (size_t)total_tab - 15 * nbg -
-                                                    CE_INTG_ARGS_BYTES / sizeof(double)));
+                         (size_t)nz + (size_t)nmu + (size_t)nsep + ngl + (size_t)nguess + 1 + nout + nrec +
+                         CE_INTG_ARGS_BYTES / sizeof(double);
+    double *h = (double *)malloc(sizeof(double) * (total - nout - nrec - CE_INTG_ARGS_BYTES / sizeof(double)));
     double *work = (double *)malloc(sizeof(double) * (2 * nren + 2));
     if (!h || !work) {
         free(h);
@@ -1975,8 +1976,8 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
 #undef IG_PUT
     double *d_out = d + o;
     dv.ntab_values = (int)total_tab;
-    dv.coef = d_out + nout;
-    dv.args_dev = d_out + nout + 3 * (size_t)total_tab + 15 * nbg;
+    dv.coef = (double *)(((uintptr_t)(d_out + nout) + 63) & ~(uintptr_t)63);
+    dv.args_dev = d_out + nout + nrec;
     hipStream_t s = e->stream;
     if (st == COFFE_OK) st = hip_ok(hipMemcpyAsync(d, h, sizeof(double) * o, hipMemcpyHostToDevice, s));
     if (st == COFFE_OK && ce_launch_integrated(s, &dv, flags, nz, d_z, nmu, d_mu, nsep, d_sep, r, d_gl, d_out))

@@ -46,9 +46,9 @@ struct intg_args {
     double tab_lx0[NSLOT], tab_lsc[NSLOT];   // its log(x[f0]) and nodes per unit of log x
     int nguess;                        // chi -> interval guesses on [0, chi_max], nguess + 1 of them
     const double *chi_guess;
-    // Akima coefficients (b, c, d per node, the last node's unused): the tables' in the
-    // layout of tab_y, the background curves' [NBGC][nbg]
-    const double *tab_b, *tab_c, *tab_d, *bgc_b, *bgc_c, *bgc_d;
+    // Akima segment records (SEG doubles per node, below): the tables' in the layout of
+    // tab_y, the background curves' [NBGC][nbg]
+    const double *tab_rec, *bgc_rec;
     int nren0;
     const double *ren0_x, *ren0_y;
     int nren;
@@ -64,13 +64,37 @@ struct intg_args {
 
 __device__ inline double ak(const double *x, const double *y, int n, double v) { return ce::akima_general(x, y, n, v); }
 
-// y(xq) in interval j from the precomputed Akima coefficients (the value akima_at gives)
-__device__ __forceinline__ double akc(const double *__restrict__ x, const double *__restrict__ y,
-                                      const double *__restrict__ b, const double *__restrict__ c,
-                                      const double *__restrict__ d, int j, double xq)
+// The Akima segment of node j of a curve as one 64-byte record {x_j, y_j, b_j, c_j, d_j,
+// x_(j+1), -, -}: a lookup reads one cache line in three 16-byte loads, where separate
+// node and coefficient arrays cost five scattered lines, and the record holds both ends
+// of its interval, so a guessed interval is tested from the same loads.  (The lookups
+// of this kernel gather from the L1 cache at random per lane; its throughput, not
+// arithmetic, bounds them.)
+constexpr int SEG = CE_INTG_SEG;
+struct segv {
+    double x0, y, b, c, d, x1;
+};
+
+__device__ __forceinline__ segv seg_load(const double *__restrict__ r)
 {
-    const double dx = xq - x[j];
-    return y[j] + dx * (b[j] + dx * (c[j] + d[j] * dx));
+    const double2 p = *reinterpret_cast<const double2 *>(r), q = *reinterpret_cast<const double2 *>(r + 2),
+                  u = *reinterpret_cast<const double2 *>(r + 4);
+    return segv{p.x, p.y, q.x, q.y, u.x, u.y};
+}
+
+// y(xq) on the segment (the value akima_at gives in its interval)
+__device__ __forceinline__ double seg_at(const segv &g, double xq)
+{
+    const double dx = xq - g.x0;
+    return g.y + dx * (g.b + dx * (g.c + g.d * dx));
+}
+
+// whether interval j (0 <= j <= n - 2), the segment's, is the one ce::interval_from
+// returns from the guess j at once (false for a NaN v, whose callers then search and
+// get j back)
+__device__ __forceinline__ bool seg_hit(const segv &g, int j, int n, double v)
+{
+    return g.x0 <= v && (j == n - 2 || v < g.x1);
 }
 
 // The lookups: Akima of raw nodes as ce::akima_general, with the interval found from a
@@ -88,15 +112,44 @@ struct lk {
         if (!(z >= x[0] && z <= x[n - 1])) return __builtin_nan("");
         return ce::akima_at(x, col, n, ce::interval_from(x, n, z, ce::interval_guess(x, n, z)), z);
     }
-    // D1, a, f or conformal H (curve k, values col) by the precomputed coefficients
-    __device__ double bgc(int k, const double *col, double z) const
+    static __device__ int below_last(int g, int n) { return g > n - 2 ? n - 2 : g; }
+    // the first guess of z's interval on the background's redshift grid (-1 outside it)
+    __device__ int bg_guess(double z) const
     {
         const double *x = a.bg_z;
         const int n = a.nbg;
-        if (!(z >= x[0] && z <= x[n - 1])) return __builtin_nan("");
-        const int j = ce::interval_from(x, n, z, ce::interval_guess(x, n, z));
-        const size_t o = (size_t)k * n;
-        return akc(x, col, a.bgc_b + o, a.bgc_c + o, a.bgc_d + o, j, z);
+        if (!(z >= x[0] && z <= x[n - 1])) return -1;
+        return below_last(ce::interval_guess(x, n, z), n);
+    }
+    // the interval of z on the redshift grid (-1 outside it), which the curves read at
+    // one z share
+    __device__ int bg_interval(double z) const
+    {
+        const int g = bg_guess(z);
+        return g < 0 ? -1 : ce::interval_from(a.bg_z, a.nbg, z, g);
+    }
+    __device__ segv bgseg(int k, int j) const { return seg_load(a.bgc_rec + SEG * ((size_t)k * a.nbg + j)); }
+    // D1, a, f or conformal H (curve k) in interval j (NaN for j < 0)
+    __device__ double bgcj(int k, int j, double z) const
+    {
+        return j < 0 ? __builtin_nan("") : seg_at(bgseg(k, j), z);
+    }
+    // curve k at z, the interval found from the guess; *jz (if given) receives it
+    __device__ double bgc(int k, double z, int *jz = nullptr) const
+    {
+        int j = bg_guess(z);
+        double v = __builtin_nan("");
+        if (j >= 0) {
+            const segv g = bgseg(k, j);
+            if (seg_hit(g, j, a.nbg, z)) {
+                v = seg_at(g, z);
+            } else {
+                j = ce::interval_from(a.bg_z, a.nbg, z, j);
+                v = bgcj(k, j, z);
+            }
+        }
+        if (jz) *jz = j;
+        return v;
     }
     __device__ double z_of(double chi) const
     {
@@ -104,41 +157,44 @@ struct lk {
         const int n = a.nbg;
         if (!(chi >= x[0] && chi <= x[n - 1])) return __builtin_nan("");
         const int k = ce::clamp_index(chi / x[n - 1] * a.nguess, a.nguess + 1);
-        const int j = ce::interval_from(x, n, chi, (int)a.chi_guess[k]);
-        const size_t o = (size_t)C_ZOF * n;
-        return akc(x, a.bg_z, a.bgc_b + o, a.bgc_c + o, a.bgc_d + o, j, chi);
+        const int j = below_last(max((int)a.chi_guess[k], 0), n);
+        const segv g = bgseg(C_ZOF, j);
+        if (seg_hit(g, j, n, chi)) return seg_at(g, chi);
+        return seg_at(bgseg(C_ZOF, ce::interval_from(x, n, chi, j)), chi);
     }
-    __device__ double tab(int s, double r) const
+    // a first interval of r on table s's nodes (-1: r outside them): by bisection below
+    // the log-uniform part (exact), from the log above it (a guess)
+    __device__ int tab_guess(int s, double r) const
     {
-        const double *x = a.tab_x + a.tab_off[s], *y = a.tab_y + a.tab_off[s];
-        const int n = a.tab_len[s], f0 = a.tab_f0[s];
-        if (!(r >= x[0] && r <= x[n - 1])) return __builtin_nan("");
-        int j;
-        if (f0 < 0) j = ce::interval_bisect(x, 0, n - 1, r);
-        else if (r < x[f0]) j = ce::interval_bisect(x, 0, f0, r);
-        else j = ce::interval_from(x, n, r, f0 + ce::clamp_index((log(r) - a.tab_lx0[s]) * a.tab_lsc[s], n - f0));
-        const int o = a.tab_off[s];
-        return akc(x, y, a.tab_b + o, a.tab_c + o, a.tab_d + o, j, r);
-    }
-    // the interval of r on the standard tables' shared nodes (-1: none shared, or r
-    // outside them), and a standard table's value there
-    __device__ int tab_interval(double r) const
-    {
-        const int s = a.tab_ref;
-        if (s < 0) return -1;
         const double *x = a.tab_x + a.tab_off[s];
         const int n = a.tab_len[s], f0 = a.tab_f0[s];
         if (!(r >= x[0] && r <= x[n - 1])) return -1;
         if (f0 < 0) return ce::interval_bisect(x, 0, n - 1, r);
         if (r < x[f0]) return ce::interval_bisect(x, 0, f0, r);
-        return ce::interval_from(x, n, r, f0 + ce::clamp_index((log(r) - a.tab_lx0[s]) * a.tab_lsc[s], n - f0));
+        return below_last(f0 + ce::clamp_index((log(r) - a.tab_lx0[s]) * a.tab_lsc[s], n - f0), n);
     }
-    __device__ double tabj(int s, int j, double r) const
+    __device__ segv tabseg(int s, int j) const { return seg_load(a.tab_rec + SEG * ((size_t)a.tab_off[s] + j)); }
+    // interval j of table s settled from its guess g (-1 stays -1)
+    __device__ int tab_settle(int s, int g, double r) const
     {
-        if (j < 0) return tab(s, r);
-        const int o = a.tab_off[s];
-        return akc(a.tab_x + o, a.tab_y + o, a.tab_b + o, a.tab_c + o, a.tab_d + o, j, r);
+        if (g < 0 || seg_hit(tabseg(s, g), g, a.tab_len[s], r)) return g;
+        return ce::interval_from(a.tab_x + a.tab_off[s], a.tab_len[s], r, g);
     }
+    __device__ double tab(int s, double r) const
+    {
+        const int g = tab_guess(s, r);
+        if (g < 0) return __builtin_nan("");
+        const segv v = tabseg(s, g);
+        if (seg_hit(v, g, a.tab_len[s], r)) return seg_at(v, r);
+        return seg_at(tabseg(s, ce::interval_from(a.tab_x + a.tab_off[s], a.tab_len[s], r, g)), r);
+    }
+    // the interval of r on the standard tables' shared nodes (-1: none shared, or r
+    // outside them), and a standard table's value there
+    __device__ int tab_interval(double r) const
+    {
+        return a.tab_ref < 0 ? -1 : tab_settle(a.tab_ref, tab_guess(a.tab_ref, r), r);
+    }
+    __device__ double tabj(int s, int j, double r) const { return j < 0 ? tab(s, r) : seg_at(tabseg(s, j), r); }
     // GSL interp2d bicubic of the renormalisation grid at (x, y) (node derivatives of
     // natural cubic splines, Hermite patches; common.c:712-719)
     __device__ double bicubic(double x, double y) const
@@ -202,22 +258,42 @@ __device__ pt make_pt(const intg_args &a, double zm, double mu, double sep)
     p.fe2 = ak(a.mb_z, a.fe2, a.nmb, z2c);
     p.b1 = ak(a.bias_z, a.b1, a.nb, z1c);
     p.b2 = ak(a.bias_z, a.b2, a.nb, z2c);
-    p.Dc1 = L.bgc(C_D1, a.bg_D1, z1c);
-    p.Dc2 = L.bgc(C_D1, a.bg_D1, z2c);
-    p.fc1 = L.bgc(C_F, a.bg_f, z1c);
-    p.fc2 = L.bgc(C_F, a.bg_f, z2c);
-    p.cHc1 = L.bgc(C_CH, a.bg_cH, z1c);
-    p.cHc2 = L.bgc(C_CH, a.bg_cH, z2c);
-    p.ac1 = L.bgc(C_A, a.bg_a, z1c);
-    p.ac2 = L.bgc(C_A, a.bg_a, z2c);
+    p.Dc1 = L.bgc(C_D1, z1c);
+    p.Dc2 = L.bgc(C_D1, z2c);
+    p.fc1 = L.bgc(C_F, z1c);
+    p.fc2 = L.bgc(C_F, z2c);
+    p.cHc1 = L.bgc(C_CH, z1c);
+    p.cHc2 = L.bgc(C_CH, z2c);
+    p.ac1 = L.bgc(C_A, z1c);
+    p.ac2 = L.bgc(C_A, z2c);
     p.G1c = L.bg(a.bg_G1, z1c);
     p.G2c = L.bg(a.bg_G2, z2c);
     p.G1c2 = L.bg(a.bg_G1, z2c);
     return p;
 }
 
-// functions_double_integrated (functions.c:3460-4017) at (x1, x2), times (H0 h)^4
-__device__ __forceinline__ double double_point(const intg_args &a, const pt &p, double x1, double x2)
+// The values along line of sight 1 that the double-integrated terms read at one x1:
+// D1, a, conformal H and f at z(chi1 x1) (the last two with g5 only)
+enum { X1_D1, X1_A, X1_CH, X1_F, X1_N };
+
+__device__ void x1_values(const intg_args &a, double l1, double *v)
+{
+    const lk L(a);
+    const double z1 = L.z_of(l1);
+    const int j1 = L.bg_interval(z1);
+    v[X1_D1] = L.bgcj(C_D1, j1, z1);
+    v[X1_A] = L.bgcj(C_A, j1, z1);
+    v[X1_CH] = v[X1_F] = 0.0;
+    if (a.flags & F_G5) {
+        v[X1_CH] = L.bgcj(C_CH, j1, z1);
+        v[X1_F] = L.bgcj(C_F, j1, z1);
+    }
+}
+
+// functions_double_integrated (functions.c:3460-4017) at (x1, x2), times (H0 h)^4; v1
+// holds x1_values at chi1 x1
+__device__ __forceinline__ double double_point(const intg_args &a, const pt &p, double x1, double x2,
+                                               const double *v1)
 {
     const lk L(a);
     const unsigned f = a.flags;
@@ -225,9 +301,10 @@ __device__ __forceinline__ double double_point(const intg_args &a, const pt &p, 
     const double l1 = p.chi1 * x1, l2 = p.chi2 * x2;
     const double r2 = fmax(l1 * l1 + l2 * l2 - 2 * l1 * l2 * c, 0.0);
     const double r = sqrt(r2);
-    const double z1 = L.z_of(l1), z2 = L.z_of(l2);
-    const double D1 = L.bgc(C_D1, a.bg_D1, z1), D2 = L.bgc(C_D1, a.bg_D1, z2);
-    const double DD = D1 * D2 / L.bgc(C_A, a.bg_a, z1) / L.bgc(C_A, a.bg_a, z2);
+    const double z2 = L.z_of(l2);
+    int j2;
+    const double D1 = v1[X1_D1], D2 = L.bgc(C_D1, z2, &j2);
+    const double DD = D1 * D2 / v1[X1_A] / L.bgcj(C_A, j2, z2);
     const double LL = l1 * l2;
     const double q2 = (2 - 5 * p.s1) * (2 - 5 * p.s2);
     const bool lens = f & F_LEN, g4 = f & F_G4, g5 = f & F_G5, oc = f & F_ONLY_CROSS;
@@ -251,10 +328,10 @@ __device__ __forceinline__ double double_point(const intg_args &a, const pt &p, 
     }
     double cH1 = 0, cH2 = 0, f1 = 0, f2 = 0;
     if (g5) {
-        cH1 = L.bgc(C_CH, a.bg_cH, z1);
-        cH2 = L.bgc(C_CH, a.bg_cH, z2);
-        f1 = L.bgc(C_F, a.bg_f, z1);
-        f2 = L.bgc(C_F, a.bg_f, z2);
+        cH1 = v1[X1_CH];
+        cH2 = L.bgcj(C_CH, j2, z2);
+        f1 = v1[X1_F];
+        f2 = L.bgcj(C_F, j2, z2);
     }
     if (g4 && !oc) res += 9 * Om2 * q2 * DD * ren;
     if (g5 && !oc) res += 9 * Om2 * p.G1c * p.G2c * p.chi1 * p.chi2 * DD * cH1 * cH2 * (f1 - 1) * (f2 - 1) * ren;
@@ -317,8 +394,9 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
     const double rA = sqrt(r21), rB = sqrt(r22);
     const int jA = L.tab_interval(rA), jB = L.tab_interval(rB);
     const double z1 = L.z_of(l1), z2 = L.z_of(l2);
-    const double D1x = L.bgc(C_D1, a.bg_D1, z1), D2x = L.bgc(C_D1, a.bg_D1, z2);
-    const double a1x = L.bgc(C_A, a.bg_a, z1), a2x = L.bgc(C_A, a.bg_a, z2);
+    const int j1 = L.bg_interval(z1), j2 = L.bg_interval(z2);
+    const double D1x = L.bgcj(C_D1, j1, z1), D2x = L.bgcj(C_D1, j2, z2);
+    const double a1x = L.bgcj(C_A, j1, z1), a2x = L.bgcj(C_A, j2, z2);
     const double VA = D2x / a2x, VB = D1x / a1x;
     const double WA = (1 - x) * VA, WB = (1 - x) * VB;
     const double s1 = p.s1, s2 = p.s2, c2 = 2 * c * c - 1, e = p.omc;
@@ -373,7 +451,8 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
         const bool div = f & (F_D2 | F_G1 | F_G2 | F_G3 | F_G4 | F_G5);
         const double ren1 = div ? L.ren(r21, l2, chi1, r21 == 0.0, l2) : 0.0;
         const double ren2 = div ? L.ren(r22, l1, chi2, r22 == 0.0, l1) : 0.0;
-        const double E1 = L.bgc(C_CH, a.bg_cH, z1) * (L.bgc(C_F, a.bg_f, z1) - 1), E2 = L.bgc(C_CH, a.bg_cH, z2) * (L.bgc(C_F, a.bg_f, z2) - 1);
+        const double E1 = L.bgcj(C_CH, j1, z1) * (L.bgcj(C_F, j1, z1) - 1),
+                     E2 = L.bgcj(C_CH, j2, z2) * (L.bgcj(C_F, j2, z2) - 1);
         auto rsdk = [&](double rr, double lam, double rv, int jv) {
             return (2 * rr / 3 + (c * c - 1) * lam * lam) * L.tabj(S22, jv, rv) - L.tabj(S20, jv, rv) / 3;
         };
@@ -483,12 +562,12 @@ __device__ int single_pieces(const pt &p, piece *pc)
     return np;
 }
 
-// the Akima coefficients of up to 16 curves, one thread per (node, curve)
+// the Akima segment records of up to 16 curves, one thread per (node, curve)
 struct coef_list {
     int ncurve;
     const double *x[16], *y[16];
     int n[16];
-    double *b[16], *c[16], *d[16];
+    double *rec[16];
 };
 
 __global__ void __launch_bounds__(256) k_intg_coef(coef_list L)
@@ -496,16 +575,28 @@ __global__ void __launch_bounds__(256) k_intg_coef(coef_list L)
     const int t = blockIdx.y;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = L.n[t];
-    if (j >= n - 1) return;
-    const ce::akima_cf<double> cf = ce::akima_coefs<double>(L.x[t], nullptr, L.y[t], nullptr, n, j);
-    L.b[t][j] = cf.b;
-    L.c[t][j] = cf.c;
-    L.d[t][j] = cf.d;
+    if (j >= n) return;
+    double *r = L.rec[t] + SEG * (size_t)j;
+    const double *x = L.x[t], *y = L.y[t];
+    double v[SEG] = {x[j], y[j], 0.0, 0.0, 0.0, x[j], 0.0, 0.0};   // the last node's: no segment
+    if (j < n - 1) {
+        const ce::akima_cf<double> cf = ce::akima_coefs<double>(x, nullptr, y, nullptr, n, j);
+        v[2] = cf.b;
+        v[3] = cf.c;
+        v[4] = cf.d;
+        v[5] = x[j + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < SEG; k += 2) *reinterpret_cast<double2 *>(r + k) = double2{v[k], v[k + 1]};
 }
 
-// the arguments live in device memory (a pointer, not a by-value struct, whose address
-// the lookups would otherwise take in private memory)
-__global__ void __launch_bounds__(256) k_integrated(const intg_args *__restrict__ pa)
+// one workgroup per (z, mu, r): PART 0 sums the single-integrated terms into out[..][0],
+// PART 1 the double-integrated ones into out[..][1]; two kernels, so that each gets the
+// registers of its own part (together they held 252 VGPRs, two wavefronts per SIMD).
+// The arguments live in device memory (a pointer, not a by-value struct, whose address
+// the lookups would otherwise take in private memory).
+template <int PART>
+__device__ __forceinline__ void integrated_body(const intg_args *__restrict__ pa)
 {
     const intg_args &a = *pa;
     const int idx = blockIdx.x;   // (z, mu, r), r fastest
@@ -518,11 +609,13 @@ __global__ void __launch_bounds__(256) k_integrated(const intg_args *__restrict_
     const bool want_single =
         ((f & F_LEN) && (f & (F_DEN | F_RSD | F_D1 | F_D2 | F_G1 | F_G2 | F_G3))) ||
         ((f & (F_G4 | F_G5)) && (f & (F_DEN | F_RSD | F_D1 | F_D2 | F_G1 | F_G2 | F_G3)));
-    if (want_single) {
+    if (PART == 0 && want_single) {
         // the pieces in LDS (a private array indexed by node would live in scratch)
         __shared__ piece pc[4];
         __shared__ int npc;
+        __shared__ pt sp;
         if (threadIdx.x == 0) {
+            sp = p;
             piece t[4];
             npc = single_pieces(p, t);
             for (int i = 0; i < 4; ++i) pc[i] = t[i];
@@ -537,16 +630,18 @@ __global__ void __launch_bounds__(256) k_integrated(const intg_args *__restrict_
             double w;
             const double u = composite(fmin(ua, ub), fmax(ua, ub), a.ps, a.ns, a.gsx, a.gsw, q % per, &w);
             const double x = P.cc + sgn * P.s * sinh(u);
-            acc_s += w * P.s * cosh(u) * single_point(a, p, x);
+            acc_s += w * P.s * cosh(u) * single_point(a, sp, x);
         }
     }
-    if (f & (F_LEN | F_G4 | F_G5)) {
+    if (PART == 1 && (f & (F_LEN | F_G4 | F_G5))) {
         const double xb = p.chi1 * p.c > p.chi2 ? p.chi2 / (p.chi1 * p.c) : 1.0;
         const int pA = a.p1, pB = xb < 1.0 ? (a.p1 / 2 > 1 ? a.p1 / 2 : 1) : 0;
         const int n1 = (pA + pB) * a.n1, nuu = a.pu * a.nu;
-        // per x1 node (CE_INTG_MAX_X1 at most, checked by the host): the node, its weight,
-        // the sinh scale and the u interval, once per workgroup
-        __shared__ double sx[CE_INTG_MAX_X1][5];
+        // per x1 node (CE_INTG_MAX_X1 at most, checked by the host; the launch sizes the
+        // dynamic LDS for the rule's count): the node, its weight, the sinh scale, the u
+        // interval and x1_values, once per workgroup
+        extern __shared__ double sxd[];
+        double(*sx)[5 + X1_N] = reinterpret_cast<double(*)[5 + X1_N]>(sxd);
         for (int i1 = threadIdx.x; i1 < n1; i1 += blockDim.x) {
             double w1;
             const double x1 = i1 < pA * a.n1 ? composite(0.0, xb, pA, a.n1, a.g1x, a.g1w, i1, &w1)
@@ -558,6 +653,7 @@ __global__ void __launch_bounds__(256) k_integrated(const intg_args *__restrict_
             sx[i1][2] = sc;
             sx[i1][3] = asinh((0.0 - centre) / sc);
             sx[i1][4] = asinh((p.chi2 - centre) / sc);
+            x1_values(a, l1, &sx[i1][5]);
         }
         __syncthreads();
         for (int q = threadIdx.x; q < n1 * nuu; q += blockDim.x) {
@@ -568,31 +664,32 @@ __global__ void __launch_bounds__(256) k_integrated(const intg_args *__restrict_
             // sinh and cosh from one exponential (lambda2's absolute error stays ~eps sc)
             const double eu = exp(u), ei = 1.0 / eu;
             const double lam2 = p.chi1 * x1 * p.c + sc * (0.5 * (eu - ei));
-            acc_d += w1 * wu * sc * (0.5 * (eu + ei)) / p.chi2 * double_point(a, p, x1, lam2 / p.chi2);
+            acc_d += w1 * wu * sc * (0.5 * (eu + ei)) / p.chi2 * double_point(a, p, x1, lam2 / p.chi2, &sx[i1][5]);
         }
     }
-    // workgroup sums: wavefront shuffles, then one slot per wavefront in LDS
+    // workgroup sum: wavefront shuffles, then one slot per wavefront in LDS
+    double acc = PART == 0 ? acc_s : acc_d;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        acc_s += __shfl_down(acc_s, o);
-        acc_d += __shfl_down(acc_d, o);
-    }
-    __shared__ double red[2][4];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+    __shared__ double red[4];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
-        red[0][wv] = acc_s;
-        red[1][wv] = acc_d;
-    }
+    if (lane == 0) red[wv] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double ss = 0.0, dd = 0.0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-            ss += red[0][k];
-            dd += red[1][k];
-        }
-        a.out[2 * (int64_t)idx] = ss;
-        a.out[2 * (int64_t)idx + 1] = dd;
+        double ss = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) ss += red[k];
+        a.out[2 * (int64_t)idx + PART] = ss;
     }
+}
+
+__global__ void __launch_bounds__(256) k_integrated_single(const intg_args *__restrict__ pa)
+{
+    integrated_body<0>(pa);
+}
+
+__global__ void __launch_bounds__(256) k_integrated_double(const intg_args *__restrict__ pa)
+{
+    integrated_body<1>(pa);
 }
 
 }  // namespace
@@ -633,15 +730,9 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
     a.nguess = in->nguess;
     a.chi_guess = in->chi_guess;
     a.tab_ref = in->tab_ref;
-    // the coefficients: the tables' in tab_y's layout, then the background curves
-    double *tb = in->coef, *tc = tb + in->ntab_values, *td = tc + in->ntab_values;
-    double *gb = td + in->ntab_values, *gc = gb + (size_t)NBGC * in->nbg, *gd = gc + (size_t)NBGC * in->nbg;
-    a.tab_b = tb;
-    a.tab_c = tc;
-    a.tab_d = td;
-    a.bgc_b = gb;
-    a.bgc_c = gc;
-    a.bgc_d = gd;
+    // the segment records: the tables' in tab_y's layout, then the background curves'
+    a.tab_rec = in->coef;
+    a.bgc_rec = in->coef + (size_t)SEG * in->ntab_values;
     coef_list cl;
     cl.ncurve = 0;
     int nmax = 0;
@@ -651,9 +742,7 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
         cl.x[q] = in->tab_x + in->tab_off[t];
         cl.y[q] = in->tab_y + in->tab_off[t];
         cl.n[q] = in->tab_len[t];
-        cl.b[q] = tb + in->tab_off[t];
-        cl.c[q] = tc + in->tab_off[t];
-        cl.d[q] = td + in->tab_off[t];
+        cl.rec[q] = in->coef + (size_t)SEG * in->tab_off[t];
         nmax = cl.n[q] > nmax ? cl.n[q] : nmax;
     }
     const double *bx[NBGC] = {in->bg_z, in->bg_z, in->bg_z, in->bg_z, in->bg_chi};
@@ -663,9 +752,7 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
         cl.x[q] = bx[k];
         cl.y[q] = by[k];
         cl.n[q] = in->nbg;
-        cl.b[q] = gb + (size_t)k * in->nbg;
-        cl.c[q] = gc + (size_t)k * in->nbg;
-        cl.d[q] = gd + (size_t)k * in->nbg;
+        cl.rec[q] = in->coef + (size_t)SEG * (in->ntab_values + (size_t)k * in->nbg);
         nmax = in->nbg > nmax ? in->nbg : nmax;
     }
     hipLaunchKernelGGL(k_intg_coef, dim3((unsigned)((nmax + 255) / 256), (unsigned)cl.ncurve), dim3(256), 0,
@@ -704,10 +791,15 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
     const int64_t blocks = (int64_t)nz * nmu * nsep;
     if (blocks <= 0 || blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
     static_assert(sizeof(intg_args) <= CE_INTG_ARGS_BYTES, "intg_args outgrew its device slot");
+    static_assert(NBGC == CE_INTG_NBGC, "background curves with segment records");
     hipError_t e = hipMemcpyAsync(in->args_dev, &a, sizeof(a), hipMemcpyHostToDevice, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);   /* `a` is on this stack */
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL(k_integrated, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_integrated_single, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const intg_args *)in->args_dev);
+    const int p1b = a.p1 / 2 > 1 ? a.p1 / 2 : 1;
+    const size_t x1_lds = (size_t)(a.p1 + p1b) * a.n1 * (5 + X1_N) * sizeof(double);
+    hipLaunchKernelGGL(k_integrated_double, dim3((unsigned)blocks), dim3(256), x1_lds, (hipStream_t)stream,
                        (const intg_args *)in->args_dev);
     return (int)hipGetLastError();
 }
