@@ -243,8 +243,9 @@ int ce_launch_mu_weights_jvp(void *stream, const ce_mu_dev *in, const ce_mu_tan 
 #define CE_INTG_NSLOT 9
 #define CE_INTG_ARGS_BYTES 2048
 #define CE_INTG_MAX_X1 512    /* x1 nodes of the double rule: (p1 + p1 / 2) n1 at most */
-#define CE_INTG_SEG 8         /* doubles per Akima segment record (64 bytes) */
-#define CE_INTG_NBGC 5        /* background curves with segment records */
+#define CE_INTG_VAL 4         /* doubles per Akima value record {y, b, c, d} */
+#define CE_INTG_PAIR 2        /* doubles per node pair {x_j, x_(j+1)} */
+#define CE_INTG_NBGC 5        /* background curves with value records */
 typedef struct ce_intg_dev {
     int nbg;
     const double *bg_z, *bg_chi, *bg_D1, *bg_a, *bg_f, *bg_cH, *bg_G1, *bg_G2;
@@ -262,16 +263,17 @@ typedef struct ce_intg_dev {
     int tab_ref;   /* a slot whose nodes every present standard table shares, or -1 */
     int nguess;
     const double *chi_guess;
-    /* workspace of CE_INTG_SEG (ntab_values + CE_INTG_NBGC nbg) doubles, 64-byte
-       aligned, for the Akima segment records, and CE_INTG_ARGS_BYTES of device memory
-       for the kernel's arguments */
+    /* workspace of (CE_INTG_VAL + CE_INTG_PAIR) (ntab_values + CE_INTG_NBGC nbg)
+       doubles, 64-byte aligned, for the Akima value records and node pairs, and
+       CE_INTG_ARGS_BYTES of device memory for the kernel's arguments */
     int ntab_values;
     double *coef;
     void *args_dev;
     int nren0;
     const double *ren0_x, *ren0_y;
     int nren;
-    const double *ren_chi, *ren_v, *ren_x, *ren_y, *ren_xy;
+    const double *ren_chi;
+    const double *ren_rec;   /* 16-byte aligned, per node (x fastest): {value, d/dx, d/dy, d2/dxdy} */
     double Om, H0h;
 } ce_intg_dev;
 /* one workgroup per (z, mu, sep): out [nz][nmu][nsep][2] = (single, double); rule =

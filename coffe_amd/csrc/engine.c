@@ -1854,16 +1854,19 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
     const size_t nren = want_ren ? (size_t)in->nren : 0, nnode = nren * nren, n0 = want_ren ? (size_t)nren0 : 0;
     const size_t ngl = 2 * (size_t)(r[1] + r[3] + r[5]), nout = 2 * (size_t)nz * nmu * nsep;
     const int nguess = 4096;
-    /* the segment records, and 7 doubles of slack to align them to 64 bytes */
-    const size_t nrec = CE_INTG_SEG * ((size_t)total_tab + CE_INTG_NBGC * nbg) + 7;
+    /* the Akima value records and node pairs, and 7 doubles of slack to align them to
+       64 bytes */
+    const size_t nrec = (CE_INTG_VAL + CE_INTG_PAIR) * ((size_t)total_tab + CE_INTG_NBGC * nbg) + 7;
     const size_t total = 8 * nbg + 3 * nb + 5 * nmb + 2 * (size_t)total_tab + 2 * n0 + nren + 4 * nnode +
-                         (size_t)nz + (size_t)nmu + (size_t)nsep + ngl + (size_t)nguess + 1 + nout + nrec +
+                         (size_t)nz + (size_t)nmu + (size_t)nsep + ngl + (size_t)nguess + 2 + nout + nrec +
                          CE_INTG_ARGS_BYTES / sizeof(double);
     double *h = (double *)malloc(sizeof(double) * (total - nout - nrec - CE_INTG_ARGS_BYTES / sizeof(double)));
     double *work = (double *)malloc(sizeof(double) * (2 * nren + 2));
-    if (!h || !work) {
+    double *planes = (double *)malloc(sizeof(double) * (4 * nnode + 1));
+    if (!h || !work || !planes) {
         free(h);
         free(work);
+        free(planes);
         return COFFE_ENOMEM;
     }
     double *d = NULL;
@@ -1948,16 +1951,21 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
         dv.ren0_y = IG_PUT(ren0_y, n0);
         dv.nren = in->nren;
         dv.ren_chi = IG_PUT(in->ren_chi, nren);
-        /* values and the bicubic's node-derivative tables (as coffe_mu_weights_ex) */
-        double *hv = h + o, *hx = hv + nnode, *hy = hx + nnode, *hxy = hy + nnode;
+        /* values and the bicubic's node-derivative tables (as coffe_mu_weights_ex),
+           interleaved per node: one node's four numbers are two 16-byte loads */
+        double *hv = planes, *hx = hv + nnode, *hy = hx + nnode, *hxy = hy + nnode;
         memcpy(hv, in->ren, sizeof(double) * nnode);
         for (size_t j = 0; j < nren; ++j) spline_node_slopes((int)nren, in->ren_chi, hv + j * nren, 1, hx + j * nren, 1, work);
         for (size_t i = 0; i < nren; ++i) spline_node_slopes((int)nren, in->ren_chi, hv + i, (int)nren, hy + i, (int)nren, work);
         for (size_t i = 0; i < nren; ++i) spline_node_slopes((int)nren, in->ren_chi, hx + i, (int)nren, hxy + i, (int)nren, work);
-        dv.ren_v = d + o;
-        dv.ren_x = d + o + nnode;
-        dv.ren_y = d + o + 2 * nnode;
-        dv.ren_xy = d + o + 3 * nnode;
+        o += o & 1;
+        for (size_t k = 0; k < nnode; ++k) {
+            h[o + 4 * k] = hv[k];
+            h[o + 4 * k + 1] = hx[k];
+            h[o + 4 * k + 2] = hy[k];
+            h[o + 4 * k + 3] = hxy[k];
+        }
+        dv.ren_rec = d + o;
         o += 4 * nnode;
     }
     dv.Om = in->Omega_m;
@@ -1988,5 +1996,6 @@ int coffe_integrated_corrfunc(const coffe_mu_inputs *in, unsigned flags, int nta
     hipFree(d);
     free(h);
     free(work);
+    free(planes);
     return st;
 }

@@ -46,13 +46,14 @@ struct intg_args {
     double tab_lx0[NSLOT], tab_lsc[NSLOT];   // its log(x[f0]) and nodes per unit of log x
     int nguess;                        // chi -> interval guesses on [0, chi_max], nguess + 1 of them
     const double *chi_guess;
-    // Akima segment records (SEG doubles per node, below): the tables' in the layout of
-    // tab_y, the background curves' [NBGC][nbg]
-    const double *tab_rec, *bgc_rec;
+    // Akima value records and node pairs (below): the tables' in the layout of tab_y,
+    // the background curves' [NBGC][nbg]
+    const double *tab_val, *tab_pair, *bgc_val, *bgc_pair;
     int nren0;
     const double *ren0_x, *ren0_y;
     int nren;
-    const double *ren_chi, *ren_v, *ren_x, *ren_y, *ren_xy;
+    const double *ren_chi;
+    const double *ren_rec;   // per grid node (x fastest): {value, d/dx, d/dy, d2/dxdy}
     double Om, H0h;
     unsigned flags;
     int nz, nmu, nsep;
@@ -64,37 +65,39 @@ struct intg_args {
 
 __device__ inline double ak(const double *x, const double *y, int n, double v) { return ce::akima_general(x, y, n, v); }
 
-// The Akima segment of node j of a curve as one 64-byte record {x_j, y_j, b_j, c_j, d_j,
-// x_(j+1), -, -}: a lookup reads one cache line in three 16-byte loads, where separate
-// node and coefficient arrays cost five scattered lines, and the record holds both ends
-// of its interval, so a guessed interval is tested from the same loads.  (The lookups
-// of this kernel gather from the L1 cache at random per lane; its throughput, not
-// arithmetic, bounds them.)
-constexpr int SEG = CE_INTG_SEG;
-struct segv {
-    double x0, y, b, c, d, x1;
+// An Akima lookup reads two arrays: node pairs {x_j, x_(j+1)} (16 bytes, one load: both
+// ends of interval j, which also test a guessed interval), shared by the curves on one
+// grid, and each curve's value records {y_j, b_j, c_j, d_j} (32 bytes, two loads).  The
+// lookups of this kernel gather at random per lane, so the L1 cache's throughput in
+// cache lines, not arithmetic, bounds them: separate node and coefficient arrays cost
+// five scattered lines per lookup, this layout three, and two for each further curve
+// or table on the same interval.
+constexpr int VAL = CE_INTG_VAL, PAIR = CE_INTG_PAIR;
+struct nodes {
+    double x0, x1;
 };
 
-__device__ __forceinline__ segv seg_load(const double *__restrict__ r)
+__device__ __forceinline__ nodes pair_at(const double *__restrict__ p, int j)
 {
-    const double2 p = *reinterpret_cast<const double2 *>(r), q = *reinterpret_cast<const double2 *>(r + 2),
-                  u = *reinterpret_cast<const double2 *>(r + 4);
-    return segv{p.x, p.y, q.x, q.y, u.x, u.y};
+    const double2 q = *reinterpret_cast<const double2 *>(p + (size_t)PAIR * j);
+    return nodes{q.x, q.y};
 }
 
-// y(xq) on the segment (the value akima_at gives in its interval)
-__device__ __forceinline__ double seg_at(const segv &g, double xq)
-{
-    const double dx = xq - g.x0;
-    return g.y + dx * (g.b + dx * (g.c + g.d * dx));
-}
-
-// whether interval j (0 <= j <= n - 2), the segment's, is the one ce::interval_from
-// returns from the guess j at once (false for a NaN v, whose callers then search and
-// get j back)
-__device__ __forceinline__ bool seg_hit(const segv &g, int j, int n, double v)
+// whether interval j (0 <= j <= n - 2), whose nodes g holds, is the one
+// ce::interval_from returns from the guess j at once (false for a NaN v, whose callers
+// then search and get j back)
+__device__ __forceinline__ bool pair_hit(const nodes &g, int j, int n, double v)
 {
     return g.x0 <= v && (j == n - 2 || v < g.x1);
+}
+
+// y(xq) on the segment of interval j with lower node x0 (the value akima_at gives there)
+__device__ __forceinline__ double val_at(const double *__restrict__ r, int j, double x0, double xq)
+{
+    const double2 p = *reinterpret_cast<const double2 *>(r + (size_t)VAL * j),
+                  q = *reinterpret_cast<const double2 *>(r + (size_t)VAL * j + 2);
+    const double dx = xq - x0;
+    return p.x + dx * (p.y + dx * (q.x + q.y * dx));
 }
 
 // The lookups: Akima of raw nodes as ce::akima_general, with the interval found from a
@@ -128,11 +131,17 @@ struct lk {
         const int g = bg_guess(z);
         return g < 0 ? -1 : ce::interval_from(a.bg_z, a.nbg, z, g);
     }
-    __device__ segv bgseg(int k, int j) const { return seg_load(a.bgc_rec + SEG * ((size_t)k * a.nbg + j)); }
+    // curve k's value records, and the node pairs of its grid (bg_z for D1, a, f and
+    // conformal H; bg_chi for z(chi))
+    __device__ const double *bgv(int k) const { return a.bgc_val + (size_t)VAL * k * a.nbg; }
+    __device__ const double *bgp(int k) const
+    {
+        return a.bgc_pair + (size_t)PAIR * (k == C_ZOF ? C_ZOF : C_D1) * a.nbg;
+    }
     // D1, a, f or conformal H (curve k) in interval j (NaN for j < 0)
     __device__ double bgcj(int k, int j, double z) const
     {
-        return j < 0 ? __builtin_nan("") : seg_at(bgseg(k, j), z);
+        return j < 0 ? __builtin_nan("") : val_at(bgv(k), j, pair_at(bgp(k), j).x0, z);
     }
     // curve k at z, the interval found from the guess; *jz (if given) receives it
     __device__ double bgc(int k, double z, int *jz = nullptr) const
@@ -140,10 +149,9 @@ struct lk {
         int j = bg_guess(z);
         double v = __builtin_nan("");
         if (j >= 0) {
-            const segv g = bgseg(k, j);
-            if (seg_hit(g, j, a.nbg, z)) {
-                v = seg_at(g, z);
-            } else {
+            const nodes g = pair_at(bgp(k), j);
+            v = val_at(bgv(k), j, g.x0, z);
+            if (!pair_hit(g, j, a.nbg, z)) {
                 j = ce::interval_from(a.bg_z, a.nbg, z, j);
                 v = bgcj(k, j, z);
             }
@@ -158,9 +166,10 @@ struct lk {
         if (!(chi >= x[0] && chi <= x[n - 1])) return __builtin_nan("");
         const int k = ce::clamp_index(chi / x[n - 1] * a.nguess, a.nguess + 1);
         const int j = below_last(max((int)a.chi_guess[k], 0), n);
-        const segv g = bgseg(C_ZOF, j);
-        if (seg_hit(g, j, n, chi)) return seg_at(g, chi);
-        return seg_at(bgseg(C_ZOF, ce::interval_from(x, n, chi, j)), chi);
+        const nodes g = pair_at(bgp(C_ZOF), j);
+        const double v = val_at(bgv(C_ZOF), j, g.x0, chi);
+        if (pair_hit(g, j, n, chi)) return v;
+        return bgcj(C_ZOF, ce::interval_from(x, n, chi, j), chi);
     }
     // a first interval of r on table s's nodes (-1: r outside them): by bisection below
     // the log-uniform part (exact), from the log above it (a guess)
@@ -173,28 +182,40 @@ struct lk {
         if (r < x[f0]) return ce::interval_bisect(x, 0, f0, r);
         return below_last(f0 + ce::clamp_index((log(r) - a.tab_lx0[s]) * a.tab_lsc[s], n - f0), n);
     }
-    __device__ segv tabseg(int s, int j) const { return seg_load(a.tab_rec + SEG * ((size_t)a.tab_off[s] + j)); }
-    // interval j of table s settled from its guess g (-1 stays -1)
-    __device__ int tab_settle(int s, int g, double r) const
+    __device__ const double *tabv(int s) const { return a.tab_val + (size_t)VAL * a.tab_off[s]; }
+    __device__ const double *tabp(int s) const { return a.tab_pair + (size_t)PAIR * a.tab_off[s]; }
+    // interval j of table s settled from its guess g (-1 stays -1); *x0 its lower node
+    __device__ int tab_settle(int s, int g, double r, double *x0) const
     {
-        if (g < 0 || seg_hit(tabseg(s, g), g, a.tab_len[s], r)) return g;
-        return ce::interval_from(a.tab_x + a.tab_off[s], a.tab_len[s], r, g);
+        if (g < 0) return g;
+        nodes nd = pair_at(tabp(s), g);
+        if (!pair_hit(nd, g, a.tab_len[s], r)) {
+            g = ce::interval_from(a.tab_x + a.tab_off[s], a.tab_len[s], r, g);
+            nd = pair_at(tabp(s), g);
+        }
+        *x0 = nd.x0;
+        return g;
     }
     __device__ double tab(int s, double r) const
     {
         const int g = tab_guess(s, r);
         if (g < 0) return __builtin_nan("");
-        const segv v = tabseg(s, g);
-        if (seg_hit(v, g, a.tab_len[s], r)) return seg_at(v, r);
-        return seg_at(tabseg(s, ce::interval_from(a.tab_x + a.tab_off[s], a.tab_len[s], r, g)), r);
+        const nodes nd = pair_at(tabp(s), g);
+        const double v = val_at(tabv(s), g, nd.x0, r);
+        if (pair_hit(nd, g, a.tab_len[s], r)) return v;
+        const int j = ce::interval_from(a.tab_x + a.tab_off[s], a.tab_len[s], r, g);
+        return val_at(tabv(s), j, pair_at(tabp(s), j).x0, r);
     }
     // the interval of r on the standard tables' shared nodes (-1: none shared, or r
-    // outside them), and a standard table's value there
-    __device__ int tab_interval(double r) const
+    // outside them) and its lower node; a standard table's value there
+    __device__ int tab_interval(double r, double *x0) const
     {
-        return a.tab_ref < 0 ? -1 : tab_settle(a.tab_ref, tab_guess(a.tab_ref, r), r);
+        return a.tab_ref < 0 ? -1 : tab_settle(a.tab_ref, tab_guess(a.tab_ref, r), r, x0);
     }
-    __device__ double tabj(int s, int j, double r) const { return j < 0 ? tab(s, r) : seg_at(tabseg(s, j), r); }
+    __device__ double tabj(int s, int j, double x0, double r) const
+    {
+        return j < 0 ? tab(s, r) : val_at(tabv(s), j, x0, r);
+    }
     // GSL interp2d bicubic of the renormalisation grid at (x, y) (node derivatives of
     // natural cubic splines, Hermite patches; common.c:712-719)
     __device__ double bicubic(double x, double y) const
@@ -215,9 +236,10 @@ struct lk {
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                const int o = (j + q) * n + i + p;
-                r += a.ren_v[o] * ht[p] * hu[q] + a.ren_x[o] * dx * ht[2 + p] * hu[q] +
-                     a.ren_y[o] * dy * ht[p] * hu[2 + q] + a.ren_xy[o] * dx * dy * ht[2 + p] * hu[2 + q];
+                const double *g = a.ren_rec + 4 * ((size_t)(j + q) * n + i + p);
+                const double2 vx = *reinterpret_cast<const double2 *>(g), yxy = *reinterpret_cast<const double2 *>(g + 2);
+                r += vx.x * ht[p] * hu[q] + vx.y * dx * ht[2 + p] * hu[q] + yxy.x * dy * ht[p] * hu[2 + q] +
+                     yxy.y * dx * dy * ht[2 + p] * hu[2 + q];
             }
         return r;
     }
@@ -308,7 +330,8 @@ __device__ __forceinline__ double double_point(const intg_args &a, const pt &p, 
     const double LL = l1 * l2;
     const double q2 = (2 - 5 * p.s1) * (2 - 5 * p.s2);
     const bool lens = f & F_LEN, g4 = f & F_G4, g5 = f & F_G5, oc = f & F_ONLY_CROSS;
-    const int jr = L.tab_interval(r);
+    double xr = 0.0;
+    const int jr = L.tab_interval(r, &xr);
     double res = 0.0;
     double ren = 0.0;
     if (g4 || g5) ren = L.ren(r2, l1, l2, r2 <= 1e-12, l1);
@@ -316,11 +339,11 @@ __device__ __forceinline__ double double_point(const intg_args &a, const pt &p, 
         double T;
         if (r2 > 1e-20) {
             const double cc1 = c * c - 1;
-            T = 2 * cc1 * LL * L.tabj(S00, jr, r) / 5 + 4 * c * L.tabj(S20, jr, r) / 3 +
-                4 * c * (r2 + 6 * c * LL) * L.tabj(S11, jr, r) / 15 +
-                2 * cc1 * LL * (2 * r2 + 3 * c * LL) * L.tabj(S02, jr, r) / 7 / r2 +
-                2 * c * (2 * r2 * r2 + 12 * c * r2 * LL + 15 * cc1 * LL * LL) * L.tabj(S13, jr, r) / 15 / r2 +
-                cc1 * LL * (6 * r2 * r2 + 30 * c * r2 * LL + 35 * cc1 * LL * LL) * L.tabj(S04, jr, r) / 35 / r2 / r2;
+            T = 2 * cc1 * LL * L.tabj(S00, jr, xr, r) / 5 + 4 * c * L.tabj(S20, jr, xr, r) / 3 +
+                4 * c * (r2 + 6 * c * LL) * L.tabj(S11, jr, xr, r) / 15 +
+                2 * cc1 * LL * (2 * r2 + 3 * c * LL) * L.tabj(S02, jr, xr, r) / 7 / r2 +
+                2 * c * (2 * r2 * r2 + 12 * c * r2 * LL + 15 * cc1 * LL * LL) * L.tabj(S13, jr, xr, r) / 15 / r2 +
+                cc1 * LL * (6 * r2 * r2 + 30 * c * r2 * LL + 35 * cc1 * LL * LL) * L.tabj(S04, jr, xr, r) / 35 / r2 / r2;
         } else {
             T = 4 + 24 * LL * L.tab(S11, 0.0) / 15;
         }
@@ -336,7 +359,7 @@ __device__ __forceinline__ double double_point(const intg_args &a, const pt &p, 
     if (g4 && !oc) res += 9 * Om2 * q2 * DD * ren;
     if (g5 && !oc) res += 9 * Om2 * p.G1c * p.G2c * p.chi1 * p.chi2 * DD * cH1 * cH2 * (f1 - 1) * (f2 - 1) * ren;
     if ((g4 || g5) && lens) {
-        const double P = r2 != 0.0 ? 2 * LL * c * L.tabj(S31, jr, r) - LL * LL * (1 - c * c) * L.tabj(S22, jr, r)
+        const double P = r2 != 0.0 ? 2 * LL * c * L.tabj(S31, jr, xr, r) - LL * LL * (1 - c * c) * L.tabj(S22, jr, xr, r)
                                    : 2 * LL * L.tab(S31, 0.0);
         const double w2 = (1 - x2) / x2, w1 = (1 - x1) / x1;
         if (g4) res += 9 * Om2 / 2 * q2 * DD * (w2 + w1) * P;
@@ -392,7 +415,8 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
     const double r21 = fmax(l2 * l2 + chi1 * chi1 - 2 * chi1 * l2 * c, 0.0);
     const double r22 = fmax(l1 * l1 + chi2 * chi2 - 2 * chi2 * l1 * c, 0.0);
     const double rA = sqrt(r21), rB = sqrt(r22);
-    const int jA = L.tab_interval(rA), jB = L.tab_interval(rB);
+    double xA = 0.0, xB = 0.0;
+    const int jA = L.tab_interval(rA, &xA), jB = L.tab_interval(rB, &xB);
     const double z1 = L.z_of(l1), z2 = L.z_of(l2);
     const int j1 = L.bg_interval(z1), j2 = L.bg_interval(z2);
     const double D1x = L.bgcj(C_D1, j1, z1), D2x = L.bgcj(C_D1, j2, z2);
@@ -405,38 +429,38 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
     if (lens && (f & F_DEN) && full_dl)   // functions.c:1707-1781
         res += H2 * (-3 * Om / 2) *
                (p.b1 * (2 - 5 * s2) * p.Dc1 * chi2 * WA *
-                    (2 * chi1 * c * L.tabj(S11, jA, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tabj(S02, jA, rA) / r21) +
+                    (2 * chi1 * c * L.tabj(S11, jA, xA, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tabj(S02, jA, xA, rA) / r21) +
                 p.b2 * (2 - 5 * s1) * p.Dc2 * chi1 * WB *
-                    (2 * chi2 * c * L.tabj(S11, jB, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tabj(S02, jB, rB) / r22));
+                    (2 * chi2 * c * L.tabj(S11, jB, xB, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tabj(S02, jB, xB, rB) / r22));
     if (lens && (f & F_RSD) && full_dl) {   // functions.c:2167-2198, then 2291-2324 or 2377-2390
         const bool wide = fabs(p.mu) < 0.999;
-        auto rsd = [&](double ch, double lam, double rr, double rv, int jv) {
+        auto rsd = [&](double ch, double lam, double rr, double rv, int jv, double xv) {
             (void)rr;
             const double d = ch - lam, rs = d * d + 2 * ch * lam * e;
-            double v = (lam - 6 * ch * c + 3 * lam * c2) * L.tabj(S00, jv, rv) / 15 -
-                       num_rsd_i02(ch, lam, d, e) * L.tabj(S02, jv, rv) / rs / 21;
-            if (wide) v -= num_rsd_i04(ch, lam, d, e) * L.tabj(S04, jv, rv) / rs / rs / 35;
-            else v += 4 * (lam + ch) * L.tabj(S04, jv, rv) / 35;
+            double v = (lam - 6 * ch * c + 3 * lam * c2) * L.tabj(S00, jv, xv, rv) / 15 -
+                       num_rsd_i02(ch, lam, d, e) * L.tabj(S02, jv, xv, rv) / rs / 21;
+            if (wide) v -= num_rsd_i04(ch, lam, d, e) * L.tabj(S04, jv, xv, rv) / rs / rs / 35;
+            else v += 4 * (lam + ch) * L.tabj(S04, jv, xv, rv) / 35;
             return v;
         };
         res += H2 * 3 * Om / 2 *
-               (chi2 * p.fc1 * (2 - 5 * s2) * p.Dc1 * WA * rsd(chi1, l2, r21, rA, jA) +
-                chi1 * p.fc2 * (2 - 5 * s1) * p.Dc2 * WB * rsd(chi2, l1, r22, rB, jB));
+               (chi2 * p.fc1 * (2 - 5 * s2) * p.Dc1 * WA * rsd(chi1, l2, r21, rA, jA, xA) +
+                chi1 * p.fc2 * (2 - 5 * s1) * p.Dc2 * WB * rsd(chi2, l1, r22, rB, jB, xB));
     }
     if (lens && (f & F_D1) && full_dl) {   // functions.c:2624-2661 (G1 at z(chi2), 2 - 5 s2 in the second half)
-        auto d1k = [&](double ch, double lam, double rr, double rv, int jv) {
+        auto d1k = [&](double ch, double lam, double rr, double rv, int jv, double xv) {
             (void)rr;
             const double d = ch - lam, rs = d * d + 2 * ch * lam * e;
-            return 2 * (c * (lam * lam - 2 * ch * ch) + ch * lam * (2 * c2 - 1)) * L.tabj(S11, jv, rv) / 15 +
-                   2 * c * L.tabj(S20, jv, rv) / 3 - num_d1_i13(ch, lam, d, e) * L.tabj(S13, jv, rv) / rs / 15;
+            return 2 * (c * (lam * lam - 2 * ch * ch) + ch * lam * (2 * c2 - 1)) * L.tabj(S11, jv, xv, rv) / 15 +
+                   2 * c * L.tabj(S20, jv, xv, rv) / 3 - num_d1_i13(ch, lam, d, e) * L.tabj(S13, jv, xv, rv) / rs / 15;
         };
         res += H2 * 3 * Om / 2 *
-               (chi2 * p.cHc1 * p.fc1 * p.G1c * (2 - 5 * s2) * p.Dc1 * WA * d1k(chi1, l2, r21, rA, jA) +
-                chi1 * p.cHc2 * p.fc2 * p.G1c2 * (2 - 5 * s2) * p.Dc2 * WB * d1k(chi2, l1, r22, rB, jB));
+               (chi2 * p.cHc1 * p.fc1 * p.G1c * (2 - 5 * s2) * p.Dc1 * WA * d1k(chi1, l2, r21, rA, jA, xA) +
+                chi1 * p.cHc2 * p.fc2 * p.G1c2 * (2 - 5 * s2) * p.Dc2 * WB * d1k(chi2, l1, r22, rB, jB, xB));
     }
     if (lens && (f & (F_D2 | F_G1 | F_G2 | F_G3))) {
-        const double PA = 2 * chi1 * c * L.tabj(S31, jA, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tabj(S22, jA, rA);
-        const double PB = 2 * chi2 * c * L.tabj(S31, jB, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tabj(S22, jB, rB);
+        const double PA = 2 * chi1 * c * L.tabj(S31, jA, xA, rA) - chi1 * chi1 * l2 * (1 - c * c) * L.tabj(S22, jA, xA, rA);
+        const double PB = 2 * chi2 * c * L.tabj(S31, jB, xB, rB) - chi2 * chi2 * l1 * (1 - c * c) * L.tabj(S22, jB, xB, rB);
         const double A = chi2 * p.Dc1 * WA * PA, B = chi1 * p.Dc2 * WB * PB;
         const double q = 9 * Om * Om / 4 * H4;
         if (f & F_D2)   // functions.c:3000-3030
@@ -453,23 +477,23 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
         const double ren2 = div ? L.ren(r22, l1, chi2, r22 == 0.0, l1) : 0.0;
         const double E1 = L.bgcj(C_CH, j1, z1) * (L.bgcj(C_F, j1, z1) - 1),
                      E2 = L.bgcj(C_CH, j2, z2) * (L.bgcj(C_F, j2, z2) - 1);
-        auto rsdk = [&](double rr, double lam, double rv, int jv) {
-            return (2 * rr / 3 + (c * c - 1) * lam * lam) * L.tabj(S22, jv, rv) - L.tabj(S20, jv, rv) / 3;
+        auto rsdk = [&](double rr, double lam, double rv, int jv, double xv) {
+            return (2 * rr / 3 + (c * c - 1) * lam * lam) * L.tabj(S22, jv, xv, rv) - L.tabj(S20, jv, xv, rv) / 3;
         };
         const double q = 9 * Om * Om / 2 * H4;
         const double cH2c1 = p.cHc1 * p.cHc1, cH2c2 = p.cHc2 * p.cHc2;
         if (g4) {
             if (f & F_DEN)   // functions.c:3122-3140
                 res += H2 * (-3 * Om) *
-                       (p.b1 * (2 - 5 * s2) * p.Dc1 * VA * L.tabj(S20, jA, rA) + p.b2 * (2 - 5 * s1) * p.Dc2 * VB * L.tabj(S20, jB, rB));
+                       (p.b1 * (2 - 5 * s2) * p.Dc1 * VA * L.tabj(S20, jA, xA, rA) + p.b2 * (2 - 5 * s1) * p.Dc2 * VB * L.tabj(S20, jB, xB, rB));
             if (f & F_RSD)   // functions.c:3164-3189
                 res += H2 * 3 * Om *
-                       (p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * rsdk(r21, l2, rA, jA) +
-                        p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * rsdk(r22, l1, rB, jB));
+                       (p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * rsdk(r21, l2, rA, jA, xA) +
+                        p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * rsdk(r22, l1, rB, jB, xB));
             if (f & F_D1)    // functions.c:3220-3235
                 res += H2 * 3 * Om *
-                       (p.cHc1 * p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * (l2 * c - chi1) * L.tabj(S31, jA, rA) +
-                        p.cHc2 * p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * (l1 * c - chi2) * L.tabj(S31, jB, rB));
+                       (p.cHc1 * p.fc1 * (2 - 5 * s2) * p.Dc1 * VA * (l2 * c - chi1) * L.tabj(S31, jA, xA, rA) +
+                        p.cHc2 * p.fc2 * (2 - 5 * s1) * p.Dc2 * VB * (l1 * c - chi2) * L.tabj(S31, jB, xB, rB));
             if (f & F_D2)    // functions.c:3258-3275
                 res += H2 * (-3 * Om) *
                        ((3 - p.fe1) * p.fc1 * cH2c1 * (2 - 5 * s2) * p.Dc1 * VA * ren1 +
@@ -482,16 +506,16 @@ __device__ double single_point(const intg_args &a, const pt &p, double x)
         if (g5) {
             if (f & F_DEN)   // functions.c:3142-3162 (D1 a along the line of sight, as written there)
                 res += H2 * (-3 * Om) *
-                       (chi2 * p.b1 * p.G2c * p.Dc1 * E2 * D2x * a2x * L.tabj(S20, jA, rA) +
-                        chi1 * p.b2 * p.G1c * p.Dc2 * E1 * D1x * a1x * L.tabj(S20, jB, rB));
+                       (chi2 * p.b1 * p.G2c * p.Dc1 * E2 * D2x * a2x * L.tabj(S20, jA, xA, rA) +
+                        chi1 * p.b2 * p.G1c * p.Dc2 * E1 * D1x * a1x * L.tabj(S20, jB, xB, rB));
             if (f & F_RSD)   // functions.c:3191-3218
                 res += H2 * 3 * Om *
-                       (chi2 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * rsdk(r21, l2, rA, jA) +
-                        chi1 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * rsdk(r22, l1, rB, jB));
+                       (chi2 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * rsdk(r21, l2, rA, jA, xA) +
+                        chi1 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * rsdk(r22, l1, rB, jB, xB));
             if (f & F_D1)    // functions.c:3237-3256
                 res += H2 * 3 * Om *
-                       (chi2 * p.cHc1 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * (l2 * c - chi1) * L.tabj(S31, jA, rA) +
-                        chi1 * p.cHc2 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * (l1 * c - chi2) * L.tabj(S31, jB, rB));
+                       (chi2 * p.cHc1 * p.fc1 * p.G2c * p.Dc1 * E2 * VA * (l2 * c - chi1) * L.tabj(S31, jA, xA, rA) +
+                        chi1 * p.cHc2 * p.fc2 * p.G1c * p.Dc2 * E1 * VB * (l1 * c - chi2) * L.tabj(S31, jB, xB, rB));
             if (f & F_D2)    // functions.c:3277-3296
                 res += H2 * (-3 * Om) *
                        (chi2 * (3 - p.fe1) * p.fc1 * cH2c1 * p.G2c * p.Dc1 * E2 * VA * ren1 +
@@ -562,12 +586,12 @@ __device__ int single_pieces(const pt &p, piece *pc)
     return np;
 }
 
-// the Akima segment records of up to 16 curves, one thread per (node, curve)
+// the value records and node pairs of up to 16 curves, one thread per (node, curve)
 struct coef_list {
     int ncurve;
     const double *x[16], *y[16];
     int n[16];
-    double *rec[16];
+    double *val[16], *pair[16];
 };
 
 __global__ void __launch_bounds__(256) k_intg_coef(coef_list L)
@@ -576,18 +600,19 @@ __global__ void __launch_bounds__(256) k_intg_coef(coef_list L)
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = L.n[t];
     if (j >= n) return;
-    double *r = L.rec[t] + SEG * (size_t)j;
     const double *x = L.x[t], *y = L.y[t];
-    double v[SEG] = {x[j], y[j], 0.0, 0.0, 0.0, x[j], 0.0, 0.0};   // the last node's: no segment
+    double v[VAL] = {y[j], 0.0, 0.0, 0.0}, x1 = x[j];   // the last node's: no segment
     if (j < n - 1) {
         const ce::akima_cf<double> cf = ce::akima_coefs<double>(x, nullptr, y, nullptr, n, j);
-        v[2] = cf.b;
-        v[3] = cf.c;
-        v[4] = cf.d;
-        v[5] = x[j + 1];
+        v[1] = cf.b;
+        v[2] = cf.c;
+        v[3] = cf.d;
+        x1 = x[j + 1];
     }
-#pragma unroll
-    for (int k = 0; k < SEG; k += 2) *reinterpret_cast<double2 *>(r + k) = double2{v[k], v[k + 1]};
+    double *r = L.val[t] + (size_t)VAL * j;
+    *reinterpret_cast<double2 *>(r) = double2{v[0], v[1]};
+    *reinterpret_cast<double2 *>(r + 2) = double2{v[2], v[3]};
+    *reinterpret_cast<double2 *>(L.pair[t] + (size_t)PAIR * j) = double2{x[j], x1};
 }
 
 // one workgroup per (z, mu, r): PART 0 sums the single-integrated terms into out[..][0],
@@ -730,9 +755,15 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
     a.nguess = in->nguess;
     a.chi_guess = in->chi_guess;
     a.tab_ref = in->tab_ref;
-    // the segment records: the tables' in tab_y's layout, then the background curves'
-    a.tab_rec = in->coef;
-    a.bgc_rec = in->coef + (size_t)SEG * in->ntab_values;
+    // the value records, then the node pairs; each the tables' in tab_y's layout, then
+    // the background curves'
+    const size_t nrec = (size_t)in->ntab_values + (size_t)NBGC * in->nbg;
+    double *tval = in->coef, *bval = tval + (size_t)VAL * in->ntab_values;
+    double *tpair = in->coef + (size_t)VAL * nrec, *bpair = tpair + (size_t)PAIR * in->ntab_values;
+    a.tab_val = tval;
+    a.bgc_val = bval;
+    a.tab_pair = tpair;
+    a.bgc_pair = bpair;
     coef_list cl;
     cl.ncurve = 0;
     int nmax = 0;
@@ -742,7 +773,8 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
         cl.x[q] = in->tab_x + in->tab_off[t];
         cl.y[q] = in->tab_y + in->tab_off[t];
         cl.n[q] = in->tab_len[t];
-        cl.rec[q] = in->coef + (size_t)SEG * in->tab_off[t];
+        cl.val[q] = tval + (size_t)VAL * in->tab_off[t];
+        cl.pair[q] = tpair + (size_t)PAIR * in->tab_off[t];
         nmax = cl.n[q] > nmax ? cl.n[q] : nmax;
     }
     const double *bx[NBGC] = {in->bg_z, in->bg_z, in->bg_z, in->bg_z, in->bg_chi};
@@ -752,7 +784,8 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
         cl.x[q] = bx[k];
         cl.y[q] = by[k];
         cl.n[q] = in->nbg;
-        cl.rec[q] = in->coef + (size_t)SEG * (in->ntab_values + (size_t)k * in->nbg);
+        cl.val[q] = bval + (size_t)VAL * k * in->nbg;
+        cl.pair[q] = bpair + (size_t)PAIR * k * in->nbg;
         nmax = in->nbg > nmax ? in->nbg : nmax;
     }
     hipLaunchKernelGGL(k_intg_coef, dim3((unsigned)((nmax + 255) / 256), (unsigned)cl.ncurve), dim3(256), 0,
@@ -762,10 +795,7 @@ extern "C" int ce_launch_integrated(void *stream, const ce_intg_dev *in, unsigne
     a.ren0_y = in->ren0_y;
     a.nren = in->nren;
     a.ren_chi = in->ren_chi;
-    a.ren_v = in->ren_v;
-    a.ren_x = in->ren_x;
-    a.ren_y = in->ren_y;
-    a.ren_xy = in->ren_xy;
+    a.ren_rec = in->ren_rec;
     a.Om = in->Om;
     a.H0h = in->H0h;
     a.flags = flags;
