@@ -421,7 +421,7 @@ def run_divergent(args):
     print(json.dumps(res))
 
 
-# FP64 operations per integrand evaluation of k_integrated with every contribution on,
+# FP64 operations per integrand evaluation of the integrated kernels with every contribution on,
 # counted from the source: ~19 Akima lookups of ~40 flop (five slopes and the cubic; the
 # bisections are comparisons), the bicubic renormalisation ~60, the terms ~200
 INTG_FLOP_PER_NODE = 1000.0
@@ -502,13 +502,13 @@ def run_integrated(args):
         "config": {"workload": "single- and double-integrated correlation function, every contribution, z = 1, "
                                "64 theta nodes x 64 separations 5-300 Mpc/h, projected on l = 0, 2, 4",
                    "points_per_step": points, "integrand_nodes_per_step": nodes},
-        "roofline": {"bound": "mfma", "kernel": "k_integrated", "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "kernel": "k_integrated_single + k_integrated_double", "unit": "TFLOP/s",
                      "achieved": flops / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
                      "frac": flops / (ms * 1e-3) / PEAK_FP64, "traffic": None,
                      "note": f"{INTG_FLOP_PER_NODE:.0f} flop per integrand node (estimated from the source), "
                              "divided by the whole synchronous call (uploads, allocation, the kernel, download): "
-                             "a lower bound of the kernel's own rate, whose duration the rocprof summary gives; "
-                             "FP64 VALU with L2-latency-bound table bisections"},
+                             "a lower bound of the kernels' own rate, whose durations the rocprof summary gives; "
+                             "FP64 VALU, with table lookups bound by the L1's gather throughput (random lines per lane)"},
         "parity": {"vs_oracle_max_rel": max(chk), "points": len(chk)},
     }
     if not args.no_cpu:
