@@ -432,7 +432,8 @@ def run_integrated(args):
     RSD, Doppler, d2, g1-g3, lensing, g4, g5, full sky; functions.c:1486-4017): the
     single- and double-integrated correlation function at z = 1 on the 64 theta nodes of
     the multipoles and 64 separations 5-300 Mpc/h, projected on l = 0, 2, 4
-    (coffe_integrated_corrfunc, one k_integrated workgroup per (mu, separation))."""
+    (coffe_integrated_corrfunc, one workgroup of k_integrated_single and one of
+    k_integrated_double per (mu, separation))."""
     import torch
     import oracle as O
     from oracle import integrated as OI

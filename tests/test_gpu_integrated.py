@@ -1,8 +1,8 @@
 """The integrated terms on the GPU (coffe_integrated_corrfunc, k_integrated_single and
-k_integrated_double): the kernels against the oracle (oracle/integrated.py, the same product rules) for every group of
-contributions with unequal populations, and the Coffe API (full-sky lensing, g4, g5,
-their cross terms with the local ones) against the reference's fixtures the way its
-tests hold them (test_coffe.py:235-303, rtol 5e-4)."""
+k_integrated_double): the kernels against the oracle (oracle/integrated.py, the same
+product rules) for every group of contributions with unequal populations, and the
+Coffe API (full-sky lensing, g4, g5, their cross terms with the local ones) against the
+reference's fixtures the way its tests hold them (test_coffe.py:235-303, rtol 5e-4)."""
 import os
 
 import numpy as np
